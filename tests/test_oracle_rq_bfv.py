@@ -1,0 +1,189 @@
+"""Pins the oracle's ring layer (switch_down, substitute, rq::Scaler, key switch)
+and BFV layer (Multiplicator, relinearization, Galois keys) with closed forms,
+Python big-integer recomputation and decrypt-correctness."""
+import random
+
+import numpy as np
+import pytest
+
+# math/src/rq/mod.rs test moduli
+MODULI = [1153, 4611686018326724609, 4611686018309947393, 4611686018232352769, 4611686018171535361,
+          4611686018106523649]
+
+
+def lift_poly(oracle, moduli, a, n):
+    a = np.asarray(a, np.uint64).reshape(len(moduli), n)
+    return [oracle.rns_lift(moduli, [int(a[i, j]) for i in range(len(moduli))]) for j in range(n)]
+
+
+def project(vals, moduli):
+    return np.array([[v % m for v in vals] for m in moduli], np.uint64).reshape(-1)
+
+
+def test_switch_down_closed_form(oracle):
+    n = 16
+    ctx = oracle.Context(MODULI, n)
+    a = ctx.random(1, seed=3)
+    ref = lift_poly(oracle, MODULI, a, n)
+    moduli = list(MODULI)
+    cur = a
+    while len(moduli) > 1:
+        Q = 1
+        for m in moduli:
+            Q *= m
+        c = oracle.Context(moduli, n)
+        cur = c.switch_down(cur)
+        moduli = moduli[:-1]
+        Qn = Q // MODULI[len(moduli)]
+        exp = [((b * Qn) + (Q >> 1)) // Q % Qn for b in ref]
+        assert lift_poly(oracle, moduli, cur, n) == exp
+        ref = exp
+
+
+def negacyclic_mul(a, b, q):
+    n = len(a)
+    c = [0] * n
+    for i in range(n):
+        if a[i] == 0:
+            continue
+        for j in range(n):
+            k = i + j
+            if k < n:
+                c[k] += a[i] * b[j]
+            else:
+                c[k - n] -= a[i] * b[j]
+    return [x % q for x in c]
+
+
+def test_substitute_closed_form(oracle):
+    n = 16
+    ctx = oracle.Context(MODULI[:3], n)
+    a = ctx.random(1, seed=5)
+    A = a.reshape(3, n)
+    for k in [1, 3, 5, 11, 31, 2 * n - 1]:
+        out = ctx.substitute(a, k, ntt_repr=False).reshape(3, n)
+        for li, q in enumerate(MODULI[:3]):
+            exp = [0] * n
+            for i in range(n):
+                e = (k * i) % (2 * n)
+                if e >= n:
+                    exp[e - n] = (exp[e - n] - int(A[li, i])) % q
+                else:
+                    exp[e] = (exp[e] + int(A[li, i])) % q
+            assert [int(x) for x in out[li]] == exp
+        # NTT-domain branch: substitute commutes with the NTT
+        ntt_sub = ctx.substitute(ctx.forward(a), k, ntt_repr=True)
+        assert np.array_equal(ntt_sub, ctx.forward(out.reshape(-1)))
+    # 11 = 3^-1 mod 32
+    assert np.array_equal(ctx.substitute(ctx.substitute(a, 3, False), 11, False), a)
+
+
+def test_poly_scaler_switcher_closed_form(oracle):
+    # math/src/rq/mod.rs:1044-1067 (Switcher = scale by to.modulus / from.modulus)
+    n = 16
+    c1 = oracle.Context(MODULI[:2], n)
+    c2 = oracle.Context(MODULI[3:], n)
+    Q1 = MODULI[0] * MODULI[1]
+    Q2 = 1
+    for m in MODULI[3:]:
+        Q2 *= m
+    sw = oracle.PolyScaler(c1, c2, Q2, Q1)
+    a = c1.random(1, seed=8)
+    ref = lift_poly(oracle, MODULI[:2], a, n)
+    out = sw.scale(a, ntt_repr=False)
+    assert lift_poly(oracle, MODULI[3:], out, n) == [((b * Q2) + (Q1 >> 1)) // Q1 for b in ref]
+    # NTT branch equals power-basis branch framed by transforms
+    out_ntt = sw.scale(c1.forward(a), ntt_repr=True)
+    assert np.array_equal(out_ntt, c2.forward(out))
+
+
+def test_key_switch_bigint_recomputation(oracle):
+    """key_switch(p) = sum_i p_i * k_i over the key moduli, where p_i is the i-th
+    residue row of p read as an integer polynomial (key_switching_key.rs:214-241),
+    recomputed with Python integers from the key in power basis."""
+    n = 16
+    bfv = oracle.Bfv(n, 1153, oracle.generate_moduli([62, 62, 62], n), sk_seed=2)
+    ct_level, key_level = 1, 0
+    rk = bfv.relin_key(ct_level=ct_level, key_level=key_level, seed=4)
+    cm = bfv.level_moduli(ct_level)
+    km = bfv.level_moduli(key_level)
+    kctx = oracle.Context(km, n)
+    c0, c1 = rk.export()
+    p = oracle.random_polys(cm, n, 1, seed=77)
+    o0, o1 = rk.key_switch(p)
+    P = p.reshape(len(cm), n)
+    for key, out in ((c0, o0), (c1, o1)):
+        kpb = [kctx.backward(key[i].reshape(-1)).reshape(len(km), n) for i in range(len(cm))]
+        out_pb = kctx.backward(out).reshape(len(km), n)
+        for j, q in enumerate(km):
+            acc = [0] * n
+            for i in range(len(cm)):
+                term = negacyclic_mul([int(x) for x in P[i]], [int(x) for x in kpb[i][j]], q)
+                acc = [(x + y) % q for x, y in zip(acc, term)]
+            assert [int(x) for x in out_pb[j]] == acc
+
+
+def _slots(rng, n, t):
+    return [rng.randrange(t) for _ in range(n)]
+
+
+@pytest.mark.parametrize("L,ct_level,key_level", [(2, 0, 0), (3, 0, 0), (3, 1, 0), (4, 1, 0), (4, 2, 1)])
+def test_mul_relin_decrypts_tiny(oracle, L, ct_level, key_level):
+    n, t = 16, 1153
+    bfv = oracle.Bfv(n, t, oracle.generate_moduli([62] * L, n), sk_seed=10 + L)
+    rng = random.Random(L)
+    v1, v2 = _slots(rng, n, t), _slots(rng, n, t)
+    c1 = bfv.encrypt(v1, level=ct_level, seed=21)
+    c2 = bfv.encrypt(v2, level=ct_level, seed=22)
+    assert [int(x) for x in bfv.decrypt(c1, level=ct_level)] == v1
+    prod = [(a * b) % t for a, b in zip(v1, v2)]
+    # no relinearization: 3-part ciphertext decrypts to the product
+    m0 = oracle.Multiplicator(bfv, None, level=ct_level)
+    ct3 = m0.multiply(c1, c2)
+    assert [int(x) for x in bfv.decrypt(ct3, level=ct_level)] == prod
+    assert np.array_equal(ct3, bfv.ct_mul(c1, c2, level=ct_level))
+    rk = bfv.relin_key(ct_level=ct_level, key_level=key_level, seed=5)
+    m = oracle.Multiplicator(bfv, rk, level=ct_level)
+    ct = m.multiply(c1, c2)
+    assert [int(x) for x in bfv.decrypt(ct, level=ct_level)] == prod
+    # mul.rs path == (&ct * &ct) then relinearizes (relinearization_key.rs:71-98)
+    assert np.array_equal(ct, bfv.relinearize(rk, ct3))
+
+
+def _rotate_columns(v, k):
+    h = len(v) // 2
+    return v[k:h] + v[:k] + v[h + k:] + v[h:h + k]
+
+
+@pytest.mark.parametrize("key_level", [0, 1])
+def test_galois_rotation_decrypts_tiny(oracle, key_level):
+    n, t = 16, 1153
+    bfv = oracle.Bfv(n, t, oracle.generate_moduli([62] * 3, n), sk_seed=31)
+    rng = random.Random(key_level)
+    v = _slots(rng, n, t)
+    ct = bfv.encrypt(v, level=1, seed=3)
+    for k in [1, 2, 3]:
+        e = pow(3, k, 2 * n)
+        gk = bfv.galois_key(e, ct_level=1, key_level=key_level, seed=9 + k)
+        out = bfv.galois(gk, ct)
+        assert [int(x) for x in bfv.decrypt(out, level=1)] == _rotate_columns(v, k)
+    gk = bfv.galois_key(2 * n - 1, ct_level=1, key_level=key_level, seed=40)
+    out = bfv.galois(gk, ct)
+    h = n // 2
+    assert [int(x) for x in bfv.decrypt(out, level=1)] == v[h:] + v[:h]
+
+
+@pytest.mark.slow
+def test_mul_relin_decrypts_cfg1(oracle):
+    """N=8192, 3 x 62-bit, t=65537 (the single-instance configuration)."""
+    n, t = 8192, 65537
+    bfv = oracle.Bfv(n, t, oracle.generate_moduli([62] * 3, n), sk_seed=1)
+    rng = random.Random(1)
+    v1, v2 = _slots(rng, n, t), _slots(rng, n, t)
+    c1, c2 = bfv.encrypt(v1, seed=2), bfv.encrypt(v2, seed=3)
+    rk = bfv.relin_key(seed=4)
+    m = oracle.Multiplicator(bfv, rk)
+    assert m.ext_moduli()[3:] == [4611686018426257409, 4611686018425815041, 4611686018424881153,
+                                  4611686018424733697]
+    ct = m.multiply(c1, c2)
+    assert [int(x) for x in bfv.decrypt(ct)] == [(a * b) % t for a, b in zip(v1, v2)]
