@@ -1,0 +1,79 @@
+// Argument blocks and launchers of the BFV kernels (bfv_kernels.hip).
+#pragma once
+
+#include "fhehip_internal.h"
+
+namespace fhehip {
+
+struct InttRowsArgs {
+    const u64* src[4];        // per part: base of instance 0, row 0
+    size_t src_inst_stride;   // elements between instances (same for all parts)
+    u64* dst;
+    size_t dst_inst_stride;
+    size_t dst_part_stride;
+    const ulonglong2* inv_tw;  // context tables
+    const ModInfo* mods;
+    u32 table_offset;         // context limb of row 0
+    const u32* perm;          // optional NTT-domain gather: q[m] = p[perm[m]]
+};
+
+struct TensorArgs {
+    const u64* lhs;  // [inst][2][L][N] NTT
+    const u64* rhs;
+    size_t in_inst_stride, in_part_stride;
+    const u64* ext;  // [inst][4][M - common][N] NTT (extension limbs of a0, a1, b0, b1)
+    size_t ext_inst_stride, ext_part_stride;
+    u32 common_lhs, common_rhs;
+    u64* out;  // [inst][3][M][N] power basis
+    size_t out_inst_stride, out_part_stride;
+    const ulonglong2* inv_tw;  // multiplication context
+    const ModInfo* mods;
+};
+
+struct KeySwitchArgs {
+    const u64* src;  // [inst][digits][N] power basis (digit i = ciphertext limb i)
+    size_t src_inst_stride;
+    u32 digits, key_limbs;
+    const ulonglong2* key0;  // [digits][key_limbs][N] (value, shoup)
+    const ulonglong2* key1;
+    const ulonglong2* key_fwd_tw;
+    const ulonglong2* key_inv_tw;
+    const ModInfo* key_mods;
+    u64* acc;  // [inst][2][key_limbs][N]
+    size_t acc_inst_stride, acc_part_stride;
+    u32 to_power_basis;
+};
+
+// Constants of one switch-down step applied to one remaining limb.
+struct SdConst {
+    u64 q, bl, bh;    // modulus of the remaining limb and its Barrett constant
+    u64 half_mod;     // q - (floor(q_last / 2) mod q)
+    u64 inv, inv_shoup;  // q_last^-1 mod q
+};
+
+struct SdNttArgs {
+    const u64* ks;  // [inst][parts][L + extra][N] power basis
+    size_t ks_inst_stride, ks_part_stride;
+    u32 L, extra;
+    const u64* extra_mods;  // [extra] moduli of the dropped limbs
+    const SdConst* sd;      // [extra][sd_stride]
+    u32 sd_stride;
+    const u64* add_pb;  // optional power-basis addend [inst][parts][L][N]
+    size_t add_inst_stride, add_part_stride;
+    const u64* add_ntt;  // optional NTT-domain addend (part 0 only), gathered with perm
+    size_t add_ntt_inst_stride;
+    const u32* perm;
+    u64* out;  // [inst][parts][L][N] NTT, reduced
+    size_t out_inst_stride, out_part_stride;
+    const ulonglong2* fwd_tw;  // ciphertext context
+    const ModInfo* mods;
+};
+
+void launch_rns_scale(const u64* in, size_t in_poly_stride, u64* out, size_t out_poly_stride, const u64* d_sc,
+                      u32 n, u32 polys, hipStream_t s);
+void launch_intt_rows(int logn, const InttRowsArgs& a, u32 limbs, u32 parts, u32 inst, hipStream_t s);
+void launch_tensor_intt(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_t s);
+void launch_keyswitch(int logn, const KeySwitchArgs& a, u32 key_limbs, u32 inst, hipStream_t s);
+void launch_sd_add_ntt(int logn, const SdNttArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s);
+
+}  // namespace fhehip

@@ -1,0 +1,94 @@
+// Resident context tables (rq::Context + NttOperator per modulus).
+#include <cstring>
+
+#include "fhehip_internal.h"
+
+namespace fhehip {
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        int code = (e == hipErrorOutOfMemory) ? -5 : -6;
+        throw Error(code, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+DevBuf::DevBuf(size_t n) { reset(n); }
+
+DevBuf::~DevBuf() {
+    if (ptr) (void)hipFree(ptr);
+}
+
+void DevBuf::reset(size_t n) {
+    if (ptr) {
+        (void)hipFree(ptr);
+        ptr = nullptr;
+    }
+    bytes = n;
+    if (n) HIPCHK(hipMalloc(&ptr, n));
+}
+
+// NttOperator::new tables (math/src/ntt/native.rs:35-73) for every modulus,
+// uploaded as interleaved (value, shoup) pairs.
+std::shared_ptr<DeviceContext> make_context(const std::vector<u64>& moduli, u32 n, const u64* psi) {
+    if (moduli.empty()) throw Error(-1, "empty moduli list");
+    if (n < 8 || (n & (n - 1))) throw Error(-1, "The degree is not a power of two larger or equal to 8");
+    if (n > 16384) throw Error(-2, "degree above 16384 is not supported by the LDS-resident NTT kernels");
+    auto c = std::make_shared<DeviceContext>();
+    c->moduli = moduli;
+    c->L = (u32)moduli.size();
+    c->N = n;
+    c->logn = ilog2_u64(n);
+    for (u64 p : moduli) {
+        if ((p >> 62) != 0 || p < 2) throw Error(-1, "InvalidModulus");
+        if (!supports_ntt64(p, n)) throw Error(-2, "Impossible to construct a Ntt operator");
+    }
+    try {
+        c->rns.init(moduli);
+    } catch (const std::invalid_argument& e) {
+        throw Error(-1, e.what());
+    }
+    std::vector<ulonglong2> fwd((size_t)c->L * n), inv((size_t)c->L * n);
+    std::vector<u64> pw(n), pwi(n);
+    for (u32 l = 0; l < c->L; l++) {
+        const u64 p = moduli[l];
+        u64 r = psi ? psi[l] : primitive_root64(n, p);
+        if (psi && (powmod64(r, n, p) != p - 1)) throw Error(-1, "psi is not a primitive 2N-th root of unity");
+        c->psi.push_back(r);
+        const u64 rinv = invmod64(r, p);
+        pw[0] = 1;
+        for (u32 i = 1; i < n; i++) pw[i] = mulmod64(pw[i - 1], r, p);
+        pwi[0] = rinv;
+        for (u32 i = 1; i < n; i++) pwi[i] = mulmod64(pwi[i - 1], rinv, p);
+        for (u32 i = 0; i < n; i++) {
+            u32 j = bit_reverse(i, c->logn);
+            fwd[(size_t)l * n + i] = make_ulonglong2(pw[j], shoup64(pw[j], p));
+            inv[(size_t)l * n + i] = make_ulonglong2(pwi[j], shoup64(pwi[j], p));
+        }
+        ModInfo mi;
+        std::memset(&mi, 0, sizeof mi);
+        mi.p = p;
+        barrett128(p, mi.barrett_lo, mi.barrett_hi);
+        mi.n_inv = invmod64(n % p, p);
+        mi.n_inv_shoup = shoup64(mi.n_inv, p);
+        c->h_mod.push_back(mi);
+    }
+    // q_last^-1 mod q_i (context.rs:67-74)
+    std::vector<ulonglong2> il;
+    for (u32 l = 0; l + 1 < c->L; l++) {
+        u64 v = invmod64(moduli.back() % moduli[l], moduli[l]);
+        il.push_back(make_ulonglong2(v, shoup64(v, moduli[l])));
+    }
+    c->fwd_tw.reset(fwd.size() * sizeof(ulonglong2));
+    c->inv_tw.reset(inv.size() * sizeof(ulonglong2));
+    c->mod.reset(c->h_mod.size() * sizeof(ModInfo));
+    HIPCHK(hipMemcpy(c->fwd_tw.ptr, fwd.data(), c->fwd_tw.bytes, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->inv_tw.ptr, inv.data(), c->inv_tw.bytes, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->mod.ptr, c->h_mod.data(), c->mod.bytes, hipMemcpyHostToDevice));
+    if (!il.empty()) {
+        c->inv_last.reset(il.size() * sizeof(ulonglong2));
+        HIPCHK(hipMemcpy(c->inv_last.ptr, il.data(), c->inv_last.bytes, hipMemcpyHostToDevice));
+    }
+    return c;
+}
+
+}  // namespace fhehip

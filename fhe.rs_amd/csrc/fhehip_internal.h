@@ -1,0 +1,74 @@
+// Internal host-side declarations of the fhehip library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "device_math.h"
+#include "host_math.h"
+
+namespace fhehip {
+
+// Error with a status code from include/fhehip.h.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what);
+#define HIPCHK(x) ::fhehip::hip_check((x), #x)
+
+// Device allocation owned by a handle.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() {}
+    explicit DevBuf(size_t n);
+    ~DevBuf();
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    void reset(size_t n);
+    template <class T>
+    T* as() const { return (T*)ptr; }
+};
+
+// Resident per-context data (rq::Context, math/src/rq/context.rs:9-19).
+struct DeviceContext {
+    std::vector<u64> moduli;
+    u32 L = 0, N = 0;
+    int logn = 0;
+    std::vector<u64> psi;
+    std::vector<ModInfo> h_mod;
+    DevBuf fwd_tw;   // [L][N] ulonglong2 (omega, omega_shoup)
+    DevBuf inv_tw;   // [L][N] ulonglong2 (zeta_inv, zeta_inv_shoup)
+    DevBuf mod;      // [L] ModInfo
+    DevBuf inv_last; // [L-1] ulonglong2 (q_last^-1 mod q_i, shoup)
+    RnsBasis rns;    // host-side CRT constants
+
+    const ulonglong2* fwd(u32 limb) const { return fwd_tw.as<ulonglong2>() + (size_t)limb * N; }
+    const ulonglong2* inv(u32 limb) const { return inv_tw.as<ulonglong2>() + (size_t)limb * N; }
+};
+
+std::shared_ptr<DeviceContext> make_context(const std::vector<u64>& moduli, u32 n, const u64* psi);
+
+// Rows of a batch of polynomials laid out [poly][limbs_per_poly][N]. Row r of a
+// launch covers limb first_limb + r % rows_per_poly of poly r / rows_per_poly.
+struct RowMap {
+    u64* data;
+    u32 rows_per_poly;
+    u32 limbs_per_poly;
+    u32 first_limb;
+    u32 table_offset;  // context limb index of limb first_limb
+};
+
+// NTT launchers (ntt_kernels.hip). `rows` = total rows in the launch.
+void launch_ntt_forward(const DeviceContext& c, RowMap m, u32 rows, bool lazy, hipStream_t s);
+void launch_ntt_backward(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s);
+
+}  // namespace fhehip

@@ -1,0 +1,195 @@
+// Negacyclic NTT building blocks for one polynomial row held by one workgroup.
+//
+// Semantics follow NttOperator (math/src/ntt/native.rs:77-234): the forward
+// transform is the Cooley-Tukey network with merged psi twiddles
+// omegas[k] = psi^brv(k) and bit-reversed output; the inverse is the
+// Gentleman-Sande network with zetas_inv[k] = psi^-(brv(k)+1) followed by a
+// multiplication by N^-1. The butterflies are the Harvey lazy ones of
+// native.rs:256-300 (values < 4p forward, < 2p inverse).
+//
+// Layout of the work: a row of N = 2^LOGN u64 is split over T = N / 16
+// threads, each holding E = 16 values in registers. A "pass" runs up to four
+// consecutive stages on those registers; between passes the row is exchanged
+// through LDS (N + N/16 words with one pad word every 16 to spread banks).
+#pragma once
+
+#include "device_math.h"
+
+namespace fhehip {
+
+template <int LOGN>
+struct NttShape {
+    static constexpr int N = 1 << LOGN;
+    static constexpr int RE = LOGN >= 4 ? 4 : LOGN;  // log2 elements per thread
+    static constexpr int E = 1 << RE;
+    static constexpr int T = N / E;  // threads per row
+    static constexpr int LDS_WORDS = N + (N >> 4) + 1;
+};
+
+__device__ __forceinline__ int lds_slot(int j) { return j + (j >> 4); }
+
+// Forward butterfly (native.rs:256-269): x, y < 4p -> x', y' < 4p.
+__device__ __forceinline__ void fwd_bfly(u64& x, u64& y, u64 w, u64 ws, u64 p, u64 p2) {
+    u64 xr = reduce1(x, p2);
+    u64 t = lazy_mul_shoup(y, w, ws, p);
+    y = xr + p2 - t;
+    x = xr + t;
+}
+
+// Inverse butterfly (native.rs:288-300): x, y < 2p -> x', y' < 2p.
+__device__ __forceinline__ void inv_bfly(u64& x, u64& y, u64 z, u64 zs, u64 p, u64 p2) {
+    u64 t = x;
+    x = reduce1(y + t, p2);
+    y = lazy_mul_shoup(t + p2 - y, z, zs, p);
+}
+
+// Element index owned by thread t in slot k for a forward pass whose
+// (virtual) first stage is S0V: stage s pairs bit LOGN-1-s.
+template <int LOGN, int S0V>
+__device__ __forceinline__ int fwd_index(int t, int k) {
+    constexpr int RE = NttShape<LOGN>::RE;
+    constexpr int LOB = LOGN - S0V - RE;
+    return ((t >> LOB) << (LOGN - S0V)) | (k << LOB) | (t & ((1 << LOB) - 1));
+}
+
+// Element index for an inverse pass whose (virtual) first stage pairs bit B0V.
+template <int LOGN, int B0V>
+__device__ __forceinline__ int inv_index(int t, int k) {
+    constexpr int RE = NttShape<LOGN>::RE;
+    return ((t >> B0V) << (B0V + RE)) | (k << B0V) | (t & ((1 << B0V) - 1));
+}
+
+// Forward stages S0V .. S0V+RE-1, skipping stages below SMIN.
+// tw: interleaved (omega, omega_shoup) pairs of the row's modulus.
+template <int LOGN, int S0V, int SMIN>
+__device__ __forceinline__ void fwd_pass(u64 (&x)[NttShape<LOGN>::E], int t, const ulonglong2* __restrict__ tw,
+                                         u64 p, u64 p2) {
+    constexpr int RE = NttShape<LOGN>::RE;
+    constexpr int LOB = LOGN - S0V - RE;
+    const int hi = t >> LOB;
+#pragma unroll
+    for (int u = 0; u < RE; u++) {
+        const int s = S0V + u;
+        if (s < SMIN) continue;
+        const int half = 1 << (RE - 1 - u);
+#pragma unroll
+        for (int g = 0; g < (1 << u); g++) {
+            const ulonglong2 w = tw[(1 << s) + (hi << u) + g];
+#pragma unroll
+            for (int q = 0; q < half; q++) {
+                const int k0 = (g << (RE - u)) + q;
+                fwd_bfly(x[k0], x[k0 + half], w.x, w.y, p, p2);
+            }
+        }
+    }
+}
+
+// Inverse stages pairing bits B0V .. B0V+RE-1, skipping bits below VMIN.
+// tw: interleaved (zeta_inv, zeta_inv_shoup) pairs of the row's modulus.
+template <int LOGN, int B0V, int VMIN>
+__device__ __forceinline__ void inv_pass(u64 (&x)[NttShape<LOGN>::E], int t, const ulonglong2* __restrict__ tw,
+                                         u64 p, u64 p2) {
+    constexpr int RE = NttShape<LOGN>::RE;
+    constexpr int N = 1 << LOGN;
+    const int hi = t >> B0V;
+#pragma unroll
+    for (int u = 0; u < RE; u++) {
+        const int v = B0V + u;
+        if (v < VMIN) continue;
+        const int half = 1 << u;
+        const int offset = N - (N >> v);
+#pragma unroll
+        for (int g = 0; g < (1 << (RE - 1 - u)); g++) {
+            const ulonglong2 z = tw[offset + (hi << (RE - 1 - u)) + g];
+#pragma unroll
+            for (int q = 0; q < half; q++) {
+                const int k0 = (g << (u + 1)) + q;
+                inv_bfly(x[k0], x[k0 + half], z.x, z.y, p, p2);
+            }
+        }
+    }
+}
+
+template <int LOGN, int S0V>
+__device__ __forceinline__ void lds_put_fwd(u64* lds, const u64 (&x)[NttShape<LOGN>::E], int t) {
+#pragma unroll
+    for (int k = 0; k < NttShape<LOGN>::E; k++) lds[lds_slot(fwd_index<LOGN, S0V>(t, k))] = x[k];
+}
+template <int LOGN, int S0V>
+__device__ __forceinline__ void lds_get_fwd(const u64* lds, u64 (&x)[NttShape<LOGN>::E], int t) {
+#pragma unroll
+    for (int k = 0; k < NttShape<LOGN>::E; k++) x[k] = lds[lds_slot(fwd_index<LOGN, S0V>(t, k))];
+}
+template <int LOGN, int B0V>
+__device__ __forceinline__ void lds_put_inv(u64* lds, const u64 (&x)[NttShape<LOGN>::E], int t) {
+#pragma unroll
+    for (int k = 0; k < NttShape<LOGN>::E; k++) lds[lds_slot(inv_index<LOGN, B0V>(t, k))] = x[k];
+}
+template <int LOGN, int B0V>
+__device__ __forceinline__ void lds_get_inv(const u64* lds, u64 (&x)[NttShape<LOGN>::E], int t) {
+#pragma unroll
+    for (int k = 0; k < NttShape<LOGN>::E; k++) x[k] = lds[lds_slot(inv_index<LOGN, B0V>(t, k))];
+}
+
+// The last pass of either direction always starts at LOGN - RE, so that its
+// element order is fwd_index<LOGN, LOGN-RE> (16 consecutive values per thread)
+// or inv_index<LOGN, LOGN-RE> (value k of thread t at k * T + t).
+template <int LOGN>
+constexpr int last_start() {
+    return LOGN - NttShape<LOGN>::RE;
+}
+
+template <int LOGN, int S0V>
+__device__ __forceinline__ void fwd_rest(u64* lds, u64 (&x)[NttShape<LOGN>::E], int t, const ulonglong2* tw, u64 p,
+                                         u64 p2) {
+    constexpr int RE = NttShape<LOGN>::RE;
+    // Called after the pass starting at S0V has run; continues with the next one.
+    if constexpr (S0V + RE < LOGN) {
+        constexpr int NEXT = (S0V + 2 * RE <= LOGN) ? S0V + RE : LOGN - RE;
+        constexpr int SMIN = S0V + RE;
+        lds_put_fwd<LOGN, S0V>(lds, x, t);
+        __syncthreads();
+        lds_get_fwd<LOGN, NEXT>(lds, x, t);
+        __syncthreads();
+        fwd_pass<LOGN, NEXT, SMIN>(x, t, tw, p, p2);
+        fwd_rest<LOGN, NEXT>(lds, x, t, tw, p, p2);
+    }
+}
+
+template <int LOGN, int B0V>
+__device__ __forceinline__ void inv_rest(u64* lds, u64 (&x)[NttShape<LOGN>::E], int t, const ulonglong2* tw, u64 p,
+                                         u64 p2) {
+    constexpr int RE = NttShape<LOGN>::RE;
+    if constexpr (B0V + RE < LOGN) {
+        constexpr int NEXT = (B0V + 2 * RE <= LOGN) ? B0V + RE : LOGN - RE;
+        constexpr int VMIN = B0V + RE;
+        lds_put_inv<LOGN, B0V>(lds, x, t);
+        __syncthreads();
+        lds_get_inv<LOGN, NEXT>(lds, x, t);
+        __syncthreads();
+        inv_pass<LOGN, NEXT, VMIN>(x, t, tw, p, p2);
+        inv_rest<LOGN, NEXT>(lds, x, t, tw, p, p2);
+    }
+}
+
+// Full forward transform on registers: input order fwd_index<LOGN,0>, output
+// order fwd_index<LOGN, last_start<LOGN>()>, values < 4p.
+template <int LOGN>
+__device__ __forceinline__ void ntt_fwd_regs(u64* lds, u64 (&x)[NttShape<LOGN>::E], int t, const ulonglong2* tw,
+                                             u64 p) {
+    const u64 p2 = 2 * p;
+    fwd_pass<LOGN, 0, 0>(x, t, tw, p, p2);
+    fwd_rest<LOGN, 0>(lds, x, t, tw, p, p2);
+}
+
+// Full inverse transform on registers (without the N^-1 factor): input order
+// inv_index<LOGN,0>, output order inv_index<LOGN, last_start<LOGN>()>, < 2p.
+template <int LOGN>
+__device__ __forceinline__ void ntt_inv_regs(u64* lds, u64 (&x)[NttShape<LOGN>::E], int t, const ulonglong2* tw,
+                                             u64 p) {
+    const u64 p2 = 2 * p;
+    inv_pass<LOGN, 0, 0>(x, t, tw, p, p2);
+    inv_rest<LOGN, 0>(lds, x, t, tw, p, p2);
+}
+
+}  // namespace fhehip

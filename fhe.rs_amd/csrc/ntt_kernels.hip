@@ -1,0 +1,113 @@
+// Batched negacyclic NTT kernels: one workgroup per polynomial row.
+#include "fhehip_internal.h"
+#include "ntt_device.h"
+
+namespace fhehip {
+
+__device__ __forceinline__ u64* row_ptr(const RowMap& m, u32 r, int logn, u32& limb) {
+    u32 poly = r / m.rows_per_poly;
+    u32 l = m.first_limb + r % m.rows_per_poly;
+    limb = l - m.first_limb + m.table_offset;
+    return m.data + (((size_t)poly * m.limbs_per_poly + l) << logn);
+}
+
+// Forward NTT (NttOperator::forward / forward_vt_lazy, native.rs:77-189).
+template <int LOGN, bool LAZY>
+__global__ __launch_bounds__(NttShape<LOGN>::T) void ntt_fwd_kernel(RowMap m, const ulonglong2* __restrict__ tw_all,
+                                                                   const ModInfo* __restrict__ mods) {
+    using S = NttShape<LOGN>;
+    __shared__ u64 lds[S::LDS_WORDS];
+    const int t = threadIdx.x;
+    u32 limb;
+    u64* row = row_ptr(m, blockIdx.x, LOGN, limb);
+    const ulonglong2* tw = tw_all + ((size_t)limb << LOGN);
+    const u64 p = mods[limb].p;
+    u64 x[S::E];
+#pragma unroll
+    for (int k = 0; k < S::E; k++) x[k] = row[fwd_index<LOGN, 0>(t, k)];
+    ntt_fwd_regs<LOGN>(lds, x, t, tw, p);
+    const u64 p2 = 2 * p;
+#pragma unroll
+    for (int k = 0; k < S::E; k++) {
+        u64 v = x[k];
+        if (!LAZY) v = reduce1(reduce1(v, p2), p);
+        x[k] = v;
+    }
+    // Output order: thread t owns the E consecutive values t*E .. t*E+E-1.
+    ulonglong2* out = reinterpret_cast<ulonglong2*>(row + ((size_t)t << S::RE));
+#pragma unroll
+    for (int k = 0; k < S::E / 2; k++) out[k] = make_ulonglong2(x[2 * k], x[2 * k + 1]);
+}
+
+// Inverse NTT (NttOperator::backward, native.rs:106-132).
+template <int LOGN>
+__global__ __launch_bounds__(NttShape<LOGN>::T) void ntt_inv_kernel(RowMap m, const ulonglong2* __restrict__ tw_all,
+                                                                   const ModInfo* __restrict__ mods) {
+    using S = NttShape<LOGN>;
+    __shared__ u64 lds[S::LDS_WORDS];
+    const int t = threadIdx.x;
+    u32 limb;
+    u64* row = row_ptr(m, blockIdx.x, LOGN, limb);
+    const ulonglong2* tw = tw_all + ((size_t)limb << LOGN);
+    const ModInfo mi = mods[limb];
+    u64 x[S::E];
+    const ulonglong2* in = reinterpret_cast<const ulonglong2*>(row + ((size_t)t << S::RE));
+#pragma unroll
+    for (int k = 0; k < S::E / 2; k++) {
+        ulonglong2 v = in[k];
+        x[2 * k] = v.x;
+        x[2 * k + 1] = v.y;
+    }
+    ntt_inv_regs<LOGN>(lds, x, t, tw, mi.p);
+#pragma unroll
+    for (int k = 0; k < S::E; k++)
+        row[inv_index<LOGN, last_start<LOGN>()>(t, k)] = mul_shoup(x[k], mi.n_inv, mi.n_inv_shoup, mi.p);
+}
+
+template <int LOGN>
+static void fwd_launch(const DeviceContext& c, RowMap m, u32 rows, bool lazy, hipStream_t s) {
+    dim3 grid(rows), block(NttShape<LOGN>::T);
+    if (lazy)
+        hipLaunchKernelGGL((ntt_fwd_kernel<LOGN, true>), grid, block, 0, s, m, c.fwd_tw.as<ulonglong2>(),
+                           c.mod.as<ModInfo>());
+    else
+        hipLaunchKernelGGL((ntt_fwd_kernel<LOGN, false>), grid, block, 0, s, m, c.fwd_tw.as<ulonglong2>(),
+                           c.mod.as<ModInfo>());
+}
+
+template <int LOGN>
+static void inv_launch(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s) {
+    dim3 grid(rows), block(NttShape<LOGN>::T);
+    hipLaunchKernelGGL((ntt_inv_kernel<LOGN>), grid, block, 0, s, m, c.inv_tw.as<ulonglong2>(), c.mod.as<ModInfo>());
+}
+
+#define FHEHIP_LOGN_SWITCH(logn, F, ...)                                    \
+    switch (logn) {                                                         \
+        case 3: F<3>(__VA_ARGS__); break;                                   \
+        case 4: F<4>(__VA_ARGS__); break;                                   \
+        case 5: F<5>(__VA_ARGS__); break;                                   \
+        case 6: F<6>(__VA_ARGS__); break;                                   \
+        case 7: F<7>(__VA_ARGS__); break;                                   \
+        case 8: F<8>(__VA_ARGS__); break;                                   \
+        case 9: F<9>(__VA_ARGS__); break;                                   \
+        case 10: F<10>(__VA_ARGS__); break;                                 \
+        case 11: F<11>(__VA_ARGS__); break;                                 \
+        case 12: F<12>(__VA_ARGS__); break;                                 \
+        case 13: F<13>(__VA_ARGS__); break;                                 \
+        case 14: F<14>(__VA_ARGS__); break;                                 \
+        default: throw Error(-2, "degree not supported by the NTT kernels"); \
+    }
+
+void launch_ntt_forward(const DeviceContext& c, RowMap m, u32 rows, bool lazy, hipStream_t s) {
+    if (rows == 0) return;
+    FHEHIP_LOGN_SWITCH(c.logn, fwd_launch, c, m, rows, lazy, s);
+    HIPCHK(hipGetLastError());
+}
+
+void launch_ntt_backward(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s) {
+    if (rows == 0) return;
+    FHEHIP_LOGN_SWITCH(c.logn, inv_launch, c, m, rows, s);
+    HIPCHK(hipGetLastError());
+}
+
+}  // namespace fhehip

@@ -1,0 +1,90 @@
+"""GPU parity of the batched NTT kernels against the oracle (bit-exact)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CFG2_MODULI = [4611686018427322369, 4611686018427289601, 4611686018425815041, 4611686018424733697,
+               4611686018423881729, 4611686018423390209, 4611686018423062529, 4611686018422669313]
+
+
+@pytest.fixture(scope="module")
+def fh():
+    import torch
+    import fhehip
+    assert torch.cuda.is_available()
+    fhehip.lib()
+    return fhehip
+
+
+@pytest.mark.parametrize("n,L", [(8, 1), (16, 2), (32, 3), (64, 2), (1024, 3), (4096, 2), (8192, 3), (16384, 2)])
+def test_ntt_matches_oracle(fh, oracle, n, L):
+    moduli = oracle.generate_moduli([62] * L, n)
+    print(f"ntt parity n={n} L={L}", flush=True)
+    ctx = fh.Context(moduli, n)
+    octx = oracle.Context(moduli, n)
+    assert ctx.psi() == [int(x) for x in octx.psi()]
+    count = 3
+    a = oracle.random_polys(moduli, n, count, seed=n + L)
+    d = fh.to_device(a)
+    ctx.ntt_forward(d, count)
+    fwd = fh.to_host(d)
+    assert np.array_equal(fwd, octx.forward(a))
+    ctx.ntt_backward(d, count)
+    assert np.array_equal(fh.to_host(d), a)
+    # lazy variant: outputs < 4p and congruent to the reduced transform
+    d2 = fh.to_device(a)
+    ctx.ntt_forward(d2, count, lazy=True)
+    lz = fh.to_host(d2).reshape(count, L, n)
+    ref = fwd.reshape(count, L, n)
+    for l, p in enumerate(moduli):
+        assert (lz[:, l] < np.uint64(4 * p)).all() if 4 * p < 2**64 else True
+        assert np.array_equal(lz[:, l] % np.uint64(p), ref[:, l])
+
+
+def test_ntt_edge_values(fh, oracle):
+    n = 4096
+    moduli = [4611686018427322369, 1152921504606748673]
+    ctx = fh.Context(moduli, n)
+    octx = oracle.Context(moduli, n)
+    a = np.zeros((3, 2, n), np.uint64)
+    a[1] = np.array(moduli, np.uint64)[:, None] - np.uint64(1)  # all p-1
+    a[2, :, 0] = 1  # delta
+    d = fh.to_device(a.reshape(-1))
+    ctx.ntt_forward(d, 3)
+    assert np.array_equal(fh.to_host(d), octx.forward(a.reshape(-1)))
+    ctx.ntt_backward(d, 3)
+    assert np.array_equal(fh.to_host(d), a.reshape(-1))
+
+
+def test_ntt_cfg2_shape_roundtrip(fh, oracle):
+    """256 polynomials x 8 x 62-bit primes x N=16384: forward vs oracle on every
+    row, then the inverse returns the input bit for bit."""
+    n, count = 16384, 256
+    print("ntt cfg2 shape", flush=True)
+    ctx = fh.Context(CFG2_MODULI, n)
+    octx = oracle.Context(CFG2_MODULI, n)
+    a = oracle.random_polys(CFG2_MODULI, n, count, seed=2024)
+    d = fh.to_device(a)
+    ctx.ntt_forward(d, count)
+    assert np.array_equal(fh.to_host(d), octx.forward(a))
+    ctx.ntt_backward(d, count)
+    assert np.array_equal(fh.to_host(d), a)
+
+
+def test_ntt_explicit_psi(fh, oracle):
+    n, p = 1024, 4611686018427322369
+    ps = oracle.primitive_root(n, p)
+    other = pow(ps, 3, p)  # another primitive 2N-th root
+    ctx = fh.Context([p], n, psi=[other])
+    assert ctx.psi() == [other]
+    a = oracle.random_polys([p], n, 1, seed=5)
+    d = fh.to_device(a)
+    ctx.ntt_forward(d, 1)
+    assert np.array_equal(fh.to_host(d), oracle.ntt(p, n, a, "forward", psi=other))
+
+
+def test_ntt_unsupported_modulus(fh):
+    with pytest.raises(fh.FheHipError) as e:
+        fh.Context([4611686018427387847], 1024)  # prime but not 1 mod 2048
+    assert e.value.code == fh.ERR_UNSUPPORTED
