@@ -1,0 +1,247 @@
+#!/usr/bin/env python3
+"""Headline benchmark: batched ct x ct multiply + relinearize at N = 2^14.
+
+Workload: 512 independent BFV ciphertext pairs at N=16384 with 6 x 60-bit
+ciphertext primes and one 60-bit special prime (key at the level above the
+ciphertexts), t = 65537, Multiplicator::default with a relinearization key.
+One step = one fhehip_multiply call over the whole batch (extension, tensor,
+t/Q down-scaling, key switch, switch-down), inputs already resident in HBM.
+
+Also measured: the per-call GPU drop-in (the same entry point called once per
+operation), per-kernel durations with HIP events on the launch stream for the
+roofline figure, and the CPU baseline: the oracle (a C++ restatement of the
+fhe.rs CPU path) timed on the host cores on a bounded sample, which also checks
+one GPU output bit for bit.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "fhe.rs_amd"))
+
+METRIC = "ct×ct mul+relin/sec at N=2^14 (batched vs per-call GPU), bit-exact; p99 ms"
+UNIT = "mul+relin/s"
+N = 16384
+# generate_moduli([60] * 7, 16384) of fhe/src/bfv/parameters.rs:383-423 (pinned in
+# tests/test_oracle_zq.py::test_config_moduli): 6 ciphertext primes + 1 special.
+MODULI = [1152921504606748673, 1152921504606683137, 1152921504606584833, 1152921504605962241,
+          1152921504604979201, 1152921504600260609, 1152921504599080961]
+T_PLAIN = 65537
+HBM_PEAK_GBS = 8000.0
+
+
+def percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = (len(xs) - 1) * q / 100.0
+    f = int(k)
+    c = min(f + 1, len(xs) - 1)
+    return xs[f] + (xs[c] - xs[f]) * (k - f)
+
+
+def kernel_bytes(name, B, L, Lk, M):
+    """Algorithmic HBM bytes of one launch of a kernel of the mul+relin
+    pipeline for a batch of B instances (each u64 row = N * 8 bytes read or
+    written once)."""
+    row = N * 8
+    kind = name.split("/")[0]
+    if kind == "intt_rows":      # 4 input polys x L rows, read + write
+        return B * 4 * L * row * 2
+    if kind == "rns_scale":      # extension (2 polys L -> M-L) or down-scale (3 polys M -> L); average case
+        return None
+    if kind == "ntt_forward":    # the new extension rows, in place
+        return B * 4 * (M - L) * row * 2
+    if kind == "tensor_intt":    # read a0,a1,b0,b1 over M limbs, write 3 x M rows
+        return B * (4 + 3) * M * row
+    if kind == "keyswitch":      # read c2 (L rows), write (c0, c1) over Lk rows; key rows once
+        return (B * (L + 2 * Lk) + 2 * L * Lk * 2) * row
+    if kind == "ntt_backward":   # the key-switch accumulators, in place
+        return B * 2 * Lk * row * 2
+    if kind == "sd_add_ntt":     # read ks rows (L + extra per part) + D rows, write 2 x L rows
+        return B * 2 * (Lk + L + L) * row
+    return None
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary, when present."""
+    path = os.path.join(ROOT, "profiles", "traffic_mul_relin_n16384_b512.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(lhs_h, rhs_h, c0, c1, gpu_out0, budget_s):
+    """Oracle (C++ restatement of the fhe.rs CPU path) on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyoracle as oracle
+    bfv = oracle.Bfv(N, T_PLAIN, MODULI, sk_seed=1)
+    rk = bfv.key_from_rows(c0, c1, ct_level=1, key_level=0)
+    om = oracle.Multiplicator(bfv, rk, level=1)
+    ref = om.multiply(lhs_h, rhs_h)
+    bit_exact = bool(np.array_equal(ref, gpu_out0))
+    # one op to size the sample
+    t0 = time.perf_counter()
+    om.multiply(lhs_h, rhs_h)
+    one = time.perf_counter() - t0
+    threads = max(1, min(16, os.cpu_count() or 1))
+    reps = max(1, int(budget_s / max(one, 1e-3) / 1.5))
+    wall, lat0 = om.bench(lhs_h, rhs_h, threads, reps)
+    return {
+        "value": threads * reps / wall,
+        "unit": UNIT,
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{threads} threads x {reps} Multiplicator::multiply each at the same shape "
+                  f"(oracle C++ restatement, -O3); single-op p50 {percentile(list(lat0 * 1e3), 50):.1f} ms",
+        "p50_ms_1core": percentile(list(lat0 * 1e3), 50),
+    }, bit_exact
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--per-call-ops", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import fhehip
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    dev = torch.device("cuda:0")
+    B, L, Lk = args.batch, len(MODULI) - 1, len(MODULI)
+
+    # Resident contexts and a relinearization key with seeded uniform rows
+    # (the key values do not change the work; they are checked by the oracle).
+    rng = np.random.default_rng(2024)
+    cctx = fhehip.Context(MODULI[:L], N)
+    kctx = fhehip.Context(MODULI, N)
+    c0 = np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in MODULI]) for _ in range(L)])
+    c1 = np.stack([np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in MODULI]) for _ in range(L)])
+    ksk = fhehip.KeySwitchingKey(cctx, kctx, c0, c1)
+    mult = fhehip.Multiplicator(rk=ksk, plaintext=T_PLAIN)
+    M = len(mult.extended_basis())
+
+    # Seeded uniform ciphertext residues, generated on the device.
+    g = torch.Generator(device=dev)
+    g.manual_seed(7)
+    per = 2 * L * N
+
+    def ct_batch(count):
+        t = torch.empty(count * per, dtype=torch.int64, device=dev)
+        v = t.view(count, 2, L, N)
+        for i, q in enumerate(MODULI[:L]):
+            v[:, :, i, :] = torch.randint(0, q, (count, 2, N), dtype=torch.int64, device=dev, generator=g)
+        return t
+
+    lhs, rhs = ct_batch(B), ct_batch(B)
+    out = torch.empty(B * per, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream()
+
+    for _ in range(args.warmup):
+        mult.multiply(lhs, rhs, out, B, stream)
+    torch.cuda.synchronize()
+
+    # Timed region: K batched steps, per-step events, per-kernel events.
+    fhehip.profiler_enable(True)
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    evs[0].record(stream)
+    for k in range(args.steps):
+        mult.multiply(lhs, rhs, out, B, stream)
+        evs[k + 1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernels = fhehip.profiler_report()
+    fhehip.profiler_enable(False)
+    step_ms = [evs[k].elapsed_time(evs[k + 1]) for k in range(args.steps)]
+    total_ms = evs[0].elapsed_time(evs[-1])
+    value = B * args.steps / (total_ms / 1e3)
+
+    # Per-call GPU drop-in: one operation per call.
+    one_l, one_r, one_o = lhs[:per], rhs[:per], out[:per]
+    for _ in range(3):
+        mult.multiply(one_l, one_r, one_o, 1, stream)
+    torch.cuda.synchronize()
+    pe = [torch.cuda.Event(enable_timing=True) for _ in range(args.per_call_ops + 1)]
+    pe[0].record(stream)
+    for k in range(args.per_call_ops):
+        mult.multiply(lhs[k % B * per:(k % B + 1) * per], rhs[k % B * per:(k % B + 1) * per], one_o, 1, stream)
+        pe[k + 1].record(stream)
+    torch.cuda.synchronize()
+    call_ms = [pe[k].elapsed_time(pe[k + 1]) for k in range(args.per_call_ops)]
+    per_call_value = args.per_call_ops / (pe[0].elapsed_time(pe[-1]) / 1e3)
+
+    # Roofline of the dominant kernel.
+    dom = max(kernels.items(), key=lambda kv: kv[1][1])
+    dname, (dcalls, dms) = dom
+    avg_s = dms / dcalls / 1e3
+    abytes = kernel_bytes(dname, B, L, Lk, M)
+    achieved = abytes / avg_s / 1e9 if abytes else None
+    traffic = load_traffic(dname.split("/")[0])
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
+                "kernel": dname, "avg_launch_ms": avg_s * 1e3, "algorithmic_bytes": abytes}
+
+    cpu = None
+    bit_exact = None
+    if not args.no_cpu:
+        out0 = fhehip.to_host(out[:per]).copy()
+        cpu, bit_exact = cpu_baseline(fhehip.to_host(lhs[:per]).copy(), fhehip.to_host(rhs[:per]).copy(),
+                                      c0, c1, out0, args.cpu_seconds)
+
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": UNIT,
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": total_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic: seeded uniform ciphertext residues and relinearization-key rows",
+        "config": {"workload": "batched uniform BFV: 512 x ct*ct multiply+relinearize, N=16384, "
+                               "6x60-bit ciphertext primes + 1 special prime, t=65537",
+                   "batch": B, "degree": N, "ct_moduli": L, "key_moduli": Lk, "mul_moduli": M,
+                   "parallelism": "single GPU"},
+        "p50_step_ms": percentile(step_ms, 50),
+        "p99_step_ms": percentile(step_ms, 99),
+        "per_call": {"value": per_call_value, "unit": UNIT, "p50_ms": percentile(call_ms, 50),
+                     "p99_ms": percentile(call_ms, 99), "ops": args.per_call_ops},
+        "batched_over_per_call": value / per_call_value,
+        "bit_exact_vs_oracle": bit_exact,
+        "kernels_ms": {k: {"launches": c, "avg_ms": ms / c} for k, (c, ms) in sorted(kernels.items())},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
+        "host_wall_s": wall,
+    }
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,395 @@
+// Host orchestration of the BFV evaluation path: rq::Scaler, key-switching
+// keys, Multiplicator, Galois keys. Every operation is a fixed sequence of
+// batched kernel launches on the caller's stream; all data stays on the device.
+#include "bfv.h"
+
+#include <algorithm>
+#include <cstring>
+
+namespace fhehip {
+
+// ---------------------------------------------------------------------------
+// Scaler constants blob (layout documented in rns_device.h).
+std::vector<u64> scaler_blob(const ScalerConsts& c, const std::vector<ModInfo>& to_mods, u32 start) {
+    const u32 nf = (u32)c.theta_garner_lo.size();
+    const u32 nt = (u32)c.gamma.size();
+    const u32 count = nt - start;
+    std::vector<u64> b(sc_out_base(nf) + (size_t)count * sc_out_stride(nf), 0);
+    b[SC_NF] = nf;
+    b[SC_NT] = nt;
+    b[SC_START] = start;
+    b[SC_COUNT] = count;
+    b[SC_SHIFT] = c.theta_garner_shift;
+    b[SC_IS_ONE] = c.is_one ? 1 : 0;
+    b[SC_TG_LO] = c.theta_gamma_lo;
+    b[SC_TG_HI] = c.theta_gamma_hi;
+    b[SC_TG_SIGN] = c.theta_gamma_sign ? 1 : 0;
+    u64* tg = b.data() + SC_HEADER;
+    u64* tom = tg + 2 * nf;
+    for (u32 j = 0; j < nf; j++) {
+        tg[2 * j] = c.theta_garner_lo[j];
+        tg[2 * j + 1] = c.theta_garner_hi[j];
+        tom[3 * j] = c.theta_omega_lo[j];
+        tom[3 * j + 1] = c.theta_omega_hi[j];
+        tom[3 * j + 2] = c.theta_omega_sign[j];
+    }
+    for (u32 i = 0; i < count; i++) {
+        u64* o = b.data() + sc_out_base(nf) + (size_t)i * sc_out_stride(nf);
+        const u32 ti = start + i;
+        o[0] = to_mods[ti].p;
+        o[1] = to_mods[ti].barrett_lo;
+        o[2] = to_mods[ti].barrett_hi;
+        o[3] = c.gamma[ti];
+        o[4] = c.gamma_shoup[ti];
+        for (u32 j = 0; j < nf; j++) {
+            o[5 + j] = c.omega[(size_t)ti * nf + j];
+            o[5 + nf + j] = c.omega_shoup[(size_t)ti * nf + j];
+        }
+    }
+    return b;
+}
+
+// ---------------------------------------------------------------------------
+PolyScaler::PolyScaler(std::shared_ptr<DeviceContext> f, std::shared_ptr<DeviceContext> t, const BigUint& num,
+                       const BigUint& den)
+    : from(std::move(f)), to(std::move(t)) {
+    if (from->N != to->N) throw Error(-4, "Incompatible degrees");
+    if (den.is_zero()) throw Error(-1, "zero denominator");
+    is_one = (num == den);
+    common = 0;
+    if (is_one) {
+        while (common < from->L && common < to->L && from->moduli[common] == to->moduli[common]) ++common;
+    }
+    consts.build(from->rns, to->rns, num, den);
+    std::vector<u64> blob = scaler_blob(consts, to->h_mod, common);
+    d_consts.reset(blob.size() * sizeof(u64));
+    HIPCHK(hipMemcpy(d_consts.ptr, blob.data(), d_consts.bytes, hipMemcpyHostToDevice));
+}
+
+// rq::Scaler::scale (math/src/rq/scaler.rs:52-126) on `polys` polynomials.
+void PolyScaler::scale(const u64* in, u64* out, u32 polys, bool ntt_repr, hipStream_t s) {
+    const size_t n = from->N;
+    const size_t isz = (size_t)from->L * n, osz = (size_t)to->L * n;
+    if (!polys) return;
+    if (common > 0)
+        HIPCHK(hipMemcpy2DAsync(out, osz * 8, in, isz * 8, (size_t)common * n * 8, polys, hipMemcpyDeviceToDevice, s));
+    if (common >= to->L) return;
+    const u64* src = in;
+    if (ntt_repr) {
+        work.ensure((size_t)polys * isz * 8);
+        InttRowsArgs a{};
+        a.src[0] = in;
+        a.src_inst_stride = isz;
+        a.dst = work.as<u64>();
+        a.dst_inst_stride = isz;
+        a.dst_part_stride = 0;
+        a.inv_tw = from->inv_tw.as<ulonglong2>();
+        a.mods = from->mod.as<ModInfo>();
+        launch_intt_rows(from->logn, a, from->L, 1, polys, s);
+        src = work.as<u64>();
+    }
+    launch_rns_scale(src, isz, 0, out + (size_t)common * n, osz, 0, d_consts.as<u64>(), (u32)n, 1, polys, s);
+    if (ntt_repr) {
+        RowMap m{out, to->L - common, to->L, common, common};
+        launch_ntt_forward(*to, m, polys * (to->L - common), false, s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+Ksk::Ksk(std::shared_ptr<DeviceContext> ct, std::shared_ptr<DeviceContext> key, const u64* c0, const u64* c1,
+         u32 ndigits)
+    : ct_ctx(std::move(ct)), key_ctx(std::move(key)), digits(ndigits) {
+    if (ct_ctx->N != key_ctx->N) throw Error(-4, "Incompatible degrees");
+    if (key_ctx->L < ct_ctx->L) throw Error(-4, "key context below the ciphertext context");
+    for (u32 i = 0; i < ct_ctx->L; i++)
+        if (ct_ctx->moduli[i] != key_ctx->moduli[i])
+            throw Error(-4, "the ciphertext context is not a level of the key context");
+    if (digits != ct_ctx->L) throw Error(-1, "one key row per ciphertext modulus is required");
+    if (key_ctx->L == 1) throw Error(-2, "single-modulus key switching is not supported");
+    extra = key_ctx->L - ct_ctx->L;
+    if (extra > 2) throw Error(-2, "key context more than two levels above the ciphertext context");
+    const u32 Lk = key_ctx->L, n = key_ctx->N;
+    const size_t rows = (size_t)digits * Lk;
+    std::vector<ulonglong2> h0(rows * n), h1(rows * n);
+    for (size_t r = 0; r < rows; r++) {
+        const u64 p = key_ctx->moduli[r % Lk];
+        for (u32 j = 0; j < n; j++) {
+            const size_t x = r * n + j;
+            if (c0[x] >= p || c1[x] >= p) throw Error(-1, "key coefficient not reduced");
+            h0[x] = make_ulonglong2(c0[x], shoup64(c0[x], p));
+            h1[x] = make_ulonglong2(c1[x], shoup64(c1[x], p));
+        }
+    }
+    key0.reset(h0.size() * sizeof(ulonglong2));
+    key1.reset(h1.size() * sizeof(ulonglong2));
+    HIPCHK(hipMemcpy(key0.ptr, h0.data(), key0.bytes, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(key1.ptr, h1.data(), key1.bytes, hipMemcpyHostToDevice));
+    // switch_down constants for dropping limbs L'-1 ... L (rq/mod.rs:390-449)
+    const u32 L = ct_ctx->L;
+    std::vector<u64> em;
+    std::vector<SdConst> sd;
+    sd_stride = Lk;
+    sd.assign((size_t)std::max<u32>(extra, 1) * sd_stride, SdConst{});
+    for (u32 s = 0; s < extra; s++) {
+        const u64 ql = key_ctx->moduli[L + s];
+        em.push_back(ql);
+        for (u32 m = 0; m < L + s; m++) {
+            const u64 q = key_ctx->moduli[m];
+            SdConst c;
+            c.q = q;
+            c.bl = key_ctx->h_mod[m].barrett_lo;
+            c.bh = key_ctx->h_mod[m].barrett_hi;
+            c.half_mod = q - ((ql >> 1) % q);
+            c.inv = invmod64(ql % q, q);
+            c.inv_shoup = shoup64(c.inv, q);
+            sd[(size_t)s * sd_stride + m] = c;
+        }
+    }
+    if (extra) {
+        extra_mods.reset(em.size() * 8);
+        HIPCHK(hipMemcpy(extra_mods.ptr, em.data(), extra_mods.bytes, hipMemcpyHostToDevice));
+    }
+    sd_consts.reset(sd.size() * sizeof(SdConst));
+    HIPCHK(hipMemcpy(sd_consts.ptr, sd.data(), sd_consts.bytes, hipMemcpyHostToDevice));
+}
+
+void Ksk::key_switch_pb(const u64* src, size_t src_inst_stride, u64* acc, size_t acc_inst_stride, u32 inst,
+                        bool to_power_basis, hipStream_t s) const {
+    KeySwitchArgs a{};
+    a.src = src;
+    a.src_inst_stride = src_inst_stride;
+    a.digits = digits;
+    a.key_limbs = key_ctx->L;
+    a.key0 = key0.as<ulonglong2>();
+    a.key1 = key1.as<ulonglong2>();
+    a.key_fwd_tw = key_ctx->fwd_tw.as<ulonglong2>();
+    a.key_mods = key_ctx->mod.as<ModInfo>();
+    a.acc = acc;
+    a.acc_inst_stride = acc_inst_stride;
+    a.acc_part_stride = (size_t)key_ctx->L * key_ctx->N;
+    launch_keyswitch(key_ctx->logn, a, key_ctx->L, inst, s);
+    if (to_power_basis) {
+        // acc is [inst][2][L'][N] contiguous when acc_inst_stride == 2 L' N
+        if (acc_inst_stride != 2 * a.acc_part_stride) throw Error(-1, "accumulator layout");
+        RowMap m{acc, key_ctx->L, key_ctx->L, 0, 0};
+        launch_ntt_backward(*key_ctx, m, inst * 2 * key_ctx->L, s);
+    }
+}
+
+void Ksk::finish(const u64* ks, u64* out, u32 inst, const u64* add_pb, size_t add_inst, size_t add_part,
+                 const u64* add_ntt, size_t add_ntt_inst, size_t add_ntt_part, u32 add_ntt_parts, const u32* perm,
+                 hipStream_t s) const {
+    const size_t n = ct_ctx->N;
+    SdNttArgs a{};
+    a.ks = ks;
+    a.ks_part_stride = (size_t)key_ctx->L * n;
+    a.ks_inst_stride = 2 * a.ks_part_stride;
+    a.L = ct_ctx->L;
+    a.extra = extra;
+    a.extra_mods = extra ? extra_mods.as<u64>() : nullptr;
+    a.sd = sd_consts.as<SdConst>();
+    a.sd_stride = sd_stride;
+    a.add_pb = add_pb;
+    a.add_inst_stride = add_inst;
+    a.add_part_stride = add_part;
+    a.add_ntt = add_ntt;
+    a.add_ntt_inst_stride = add_ntt_inst;
+    a.add_ntt_part_stride = add_ntt_part;
+    a.add_ntt_parts = add_ntt_parts;
+    a.perm = perm;
+    a.out = out;
+    a.out_part_stride = (size_t)ct_ctx->L * n;
+    a.out_inst_stride = 2 * a.out_part_stride;
+    a.fwd_tw = ct_ctx->fwd_tw.as<ulonglong2>();
+    a.mods = ct_ctx->mod.as<ModInfo>();
+    launch_sd_add_ntt(ct_ctx->logn, a, ct_ctx->L, 2, inst, s);
+}
+
+// KeySwitchingKey::key_switch (key_switching_key.rs:214-241): power-basis p in
+// the ciphertext context -> NTT-domain (c0, c1) in the key context.
+void Ksk::key_switch(const u64* p, u64* out, u32 inst, hipStream_t s) const {
+    key_switch_pb(p, (size_t)ct_ctx->L * ct_ctx->N, out, 2 * (size_t)key_ctx->L * key_ctx->N, inst, false, s);
+}
+
+// RelinearizationKey::relinearizes (relinearization_key.rs:71-98) on
+// [inst][3][L][N] NTT ciphertexts -> [inst][2][L][N].
+void Ksk::relinearize(const u64* ct3, u64* out, u32 inst, hipStream_t s) {
+    const size_t n = ct_ctx->N, L = ct_ctx->L;
+    const size_t c2_bytes = (size_t)inst * L * n * 8;
+    const size_t r_bytes = (size_t)inst * 2 * key_ctx->L * n * 8;
+    work.ensure(c2_bytes + r_bytes);
+    u64* c2 = work.as<u64>();
+    u64* r = c2 + (size_t)inst * L * n;
+    InttRowsArgs a{};
+    a.src[0] = ct3 + 2 * L * n;
+    a.src_inst_stride = 3 * L * n;
+    a.dst = c2;
+    a.dst_inst_stride = L * n;
+    a.inv_tw = ct_ctx->inv_tw.as<ulonglong2>();
+    a.mods = ct_ctx->mod.as<ModInfo>();
+    launch_intt_rows(ct_ctx->logn, a, (u32)L, 1, inst, s);
+    key_switch_pb(c2, L * n, r, 2 * (size_t)key_ctx->L * n, inst, true, s);
+    // (c0, c1) + switch_down(key switch), added in the NTT domain after the transform.
+    finish(r, out, inst, nullptr, 0, 0, ct3, 3 * L * n, L * n, 2, nullptr, s);
+}
+
+// ---------------------------------------------------------------------------
+static std::vector<u64> default_extended_basis(const DeviceContext& base) {
+    // Multiplicator::default (fhe/src/bfv/ops/mul.rs:101-117)
+    size_t bits = 0;
+    for (u64 q : base.moduli) bits += 64 - __builtin_clzll(q);
+    const size_t n_moduli = (bits + 60 + 61) / 62;
+    std::vector<u64> ext = base.moduli;
+    u64 ub = (u64)1 << 62;
+    while (ext.size() != base.moduli.size() + n_moduli) {
+        ub = generate_prime64(62, 2 * (u64)base.N, ub);
+        if (!ub) throw Error(-2, "not enough 62-bit primes for the extended basis");
+        if (std::find(ext.begin(), ext.end(), ub) == ext.end()) ext.push_back(ub);
+    }
+    return ext;
+}
+
+Multiplicator::Multiplicator(std::shared_ptr<DeviceContext> b, const std::vector<u64>& extended_basis,
+                             const BigUint& lhs_num, const BigUint& lhs_den, const BigUint& rhs_num,
+                             const BigUint& rhs_den, const BigUint& post_num, const BigUint& post_den)
+    : base(std::move(b)) {
+    mul = make_context(extended_basis, base->N, nullptr);
+    ext_lhs.reset(new PolyScaler(base, mul, lhs_num, lhs_den));
+    ext_rhs.reset(new PolyScaler(base, mul, rhs_num, rhs_den));
+    down.reset(new PolyScaler(mul, base, post_num, post_den));
+}
+
+std::unique_ptr<Multiplicator> Multiplicator::make_default(std::shared_ptr<Ksk> rk, u64 plaintext) {
+    auto base = rk->ct_ctx;
+    std::vector<u64> ext = default_extended_basis(*base);
+    std::unique_ptr<Multiplicator> m(new Multiplicator(base, ext, BigUint(1), BigUint(1), BigUint(1), BigUint(1),
+                                                       BigUint(plaintext), base->rns.product));
+    m->rk = std::move(rk);
+    return m;
+}
+
+// Multiplicator::multiply (fhe/src/bfv/ops/mul.rs:158-227) on a batch.
+void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out, u32 inst, hipStream_t s) {
+    const size_t n = base->N, L = base->L, M = mul->L;
+    const u32 cl = ext_lhs->common, cr = ext_rhs->common;
+    const size_t xr = M - std::min(cl, cr);  // extension rows per input polynomial
+    const size_t ct_inst = 2 * L * n;
+    const size_t P = (size_t)inst * 4 * L * n;
+    const size_t X = (size_t)inst * 4 * xr * n;
+    const size_t T = (size_t)inst * 3 * M * n;
+    const size_t D = (size_t)inst * 3 * L * n;
+    const size_t R = rk ? (size_t)inst * 2 * rk->key_ctx->L * n : 0;
+    work.ensure((P + X + T + D + R) * 8);
+    u64* wP = work.as<u64>();
+    u64* wX = wP + P;
+    u64* wT = wX + X;
+    u64* wD = wT + T;
+    u64* wR = wD + D;
+    // 1. inverse NTT of the four input polynomials (rq/scaler.rs:68-78)
+    InttRowsArgs ia{};
+    ia.src[0] = lhs;
+    ia.src[1] = lhs + L * n;
+    ia.src[2] = rhs;
+    ia.src[3] = rhs + L * n;
+    ia.src_inst_stride = ct_inst;
+    ia.dst = wP;
+    ia.dst_inst_stride = 4 * L * n;
+    ia.dst_part_stride = L * n;
+    ia.inv_tw = base->inv_tw.as<ulonglong2>();
+    ia.mods = base->mod.as<ModInfo>();
+    launch_intt_rows(base->logn, ia, (u32)L, 4, inst, s);
+    // 2. base extension of the power-basis columns (rns/scaler.rs:249-352)
+    if (cl < M)
+        launch_rns_scale(wP, 4 * L * n, L * n, wX, 4 * xr * n, xr * n, ext_lhs->d_consts.as<u64>(), (u32)n, 2, inst, s);
+    if (cr < M)
+        launch_rns_scale(wP + 2 * L * n, 4 * L * n, L * n, wX + 2 * xr * n, 4 * xr * n, xr * n,
+                         ext_rhs->d_consts.as<u64>(), (u32)n, 2, inst, s);
+    // 3. forward NTT of the new limbs (rq/scaler.rs:96-114)
+    if (cl == cr) {
+        RowMap m{wX, (u32)(M - cl), (u32)xr, 0, cl};
+        launch_ntt_forward(*mul, m, inst * 4 * (u32)(M - cl), false, s);
+    } else {
+        for (int k = 0; k < 4; k++) {
+            const u32 c = k < 2 ? cl : cr;
+            RowMap m{wX + k * xr * n, (u32)(M - c), (u32)(4 * xr), 0, c};
+            launch_ntt_forward(*mul, m, inst * (u32)(M - c), false, s);
+        }
+    }
+    // 4. tensor product + inverse NTT (mul.rs:181-185, rq/scaler.rs:68-78)
+    TensorArgs ta{};
+    ta.lhs = lhs;
+    ta.rhs = rhs;
+    ta.in_inst_stride = ct_inst;
+    ta.in_part_stride = L * n;
+    ta.ext = wX;
+    ta.ext_inst_stride = 4 * xr * n;
+    ta.ext_part_stride = xr * n;
+    ta.common_lhs = cl;
+    ta.common_rhs = cr;
+    ta.out = wT;
+    ta.out_inst_stride = 3 * M * n;
+    ta.out_part_stride = M * n;
+    ta.inv_tw = mul->inv_tw.as<ulonglong2>();
+    ta.mods = mul->mod.as<ModInfo>();
+    launch_tensor_intt(mul->logn, ta, (u32)M, inst, s);
+    // 5. t/Q down-scaling of the three polynomials (mul.rs:187-190)
+    u64* dst = rk ? wD : out;
+    launch_rns_scale(wT, 3 * M * n, M * n, dst, 3 * L * n, L * n, down->d_consts.as<u64>(), (u32)n, 3, inst, s);
+    if (!rk) {
+        RowMap m{out, (u32)L, (u32)L, 0, 0};
+        launch_ntt_forward(*base, m, inst * 3 * (u32)L, false, s);
+        return;
+    }
+    // 6. key switch of c2 (power basis already) (relinearization_key.rs:101-106)
+    rk->key_switch_pb(wD + 2 * L * n, 3 * L * n, wR, 2 * (size_t)rk->key_ctx->L * n, inst, true, s);
+    // 7. switch down to the ciphertext context, add c0/c1, forward NTT (mul.rs:194-210)
+    rk->finish(wR, out, inst, wD, 3 * L * n, L * n, nullptr, 0, 0, 0, nullptr, s);
+}
+
+// ---------------------------------------------------------------------------
+Galois::Galois(std::shared_ptr<Ksk> k, u32 exp) : ksk(std::move(k)) {
+    const u32 n = ksk->ct_ctx->N;
+    exponent = exp % (2 * n);
+    if ((exponent & 1) == 0) throw Error(-1, "The exponent should be odd modulo 2 * degree");
+    // SubstitutionExponent::new (math/src/rq/mod.rs:97-120) and the NTT-domain
+    // gather q[bitrev[j]] = p[power_bitrev[j]] (rq/mod.rs:324-334) as q[m] = p[perm[m]].
+    const int logn = ksk->ct_ctx->logn;
+    std::vector<u32> perm(n);
+    u64 power = (exponent - 1) / 2;
+    for (u32 j = 0; j < n; j++) {
+        u32 pb = bit_reverse((u32)(power & (n - 1)), logn);
+        perm[bit_reverse(j, logn)] = pb;
+        power += exponent;
+    }
+    d_perm.reset(n * 4);
+    HIPCHK(hipMemcpy(d_perm.ptr, perm.data(), d_perm.bytes, hipMemcpyHostToDevice));
+}
+
+// GaloisKey::relinearize_into (fhe/src/bfv/keys/galois_key.rs:90-124).
+void Galois::apply(const u64* ct, u64* out, u32 inst, hipStream_t s) {
+    const size_t n = ksk->ct_ctx->N, L = ksk->ct_ctx->L;
+    const size_t C = (size_t)inst * L * n;
+    const size_t R = (size_t)inst * 2 * ksk->key_ctx->L * n;
+    work.ensure((C + R) * 8);
+    u64* c2 = work.as<u64>();
+    u64* r = c2 + C;
+    InttRowsArgs a{};
+    a.src[0] = ct + L * n;
+    a.src_inst_stride = 2 * L * n;
+    a.dst = c2;
+    a.dst_inst_stride = L * n;
+    a.inv_tw = ksk->ct_ctx->inv_tw.as<ulonglong2>();
+    a.mods = ksk->ct_ctx->mod.as<ModInfo>();
+    a.perm = d_perm.as<u32>();
+    launch_intt_rows(ksk->ct_ctx->logn, a, (u32)L, 1, inst, s);
+    ksk->key_switch_pb(c2, L * n, r, 2 * (size_t)ksk->key_ctx->L * n, inst, true, s);
+    ksk->finish(r, out, inst, nullptr, 0, 0, ct, 2 * L * n, 0, 1, d_perm.as<u32>(), s);
+}
+
+// ---------------------------------------------------------------------------
+void Workspace::ensure(size_t bytes) {
+    if (bytes <= buf.bytes) return;
+    buf.reset(bytes);
+}
+
+}  // namespace fhehip

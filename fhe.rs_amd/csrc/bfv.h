@@ -1,0 +1,77 @@
+// Host-side objects of the BFV evaluation path.
+#pragma once
+
+#include "bfv_launch.h"
+#include "rns_device.h"
+
+namespace fhehip {
+
+// Device scratch owned by a handle; grows to the largest batch seen.
+struct Workspace {
+    DevBuf buf;
+    void ensure(size_t bytes);
+    template <class T>
+    T* as() const { return buf.as<T>(); }
+};
+
+std::vector<u64> scaler_blob(const ScalerConsts& c, const std::vector<ModInfo>& to_mods, u32 start);
+
+// rq::scaler::Scaler (math/src/rq/scaler.rs:18-126).
+struct PolyScaler {
+    std::shared_ptr<DeviceContext> from, to;
+    bool is_one = true;
+    u32 common = 0;  // number_common_moduli
+    ScalerConsts consts;
+    DevBuf d_consts;  // outputs start .. to->L
+    Workspace work;
+    PolyScaler(std::shared_ptr<DeviceContext> f, std::shared_ptr<DeviceContext> t, const BigUint& num,
+               const BigUint& den);
+    void scale(const u64* in, u64* out, u32 polys, bool ntt_repr, hipStream_t s);
+};
+
+// KeySwitchingKey (fhe/src/bfv/keys/key_switching_key.rs:22-45) with resident
+// NttShoup rows and the switch-down constants from the key context back to the
+// ciphertext context.
+struct Ksk {
+    std::shared_ptr<DeviceContext> ct_ctx, key_ctx;
+    u32 digits = 0, extra = 0, sd_stride = 0;
+    DevBuf key0, key1;  // [digits][L'][N] (value, shoup)
+    DevBuf extra_mods, sd_consts;
+    Workspace work;
+    Ksk(std::shared_ptr<DeviceContext> ct, std::shared_ptr<DeviceContext> key, const u64* c0, const u64* c1,
+        u32 digits);
+    // Digit lift + MAC (+ inverse NTT when to_power_basis) into acc [inst][2][L'][N].
+    void key_switch_pb(const u64* src, size_t src_inst_stride, u64* acc, size_t acc_inst_stride, u32 inst,
+                       bool to_power_basis, hipStream_t s) const;
+    // switch_down_to the ciphertext context + addends + forward NTT into out [inst][2][L][N].
+    void finish(const u64* ks, u64* out, u32 inst, const u64* add_pb, size_t add_inst, size_t add_part,
+                const u64* add_ntt, size_t add_ntt_inst, size_t add_ntt_part, u32 add_ntt_parts, const u32* perm,
+                hipStream_t s) const;
+    void key_switch(const u64* p, u64* out, u32 inst, hipStream_t s) const;
+    void relinearize(const u64* ct3, u64* out, u32 inst, hipStream_t s);
+};
+
+// Multiplicator (fhe/src/bfv/ops/mul.rs:22-227).
+struct Multiplicator {
+    std::shared_ptr<DeviceContext> base, mul;
+    std::unique_ptr<PolyScaler> ext_lhs, ext_rhs, down;
+    std::shared_ptr<Ksk> rk;
+    Workspace work;
+    Multiplicator(std::shared_ptr<DeviceContext> base, const std::vector<u64>& extended_basis, const BigUint& lhs_num,
+                  const BigUint& lhs_den, const BigUint& rhs_num, const BigUint& rhs_den, const BigUint& post_num,
+                  const BigUint& post_den);
+    static std::unique_ptr<Multiplicator> make_default(std::shared_ptr<Ksk> rk, u64 plaintext);
+    void multiply(const u64* lhs, const u64* rhs, u64* out, u32 inst, hipStream_t s);
+};
+
+// GaloisKey (fhe/src/bfv/keys/galois_key.rs:15-124).
+struct Galois {
+    std::shared_ptr<Ksk> ksk;
+    u32 exponent = 1;
+    DevBuf d_perm;
+    Workspace work;
+    Galois(std::shared_ptr<Ksk> k, u32 exponent);
+    void apply(const u64* ct, u64* out, u32 inst, hipStream_t s);
+};
+
+}  // namespace fhehip

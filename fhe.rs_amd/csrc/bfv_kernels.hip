@@ -3,32 +3,35 @@
 // Shoup multiply-accumulate), and fused switch-down + add + NTT.
 #include "bfv_launch.h"
 #include "ntt_device.h"
+#include "profiler.h"
 #include "rns_device.h"
 
 namespace fhehip {
 
 // ---------------------------------------------------------------------------
 // RnsScaler::scale over the columns of a batch (rq::Scaler::scale conversion
-// loop, math/src/rq/scaler.rs:84-93). Power-basis input [poly][nf][N] (rows of
-// poly at in + poly * in_poly_stride), output rows start at out + poly *
-// out_poly_stride. grid = (N / 256, polys)
+// loop, math/src/rq/scaler.rs:84-93). Power-basis rows of part k of instance b
+// start at in + b * in_inst + k * in_part; outputs likewise.
+// grid = (N / 256, parts, instances)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void rns_scale_kernel(const u64* __restrict__ in, size_t in_poly_stride,
-                                                        u64* __restrict__ out, size_t out_poly_stride,
+__global__ __launch_bounds__(256) void rns_scale_kernel(const u64* __restrict__ in, size_t in_inst, size_t in_part,
+                                                        u64* __restrict__ out, size_t out_inst, size_t out_part,
                                                         const u64* __restrict__ sc, u32 n) {
     const u32 col = blockIdx.x * 256 + threadIdx.x;
     if (col >= n) return;
-    const u64* src = in + blockIdx.y * in_poly_stride + col;
-    u64* dst = out + blockIdx.y * out_poly_stride + col;
+    const u64* src = in + blockIdx.z * in_inst + blockIdx.y * in_part + col;
+    u64* dst = out + blockIdx.z * out_inst + blockIdx.y * out_part + col;
     rns_scale_column(
         sc, [&](int j) { return src[(size_t)j * n]; }, [&](int i, u64 v) { dst[(size_t)i * n] = v; });
 }
 
-void launch_rns_scale(const u64* in, size_t in_poly_stride, u64* out, size_t out_poly_stride, const u64* d_sc,
-                      u32 n, u32 polys, hipStream_t s) {
-    if (!polys) return;
-    dim3 grid((n + 255) / 256, polys);
-    hipLaunchKernelGGL(rns_scale_kernel, grid, dim3(256), 0, s, in, in_poly_stride, out, out_poly_stride, d_sc, n);
+void launch_rns_scale(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst, size_t out_part,
+                      const u64* d_sc, u32 n, u32 parts, u32 inst, hipStream_t s) {
+    if (!parts || !inst) return;
+    ProfScope ps("rns_scale", 31 - __builtin_clz(n), s);
+    dim3 grid((n + 255) / 256, parts, inst);
+    hipLaunchKernelGGL(rns_scale_kernel, grid, dim3(256), 0, s, in, in_inst, in_part, out, out_inst, out_part, d_sc,
+                       n);
     HIPCHK(hipGetLastError());
 }
 
@@ -127,10 +130,9 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void tensor_intt_kernel(TensorAr
 // one key limb j of one instance: for every digit i, the power-basis row i of
 // the input is reduced modulo q'_j (lazy_reduce_vec), forward transformed
 // lazily (forward_vt_lazy; together the digit lift of rq/mod.rs:518-540) and
-// multiplied by the key rows (Shoup, exact) into two accumulators. The
-// accumulators are then inverse transformed, so `acc` receives the power-basis
-// (c0, c1) of the key switch: rows acc + inst * acc_inst_stride + {0, 1} *
-// acc_part_stride + j * N. grid = (L', instances).
+// multiplied by the key rows (Shoup, exact) into two accumulators, so `acc`
+// receives the NTT-domain (c0, c1) of the key switch: rows acc + inst *
+// acc_inst_stride + {0, 1} * acc_part_stride + j * N. grid = (L', instances).
 // ---------------------------------------------------------------------------
 template <int LOGN>
 __global__ __launch_bounds__(NttShape<LOGN>::T) void keyswitch_kernel(KeySwitchArgs a) {
@@ -143,53 +145,40 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void keyswitch_kernel(KeySwitchA
     const ulonglong2* ftw = a.key_fwd_tw + ((size_t)j << LOGN);
     u64* acc0 = a.acc + (size_t)b * a.acc_inst_stride + ((size_t)j << LOGN);
     u64* acc1 = acc0 + a.acc_part_stride;
-    const size_t off = (size_t)t << S::RE;
     for (u32 i = 0; i < a.digits; i++) {
         const u64* src = a.src + (size_t)b * a.src_inst_stride + ((size_t)i << LOGN);
         u64 x[S::E];
 #pragma unroll
         for (int e = 0; e < S::E; e++)
             x[e] = lazy_reduce_u64(src[fwd_index<LOGN, 0>(t, e)], mi.barrett_lo, mi.barrett_hi, p);
-        ntt_fwd_regs<LOGN>(lds, x, t, ftw, p);
-        const ulonglong2* k0 = a.key0 + (((size_t)i * a.key_limbs + j) << LOGN) + off;
-        const ulonglong2* k1 = a.key1 + (((size_t)i * a.key_limbs + j) << LOGN) + off;
+        // Launder the twiddle pointer so that the per-thread twiddles of the
+        // later passes are reloaded per digit instead of being hoisted out of
+        // the loop (which would keep ~170 extra registers live).
+        u64 twp = (u64)ftw;
+        asm volatile("" : "+s"(twp));
+        ntt_fwd_regs<LOGN>(lds, x, t, (const ulonglong2*)twp, p);
+        // Stage the lazily transformed digit through LDS so that the key rows
+        // and the accumulators are walked with coalesced accesses.
 #pragma unroll
-        for (int e = 0; e < S::E; e += 2) {
-            ulonglong2 w0a = k0[e], w0b = k0[e + 1], w1a = k1[e], w1b = k1[e + 1];
-            u64 v0a = mul_shoup(x[e], w0a.x, w0a.y, p), v0b = mul_shoup(x[e + 1], w0b.x, w0b.y, p);
-            u64 v1a = mul_shoup(x[e], w1a.x, w1a.y, p), v1b = mul_shoup(x[e + 1], w1b.x, w1b.y, p);
-            ulonglong2* d0 = reinterpret_cast<ulonglong2*>(acc0 + off + e);
-            ulonglong2* d1 = reinterpret_cast<ulonglong2*>(acc1 + off + e);
-            if (i) {
-                ulonglong2 o0 = *d0, o1 = *d1;
-                v0a = add_mod(v0a, o0.x, p);
-                v0b = add_mod(v0b, o0.y, p);
-                v1a = add_mod(v1a, o1.x, p);
-                v1b = add_mod(v1b, o1.y, p);
-            }
-            *d0 = make_ulonglong2(v0a, v0b);
-            *d1 = make_ulonglong2(v1a, v1b);
-        }
-    }
-    if (!a.to_power_basis) return;
-    const ulonglong2* itw = a.key_inv_tw + ((size_t)j << LOGN);
-    for (int part = 0; part < 2; part++) {
-        u64* row = part ? acc1 : acc0;
-        u64 x[S::E];
-        const ulonglong2* in = reinterpret_cast<const ulonglong2*>(row + off);
-#pragma unroll
-        for (int e = 0; e < S::E / 2; e++) {
-            ulonglong2 v = in[e];
-            x[2 * e] = v.x;
-            x[2 * e + 1] = v.y;
-        }
-        ntt_inv_regs<LOGN>(lds, x, t, itw, p);
-        // Every thread has read its own slice of `row` above; wait for all of
-        // them before the transposed write-back.
+        for (int e = 0; e < S::E; e++) lds[lds_slot((t << S::RE) + e)] = x[e];
         __syncthreads();
-#pragma unroll
-        for (int e = 0; e < S::E; e++)
-            row[inv_index<LOGN, last_start<LOGN>()>(t, e)] = mul_shoup(x[e], mi.n_inv, mi.n_inv_shoup, p);
+        const ulonglong2* k0 = a.key0 + (((size_t)i * a.key_limbs + j) << LOGN);
+        const ulonglong2* k1 = a.key1 + (((size_t)i * a.key_limbs + j) << LOGN);
+#pragma unroll 4
+        for (int r = 0; r < S::E; r++) {
+            const int jj = r * S::T + t;
+            const u64 v = lds[lds_slot(jj)];
+            const ulonglong2 w0 = k0[jj], w1 = k1[jj];
+            u64 v0 = mul_shoup(v, w0.x, w0.y, p);
+            u64 v1 = mul_shoup(v, w1.x, w1.y, p);
+            if (i) {
+                v0 = add_mod(v0, acc0[jj], p);
+                v1 = add_mod(v1, acc1[jj], p);
+            }
+            acc0[jj] = v0;
+            acc1[jj] = v1;
+        }
+        __syncthreads();
     }
 }
 
@@ -257,12 +246,13 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void sd_add_ntt_kernel(SdNttArgs
     ntt_fwd_regs<LOGN>(lds, x, t, a.fwd_tw + ((size_t)i << LOGN), p);
     const u64 p2 = 2 * p;
     const size_t off = (size_t)t << S::RE;
-    if (a.add_ntt && part == 0) {
-        const u64* g = a.add_ntt + (size_t)b * a.add_ntt_inst_stride + ((size_t)i << LOGN);
+    if (a.add_ntt && part < a.add_ntt_parts) {
+        const u64* g = a.add_ntt + (size_t)b * a.add_ntt_inst_stride + (size_t)part * a.add_ntt_part_stride +
+                       ((size_t)i << LOGN);
 #pragma unroll
         for (int e = 0; e < S::E; e++) {
             u64 v = reduce1(reduce1(x[e], p2), p);
-            x[e] = add_mod(v, g[a.perm[off + e]], p);
+            x[e] = add_mod(v, g[a.perm ? a.perm[off + e] : (u32)(off + e)], p);
         }
     } else {
 #pragma unroll
@@ -295,24 +285,28 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void sd_add_ntt_kernel(SdNttArgs
 
 void launch_intt_rows(int logn, const InttRowsArgs& a, u32 limbs, u32 parts, u32 inst, hipStream_t s) {
     if (!limbs || !parts || !inst) return;
+    ProfScope ps(a.perm ? "intt_rows_gather" : "intt_rows", logn, s);
     dim3 grid(limbs, parts, inst);
     FHEHIP_LOGN_DISPATCH(logn, intt_rows_kernel, grid, 0, s, a);
 }
 
 void launch_tensor_intt(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_t s) {
     if (!M || !inst) return;
+    ProfScope ps("tensor_intt", logn, s);
     dim3 grid(M, 3, inst);
     FHEHIP_LOGN_DISPATCH(logn, tensor_intt_kernel, grid, 0, s, a);
 }
 
 void launch_keyswitch(int logn, const KeySwitchArgs& a, u32 key_limbs, u32 inst, hipStream_t s) {
     if (!key_limbs || !inst) return;
+    ProfScope ps("keyswitch", logn, s);
     dim3 grid(key_limbs, inst);
     FHEHIP_LOGN_DISPATCH(logn, keyswitch_kernel, grid, 0, s, a);
 }
 
 void launch_sd_add_ntt(int logn, const SdNttArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s) {
     if (!L || !parts || !inst) return;
+    ProfScope ps("sd_add_ntt", logn, s);
     dim3 grid(L, parts, inst);
     FHEHIP_LOGN_DISPATCH(logn, sd_add_ntt_kernel, grid, 0, s, a);
 }

@@ -37,11 +37,9 @@ struct KeySwitchArgs {
     const ulonglong2* key0;  // [digits][key_limbs][N] (value, shoup)
     const ulonglong2* key1;
     const ulonglong2* key_fwd_tw;
-    const ulonglong2* key_inv_tw;
     const ModInfo* key_mods;
     u64* acc;  // [inst][2][key_limbs][N]
     size_t acc_inst_stride, acc_part_stride;
-    u32 to_power_basis;
 };
 
 // Constants of one switch-down step applied to one remaining limb.
@@ -60,17 +58,18 @@ struct SdNttArgs {
     u32 sd_stride;
     const u64* add_pb;  // optional power-basis addend [inst][parts][L][N]
     size_t add_inst_stride, add_part_stride;
-    const u64* add_ntt;  // optional NTT-domain addend (part 0 only), gathered with perm
-    size_t add_ntt_inst_stride;
-    const u32* perm;
+    const u64* add_ntt;  // optional NTT-domain addend for parts < add_ntt_parts
+    size_t add_ntt_inst_stride, add_ntt_part_stride;
+    u32 add_ntt_parts;
+    const u32* perm;     // optional gather of the NTT addend: value m comes from perm[m]
     u64* out;  // [inst][parts][L][N] NTT, reduced
     size_t out_inst_stride, out_part_stride;
     const ulonglong2* fwd_tw;  // ciphertext context
     const ModInfo* mods;
 };
 
-void launch_rns_scale(const u64* in, size_t in_poly_stride, u64* out, size_t out_poly_stride, const u64* d_sc,
-                      u32 n, u32 polys, hipStream_t s);
+void launch_rns_scale(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst, size_t out_part,
+                      const u64* d_sc, u32 n, u32 parts, u32 inst, hipStream_t s);
 void launch_intt_rows(int logn, const InttRowsArgs& a, u32 limbs, u32 parts, u32 inst, hipStream_t s);
 void launch_tensor_intt(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_t s);
 void launch_keyswitch(int logn, const KeySwitchArgs& a, u32 key_limbs, u32 inst, hipStream_t s);

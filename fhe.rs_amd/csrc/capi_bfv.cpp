@@ -1,0 +1,158 @@
+// extern "C" boundary: scaler, key switching, Multiplicator, Galois keys.
+#include "../../include/fhehip.h"
+
+#include <cstring>
+
+#include "bfv.h"
+#include "capi_internal.h"
+
+using namespace fhehip;
+
+struct fhehip_scaler {
+    std::unique_ptr<PolyScaler> s;
+};
+struct fhehip_ksk {
+    std::shared_ptr<Ksk> k;
+};
+struct fhehip_multiplicator {
+    std::unique_ptr<Multiplicator> m;
+};
+struct fhehip_galois {
+    std::unique_ptr<Galois> g;
+};
+
+static BigUint factor_num(const fhehip_scaling_factor* f) {
+    if (!f || !f->num) return BigUint(1);
+    return BigUint::from_words32(f->num, f->num_len);
+}
+static BigUint factor_den(const fhehip_scaling_factor* f) {
+    if (!f || !f->den) return BigUint(1);
+    return BigUint::from_words32(f->den, f->den_len);
+}
+
+static hipStream_t S(void* s) { return (hipStream_t)s; }
+
+extern "C" {
+
+int fhehip_scaler_create(const fhehip_context* from, const fhehip_context* to, const fhehip_scaling_factor* factor,
+                         fhehip_scaler** out) {
+    FHEHIP_API_BEGIN
+    if (!from || !to || !out) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    BigUint den = factor_den(factor);
+    if (den.is_zero()) return fail(FHEHIP_ERR_INVALID_ARG, "zero denominator");
+    std::unique_ptr<PolyScaler> s(new PolyScaler(from->c, to->c, factor_num(factor), den));
+    *out = new fhehip_scaler{std::move(s)};
+    FHEHIP_API_END
+}
+
+void fhehip_scaler_destroy(fhehip_scaler* s) { delete s; }
+
+int fhehip_scaler_scale(fhehip_scaler* s, const uint64_t* d_in, uint64_t* d_out, uint32_t num_polys, int ntt_repr,
+                        void* stream) {
+    FHEHIP_API_BEGIN
+    if (!s || ((!d_in || !d_out) && num_polys)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    s->s->scale(d_in, d_out, num_polys, ntt_repr != 0, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_ksk_create(const fhehip_context* ct_ctx, const fhehip_context* key_ctx, const uint64_t* c0,
+                      const uint64_t* c1, uint32_t digits, fhehip_ksk** out) {
+    FHEHIP_API_BEGIN
+    if (!ct_ctx || !key_ctx || !c0 || !c1 || !out) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    auto k = std::make_shared<Ksk>(ct_ctx->c, key_ctx->c, c0, c1, digits);
+    *out = new fhehip_ksk{k};
+    FHEHIP_API_END
+}
+
+void fhehip_ksk_destroy(fhehip_ksk* k) { delete k; }
+
+int fhehip_key_switch(const fhehip_ksk* k, const uint64_t* d_p, uint64_t* d_out, uint32_t num, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!k || ((!d_p || !d_out) && num)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    k->k->key_switch(d_p, d_out, num, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_relinearize(fhehip_ksk* rk, const uint64_t* d_ct3, uint64_t* d_out, uint32_t num, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!rk || ((!d_ct3 || !d_out) && num)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    rk->k->relinearize(d_ct3, d_out, num, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_multiplicator_default(const fhehip_ksk* rk, uint64_t plaintext_modulus, fhehip_multiplicator** out) {
+    FHEHIP_API_BEGIN
+    if (!rk || !out || plaintext_modulus < 2) return fail(FHEHIP_ERR_INVALID_ARG, "bad argument");
+    *out = nullptr;
+    auto m = Multiplicator::make_default(rk->k, plaintext_modulus);
+    *out = new fhehip_multiplicator{std::move(m)};
+    FHEHIP_API_END
+}
+
+int fhehip_multiplicator_create(const fhehip_context* base_ctx, const uint64_t* extended_basis, uint32_t ext_len,
+                                const fhehip_scaling_factor* lhs, const fhehip_scaling_factor* rhs,
+                                const fhehip_scaling_factor* post, fhehip_multiplicator** out) {
+    FHEHIP_API_BEGIN
+    if (!base_ctx || !extended_basis || !ext_len || !out) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    std::vector<u64> ext(extended_basis, extended_basis + ext_len);
+    std::unique_ptr<Multiplicator> m(new Multiplicator(base_ctx->c, ext, factor_num(lhs), factor_den(lhs),
+                                                       factor_num(rhs), factor_den(rhs), factor_num(post),
+                                                       factor_den(post)));
+    *out = new fhehip_multiplicator{std::move(m)};
+    FHEHIP_API_END
+}
+
+int fhehip_multiplicator_enable_relinearization(fhehip_multiplicator* m, const fhehip_ksk* rk) {
+    FHEHIP_API_BEGIN
+    if (!m || !rk) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    if (rk->k->ct_ctx->moduli != m->m->base->moduli || rk->k->ct_ctx->N != m->m->base->N)
+        return fail(FHEHIP_ERR_CTX_MISMATCH, "Invalid relinearization key context");
+    m->m->rk = rk->k;
+    FHEHIP_API_END
+}
+
+void fhehip_multiplicator_destroy(fhehip_multiplicator* m) { delete m; }
+
+int fhehip_multiplicator_extended_basis(const fhehip_multiplicator* m, uint64_t* out, uint32_t* len) {
+    FHEHIP_API_BEGIN
+    if (!m || !len) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    const auto& mod = m->m->mul->moduli;
+    if (out) {
+        if (*len < mod.size()) return fail(FHEHIP_ERR_INVALID_ARG, "output too small");
+        std::memcpy(out, mod.data(), mod.size() * 8);
+    }
+    *len = (uint32_t)mod.size();
+    FHEHIP_API_END
+}
+
+int fhehip_multiply(fhehip_multiplicator* m, const uint64_t* d_lhs, const uint64_t* d_rhs, uint64_t* d_out,
+                    uint32_t num, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!m || ((!d_lhs || !d_rhs || !d_out) && num)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    m->m->multiply(d_lhs, d_rhs, d_out, num, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_galois_create(const fhehip_ksk* ksk, uint32_t exponent, fhehip_galois** out) {
+    FHEHIP_API_BEGIN
+    if (!ksk || !out) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    std::unique_ptr<Galois> g(new Galois(ksk->k, exponent));
+    *out = new fhehip_galois{std::move(g)};
+    FHEHIP_API_END
+}
+
+void fhehip_galois_destroy(fhehip_galois* g) { delete g; }
+
+int fhehip_galois_relinearize(fhehip_galois* g, const uint64_t* d_ct, uint64_t* d_out, uint32_t num, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!g || ((!d_ct || !d_out) && num)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    if (d_ct == d_out && num) return fail(FHEHIP_ERR_INVALID_ARG, "in-place Galois relinearization is not supported");
+    g->g->apply(d_ct, d_out, num, S(stream));
+    FHEHIP_API_END
+}
+
+}  // extern "C"

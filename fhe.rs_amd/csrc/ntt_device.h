@@ -66,7 +66,9 @@ __device__ __forceinline__ void fwd_pass(u64 (&x)[NttShape<LOGN>::E], int t, con
                                          u64 p, u64 p2) {
     constexpr int RE = NttShape<LOGN>::RE;
     constexpr int LOB = LOGN - S0V - RE;
-    const int hi = t >> LOB;
+    // In the first pass every thread uses the same twiddles (hi == 0 because
+    // t < T = 2^LOB); spelling it out lets them live in scalar registers.
+    const int hi = (S0V == 0) ? 0 : (t >> LOB);
 #pragma unroll
     for (int u = 0; u < RE; u++) {
         const int s = S0V + u;

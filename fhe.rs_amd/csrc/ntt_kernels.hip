@@ -1,6 +1,7 @@
 // Batched negacyclic NTT kernels: one workgroup per polynomial row.
 #include "fhehip_internal.h"
 #include "ntt_device.h"
+#include "profiler.h"
 
 namespace fhehip {
 
@@ -100,12 +101,14 @@ static void inv_launch(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s
 
 void launch_ntt_forward(const DeviceContext& c, RowMap m, u32 rows, bool lazy, hipStream_t s) {
     if (rows == 0) return;
+    ProfScope ps(lazy ? "ntt_forward_lazy" : "ntt_forward", c.logn, s);
     FHEHIP_LOGN_SWITCH(c.logn, fwd_launch, c, m, rows, lazy, s);
     HIPCHK(hipGetLastError());
 }
 
 void launch_ntt_backward(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s) {
     if (rows == 0) return;
+    ProfScope ps("ntt_backward", c.logn, s);
     FHEHIP_LOGN_SWITCH(c.logn, inv_launch, c, m, rows, s);
     HIPCHK(hipGetLastError());
 }

@@ -34,7 +34,47 @@ SIGNATURES = {
     "fhehip_context_psi": (C.c_int, [_vp, _u64p]),
     "fhehip_ntt_forward": (C.c_int, [_vp, _vp, C.c_uint32, C.c_int, _vp]),
     "fhehip_ntt_backward": (C.c_int, [_vp, _vp, C.c_uint32, _vp]),
+    "fhehip_scaler_create": (C.c_int, [_vp, _vp, _vp, _vpp]),
+    "fhehip_scaler_destroy": (None, [_vp]),
+    "fhehip_scaler_scale": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_int, _vp]),
+    "fhehip_ksk_create": (C.c_int, [_vp, _vp, _u64p, _u64p, C.c_uint32, _vpp]),
+    "fhehip_ksk_destroy": (None, [_vp]),
+    "fhehip_key_switch": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_relinearize": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_multiplicator_default": (C.c_int, [_vp, C.c_uint64, _vpp]),
+    "fhehip_multiplicator_create": (C.c_int, [_vp, _u64p, C.c_uint32, _vp, _vp, _vp, _vpp]),
+    "fhehip_multiplicator_enable_relinearization": (C.c_int, [_vp, _vp]),
+    "fhehip_multiplicator_destroy": (None, [_vp]),
+    "fhehip_multiplicator_extended_basis": (C.c_int, [_vp, _u64p, _u32p]),
+    "fhehip_multiply": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_galois_create": (C.c_int, [_vp, C.c_uint32, _vpp]),
+    "fhehip_galois_destroy": (None, [_vp]),
+    "fhehip_galois_relinearize": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_profiler_enable": (C.c_int, [C.c_int]),
+    "fhehip_profiler_report": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
 }
+
+
+class ScalingFactorC(C.Structure):
+    _fields_ = [("num", _u32p), ("num_len", C.c_uint32), ("den", _u32p), ("den_len", C.c_uint32)]
+
+
+def _words32(x: int):
+    w = []
+    while x:
+        w.append(x & 0xFFFFFFFF)
+        x >>= 32
+    w = w or [0]
+    return (C.c_uint32 * len(w))(*w), len(w)
+
+
+def scaling_factor(num: int, den: int):
+    """Build a fhehip_scaling_factor; keeps the word arrays alive on the struct."""
+    n, nl = _words32(num)
+    d, dl = _words32(den)
+    f = ScalingFactorC(C.cast(n, _u32p), nl, C.cast(d, _u32p), dl)
+    f._keep = (n, d)
+    return f
 
 _lib = None
 
@@ -97,9 +137,7 @@ class Context:
         self.h = h
 
     def __del__(self):
-        if getattr(self, "h", None) and _lib is not None:
-            _lib.fhehip_context_destroy(self.h)
-            self.h = None
+        _destroy("fhehip_context_destroy", getattr(self, "h", None))
 
     @property
     def L(self):
@@ -133,3 +171,116 @@ def to_host(t):
 def empty(n, device="cuda"):
     import torch
     return torch.empty(int(n), dtype=torch.int64, device=device)
+
+
+def _destroy(name, h):
+    if h and _lib is not None:
+        try:
+            getattr(_lib, name)(h)
+        except Exception:  # interpreter teardown: ctypes may already be gone
+            pass
+
+
+class Scaler:
+    """rq::scaler::Scaler between two contexts by num/den (None = one)."""
+
+    def __init__(self, from_ctx, to_ctx, num=None, den=None):
+        self.from_ctx, self.to_ctx = from_ctx, to_ctx
+        h = C.c_void_p()
+        f = scaling_factor(num, den) if num is not None else None
+        check(lib().fhehip_scaler_create(from_ctx.h, to_ctx.h, C.byref(f) if f is not None else None, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        _destroy("fhehip_scaler_destroy", getattr(self, "h", None))
+
+    def scale(self, d_in, d_out, num_polys, ntt_repr, stream=None):
+        check(lib().fhehip_scaler_scale(self.h, dptr(d_in), dptr(d_out), num_polys, 1 if ntt_repr else 0,
+                                        stream_ptr(stream)))
+
+
+class KeySwitchingKey:
+    """KeySwitchingKey from host NTT-domain rows c0, c1 [digits][L_key][N]."""
+
+    def __init__(self, ct_ctx, key_ctx, c0, c1):
+        import numpy as np
+        c0 = np.ascontiguousarray(c0, dtype=np.uint64)
+        c1 = np.ascontiguousarray(c1, dtype=np.uint64)
+        digits = c0.shape[0]
+        h = C.c_void_p()
+        check(lib().fhehip_ksk_create(ct_ctx.h, key_ctx.h, c0.ctypes.data_as(_u64p), c1.ctypes.data_as(_u64p),
+                                      digits, C.byref(h)))
+        self.h = h
+        self.ct_ctx, self.key_ctx = ct_ctx, key_ctx
+
+    def __del__(self):
+        _destroy("fhehip_ksk_destroy", getattr(self, "h", None))
+
+    def key_switch(self, d_p, d_out, num, stream=None):
+        check(lib().fhehip_key_switch(self.h, dptr(d_p), dptr(d_out), num, stream_ptr(stream)))
+
+    def relinearize(self, d_ct3, d_out, num, stream=None):
+        check(lib().fhehip_relinearize(self.h, dptr(d_ct3), dptr(d_out), num, stream_ptr(stream)))
+
+
+class Multiplicator:
+    """Multiplicator::default(rk) when rk is given, else Multiplicator::new."""
+
+    def __init__(self, rk=None, plaintext=None, base_ctx=None, extended_basis=None, lhs=None, rhs=None,
+                 post=None):
+        h = C.c_void_p()
+        self.rk = rk
+        if base_ctx is None:
+            check(lib().fhehip_multiplicator_default(rk.h, plaintext, C.byref(h)))
+        else:
+            self._f = [scaling_factor(*f) if f is not None else None for f in (lhs, rhs, post)]
+            refs = [C.byref(f) if f is not None else None for f in self._f]
+            check(lib().fhehip_multiplicator_create(base_ctx.h, _arr64(extended_basis), len(extended_basis),
+                                                    refs[0], refs[1], refs[2], C.byref(h)))
+            if rk is not None:
+                check(lib().fhehip_multiplicator_enable_relinearization(h, rk.h))
+        self.h = h
+
+    def __del__(self):
+        _destroy("fhehip_multiplicator_destroy", getattr(self, "h", None))
+
+    def extended_basis(self):
+        n = C.c_uint32(64)
+        out = (C.c_uint64 * 64)()
+        check(lib().fhehip_multiplicator_extended_basis(self.h, out, C.byref(n)))
+        return [int(out[i]) for i in range(n.value)]
+
+    def multiply(self, d_lhs, d_rhs, d_out, num, stream=None):
+        check(lib().fhehip_multiply(self.h, dptr(d_lhs), dptr(d_rhs), dptr(d_out), num, stream_ptr(stream)))
+
+
+class GaloisKey:
+    def __init__(self, ksk, exponent):
+        h = C.c_void_p()
+        check(lib().fhehip_galois_create(ksk.h, exponent, C.byref(h)))
+        self.h = h
+        self.ksk = ksk
+        self.exponent = exponent
+
+    def __del__(self):
+        _destroy("fhehip_galois_destroy", getattr(self, "h", None))
+
+    def relinearize(self, d_ct, d_out, num, stream=None):
+        check(lib().fhehip_galois_relinearize(self.h, dptr(d_ct), dptr(d_out), num, stream_ptr(stream)))
+
+
+def profiler_enable(on=True):
+    check(lib().fhehip_profiler_enable(1 if on else 0))
+
+
+def profiler_report():
+    """{kernel: (launches, total_ms)} accumulated since profiler_enable()."""
+    need = C.c_size_t(0)
+    check(lib().fhehip_profiler_report(None, 0, C.byref(need)))
+    buf = C.create_string_buffer(need.value + 1)
+    check(lib().fhehip_profiler_report(buf, len(buf), C.byref(need)))
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, calls, ms = line.split()
+        out[name] = (int(calls), float(ms))
+    return out

@@ -71,6 +71,87 @@ int fhehip_ntt_forward(const fhehip_context* ctx, uint64_t* d_polys, uint32_t nu
  * NttOperator::backward, native.rs:106-132). Inputs must be < 2p. */
 int fhehip_ntt_backward(const fhehip_context* ctx, uint64_t* d_polys, uint32_t num_polys, void* stream);
 
+/* ---- Scaling (RNS base extension and t/Q rescaling) ----------------------
+ * A scaling factor numerator/denominator as little-endian 32-bit words
+ * (ScalingFactor, math/src/rns/scaler.rs:20-47). A NULL factor, or NULL
+ * words, stand for one. */
+typedef struct fhehip_scaling_factor {
+    const uint32_t* num;
+    uint32_t num_len;
+    const uint32_t* den;
+    uint32_t den_len;
+} fhehip_scaling_factor;
+
+/* rq::scaler::Scaler::new (math/src/rq/scaler.rs:27-49) with the RnsScaler of
+ * math/src/rns/scaler.rs:79-175. Contexts must share the degree. */
+int fhehip_scaler_create(const fhehip_context* from, const fhehip_context* to, const fhehip_scaling_factor* factor,
+                         fhehip_scaler** out);
+void fhehip_scaler_destroy(fhehip_scaler* s);
+/* Poly::scale (math/src/rq/mod.rs:623-625 -> rq/scaler.rs:52-126) of
+ * `num_polys` polynomials [num][L_from][N] into [num][L_to][N]; ntt_repr != 0
+ * for NTT-domain polynomials (Ntt representation), 0 for power basis. The
+ * scaler's workspace makes concurrent calls on one handle unsafe. */
+int fhehip_scaler_scale(fhehip_scaler* s, const uint64_t* d_in, uint64_t* d_out, uint32_t num_polys, int ntt_repr,
+                        void* stream);
+
+/* ---- Key switching ---------------------------------------------------------
+ * KeySwitchingKey (fhe/src/bfv/keys/key_switching_key.rs:22-45) from its
+ * NTT-domain rows c0, c1 given as HOST arrays [digits][L_key][N]; digits must
+ * equal the number of ciphertext moduli (RNS-limb decomposition,
+ * key_switching_key.rs:89-104). The key context must extend the ciphertext
+ * context by at most two moduli (the special primes removed by
+ * switch_down_to). The rows are copied to the device with their Shoup
+ * companions (NttShoup representation). */
+int fhehip_ksk_create(const fhehip_context* ct_ctx, const fhehip_context* key_ctx, const uint64_t* c0,
+                      const uint64_t* c1, uint32_t digits, fhehip_ksk** out);
+void fhehip_ksk_destroy(fhehip_ksk* k);
+/* KeySwitchingKey::key_switch (key_switching_key.rs:214-241): power-basis
+ * polynomials [num][L_ct][N] -> NTT-domain (c0, c1) pairs [num][2][L_key][N]. */
+int fhehip_key_switch(const fhehip_ksk* k, const uint64_t* d_p, uint64_t* d_out, uint32_t num, void* stream);
+/* RelinearizationKey::relinearizes (fhe/src/bfv/keys/relinearization_key.rs:71-98)
+ * on NTT-domain 3-part ciphertexts [num][3][L][N] -> [num][2][L][N]. */
+int fhehip_relinearize(fhehip_ksk* rk, const uint64_t* d_ct3, uint64_t* d_out, uint32_t num, void* stream);
+
+/* ---- Multiplication ---------------------------------------------------------
+ * Multiplicator::default (fhe/src/bfv/ops/mul.rs:101-130): extended basis of
+ * ceil((bits(Q) + 60) / 62) generated 62-bit primes, extension by one, t/Q
+ * down-scaling, relinearization with `rk`. */
+int fhehip_multiplicator_default(const fhehip_ksk* rk, uint64_t plaintext_modulus, fhehip_multiplicator** out);
+/* Multiplicator::new (mul.rs:37-98): custom factors and extended basis (the
+ * full multiplication context moduli). NULL factors stand for one. */
+int fhehip_multiplicator_create(const fhehip_context* base_ctx, const uint64_t* extended_basis, uint32_t ext_len,
+                                const fhehip_scaling_factor* lhs, const fhehip_scaling_factor* rhs,
+                                const fhehip_scaling_factor* post, fhehip_multiplicator** out);
+/* Multiplicator::enable_relinearization (mul.rs:133-142). */
+int fhehip_multiplicator_enable_relinearization(fhehip_multiplicator* m, const fhehip_ksk* rk);
+void fhehip_multiplicator_destroy(fhehip_multiplicator* m);
+/* Moduli of the multiplication context; *len is in/out (capacity / count). */
+int fhehip_multiplicator_extended_basis(const fhehip_multiplicator* m, uint64_t* out, uint32_t* len);
+/* Multiplicator::multiply (mul.rs:158-227) on `num` independent pairs:
+ * lhs, rhs [num][2][L][N] NTT -> out [num][2][L][N] (or [num][3][L][N]
+ * without relinearization). One call runs the whole batch as a fixed
+ * sequence of batched kernels; concurrent calls on one handle are unsafe. */
+int fhehip_multiply(fhehip_multiplicator* m, const uint64_t* d_lhs, const uint64_t* d_rhs, uint64_t* d_out,
+                    uint32_t num, void* stream);
+
+/* ---- Galois automorphisms -------------------------------------------------
+ * GaloisKey (fhe/src/bfv/keys/galois_key.rs:15-60) from a key-switching key
+ * built for s(x^exponent) and the exponent (odd modulo 2N). */
+int fhehip_galois_create(const fhehip_ksk* ksk, uint32_t exponent, fhehip_galois** out);
+void fhehip_galois_destroy(fhehip_galois* g);
+/* GaloisKey::relinearize (galois_key.rs:63-124) on [num][2][L][N] NTT
+ * ciphertexts; out must not alias ct. */
+int fhehip_galois_relinearize(fhehip_galois* g, const uint64_t* d_ct, uint64_t* d_out, uint32_t num, void* stream);
+
+/* ---- Measurement ------------------------------------------------------------
+ * No counterpart in fhe.rs. When enabled, every kernel launch is bracketed by a
+ * pair of HIP events on its stream; enabling (or disabling) clears the totals.
+ * The report is text, one line per kernel and degree:
+ * "<kernel>/N=<degree> <launches> <total milliseconds>". `needed` receives the
+ * buffer size the full report requires. */
+int fhehip_profiler_enable(int on);
+int fhehip_profiler_report(char* buf, size_t cap, size_t* needed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -474,6 +474,27 @@ struct KeySwitchingKey {
         }
     }
 
+    // A key given by its NTT-domain rows c0, c1 [digits][L_key][N].
+    KeySwitchingKey(const BfvParams& par, size_t ct_level, size_t key_level, const u64* r0, const u64* r1)
+        : ciphertext_level(ct_level), ksk_level(key_level) {
+        ctx_ksk = par.levels[key_level].ctx;
+        ctx_ct = par.levels[ct_level].ctx;
+        const Context& k = *ctx_ksk;
+        size_t n = k.degree, sz = k.L() * n;
+        for (size_t i = 0; i < ctx_ct->L(); i++) {
+            c0.emplace_back(r0 + i * sz, r0 + (i + 1) * sz);
+            c1.emplace_back(r1 + i * sz, r1 + (i + 1) * sz);
+            PolyData a(sz), b(sz);
+            for (size_t l = 0; l < k.L(); l++)
+                for (size_t j = 0; j < n; j++) {
+                    a[l * n + j] = k.q[l].shoup(c0[i][l * n + j]);
+                    b[l * n + j] = k.q[l].shoup(c1[i][l * n + j]);
+                }
+            c0s.push_back(a);
+            c1s.push_back(b);
+        }
+    }
+
     // key_switching_key.rs:214-241 ; p is power basis in ctx_ct. Returns NTT polys.
     void key_switch(const u64* p, PolyData& out0, PolyData& out1) const {
         const Context& k = *ctx_ksk;

@@ -316,6 +316,15 @@ void* or_ksk_new(void* h, uint32_t kind, uint32_t exponent, uint32_t ct_level, u
     }
 }
 void or_ksk_free(void* k) { delete (KeySwitchingKey*)k; }
+void* or_ksk_from_rows(void* h, uint32_t ct_level, uint32_t key_level, const uint64_t* c0, const uint64_t* c1) {
+    try {
+        OrBfv& b = *(OrBfv*)h;
+        return new KeySwitchingKey(*b.par, ct_level, key_level, c0, c1);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
 // c0, c1: [digits][L'][N] NTT-domain values
 int or_ksk_export(void* k, uint64_t* c0, uint64_t* c1, uint32_t* digits, uint32_t* key_L) {
     OR_TRY({

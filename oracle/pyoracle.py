@@ -64,6 +64,7 @@ def lib():
             "or_bfv_decrypt": (C.c_int, [_vp, C.c_uint32, _u64p, C.c_uint32, _u64p]),
             "or_ksk_new": (_vp, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]),
             "or_ksk_free": (None, [_vp]),
+            "or_ksk_from_rows": (_vp, [_vp, C.c_uint32, C.c_uint32, _u64p, _u64p]),
             "or_ksk_export": (C.c_int, [_vp, _u64p, _u64p, _u32p, _u32p]),
             "or_key_switch": (C.c_int, [_vp, _u64p, _u64p, _u64p, C.c_uint32]),
             "or_mult_new": (_vp, [_vp, _vp, C.c_uint32]),
@@ -213,8 +214,11 @@ class RnsScaler:
                                                _p32(n), len(n), _p32(d), len(d)))
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().or_rns_scaler_free(self.h)
+        if getattr(self, "h", None) and _lib is not None:
+            try:
+                _lib.or_rns_scaler_free(self.h)
+            except Exception:  # interpreter teardown
+                pass
 
     def scale(self, rests, start=0, length=None):
         r = u64(rests).reshape(-1, len(self.f))
@@ -245,8 +249,11 @@ class Context:
         self.h = _chkp(lib().or_ctx_new(_p64(m), len(m), n))
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().or_ctx_free(self.h)
+        if getattr(self, "h", None) and _lib is not None:
+            try:
+                _lib.or_ctx_free(self.h)
+            except Exception:  # interpreter teardown
+                pass
 
     @property
     def L(self):
@@ -299,8 +306,11 @@ class PolyScaler:
             self.h = _chkp(lib().or_poly_scaler_new(from_ctx.h, to_ctx.h, _p32(n), len(n), _p32(d), len(d)))
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().or_poly_scaler_free(self.h)
+        if getattr(self, "h", None) and _lib is not None:
+            try:
+                _lib.or_poly_scaler_free(self.h)
+            except Exception:  # interpreter teardown
+                pass
 
     def scale(self, a, ntt_repr):
         a = u64(a)
@@ -327,8 +337,11 @@ class Bfv:
         self.h = _chkp(lib().or_bfv_new(degree, t, _p64(m), len(m), variance, sk_seed))
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().or_bfv_free(self.h)
+        if getattr(self, "h", None) and _lib is not None:
+            try:
+                _lib.or_bfv_free(self.h)
+            except Exception:  # interpreter teardown
+                pass
 
     def level_moduli(self, level):
         out = np.zeros(len(self.moduli), np.uint64)
@@ -365,6 +378,13 @@ class Bfv:
         k.exponent = exponent
         return k
 
+    def key_from_rows(self, c0, c1, ct_level=0, key_level=0):
+        """KeySwitchingKey from NTT-domain rows [digits][L_key][N]."""
+        c0, c1 = u64(c0).reshape(-1), u64(c1).reshape(-1)
+        k = KeySwitchingKey(self, lib().or_ksk_from_rows(self.h, ct_level, key_level, _p64(c0), _p64(c1)),
+                            ct_level, key_level)
+        return k
+
     def ct_mul(self, lhs, rhs, level=0):
         L = len(self.level_moduli(level))
         out = np.zeros(3 * L * self.degree, np.uint64)
@@ -392,8 +412,11 @@ class KeySwitchingKey:
         self.ct_level, self.key_level = ct_level, key_level
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().or_ksk_free(self.h)
+        if getattr(self, "h", None) and _lib is not None:
+            try:
+                _lib.or_ksk_free(self.h)
+            except Exception:  # interpreter teardown
+                pass
 
     def export(self):
         d = np.zeros(1, np.uint32)
@@ -426,8 +449,11 @@ class Multiplicator:
         self.h = _chkp(lib().or_mult_new(bfv.h, rk.h if rk is not None else None, level))
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().or_mult_free(self.h)
+        if getattr(self, "h", None) and _lib is not None:
+            try:
+                _lib.or_mult_free(self.h)
+            except Exception:  # interpreter teardown
+                pass
 
     def ext_moduli(self):
         out = np.zeros(64, np.uint64)
