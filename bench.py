@@ -179,19 +179,22 @@ def main():
     total_ms = evs[0].elapsed_time(evs[-1])
     value = B * args.steps / (total_ms / 1e3)
 
-    # Per-call GPU drop-in: one operation per call.
-    one_l, one_r, one_o = lhs[:per], rhs[:per], out[:per]
-    for _ in range(3):
-        mult.multiply(one_l, one_r, one_o, 1, stream)
-    torch.cuda.synchronize()
-    pe = [torch.cuda.Event(enable_timing=True) for _ in range(args.per_call_ops + 1)]
-    pe[0].record(stream)
-    for k in range(args.per_call_ops):
-        mult.multiply(lhs[k % B * per:(k % B + 1) * per], rhs[k % B * per:(k % B + 1) * per], one_o, 1, stream)
-        pe[k + 1].record(stream)
-    torch.cuda.synchronize()
-    call_ms = [pe[k].elapsed_time(pe[k + 1]) for k in range(args.per_call_ops)]
-    per_call_value = args.per_call_ops / (pe[0].elapsed_time(pe[-1]) / 1e3)
+    # Per-call GPU drop-in: one operation per call, into its own output buffer.
+    call_ms, per_call_value = [], None
+    if args.per_call_ops > 0:
+        one_o = torch.empty(per, dtype=torch.int64, device=dev)
+        for _ in range(3):
+            mult.multiply(lhs[:per], rhs[:per], one_o, 1, stream)
+        torch.cuda.synchronize()
+        pe = [torch.cuda.Event(enable_timing=True) for _ in range(args.per_call_ops + 1)]
+        pe[0].record(stream)
+        for k in range(args.per_call_ops):
+            i = k % B
+            mult.multiply(lhs[i * per:(i + 1) * per], rhs[i * per:(i + 1) * per], one_o, 1, stream)
+            pe[k + 1].record(stream)
+        torch.cuda.synchronize()
+        call_ms = [pe[k].elapsed_time(pe[k + 1]) for k in range(args.per_call_ops)]
+        per_call_value = args.per_call_ops / (pe[0].elapsed_time(pe[-1]) / 1e3)
 
     # Roofline of the dominant kernel.
     dom = max(kernels.items(), key=lambda kv: kv[1][1])
@@ -232,7 +235,7 @@ def main():
         "p99_step_ms": percentile(step_ms, 99),
         "per_call": {"value": per_call_value, "unit": UNIT, "p50_ms": percentile(call_ms, 50),
                      "p99_ms": percentile(call_ms, 99), "ops": args.per_call_ops},
-        "batched_over_per_call": value / per_call_value,
+        "batched_over_per_call": (value / per_call_value) if per_call_value else None,
         "bit_exact_vs_oracle": bit_exact,
         "kernels_ms": {k: {"launches": c, "avg_ms": ms / c} for k, (c, ms) in sorted(kernels.items())},
         "roofline": roofline,

@@ -88,7 +88,7 @@ void PolyScaler::scale(const u64* in, u64* out, u32 polys, bool ntt_repr, hipStr
         launch_intt_rows(from->logn, a, from->L, 1, polys, s);
         src = work.as<u64>();
     }
-    launch_rns_scale(src, isz, 0, out + (size_t)common * n, osz, 0, d_consts.as<u64>(), (u32)n, 1, polys, s);
+    launch_rns_scale(src, isz, 0, out + (size_t)common * n, osz, 0, d_consts.as<u64>(), from->L, (u32)n, 1, polys, s);
     if (ntt_repr) {
         RowMap m{out, to->L - common, to->L, common, common};
         launch_ntt_forward(*to, m, polys * (to->L - common), false, s);
@@ -205,6 +205,49 @@ void Ksk::finish(const u64* ks, u64* out, u32 inst, const u64* add_pb, size_t ad
     launch_sd_add_ntt(ct_ctx->logn, a, ct_ctx->L, 2, inst, s);
 }
 
+void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* acc, u64* out, u32 inst,
+                            const u64* add_pb, size_t add_inst, size_t add_part, const u64* add_ntt,
+                            size_t add_ntt_inst, size_t add_ntt_part, u32 add_ntt_parts, const u32* perm,
+                            hipStream_t s) const {
+    const size_t n = ct_ctx->N, Lk = key_ctx->L;
+    if (extra > 1) {
+        // Two dropped moduli: power-basis switch-down chain on all limbs.
+        key_switch_pb(src, src_inst_stride, acc, 2 * Lk * n, inst, true, s);
+        finish(acc, out, inst, add_pb, add_inst, add_part, add_ntt, add_ntt_inst, add_ntt_part, add_ntt_parts, perm,
+               s);
+        return;
+    }
+    key_switch_pb(src, src_inst_stride, acc, 2 * Lk * n, inst, false, s);
+    if (extra == 1) {
+        // Only the dropped limb goes back to power basis.
+        RowMap m{acc, 1, (u32)Lk, (u32)(Lk - 1), (u32)(Lk - 1)};
+        launch_ntt_backward(*key_ctx, m, inst * 2, s);
+    }
+    SdNttArgs a{};
+    a.ks = acc;
+    a.ks_part_stride = Lk * n;
+    a.ks_inst_stride = 2 * Lk * n;
+    a.L = ct_ctx->L;
+    a.extra = extra;
+    a.extra_mods = extra ? extra_mods.as<u64>() : nullptr;
+    a.sd = sd_consts.as<SdConst>();
+    a.sd_stride = sd_stride;
+    a.add_pb = add_pb;
+    a.add_inst_stride = add_inst;
+    a.add_part_stride = add_part;
+    a.add_ntt = add_ntt;
+    a.add_ntt_inst_stride = add_ntt_inst;
+    a.add_ntt_part_stride = add_ntt_part;
+    a.add_ntt_parts = add_ntt_parts;
+    a.perm = perm;
+    a.out = out;
+    a.out_part_stride = (size_t)ct_ctx->L * n;
+    a.out_inst_stride = 2 * a.out_part_stride;
+    a.fwd_tw = ct_ctx->fwd_tw.as<ulonglong2>();
+    a.mods = ct_ctx->mod.as<ModInfo>();
+    launch_ks_finish(ct_ctx->logn, a, ct_ctx->L, 2, inst, s);
+}
+
 // KeySwitchingKey::key_switch (key_switching_key.rs:214-241): power-basis p in
 // the ciphertext context -> NTT-domain (c0, c1) in the key context.
 void Ksk::key_switch(const u64* p, u64* out, u32 inst, hipStream_t s) const {
@@ -228,9 +271,8 @@ void Ksk::relinearize(const u64* ct3, u64* out, u32 inst, hipStream_t s) {
     a.inv_tw = ct_ctx->inv_tw.as<ulonglong2>();
     a.mods = ct_ctx->mod.as<ModInfo>();
     launch_intt_rows(ct_ctx->logn, a, (u32)L, 1, inst, s);
-    key_switch_pb(c2, L * n, r, 2 * (size_t)key_ctx->L * n, inst, true, s);
     // (c0, c1) + switch_down(key switch), added in the NTT domain after the transform.
-    finish(r, out, inst, nullptr, 0, 0, ct3, 3 * L * n, L * n, 2, nullptr, s);
+    switch_and_finish(c2, L * n, r, out, inst, nullptr, 0, 0, ct3, 3 * L * n, L * n, 2, nullptr, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -300,10 +342,11 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out, u32 inst,
     launch_intt_rows(base->logn, ia, (u32)L, 4, inst, s);
     // 2. base extension of the power-basis columns (rns/scaler.rs:249-352)
     if (cl < M)
-        launch_rns_scale(wP, 4 * L * n, L * n, wX, 4 * xr * n, xr * n, ext_lhs->d_consts.as<u64>(), (u32)n, 2, inst, s);
+        launch_rns_scale(wP, 4 * L * n, L * n, wX, 4 * xr * n, xr * n, ext_lhs->d_consts.as<u64>(), (u32)L, (u32)n, 2,
+                         inst, s);
     if (cr < M)
         launch_rns_scale(wP + 2 * L * n, 4 * L * n, L * n, wX + 2 * xr * n, 4 * xr * n, xr * n,
-                         ext_rhs->d_consts.as<u64>(), (u32)n, 2, inst, s);
+                         ext_rhs->d_consts.as<u64>(), (u32)L, (u32)n, 2, inst, s);
     // 3. forward NTT of the new limbs (rq/scaler.rs:96-114)
     if (cl == cr) {
         RowMap m{wX, (u32)(M - cl), (u32)xr, 0, cl};
@@ -334,16 +377,17 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out, u32 inst,
     launch_tensor_intt(mul->logn, ta, (u32)M, inst, s);
     // 5. t/Q down-scaling of the three polynomials (mul.rs:187-190)
     u64* dst = rk ? wD : out;
-    launch_rns_scale(wT, 3 * M * n, M * n, dst, 3 * L * n, L * n, down->d_consts.as<u64>(), (u32)n, 3, inst, s);
+    launch_rns_scale(wT, 3 * M * n, M * n, dst, 3 * L * n, L * n, down->d_consts.as<u64>(), (u32)M, (u32)n, 3, inst,
+                     s);
     if (!rk) {
         RowMap m{out, (u32)L, (u32)L, 0, 0};
         launch_ntt_forward(*base, m, inst * 3 * (u32)L, false, s);
         return;
     }
-    // 6. key switch of c2 (power basis already) (relinearization_key.rs:101-106)
-    rk->key_switch_pb(wD + 2 * L * n, 3 * L * n, wR, 2 * (size_t)rk->key_ctx->L * n, inst, true, s);
+    // 6. key switch of c2 (power basis already) (relinearization_key.rs:101-106), then
     // 7. switch down to the ciphertext context, add c0/c1, forward NTT (mul.rs:194-210)
-    rk->finish(wR, out, inst, wD, 3 * L * n, L * n, nullptr, 0, 0, 0, nullptr, s);
+    rk->switch_and_finish(wD + 2 * L * n, 3 * L * n, wR, out, inst, wD, 3 * L * n, L * n, nullptr, 0, 0, 0, nullptr,
+                          s);
 }
 
 // ---------------------------------------------------------------------------
@@ -382,8 +426,7 @@ void Galois::apply(const u64* ct, u64* out, u32 inst, hipStream_t s) {
     a.mods = ksk->ct_ctx->mod.as<ModInfo>();
     a.perm = d_perm.as<u32>();
     launch_intt_rows(ksk->ct_ctx->logn, a, (u32)L, 1, inst, s);
-    ksk->key_switch_pb(c2, L * n, r, 2 * (size_t)ksk->key_ctx->L * n, inst, true, s);
-    ksk->finish(r, out, inst, nullptr, 0, 0, ct, 2 * L * n, 0, 1, d_perm.as<u32>(), s);
+    ksk->switch_and_finish(c2, L * n, r, out, inst, nullptr, 0, 0, ct, 2 * L * n, 0, 1, d_perm.as<u32>(), s);
 }
 
 // ---------------------------------------------------------------------------

@@ -43,7 +43,12 @@ struct Ksk {
     // Digit lift + MAC (+ inverse NTT when to_power_basis) into acc [inst][2][L'][N].
     void key_switch_pb(const u64* src, size_t src_inst_stride, u64* acc, size_t acc_inst_stride, u32 inst,
                        bool to_power_basis, hipStream_t s) const;
-    // switch_down_to the ciphertext context + addends + forward NTT into out [inst][2][L][N].
+    // Key switch of power-basis digits followed by switch_down_to the
+    // ciphertext context, addends and forward NTT into out [inst][2][L][N].
+    // acc: scratch [inst][2][L'][N].
+    void switch_and_finish(const u64* src, size_t src_inst_stride, u64* acc, u64* out, u32 inst, const u64* add_pb,
+                           size_t add_inst, size_t add_part, const u64* add_ntt, size_t add_ntt_inst,
+                           size_t add_ntt_part, u32 add_ntt_parts, const u32* perm, hipStream_t s) const;
     void finish(const u64* ks, u64* out, u32 inst, const u64* add_pb, size_t add_inst, size_t add_part,
                 const u64* add_ntt, size_t add_ntt_inst, size_t add_ntt_part, u32 add_ntt_parts, const u32* perm,
                 hipStream_t s) const;

@@ -14,6 +14,7 @@ namespace fhehip {
 // start at in + b * in_inst + k * in_part; outputs likewise.
 // grid = (N / 256, parts, instances)
 // ---------------------------------------------------------------------------
+template <int NF>
 __global__ __launch_bounds__(256) void rns_scale_kernel(const u64* __restrict__ in, size_t in_inst, size_t in_part,
                                                         u64* __restrict__ out, size_t out_inst, size_t out_part,
                                                         const u64* __restrict__ sc, u32 n) {
@@ -21,17 +22,38 @@ __global__ __launch_bounds__(256) void rns_scale_kernel(const u64* __restrict__ 
     if (col >= n) return;
     const u64* src = in + blockIdx.z * in_inst + blockIdx.y * in_part + col;
     u64* dst = out + blockIdx.z * out_inst + blockIdx.y * out_part + col;
-    rns_scale_column(
-        sc, [&](int j) { return src[(size_t)j * n]; }, [&](int i, u64 v) { dst[(size_t)i * n] = v; });
+    u64 r[NF];
+#pragma unroll
+    for (int j = 0; j < NF; j++) r[j] = src[(size_t)j * n];
+    rns_scale_column<NF>(
+        sc, [&](int j) { return r[j]; }, [&](int i, u64 v) { dst[(size_t)i * n] = v; });
+}
+
+template <int NF>
+static void rns_scale_launch(dim3 grid, hipStream_t s, const u64* in, size_t in_inst, size_t in_part, u64* out,
+                             size_t out_inst, size_t out_part, const u64* d_sc, u32 n) {
+    hipLaunchKernelGGL(rns_scale_kernel<NF>, grid, dim3(256), 0, s, in, in_inst, in_part, out, out_inst, out_part,
+                       d_sc, n);
 }
 
 void launch_rns_scale(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst, size_t out_part,
-                      const u64* d_sc, u32 n, u32 parts, u32 inst, hipStream_t s) {
+                      const u64* d_sc, u32 nf, u32 n, u32 parts, u32 inst, hipStream_t s) {
     if (!parts || !inst) return;
     ProfScope ps("rns_scale", 31 - __builtin_clz(n), s);
     dim3 grid((n + 255) / 256, parts, inst);
-    hipLaunchKernelGGL(rns_scale_kernel, grid, dim3(256), 0, s, in, in_inst, in_part, out, out_inst, out_part, d_sc,
-                       n);
+#define FHEHIP_SC_CASE(K) \
+    case K: rns_scale_launch<K>(grid, s, in, in_inst, in_part, out, out_inst, out_part, d_sc, n); break;
+    switch (nf) {
+        FHEHIP_SC_CASE(1) FHEHIP_SC_CASE(2) FHEHIP_SC_CASE(3) FHEHIP_SC_CASE(4) FHEHIP_SC_CASE(5)
+        FHEHIP_SC_CASE(6) FHEHIP_SC_CASE(7) FHEHIP_SC_CASE(8) FHEHIP_SC_CASE(9) FHEHIP_SC_CASE(10)
+        FHEHIP_SC_CASE(11) FHEHIP_SC_CASE(12) FHEHIP_SC_CASE(13) FHEHIP_SC_CASE(14) FHEHIP_SC_CASE(15)
+        FHEHIP_SC_CASE(16) FHEHIP_SC_CASE(17) FHEHIP_SC_CASE(18) FHEHIP_SC_CASE(19) FHEHIP_SC_CASE(20)
+        FHEHIP_SC_CASE(21) FHEHIP_SC_CASE(22) FHEHIP_SC_CASE(23) FHEHIP_SC_CASE(24) FHEHIP_SC_CASE(25)
+        FHEHIP_SC_CASE(26) FHEHIP_SC_CASE(27) FHEHIP_SC_CASE(28) FHEHIP_SC_CASE(29) FHEHIP_SC_CASE(30)
+        FHEHIP_SC_CASE(31) FHEHIP_SC_CASE(32)
+        default: throw Error(-2, "more than 32 source moduli in one scaling");
+    }
+#undef FHEHIP_SC_CASE
     HIPCHK(hipGetLastError());
 }
 
@@ -53,16 +75,10 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void intt_rows_kernel(InttRowsAr
     const ModInfo mi = a.mods[limb];
     u64 x[S::E];
     if (a.perm) {
-#pragma unroll
-        for (int e = 0; e < S::E; e++) x[e] = src[a.perm[(t << S::RE) + e]];
+        const u32* perm = a.perm;
+        load_inv_order<LOGN>(lds, x, t, [&](int j) { return src[perm[j]]; });
     } else {
-        const ulonglong2* in = reinterpret_cast<const ulonglong2*>(src + ((size_t)t << S::RE));
-#pragma unroll
-        for (int e = 0; e < S::E / 2; e++) {
-            ulonglong2 v = in[e];
-            x[2 * e] = v.x;
-            x[2 * e + 1] = v.y;
-        }
+        load_inv_order<LOGN>(lds, x, t, [&](int j) { return src[j]; });
     }
     ntt_inv_regs<LOGN>(lds, x, t, tw, mi.p);
 #pragma unroll
@@ -96,33 +112,37 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void tensor_intt_kernel(TensorAr
                       ((size_t)(m - common) << LOGN);
         }
     }
+    // The products are Montgomery products (each carries a factor 2^-64); the
+    // inverse transform is linear, so its final scaling by N^-1 * 2^64 returns
+    // exactly INTT(c) (c0 = a0 b0, c1 = a0 b1 + a1 b0, c2 = a1 b1 mod q_m).
+    // The product is elementwise, so it is formed in the coalesced order and
+    // transposed through LDS into the transform's input order.
+    const u64 p = mi.p, pn = mi.pinv_neg;
     u64 x[S::E];
-    const size_t off = (size_t)t << S::RE;
-#pragma unroll
-    for (int e = 0; e < S::E; e += 2) {
-        ulonglong2 a0 = *reinterpret_cast<const ulonglong2*>(rows[0] + off + e);
-        ulonglong2 a1 = *reinterpret_cast<const ulonglong2*>(rows[1] + off + e);
-        ulonglong2 b0 = *reinterpret_cast<const ulonglong2*>(rows[2] + off + e);
-        ulonglong2 b1 = *reinterpret_cast<const ulonglong2*>(rows[3] + off + e);
-        if (c == 0) {
-            x[e] = mul_mod(a0.x, b0.x, mi.barrett_lo, mi.barrett_hi, mi.p);
-            x[e + 1] = mul_mod(a0.y, b0.y, mi.barrett_lo, mi.barrett_hi, mi.p);
-        } else if (c == 1) {
-            x[e] = add_mod(mul_mod(a0.x, b1.x, mi.barrett_lo, mi.barrett_hi, mi.p),
-                           mul_mod(a1.x, b0.x, mi.barrett_lo, mi.barrett_hi, mi.p), mi.p);
-            x[e + 1] = add_mod(mul_mod(a0.y, b1.y, mi.barrett_lo, mi.barrett_hi, mi.p),
-                               mul_mod(a1.y, b0.y, mi.barrett_lo, mi.barrett_hi, mi.p), mi.p);
-        } else {
-            x[e] = mul_mod(a1.x, b1.x, mi.barrett_lo, mi.barrett_hi, mi.p);
-            x[e + 1] = mul_mod(a1.y, b1.y, mi.barrett_lo, mi.barrett_hi, mi.p);
-        }
+    if (c == 1) {
+        const u64 *ra0 = rows[0], *ra1 = rows[1], *rb0 = rows[2], *rb1 = rows[3];
+        load_inv_order<LOGN, 2>(lds, x, t, [&](int j) {
+            u64 lo, hi, l2, h2;
+            mul_wide(ra0[j], rb1[j], lo, hi);
+            mul_wide(ra1[j], rb0[j], l2, h2);
+            add128(lo, hi, l2, h2);
+            return reduce1(mont_redc(lo, hi, p, pn), p);
+        });
+    } else {
+        const u64* ra = rows[c == 0 ? 0 : 1];
+        const u64* rb = rows[c == 0 ? 2 : 3];
+        load_inv_order<LOGN>(lds, x, t, [&](int j) {
+            u64 lo, hi;
+            mul_wide(ra[j], rb[j], lo, hi);
+            return reduce1(mont_redc(lo, hi, p, pn), p);
+        });
     }
     const ulonglong2* tw = a.inv_tw + ((size_t)m << LOGN);
     ntt_inv_regs<LOGN>(lds, x, t, tw, mi.p);
     u64* dst = a.out + (size_t)b * a.out_inst_stride + (size_t)c * a.out_part_stride + ((size_t)m << LOGN);
 #pragma unroll
     for (int e = 0; e < S::E; e++)
-        dst[inv_index<LOGN, last_start<LOGN>()>(t, e)] = mul_shoup(x[e], mi.n_inv, mi.n_inv_shoup, mi.p);
+        dst[inv_index<LOGN, last_start<LOGN>()>(t, e)] = mul_shoup(x[e], mi.n_inv_r, mi.n_inv_r_shoup, mi.p);
 }
 
 // ---------------------------------------------------------------------------
@@ -245,23 +265,82 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void sd_add_ntt_kernel(SdNttArgs
     }
     ntt_fwd_regs<LOGN>(lds, x, t, a.fwd_tw + ((size_t)i << LOGN), p);
     const u64 p2 = 2 * p;
-    const size_t off = (size_t)t << S::RE;
+    u64* dst = a.out + (size_t)b * a.out_inst_stride + (size_t)part * a.out_part_stride + ((size_t)i << LOGN);
     if (a.add_ntt && part < a.add_ntt_parts) {
         const u64* g = a.add_ntt + (size_t)b * a.add_ntt_inst_stride + (size_t)part * a.add_ntt_part_stride +
                        ((size_t)i << LOGN);
+        const u32* perm = a.perm;
+        store_fwd_order<LOGN>(lds, x, t, [&](int j, u64 v) {
+            dst[j] = add_mod(reduce1(reduce1(v, p2), p), g[perm ? perm[j] : (u32)j], p);
+        });
+    } else {
+        store_fwd_order<LOGN>(lds, x, t, [&](int j, u64 v) { dst[j] = reduce1(reduce1(v, p2), p); });
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Key-switch finish for a key context at most one modulus above the ciphertext
+// context. switch_down (math/src/rq/mod.rs:390-449) maps limb i to
+//   q_l^-1 (x_i - [y]_i) mod q_i,  y = (x_l + floor(q_l/2)) mod q_l,
+// [y]_i = lazy_reduce(y) - floor(q_l/2) mod q_i,
+// which is linear in x_i. Only the dropped limb l must be in power basis; the
+// other limbs stay in the NTT domain:
+//   out_i = NTT(add_pb_i - q_l^-1 [y]_i) + q_l^-1 ks_i  (+ NTT-domain addend).
+// With no extra modulus, out_i = NTT(add_pb_i) + ks_i. `ks` rows i < L are
+// NTT-domain; row L (when extra == 1) is power basis.
+// grid = (L, parts, instances)
+// ---------------------------------------------------------------------------
+template <int LOGN>
+__global__ __launch_bounds__(NttShape<LOGN>::T) void ks_finish_kernel(SdNttArgs a) {
+    using S = NttShape<LOGN>;
+    __shared__ u64 lds[S::LDS_WORDS];
+    const int t = threadIdx.x;
+    const u32 i = blockIdx.x, part = blockIdx.y, b = blockIdx.z;
+    const ModInfo mi = a.mods[i];
+    const u64 p = mi.p;
+    const u64* ks = a.ks + (size_t)b * a.ks_inst_stride + (size_t)part * a.ks_part_stride;
+    const u64* ad = a.add_pb ? a.add_pb + (size_t)b * a.add_inst_stride + (size_t)part * a.add_part_stride +
+                                   ((size_t)i << LOGN)
+                             : nullptr;
+    u64 x[S::E];
+    SdConst cst;
+    u64 ql = 0;
+    if (a.extra) {
+        cst = a.sd[i];
+        ql = a.extra_mods[0];
+        const u64 half = ql >> 1;
+        const u64* lrow = ks + ((size_t)a.L << LOGN);
 #pragma unroll
         for (int e = 0; e < S::E; e++) {
-            u64 v = reduce1(reduce1(x[e], p2), p);
-            x[e] = add_mod(v, g[a.perm ? a.perm[off + e] : (u32)(off + e)], p);
+            const int j = fwd_index<LOGN, 0>(t, e);
+            u64 y = lrow[j] + half;
+            y = y >= ql ? y - ql : y;
+            const u64 tmp = lazy_reduce_u64(y, mi.barrett_lo, mi.barrett_hi, p) + cst.half_mod;
+            u64 z = mul_shoup(3 * p - tmp, cst.inv, cst.inv_shoup, p);
+            if (ad) z = add_mod(z, ad[j], p);
+            x[e] = z;
         }
     } else {
 #pragma unroll
-        for (int e = 0; e < S::E; e++) x[e] = reduce1(reduce1(x[e], p2), p);
+        for (int e = 0; e < S::E; e++) x[e] = ad ? ad[fwd_index<LOGN, 0>(t, e)] : 0;
     }
+    ntt_fwd_regs<LOGN>(lds, x, t, a.fwd_tw + ((size_t)i << LOGN), p);
+    const u64 p2 = 2 * p;
+    const u64* krow = ks + ((size_t)i << LOGN);
     u64* dst = a.out + (size_t)b * a.out_inst_stride + (size_t)part * a.out_part_stride + ((size_t)i << LOGN);
-    ulonglong2* o = reinterpret_cast<ulonglong2*>(dst + off);
-#pragma unroll
-    for (int e = 0; e < S::E / 2; e++) o[e] = make_ulonglong2(x[2 * e], x[2 * e + 1]);
+    const u64* g = (a.add_ntt && part < a.add_ntt_parts)
+                       ? a.add_ntt + (size_t)b * a.add_ntt_inst_stride + (size_t)part * a.add_ntt_part_stride +
+                             ((size_t)i << LOGN)
+                       : nullptr;
+    const u32* perm = a.perm;
+    const bool extra = a.extra != 0;
+    store_fwd_order<LOGN>(lds, x, t, [&](int j, u64 v) {
+        u64 r = reduce1(reduce1(v, p2), p);
+        const u64 kv = krow[j];
+        r = add_mod(r, extra ? mul_shoup(kv, cst.inv, cst.inv_shoup, p) : kv, p);
+        if (g) r = add_mod(r, g[perm ? perm[j] : (u32)j], p);
+        dst[j] = r;
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -302,6 +381,13 @@ void launch_keyswitch(int logn, const KeySwitchArgs& a, u32 key_limbs, u32 inst,
     ProfScope ps("keyswitch", logn, s);
     dim3 grid(key_limbs, inst);
     FHEHIP_LOGN_DISPATCH(logn, keyswitch_kernel, grid, 0, s, a);
+}
+
+void launch_ks_finish(int logn, const SdNttArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s) {
+    if (!L || !parts || !inst) return;
+    ProfScope ps("ks_finish", logn, s);
+    dim3 grid(L, parts, inst);
+    FHEHIP_LOGN_DISPATCH(logn, ks_finish_kernel, grid, 0, s, a);
 }
 
 void launch_sd_add_ntt(int logn, const SdNttArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s) {

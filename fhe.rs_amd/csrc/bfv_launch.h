@@ -69,10 +69,11 @@ struct SdNttArgs {
 };
 
 void launch_rns_scale(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst, size_t out_part,
-                      const u64* d_sc, u32 n, u32 parts, u32 inst, hipStream_t s);
+                      const u64* d_sc, u32 nf, u32 n, u32 parts, u32 inst, hipStream_t s);
 void launch_intt_rows(int logn, const InttRowsArgs& a, u32 limbs, u32 parts, u32 inst, hipStream_t s);
 void launch_tensor_intt(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_t s);
 void launch_keyswitch(int logn, const KeySwitchArgs& a, u32 key_limbs, u32 inst, hipStream_t s);
 void launch_sd_add_ntt(int logn, const SdNttArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s);
+void launch_ks_finish(int logn, const SdNttArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s);
 
 }  // namespace fhehip

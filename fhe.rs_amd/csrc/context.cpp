@@ -70,6 +70,11 @@ std::shared_ptr<DeviceContext> make_context(const std::vector<u64>& moduli, u32 
         barrett128(p, mi.barrett_lo, mi.barrett_hi);
         mi.n_inv = invmod64(n % p, p);
         mi.n_inv_shoup = shoup64(mi.n_inv, p);
+        u64 inv = 1;  // p^-1 mod 2^64 by Newton iteration (p odd)
+        for (int it = 0; it < 6; it++) inv *= 2 - p * inv;
+        mi.pinv_neg = (u64)0 - inv;
+        mi.n_inv_r = (u64)(((u128)mi.n_inv << 64) % p);
+        mi.n_inv_r_shoup = shoup64(mi.n_inv_r, p);
         c->h_mod.push_back(mi);
     }
     // q_last^-1 mod q_i (context.rs:67-74)

@@ -194,4 +194,39 @@ __device__ __forceinline__ void ntt_inv_regs(u64* lds, u64 (&x)[NttShape<LOGN>::
     inv_rest<LOGN, 0>(lds, x, t, tw, p, p2);
 }
 
+// Coalesced global row -> registers in the inverse-transform input order
+// (value e of thread t is element t*E + e): the row is read with consecutive
+// lanes on consecutive elements and transposed through LDS. Ends with a
+// barrier, so the LDS is free for the transform.
+template <int LOGN, int UNROLL = 4, class Load>
+__device__ __forceinline__ void load_inv_order(u64* lds, u64 (&x)[NttShape<LOGN>::E], int t, Load load) {
+    using S = NttShape<LOGN>;
+    // Partial unrolling bounds the number of loads in flight (and registers).
+#pragma unroll UNROLL
+    for (int e = 0; e < S::E; e++) {
+        const int j = e * S::T + t;
+        lds[lds_slot(j)] = load(j);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < S::E; e++) x[e] = lds[lds_slot((t << S::RE) + e)];
+    __syncthreads();
+}
+
+// Registers in the forward-transform output order (value e of thread t is
+// element t*E + e) -> coalesced `store(j, value)`. Must follow a forward
+// transform (whose last LDS access is fenced by a barrier).
+template <int LOGN, class Store>
+__device__ __forceinline__ void store_fwd_order(u64* lds, const u64 (&x)[NttShape<LOGN>::E], int t, Store store) {
+    using S = NttShape<LOGN>;
+#pragma unroll
+    for (int e = 0; e < S::E; e++) lds[lds_slot((t << S::RE) + e)] = x[e];
+    __syncthreads();
+#pragma unroll 4
+    for (int e = 0; e < S::E; e++) {
+        const int j = e * S::T + t;
+        store(j, lds[lds_slot(j)]);
+    }
+}
+
 }  // namespace fhehip

@@ -28,16 +28,7 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void ntt_fwd_kernel(RowMap m, co
     for (int k = 0; k < S::E; k++) x[k] = row[fwd_index<LOGN, 0>(t, k)];
     ntt_fwd_regs<LOGN>(lds, x, t, tw, p);
     const u64 p2 = 2 * p;
-#pragma unroll
-    for (int k = 0; k < S::E; k++) {
-        u64 v = x[k];
-        if (!LAZY) v = reduce1(reduce1(v, p2), p);
-        x[k] = v;
-    }
-    // Output order: thread t owns the E consecutive values t*E .. t*E+E-1.
-    ulonglong2* out = reinterpret_cast<ulonglong2*>(row + ((size_t)t << S::RE));
-#pragma unroll
-    for (int k = 0; k < S::E / 2; k++) out[k] = make_ulonglong2(x[2 * k], x[2 * k + 1]);
+    store_fwd_order<LOGN>(lds, x, t, [&](int j, u64 v) { row[j] = LAZY ? v : reduce1(reduce1(v, p2), p); });
 }
 
 // Inverse NTT (NttOperator::backward, native.rs:106-132).
@@ -52,13 +43,7 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void ntt_inv_kernel(RowMap m, co
     const ulonglong2* tw = tw_all + ((size_t)limb << LOGN);
     const ModInfo mi = mods[limb];
     u64 x[S::E];
-    const ulonglong2* in = reinterpret_cast<const ulonglong2*>(row + ((size_t)t << S::RE));
-#pragma unroll
-    for (int k = 0; k < S::E / 2; k++) {
-        ulonglong2 v = in[k];
-        x[2 * k] = v.x;
-        x[2 * k + 1] = v.y;
-    }
+    load_inv_order<LOGN>(lds, x, t, [&](int j) { return row[j]; });
     ntt_inv_regs<LOGN>(lds, x, t, tw, mi.p);
 #pragma unroll
     for (int k = 0; k < S::E; k++)

@@ -33,12 +33,12 @@ __device__ __forceinline__ void u256_zero(U256& a) { a.w[0] = a.w[1] = a.w[2] = 
 
 // a += r * (lo + 2^64 hi)  (product < 2^192), wrapping at 2^256
 __device__ __forceinline__ void u256_mac(U256& a, u64 r, u64 lo, u64 hi, bool subtract) {
-    u128 p0 = (u128)r * lo;
-    u128 p1 = (u128)r * hi;
-    u64 m0 = (u64)p0;
-    u128 mid = (p0 >> 64) + (u64)p1;
-    u64 m1 = (u64)mid;
-    u64 m2 = (u64)((mid >> 64) + (p1 >> 64));
+    u64 p0l, p0h, p1l, p1h;
+    mul_wide(r, lo, p0l, p0h);
+    mul_wide(r, hi, p1l, p1h);
+    const u64 m0 = p0l;
+    const u64 m1 = p0h + p1l;
+    const u64 m2 = p1h + (m1 < p0h ? 1 : 0);
     if (!subtract) {
         u128 s = (u128)a.w[0] + m0;
         a.w[0] = (u64)s;
@@ -95,11 +95,12 @@ __device__ __forceinline__ u128 u256_shr_lo128(const U256& a, unsigned s) {
     return ((u128)r[1] << 64) | r[0];
 }
 
-// One column of RnsScaler::scale. `rests(j)` yields residue j of the column;
-// `emit(i, value)` receives output i (0-based, output modulus start + i).
-template <class Rests, class Emit>
+// One column of RnsScaler::scale with NF input residues. `rests(j)` yields
+// residue j of the column; `emit(i, value)` receives output i (0-based, output
+// modulus start + i).
+template <int NF, class Rests, class Emit>
 __device__ __forceinline__ void rns_scale_column(const u64* __restrict__ sc, Rests rests, Emit emit) {
-    const int nf = (int)sc[SC_NF];
+    constexpr int nf = NF;
     const int count = (int)sc[SC_COUNT];
     const unsigned shift = (unsigned)sc[SC_SHIFT];
     const bool is_one = sc[SC_IS_ONE] != 0;
@@ -109,6 +110,7 @@ __device__ __forceinline__ void rns_scale_column(const u64* __restrict__ sc, Res
     // v = ceil((sum r_j theta_garner_j >> (shift - 1)) / 2)
     U256 sum;
     u256_zero(sum);
+#pragma unroll
     for (int j = 0; j < nf; j++) u256_mac(sum, rests(j), tg[2 * j], tg[2 * j + 1], false);
     u128 vv = u256_shr_lo128(sum, shift - 1);
     const u128 v = (vv >> 1) + (vv & 1);
@@ -118,6 +120,7 @@ __device__ __forceinline__ void rns_scale_column(const u64* __restrict__ sc, Res
     if (!is_one) {
         U256 so;
         u256_zero(so);
+#pragma unroll
         for (int j = 0; j < nf; j++) u256_mac(so, rests(j), tom[3 * j], tom[3 * j + 1], tom[3 * j + 2] != 0);
         // v * theta_gamma (128 x 128 -> 256)
         const u64 tgl = sc[SC_TG_LO], tgh = sc[SC_TG_HI];
@@ -148,21 +151,33 @@ __device__ __forceinline__ void rns_scale_column(const u64* __restrict__ sc, Res
         }
     }
 
+    // Each output residue is the exact value of
+    //   2 q - [v]_q gamma (+/- [w]_q) + sum_j r_j omega_ij   mod q,
+    // which the reference reduces term by term; here the products are summed
+    // exactly in 128 bits (folded every four terms) and reduced once.
     const u64* outc = sc + sc_out_base(nf);
     const int ostride = sc_out_stride(nf);
     for (int i = 0; i < count; i++) {
         const u64* o = outc + (size_t)i * ostride;
         const u64 q = o[0], bl = o[1], bh = o[2];
         const u64 vq = reduce_u128((u64)v, (u64)(v >> 64), bl, bh, q);
-        u128 yi = (u128)(2 * q - lazy_mul_shoup(vq, o[3], o[4], q));
+        u64 slo = 2 * q - lazy_mul_shoup(vq, o[3], o[4], q), shi = 0;
         if (!is_one) {
-            u64 wi = lazy_reduce_u128((u64)w, (u64)(w >> 64), bl, bh, q);
-            yi += w_sign ? (2 * q - wi) : wi;
+            const u64 wi = lazy_reduce_u128((u64)w, (u64)(w >> 64), bl, bh, q);
+            add128(slo, shi, w_sign ? (2 * q - wi) : wi, 0);
         }
         const u64* om = o + 5;
-        const u64* oms = om + nf;
-        for (int j = 0; j < nf; j++) yi += lazy_mul_shoup(rests(j), om[j], oms[j], q);
-        emit(i, reduce_u128((u64)yi, (u64)(yi >> 64), bl, bh, q));
+#pragma unroll
+        for (int j = 0; j < nf; j++) {
+            u64 lo, hi;
+            mul_wide(rests(j), om[j], lo, hi);
+            add128(slo, shi, lo, hi);
+            if ((j & 3) == 3) {
+                slo = lazy_reduce_u128(slo, shi, bl, bh, q);
+                shi = 0;
+            }
+        }
+        emit(i, reduce_u128(slo, shi, bl, bh, q));
     }
 }
 
