@@ -36,15 +36,16 @@ std::vector<u64> scaler_blob(const ScalerConsts& c, const std::vector<ModInfo>& 
     for (u32 i = 0; i < count; i++) {
         u64* o = b.data() + sc_out_base(nf) + (size_t)i * sc_out_stride(nf);
         const u32 ti = start + i;
-        o[0] = to_mods[ti].p;
+        const u64 q = to_mods[ti].p;
+        const u64 gneg = (q - c.gamma[ti] % q) % q;  // -gamma mod q
+        o[0] = q;
         o[1] = to_mods[ti].barrett_lo;
         o[2] = to_mods[ti].barrett_hi;
-        o[3] = c.gamma[ti];
-        o[4] = c.gamma_shoup[ti];
-        for (u32 j = 0; j < nf; j++) {
-            o[5 + j] = c.omega[(size_t)ti * nf + j];
-            o[5 + nf + j] = c.omega_shoup[(size_t)ti * nf + j];
-        }
+        o[3] = gneg;
+        o[4] = (u64)(((u128)gneg << 64) % q);
+        o[5] = (u64)(((u128)1 << 64) % q);
+        o[6] = (u64)(((u128)o[5] << 64) % q);
+        for (u32 j = 0; j < nf; j++) o[7 + j] = c.omega[(size_t)ti * nf + j];
     }
     return b;
 }
@@ -427,6 +428,39 @@ void Galois::apply(const u64* ct, u64* out, u32 inst, hipStream_t s) {
     a.perm = d_perm.as<u32>();
     launch_intt_rows(ksk->ct_ctx->logn, a, (u32)L, 1, inst, s);
     ksk->switch_and_finish(c2, L * n, r, out, inst, nullptr, 0, 0, ct, 2 * L * n, 0, 1, d_perm.as<u32>(), s);
+}
+
+// ---------------------------------------------------------------------------
+std::shared_ptr<Galois> EvaluationKey::find(u32 exponent) const {
+    for (auto& g : keys)
+        if (g->exponent == exponent % (2 * degree)) return g;
+    return nullptr;
+}
+
+u32 EvaluationKey::column_exponent(u32 i) const { return (u32)powmod64(3, i, 2 * (u64)degree); }
+
+void EvaluationKey::rotate(u32 exponent, const u64* ct, u64* out, u32 inst, hipStream_t s) const {
+    auto g = find(exponent);
+    if (!g) throw Error(-1, "This key does not support the requested rotation");
+    g->apply(ct, out, inst, s);
+}
+
+// EvaluationKey::computes_inner_sum (evaluation_key.rs:56-82): out starts as ct
+// and accumulates the rotations of itself.
+void EvaluationKey::rotate_and_sum(const std::vector<u32>& exponents, const u64* ct, u64* out, u32 inst,
+                                   hipStream_t s) {
+    if (keys.empty()) throw Error(-1, "empty evaluation key");
+    for (u32 e : exponents)
+        if (!find(e)) throw Error(-1, "This key does not support the inner sum functionality");
+    const auto& c = *keys[0]->ksk->ct_ctx;
+    const size_t sz = (size_t)inst * 2 * c.L * c.N;
+    work.ensure(sz * 8);
+    u64* tmp = work.as<u64>();
+    if (out != ct) HIPCHK(hipMemcpyAsync(out, ct, sz * 8, hipMemcpyDeviceToDevice, s));
+    for (u32 e : exponents) {
+        find(e)->apply(out, tmp, inst, s);
+        launch_add_rows(out, tmp, out, c.mod.as<ModInfo>(), c.L, c.logn, (size_t)inst * 2 * c.L, s);
+    }
 }
 
 // ---------------------------------------------------------------------------

@@ -79,4 +79,18 @@ struct Galois {
     void apply(const u64* ct, u64* out, u32 inst, hipStream_t s);
 };
 
+// EvaluationKey (fhe/src/bfv/keys/evaluation_key.rs:56-131): Galois keys by
+// exponent, column/row rotations and inner sums.
+struct EvaluationKey {
+    std::vector<std::shared_ptr<Galois>> keys;
+    u32 degree = 0;
+    Workspace work;
+    std::shared_ptr<Galois> find(u32 exponent) const;
+    // 3^i mod 2N (evaluation_key.rs:195-203)
+    u32 column_exponent(u32 i) const;
+    void rotate(u32 exponent, const u64* ct, u64* out, u32 inst, hipStream_t s) const;
+    // out = ct; for each listed exponent: out += rotate(out)
+    void rotate_and_sum(const std::vector<u32>& exponents, const u64* ct, u64* out, u32 inst, hipStream_t s);
+};
+
 }  // namespace fhehip

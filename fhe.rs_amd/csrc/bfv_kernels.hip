@@ -344,6 +344,30 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void ks_finish_kernel(SdNttArgs 
 }
 
 // ---------------------------------------------------------------------------
+// Ciphertext addition (fhe/src/bfv/ops/mod.rs:15-69 -> Poly AddAssign,
+// math/src/rq/ops.rs:92-118): out = a + b mod q_limb, elementwise over
+// [rows][N] where row r has modulus r % L.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void add_rows_kernel(const u64* __restrict__ a, const u64* __restrict__ b,
+                                                       u64* __restrict__ out, const ModInfo* __restrict__ mods,
+                                                       u32 L, u32 logn, size_t total) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= total) return;
+    const u64 p = mods[(i >> logn) % L].p;
+    out[i] = add_mod(a[i], b[i], p);
+}
+
+void launch_add_rows(const u64* a, const u64* b, u64* out, const ModInfo* mods, u32 L, int logn, size_t rows,
+                     hipStream_t s) {
+    const size_t total = rows << logn;
+    if (!total) return;
+    ProfScope ps("add_rows", logn, s);
+    hipLaunchKernelGGL(add_rows_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a, b, out, mods, L,
+                       (u32)logn, total);
+    HIPCHK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
 #define FHEHIP_LOGN_DISPATCH(logn, KERNEL, grid, block, stream, arg)                         \
     switch (logn) {                                                                          \
         case 3: hipLaunchKernelGGL(KERNEL<3>, grid, dim3(NttShape<3>::T), 0, stream, arg); break;   \

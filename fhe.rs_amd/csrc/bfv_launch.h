@@ -75,5 +75,7 @@ void launch_tensor_intt(int logn, const TensorArgs& a, u32 M, u32 inst, hipStrea
 void launch_keyswitch(int logn, const KeySwitchArgs& a, u32 key_limbs, u32 inst, hipStream_t s);
 void launch_sd_add_ntt(int logn, const SdNttArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s);
 void launch_ks_finish(int logn, const SdNttArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s);
+void launch_add_rows(const u64* a, const u64* b, u64* out, const ModInfo* mods, u32 L, int logn, size_t rows,
+                     hipStream_t s);
 
 }  // namespace fhehip

@@ -18,7 +18,10 @@ struct fhehip_multiplicator {
     std::unique_ptr<Multiplicator> m;
 };
 struct fhehip_galois {
-    std::unique_ptr<Galois> g;
+    std::shared_ptr<Galois> g;
+};
+struct fhehip_evaluation_key {
+    std::unique_ptr<EvaluationKey> ek;
 };
 
 static BigUint factor_num(const fhehip_scaling_factor* f) {
@@ -140,8 +143,8 @@ int fhehip_galois_create(const fhehip_ksk* ksk, uint32_t exponent, fhehip_galois
     FHEHIP_API_BEGIN
     if (!ksk || !out) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
-    std::unique_ptr<Galois> g(new Galois(ksk->k, exponent));
-    *out = new fhehip_galois{std::move(g)};
+    auto g = std::make_shared<Galois>(ksk->k, exponent);
+    *out = new fhehip_galois{g};
     FHEHIP_API_END
 }
 
@@ -152,6 +155,75 @@ int fhehip_galois_relinearize(fhehip_galois* g, const uint64_t* d_ct, uint64_t* 
     if (!g || ((!d_ct || !d_out) && num)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
     if (d_ct == d_out && num) return fail(FHEHIP_ERR_INVALID_ARG, "in-place Galois relinearization is not supported");
     g->g->apply(d_ct, d_out, num, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_evaluation_key_create(fhehip_galois* const* keys, uint32_t count, fhehip_evaluation_key** out) {
+    FHEHIP_API_BEGIN
+    if (!keys || !count || !out) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    std::unique_ptr<EvaluationKey> ek(new EvaluationKey);
+    for (uint32_t i = 0; i < count; i++) {
+        if (!keys[i]) return fail(FHEHIP_ERR_INVALID_ARG, "null Galois key");
+        const auto& g = keys[i]->g;
+        if (i && (g->ksk->ct_ctx->moduli != ek->keys[0]->ksk->ct_ctx->moduli ||
+                  g->ksk->ct_ctx->N != ek->keys[0]->ksk->ct_ctx->N))
+            return fail(FHEHIP_ERR_CTX_MISMATCH, "Galois keys for different ciphertext contexts");
+        ek->keys.push_back(g);
+    }
+    ek->degree = ek->keys[0]->ksk->ct_ctx->N;
+    *out = new fhehip_evaluation_key{std::move(ek)};
+    FHEHIP_API_END
+}
+
+void fhehip_evaluation_key_destroy(fhehip_evaluation_key* ek) { delete ek; }
+
+int fhehip_rotate_columns(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint64_t* d_out, uint32_t step,
+                          uint32_t num, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!ek || ((!d_ct || !d_out) && num)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    if (d_ct == d_out && num) return fail(FHEHIP_ERR_INVALID_ARG, "out must not alias ct");
+    if (step == 0 || step >= ek->ek->degree / 2) return fail(FHEHIP_ERR_INVALID_ARG, "rotation step out of range");
+    ek->ek->rotate(ek->ek->column_exponent(step), d_ct, d_out, num, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_rotate_rows(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint64_t* d_out, uint32_t num, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!ek || ((!d_ct || !d_out) && num)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    if (d_ct == d_out && num) return fail(FHEHIP_ERR_INVALID_ARG, "out must not alias ct");
+    ek->ek->rotate(2 * ek->ek->degree - 1, d_ct, d_out, num, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_inner_sum(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint64_t* d_out, uint32_t num, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!ek || ((!d_ct || !d_out) && num)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    std::vector<u32> ex;
+    for (u32 i = 1; i < ek->ek->degree / 2; i *= 2) ex.push_back(ek->ek->column_exponent(i));
+    ex.push_back(2 * ek->ek->degree - 1);
+    ek->ek->rotate_and_sum(ex, d_ct, d_out, num, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_rotate_and_sum(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint64_t* d_out, uint32_t log2_slots,
+                          int rows, uint32_t num, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!ek || ((!d_ct || !d_out) && num)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    if ((1ull << log2_slots) > ek->ek->degree / 2) return fail(FHEHIP_ERR_INVALID_ARG, "log2_slots too large");
+    std::vector<u32> ex;
+    for (u32 k = 0; k < log2_slots; k++) ex.push_back(ek->ek->column_exponent(1u << k));
+    if (rows) ex.push_back(2 * ek->ek->degree - 1);
+    ek->ek->rotate_and_sum(ex, d_ct, d_out, num, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_ciphertext_add(const fhehip_context* ctx, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out,
+                          uint32_t num_polys, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!ctx || ((!d_a || !d_b || !d_out) && num_polys)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    const auto& c = *ctx->c;
+    launch_add_rows(d_a, d_b, d_out, c.mod.as<ModInfo>(), c.L, c.logn, (size_t)num_polys * c.L, S(stream));
     FHEHIP_API_END
 }
 

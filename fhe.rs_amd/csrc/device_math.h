@@ -85,6 +85,16 @@ __device__ __forceinline__ void add128(u64& lo, u64& hi, u64 alo, u64 ahi) {
     hi += ahi + (lo < alo ? 1 : 0);
 }
 
+// (lo, hi, top) += (alo, ahi): 192-bit accumulation of 128-bit terms.
+__device__ __forceinline__ void add192(u64& lo, u64& hi, u64& top, u64 alo, u64 ahi) {
+    lo += alo;
+    const u64 c0 = lo < alo ? 1 : 0;
+    const u64 h = hi + ahi;
+    const u64 c1 = h < hi ? 1 : 0;
+    hi = h + c0;
+    top += c1 + (hi < c0 ? 1 : 0);
+}
+
 // Montgomery reduction with R = 2^64 of t = hi * 2^64 + lo < p * 2^63:
 // returns t * R^-1 mod p in [0, 2p). pinv_neg = -p^-1 mod 2^64.
 __device__ __forceinline__ u64 mont_redc(u64 lo, u64 hi, u64 p, u64 pinv_neg) {

@@ -17,15 +17,19 @@
 
 namespace fhehip {
 
-template <int LOGN>
+// RE_WANT = log2 of the values each thread holds (4: 16 values, 1024 threads
+// for N = 16384; 5: 32 values, 512 threads).
+template <int LOGN, int RE_WANT = 4>
 struct NttShape {
     static constexpr int N = 1 << LOGN;
-    static constexpr int RE = LOGN >= 4 ? 4 : LOGN;  // log2 elements per thread
+    static constexpr int RE = LOGN >= RE_WANT ? RE_WANT : LOGN;  // log2 elements per thread
     static constexpr int E = 1 << RE;
     static constexpr int T = N / E;  // threads per row
     static constexpr int LDS_WORDS = N + (N >> 4) + 1;
 };
 
+// LDS word of element j: one pad word every 16 spreads the strided exchange
+// patterns over the banks.
 __device__ __forceinline__ int lds_slot(int j) { return j + (j >> 4); }
 
 // Forward butterfly (native.rs:256-269): x, y < 4p -> x', y' < 4p.
@@ -45,26 +49,26 @@ __device__ __forceinline__ void inv_bfly(u64& x, u64& y, u64 z, u64 zs, u64 p, u
 
 // Element index owned by thread t in slot k for a forward pass whose
 // (virtual) first stage is S0V: stage s pairs bit LOGN-1-s.
-template <int LOGN, int S0V>
+template <int LOGN, int S0V, int R = 4>
 __device__ __forceinline__ int fwd_index(int t, int k) {
-    constexpr int RE = NttShape<LOGN>::RE;
+    constexpr int RE = NttShape<LOGN, R>::RE;
     constexpr int LOB = LOGN - S0V - RE;
     return ((t >> LOB) << (LOGN - S0V)) | (k << LOB) | (t & ((1 << LOB) - 1));
 }
 
 // Element index for an inverse pass whose (virtual) first stage pairs bit B0V.
-template <int LOGN, int B0V>
+template <int LOGN, int B0V, int R = 4>
 __device__ __forceinline__ int inv_index(int t, int k) {
-    constexpr int RE = NttShape<LOGN>::RE;
+    constexpr int RE = NttShape<LOGN, R>::RE;
     return ((t >> B0V) << (B0V + RE)) | (k << B0V) | (t & ((1 << B0V) - 1));
 }
 
 // Forward stages S0V .. S0V+RE-1, skipping stages below SMIN.
 // tw: interleaved (omega, omega_shoup) pairs of the row's modulus.
-template <int LOGN, int S0V, int SMIN>
-__device__ __forceinline__ void fwd_pass(u64 (&x)[NttShape<LOGN>::E], int t, const ulonglong2* __restrict__ tw,
+template <int LOGN, int S0V, int SMIN, int R = 4>
+__device__ __forceinline__ void fwd_pass(u64 (&x)[(NttShape<LOGN, R>::E)], int t, const ulonglong2* __restrict__ tw,
                                          u64 p, u64 p2) {
-    constexpr int RE = NttShape<LOGN>::RE;
+    constexpr int RE = NttShape<LOGN, R>::RE;
     constexpr int LOB = LOGN - S0V - RE;
     // In the first pass every thread uses the same twiddles (hi == 0 because
     // t < T = 2^LOB); spelling it out lets them live in scalar registers.
@@ -83,15 +87,18 @@ __device__ __forceinline__ void fwd_pass(u64 (&x)[NttShape<LOGN>::E], int t, con
                 fwd_bfly(x[k0], x[k0 + half], w.x, w.y, p, p2);
             }
         }
+        // With 32 values per thread, keep the next stage's twiddle loads from
+        // being hoisted above this one (bounds the live registers).
+        if constexpr (RE >= 5) asm volatile("" ::: "memory");
     }
 }
 
 // Inverse stages pairing bits B0V .. B0V+RE-1, skipping bits below VMIN.
 // tw: interleaved (zeta_inv, zeta_inv_shoup) pairs of the row's modulus.
-template <int LOGN, int B0V, int VMIN>
-__device__ __forceinline__ void inv_pass(u64 (&x)[NttShape<LOGN>::E], int t, const ulonglong2* __restrict__ tw,
+template <int LOGN, int B0V, int VMIN, int R = 4>
+__device__ __forceinline__ void inv_pass(u64 (&x)[(NttShape<LOGN, R>::E)], int t, const ulonglong2* __restrict__ tw,
                                          u64 p, u64 p2) {
-    constexpr int RE = NttShape<LOGN>::RE;
+    constexpr int RE = NttShape<LOGN, R>::RE;
     constexpr int N = 1 << LOGN;
     const int hi = t >> B0V;
 #pragma unroll
@@ -109,98 +116,101 @@ __device__ __forceinline__ void inv_pass(u64 (&x)[NttShape<LOGN>::E], int t, con
                 inv_bfly(x[k0], x[k0 + half], z.x, z.y, p, p2);
             }
         }
+        if constexpr (RE >= 5) asm volatile("" ::: "memory");
     }
 }
 
-template <int LOGN, int S0V>
-__device__ __forceinline__ void lds_put_fwd(u64* lds, const u64 (&x)[NttShape<LOGN>::E], int t) {
+template <int LOGN, int S0V, int R = 4>
+__device__ __forceinline__ void lds_put_fwd(u64* lds, const u64 (&x)[(NttShape<LOGN, R>::E)], int t) {
 #pragma unroll
-    for (int k = 0; k < NttShape<LOGN>::E; k++) lds[lds_slot(fwd_index<LOGN, S0V>(t, k))] = x[k];
+    for (int k = 0; k < NttShape<LOGN, R>::E; k++) lds[lds_slot(fwd_index<LOGN, S0V, R>(t, k))] = x[k];
 }
-template <int LOGN, int S0V>
-__device__ __forceinline__ void lds_get_fwd(const u64* lds, u64 (&x)[NttShape<LOGN>::E], int t) {
+template <int LOGN, int S0V, int R = 4>
+__device__ __forceinline__ void lds_get_fwd(const u64* lds, u64 (&x)[(NttShape<LOGN, R>::E)], int t) {
 #pragma unroll
-    for (int k = 0; k < NttShape<LOGN>::E; k++) x[k] = lds[lds_slot(fwd_index<LOGN, S0V>(t, k))];
+    for (int k = 0; k < NttShape<LOGN, R>::E; k++) x[k] = lds[lds_slot(fwd_index<LOGN, S0V, R>(t, k))];
 }
-template <int LOGN, int B0V>
-__device__ __forceinline__ void lds_put_inv(u64* lds, const u64 (&x)[NttShape<LOGN>::E], int t) {
+template <int LOGN, int B0V, int R = 4>
+__device__ __forceinline__ void lds_put_inv(u64* lds, const u64 (&x)[(NttShape<LOGN, R>::E)], int t) {
 #pragma unroll
-    for (int k = 0; k < NttShape<LOGN>::E; k++) lds[lds_slot(inv_index<LOGN, B0V>(t, k))] = x[k];
+    for (int k = 0; k < NttShape<LOGN, R>::E; k++) lds[lds_slot(inv_index<LOGN, B0V, R>(t, k))] = x[k];
 }
-template <int LOGN, int B0V>
-__device__ __forceinline__ void lds_get_inv(const u64* lds, u64 (&x)[NttShape<LOGN>::E], int t) {
+template <int LOGN, int B0V, int R = 4>
+__device__ __forceinline__ void lds_get_inv(const u64* lds, u64 (&x)[(NttShape<LOGN, R>::E)], int t) {
 #pragma unroll
-    for (int k = 0; k < NttShape<LOGN>::E; k++) x[k] = lds[lds_slot(inv_index<LOGN, B0V>(t, k))];
+    for (int k = 0; k < NttShape<LOGN, R>::E; k++) x[k] = lds[lds_slot(inv_index<LOGN, B0V, R>(t, k))];
 }
 
 // The last pass of either direction always starts at LOGN - RE, so that its
 // element order is fwd_index<LOGN, LOGN-RE> (16 consecutive values per thread)
 // or inv_index<LOGN, LOGN-RE> (value k of thread t at k * T + t).
-template <int LOGN>
+template <int LOGN, int R = 4>
 constexpr int last_start() {
-    return LOGN - NttShape<LOGN>::RE;
+    return LOGN - NttShape<LOGN, R>::RE;
 }
 
-template <int LOGN, int S0V>
-__device__ __forceinline__ void fwd_rest(u64* lds, u64 (&x)[NttShape<LOGN>::E], int t, const ulonglong2* tw, u64 p,
-                                         u64 p2) {
-    constexpr int RE = NttShape<LOGN>::RE;
+template <int LOGN, int S0V, int R = 4>
+__device__ __forceinline__ void fwd_rest(u64* lds, u64 (&x)[(NttShape<LOGN, R>::E)], int t, const ulonglong2* tw,
+                                         u64 p, u64 p2) {
+    constexpr int RE = NttShape<LOGN, R>::RE;
     // Called after the pass starting at S0V has run; continues with the next one.
     if constexpr (S0V + RE < LOGN) {
         constexpr int NEXT = (S0V + 2 * RE <= LOGN) ? S0V + RE : LOGN - RE;
         constexpr int SMIN = S0V + RE;
-        lds_put_fwd<LOGN, S0V>(lds, x, t);
+        lds_put_fwd<LOGN, S0V, R>(lds, x, t);
         __syncthreads();
-        lds_get_fwd<LOGN, NEXT>(lds, x, t);
+        lds_get_fwd<LOGN, NEXT, R>(lds, x, t);
         __syncthreads();
-        fwd_pass<LOGN, NEXT, SMIN>(x, t, tw, p, p2);
-        fwd_rest<LOGN, NEXT>(lds, x, t, tw, p, p2);
+        fwd_pass<LOGN, NEXT, SMIN, R>(x, t, tw, p, p2);
+        fwd_rest<LOGN, NEXT, R>(lds, x, t, tw, p, p2);
     }
 }
 
-template <int LOGN, int B0V>
-__device__ __forceinline__ void inv_rest(u64* lds, u64 (&x)[NttShape<LOGN>::E], int t, const ulonglong2* tw, u64 p,
-                                         u64 p2) {
-    constexpr int RE = NttShape<LOGN>::RE;
+template <int LOGN, int B0V, int R = 4>
+__device__ __forceinline__ void inv_rest(u64* lds, u64 (&x)[(NttShape<LOGN, R>::E)], int t, const ulonglong2* tw,
+                                         u64 p, u64 p2) {
+    constexpr int RE = NttShape<LOGN, R>::RE;
     if constexpr (B0V + RE < LOGN) {
         constexpr int NEXT = (B0V + 2 * RE <= LOGN) ? B0V + RE : LOGN - RE;
         constexpr int VMIN = B0V + RE;
-        lds_put_inv<LOGN, B0V>(lds, x, t);
+        lds_put_inv<LOGN, B0V, R>(lds, x, t);
         __syncthreads();
-        lds_get_inv<LOGN, NEXT>(lds, x, t);
+        lds_get_inv<LOGN, NEXT, R>(lds, x, t);
         __syncthreads();
-        inv_pass<LOGN, NEXT, VMIN>(x, t, tw, p, p2);
-        inv_rest<LOGN, NEXT>(lds, x, t, tw, p, p2);
+        inv_pass<LOGN, NEXT, VMIN, R>(x, t, tw, p, p2);
+        inv_rest<LOGN, NEXT, R>(lds, x, t, tw, p, p2);
     }
 }
 
 // Full forward transform on registers: input order fwd_index<LOGN,0>, output
-// order fwd_index<LOGN, last_start<LOGN>()>, values < 4p.
-template <int LOGN>
-__device__ __forceinline__ void ntt_fwd_regs(u64* lds, u64 (&x)[NttShape<LOGN>::E], int t, const ulonglong2* tw,
+// order fwd_index<LOGN, last_start<LOGN>()> (E consecutive values per
+// thread), values < 4p.
+template <int LOGN, int R = 4>
+__device__ __forceinline__ void ntt_fwd_regs(u64* lds, u64 (&x)[(NttShape<LOGN, R>::E)], int t, const ulonglong2* tw,
                                              u64 p) {
     const u64 p2 = 2 * p;
-    fwd_pass<LOGN, 0, 0>(x, t, tw, p, p2);
-    fwd_rest<LOGN, 0>(lds, x, t, tw, p, p2);
+    fwd_pass<LOGN, 0, 0, R>(x, t, tw, p, p2);
+    fwd_rest<LOGN, 0, R>(lds, x, t, tw, p, p2);
 }
 
 // Full inverse transform on registers (without the N^-1 factor): input order
-// inv_index<LOGN,0>, output order inv_index<LOGN, last_start<LOGN>()>, < 2p.
-template <int LOGN>
-__device__ __forceinline__ void ntt_inv_regs(u64* lds, u64 (&x)[NttShape<LOGN>::E], int t, const ulonglong2* tw,
+// inv_index<LOGN,0> (E consecutive values per thread), output order
+// inv_index<LOGN, last_start<LOGN>()>, values < 2p.
+template <int LOGN, int R = 4>
+__device__ __forceinline__ void ntt_inv_regs(u64* lds, u64 (&x)[(NttShape<LOGN, R>::E)], int t, const ulonglong2* tw,
                                              u64 p) {
     const u64 p2 = 2 * p;
-    inv_pass<LOGN, 0, 0>(x, t, tw, p, p2);
-    inv_rest<LOGN, 0>(lds, x, t, tw, p, p2);
+    inv_pass<LOGN, 0, 0, R>(x, t, tw, p, p2);
+    inv_rest<LOGN, 0, R>(lds, x, t, tw, p, p2);
 }
 
 // Coalesced global row -> registers in the inverse-transform input order
 // (value e of thread t is element t*E + e): the row is read with consecutive
 // lanes on consecutive elements and transposed through LDS. Ends with a
 // barrier, so the LDS is free for the transform.
-template <int LOGN, int UNROLL = 4, class Load>
-__device__ __forceinline__ void load_inv_order(u64* lds, u64 (&x)[NttShape<LOGN>::E], int t, Load load) {
-    using S = NttShape<LOGN>;
+template <int LOGN, int UNROLL = 4, int R = 4, class Load>
+__device__ __forceinline__ void load_inv_order(u64* lds, u64 (&x)[(NttShape<LOGN, R>::E)], int t, Load load) {
+    using S = NttShape<LOGN, R>;
     // Partial unrolling bounds the number of loads in flight (and registers).
 #pragma unroll UNROLL
     for (int e = 0; e < S::E; e++) {
@@ -216,9 +226,9 @@ __device__ __forceinline__ void load_inv_order(u64* lds, u64 (&x)[NttShape<LOGN>
 // Registers in the forward-transform output order (value e of thread t is
 // element t*E + e) -> coalesced `store(j, value)`. Must follow a forward
 // transform (whose last LDS access is fenced by a barrier).
-template <int LOGN, class Store>
-__device__ __forceinline__ void store_fwd_order(u64* lds, const u64 (&x)[NttShape<LOGN>::E], int t, Store store) {
-    using S = NttShape<LOGN>;
+template <int LOGN, int R = 4, class Store>
+__device__ __forceinline__ void store_fwd_order(u64* lds, const u64 (&x)[(NttShape<LOGN, R>::E)], int t, Store store) {
+    using S = NttShape<LOGN, R>;
 #pragma unroll
     for (int e = 0; e < S::E; e++) lds[lds_slot((t << S::RE) + e)] = x[e];
     __syncthreads();

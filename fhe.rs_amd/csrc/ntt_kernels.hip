@@ -3,6 +3,8 @@
 #include "ntt_device.h"
 #include "profiler.h"
 
+#include <cstdlib>
+
 namespace fhehip {
 
 __device__ __forceinline__ u64* row_ptr(const RowMap& m, u32 r, int logn, u32& limb) {
@@ -13,10 +15,10 @@ __device__ __forceinline__ u64* row_ptr(const RowMap& m, u32 r, int logn, u32& l
 }
 
 // Forward NTT (NttOperator::forward / forward_vt_lazy, native.rs:77-189).
-template <int LOGN, bool LAZY>
-__global__ __launch_bounds__(NttShape<LOGN>::T) void ntt_fwd_kernel(RowMap m, const ulonglong2* __restrict__ tw_all,
-                                                                   const ModInfo* __restrict__ mods) {
-    using S = NttShape<LOGN>;
+template <int LOGN, bool LAZY, int R>
+__global__ __launch_bounds__((NttShape<LOGN, R>::T)) void ntt_fwd_kernel(RowMap m, const ulonglong2* __restrict__ tw_all,
+                                                                      const ModInfo* __restrict__ mods) {
+    using S = NttShape<LOGN, R>;
     __shared__ u64 lds[S::LDS_WORDS];
     const int t = threadIdx.x;
     u32 limb;
@@ -25,17 +27,17 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void ntt_fwd_kernel(RowMap m, co
     const u64 p = mods[limb].p;
     u64 x[S::E];
 #pragma unroll
-    for (int k = 0; k < S::E; k++) x[k] = row[fwd_index<LOGN, 0>(t, k)];
-    ntt_fwd_regs<LOGN>(lds, x, t, tw, p);
+    for (int k = 0; k < S::E; k++) x[k] = row[fwd_index<LOGN, 0, R>(t, k)];
+    ntt_fwd_regs<LOGN, R>(lds, x, t, tw, p);
     const u64 p2 = 2 * p;
-    store_fwd_order<LOGN>(lds, x, t, [&](int j, u64 v) { row[j] = LAZY ? v : reduce1(reduce1(v, p2), p); });
+    store_fwd_order<LOGN, R>(lds, x, t, [&](int j, u64 v) { row[j] = LAZY ? v : reduce1(reduce1(v, p2), p); });
 }
 
 // Inverse NTT (NttOperator::backward, native.rs:106-132).
-template <int LOGN>
-__global__ __launch_bounds__(NttShape<LOGN>::T) void ntt_inv_kernel(RowMap m, const ulonglong2* __restrict__ tw_all,
-                                                                   const ModInfo* __restrict__ mods) {
-    using S = NttShape<LOGN>;
+template <int LOGN, int R>
+__global__ __launch_bounds__((NttShape<LOGN, R>::T)) void ntt_inv_kernel(RowMap m, const ulonglong2* __restrict__ tw_all,
+                                                                      const ModInfo* __restrict__ mods) {
+    using S = NttShape<LOGN, R>;
     __shared__ u64 lds[S::LDS_WORDS];
     const int t = threadIdx.x;
     u32 limb;
@@ -43,28 +45,48 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void ntt_inv_kernel(RowMap m, co
     const ulonglong2* tw = tw_all + ((size_t)limb << LOGN);
     const ModInfo mi = mods[limb];
     u64 x[S::E];
-    load_inv_order<LOGN>(lds, x, t, [&](int j) { return row[j]; });
-    ntt_inv_regs<LOGN>(lds, x, t, tw, mi.p);
+    load_inv_order<LOGN, 4, R>(lds, x, t, [&](int j) { return row[j]; });
+    ntt_inv_regs<LOGN, R>(lds, x, t, tw, mi.p);
 #pragma unroll
     for (int k = 0; k < S::E; k++)
-        row[inv_index<LOGN, last_start<LOGN>()>(t, k)] = mul_shoup(x[k], mi.n_inv, mi.n_inv_shoup, mi.p);
+        row[inv_index<LOGN, last_start<LOGN, R>(), R>(t, k)] = mul_shoup(x[k], mi.n_inv, mi.n_inv_shoup, mi.p);
+}
+
+// Elements per thread of the plain NTT kernels: 2^4 by default; the
+// environment variable FHEHIP_NTT_RE=5 selects 2^5 (half the threads).
+static int ntt_re() {
+    static int re = [] {
+        const char* e = getenv("FHEHIP_NTT_RE");
+        return (e && atoi(e) == 5) ? 5 : 4;
+    }();
+    return re;
 }
 
 template <int LOGN>
 static void fwd_launch(const DeviceContext& c, RowMap m, u32 rows, bool lazy, hipStream_t s) {
-    dim3 grid(rows), block(NttShape<LOGN>::T);
-    if (lazy)
-        hipLaunchKernelGGL((ntt_fwd_kernel<LOGN, true>), grid, block, 0, s, m, c.fwd_tw.as<ulonglong2>(),
-                           c.mod.as<ModInfo>());
-    else
-        hipLaunchKernelGGL((ntt_fwd_kernel<LOGN, false>), grid, block, 0, s, m, c.fwd_tw.as<ulonglong2>(),
-                           c.mod.as<ModInfo>());
+    dim3 grid(rows);
+    const ulonglong2* tw = c.fwd_tw.as<ulonglong2>();
+    const ModInfo* md = c.mod.as<ModInfo>();
+    if (ntt_re() == 5) {
+        dim3 block(NttShape<LOGN, 5>::T);
+        if (lazy) hipLaunchKernelGGL((ntt_fwd_kernel<LOGN, true, 5>), grid, block, 0, s, m, tw, md);
+        else hipLaunchKernelGGL((ntt_fwd_kernel<LOGN, false, 5>), grid, block, 0, s, m, tw, md);
+    } else {
+        dim3 block(NttShape<LOGN, 4>::T);
+        if (lazy) hipLaunchKernelGGL((ntt_fwd_kernel<LOGN, true, 4>), grid, block, 0, s, m, tw, md);
+        else hipLaunchKernelGGL((ntt_fwd_kernel<LOGN, false, 4>), grid, block, 0, s, m, tw, md);
+    }
 }
 
 template <int LOGN>
 static void inv_launch(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s) {
-    dim3 grid(rows), block(NttShape<LOGN>::T);
-    hipLaunchKernelGGL((ntt_inv_kernel<LOGN>), grid, block, 0, s, m, c.inv_tw.as<ulonglong2>(), c.mod.as<ModInfo>());
+    dim3 grid(rows);
+    const ulonglong2* tw = c.inv_tw.as<ulonglong2>();
+    const ModInfo* md = c.mod.as<ModInfo>();
+    if (ntt_re() == 5)
+        hipLaunchKernelGGL((ntt_inv_kernel<LOGN, 5>), grid, dim3(NttShape<LOGN, 5>::T), 0, s, m, tw, md);
+    else
+        hipLaunchKernelGGL((ntt_inv_kernel<LOGN, 4>), grid, dim3(NttShape<LOGN, 4>::T), 0, s, m, tw, md);
 }
 
 #define FHEHIP_LOGN_SWITCH(logn, F, ...)                                    \

@@ -16,13 +16,13 @@ namespace fhehip {
 // Flat constant block (u64 words) written by the host (bfv.cpp):
 //   header[16]: nf, nt, start, count, shift, is_one, tg_lo, tg_hi, tg_sign
 //   theta_garner[nf][2], theta_omega[nf][3] (lo, hi, sign)
-//   per output i < count: p, barrett_lo, barrett_hi, gamma, gamma_shoup,
-//                         omega[nf], omega_shoup[nf]
+//   per output i < count: p, barrett_lo, barrett_hi, q - gamma,
+//                         [2^64 (q - gamma)]_q, [2^64]_q, [2^128]_q, omega[nf]
 enum : int {
     SC_NF = 0, SC_NT, SC_START, SC_COUNT, SC_SHIFT, SC_IS_ONE, SC_TG_LO, SC_TG_HI, SC_TG_SIGN,
     SC_HEADER = 16
 };
-__host__ __device__ inline int sc_out_stride(int nf) { return 5 + 2 * nf; }
+__host__ __device__ inline int sc_out_stride(int nf) { return 7 + nf; }
 __host__ __device__ inline int sc_out_base(int nf) { return SC_HEADER + 2 * nf + 3 * nf; }
 
 struct U256 {
@@ -153,31 +153,54 @@ __device__ __forceinline__ void rns_scale_column(const u64* __restrict__ sc, Res
 
     // Each output residue is the exact value of
     //   2 q - [v]_q gamma (+/- [w]_q) + sum_j r_j omega_ij   mod q,
-    // which the reference reduces term by term; here the products are summed
-    // exactly in 128 bits (folded every four terms) and reduced once.
+    // which the reference reduces term by term. Here every term is an exact
+    // product added into a 192-bit accumulator (v and w enter through their
+    // 64-bit halves: v = v1 2^64 + v0 contributes v0 (q - gamma) +
+    // v1 [2^64 (q - gamma)]_q), and the sum is reduced once.
     const u64* outc = sc + sc_out_base(nf);
     const int ostride = sc_out_stride(nf);
+    const u64 v0 = (u64)v, v1 = (u64)(v >> 64);
+    const u64 w0 = (u64)w, w1 = (u64)(w >> 64);
     for (int i = 0; i < count; i++) {
         const u64* o = outc + (size_t)i * ostride;
         const u64 q = o[0], bl = o[1], bh = o[2];
-        const u64 vq = reduce_u128((u64)v, (u64)(v >> 64), bl, bh, q);
-        u64 slo = 2 * q - lazy_mul_shoup(vq, o[3], o[4], q), shi = 0;
-        if (!is_one) {
-            const u64 wi = lazy_reduce_u128((u64)w, (u64)(w >> 64), bl, bh, q);
-            add128(slo, shi, w_sign ? (2 * q - wi) : wi, 0);
+        // o[3] = q - gamma, o[4] = [2^64 (q - gamma)]_q, o[5] = [2^64]_q, o[6] = [2^128]_q
+        u64 lo, hi, top = 0;
+        mul_wide(v0, o[3], lo, hi);
+        {
+            u64 l, h;
+            mul_wide(v1, o[4], l, h);
+            add192(lo, hi, top, l, h);
         }
-        const u64* om = o + 5;
-#pragma unroll
-        for (int j = 0; j < nf; j++) {
-            u64 lo, hi;
-            mul_wide(rests(j), om[j], lo, hi);
-            add128(slo, shi, lo, hi);
-            if ((j & 3) == 3) {
-                slo = lazy_reduce_u128(slo, shi, bl, bh, q);
-                shi = 0;
+        if (!is_one) {
+            // +w or -w = w (q - 1) mod q
+            u64 l, h;
+            if (w_sign) {
+                mul_wide(w0, q - 1, l, h);
+                add192(lo, hi, top, l, h);
+                mul_wide(w1, q - o[5], l, h);  // w1 2^64 (q - 1) = -w1 [2^64]_q
+                add192(lo, hi, top, l, h);
+            } else {
+                add192(lo, hi, top, w0, 0);
+                mul_wide(w1, o[5], l, h);
+                add192(lo, hi, top, l, h);
             }
         }
-        emit(i, reduce_u128(slo, shi, bl, bh, q));
+        const u64* om = o + 7;
+#pragma unroll
+        for (int j = 0; j < nf; j++) {
+            u64 l, h;
+            mul_wide(rests(j), om[j], l, h);
+            add192(lo, hi, top, l, h);
+        }
+        // lo + 2^64 hi + 2^128 top  ->  lo + hi [2^64]_q + top [2^128]_q  (< 2^127) -> Barrett
+        u64 l2, h2;
+        mul_wide(hi, o[5], l2, h2);
+        add128(l2, h2, lo, 0);
+        u64 l3, h3;
+        mul_wide(top, o[6], l3, h3);
+        add128(l2, h2, l3, h3);
+        emit(i, reduce_u128(l2, h2, bl, bh, q));
     }
 }
 

@@ -50,6 +50,13 @@ SIGNATURES = {
     "fhehip_galois_create": (C.c_int, [_vp, C.c_uint32, _vpp]),
     "fhehip_galois_destroy": (None, [_vp]),
     "fhehip_galois_relinearize": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_evaluation_key_create": (C.c_int, [_vpp, C.c_uint32, _vpp]),
+    "fhehip_evaluation_key_destroy": (None, [_vp]),
+    "fhehip_rotate_columns": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_uint32, _vp]),
+    "fhehip_rotate_rows": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_inner_sum": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_rotate_and_sum": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_int, C.c_uint32, _vp]),
+    "fhehip_ciphertext_add": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
     "fhehip_profiler_enable": (C.c_int, [C.c_int]),
     "fhehip_profiler_report": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
 }
@@ -267,6 +274,37 @@ class GaloisKey:
 
     def relinearize(self, d_ct, d_out, num, stream=None):
         check(lib().fhehip_galois_relinearize(self.h, dptr(d_ct), dptr(d_out), num, stream_ptr(stream)))
+
+
+class EvaluationKey:
+    """EvaluationKey over GaloisKey objects (shares ownership of them)."""
+
+    def __init__(self, galois_keys):
+        arr = (C.c_void_p * len(galois_keys))(*[g.h.value for g in galois_keys])
+        h = C.c_void_p()
+        check(lib().fhehip_evaluation_key_create(arr, len(galois_keys), C.byref(h)))
+        self.h = h
+        self.keys = list(galois_keys)
+
+    def __del__(self):
+        _destroy("fhehip_evaluation_key_destroy", getattr(self, "h", None))
+
+    def rotate_columns(self, d_ct, d_out, step, num, stream=None):
+        check(lib().fhehip_rotate_columns(self.h, dptr(d_ct), dptr(d_out), step, num, stream_ptr(stream)))
+
+    def rotate_rows(self, d_ct, d_out, num, stream=None):
+        check(lib().fhehip_rotate_rows(self.h, dptr(d_ct), dptr(d_out), num, stream_ptr(stream)))
+
+    def inner_sum(self, d_ct, d_out, num, stream=None):
+        check(lib().fhehip_inner_sum(self.h, dptr(d_ct), dptr(d_out), num, stream_ptr(stream)))
+
+    def rotate_and_sum(self, d_ct, d_out, log2_slots, rows, num, stream=None):
+        check(lib().fhehip_rotate_and_sum(self.h, dptr(d_ct), dptr(d_out), log2_slots, 1 if rows else 0, num,
+                                          stream_ptr(stream)))
+
+
+def ciphertext_add(ctx, d_a, d_b, d_out, num_polys, stream=None):
+    check(lib().fhehip_ciphertext_add(ctx.h, dptr(d_a), dptr(d_b), dptr(d_out), num_polys, stream_ptr(stream)))
 
 
 def profiler_enable(on=True):

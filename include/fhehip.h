@@ -40,6 +40,7 @@ typedef struct fhehip_scaler fhehip_scaler;
 typedef struct fhehip_ksk fhehip_ksk;
 typedef struct fhehip_multiplicator fhehip_multiplicator;
 typedef struct fhehip_galois fhehip_galois;
+typedef struct fhehip_evaluation_key fhehip_evaluation_key;
 
 /* Text of the last error raised on the calling thread. */
 const char* fhehip_last_error(void);
@@ -142,6 +143,31 @@ void fhehip_galois_destroy(fhehip_galois* g);
 /* GaloisKey::relinearize (galois_key.rs:63-124) on [num][2][L][N] NTT
  * ciphertexts; out must not alias ct. */
 int fhehip_galois_relinearize(fhehip_galois* g, const uint64_t* d_ct, uint64_t* d_out, uint32_t num, void* stream);
+
+/* ---- Evaluation keys: rotations and inner sums -------------------------------
+ * EvaluationKey (fhe/src/bfv/keys/evaluation_key.rs:19-131) over a set of
+ * Galois keys for one ciphertext context; it shares ownership of them. */
+int fhehip_evaluation_key_create(fhehip_galois* const* keys, uint32_t count, fhehip_evaluation_key** out);
+void fhehip_evaluation_key_destroy(fhehip_evaluation_key* ek);
+/* EvaluationKey::rotates_columns_by (evaluation_key.rs:117-131): Galois
+ * element 3^step mod 2N (:195-203); 0 < step < N/2. out must not alias ct. */
+int fhehip_rotate_columns(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint64_t* d_out, uint32_t step,
+                          uint32_t num, void* stream);
+/* EvaluationKey::rotates_rows (evaluation_key.rs:92-103): element 2N - 1. */
+int fhehip_rotate_rows(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint64_t* d_out, uint32_t num, void* stream);
+/* EvaluationKey::computes_inner_sum (evaluation_key.rs:56-82). */
+int fhehip_inner_sum(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint64_t* d_out, uint32_t num, void* stream);
+/* The first `log2_slots` steps of the inner sum (rotations by 1, 2, ..., with
+ * an add after each), plus the row rotation when rows != 0: the sum over
+ * 2^log2_slots consecutive slots of each row. No single fhe.rs call; it is
+ * the computes_inner_sum loop truncated. */
+int fhehip_rotate_and_sum(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint64_t* d_out, uint32_t log2_slots,
+                          int rows, uint32_t num, void* stream);
+/* Ciphertext / polynomial addition (fhe/src/bfv/ops/mod.rs:15-69, Poly
+ * AddAssign math/src/rq/ops.rs:92-118) of `num_polys` polynomials of the
+ * context: out = a + b (out may alias a or b). */
+int fhehip_ciphertext_add(const fhehip_context* ctx, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out,
+                          uint32_t num_polys, void* stream);
 
 /* ---- Measurement ------------------------------------------------------------
  * No counterpart in fhe.rs. When enabled, every kernel launch is bracketed by a
