@@ -80,6 +80,83 @@ def load_traffic(kernel):
         return None
 
 
+# Single-instance configuration: N = 8192, generate_moduli([62] * 3, 8192), t = 65537,
+# relinearization key at the ciphertext level.
+CFG1_N = 8192
+CFG1_MODULI = [4611686018427322369, 4611686018427289601, 4611686018426454017]
+# NTT microbenchmark: 256 polynomials x 8 primes x N = 16384.
+NTT_MODULI = [4611686018427322369, 4611686018427289601, 4611686018425815041, 4611686018424733697,
+              4611686018423881729, 4611686018423390209, 4611686018423062529, 4611686018422669313]
+
+
+def single_instance_latency(fhehip, torch, reps):
+    """One ct x ct multiply + relinearize per call at N = 8192, 3 x 62-bit."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(11)
+    ctx = fhehip.Context(CFG1_MODULI, CFG1_N)
+    L = len(CFG1_MODULI)
+    c0 = np.stack([np.stack([rng.integers(0, q, CFG1_N, dtype=np.uint64) for q in CFG1_MODULI]) for _ in range(L)])
+    c1 = np.stack([np.stack([rng.integers(0, q, CFG1_N, dtype=np.uint64) for q in CFG1_MODULI]) for _ in range(L)])
+    mult = fhehip.Multiplicator(rk=fhehip.KeySwitchingKey(ctx, ctx, c0, c1), plaintext=T_PLAIN)
+    per = 2 * L * CFG1_N
+    cts = np.concatenate([np.stack([rng.integers(0, q, (2, CFG1_N), dtype=np.uint64) for q in CFG1_MODULI], 1)
+                          .reshape(-1) for _ in range(2)])
+    d = fhehip.to_device(cts)
+    out = torch.empty(per, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream()
+    for _ in range(5):
+        mult.multiply(d[:per], d[per:], out, 1, stream)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+    ev[0].record(stream)
+    for k in range(reps):
+        mult.multiply(d[:per], d[per:], out, 1, stream)
+        ev[k + 1].record(stream)
+    torch.cuda.synchronize()
+    lat = [ev[k].elapsed_time(ev[k + 1]) for k in range(reps)]
+    return {"workload": "single instance: N=8192, 3x62-bit, t=65537, 1 ct*ct mul+relin per call",
+            "p50_ms": percentile(lat, 50), "p99_ms": percentile(lat, 99),
+            "value": reps / (ev[0].elapsed_time(ev[-1]) / 1e3), "unit": UNIT, "reps": reps}
+
+
+def ntt_microbench(fhehip, torch, reps):
+    """Forward then inverse NTT of 256 x 8 x 16384 residues (bit-exact round trip)."""
+    dev = torch.device("cuda:0")
+    n, count, L = 16384, 256, len(NTT_MODULI)
+    ctx = fhehip.Context(NTT_MODULI, n)
+    g = torch.Generator(device=dev)
+    g.manual_seed(3)
+    d = torch.empty(count * L * n, dtype=torch.int64, device=dev)
+    v = d.view(count, L, n)
+    for i, q in enumerate(NTT_MODULI):
+        v[:, i, :] = torch.randint(0, q, (count, n), dtype=torch.int64, device=dev, generator=g)
+    ref = d.clone()
+    stream = torch.cuda.current_stream()
+    ctx.ntt_forward(d, count, stream=stream)
+    ctx.ntt_backward(d, count, stream=stream)
+    torch.cuda.synchronize()
+    exact = bool(torch.equal(d, ref))
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record(stream)
+    for _ in range(reps):
+        ctx.ntt_forward(d, count, stream=stream)
+    ev[1].record(stream)
+    for _ in range(reps):
+        ctx.ntt_backward(d, count, stream=stream)
+    ev[2].record(stream)
+    torch.cuda.synchronize()
+    f_ms = ev[0].elapsed_time(ev[1]) / reps
+    b_ms = ev[1].elapsed_time(ev[2]) / reps
+    bytes_one = count * L * n * 16  # read + write once per transform
+    rt_ms = f_ms + b_ms
+    return {"workload": "NTT 256 polys x 8 x 62-bit primes x N=16384, forward then inverse",
+            "forward_us": f_ms * 1e3, "inverse_us": b_ms * 1e3, "roundtrip_us": rt_ms * 1e3,
+            "roundtrip_GBps": 2 * bytes_one / (rt_ms / 1e3) / 1e9,
+            "hbm_frac": 2 * bytes_one / (rt_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "roundtrip_bit_exact": exact}
+
+
 def cpu_baseline(lhs_h, rhs_h, c0, c1, gpu_out0, budget_s):
     """Oracle (C++ restatement of the fhe.rs CPU path) on the host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -117,6 +194,9 @@ def main():
     ap.add_argument("--per-call-ops", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the single-instance and NTT measurements")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="run one device copy of this many MiB (reference for rocprofv3 byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); run with --gpus 1", file=sys.stderr)
@@ -207,6 +287,17 @@ def main():
                 "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                 "kernel": dname, "avg_launch_ms": avg_s * 1e3, "algorithmic_bytes": abytes}
 
+    extras = {}
+    if not args.no_extras:
+        extras["single_instance"] = single_instance_latency(fhehip, torch, 200)
+        extras["ntt_microbench"] = ntt_microbench(fhehip, torch, 10)
+    if args.copy_calibration:
+        src = torch.empty(args.copy_calibration << 17, dtype=torch.int64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        extras["copy_calibration_bytes"] = args.copy_calibration << 20
+
     cpu = None
     bit_exact = None
     if not args.no_cpu:
@@ -240,6 +331,7 @@ def main():
         "kernels_ms": {k: {"launches": c, "avg_ms": ms / c} for k, (c, ms) in sorted(kernels.items())},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        **extras,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
         "host_wall_s": wall,
     }

@@ -51,6 +51,40 @@ std::vector<u64> scaler_blob(const ScalerConsts& c, const std::vector<ModInfo>& 
 }
 
 // ---------------------------------------------------------------------------
+// Rows of up to 16384 coefficients are transformed inside one workgroup's LDS
+// together with the elementwise work around the transform; longer rows
+// (N = 32768) run the unfused pipeline: elementwise kernels around the
+// two-pass NTT.
+static bool fused(const DeviceContext& c) { return c.logn <= 14; }
+
+// Inverse NTT of `parts` polynomials per instance into contiguous rows.
+static void intt_rows(const DeviceContext& c, const InttRowsArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s) {
+    if (fused(c)) {
+        launch_intt_rows(c.logn, a, L, parts, inst, s);
+        return;
+    }
+    const size_t rows = (size_t)L * c.N;
+    if (a.dst_inst_stride != parts * rows || (parts > 1 && a.dst_part_stride != rows))
+        throw Error(-1, "the unfused inverse NTT needs contiguous destination rows");
+    for (u32 k = 0; k < parts; k++)
+        launch_gather_rows(a.src[k], a.src_inst_stride, a.dst + k * a.dst_part_stride, a.dst_inst_stride, a.perm,
+                           c.logn, L, inst, s);
+    launch_ntt_backward(c, RowMap{a.dst, L, L, 0, a.table_offset}, inst * parts * L, s);
+}
+
+// Tensor product followed by the inverse NTT of the three output polynomials.
+static void tensor_intt(const DeviceContext& mul, const TensorArgs& a, u32 M, u32 inst, hipStream_t s) {
+    if (fused(mul)) {
+        launch_tensor_intt(mul.logn, a, M, inst, s);
+        return;
+    }
+    if (a.out_inst_stride != 3 * (size_t)M * mul.N || a.out_part_stride != (size_t)M * mul.N)
+        throw Error(-1, "the unfused tensor product needs contiguous output rows");
+    launch_tensor_ew(mul.logn, a, M, inst, s);
+    launch_ntt_backward(mul, RowMap{a.out, M, M, 0, 0}, inst * 3 * M, s);
+}
+
+// ---------------------------------------------------------------------------
 PolyScaler::PolyScaler(std::shared_ptr<DeviceContext> f, std::shared_ptr<DeviceContext> t, const BigUint& num,
                        const BigUint& den)
     : from(std::move(f)), to(std::move(t)) {
@@ -86,7 +120,7 @@ void PolyScaler::scale(const u64* in, u64* out, u32 polys, bool ntt_repr, hipStr
         a.dst_part_stride = 0;
         a.inv_tw = from->inv_tw.as<ulonglong2>();
         a.mods = from->mod.as<ModInfo>();
-        launch_intt_rows(from->logn, a, from->L, 1, polys, s);
+        intt_rows(*from, a, from->L, 1, polys, s);
         src = work.as<u64>();
     }
     launch_rns_scale(src, isz, 0, out + (size_t)common * n, osz, 0, d_consts.as<u64>(), from->L, (u32)n, 1, polys, s);
@@ -109,6 +143,8 @@ Ksk::Ksk(std::shared_ptr<DeviceContext> ct, std::shared_ptr<DeviceContext> key, 
     if (key_ctx->L == 1) throw Error(-2, "single-modulus key switching is not supported");
     extra = key_ctx->L - ct_ctx->L;
     if (extra > 2) throw Error(-2, "key context more than two levels above the ciphertext context");
+    if (extra > 1 && !fused(*key_ctx))
+        throw Error(-2, "at degree 32768 the key context may be at most one level above the ciphertext context");
     const u32 Lk = key_ctx->L, n = key_ctx->N;
     const size_t rows = (size_t)digits * Lk;
     std::vector<ulonglong2> h0(rows * n), h1(rows * n);
@@ -168,7 +204,18 @@ void Ksk::key_switch_pb(const u64* src, size_t src_inst_stride, u64* acc, size_t
     a.acc = acc;
     a.acc_inst_stride = acc_inst_stride;
     a.acc_part_stride = (size_t)key_ctx->L * key_ctx->N;
-    launch_keyswitch(key_ctx->logn, a, key_ctx->L, inst, s);
+    if (fused(*key_ctx)) {
+        launch_keyswitch(key_ctx->logn, a, key_ctx->L, inst, s);
+    } else {
+        // digit lift, lazy forward NTT of every lifted row, then the MAC
+        const u32 Lk = key_ctx->L;
+        if (acc_inst_stride != 2 * a.acc_part_stride) throw Error(-1, "accumulator layout");
+        lift_work.ensure((size_t)inst * digits * Lk * key_ctx->N * 8);
+        u64* y = lift_work.as<u64>();
+        launch_lift_ew(src, src_inst_stride, y, a.key_mods, digits, Lk, key_ctx->logn, inst, s);
+        launch_ntt_forward(*key_ctx, RowMap{y, Lk, Lk, 0, 0}, inst * digits * Lk, true, s);
+        launch_mac_ew(y, digits, Lk, a.key0, a.key1, a.key_mods, acc, key_ctx->logn, inst, s);
+    }
     if (to_power_basis) {
         // acc is [inst][2][L'][N] contiguous when acc_inst_stride == 2 L' N
         if (acc_inst_stride != 2 * a.acc_part_stride) throw Error(-1, "accumulator layout");
@@ -246,7 +293,16 @@ void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* acc, u6
     a.out_inst_stride = 2 * a.out_part_stride;
     a.fwd_tw = ct_ctx->fwd_tw.as<ulonglong2>();
     a.mods = ct_ctx->mod.as<ModInfo>();
-    launch_ks_finish(ct_ctx->logn, a, ct_ctx->L, 2, inst, s);
+    if (fused(*ct_ctx)) {
+        launch_ks_finish(ct_ctx->logn, a, ct_ctx->L, 2, inst, s);
+    } else {
+        const u32 L = ct_ctx->L;
+        sd_work.ensure((size_t)inst * 2 * L * n * 8);
+        u64* x = sd_work.as<u64>();
+        launch_sd_prepare(ct_ctx->logn, a, x, inst, s);
+        launch_ntt_forward(*ct_ctx, RowMap{x, L, L, 0, 0}, inst * 2 * L, false, s);
+        launch_sd_combine(ct_ctx->logn, a, x, inst, s);
+    }
 }
 
 // KeySwitchingKey::key_switch (key_switching_key.rs:214-241): power-basis p in
@@ -271,7 +327,7 @@ void Ksk::relinearize(const u64* ct3, u64* out, u32 inst, hipStream_t s) {
     a.dst_inst_stride = L * n;
     a.inv_tw = ct_ctx->inv_tw.as<ulonglong2>();
     a.mods = ct_ctx->mod.as<ModInfo>();
-    launch_intt_rows(ct_ctx->logn, a, (u32)L, 1, inst, s);
+    intt_rows(*ct_ctx, a, (u32)L, 1, inst, s);
     // (c0, c1) + switch_down(key switch), added in the NTT domain after the transform.
     switch_and_finish(c2, L * n, r, out, inst, nullptr, 0, 0, ct3, 3 * L * n, L * n, 2, nullptr, s);
 }
@@ -340,7 +396,7 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out, u32 inst,
     ia.dst_part_stride = L * n;
     ia.inv_tw = base->inv_tw.as<ulonglong2>();
     ia.mods = base->mod.as<ModInfo>();
-    launch_intt_rows(base->logn, ia, (u32)L, 4, inst, s);
+    intt_rows(*base, ia, (u32)L, 4, inst, s);
     // 2. base extension of the power-basis columns (rns/scaler.rs:249-352)
     if (cl < M)
         launch_rns_scale(wP, 4 * L * n, L * n, wX, 4 * xr * n, xr * n, ext_lhs->d_consts.as<u64>(), (u32)L, (u32)n, 2,
@@ -375,7 +431,7 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out, u32 inst,
     ta.out_part_stride = M * n;
     ta.inv_tw = mul->inv_tw.as<ulonglong2>();
     ta.mods = mul->mod.as<ModInfo>();
-    launch_tensor_intt(mul->logn, ta, (u32)M, inst, s);
+    tensor_intt(*mul, ta, (u32)M, inst, s);
     // 5. t/Q down-scaling of the three polynomials (mul.rs:187-190)
     u64* dst = rk ? wD : out;
     launch_rns_scale(wT, 3 * M * n, M * n, dst, 3 * L * n, L * n, down->d_consts.as<u64>(), (u32)M, (u32)n, 3, inst,
@@ -426,7 +482,7 @@ void Galois::apply(const u64* ct, u64* out, u32 inst, hipStream_t s) {
     a.inv_tw = ksk->ct_ctx->inv_tw.as<ulonglong2>();
     a.mods = ksk->ct_ctx->mod.as<ModInfo>();
     a.perm = d_perm.as<u32>();
-    launch_intt_rows(ksk->ct_ctx->logn, a, (u32)L, 1, inst, s);
+    intt_rows(*ksk->ct_ctx, a, (u32)L, 1, inst, s);
     ksk->switch_and_finish(c2, L * n, r, out, inst, nullptr, 0, 0, ct, 2 * L * n, 0, 1, d_perm.as<u32>(), s);
 }
 

@@ -38,6 +38,7 @@ struct Ksk {
     DevBuf key0, key1;  // [digits][L'][N] (value, shoup)
     DevBuf extra_mods, sd_consts;
     Workspace work;
+    mutable Workspace lift_work, sd_work;  // unfused pipeline (N = 32768) only
     Ksk(std::shared_ptr<DeviceContext> ct, std::shared_ptr<DeviceContext> key, const u64* c0, const u64* c1,
         u32 digits);
     // Digit lift + MAC (+ inverse NTT when to_power_basis) into acc [inst][2][L'][N].

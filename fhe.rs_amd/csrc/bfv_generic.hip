@@ -1,0 +1,187 @@
+// Elementwise kernels of the unfused BFV pipeline. The fused kernels keep a
+// whole row in one workgroup's LDS, which holds up to N = 16384; at N = 32768
+// the same operations run as these elementwise passes around the split NTT
+// (ntt_kernels.hip). One thread per coefficient; grid.y / grid.z index rows.
+#include "bfv_launch.h"
+#include "profiler.h"
+
+namespace fhehip {
+
+static inline dim3 ew_grid(u32 n, u32 y, u32 z) { return dim3((n + 255) / 256, y, z); }
+
+// dst[b][r][m] = src[b][r][perm[m]]  (Poly::substitute, NTT representation,
+// math/src/rq/mod.rs:324-346), or a plain copy without perm.
+// grid = (N/256, rows, instances)
+__global__ __launch_bounds__(256) void gather_rows_kernel(const u64* __restrict__ src, size_t src_inst,
+                                                          u64* __restrict__ dst, size_t dst_inst,
+                                                          const u32* __restrict__ perm, u32 logn) {
+    const u32 m = blockIdx.x * 256 + threadIdx.x;
+    if (m >> logn) return;
+    const size_t r = (size_t)blockIdx.y << logn;
+    dst[blockIdx.z * dst_inst + r + m] = src[blockIdx.z * src_inst + r + (perm ? perm[m] : m)];
+}
+
+void launch_gather_rows(const u64* src, size_t src_inst, u64* dst, size_t dst_inst, const u32* perm, int logn,
+                        u32 rows, u32 inst, hipStream_t s) {
+    if (!rows || !inst) return;
+    ProfScope ps("gather_rows", logn, s);
+    hipLaunchKernelGGL(gather_rows_kernel, ew_grid(1u << logn, rows, inst), dim3(256), 0, s, src, src_inst, dst,
+                       dst_inst, perm, (u32)logn);
+    HIPCHK(hipGetLastError());
+}
+
+// Exact NTT-domain tensor product (mul.rs:181-185) into [b][3][M][N].
+// grid = (N/256, M * 3, instances)
+__global__ __launch_bounds__(256) void tensor_ew_kernel(TensorArgs a, u32 M, u32 logn) {
+    const u32 e = blockIdx.x * 256 + threadIdx.x;
+    if (e >> logn) return;
+    const u32 m = blockIdx.y % M, c = blockIdx.y / M, b = blockIdx.z;
+    const ModInfo mi = a.mods[m];
+    const u64* rows[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const u32 common = k < 2 ? a.common_lhs : a.common_rhs;
+        if (m < common) {
+            const u64* base = k < 2 ? a.lhs : a.rhs;
+            rows[k] = base + (size_t)b * a.in_inst_stride + (size_t)(k & 1) * a.in_part_stride + ((size_t)m << logn);
+        } else {
+            rows[k] = a.ext + (size_t)b * a.ext_inst_stride + (size_t)k * a.ext_part_stride +
+                      ((size_t)(m - common) << logn);
+        }
+    }
+    u64 lo, hi;
+    if (c == 1) {
+        u64 l2, h2;
+        mul_wide(rows[0][e], rows[3][e], lo, hi);
+        mul_wide(rows[1][e], rows[2][e], l2, h2);
+        add128(lo, hi, l2, h2);
+    } else {
+        mul_wide(rows[c == 0 ? 0 : 1][e], rows[c == 0 ? 2 : 3][e], lo, hi);
+    }
+    a.out[(size_t)b * a.out_inst_stride + (size_t)c * a.out_part_stride + ((size_t)m << logn) + e] =
+        reduce_u128(lo, hi, mi.barrett_lo, mi.barrett_hi, mi.p);
+}
+
+void launch_tensor_ew(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_t s) {
+    if (!M || !inst) return;
+    ProfScope ps("tensor_ew", logn, s);
+    hipLaunchKernelGGL(tensor_ew_kernel, ew_grid(1u << logn, 3 * M, inst), dim3(256), 0, s, a, M, (u32)logn);
+    HIPCHK(hipGetLastError());
+}
+
+// Digit lift (rq/mod.rs:518-540) before the lazy NTT: Y[b][i][j] =
+// lazy_reduce_{q'_j}(src[b][i]). grid = (N/256, digits * L', instances)
+__global__ __launch_bounds__(256) void lift_ew_kernel(const u64* __restrict__ src, size_t src_inst,
+                                                      u64* __restrict__ y, const ModInfo* __restrict__ key_mods,
+                                                      u32 key_limbs, u32 logn) {
+    const u32 e = blockIdx.x * 256 + threadIdx.x;
+    if (e >> logn) return;
+    const u32 j = blockIdx.y % key_limbs, i = blockIdx.y / key_limbs, b = blockIdx.z;
+    const ModInfo mi = key_mods[j];
+    const u64 v = src[b * src_inst + ((size_t)i << logn) + e];
+    y[(((size_t)b * gridDim.y + blockIdx.y) << logn) + e] = lazy_reduce_u64(v, mi.barrett_lo, mi.barrett_hi, mi.p);
+}
+
+void launch_lift_ew(const u64* src, size_t src_inst, u64* y, const ModInfo* key_mods, u32 digits, u32 key_limbs,
+                    int logn, u32 inst, hipStream_t s) {
+    if (!digits || !inst) return;
+    ProfScope ps("lift_ew", logn, s);
+    hipLaunchKernelGGL(lift_ew_kernel, ew_grid(1u << logn, digits * key_limbs, inst), dim3(256), 0, s, src, src_inst,
+                       y, key_mods, key_limbs, (u32)logn);
+    HIPCHK(hipGetLastError());
+}
+
+// Key inner product (key_switching_key.rs:227-239): acc[b][p][j] =
+// sum_i Y[b][i][j] * key_p[i][j] (Shoup, exact). grid = (N/256, L' * 2, instances)
+__global__ __launch_bounds__(256) void mac_ew_kernel(const u64* __restrict__ y, u32 digits, u32 key_limbs,
+                                                     const ulonglong2* __restrict__ key0,
+                                                     const ulonglong2* __restrict__ key1,
+                                                     const ModInfo* __restrict__ key_mods, u64* __restrict__ acc,
+                                                     u32 logn) {
+    const u32 e = blockIdx.x * 256 + threadIdx.x;
+    if (e >> logn) return;
+    const u32 j = blockIdx.y % key_limbs, part = blockIdx.y / key_limbs, b = blockIdx.z;
+    const u64 p = key_mods[j].p;
+    const ulonglong2* key = part ? key1 : key0;
+    u64 r = 0;
+    for (u32 i = 0; i < digits; i++) {
+        const u64 v = y[((((size_t)b * digits + i) * key_limbs + j) << logn) + e];
+        const ulonglong2 w = key[(((size_t)i * key_limbs + j) << logn) + e];
+        r = add_mod(r, mul_shoup(v, w.x, w.y, p), p);
+    }
+    acc[((((size_t)b * 2 + part) * key_limbs + j) << logn) + e] = r;
+}
+
+void launch_mac_ew(const u64* y, u32 digits, u32 key_limbs, const ulonglong2* key0, const ulonglong2* key1,
+                   const ModInfo* key_mods, u64* acc, int logn, u32 inst, hipStream_t s) {
+    if (!key_limbs || !inst) return;
+    ProfScope ps("mac_ew", logn, s);
+    hipLaunchKernelGGL(mac_ew_kernel, ew_grid(1u << logn, 2 * key_limbs, inst), dim3(256), 0, s, y, digits,
+                       key_limbs, key0, key1, key_mods, acc, (u32)logn);
+    HIPCHK(hipGetLastError());
+}
+
+// Power-basis part of the switch-down finish (see ks_finish_kernel):
+// x[b][p][i] = add_pb - q_l^-1 [y]_i, y from the power-basis dropped limb.
+// grid = (N/256, L * 2, instances)
+__global__ __launch_bounds__(256) void sd_prepare_kernel(SdNttArgs a, u64* __restrict__ x, u32 logn) {
+    const u32 e = blockIdx.x * 256 + threadIdx.x;
+    if (e >> logn) return;
+    const u32 i = blockIdx.y % a.L, part = blockIdx.y / a.L, b = blockIdx.z;
+    const ModInfo mi = a.mods[i];
+    const u64 p = mi.p;
+    const u64* ks = a.ks + (size_t)b * a.ks_inst_stride + (size_t)part * a.ks_part_stride;
+    u64 z = 0;
+    if (a.extra) {
+        const SdConst cst = a.sd[i];
+        const u64 ql = a.extra_mods[0];
+        u64 yv = ks[((size_t)a.L << logn) + e] + (ql >> 1);
+        yv = yv >= ql ? yv - ql : yv;
+        const u64 tmp = lazy_reduce_u64(yv, mi.barrett_lo, mi.barrett_hi, p) + cst.half_mod;
+        z = mul_shoup(3 * p - tmp, cst.inv, cst.inv_shoup, p);
+    }
+    if (a.add_pb)
+        z = add_mod(z, a.add_pb[(size_t)b * a.add_inst_stride + (size_t)part * a.add_part_stride + ((size_t)i << logn) + e],
+                    p);
+    x[((((size_t)b * 2 + part) * a.L + i) << logn) + e] = z;
+}
+
+// out = NTT(x) + q_l^-1 ks_i (or ks_i without an extra modulus) + NTT-domain
+// addend. grid = (N/256, L * 2, instances)
+__global__ __launch_bounds__(256) void sd_combine_kernel(SdNttArgs a, const u64* __restrict__ xn, u32 logn) {
+    const u32 e = blockIdx.x * 256 + threadIdx.x;
+    if (e >> logn) return;
+    const u32 i = blockIdx.y % a.L, part = blockIdx.y / a.L, b = blockIdx.z;
+    const u64 p = a.mods[i].p;
+    const u64* ks = a.ks + (size_t)b * a.ks_inst_stride + (size_t)part * a.ks_part_stride;
+    const u64 kv = ks[((size_t)i << logn) + e];
+    u64 r = xn[((((size_t)b * 2 + part) * a.L + i) << logn) + e];
+    if (a.extra) {
+        const SdConst cst = a.sd[i];
+        r = add_mod(r, mul_shoup(kv, cst.inv, cst.inv_shoup, p), p);
+    } else {
+        r = add_mod(r, kv, p);
+    }
+    if (a.add_ntt && part < a.add_ntt_parts) {
+        const u64* g = a.add_ntt + (size_t)b * a.add_ntt_inst_stride + (size_t)part * a.add_ntt_part_stride +
+                       ((size_t)i << logn);
+        r = add_mod(r, g[a.perm ? a.perm[e] : e], p);
+    }
+    a.out[(size_t)b * a.out_inst_stride + (size_t)part * a.out_part_stride + ((size_t)i << logn) + e] = r;
+}
+
+void launch_sd_prepare(int logn, const SdNttArgs& a, u64* x, u32 inst, hipStream_t s) {
+    if (!inst) return;
+    ProfScope ps("sd_prepare", logn, s);
+    hipLaunchKernelGGL(sd_prepare_kernel, ew_grid(1u << logn, 2 * a.L, inst), dim3(256), 0, s, a, x, (u32)logn);
+    HIPCHK(hipGetLastError());
+}
+
+void launch_sd_combine(int logn, const SdNttArgs& a, const u64* xn, u32 inst, hipStream_t s) {
+    if (!inst) return;
+    ProfScope ps("sd_combine", logn, s);
+    hipLaunchKernelGGL(sd_combine_kernel, ew_grid(1u << logn, 2 * a.L, inst), dim3(256), 0, s, a, xn, (u32)logn);
+    HIPCHK(hipGetLastError());
+}
+
+}  // namespace fhehip

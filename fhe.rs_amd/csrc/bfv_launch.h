@@ -78,4 +78,16 @@ void launch_ks_finish(int logn, const SdNttArgs& a, u32 L, u32 parts, u32 inst, 
 void launch_add_rows(const u64* a, const u64* b, u64* out, const ModInfo* mods, u32 L, int logn, size_t rows,
                      hipStream_t s);
 
+// Elementwise kernels of the unfused pipeline (bfv_generic.hip), used where a
+// row does not fit one workgroup's LDS (N = 32768).
+void launch_gather_rows(const u64* src, size_t src_inst, u64* dst, size_t dst_inst, const u32* perm, int logn,
+                        u32 rows, u32 inst, hipStream_t s);
+void launch_tensor_ew(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_t s);
+void launch_lift_ew(const u64* src, size_t src_inst, u64* y, const ModInfo* key_mods, u32 digits, u32 key_limbs,
+                    int logn, u32 inst, hipStream_t s);
+void launch_mac_ew(const u64* y, u32 digits, u32 key_limbs, const ulonglong2* key0, const ulonglong2* key1,
+                   const ModInfo* key_mods, u64* acc, int logn, u32 inst, hipStream_t s);
+void launch_sd_prepare(int logn, const SdNttArgs& a, u64* x, u32 inst, hipStream_t s);
+void launch_sd_combine(int logn, const SdNttArgs& a, const u64* xn, u32 inst, hipStream_t s);
+
 }  // namespace fhehip

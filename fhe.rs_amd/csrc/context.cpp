@@ -32,7 +32,7 @@ void DevBuf::reset(size_t n) {
 std::shared_ptr<DeviceContext> make_context(const std::vector<u64>& moduli, u32 n, const u64* psi) {
     if (moduli.empty()) throw Error(-1, "empty moduli list");
     if (n < 8 || (n & (n - 1))) throw Error(-1, "The degree is not a power of two larger or equal to 8");
-    if (n > 16384) throw Error(-2, "degree above 16384 is not supported by the LDS-resident NTT kernels");
+    if (n > 32768) throw Error(-2, "degree above 32768 is not supported");
     auto c = std::make_shared<DeviceContext>();
     c->moduli = moduli;
     c->L = (u32)moduli.size();
@@ -82,6 +82,57 @@ std::shared_ptr<DeviceContext> make_context(const std::vector<u64>& moduli, u32 
     for (u32 l = 0; l + 1 < c->L; l++) {
         u64 v = invmod64(moduli.back() % moduli[l], moduli[l]);
         il.push_back(make_ulonglong2(v, shoup64(v, moduli[l])));
+    }
+    if (n == 32768) {
+        // Half tables (see DeviceContext::half). Forward: local index k = 2^s + g
+        // of a 16384-point transform at stage s maps to the global index
+        // 2^(s+1) + h 2^s + g. Inverse: local (N' - (N' >> v)) + g maps to
+        // (N - (N >> v)) + h 2^(13 - v) + g.
+        const u32 nh = n / 2;
+        auto hc = std::make_shared<DeviceContext>();
+        hc->L = 2 * c->L;
+        hc->N = nh;
+        hc->logn = 14;
+        std::vector<ulonglong2> hf((size_t)hc->L * nh), hi((size_t)hc->L * nh);
+        std::vector<ulonglong2> sp;
+        for (u32 l = 0; l < c->L; l++) {
+            const u64 p = moduli[l];
+            for (u32 h = 0; h < 2; h++) {
+                ulonglong2* F = hf.data() + ((size_t)2 * l + h) * nh;
+                ulonglong2* I = hi.data() + ((size_t)2 * l + h) * nh;
+                F[0] = fwd[(size_t)l * n + 1];
+                for (u32 k = 1; k < nh; k++) {
+                    const u32 sft = 31 - __builtin_clz(k);
+                    F[k] = fwd[(size_t)l * n + k + ((1u + h) << sft)];
+                }
+                for (u32 v = 0; v < 14; v++) {
+                    const u32 lo = nh - (nh >> v), go = n - (n >> v);
+                    for (u32 g = 0; g < (1u << (13 - v)); g++) I[lo + g] = inv[(size_t)l * n + go + (h << (13 - v)) + g];
+                }
+                I[nh - 1] = inv[(size_t)l * n + n - 1];
+                ModInfo mi = c->h_mod[l];
+                mi.n_inv = 1;
+                mi.n_inv_shoup = shoup64(1, p);
+                mi.n_inv_r = (u64)(((u128)1 << 64) % p);
+                mi.n_inv_r_shoup = shoup64(mi.n_inv_r, p);
+                hc->h_mod.push_back(mi);
+                hc->moduli.push_back(p);
+            }
+            const u64 ninv = c->h_mod[l].n_inv;
+            const u64 zl = mulmod64(inv[(size_t)l * n + n - 2].x, ninv, p);
+            sp.push_back(fwd[(size_t)l * n + 1]);
+            sp.push_back(make_ulonglong2(zl, shoup64(zl, p)));
+            sp.push_back(make_ulonglong2(ninv, shoup64(ninv, p)));
+        }
+        hc->fwd_tw.reset(hf.size() * sizeof(ulonglong2));
+        hc->inv_tw.reset(hi.size() * sizeof(ulonglong2));
+        hc->mod.reset(hc->h_mod.size() * sizeof(ModInfo));
+        HIPCHK(hipMemcpy(hc->fwd_tw.ptr, hf.data(), hc->fwd_tw.bytes, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(hc->inv_tw.ptr, hi.data(), hc->inv_tw.bytes, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(hc->mod.ptr, hc->h_mod.data(), hc->mod.bytes, hipMemcpyHostToDevice));
+        c->split.reset(sp.size() * sizeof(ulonglong2));
+        HIPCHK(hipMemcpy(c->split.ptr, sp.data(), c->split.bytes, hipMemcpyHostToDevice));
+        c->half = hc;
     }
     c->fwd_tw.reset(fwd.size() * sizeof(ulonglong2));
     c->inv_tw.reset(inv.size() * sizeof(ulonglong2));

@@ -50,6 +50,13 @@ struct DeviceContext {
     DevBuf mod;      // [L] ModInfo
     DevBuf inv_last; // [L-1] ulonglong2 (q_last^-1 mod q_i, shoup)
     RnsBasis rns;    // host-side CRT constants
+    // N = 32768 only: the forward transform's first stage and the inverse's
+    // last stage pair the two halves of a row; each half is then an
+    // independent 16384-point transform. `half` is a context of 2L rows of
+    // degree 16384 whose tables are the full tables re-indexed for half h of
+    // limb l at row 2l + h (its inverse leaves out the N^-1 factor).
+    std::shared_ptr<DeviceContext> half;
+    DevBuf split;    // [L] ulonglong2 pairs: (omega_1, shoup), (zeta_last * N^-1, shoup), (N^-1, shoup)
 
     const ulonglong2* fwd(u32 limb) const { return fwd_tw.as<ulonglong2>() + (size_t)limb * N; }
     const ulonglong2* inv(u32 limb) const { return inv_tw.as<ulonglong2>() + (size_t)limb * N; }

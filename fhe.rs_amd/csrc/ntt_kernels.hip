@@ -106,17 +106,65 @@ static void inv_launch(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s
         default: throw Error(-2, "degree not supported by the NTT kernels"); \
     }
 
+// N = 32768: first forward stage (pairs j, j + N/2 with omega_1; Harvey
+// butterfly of native.rs:256-269, outputs < 4p). grid = (N / 512, rows)
+__global__ __launch_bounds__(256) void ntt_split_fwd_kernel(RowMap m, const ulonglong2* __restrict__ split,
+                                                            const ModInfo* __restrict__ mods) {
+    u32 limb;
+    u64* row = row_ptr(m, blockIdx.y, 15, limb);
+    const u32 j = blockIdx.x * 256 + threadIdx.x;
+    const ulonglong2 w = split[3 * limb];
+    const u64 p = mods[limb].p;
+    u64 x = row[j], y = row[j + 16384];
+    fwd_bfly(x, y, w.x, w.y, p, 2 * p);
+    row[j] = x;
+    row[j + 16384] = y;
+}
+
+// N = 32768: last inverse stage on the two reduced half transforms, with the
+// N^-1 factor folded in: X = (x + y) N^-1, Y = (x - y) zeta N^-1 (native.rs
+// :288-300 and :130-131).
+__global__ __launch_bounds__(256) void ntt_split_inv_kernel(RowMap m, const ulonglong2* __restrict__ split,
+                                                            const ModInfo* __restrict__ mods) {
+    u32 limb;
+    u64* row = row_ptr(m, blockIdx.y, 15, limb);
+    const u32 j = blockIdx.x * 256 + threadIdx.x;
+    const ulonglong2 z = split[3 * limb + 1], ni = split[3 * limb + 2];
+    const u64 p = mods[limb].p;
+    const u64 x = row[j], y = row[j + 16384];
+    row[j] = mul_shoup(x + y, ni.x, ni.y, p);
+    row[j + 16384] = mul_shoup(x + 2 * p - y, z.x, z.y, p);
+}
+
+static RowMap half_rows(RowMap m) {
+    return RowMap{m.data, 2 * m.rows_per_poly, 2 * m.limbs_per_poly, 2 * m.first_limb, 2 * m.table_offset};
+}
+
 void launch_ntt_forward(const DeviceContext& c, RowMap m, u32 rows, bool lazy, hipStream_t s) {
     if (rows == 0) return;
     ProfScope ps(lazy ? "ntt_forward_lazy" : "ntt_forward", c.logn, s);
-    FHEHIP_LOGN_SWITCH(c.logn, fwd_launch, c, m, rows, lazy, s);
+    if (c.logn == 15) {
+        hipLaunchKernelGGL(ntt_split_fwd_kernel, dim3(16384 / 256, rows), dim3(256), 0, s, m,
+                           c.split.as<ulonglong2>(), c.mod.as<ModInfo>());
+        HIPCHK(hipGetLastError());
+        fwd_launch<14>(*c.half, half_rows(m), 2 * rows, lazy, s);
+    } else {
+        FHEHIP_LOGN_SWITCH(c.logn, fwd_launch, c, m, rows, lazy, s);
+    }
     HIPCHK(hipGetLastError());
 }
 
 void launch_ntt_backward(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s) {
     if (rows == 0) return;
     ProfScope ps("ntt_backward", c.logn, s);
-    FHEHIP_LOGN_SWITCH(c.logn, inv_launch, c, m, rows, s);
+    if (c.logn == 15) {
+        inv_launch<14>(*c.half, half_rows(m), 2 * rows, s);
+        HIPCHK(hipGetLastError());
+        hipLaunchKernelGGL(ntt_split_inv_kernel, dim3(16384 / 256, rows), dim3(256), 0, s, m,
+                           c.split.as<ulonglong2>(), c.mod.as<ModInfo>());
+    } else {
+        FHEHIP_LOGN_SWITCH(c.logn, inv_launch, c, m, rows, s);
+    }
     HIPCHK(hipGetLastError());
 }
 

@@ -24,7 +24,7 @@ def _sync():
 
 
 # ---- scaler -----------------------------------------------------------------
-@pytest.mark.parametrize("n", [16, 4096])
+@pytest.mark.parametrize("n", [16, 4096, 32768])
 def test_scaler_extension_and_down(fh, oracle, n):
     q = oracle.generate_moduli([60] * 6, n)
     ext = q + [m for m in oracle.generate_moduli([62] * 8, n) if m not in q][:7]
@@ -82,7 +82,8 @@ def _gpu_key(fh, bfv, key, ct_level, key_level):
 
 
 @pytest.mark.parametrize("n,L,ct_level,key_level", [(16, 2, 0, 0), (64, 3, 1, 0), (1024, 4, 2, 0),
-                                                    (8192, 3, 0, 0), (16384, 7, 1, 0)])
+                                                    (8192, 3, 0, 0), (16384, 7, 1, 0), (32768, 4, 1, 0),
+                                                    (32768, 3, 0, 0)])
 def test_key_switch_matches_oracle(fh, oracle, n, L, ct_level, key_level):
     print(f"key switch n={n} L={L} levels={ct_level},{key_level}", flush=True)
     bfv = _bfv(oracle, n, [60] * L, seed=L)
@@ -101,7 +102,7 @@ def test_key_switch_matches_oracle(fh, oracle, n, L, ct_level, key_level):
 
 
 @pytest.mark.parametrize("n,L,ct_level,key_level", [(16, 3, 0, 0), (32, 3, 1, 0), (64, 4, 2, 0),
-                                                    (8192, 3, 0, 0)])
+                                                    (8192, 3, 0, 0), (32768, 3, 1, 0)])
 def test_relinearize_matches_oracle(fh, oracle, n, L, ct_level, key_level):
     bfv = _bfv(oracle, n, [62] * L, seed=L + 1)
     rng = random.Random(n)
@@ -129,6 +130,8 @@ MUL_CASES = [
     (1024, [60] * 4, 2, 0, 2),
     (8192, [62] * 3, 0, 0, 2),        # single-instance configuration shape
     (16384, [60] * 7, 1, 0, 2),       # batched configuration shape: 6 x 60 + special prime
+    (32768, [60] * 4, 1, 0, 2),       # unfused pipeline, one special prime
+    (32768, [55, 60, 62], 0, 0, 1),   # unfused pipeline, no special prime
 ]
 
 
@@ -150,7 +153,7 @@ def test_multiply_matches_oracle(fh, oracle, n, sizes, ct_level, key_level, batc
     assert np.array_equal(fh.to_host(out), om.multiply(lhs, rhs))
 
 
-@pytest.mark.parametrize("n,L", [(16, 2), (8192, 3)])
+@pytest.mark.parametrize("n,L", [(16, 2), (8192, 3), (32768, 3)])
 def test_multiply_decrypts(fh, oracle, n, L):
     t = 65537
     bfv = _bfv(oracle, n, [62] * L, seed=3)
@@ -221,7 +224,7 @@ def _rotate_columns(v, k):
 
 
 @pytest.mark.parametrize("n,L,ct_level,key_level", [(16, 3, 1, 0), (64, 3, 0, 0), (4096, 4, 2, 0),
-                                                    (16384, 7, 1, 0)])
+                                                    (16384, 7, 1, 0), (32768, 4, 1, 0)])
 def test_galois_matches_oracle(fh, oracle, n, L, ct_level, key_level):
     print(f"galois n={n} L={L}", flush=True)
     t = 65537
@@ -240,3 +243,14 @@ def test_galois_matches_oracle(fh, oracle, n, L, ct_level, key_level):
         got = fh.to_host(out)
         assert np.array_equal(got, bfv.galois(gk, ct))
         assert [int(x) for x in bfv.decrypt(got, level=ct_level)] == _rotate_columns(v, k)
+
+
+def test_degree_32768_rejects_two_extra_key_levels(fh, oracle):
+    """The unfused pipeline supports a key context at most one level above the
+    ciphertext context; deeper keys are refused with UNSUPPORTED."""
+    n = 32768
+    bfv = _bfv(oracle, n, [60] * 4, seed=1)
+    rk = bfv.relin_key(ct_level=2, key_level=0, seed=2)
+    with pytest.raises(fh.FheHipError) as e:
+        _gpu_key(fh, bfv, rk, 2, 0)
+    assert e.value.code == -2

@@ -17,7 +17,8 @@ def fh():
     return fhehip
 
 
-@pytest.mark.parametrize("n,L", [(8, 1), (16, 2), (32, 3), (64, 2), (1024, 3), (4096, 2), (8192, 3), (16384, 2)])
+@pytest.mark.parametrize("n,L", [(8, 1), (16, 2), (32, 3), (64, 2), (1024, 3), (4096, 2), (8192, 3), (16384, 2),
+                                 (32768, 1), (32768, 3)])
 def test_ntt_matches_oracle(fh, oracle, n, L):
     moduli = oracle.generate_moduli([62] * L, n)
     print(f"ntt parity n={n} L={L}", flush=True)
@@ -88,3 +89,23 @@ def test_ntt_unsupported_modulus(fh):
     with pytest.raises(fh.FheHipError) as e:
         fh.Context([4611686018427387847], 1024)  # prime but not 1 mod 2048
     assert e.value.code == fh.ERR_UNSUPPORTED
+
+
+def test_ntt_32768_split_rows(fh, oracle):
+    """N=32768 runs its first forward stage (last inverse stage) as a separate
+    pass over row halves; mixed 50/62-bit primes and edge rows pin the pairing
+    of halves to limbs."""
+    n = 32768
+    moduli = oracle.generate_moduli([62, 50, 60, 55], n)
+    ctx = fh.Context(moduli, n)
+    octx = oracle.Context(moduli, n)
+    a = oracle.random_polys(moduli, n, 2, seed=99).reshape(2, 4, n)
+    a[1, 0] = np.uint64(moduli[0] - 1)
+    a[1, 1] = 0
+    a[1, 1, n - 1] = 1
+    a = a.reshape(-1)
+    d = fh.to_device(a)
+    ctx.ntt_forward(d, 2)
+    assert np.array_equal(fh.to_host(d), octx.forward(a))
+    ctx.ntt_backward(d, 2)
+    assert np.array_equal(fh.to_host(d), a)
