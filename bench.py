@@ -53,31 +53,37 @@ def kernel_bytes(name, B, L, Lk, M):
     kind = name.split("/")[0]
     if kind == "intt_rows":      # 4 input polys x L rows, read + write
         return B * 4 * L * row * 2
-    if kind == "rns_scale":      # extension (2 polys L -> M-L) or down-scale (3 polys M -> L); average case
-        return None
     if kind == "ntt_forward":    # the new extension rows, in place
         return B * 4 * (M - L) * row * 2
     if kind == "tensor_intt":    # read a0,a1,b0,b1 over M limbs, write 3 x M rows
         return B * (4 + 3) * M * row
-    if kind == "keyswitch":      # read c2 (L rows), write (c0, c1) over Lk rows; key rows once
-        return (B * (L + 2 * Lk) + 2 * L * Lk * 2) * row
-    if kind == "ntt_backward":   # the key-switch accumulators, in place
-        return B * 2 * Lk * row * 2
-    if kind == "sd_add_ntt":     # read ks rows (L + extra per part) + D rows, write 2 x L rows
-        return B * 2 * (Lk + L + L) * row
+    if kind == "rns_extend":     # 2 polys: read L rows, write M - L rows
+        return B * 2 * M * row
+    if kind == "rns_down":       # 3 polys: read M rows, write L rows
+        return B * 3 * (M + L) * row
+    if kind == "lift_ntt":       # read c2 (L rows), write the lifted, transformed digits (L x Lk rows)
+        return B * (L + L * Lk) * row
+    if kind == "ks_special":     # read the digits of the special limb, write 2 power-basis rows; key rows once
+        return B * (L + 2) * row + 2 * L * 2 * row
+    if kind == "ks_finish":      # read digits of limbs < L, 2 special rows, 2L c0/c1 rows; write 2L rows; keys once
+        return B * (L * L + 2 + 4 * L) * row + 2 * L * L * 2 * row
     return None
 
 
 def load_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
-    summary, when present."""
-    path = os.path.join(ROOT, "profiles", "traffic_mul_relin_n16384_b512.json")
+    """HBM bytes per launch of `kernel` (its name without the degree suffix)
+    from the committed rocprofv3 counter summary, when present: the summary
+    holds a table whose rows read `<kernel> <fetch KiB> <write KiB> <bytes>`."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_mul_relin_n16384_b512.md")
     try:
         with open(path) as f:
-            d = json.load(f)
-        return d.get(kernel)
-    except (OSError, ValueError):
-        return None
+            for line in f:
+                parts = line.split()
+                if len(parts) == 4 and parts[0] == kernel and parts[3].isdigit():
+                    return int(parts[3])
+    except OSError:
+        pass
+    return None
 
 
 # Single-instance configuration: N = 8192, generate_moduli([62] * 3, 8192), t = 65537,

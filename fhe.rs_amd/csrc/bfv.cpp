@@ -190,8 +190,7 @@ Ksk::Ksk(std::shared_ptr<DeviceContext> ct, std::shared_ptr<DeviceContext> key, 
     HIPCHK(hipMemcpy(sd_consts.ptr, sd.data(), sd_consts.bytes, hipMemcpyHostToDevice));
 }
 
-void Ksk::key_switch_pb(const u64* src, size_t src_inst_stride, u64* acc, size_t acc_inst_stride, u32 inst,
-                        bool to_power_basis, hipStream_t s) const {
+KeySwitchArgs Ksk::args(const u64* src, size_t src_inst_stride) const {
     KeySwitchArgs a{};
     a.src = src;
     a.src_inst_stride = src_inst_stride;
@@ -201,35 +200,38 @@ void Ksk::key_switch_pb(const u64* src, size_t src_inst_stride, u64* acc, size_t
     a.key1 = key1.as<ulonglong2>();
     a.key_fwd_tw = key_ctx->fwd_tw.as<ulonglong2>();
     a.key_mods = key_ctx->mod.as<ModInfo>();
-    a.acc = acc;
-    a.acc_inst_stride = acc_inst_stride;
-    a.acc_part_stride = (size_t)key_ctx->L * key_ctx->N;
-    if (fused(*key_ctx)) {
-        launch_keyswitch(key_ctx->logn, a, key_ctx->L, inst, s);
-    } else {
-        // digit lift, lazy forward NTT of every lifted row, then the MAC
-        const u32 Lk = key_ctx->L;
-        if (acc_inst_stride != 2 * a.acc_part_stride) throw Error(-1, "accumulator layout");
-        lift_work.ensure((size_t)inst * digits * Lk * key_ctx->N * 8);
-        u64* y = lift_work.as<u64>();
-        launch_lift_ew(src, src_inst_stride, y, a.key_mods, digits, Lk, key_ctx->logn, inst, s);
-        launch_ntt_forward(*key_ctx, RowMap{y, Lk, Lk, 0, 0}, inst * digits * Lk, true, s);
-        launch_mac_ew(y, digits, Lk, a.key0, a.key1, a.key_mods, acc, key_ctx->logn, inst, s);
-    }
-    if (to_power_basis) {
-        // acc is [inst][2][L'][N] contiguous when acc_inst_stride == 2 L' N
-        if (acc_inst_stride != 2 * a.acc_part_stride) throw Error(-1, "accumulator layout");
-        RowMap m{acc, key_ctx->L, key_ctx->L, 0, 0};
-        launch_ntt_backward(*key_ctx, m, inst * 2 * key_ctx->L, s);
-    }
+    return a;
 }
 
-void Ksk::finish(const u64* ks, u64* out, u32 inst, const u64* add_pb, size_t add_inst, size_t add_part,
-                 const u64* add_ntt, size_t add_ntt_inst, size_t add_ntt_part, u32 add_ntt_parts, const u32* perm,
-                 hipStream_t s) const {
+// Digit lift + lazy forward NTT of every (digit, key limb) pair
+// (rq/mod.rs:518-540) into lift_work [inst][digits][L'][N].
+u64* Ksk::lift(const u64* src, size_t src_inst_stride, u32 inst, hipStream_t s) const {
+    const u32 Lk = key_ctx->L;
+    lift_work.ensure((size_t)inst * digits * Lk * key_ctx->N * 8);
+    u64* y = lift_work.as<u64>();
+    const KeySwitchArgs a = args(src, src_inst_stride);
+    if (fused(*key_ctx)) {
+        launch_lift_ntt(key_ctx->logn, a, y, inst, s);
+    } else {
+        launch_lift_ew(src, src_inst_stride, y, a.key_mods, digits, Lk, key_ctx->logn, inst, s);
+        launch_ntt_forward(*key_ctx, RowMap{y, Lk, Lk, 0, 0}, inst * digits * Lk, true, s);
+    }
+    return y;
+}
+
+void Ksk::key_switch_pb(const u64* src, size_t src_inst_stride, u64* acc, u32 inst, bool to_power_basis,
+                        hipStream_t s) const {
+    const u32 Lk = key_ctx->L;
+    const u64* y = lift(src, src_inst_stride, inst, s);
+    launch_mac_ew(y, digits, Lk, key0.as<ulonglong2>(), key1.as<ulonglong2>(), key_ctx->mod.as<ModInfo>(), acc,
+                  key_ctx->logn, inst, s);
+    if (to_power_basis) launch_ntt_backward(*key_ctx, RowMap{acc, Lk, Lk, 0, 0}, inst * 2 * Lk, s);
+}
+
+SdNttArgs Ksk::sd_args(u64* out, const u64* add_pb, size_t add_inst, size_t add_part, const u64* add_ntt,
+                       size_t add_ntt_inst, size_t add_ntt_part, u32 add_ntt_parts, const u32* perm) const {
     const size_t n = ct_ctx->N;
     SdNttArgs a{};
-    a.ks = ks;
     a.ks_part_stride = (size_t)key_ctx->L * n;
     a.ks_inst_stride = 2 * a.ks_part_stride;
     a.L = ct_ctx->L;
@@ -250,76 +252,64 @@ void Ksk::finish(const u64* ks, u64* out, u32 inst, const u64* add_pb, size_t ad
     a.out_inst_stride = 2 * a.out_part_stride;
     a.fwd_tw = ct_ctx->fwd_tw.as<ulonglong2>();
     a.mods = ct_ctx->mod.as<ModInfo>();
-    launch_sd_add_ntt(ct_ctx->logn, a, ct_ctx->L, 2, inst, s);
+    return a;
 }
 
-void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* acc, u64* out, u32 inst,
-                            const u64* add_pb, size_t add_inst, size_t add_part, const u64* add_ntt,
-                            size_t add_ntt_inst, size_t add_ntt_part, u32 add_ntt_parts, const u32* perm,
-                            hipStream_t s) const {
-    const size_t n = ct_ctx->N, Lk = key_ctx->L;
+void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* out, u32 inst, const u64* add_pb,
+                            size_t add_inst, size_t add_part, const u64* add_ntt, size_t add_ntt_inst,
+                            size_t add_ntt_part, u32 add_ntt_parts, const u32* perm, hipStream_t s) const {
+    const size_t n = ct_ctx->N, Lk = key_ctx->L, L = ct_ctx->L;
+    SdNttArgs a = sd_args(out, add_pb, add_inst, add_part, add_ntt, add_ntt_inst, add_ntt_part, add_ntt_parts, perm);
     if (extra > 1) {
-        // Two dropped moduli: power-basis switch-down chain on all limbs.
-        key_switch_pb(src, src_inst_stride, acc, 2 * Lk * n, inst, true, s);
-        finish(acc, out, inst, add_pb, add_inst, add_part, add_ntt, add_ntt_inst, add_ntt_part, add_ntt_parts, perm,
-               s);
+        // Two dropped moduli: key switch to power basis, then the switch-down
+        // chain on all limbs (degree <= 16384 only).
+        sd_work.ensure((size_t)inst * 2 * Lk * n * 8);
+        u64* acc = sd_work.as<u64>();
+        key_switch_pb(src, src_inst_stride, acc, inst, true, s);
+        a.ks = acc;
+        launch_sd_add_ntt(ct_ctx->logn, a, ct_ctx->L, 2, inst, s);
         return;
     }
-    key_switch_pb(src, src_inst_stride, acc, 2 * Lk * n, inst, false, s);
-    if (extra == 1) {
-        // Only the dropped limb goes back to power basis.
-        RowMap m{acc, 1, (u32)Lk, (u32)(Lk - 1), (u32)(Lk - 1)};
-        launch_ntt_backward(*key_ctx, m, inst * 2, s);
-    }
-    SdNttArgs a{};
-    a.ks = acc;
-    a.ks_part_stride = Lk * n;
-    a.ks_inst_stride = 2 * Lk * n;
-    a.L = ct_ctx->L;
-    a.extra = extra;
-    a.extra_mods = extra ? extra_mods.as<u64>() : nullptr;
-    a.sd = sd_consts.as<SdConst>();
-    a.sd_stride = sd_stride;
-    a.add_pb = add_pb;
-    a.add_inst_stride = add_inst;
-    a.add_part_stride = add_part;
-    a.add_ntt = add_ntt;
-    a.add_ntt_inst_stride = add_ntt_inst;
-    a.add_ntt_part_stride = add_ntt_part;
-    a.add_ntt_parts = add_ntt_parts;
-    a.perm = perm;
-    a.out = out;
-    a.out_part_stride = (size_t)ct_ctx->L * n;
-    a.out_inst_stride = 2 * a.out_part_stride;
-    a.fwd_tw = ct_ctx->fwd_tw.as<ulonglong2>();
-    a.mods = ct_ctx->mod.as<ModInfo>();
+    const u64* y = lift(src, src_inst_stride, inst, s);
     if (fused(*ct_ctx)) {
-        launch_ks_finish(ct_ctx->logn, a, ct_ctx->L, 2, inst, s);
-    } else {
-        const u32 L = ct_ctx->L;
-        sd_work.ensure((size_t)inst * 2 * L * n * 8);
-        u64* x = sd_work.as<u64>();
-        launch_sd_prepare(ct_ctx->logn, a, x, inst, s);
-        launch_ntt_forward(*ct_ctx, RowMap{x, L, L, 0, 0}, inst * 2 * L, false, s);
-        launch_sd_combine(ct_ctx->logn, a, x, inst, s);
+        u64* special = nullptr;
+        if (extra == 1) {
+            // Only the dropped limb goes back to power basis.
+            sd_work.ensure((size_t)inst * 2 * n * 8);
+            special = sd_work.as<u64>();
+            launch_ks_special(key_ctx->logn, args(src, src_inst_stride), y, key_ctx->inv_tw.as<ulonglong2>(), special,
+                              inst, s);
+        }
+        KsFinishArgs f{a, y, key0.as<ulonglong2>(), key1.as<ulonglong2>(), digits, (u32)Lk, special};
+        launch_ks_finish(ct_ctx->logn, f, inst, s);
+        return;
     }
+    // Unfused: inner product of every key limb, inverse NTT of the dropped
+    // limb, power-basis part, forward NTT, NTT-domain combination.
+    sd_work.ensure((size_t)inst * 2 * (Lk + L) * n * 8);
+    u64* acc = sd_work.as<u64>();
+    u64* x = acc + (size_t)inst * 2 * Lk * n;
+    launch_mac_ew(y, digits, (u32)Lk, key0.as<ulonglong2>(), key1.as<ulonglong2>(), key_ctx->mod.as<ModInfo>(), acc,
+                  key_ctx->logn, inst, s);
+    if (extra == 1) launch_ntt_backward(*key_ctx, RowMap{acc, 1, (u32)Lk, (u32)(Lk - 1), (u32)(Lk - 1)}, inst * 2, s);
+    a.ks = acc;
+    launch_sd_prepare(ct_ctx->logn, a, x, inst, s);
+    launch_ntt_forward(*ct_ctx, RowMap{x, (u32)L, (u32)L, 0, 0}, inst * 2 * (u32)L, false, s);
+    launch_sd_combine(ct_ctx->logn, a, x, inst, s);
 }
 
 // KeySwitchingKey::key_switch (key_switching_key.rs:214-241): power-basis p in
 // the ciphertext context -> NTT-domain (c0, c1) in the key context.
 void Ksk::key_switch(const u64* p, u64* out, u32 inst, hipStream_t s) const {
-    key_switch_pb(p, (size_t)ct_ctx->L * ct_ctx->N, out, 2 * (size_t)key_ctx->L * key_ctx->N, inst, false, s);
+    key_switch_pb(p, (size_t)ct_ctx->L * ct_ctx->N, out, inst, false, s);
 }
 
 // RelinearizationKey::relinearizes (relinearization_key.rs:71-98) on
 // [inst][3][L][N] NTT ciphertexts -> [inst][2][L][N].
 void Ksk::relinearize(const u64* ct3, u64* out, u32 inst, hipStream_t s) {
     const size_t n = ct_ctx->N, L = ct_ctx->L;
-    const size_t c2_bytes = (size_t)inst * L * n * 8;
-    const size_t r_bytes = (size_t)inst * 2 * key_ctx->L * n * 8;
-    work.ensure(c2_bytes + r_bytes);
+    work.ensure((size_t)inst * L * n * 8);
     u64* c2 = work.as<u64>();
-    u64* r = c2 + (size_t)inst * L * n;
     InttRowsArgs a{};
     a.src[0] = ct3 + 2 * L * n;
     a.src_inst_stride = 3 * L * n;
@@ -329,7 +319,7 @@ void Ksk::relinearize(const u64* ct3, u64* out, u32 inst, hipStream_t s) {
     a.mods = ct_ctx->mod.as<ModInfo>();
     intt_rows(*ct_ctx, a, (u32)L, 1, inst, s);
     // (c0, c1) + switch_down(key switch), added in the NTT domain after the transform.
-    switch_and_finish(c2, L * n, r, out, inst, nullptr, 0, 0, ct3, 3 * L * n, L * n, 2, nullptr, s);
+    switch_and_finish(c2, L * n, out, inst, nullptr, 0, 0, ct3, 3 * L * n, L * n, 2, nullptr, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -377,13 +367,11 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out, u32 inst,
     const size_t X = (size_t)inst * 4 * xr * n;
     const size_t T = (size_t)inst * 3 * M * n;
     const size_t D = (size_t)inst * 3 * L * n;
-    const size_t R = rk ? (size_t)inst * 2 * rk->key_ctx->L * n : 0;
-    work.ensure((P + X + T + D + R) * 8);
+    work.ensure((P + X + T + D) * 8);
     u64* wP = work.as<u64>();
     u64* wX = wP + P;
     u64* wT = wX + X;
     u64* wD = wT + T;
-    u64* wR = wD + D;
     // 1. inverse NTT of the four input polynomials (rq/scaler.rs:68-78)
     InttRowsArgs ia{};
     ia.src[0] = lhs;
@@ -400,10 +388,10 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out, u32 inst,
     // 2. base extension of the power-basis columns (rns/scaler.rs:249-352)
     if (cl < M)
         launch_rns_scale(wP, 4 * L * n, L * n, wX, 4 * xr * n, xr * n, ext_lhs->d_consts.as<u64>(), (u32)L, (u32)n, 2,
-                         inst, s);
+                         inst, s, "rns_extend");
     if (cr < M)
         launch_rns_scale(wP + 2 * L * n, 4 * L * n, L * n, wX + 2 * xr * n, 4 * xr * n, xr * n,
-                         ext_rhs->d_consts.as<u64>(), (u32)L, (u32)n, 2, inst, s);
+                         ext_rhs->d_consts.as<u64>(), (u32)L, (u32)n, 2, inst, s, "rns_extend");
     // 3. forward NTT of the new limbs (rq/scaler.rs:96-114)
     if (cl == cr) {
         RowMap m{wX, (u32)(M - cl), (u32)xr, 0, cl};
@@ -435,7 +423,7 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out, u32 inst,
     // 5. t/Q down-scaling of the three polynomials (mul.rs:187-190)
     u64* dst = rk ? wD : out;
     launch_rns_scale(wT, 3 * M * n, M * n, dst, 3 * L * n, L * n, down->d_consts.as<u64>(), (u32)M, (u32)n, 3, inst,
-                     s);
+                     s, "rns_down");
     if (!rk) {
         RowMap m{out, (u32)L, (u32)L, 0, 0};
         launch_ntt_forward(*base, m, inst * 3 * (u32)L, false, s);
@@ -443,7 +431,7 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out, u32 inst,
     }
     // 6. key switch of c2 (power basis already) (relinearization_key.rs:101-106), then
     // 7. switch down to the ciphertext context, add c0/c1, forward NTT (mul.rs:194-210)
-    rk->switch_and_finish(wD + 2 * L * n, 3 * L * n, wR, out, inst, wD, 3 * L * n, L * n, nullptr, 0, 0, 0, nullptr,
+    rk->switch_and_finish(wD + 2 * L * n, 3 * L * n, out, inst, wD, 3 * L * n, L * n, nullptr, 0, 0, 0, nullptr,
                           s);
 }
 
@@ -470,10 +458,8 @@ Galois::Galois(std::shared_ptr<Ksk> k, u32 exp) : ksk(std::move(k)) {
 void Galois::apply(const u64* ct, u64* out, u32 inst, hipStream_t s) {
     const size_t n = ksk->ct_ctx->N, L = ksk->ct_ctx->L;
     const size_t C = (size_t)inst * L * n;
-    const size_t R = (size_t)inst * 2 * ksk->key_ctx->L * n;
-    work.ensure((C + R) * 8);
+    work.ensure(C * 8);
     u64* c2 = work.as<u64>();
-    u64* r = c2 + C;
     InttRowsArgs a{};
     a.src[0] = ct + L * n;
     a.src_inst_stride = 2 * L * n;
@@ -483,7 +469,7 @@ void Galois::apply(const u64* ct, u64* out, u32 inst, hipStream_t s) {
     a.mods = ksk->ct_ctx->mod.as<ModInfo>();
     a.perm = d_perm.as<u32>();
     intt_rows(*ksk->ct_ctx, a, (u32)L, 1, inst, s);
-    ksk->switch_and_finish(c2, L * n, r, out, inst, nullptr, 0, 0, ct, 2 * L * n, 0, 1, d_perm.as<u32>(), s);
+    ksk->switch_and_finish(c2, L * n, out, inst, nullptr, 0, 0, ct, 2 * L * n, 0, 1, d_perm.as<u32>(), s);
 }
 
 // ---------------------------------------------------------------------------

@@ -38,21 +38,25 @@ struct Ksk {
     DevBuf key0, key1;  // [digits][L'][N] (value, shoup)
     DevBuf extra_mods, sd_consts;
     Workspace work;
-    mutable Workspace lift_work, sd_work;  // unfused pipeline (N = 32768) only
+    mutable Workspace lift_work, sd_work;  // transformed digits; switch-down rows
     Ksk(std::shared_ptr<DeviceContext> ct, std::shared_ptr<DeviceContext> key, const u64* c0, const u64* c1,
         u32 digits);
-    // Digit lift + MAC (+ inverse NTT when to_power_basis) into acc [inst][2][L'][N].
-    void key_switch_pb(const u64* src, size_t src_inst_stride, u64* acc, size_t acc_inst_stride, u32 inst,
-                       bool to_power_basis, hipStream_t s) const;
+    KeySwitchArgs args(const u64* src, size_t src_inst_stride) const;
+    // Digit lift + lazy forward NTT into lift_work [inst][digits][L'][N].
+    u64* lift(const u64* src, size_t src_inst_stride, u32 inst, hipStream_t s) const;
+    // Full key switch into acc [inst][2][L'][N] (NTT domain, or power basis
+    // when to_power_basis).
+    void key_switch_pb(const u64* src, size_t src_inst_stride, u64* acc, u32 inst, bool to_power_basis,
+                       hipStream_t s) const;
+    SdNttArgs sd_args(u64* out, const u64* add_pb, size_t add_inst, size_t add_part, const u64* add_ntt,
+                      size_t add_ntt_inst, size_t add_ntt_part, u32 add_ntt_parts, const u32* perm) const;
     // Key switch of power-basis digits followed by switch_down_to the
-    // ciphertext context, addends and forward NTT into out [inst][2][L][N].
-    // acc: scratch [inst][2][L'][N].
-    void switch_and_finish(const u64* src, size_t src_inst_stride, u64* acc, u64* out, u32 inst, const u64* add_pb,
+    // ciphertext context, addends and forward NTT into out [inst][2][L][N]:
+    // out = NTT(add_pb) + switch_down(key_switch(src)) + add_ntt[perm].
+    // `out` may alias add_ntt only without perm.
+    void switch_and_finish(const u64* src, size_t src_inst_stride, u64* out, u32 inst, const u64* add_pb,
                            size_t add_inst, size_t add_part, const u64* add_ntt, size_t add_ntt_inst,
                            size_t add_ntt_part, u32 add_ntt_parts, const u32* perm, hipStream_t s) const;
-    void finish(const u64* ks, u64* out, u32 inst, const u64* add_pb, size_t add_inst, size_t add_part,
-                const u64* add_ntt, size_t add_ntt_inst, size_t add_ntt_part, u32 add_ntt_parts, const u32* perm,
-                hipStream_t s) const;
     void key_switch(const u64* p, u64* out, u32 inst, hipStream_t s) const;
     void relinearize(const u64* ct3, u64* out, u32 inst, hipStream_t s);
 };

@@ -37,9 +37,10 @@ static void rns_scale_launch(dim3 grid, hipStream_t s, const u64* in, size_t in_
 }
 
 void launch_rns_scale(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst, size_t out_part,
-                      const u64* d_sc, u32 nf, u32 n, u32 parts, u32 inst, hipStream_t s) {
+                      const u64* d_sc, u32 nf, u32 n, u32 parts, u32 inst, hipStream_t s,
+                      const char* name) {
     if (!parts || !inst) return;
-    ProfScope ps("rns_scale", 31 - __builtin_clz(n), s);
+    ProfScope ps(name, 31 - __builtin_clz(n), s);
     dim3 grid((n + 255) / 256, parts, inst);
 #define FHEHIP_SC_CASE(K) \
     case K: rns_scale_launch<K>(grid, s, in, in_inst, in_part, out, out_inst, out_part, d_sc, n); break;
@@ -154,52 +155,69 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void tensor_intt_kernel(TensorAr
 // receives the NTT-domain (c0, c1) of the key switch: rows acc + inst *
 // acc_inst_stride + {0, 1} * acc_part_stride + j * N. grid = (L', instances).
 // ---------------------------------------------------------------------------
+// Digit lift and lazy forward NTT of key switching (rq/mod.rs:518-540 inside
+// key_switching_key.rs:227-239): Y[b][i][j] = forward_vt_lazy(
+// lazy_reduce_{q'_j}(p_i)) for ciphertext limb (digit) i and key limb j,
+// values < 4 q'_j. Each transformed digit is written once; the key inner
+// product reads it back in the kernels below. grid = (L', digits, instances),
+// key limb fastest so that the workgroups reading one source row run together.
+// ---------------------------------------------------------------------------
 template <int LOGN>
-__global__ __launch_bounds__(NttShape<LOGN>::T) void keyswitch_kernel(KeySwitchArgs a) {
+__global__ __launch_bounds__(NttShape<LOGN>::T) void lift_ntt_kernel(KeySwitchArgs a, u64* __restrict__ y) {
     using S = NttShape<LOGN>;
     __shared__ u64 lds[S::LDS_WORDS];
     const int t = threadIdx.x;
-    const u32 j = blockIdx.x, b = blockIdx.y;
+    const u32 j = blockIdx.x, i = blockIdx.y, b = blockIdx.z;
     const ModInfo mi = a.key_mods[j];
     const u64 p = mi.p;
-    const ulonglong2* ftw = a.key_fwd_tw + ((size_t)j << LOGN);
-    u64* acc0 = a.acc + (size_t)b * a.acc_inst_stride + ((size_t)j << LOGN);
-    u64* acc1 = acc0 + a.acc_part_stride;
-    for (u32 i = 0; i < a.digits; i++) {
-        const u64* src = a.src + (size_t)b * a.src_inst_stride + ((size_t)i << LOGN);
-        u64 x[S::E];
+    const u64* src = a.src + (size_t)b * a.src_inst_stride + ((size_t)i << LOGN);
+    u64 x[S::E];
 #pragma unroll
-        for (int e = 0; e < S::E; e++)
-            x[e] = lazy_reduce_u64(src[fwd_index<LOGN, 0>(t, e)], mi.barrett_lo, mi.barrett_hi, p);
-        // Launder the twiddle pointer so that the per-thread twiddles of the
-        // later passes are reloaded per digit instead of being hoisted out of
-        // the loop (which would keep ~170 extra registers live).
-        u64 twp = (u64)ftw;
-        asm volatile("" : "+s"(twp));
-        ntt_fwd_regs<LOGN>(lds, x, t, (const ulonglong2*)twp, p);
-        // Stage the lazily transformed digit through LDS so that the key rows
-        // and the accumulators are walked with coalesced accesses.
-#pragma unroll
-        for (int e = 0; e < S::E; e++) lds[lds_slot((t << S::RE) + e)] = x[e];
-        __syncthreads();
-        const ulonglong2* k0 = a.key0 + (((size_t)i * a.key_limbs + j) << LOGN);
-        const ulonglong2* k1 = a.key1 + (((size_t)i * a.key_limbs + j) << LOGN);
-#pragma unroll 4
-        for (int r = 0; r < S::E; r++) {
-            const int jj = r * S::T + t;
-            const u64 v = lds[lds_slot(jj)];
-            const ulonglong2 w0 = k0[jj], w1 = k1[jj];
-            u64 v0 = mul_shoup(v, w0.x, w0.y, p);
-            u64 v1 = mul_shoup(v, w1.x, w1.y, p);
-            if (i) {
-                v0 = add_mod(v0, acc0[jj], p);
-                v1 = add_mod(v1, acc1[jj], p);
-            }
-            acc0[jj] = v0;
-            acc1[jj] = v1;
-        }
-        __syncthreads();
+    for (int e = 0; e < S::E; e++)
+        x[e] = lazy_reduce_u64(src[fwd_index<LOGN, 0>(t, e)], mi.barrett_lo, mi.barrett_hi, p);
+    ntt_fwd_regs<LOGN>(lds, x, t, a.key_fwd_tw + ((size_t)j << LOGN), p);
+    u64* dst = y + ((((size_t)b * a.digits + i) * a.key_limbs + j) << LOGN);
+    store_fwd_order<LOGN>(lds, x, t, [&](int k, u64 v) { dst[k] = v; });
+}
+
+// Key inner product (key_switching_key.rs:227-239) of one key limb j at
+// element k: sum_i Y[i][j][k] * key[i][j][k] mod q'_j (Shoup, exact).
+__device__ __forceinline__ u64 ks_mac(const u64* __restrict__ yb, const ulonglong2* __restrict__ key, size_t stride,
+                                      u32 digits, u32 k, u64 p) {
+    u64 r = 0;
+    for (u32 i = 0; i < digits; i++) {
+        const ulonglong2 w = key[i * stride + k];
+        r = add_mod(r, mul_shoup(yb[i * stride + k], w.x, w.y, p), p);
     }
+    return r;
+}
+
+// Key inner product of the last key limb (the modulus that switch_down drops)
+// followed by its inverse NTT, for both output parts:
+// special[b][part] = INTT(sum_i Y[b][i][L'-1] * key_part[i][L'-1]).
+// grid = (instances, 2)
+template <int LOGN>
+__global__ __launch_bounds__(NttShape<LOGN>::T) void ks_special_kernel(KeySwitchArgs a, const u64* __restrict__ y,
+                                                                       const ulonglong2* __restrict__ key_inv_tw,
+                                                                       u64* __restrict__ special) {
+    using S = NttShape<LOGN>;
+    __shared__ u64 lds[S::LDS_WORDS];
+    const int t = threadIdx.x;
+    const u32 b = blockIdx.x, part = blockIdx.y;
+    const u32 j = a.key_limbs - 1;
+    const ModInfo mi = a.key_mods[j];
+    const u64 p = mi.p;
+    const size_t stride = (size_t)a.key_limbs << LOGN;
+    const ulonglong2* key = (part ? a.key1 : a.key0) + ((size_t)j << LOGN);
+    const u64* yb = y + (((size_t)b * a.digits * a.key_limbs + j) << LOGN);
+    const u32 digits = a.digits;
+    u64 x[S::E];
+    load_inv_order<LOGN, 2>(lds, x, t, [&](int k) { return ks_mac(yb, key, stride, digits, (u32)k, p); });
+    ntt_inv_regs<LOGN>(lds, x, t, key_inv_tw + ((size_t)j << LOGN), p);
+    u64* dst = special + (((size_t)b * 2 + part) << LOGN);
+#pragma unroll
+    for (int e = 0; e < S::E; e++)
+        dst[inv_index<LOGN, last_start<LOGN>()>(t, e)] = mul_shoup(x[e], mi.n_inv, mi.n_inv_shoup, p);
 }
 
 // ---------------------------------------------------------------------------
@@ -283,64 +301,94 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void sd_add_ntt_kernel(SdNttArgs
 // context. switch_down (math/src/rq/mod.rs:390-449) maps limb i to
 //   q_l^-1 (x_i - [y]_i) mod q_i,  y = (x_l + floor(q_l/2)) mod q_l,
 // [y]_i = lazy_reduce(y) - floor(q_l/2) mod q_i,
-// which is linear in x_i. Only the dropped limb l must be in power basis; the
-// other limbs stay in the NTT domain:
-//   out_i = NTT(add_pb_i - q_l^-1 [y]_i) + q_l^-1 ks_i  (+ NTT-domain addend).
-// With no extra modulus, out_i = NTT(add_pb_i) + ks_i. `ks` rows i < L are
-// NTT-domain; row L (when extra == 1) is power basis.
-// grid = (L, parts, instances)
+// which is linear in x_i. Only the dropped limb l must be in power basis
+// (`special`, from ks_special_kernel); the other limbs stay in the NTT domain:
+//   out_i = NTT(add_pb_i - q_l^-1 [y]_i) + q_l^-1 x_i  (+ NTT-domain addend),
+// where x_i = sum_d Y[d][i] * key[d][i] is the key inner product, computed
+// here while the output is stored. With no extra modulus,
+// out_i = NTT(add_pb_i) + x_i. One workgroup per (limb, instance) handles
+// both parts so that every Y row is read once: part 0's pass stores part 1's
+// inner product (plus its addend) into the part 1 output, which part 1's pass
+// completes. grid = (instances, L)
 // ---------------------------------------------------------------------------
 template <int LOGN>
-__global__ __launch_bounds__(NttShape<LOGN>::T) void ks_finish_kernel(SdNttArgs a) {
+__global__ __launch_bounds__(NttShape<LOGN>::T) void ks_finish_kernel(KsFinishArgs f) {
     using S = NttShape<LOGN>;
     __shared__ u64 lds[S::LDS_WORDS];
+    const SdNttArgs& a = f.sd;
     const int t = threadIdx.x;
-    const u32 i = blockIdx.x, part = blockIdx.y, b = blockIdx.z;
+    const u32 b = blockIdx.x, i = blockIdx.y;
     const ModInfo mi = a.mods[i];
-    const u64 p = mi.p;
-    const u64* ks = a.ks + (size_t)b * a.ks_inst_stride + (size_t)part * a.ks_part_stride;
-    const u64* ad = a.add_pb ? a.add_pb + (size_t)b * a.add_inst_stride + (size_t)part * a.add_part_stride +
-                                   ((size_t)i << LOGN)
-                             : nullptr;
-    u64 x[S::E];
-    SdConst cst;
-    u64 ql = 0;
-    if (a.extra) {
-        cst = a.sd[i];
-        ql = a.extra_mods[0];
-        const u64 half = ql >> 1;
-        const u64* lrow = ks + ((size_t)a.L << LOGN);
-#pragma unroll
-        for (int e = 0; e < S::E; e++) {
-            const int j = fwd_index<LOGN, 0>(t, e);
-            u64 y = lrow[j] + half;
-            y = y >= ql ? y - ql : y;
-            const u64 tmp = lazy_reduce_u64(y, mi.barrett_lo, mi.barrett_hi, p) + cst.half_mod;
-            u64 z = mul_shoup(3 * p - tmp, cst.inv, cst.inv_shoup, p);
-            if (ad) z = add_mod(z, ad[j], p);
-            x[e] = z;
-        }
-    } else {
-#pragma unroll
-        for (int e = 0; e < S::E; e++) x[e] = ad ? ad[fwd_index<LOGN, 0>(t, e)] : 0;
-    }
-    ntt_fwd_regs<LOGN>(lds, x, t, a.fwd_tw + ((size_t)i << LOGN), p);
-    const u64 p2 = 2 * p;
-    const u64* krow = ks + ((size_t)i << LOGN);
-    u64* dst = a.out + (size_t)b * a.out_inst_stride + (size_t)part * a.out_part_stride + ((size_t)i << LOGN);
-    const u64* g = (a.add_ntt && part < a.add_ntt_parts)
-                       ? a.add_ntt + (size_t)b * a.add_ntt_inst_stride + (size_t)part * a.add_ntt_part_stride +
-                             ((size_t)i << LOGN)
-                       : nullptr;
-    const u32* perm = a.perm;
+    const u64 p = mi.p, p2 = 2 * p;
     const bool extra = a.extra != 0;
-    store_fwd_order<LOGN>(lds, x, t, [&](int j, u64 v) {
-        u64 r = reduce1(reduce1(v, p2), p);
-        const u64 kv = krow[j];
-        r = add_mod(r, extra ? mul_shoup(kv, cst.inv, cst.inv_shoup, p) : kv, p);
-        if (g) r = add_mod(r, g[perm ? perm[j] : (u32)j], p);
-        dst[j] = r;
-    });
+    const SdConst cst = extra ? a.sd[i] : SdConst{};
+    const u64 ql = extra ? a.extra_mods[0] : 0;
+    const size_t stride = (size_t)f.key_limbs << LOGN;
+    const u64* yb = f.y + (((size_t)b * f.digits * f.key_limbs + i) << LOGN);
+    const ulonglong2* k0 = f.key0 + ((size_t)i << LOGN);
+    const ulonglong2* k1 = f.key1 + ((size_t)i << LOGN);
+    const u32 digits = f.digits;
+    const u32* perm = a.perm;
+    u64* out[2];
+    const u64* g[2];
+#pragma unroll
+    for (int part = 0; part < 2; part++) {
+        out[part] = a.out + (size_t)b * a.out_inst_stride + (size_t)part * a.out_part_stride + ((size_t)i << LOGN);
+        g[part] = (a.add_ntt && (u32)part < a.add_ntt_parts)
+                      ? a.add_ntt + (size_t)b * a.add_ntt_inst_stride + (size_t)part * a.add_ntt_part_stride +
+                            ((size_t)i << LOGN)
+                      : nullptr;
+    }
+    for (int part = 0; part < 2; part++) {
+        const u64* ad = a.add_pb ? a.add_pb + (size_t)b * a.add_inst_stride + (size_t)part * a.add_part_stride +
+                                       ((size_t)i << LOGN)
+                                 : nullptr;
+        u64 x[S::E];
+        if (extra) {
+            const u64 half = ql >> 1;
+            const u64* lrow = f.special + (((size_t)b * 2 + part) << LOGN);
+#pragma unroll
+            for (int e = 0; e < S::E; e++) {
+                const int j = fwd_index<LOGN, 0>(t, e);
+                u64 yv = lrow[j] + half;
+                yv = yv >= ql ? yv - ql : yv;
+                const u64 tmp = lazy_reduce_u64(yv, mi.barrett_lo, mi.barrett_hi, p) + cst.half_mod;
+                u64 z = mul_shoup(3 * p - tmp, cst.inv, cst.inv_shoup, p);
+                if (ad) z = add_mod(z, ad[j], p);
+                x[e] = z;
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < S::E; e++) x[e] = ad ? ad[fwd_index<LOGN, 0>(t, e)] : 0;
+        }
+        ntt_fwd_regs<LOGN>(lds, x, t, a.fwd_tw + ((size_t)i << LOGN), p);
+        if (part == 0) {
+            store_fwd_order<LOGN, 4, 2>(lds, x, t, [&](int j, u64 v) {
+                u64 m0 = 0, m1 = 0;
+                for (u32 d = 0; d < digits; d++) {
+                    const u64 yv = yb[d * stride + j];
+                    const ulonglong2 w0 = k0[d * stride + j], w1 = k1[d * stride + j];
+                    m0 = add_mod(m0, mul_shoup(yv, w0.x, w0.y, p), p);
+                    m1 = add_mod(m1, mul_shoup(yv, w1.x, w1.y, p), p);
+                }
+                if (extra) {
+                    m0 = mul_shoup(m0, cst.inv, cst.inv_shoup, p);
+                    m1 = mul_shoup(m1, cst.inv, cst.inv_shoup, p);
+                }
+                const u32 src = perm ? perm[j] : (u32)j;
+                u64 r = add_mod(reduce1(reduce1(v, p2), p), m0, p);
+                if (g[0]) r = add_mod(r, g[0][src], p);
+                if (g[1]) m1 = add_mod(m1, g[1][src], p);
+                out[0][j] = r;
+                out[1][j] = m1;
+            });
+        } else {
+            store_fwd_order<LOGN>(lds, x, t, [&](int j, u64 v) {
+                out[1][j] = add_mod(reduce1(reduce1(v, p2), p), out[1][j], p);
+            });
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -368,20 +416,20 @@ void launch_add_rows(const u64* a, const u64* b, u64* out, const ModInfo* mods, 
 }
 
 // ---------------------------------------------------------------------------
-#define FHEHIP_LOGN_DISPATCH(logn, KERNEL, grid, block, stream, arg)                         \
+#define FHEHIP_LOGN_DISPATCH(logn, KERNEL, grid, block, stream, ...)                         \
     switch (logn) {                                                                          \
-        case 3: hipLaunchKernelGGL(KERNEL<3>, grid, dim3(NttShape<3>::T), 0, stream, arg); break;   \
-        case 4: hipLaunchKernelGGL(KERNEL<4>, grid, dim3(NttShape<4>::T), 0, stream, arg); break;   \
-        case 5: hipLaunchKernelGGL(KERNEL<5>, grid, dim3(NttShape<5>::T), 0, stream, arg); break;   \
-        case 6: hipLaunchKernelGGL(KERNEL<6>, grid, dim3(NttShape<6>::T), 0, stream, arg); break;   \
-        case 7: hipLaunchKernelGGL(KERNEL<7>, grid, dim3(NttShape<7>::T), 0, stream, arg); break;   \
-        case 8: hipLaunchKernelGGL(KERNEL<8>, grid, dim3(NttShape<8>::T), 0, stream, arg); break;   \
-        case 9: hipLaunchKernelGGL(KERNEL<9>, grid, dim3(NttShape<9>::T), 0, stream, arg); break;   \
-        case 10: hipLaunchKernelGGL(KERNEL<10>, grid, dim3(NttShape<10>::T), 0, stream, arg); break; \
-        case 11: hipLaunchKernelGGL(KERNEL<11>, grid, dim3(NttShape<11>::T), 0, stream, arg); break; \
-        case 12: hipLaunchKernelGGL(KERNEL<12>, grid, dim3(NttShape<12>::T), 0, stream, arg); break; \
-        case 13: hipLaunchKernelGGL(KERNEL<13>, grid, dim3(NttShape<13>::T), 0, stream, arg); break; \
-        case 14: hipLaunchKernelGGL(KERNEL<14>, grid, dim3(NttShape<14>::T), 0, stream, arg); break; \
+        case 3: hipLaunchKernelGGL(KERNEL<3>, grid, dim3(NttShape<3>::T), 0, stream, __VA_ARGS__); break; \
+        case 4: hipLaunchKernelGGL(KERNEL<4>, grid, dim3(NttShape<4>::T), 0, stream, __VA_ARGS__); break; \
+        case 5: hipLaunchKernelGGL(KERNEL<5>, grid, dim3(NttShape<5>::T), 0, stream, __VA_ARGS__); break; \
+        case 6: hipLaunchKernelGGL(KERNEL<6>, grid, dim3(NttShape<6>::T), 0, stream, __VA_ARGS__); break; \
+        case 7: hipLaunchKernelGGL(KERNEL<7>, grid, dim3(NttShape<7>::T), 0, stream, __VA_ARGS__); break; \
+        case 8: hipLaunchKernelGGL(KERNEL<8>, grid, dim3(NttShape<8>::T), 0, stream, __VA_ARGS__); break; \
+        case 9: hipLaunchKernelGGL(KERNEL<9>, grid, dim3(NttShape<9>::T), 0, stream, __VA_ARGS__); break; \
+        case 10: hipLaunchKernelGGL(KERNEL<10>, grid, dim3(NttShape<10>::T), 0, stream, __VA_ARGS__); break; \
+        case 11: hipLaunchKernelGGL(KERNEL<11>, grid, dim3(NttShape<11>::T), 0, stream, __VA_ARGS__); break; \
+        case 12: hipLaunchKernelGGL(KERNEL<12>, grid, dim3(NttShape<12>::T), 0, stream, __VA_ARGS__); break; \
+        case 13: hipLaunchKernelGGL(KERNEL<13>, grid, dim3(NttShape<13>::T), 0, stream, __VA_ARGS__); break; \
+        case 14: hipLaunchKernelGGL(KERNEL<14>, grid, dim3(NttShape<14>::T), 0, stream, __VA_ARGS__); break; \
         default: throw Error(-2, "degree not supported");                                    \
     }                                                                                        \
     HIPCHK(hipGetLastError());
@@ -400,18 +448,26 @@ void launch_tensor_intt(int logn, const TensorArgs& a, u32 M, u32 inst, hipStrea
     FHEHIP_LOGN_DISPATCH(logn, tensor_intt_kernel, grid, 0, s, a);
 }
 
-void launch_keyswitch(int logn, const KeySwitchArgs& a, u32 key_limbs, u32 inst, hipStream_t s) {
-    if (!key_limbs || !inst) return;
-    ProfScope ps("keyswitch", logn, s);
-    dim3 grid(key_limbs, inst);
-    FHEHIP_LOGN_DISPATCH(logn, keyswitch_kernel, grid, 0, s, a);
+void launch_lift_ntt(int logn, const KeySwitchArgs& a, u64* y, u32 inst, hipStream_t s) {
+    if (!a.digits || !inst) return;
+    ProfScope ps("lift_ntt", logn, s);
+    dim3 grid(a.key_limbs, a.digits, inst);
+    FHEHIP_LOGN_DISPATCH(logn, lift_ntt_kernel, grid, 0, s, a, y);
 }
 
-void launch_ks_finish(int logn, const SdNttArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s) {
-    if (!L || !parts || !inst) return;
+void launch_ks_special(int logn, const KeySwitchArgs& a, const u64* y, const ulonglong2* key_inv_tw, u64* special,
+                       u32 inst, hipStream_t s) {
+    if (!inst) return;
+    ProfScope ps("ks_special", logn, s);
+    dim3 grid(inst, 2);
+    FHEHIP_LOGN_DISPATCH(logn, ks_special_kernel, grid, 0, s, a, y, key_inv_tw, special);
+}
+
+void launch_ks_finish(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s) {
+    if (!f.sd.L || !inst) return;
     ProfScope ps("ks_finish", logn, s);
-    dim3 grid(L, parts, inst);
-    FHEHIP_LOGN_DISPATCH(logn, ks_finish_kernel, grid, 0, s, a);
+    dim3 grid(inst, f.sd.L);
+    FHEHIP_LOGN_DISPATCH(logn, ks_finish_kernel, grid, 0, s, f);
 }
 
 void launch_sd_add_ntt(int logn, const SdNttArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s) {

@@ -38,8 +38,6 @@ struct KeySwitchArgs {
     const ulonglong2* key1;
     const ulonglong2* key_fwd_tw;
     const ModInfo* key_mods;
-    u64* acc;  // [inst][2][key_limbs][N]
-    size_t acc_inst_stride, acc_part_stride;
 };
 
 // Constants of one switch-down step applied to one remaining limb.
@@ -68,13 +66,30 @@ struct SdNttArgs {
     const ModInfo* mods;
 };
 
+// Key-switch finish with the inner product of the key limbs below the dropped
+// one computed in place (extra <= 1): sd.ks is unused, `special` holds the
+// dropped limb in power basis [inst][2][N] when sd.extra == 1.
+struct KsFinishArgs {
+    SdNttArgs sd;
+    const u64* y;  // [inst][digits][key_limbs][N]
+    const ulonglong2* key0;
+    const ulonglong2* key1;
+    u32 digits, key_limbs;
+    const u64* special;
+};
+
 void launch_rns_scale(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst, size_t out_part,
-                      const u64* d_sc, u32 nf, u32 n, u32 parts, u32 inst, hipStream_t s);
+                      const u64* d_sc, u32 nf, u32 n, u32 parts, u32 inst, hipStream_t s,
+                      const char* name = "rns_scale");
 void launch_intt_rows(int logn, const InttRowsArgs& a, u32 limbs, u32 parts, u32 inst, hipStream_t s);
 void launch_tensor_intt(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_t s);
-void launch_keyswitch(int logn, const KeySwitchArgs& a, u32 key_limbs, u32 inst, hipStream_t s);
+// Lifted, lazily transformed digits Y [inst][digits][key_limbs][N].
+void launch_lift_ntt(int logn, const KeySwitchArgs& a, u64* y, u32 inst, hipStream_t s);
+// Inner product of the last key limb + inverse NTT -> special [inst][2][N].
+void launch_ks_special(int logn, const KeySwitchArgs& a, const u64* y, const ulonglong2* key_inv_tw, u64* special,
+                       u32 inst, hipStream_t s);
 void launch_sd_add_ntt(int logn, const SdNttArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s);
-void launch_ks_finish(int logn, const SdNttArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s);
+void launch_ks_finish(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s);
 void launch_add_rows(const u64* a, const u64* b, u64* out, const ModInfo* mods, u32 L, int logn, size_t rows,
                      hipStream_t s);
 

@@ -226,13 +226,13 @@ __device__ __forceinline__ void load_inv_order(u64* lds, u64 (&x)[(NttShape<LOGN
 // Registers in the forward-transform output order (value e of thread t is
 // element t*E + e) -> coalesced `store(j, value)`. Must follow a forward
 // transform (whose last LDS access is fenced by a barrier).
-template <int LOGN, int R = 4, class Store>
+template <int LOGN, int R = 4, int UNROLL = 4, class Store>
 __device__ __forceinline__ void store_fwd_order(u64* lds, const u64 (&x)[(NttShape<LOGN, R>::E)], int t, Store store) {
     using S = NttShape<LOGN, R>;
 #pragma unroll
     for (int e = 0; e < S::E; e++) lds[lds_slot((t << S::RE) + e)] = x[e];
     __syncthreads();
-#pragma unroll 4
+#pragma unroll UNROLL
     for (int e = 0; e < S::E; e++) {
         const int j = e * S::T + t;
         store(j, lds[lds_slot(j)]);
