@@ -163,6 +163,118 @@ def ntt_microbench(fhehip, torch, reps):
             "roundtrip_bit_exact": exact}
 
 
+# Uneven batch: N drawn from these degrees, modulus count (including the one
+# special prime of the key) from the range of its degree, 50-62-bit primes,
+# 1-4 column rotations per instance with steps drawn from UNEVEN_STEPS.
+UNEVEN_DEGREES = [4096, 8192, 16384, 32768]
+UNEVEN_MODULI_COUNT = {4096: (2, 3), 8192: (3, 5), 16384: (5, 8), 32768: (8, 12)}
+UNEVEN_STEPS = [1, 2, 4]
+
+
+def _uniform_rows(torch, dev, moduli, shape, gen):
+    """Device int64 tensor [..., L, N] with limb i uniform in [0, q_i)."""
+    t = torch.empty(shape, dtype=torch.int64, device=dev)
+    for i, q in enumerate(moduli):
+        t[..., i, :] = torch.randint(0, q, t[..., i, :].shape, dtype=torch.int64, device=dev, generator=gen)
+    return t
+
+
+def _random_ksk(fhehip, np, rng, cctx, kctx, ct_moduli, key_moduli, n):
+    c = []
+    for _ in range(2):
+        rows = np.empty((len(ct_moduli), len(key_moduli), n), dtype=np.uint64)
+        for j, q in enumerate(key_moduli):
+            rows[:, j, :] = rng.integers(0, q, (len(ct_moduli), n), dtype=np.uint64)
+        c.append(rows)
+    return fhehip.KeySwitchingKey(cctx, kctx, c[0], c[1])
+
+
+def uneven_batch(fhehip, torch, instances=1024, reps=3, per_call_jobs=48):
+    """1024 independent evaluations of mixed degree and RNS width, each a
+    multiply + relinearize followed by 1-4 column rotations, run by one
+    fhehip_submit per step (jobs grouped by parameter set, groups on
+    concurrent streams); compared with the same jobs issued one call at a time."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(4)
+    inst = []
+    for _ in range(instances):
+        n = UNEVEN_DEGREES[int(rng.integers(len(UNEVEN_DEGREES)))]
+        lo, hi = UNEVEN_MODULI_COUNT[n]
+        count = int(rng.integers(lo, hi + 1))
+        steps = [int(x) for x in rng.choice(UNEVEN_STEPS, int(rng.integers(1, 5)))]
+        inst.append((n, count, steps))
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(5)
+    t0 = time.perf_counter()
+    sets = {}
+    for key in sorted({(n, c) for n, c, _ in inst}):
+        n, count = key
+        srng = np.random.default_rng(n + count)
+        moduli = fhehip.generate_moduli([int(x) for x in srng.integers(50, 63, count)], n)
+        ct_moduli = moduli[:-1]
+        cctx, kctx = fhehip.Context(ct_moduli, n), fhehip.Context(moduli, n)
+        rk = _random_ksk(fhehip, np, srng, cctx, kctx, ct_moduli, moduli, n)
+        mult = fhehip.Multiplicator(rk=rk, plaintext=T_PLAIN)
+        gks = [fhehip.GaloisKey(_random_ksk(fhehip, np, srng, cctx, kctx, ct_moduli, moduli, n), pow(3, st, 2 * n))
+               for st in UNEVEN_STEPS]
+        members = [k for k, (n2, c2, _) in enumerate(inst) if (n2, c2) == key]
+        m = len(members)
+        L = len(ct_moduli)
+        sets[key] = {"mult": mult, "ek": fhehip.EvaluationKey(gks), "ctx": (cctx, kctx), "L": L, "n": n,
+                     "lhs": _uniform_rows(torch, dev, ct_moduli, (m, 2, L, n), gen).view(-1),
+                     "rhs": _uniform_rows(torch, dev, ct_moduli, (m, 2, L, n), gen).view(-1),
+                     "out": torch.empty(m * 2 * L * n, dtype=torch.int64, device=dev), "next": 0}
+    setup_s = time.perf_counter() - t0
+    specs = []
+    for n, count, steps in inst:
+        S = sets[(n, count)]
+        w = 2 * S["L"] * n
+        j = S["next"]
+        S["next"] += 1
+        sl = slice(j * w, (j + 1) * w)
+        specs.append((S["mult"], S["ek"], S["lhs"][sl], S["rhs"][sl], S["out"][sl], steps))
+    jobs = fhehip.make_jobs(specs)
+    stream = torch.cuda.current_stream()
+    fhehip.submit(jobs, stream)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+    ev[0].record(stream)
+    for k in range(reps):
+        fhehip.submit(jobs, stream)
+        ev[k + 1].record(stream)
+    torch.cuda.synchronize()
+    sub_ms = [ev[k].elapsed_time(ev[k + 1]) for k in range(reps)]
+    total_ms = ev[0].elapsed_time(ev[-1])
+    # the same jobs, one call per operation
+    exact = True
+    outs = []
+    pe = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    pe[0].record(stream)
+    for mult, ek, a, b, o, steps in specs[:per_call_jobs]:
+        cur = torch.empty_like(o)
+        mult.multiply(a, b, cur, 1, stream)
+        for st in steps:
+            nxt = torch.empty_like(o)
+            ek.rotate_columns(cur, nxt, st, 1, stream)
+            cur = nxt
+        outs.append(cur)
+    pe[1].record(stream)
+    torch.cuda.synchronize()
+    per_ms = pe[0].elapsed_time(pe[1]) / max(1, min(per_call_jobs, len(specs)))
+    for (mult, ek, a, b, o, steps), cur in zip(specs, outs):
+        exact = exact and bool(torch.equal(cur, o))
+    degrees = {str(n): sum(1 for x in inst if x[0] == n) for n in UNEVEN_DEGREES}
+    value = instances * reps / (total_ms / 1e3)
+    return {"workload": f"{instances} instances, N from {UNEVEN_DEGREES}, 2-12 moduli of 50-62 bits scaled to N, "
+                        f"mul+relin then 1-4 column rotations (steps {UNEVEN_STEPS}), one fhehip_submit per step",
+            "value": value, "unit": "instances/s", "ms_per_submit": total_ms / reps,
+            "p50_submit_ms": percentile(sub_ms, 50), "parameter_sets": len(sets),
+            "instances_per_degree": degrees, "rotations": sum(len(x[2]) for x in inst),
+            "per_call_ms_per_instance": per_ms, "batched_over_per_call": value * per_ms / 1e3,
+            "bit_exact_vs_single_calls": exact, "single_calls_checked": len(outs), "setup_s": setup_s}
+
+
 def cpu_baseline(lhs_h, rhs_h, c0, c1, gpu_out0, budget_s):
     """Oracle (C++ restatement of the fhe.rs CPU path) on the host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -201,6 +313,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the single-instance and NTT measurements")
+    ap.add_argument("--only", choices=["uneven"], default=None,
+                    help="run only this secondary workload and print its JSON object")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="run one device copy of this many MiB (reference for rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -213,6 +327,9 @@ def main():
 
     import fhehip
 
+    if args.only == "uneven":
+        print(json.dumps(uneven_batch(fhehip, torch)))
+        return
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     dev = torch.device("cuda:0")
     B, L, Lk = args.batch, len(MODULI) - 1, len(MODULI)
@@ -297,6 +414,7 @@ def main():
     if not args.no_extras:
         extras["single_instance"] = single_instance_latency(fhehip, torch, 200)
         extras["ntt_microbench"] = ntt_microbench(fhehip, torch, 10)
+        extras["uneven_batch"] = uneven_batch(fhehip, torch)
     if args.copy_calibration:
         src = torch.empty(args.copy_calibration << 17, dtype=torch.int64, device=dev)
         dst = torch.empty_like(src)

@@ -111,17 +111,17 @@ void PolyScaler::scale(const u64* in, u64* out, u32 polys, bool ntt_repr, hipStr
     if (common >= to->L) return;
     const u64* src = in;
     if (ntt_repr) {
-        work.ensure((size_t)polys * isz * 8);
+        u64* tmp = work.get(s, (size_t)polys * isz * 8).as<u64>();
         InttRowsArgs a{};
         a.src[0] = in;
         a.src_inst_stride = isz;
-        a.dst = work.as<u64>();
+        a.dst = tmp;
         a.dst_inst_stride = isz;
         a.dst_part_stride = 0;
         a.inv_tw = from->inv_tw.as<ulonglong2>();
         a.mods = from->mod.as<ModInfo>();
         intt_rows(*from, a, from->L, 1, polys, s);
-        src = work.as<u64>();
+        src = tmp;
     }
     launch_rns_scale(src, isz, 0, out + (size_t)common * n, osz, 0, d_consts.as<u64>(), from->L, (u32)n, 1, polys, s);
     if (ntt_repr) {
@@ -207,8 +207,7 @@ KeySwitchArgs Ksk::args(const u64* src, size_t src_inst_stride) const {
 // (rq/mod.rs:518-540) into lift_work [inst][digits][L'][N].
 u64* Ksk::lift(const u64* src, size_t src_inst_stride, u32 inst, hipStream_t s) const {
     const u32 Lk = key_ctx->L;
-    lift_work.ensure((size_t)inst * digits * Lk * key_ctx->N * 8);
-    u64* y = lift_work.as<u64>();
+    u64* y = lift_work.get(s, (size_t)inst * digits * Lk * key_ctx->N * 8).as<u64>();
     const KeySwitchArgs a = args(src, src_inst_stride);
     if (fused(*key_ctx)) {
         launch_lift_ntt(key_ctx->logn, a, y, inst, s);
@@ -263,8 +262,7 @@ void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* out, u3
     if (extra > 1) {
         // Two dropped moduli: key switch to power basis, then the switch-down
         // chain on all limbs (degree <= 16384 only).
-        sd_work.ensure((size_t)inst * 2 * Lk * n * 8);
-        u64* acc = sd_work.as<u64>();
+        u64* acc = sd_work.get(s, (size_t)inst * 2 * Lk * n * 8).as<u64>();
         key_switch_pb(src, src_inst_stride, acc, inst, true, s);
         a.ks = acc;
         launch_sd_add_ntt(ct_ctx->logn, a, ct_ctx->L, 2, inst, s);
@@ -275,8 +273,7 @@ void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* out, u3
         u64* special = nullptr;
         if (extra == 1) {
             // Only the dropped limb goes back to power basis.
-            sd_work.ensure((size_t)inst * 2 * n * 8);
-            special = sd_work.as<u64>();
+            special = sd_work.get(s, (size_t)inst * 2 * n * 8).as<u64>();
             launch_ks_special(key_ctx->logn, args(src, src_inst_stride), y, key_ctx->inv_tw.as<ulonglong2>(), special,
                               inst, s);
         }
@@ -286,8 +283,7 @@ void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* out, u3
     }
     // Unfused: inner product of every key limb, inverse NTT of the dropped
     // limb, power-basis part, forward NTT, NTT-domain combination.
-    sd_work.ensure((size_t)inst * 2 * (Lk + L) * n * 8);
-    u64* acc = sd_work.as<u64>();
+    u64* acc = sd_work.get(s, (size_t)inst * 2 * (Lk + L) * n * 8).as<u64>();
     u64* x = acc + (size_t)inst * 2 * Lk * n;
     launch_mac_ew(y, digits, (u32)Lk, key0.as<ulonglong2>(), key1.as<ulonglong2>(), key_ctx->mod.as<ModInfo>(), acc,
                   key_ctx->logn, inst, s);
@@ -308,8 +304,7 @@ void Ksk::key_switch(const u64* p, u64* out, u32 inst, hipStream_t s) const {
 // [inst][3][L][N] NTT ciphertexts -> [inst][2][L][N].
 void Ksk::relinearize(const u64* ct3, u64* out, u32 inst, hipStream_t s) {
     const size_t n = ct_ctx->N, L = ct_ctx->L;
-    work.ensure((size_t)inst * L * n * 8);
-    u64* c2 = work.as<u64>();
+    u64* c2 = work.get(s, (size_t)inst * L * n * 8).as<u64>();
     InttRowsArgs a{};
     a.src[0] = ct3 + 2 * L * n;
     a.src_inst_stride = 3 * L * n;
@@ -367,8 +362,7 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out, u32 inst,
     const size_t X = (size_t)inst * 4 * xr * n;
     const size_t T = (size_t)inst * 3 * M * n;
     const size_t D = (size_t)inst * 3 * L * n;
-    work.ensure((P + X + T + D) * 8);
-    u64* wP = work.as<u64>();
+    u64* wP = work.get(s, (P + X + T + D) * 8).as<u64>();
     u64* wX = wP + P;
     u64* wT = wX + X;
     u64* wD = wT + T;
@@ -458,8 +452,7 @@ Galois::Galois(std::shared_ptr<Ksk> k, u32 exp) : ksk(std::move(k)) {
 void Galois::apply(const u64* ct, u64* out, u32 inst, hipStream_t s) {
     const size_t n = ksk->ct_ctx->N, L = ksk->ct_ctx->L;
     const size_t C = (size_t)inst * L * n;
-    work.ensure(C * 8);
-    u64* c2 = work.as<u64>();
+    u64* c2 = work.get(s, C * 8).as<u64>();
     InttRowsArgs a{};
     a.src[0] = ct + L * n;
     a.src_inst_stride = 2 * L * n;
@@ -496,8 +489,7 @@ void EvaluationKey::rotate_and_sum(const std::vector<u32>& exponents, const u64*
         if (!find(e)) throw Error(-1, "This key does not support the inner sum functionality");
     const auto& c = *keys[0]->ksk->ct_ctx;
     const size_t sz = (size_t)inst * 2 * c.L * c.N;
-    work.ensure(sz * 8);
-    u64* tmp = work.as<u64>();
+    u64* tmp = work.get(s, sz * 8).as<u64>();
     if (out != ct) HIPCHK(hipMemcpyAsync(out, ct, sz * 8, hipMemcpyDeviceToDevice, s));
     for (u32 e : exponents) {
         find(e)->apply(out, tmp, inst, s);
@@ -506,6 +498,14 @@ void EvaluationKey::rotate_and_sum(const std::vector<u32>& exponents, const u64*
 }
 
 // ---------------------------------------------------------------------------
+Workspace& StreamScratch::get(hipStream_t s, size_t bytes) const {
+    std::lock_guard<std::mutex> g(m_);
+    auto& w = ws_[s];
+    if (!w) w.reset(new Workspace);
+    w->ensure(bytes);
+    return *w;
+}
+
 void Workspace::ensure(size_t bytes) {
     if (bytes <= buf.bytes) return;
     buf.reset(bytes);

@@ -4,6 +4,9 @@
 #include "bfv_launch.h"
 #include "rns_device.h"
 
+#include <map>
+#include <mutex>
+
 namespace fhehip {
 
 // Device scratch owned by a handle; grows to the largest batch seen.
@@ -12,6 +15,19 @@ struct Workspace {
     void ensure(size_t bytes);
     template <class T>
     T* as() const { return buf.as<T>(); }
+};
+
+// Scratch kept per stream: calls issued on different streams never share a
+// buffer, so one handle may serve several streams at once, while calls on one
+// stream run in stream order and reuse theirs.
+class StreamScratch {
+  public:
+    // Buffer of at least `bytes` for work on stream `s`.
+    Workspace& get(hipStream_t s, size_t bytes) const;
+
+  private:
+    mutable std::mutex m_;
+    mutable std::map<hipStream_t, std::unique_ptr<Workspace>> ws_;
 };
 
 std::vector<u64> scaler_blob(const ScalerConsts& c, const std::vector<ModInfo>& to_mods, u32 start);
@@ -23,7 +39,7 @@ struct PolyScaler {
     u32 common = 0;  // number_common_moduli
     ScalerConsts consts;
     DevBuf d_consts;  // outputs start .. to->L
-    Workspace work;
+    StreamScratch work;
     PolyScaler(std::shared_ptr<DeviceContext> f, std::shared_ptr<DeviceContext> t, const BigUint& num,
                const BigUint& den);
     void scale(const u64* in, u64* out, u32 polys, bool ntt_repr, hipStream_t s);
@@ -37,12 +53,11 @@ struct Ksk {
     u32 digits = 0, extra = 0, sd_stride = 0;
     DevBuf key0, key1;  // [digits][L'][N] (value, shoup)
     DevBuf extra_mods, sd_consts;
-    Workspace work;
-    mutable Workspace lift_work, sd_work;  // transformed digits; switch-down rows
+    StreamScratch work, lift_work, sd_work;  // c2; transformed digits; switch-down rows
     Ksk(std::shared_ptr<DeviceContext> ct, std::shared_ptr<DeviceContext> key, const u64* c0, const u64* c1,
         u32 digits);
     KeySwitchArgs args(const u64* src, size_t src_inst_stride) const;
-    // Digit lift + lazy forward NTT into lift_work [inst][digits][L'][N].
+    // Digit lift + lazy forward NTT into scratch [inst][digits][L'][N].
     u64* lift(const u64* src, size_t src_inst_stride, u32 inst, hipStream_t s) const;
     // Full key switch into acc [inst][2][L'][N] (NTT domain, or power basis
     // when to_power_basis).
@@ -66,7 +81,7 @@ struct Multiplicator {
     std::shared_ptr<DeviceContext> base, mul;
     std::unique_ptr<PolyScaler> ext_lhs, ext_rhs, down;
     std::shared_ptr<Ksk> rk;
-    Workspace work;
+    StreamScratch work;
     Multiplicator(std::shared_ptr<DeviceContext> base, const std::vector<u64>& extended_basis, const BigUint& lhs_num,
                   const BigUint& lhs_den, const BigUint& rhs_num, const BigUint& rhs_den, const BigUint& post_num,
                   const BigUint& post_den);
@@ -79,7 +94,7 @@ struct Galois {
     std::shared_ptr<Ksk> ksk;
     u32 exponent = 1;
     DevBuf d_perm;
-    Workspace work;
+    StreamScratch work;
     Galois(std::shared_ptr<Ksk> k, u32 exponent);
     void apply(const u64* ct, u64* out, u32 inst, hipStream_t s);
 };
@@ -89,7 +104,7 @@ struct Galois {
 struct EvaluationKey {
     std::vector<std::shared_ptr<Galois>> keys;
     u32 degree = 0;
-    Workspace work;
+    StreamScratch work;
     std::shared_ptr<Galois> find(u32 exponent) const;
     // 3^i mod 2N (evaluation_key.rs:195-203)
     u32 column_exponent(u32 i) const;

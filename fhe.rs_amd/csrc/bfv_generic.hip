@@ -185,3 +185,28 @@ void launch_sd_combine(int logn, const SdNttArgs& a, const u64* xn, u32 inst, hi
 }
 
 }  // namespace fhehip
+
+namespace fhehip {
+
+// dst[k][e] = src[k][e] for `count` (src, dst) pairs of `elems` words each,
+// read from a device table of pointer pairs. grid = (ceil(elems / 1024), count)
+__global__ __launch_bounds__(256) void copy_slots_kernel(const u64* const* __restrict__ table, size_t elems) {
+    const u64* src = table[2 * blockIdx.y];
+    u64* dst = (u64*)table[2 * blockIdx.y + 1];
+    const size_t base = (size_t)blockIdx.x * 1024 + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const size_t e = base + (size_t)k * 256;
+        if (e < elems) dst[e] = src[e];
+    }
+}
+
+void launch_copy_slots(const u64* const* table, u32 count, size_t elems, hipStream_t s) {
+    if (!count || !elems) return;
+    ProfScope ps("copy_slots", 0, s);
+    hipLaunchKernelGGL(copy_slots_kernel, dim3((unsigned)((elems + 1023) / 1024), count), dim3(256), 0, s, table,
+                       elems);
+    HIPCHK(hipGetLastError());
+}
+
+}  // namespace fhehip

@@ -104,5 +104,8 @@ void launch_mac_ew(const u64* y, u32 digits, u32 key_limbs, const ulonglong2* ke
                    const ModInfo* key_mods, u64* acc, int logn, u32 inst, hipStream_t s);
 void launch_sd_prepare(int logn, const SdNttArgs& a, u64* x, u32 inst, hipStream_t s);
 void launch_sd_combine(int logn, const SdNttArgs& a, const u64* xn, u32 inst, hipStream_t s);
+// Copies `count` slots of `elems` words; `table` is a device array of
+// (src, dst) pointer pairs.
+void launch_copy_slots(const u64* const* table, u32 count, size_t elems, hipStream_t s);
 
 }  // namespace fhehip

@@ -1,10 +1,12 @@
 // extern "C" boundary: scaler, key switching, Multiplicator, Galois keys.
 #include "../../include/fhehip.h"
 
+#include <algorithm>
 #include <cstring>
 
 #include "bfv.h"
 #include "capi_internal.h"
+#include "jobs.h"
 
 using namespace fhehip;
 
@@ -224,6 +226,48 @@ int fhehip_ciphertext_add(const fhehip_context* ctx, const uint64_t* d_a, const 
     if (!ctx || ((!d_a || !d_b || !d_out) && num_polys)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
     const auto& c = *ctx->c;
     launch_add_rows(d_a, d_b, d_out, c.mod.as<ModInfo>(), c.L, c.logn, (size_t)num_polys * c.L, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_submit(const fhehip_job* jobs, uint32_t n, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!jobs && n) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    std::vector<Job> js(n);
+    for (uint32_t k = 0; k < n; k++) {
+        const fhehip_job& j = jobs[k];
+        if (!j.mult) return fail(FHEHIP_ERR_INVALID_ARG, "null multiplicator in a job");
+        js[k].mult = j.mult->m.get();
+        js[k].ek = j.ek ? j.ek->ek.get() : nullptr;
+        js[k].lhs = j.d_lhs;
+        js[k].rhs = j.d_rhs;
+        js[k].out = j.d_out;
+        js[k].num_rotations = j.num_rotations;
+        std::memcpy(js[k].steps, j.steps, sizeof(js[k].steps));
+    }
+    JobRunner::instance().submit(js.data(), n, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_generate_moduli(const uint32_t* sizes, uint32_t count, uint32_t degree, uint64_t* out) {
+    FHEHIP_API_BEGIN
+    if ((!sizes || !out) && count) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    if (degree < 8 || (degree & (degree - 1))) return fail(FHEHIP_ERR_INVALID_ARG, "degree is not a power of two >= 8");
+    std::vector<u64> chosen;
+    for (uint32_t k = 0; k < count; k++) {
+        const u32 sz = sizes[k];
+        if (sz < 10 || sz > 62) return fail(FHEHIP_ERR_INVALID_ARG, "modulus size outside [10, 62] bits");
+        u64 ub = (u64)1 << sz;
+        for (;;) {
+            const u64 p = generate_prime64(sz, 2 * (u64)degree, ub);
+            if (!p) return fail(FHEHIP_ERR_UNSUPPORTED, "not enough primes of the requested size");
+            if (std::find(chosen.begin(), chosen.end(), p) == chosen.end()) {
+                chosen.push_back(p);
+                break;
+            }
+            ub = p;
+        }
+    }
+    std::copy(chosen.begin(), chosen.end(), out);
     FHEHIP_API_END
 }
 

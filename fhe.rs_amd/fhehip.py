@@ -57,6 +57,8 @@ SIGNATURES = {
     "fhehip_inner_sum": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp]),
     "fhehip_rotate_and_sum": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_int, C.c_uint32, _vp]),
     "fhehip_ciphertext_add": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_submit": (C.c_int, [_vp, C.c_uint32, _vp]),
+    "fhehip_generate_moduli": (C.c_int, [_u32p, C.c_uint32, C.c_uint32, _u64p]),
     "fhehip_profiler_enable": (C.c_int, [C.c_int]),
     "fhehip_profiler_report": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
 }
@@ -305,6 +307,49 @@ class EvaluationKey:
 
 def ciphertext_add(ctx, d_a, d_b, d_out, num_polys, stream=None):
     check(lib().fhehip_ciphertext_add(ctx.h, dptr(d_a), dptr(d_b), dptr(d_out), num_polys, stream_ptr(stream)))
+
+
+JOB_MAX_ROTATIONS = 4
+
+
+class JobC(C.Structure):
+    """fhehip_job of include/fhehip.h."""
+    _fields_ = [("mult", _vp), ("ek", _vp), ("d_lhs", _vp), ("d_rhs", _vp), ("d_out", _vp),
+                ("num_rotations", C.c_uint32), ("steps", C.c_uint32 * JOB_MAX_ROTATIONS)]
+
+
+def make_jobs(specs):
+    """specs: iterable of (Multiplicator, EvaluationKey or None, lhs, rhs, out, steps)
+    with device tensors or pointers; returns a ctypes array of fhehip_job."""
+    specs = list(specs)
+    arr = (JobC * max(len(specs), 1))()
+    for k, (m, ek, a, b, o, steps) in enumerate(specs):
+        steps = list(steps or [])
+        if len(steps) > JOB_MAX_ROTATIONS:
+            raise ValueError("too many rotations in a job")
+        arr[k].mult = m.h.value
+        arr[k].ek = ek.h.value if ek is not None else None
+        arr[k].d_lhs = dptr(a)
+        arr[k].d_rhs = dptr(b)
+        arr[k].d_out = dptr(o)
+        arr[k].num_rotations = len(steps)
+        for r, st in enumerate(steps):
+            arr[k].steps[r] = st
+    return arr, len(specs)
+
+
+def submit(specs, stream=None):
+    """fhehip_submit over (Multiplicator, EvaluationKey, lhs, rhs, out, steps) jobs."""
+    arr, n = specs if isinstance(specs, tuple) else make_jobs(specs)
+    check(lib().fhehip_submit(arr, n, stream_ptr(stream)))
+
+
+def generate_moduli(sizes, degree):
+    """BfvParametersBuilder::generate_moduli on the library side."""
+    sz = (C.c_uint32 * max(len(sizes), 1))(*sizes)
+    out = (C.c_uint64 * max(len(sizes), 1))()
+    check(lib().fhehip_generate_moduli(sz, len(sizes), degree, out))
+    return [int(out[i]) for i in range(len(sizes))]
 
 
 def profiler_enable(on=True):

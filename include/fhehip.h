@@ -169,6 +169,35 @@ int fhehip_rotate_and_sum(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint6
 int fhehip_ciphertext_add(const fhehip_context* ctx, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out,
                           uint32_t num_polys, void* stream);
 
+/* ---- Batches of mixed parameters ----------------------------------------------
+ * No single fhe.rs call: independent evaluations, each Multiplicator::multiply
+ * (mul.rs:158-227) with relinearization followed by up to four
+ * EvaluationKey::rotates_columns_by (evaluation_key.rs:117-131) in order.
+ * Jobs may use different parameter sets (degree, moduli) and keys. */
+#define FHEHIP_JOB_MAX_ROTATIONS 4
+typedef struct fhehip_job {
+    fhehip_multiplicator* mult; /* relinearization enabled; defines the parameter set */
+    fhehip_evaluation_key* ek;  /* Galois keys of the steps; may be NULL without rotations */
+    const uint64_t* d_lhs;      /* [2][L][N] NTT ciphertext */
+    const uint64_t* d_rhs;      /* [2][L][N] NTT ciphertext */
+    uint64_t* d_out;            /* [2][L][N]; must not overlap any job's inputs */
+    uint32_t num_rotations;     /* 0 .. FHEHIP_JOB_MAX_ROTATIONS */
+    uint32_t steps[FHEHIP_JOB_MAX_ROTATIONS]; /* column rotation steps */
+} fhehip_job;
+/* Runs all jobs. Jobs that share a Multiplicator form one batch (staged
+ * through device pointer tables when their buffers are not contiguous);
+ * batches run concurrently on library-owned streams ordered after `stream`,
+ * and `stream` is ordered after all of them. Every job's result equals
+ * fhehip_multiply then fhehip_rotate_columns per step on that job alone.
+ * Every job is validated before any work is issued. */
+int fhehip_submit(const fhehip_job* jobs, uint32_t n, void* stream);
+
+/* BfvParametersBuilder's generate_moduli (fhe/src/bfv/parameters.rs:383-423)
+ * with generate_prime (math/src/zq/primes.rs:30-59): for each bit size in
+ * order, the largest prime below 2^size congruent to 1 mod 2 * degree not
+ * already chosen. out receives `count` moduli. */
+int fhehip_generate_moduli(const uint32_t* sizes, uint32_t count, uint32_t degree, uint64_t* out);
+
 /* ---- Measurement ------------------------------------------------------------
  * No counterpart in fhe.rs. When enabled, every kernel launch is bracketed by a
  * pair of HIP events on its stream; enabling (or disabling) clears the totals.

@@ -35,3 +35,21 @@ def test_version_and_null_checks():
     rc = lib.fhehip_context_create(None, 0, 16, None, ctypes.byref(h))
     assert rc == fhehip.ERR_INVALID_ARG
     assert lib.fhehip_last_error()
+
+
+def test_generate_moduli_matches_oracle_and_config_lists(oracle):
+    """fhehip_generate_moduli is host-only; it reproduces the oracle's
+    generate_moduli (pinned by the reference's KATs) and the configuration
+    moduli lists."""
+    import fhehip
+    cfg3 = [1152921504606748673, 1152921504606683137, 1152921504606584833, 1152921504605962241,
+            1152921504604979201, 1152921504600260609, 1152921504599080961]
+    assert fhehip.generate_moduli([60] * 7, 16384) == cfg3
+    cfg1 = [4611686018427322369, 4611686018427289601, 4611686018426454017]
+    assert fhehip.generate_moduli([62] * 3, 8192) == cfg1
+    for sizes, n in [([50, 55, 60, 62], 4096), ([62, 50, 62, 50, 40], 32768), ([20, 20], 8)]:
+        assert fhehip.generate_moduli(sizes, n) == [int(x) for x in oracle.generate_moduli(sizes, n)]
+    with pytest.raises(fhehip.FheHipError):
+        fhehip.generate_moduli([63], 1024)
+    with pytest.raises(fhehip.FheHipError):
+        fhehip.generate_moduli([60], 1000)
