@@ -275,6 +275,124 @@ def uneven_batch(fhehip, torch, instances=1024, reps=3, per_call_jobs=48):
             "bit_exact_vs_single_calls": exact, "single_calls_checked": len(outs), "setup_s": setup_s}
 
 
+# Serving: Poisson arrivals of encrypted 4096-slot sums (ct x ct multiply +
+# relinearize, then 12 rotate-and-add steps); 70% at N=16384 with the batched
+# configuration's moduli, 30% at N=32768 with 8 x 60-bit primes + 1 special.
+SERVE_RATE = 2000.0
+SERVE_SECONDS = 3.0
+SERVE_SHARE_16K = 0.7
+SERVE_LOG2_SLOTS = 12
+
+
+def serving(fhehip, torch, rate=SERVE_RATE, seconds=SERVE_SECONDS, seed=6, lanes=4, max_batch=64,
+            budget_gib=64):
+    import numpy as np
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(seed)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    kinds = []
+    t0 = time.perf_counter()
+    for n, moduli in ((16384, MODULI), (32768, fhehip.generate_moduli([60] * 9, 32768))):
+        ct_moduli = moduli[:-1]
+        cctx, kctx = fhehip.Context(ct_moduli, n), fhehip.Context(moduli, n)
+        krng = np.random.default_rng(n)
+        mult = fhehip.Multiplicator(rk=_random_ksk(fhehip, np, krng, cctx, kctx, ct_moduli, moduli, n),
+                                    plaintext=T_PLAIN)
+        gks = [fhehip.GaloisKey(_random_ksk(fhehip, np, krng, cctx, kctx, ct_moduli, moduli, n),
+                                pow(3, 1 << k, 2 * n)) for k in range(SERVE_LOG2_SLOTS)]
+        L = len(ct_moduli)
+        pool = 16
+        kinds.append({"n": n, "L": L, "w": 2 * L * n, "mult": mult, "ek": fhehip.EvaluationKey(gks),
+                      "ctx": (cctx, kctx), "pool": pool,
+                      "lhs": _uniform_rows(torch, dev, ct_moduli, (pool, 2, L, n), gen).view(-1),
+                      "rhs": _uniform_rows(torch, dev, ct_moduli, (pool, 2, L, n), gen).view(-1)})
+    setup_s = time.perf_counter() - t0
+    # arrival schedule
+    count = int(rate * seconds)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, count))
+    which = (rng.random(count) >= SERVE_SHARE_16K).astype(int)
+    for k, K in enumerate(kinds):
+        K["out"] = torch.empty((int((which == k).sum()) + max_batch) * K["w"], dtype=torch.int64, device=dev)
+    torch.cuda.synchronize()
+    srv = fhehip.Server(hbm_budget_bytes=budget_gib << 30, max_batch=max_batch, lanes=lanes)
+    pids = []
+    for K in kinds:
+        pid, fp = srv.add_params(K["mult"], K["ek"], SERVE_LOG2_SLOTS)
+        pids.append(pid)
+        K["footprint"] = fp
+
+    def inp(K, j):
+        i = j % K["pool"]
+        w = K["w"]
+        return K["lhs"][i * w:(i + 1) * w], K["rhs"][i * w:(i + 1) * w]
+
+    # warm-up: one full batch of each kind per lane sizes every scratch buffer
+    for _ in range(lanes):
+        for k, K in enumerate(kinds):
+            for j in range(max_batch):
+                a, b = inp(K, j)
+                srv.submit(pids[k], a, b, K["out"][:K["w"]])
+    srv.drain(300)
+    base = srv.stats()
+    # timed: open-loop Poisson arrivals
+    slot = [max_batch, max_batch]
+    tickets = []
+    start = time.perf_counter()
+    for j in range(count):
+        target = start + arrivals[j]
+        while True:
+            d = target - time.perf_counter()
+            if d <= 0:
+                break
+            if d > 2e-4:
+                time.sleep(d - 1.5e-4)
+        k = int(which[j])
+        K = kinds[k]
+        a, b = inp(K, j)
+        o = K["out"][slot[k] * K["w"]:(slot[k] + 1) * K["w"]]
+        slot[k] += 1
+        tickets.append((k, srv.submit(pids[k], a, b, o), j, o))
+    submit_end = time.perf_counter()
+    srv.drain(600)
+    wall = time.perf_counter() - start
+    st = srv.stats()
+    lat = {0: [], 1: []}
+    for k, t, j, o in tickets:
+        state, us = srv.query(t)
+        if state == 2:
+            lat[k].append(us / 1e3)
+    all_lat = lat[0] + lat[1]
+    # a few served results recomputed alone with direct calls
+    exact = True
+    stream = torch.cuda.current_stream()
+    for k, t, j, o in tickets[:: max(1, len(tickets) // 6)]:
+        K = kinds[k]
+        a, b = inp(K, j)
+        cur = torch.empty_like(o)
+        res = torch.empty_like(o)
+        K["mult"].multiply(a, b, cur, 1, stream)
+        K["ek"].rotate_and_sum(cur, res, SERVE_LOG2_SLOTS, False, 1, stream)
+        torch.cuda.synchronize()
+        exact = exact and bool(torch.equal(res, o))
+    srv.close()
+    done = st["completed"] - base["completed"]
+    return {"workload": f"Poisson arrivals, mean {rate:.0f} req/s for {seconds:.0f} s: ct*ct mul+relin + "
+                        f"{SERVE_LOG2_SLOTS} rotate-and-add steps (4096-slot sum); "
+                        f"{int(SERVE_SHARE_16K * 100)}% N=16384 (6x60-bit + special), "
+                        f"{100 - int(SERVE_SHARE_16K * 100)}% N=32768 (8x60-bit + special)",
+            "offered_rps": count / arrivals[-1], "value": done / wall, "unit": "req/s",
+            "p50_ms": percentile(all_lat, 50), "p99_ms": percentile(all_lat, 99),
+            "p50_ms_n16384": percentile(lat[0], 50), "p99_ms_n16384": percentile(lat[0], 99),
+            "p50_ms_n32768": percentile(lat[1], 50), "p99_ms_n32768": percentile(lat[1], 99),
+            "requests": count, "completed": done, "failed": st["failed"],
+            "batches": st["batches"] - base["batches"], "max_batch_seen": st["max_batch_seen"],
+            "peak_admitted_gib": st["peak_bytes"] / 2 ** 30, "budget_gib": budget_gib,
+            "footprint_mib": [K["footprint"] / 2 ** 20 for K in kinds], "lanes": lanes, "max_batch": max_batch,
+            "submit_lag_ms": (submit_end - start - arrivals[-1]) * 1e3,
+            "bit_exact_vs_direct_calls": exact, "setup_s": setup_s}
+
+
 def cpu_baseline(lhs_h, rhs_h, c0, c1, gpu_out0, budget_s):
     """Oracle (C++ restatement of the fhe.rs CPU path) on the host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -313,7 +431,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the single-instance and NTT measurements")
-    ap.add_argument("--only", choices=["uneven"], default=None,
+    ap.add_argument("--only", choices=["uneven", "serve"], default=None,
                     help="run only this secondary workload and print its JSON object")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="run one device copy of this many MiB (reference for rocprofv3 byte counters)")
@@ -329,6 +447,9 @@ def main():
 
     if args.only == "uneven":
         print(json.dumps(uneven_batch(fhehip, torch)))
+        return
+    if args.only == "serve":
+        print(json.dumps(serving(fhehip, torch)))
         return
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     dev = torch.device("cuda:0")
@@ -415,6 +536,7 @@ def main():
         extras["single_instance"] = single_instance_latency(fhehip, torch, 200)
         extras["ntt_microbench"] = ntt_microbench(fhehip, torch, 10)
         extras["uneven_batch"] = uneven_batch(fhehip, torch)
+        extras["serving"] = serving(fhehip, torch)
     if args.copy_calibration:
         src = torch.empty(args.copy_calibration << 17, dtype=torch.int64, device=dev)
         dst = torch.empty_like(src)

@@ -111,7 +111,7 @@ void PolyScaler::scale(const u64* in, u64* out, u32 polys, bool ntt_repr, hipStr
     if (common >= to->L) return;
     const u64* src = in;
     if (ntt_repr) {
-        u64* tmp = work.get(s, (size_t)polys * isz * 8).as<u64>();
+        u64* tmp = scratch(Scratch::Scale).get(s, (size_t)polys * isz * 8).as<u64>();
         InttRowsArgs a{};
         a.src[0] = in;
         a.src_inst_stride = isz;
@@ -207,7 +207,7 @@ KeySwitchArgs Ksk::args(const u64* src, size_t src_inst_stride) const {
 // (rq/mod.rs:518-540) into lift_work [inst][digits][L'][N].
 u64* Ksk::lift(const u64* src, size_t src_inst_stride, u32 inst, hipStream_t s) const {
     const u32 Lk = key_ctx->L;
-    u64* y = lift_work.get(s, (size_t)inst * digits * Lk * key_ctx->N * 8).as<u64>();
+    u64* y = scratch(Scratch::Lift).get(s, (size_t)inst * digits * Lk * key_ctx->N * 8).as<u64>();
     const KeySwitchArgs a = args(src, src_inst_stride);
     if (fused(*key_ctx)) {
         launch_lift_ntt(key_ctx->logn, a, y, inst, s);
@@ -262,7 +262,7 @@ void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* out, u3
     if (extra > 1) {
         // Two dropped moduli: key switch to power basis, then the switch-down
         // chain on all limbs (degree <= 16384 only).
-        u64* acc = sd_work.get(s, (size_t)inst * 2 * Lk * n * 8).as<u64>();
+        u64* acc = scratch(Scratch::SwitchDown).get(s, (size_t)inst * 2 * Lk * n * 8).as<u64>();
         key_switch_pb(src, src_inst_stride, acc, inst, true, s);
         a.ks = acc;
         launch_sd_add_ntt(ct_ctx->logn, a, ct_ctx->L, 2, inst, s);
@@ -273,7 +273,7 @@ void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* out, u3
         u64* special = nullptr;
         if (extra == 1) {
             // Only the dropped limb goes back to power basis.
-            special = sd_work.get(s, (size_t)inst * 2 * n * 8).as<u64>();
+            special = scratch(Scratch::SwitchDown).get(s, (size_t)inst * 2 * n * 8).as<u64>();
             launch_ks_special(key_ctx->logn, args(src, src_inst_stride), y, key_ctx->inv_tw.as<ulonglong2>(), special,
                               inst, s);
         }
@@ -283,7 +283,7 @@ void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* out, u3
     }
     // Unfused: inner product of every key limb, inverse NTT of the dropped
     // limb, power-basis part, forward NTT, NTT-domain combination.
-    u64* acc = sd_work.get(s, (size_t)inst * 2 * (Lk + L) * n * 8).as<u64>();
+    u64* acc = scratch(Scratch::SwitchDown).get(s, (size_t)inst * 2 * (Lk + L) * n * 8).as<u64>();
     u64* x = acc + (size_t)inst * 2 * Lk * n;
     launch_mac_ew(y, digits, (u32)Lk, key0.as<ulonglong2>(), key1.as<ulonglong2>(), key_ctx->mod.as<ModInfo>(), acc,
                   key_ctx->logn, inst, s);
@@ -304,7 +304,7 @@ void Ksk::key_switch(const u64* p, u64* out, u32 inst, hipStream_t s) const {
 // [inst][3][L][N] NTT ciphertexts -> [inst][2][L][N].
 void Ksk::relinearize(const u64* ct3, u64* out, u32 inst, hipStream_t s) {
     const size_t n = ct_ctx->N, L = ct_ctx->L;
-    u64* c2 = work.get(s, (size_t)inst * L * n * 8).as<u64>();
+    u64* c2 = scratch(Scratch::RelinC2).get(s, (size_t)inst * L * n * 8).as<u64>();
     InttRowsArgs a{};
     a.src[0] = ct3 + 2 * L * n;
     a.src_inst_stride = 3 * L * n;
@@ -362,7 +362,7 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out, u32 inst,
     const size_t X = (size_t)inst * 4 * xr * n;
     const size_t T = (size_t)inst * 3 * M * n;
     const size_t D = (size_t)inst * 3 * L * n;
-    u64* wP = work.get(s, (P + X + T + D) * 8).as<u64>();
+    u64* wP = scratch(Scratch::Mul).get(s, (P + X + T + D) * 8).as<u64>();
     u64* wX = wP + P;
     u64* wT = wX + X;
     u64* wD = wT + T;
@@ -452,7 +452,7 @@ Galois::Galois(std::shared_ptr<Ksk> k, u32 exp) : ksk(std::move(k)) {
 void Galois::apply(const u64* ct, u64* out, u32 inst, hipStream_t s) {
     const size_t n = ksk->ct_ctx->N, L = ksk->ct_ctx->L;
     const size_t C = (size_t)inst * L * n;
-    u64* c2 = work.get(s, C * 8).as<u64>();
+    u64* c2 = scratch(Scratch::GaloisC2).get(s, C * 8).as<u64>();
     InttRowsArgs a{};
     a.src[0] = ct + L * n;
     a.src_inst_stride = 2 * L * n;
@@ -489,7 +489,7 @@ void EvaluationKey::rotate_and_sum(const std::vector<u32>& exponents, const u64*
         if (!find(e)) throw Error(-1, "This key does not support the inner sum functionality");
     const auto& c = *keys[0]->ksk->ct_ctx;
     const size_t sz = (size_t)inst * 2 * c.L * c.N;
-    u64* tmp = work.get(s, sz * 8).as<u64>();
+    u64* tmp = scratch(Scratch::EvalTmp).get(s, sz * 8).as<u64>();
     if (out != ct) HIPCHK(hipMemcpyAsync(out, ct, sz * 8, hipMemcpyDeviceToDevice, s));
     for (u32 e : exponents) {
         find(e)->apply(out, tmp, inst, s);
@@ -498,6 +498,12 @@ void EvaluationKey::rotate_and_sum(const std::vector<u32>& exponents, const u64*
 }
 
 // ---------------------------------------------------------------------------
+StreamScratch& scratch(Scratch k) {
+    // Never destroyed: device memory is released when the process ends.
+    static StreamScratch* pools = new StreamScratch[(int)Scratch::Count];
+    return pools[(int)k];
+}
+
 Workspace& StreamScratch::get(hipStream_t s, size_t bytes) const {
     std::lock_guard<std::mutex> g(m_);
     auto& w = ws_[s];

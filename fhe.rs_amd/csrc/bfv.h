@@ -30,6 +30,13 @@ class StreamScratch {
     mutable std::map<hipStream_t, std::unique_ptr<Workspace>> ws_;
 };
 
+// Process-wide scratch pools by purpose. Work on one stream runs in order, so
+// every handle shares the pool of each purpose per stream; purposes that are
+// live at the same time (a multiplication and the key switch it calls) use
+// different pools.
+enum class Scratch { Scale, RelinC2, Lift, SwitchDown, Mul, GaloisC2, EvalTmp, Count };
+StreamScratch& scratch(Scratch k);
+
 std::vector<u64> scaler_blob(const ScalerConsts& c, const std::vector<ModInfo>& to_mods, u32 start);
 
 // rq::scaler::Scaler (math/src/rq/scaler.rs:18-126).
@@ -39,7 +46,6 @@ struct PolyScaler {
     u32 common = 0;  // number_common_moduli
     ScalerConsts consts;
     DevBuf d_consts;  // outputs start .. to->L
-    StreamScratch work;
     PolyScaler(std::shared_ptr<DeviceContext> f, std::shared_ptr<DeviceContext> t, const BigUint& num,
                const BigUint& den);
     void scale(const u64* in, u64* out, u32 polys, bool ntt_repr, hipStream_t s);
@@ -53,7 +59,6 @@ struct Ksk {
     u32 digits = 0, extra = 0, sd_stride = 0;
     DevBuf key0, key1;  // [digits][L'][N] (value, shoup)
     DevBuf extra_mods, sd_consts;
-    StreamScratch work, lift_work, sd_work;  // c2; transformed digits; switch-down rows
     Ksk(std::shared_ptr<DeviceContext> ct, std::shared_ptr<DeviceContext> key, const u64* c0, const u64* c1,
         u32 digits);
     KeySwitchArgs args(const u64* src, size_t src_inst_stride) const;
@@ -81,7 +86,6 @@ struct Multiplicator {
     std::shared_ptr<DeviceContext> base, mul;
     std::unique_ptr<PolyScaler> ext_lhs, ext_rhs, down;
     std::shared_ptr<Ksk> rk;
-    StreamScratch work;
     Multiplicator(std::shared_ptr<DeviceContext> base, const std::vector<u64>& extended_basis, const BigUint& lhs_num,
                   const BigUint& lhs_den, const BigUint& rhs_num, const BigUint& rhs_den, const BigUint& post_num,
                   const BigUint& post_den);
@@ -94,7 +98,6 @@ struct Galois {
     std::shared_ptr<Ksk> ksk;
     u32 exponent = 1;
     DevBuf d_perm;
-    StreamScratch work;
     Galois(std::shared_ptr<Ksk> k, u32 exponent);
     void apply(const u64* ct, u64* out, u32 inst, hipStream_t s);
 };
@@ -104,7 +107,6 @@ struct Galois {
 struct EvaluationKey {
     std::vector<std::shared_ptr<Galois>> keys;
     u32 degree = 0;
-    StreamScratch work;
     std::shared_ptr<Galois> find(u32 exponent) const;
     // 3^i mod 2N (evaluation_key.rs:195-203)
     u32 column_exponent(u32 i) const;

@@ -7,6 +7,7 @@
 #include "bfv.h"
 #include "capi_internal.h"
 #include "jobs.h"
+#include "server.h"
 
 using namespace fhehip;
 
@@ -24,6 +25,9 @@ struct fhehip_galois {
 };
 struct fhehip_evaluation_key {
     std::unique_ptr<EvaluationKey> ek;
+};
+struct fhehip_server {
+    std::unique_ptr<Server> s;
 };
 
 static BigUint factor_num(const fhehip_scaling_factor* f) {
@@ -245,6 +249,66 @@ int fhehip_submit(const fhehip_job* jobs, uint32_t n, void* stream) {
         std::memcpy(js[k].steps, j.steps, sizeof(js[k].steps));
     }
     JobRunner::instance().submit(js.data(), n, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_server_create(const fhehip_server_config* cfg, fhehip_server** out) {
+    FHEHIP_API_BEGIN
+    if (!cfg || !out) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    ServerConfig c;
+    c.hbm_budget = cfg->hbm_budget_bytes;
+    c.max_batch = cfg->max_batch;
+    c.lanes = cfg->lanes;
+    *out = new fhehip_server{std::unique_ptr<Server>(new Server(c))};
+    FHEHIP_API_END
+}
+
+void fhehip_server_destroy(fhehip_server* s) { delete s; }
+
+int fhehip_server_add_params(fhehip_server* s, fhehip_multiplicator* mult, fhehip_evaluation_key* ek,
+                             uint32_t log2_slots, uint32_t* id, uint64_t* footprint_bytes) {
+    FHEHIP_API_BEGIN
+    if (!s || !mult || !ek || !id) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *id = s->s->add_params(mult->m.get(), ek->ek.get(), log2_slots);
+    if (footprint_bytes) *footprint_bytes = s->s->params(*id).footprint;
+    FHEHIP_API_END
+}
+
+int fhehip_server_submit(fhehip_server* s, uint32_t params, const uint64_t* d_lhs, const uint64_t* d_rhs,
+                         uint64_t* d_out, uint64_t* ticket) {
+    FHEHIP_API_BEGIN
+    if (!s || !ticket) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *ticket = s->s->submit(params, d_lhs, d_rhs, d_out);
+    FHEHIP_API_END
+}
+
+int fhehip_server_query(fhehip_server* s, uint64_t ticket, int* state, double* latency_us) {
+    FHEHIP_API_BEGIN
+    if (!s || !state || !latency_us) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    s->s->query(ticket, *state, *latency_us);
+    FHEHIP_API_END
+}
+
+int fhehip_server_drain(fhehip_server* s, double timeout_s) {
+    FHEHIP_API_BEGIN
+    if (!s) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    if (!s->s->drain(timeout_s)) return fail(FHEHIP_ERR_DEVICE, "server drain timed out");
+    const ServerStats st = s->s->stats();
+    if (st.failed) return fail(FHEHIP_ERR_DEVICE, "requests failed: " + s->s->error());
+    FHEHIP_API_END
+}
+
+int fhehip_server_stats(fhehip_server* s, fhehip_server_statistics* out) {
+    FHEHIP_API_BEGIN
+    if (!s || !out) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    const ServerStats st = s->s->stats();
+    out->submitted = st.submitted;
+    out->completed = st.completed;
+    out->failed = st.failed;
+    out->batches = st.batches;
+    out->peak_bytes = st.peak_bytes;
+    out->max_batch_seen = st.max_batch_seen;
     FHEHIP_API_END
 }
 

@@ -59,6 +59,13 @@ SIGNATURES = {
     "fhehip_ciphertext_add": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
     "fhehip_submit": (C.c_int, [_vp, C.c_uint32, _vp]),
     "fhehip_generate_moduli": (C.c_int, [_u32p, C.c_uint32, C.c_uint32, _u64p]),
+    "fhehip_server_create": (C.c_int, [_vp, _vpp]),
+    "fhehip_server_destroy": (None, [_vp]),
+    "fhehip_server_add_params": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _u32p, _u64p]),
+    "fhehip_server_submit": (C.c_int, [_vp, C.c_uint32, _vp, _vp, _vp, _u64p]),
+    "fhehip_server_query": (C.c_int, [_vp, C.c_uint64, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "fhehip_server_drain": (C.c_int, [_vp, C.c_double]),
+    "fhehip_server_stats": (C.c_int, [_vp, _vp]),
     "fhehip_profiler_enable": (C.c_int, [C.c_int]),
     "fhehip_profiler_report": (C.c_int, [C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
 }
@@ -342,6 +349,65 @@ def submit(specs, stream=None):
     """fhehip_submit over (Multiplicator, EvaluationKey, lhs, rhs, out, steps) jobs."""
     arr, n = specs if isinstance(specs, tuple) else make_jobs(specs)
     check(lib().fhehip_submit(arr, n, stream_ptr(stream)))
+
+
+class ServerConfigC(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_batch", C.c_uint32), ("lanes", C.c_uint32)]
+
+
+class ServerStatsC(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("batches", C.c_uint64), ("peak_bytes", C.c_uint64), ("max_batch_seen", C.c_uint32)]
+
+
+class Server:
+    """Continuous-batching request server (fhehip_server_*)."""
+
+    def __init__(self, hbm_budget_bytes=0, max_batch=64, lanes=4):
+        cfg = ServerConfigC(hbm_budget_bytes, max_batch, lanes)
+        h = C.c_void_p()
+        check(lib().fhehip_server_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fhehip_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_params(self, mult, ek, log2_slots):
+        """Returns (id, footprint_bytes)."""
+        i = C.c_uint32()
+        fp = C.c_uint64()
+        check(lib().fhehip_server_add_params(self.h, mult.h, ek.h, log2_slots, C.byref(i), C.byref(fp)))
+        self._keep.append((mult, ek))
+        return i.value, fp.value
+
+    def submit(self, params, d_lhs, d_rhs, d_out):
+        t = C.c_uint64()
+        check(lib().fhehip_server_submit(self.h, params, dptr(d_lhs), dptr(d_rhs), dptr(d_out), C.byref(t)))
+        return t.value
+
+    def query(self, ticket):
+        """(state, latency_us): state 0 queued, 1 running, 2 done, -1 failed."""
+        st = C.c_int()
+        lat = C.c_double()
+        check(lib().fhehip_server_query(self.h, ticket, C.byref(st), C.byref(lat)))
+        return st.value, lat.value
+
+    def drain(self, timeout_s=60.0):
+        check(lib().fhehip_server_drain(self.h, timeout_s))
+
+    def stats(self):
+        s = ServerStatsC()
+        check(lib().fhehip_server_stats(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStatsC._fields_}
 
 
 def generate_moduli(sizes, degree):

@@ -192,6 +192,47 @@ typedef struct fhehip_job {
  * Every job is validated before any work is issued. */
 int fhehip_submit(const fhehip_job* jobs, uint32_t n, void* stream);
 
+/* ---- Serving ------------------------------------------------------------------
+ * No fhe.rs counterpart: a request server for encrypted slot sums. A request
+ * is a ct x ct multiply + relinearize (Multiplicator::multiply) followed by
+ * the sum over 2^log2_slots consecutive slots (the first log2_slots steps of
+ * EvaluationKey::computes_inner_sum). A server thread admits queued requests
+ * in arrival order into batches of one parameter set while their device
+ * footprint fits the budget, and launches each batch on a free lane (its own
+ * stream) as soon as one is free, so requests arriving while others run are
+ * batched and overlap on the device. Keys and contexts stay resident; request
+ * buffers are device memory owned by the caller and must stay valid until
+ * the request is done. */
+typedef struct fhehip_server fhehip_server;
+typedef struct fhehip_server_config {
+    uint64_t hbm_budget_bytes; /* device bytes admitted requests may occupy at once; 0: no limit */
+    uint32_t max_batch;        /* requests per batch */
+    uint32_t lanes;            /* batches in flight at once (1..16) */
+} fhehip_server_config;
+typedef struct fhehip_server_statistics {
+    uint64_t submitted, completed, failed, batches;
+    uint64_t peak_bytes;       /* largest admitted footprint at once */
+    uint32_t max_batch_seen;
+} fhehip_server_statistics;
+int fhehip_server_create(const fhehip_server_config* cfg, fhehip_server** out);
+/* Stops admitting, fails what is still queued, waits for running batches. */
+void fhehip_server_destroy(fhehip_server* s);
+/* Registers a request kind; the server shares ownership of neither handle:
+ * both must outlive it. footprint_bytes (may be NULL) receives the device
+ * bytes one admitted request of this kind is charged. */
+int fhehip_server_add_params(fhehip_server* s, fhehip_multiplicator* mult, fhehip_evaluation_key* ek,
+                             uint32_t log2_slots, uint32_t* id, uint64_t* footprint_bytes);
+/* Queues one request: lhs, rhs, out are [2][L][N] NTT ciphertexts. */
+int fhehip_server_submit(fhehip_server* s, uint32_t params, const uint64_t* d_lhs, const uint64_t* d_rhs,
+                         uint64_t* d_out, uint64_t* ticket);
+/* state: 0 queued, 1 running, 2 done, -1 failed. latency_us: from submit to
+ * the server observing completion (done requests). */
+int fhehip_server_query(fhehip_server* s, uint64_t ticket, int* state, double* latency_us);
+/* Waits until every request submitted so far is done or failed;
+ * FHEHIP_ERR_DEVICE on timeout or when a request failed. */
+int fhehip_server_drain(fhehip_server* s, double timeout_s);
+int fhehip_server_stats(fhehip_server* s, fhehip_server_statistics* out);
+
 /* BfvParametersBuilder's generate_moduli (fhe/src/bfv/parameters.rs:383-423)
  * with generate_prime (math/src/zq/primes.rs:30-59): for each bit size in
  * order, the largest prime below 2^size congruent to 1 mod 2 * degree not
