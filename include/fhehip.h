@@ -13,7 +13,9 @@
  * Polynomials are laid out like fhe.rs' Array2<u64>: [limb][degree] row major,
  * and batches of them back to back. Ciphertexts are [part][limb][degree].
  * `stream` is a hipStream_t (NULL = default stream); calls are asynchronous
- * with respect to the host unless stated otherwise.
+ * with respect to the host unless stated otherwise. Scratch memory is kept
+ * per stream: calls on different streams may run concurrently, also on one
+ * handle; calls on one stream run in stream order. Degrees 8 .. 32768.
  */
 #ifndef FHEHIP_H
 #define FHEHIP_H
@@ -100,8 +102,8 @@ int fhehip_scaler_scale(fhehip_scaler* s, const uint64_t* d_in, uint64_t* d_out,
  * NTT-domain rows c0, c1 given as HOST arrays [digits][L_key][N]; digits must
  * equal the number of ciphertext moduli (RNS-limb decomposition,
  * key_switching_key.rs:89-104). The key context must extend the ciphertext
- * context by at most two moduli (the special primes removed by
- * switch_down_to). The rows are copied to the device with their Shoup
+ * context by at most two moduli (one at degree 32768): the special primes
+ * removed by switch_down_to. The rows are copied to the device with their Shoup
  * companions (NttShoup representation). */
 int fhehip_ksk_create(const fhehip_context* ct_ctx, const fhehip_context* key_ctx, const uint64_t* c0,
                       const uint64_t* c1, uint32_t digits, fhehip_ksk** out);
@@ -131,7 +133,7 @@ int fhehip_multiplicator_extended_basis(const fhehip_multiplicator* m, uint64_t*
 /* Multiplicator::multiply (mul.rs:158-227) on `num` independent pairs:
  * lhs, rhs [num][2][L][N] NTT -> out [num][2][L][N] (or [num][3][L][N]
  * without relinearization). One call runs the whole batch as a fixed
- * sequence of batched kernels; concurrent calls on one handle are unsafe. */
+ * sequence of batched kernels. */
 int fhehip_multiply(fhehip_multiplicator* m, const uint64_t* d_lhs, const uint64_t* d_rhs, uint64_t* d_out,
                     uint32_t num, void* stream);
 
