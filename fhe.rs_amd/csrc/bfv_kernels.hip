@@ -81,24 +81,26 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void intt_rows_kernel(InttRowsAr
     } else {
         load_inv_order<LOGN>(lds, x, t, [&](int j) { return src[j]; });
     }
-    ntt_inv_regs<LOGN>(lds, x, t, tw, mi.p);
+    ntt_inv_regs<LOGN>(lds, x, t, tw, mi.p, inv_scale_n(mi));
 #pragma unroll
-    for (int e = 0; e < S::E; e++)
-        dst[inv_index<LOGN, last_start<LOGN>()>(t, e)] = mul_shoup(x[e], mi.n_inv, mi.n_inv_shoup, mi.p);
+    for (int e = 0; e < S::E; e++) dst[inv_index<LOGN, last_start<LOGN>()>(t, e)] = x[e];
 }
 
 // ---------------------------------------------------------------------------
 // Tensor product fused with the inverse NTT (Multiplicator::multiply,
 // fhe/src/bfv/ops/mul.rs:181-185 then the INTT of the down-scaler, rq/scaler.rs
 // :68-78): c0 = a0 b0, c1 = a0 b1 + a1 b0, c2 = a1 b1 over the M limbs of the
-// multiplication context. grid = (M, 3, instances).
+// multiplication context. 1-D grid over (instance, limb, product), product fastest.
 // ---------------------------------------------------------------------------
 template <int LOGN>
 __global__ __launch_bounds__(NttShape<LOGN>::T) void tensor_intt_kernel(TensorArgs a) {
     using S = NttShape<LOGN>;
     __shared__ u64 lds[S::LDS_WORDS];
     const int t = threadIdx.x;
-    const u32 m = blockIdx.x, c = blockIdx.y, b = blockIdx.z;
+    // The three products of one (limb, instance) read the same rows: they are
+    // consecutive logical indices, so they run on one XCD.
+    const u32 l = xcd_block(blockIdx.x, gridDim.x);
+    const u32 c = l % 3, m = (l / 3) % a.M, b = l / (3 * a.M);
     const ModInfo mi = a.mods[m];
     // Row pointers of the four extended inputs a0, a1, b0, b1 at limb m.
     const u64* rows[4];
@@ -139,11 +141,10 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void tensor_intt_kernel(TensorAr
         });
     }
     const ulonglong2* tw = a.inv_tw + ((size_t)m << LOGN);
-    ntt_inv_regs<LOGN>(lds, x, t, tw, mi.p);
+    ntt_inv_regs<LOGN>(lds, x, t, tw, mi.p, inv_scale_nr(mi));
     u64* dst = a.out + (size_t)b * a.out_inst_stride + (size_t)c * a.out_part_stride + ((size_t)m << LOGN);
 #pragma unroll
-    for (int e = 0; e < S::E; e++)
-        dst[inv_index<LOGN, last_start<LOGN>()>(t, e)] = mul_shoup(x[e], mi.n_inv_r, mi.n_inv_r_shoup, mi.p);
+    for (int e = 0; e < S::E; e++) dst[inv_index<LOGN, last_start<LOGN>()>(t, e)] = x[e];
 }
 
 // ---------------------------------------------------------------------------
@@ -159,15 +160,18 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void tensor_intt_kernel(TensorAr
 // key_switching_key.rs:227-239): Y[b][i][j] = forward_vt_lazy(
 // lazy_reduce_{q'_j}(p_i)) for ciphertext limb (digit) i and key limb j,
 // values < 4 q'_j. Each transformed digit is written once; the key inner
-// product reads it back in the kernels below. grid = (L', digits, instances),
-// key limb fastest so that the workgroups reading one source row run together.
+// product reads it back in the kernels below. 1-D grid over
+// (instance, digit, key limb), key limb fastest.
 // ---------------------------------------------------------------------------
 template <int LOGN>
 __global__ __launch_bounds__(NttShape<LOGN>::T) void lift_ntt_kernel(KeySwitchArgs a, u64* __restrict__ y) {
     using S = NttShape<LOGN>;
     __shared__ u64 lds[S::LDS_WORDS];
     const int t = threadIdx.x;
-    const u32 j = blockIdx.x, i = blockIdx.y, b = blockIdx.z;
+    // The key limbs of one digit read the same source row: consecutive
+    // logical indices, run on one XCD.
+    const u32 l = xcd_block(blockIdx.x, gridDim.x);
+    const u32 j = l % a.key_limbs, i = (l / a.key_limbs) % a.digits, b = l / (a.key_limbs * a.digits);
     const ModInfo mi = a.key_mods[j];
     const u64 p = mi.p;
     const u64* src = a.src + (size_t)b * a.src_inst_stride + ((size_t)i << LOGN);
@@ -213,11 +217,10 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void ks_special_kernel(KeySwitch
     const u32 digits = a.digits;
     u64 x[S::E];
     load_inv_order<LOGN, 2>(lds, x, t, [&](int k) { return ks_mac(yb, key, stride, digits, (u32)k, p); });
-    ntt_inv_regs<LOGN>(lds, x, t, key_inv_tw + ((size_t)j << LOGN), p);
+    ntt_inv_regs<LOGN>(lds, x, t, key_inv_tw + ((size_t)j << LOGN), p, inv_scale_n(mi));
     u64* dst = special + (((size_t)b * 2 + part) << LOGN);
 #pragma unroll
-    for (int e = 0; e < S::E; e++)
-        dst[inv_index<LOGN, last_start<LOGN>()>(t, e)] = mul_shoup(x[e], mi.n_inv, mi.n_inv_shoup, p);
+    for (int e = 0; e < S::E; e++) dst[inv_index<LOGN, last_start<LOGN>()>(t, e)] = x[e];
 }
 
 // ---------------------------------------------------------------------------
@@ -444,14 +447,16 @@ void launch_intt_rows(int logn, const InttRowsArgs& a, u32 limbs, u32 parts, u32
 void launch_tensor_intt(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_t s) {
     if (!M || !inst) return;
     ProfScope ps("tensor_intt", logn, s);
-    dim3 grid(M, 3, inst);
-    FHEHIP_LOGN_DISPATCH(logn, tensor_intt_kernel, grid, 0, s, a);
+    TensorArgs b = a;
+    b.M = M;
+    dim3 grid(M * 3 * inst);
+    FHEHIP_LOGN_DISPATCH(logn, tensor_intt_kernel, grid, 0, s, b);
 }
 
 void launch_lift_ntt(int logn, const KeySwitchArgs& a, u64* y, u32 inst, hipStream_t s) {
     if (!a.digits || !inst) return;
     ProfScope ps("lift_ntt", logn, s);
-    dim3 grid(a.key_limbs, a.digits, inst);
+    dim3 grid(a.key_limbs * a.digits * inst);
     FHEHIP_LOGN_DISPATCH(logn, lift_ntt_kernel, grid, 0, s, a, y);
 }
 

@@ -28,6 +28,7 @@ struct TensorArgs {
     size_t out_inst_stride, out_part_stride;
     const ulonglong2* inv_tw;  // multiplication context
     const ModInfo* mods;
+    u32 M;                     // limbs of the multiplication context (set by the launcher)
 };
 
 struct KeySwitchArgs {

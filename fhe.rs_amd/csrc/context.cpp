@@ -29,6 +29,16 @@ void DevBuf::reset(size_t n) {
 
 // NttOperator::new tables (math/src/ntt/native.rs:35-73) for every modulus,
 // uploaded as interleaved (value, shoup) pairs.
+// Constants of the inverse transform's last stage with the final scaling by
+// N^-1 (or N^-1 2^64) folded in; z is that stage's twiddle.
+static void set_last_stage(ModInfo& mi, u64 z) {
+    const u64 p = mi.p;
+    mi.last_zn = mulmod64(z, mi.n_inv, p);
+    mi.last_zn_shoup = shoup64(mi.last_zn, p);
+    mi.last_zr = mulmod64(z, mi.n_inv_r, p);
+    mi.last_zr_shoup = shoup64(mi.last_zr, p);
+}
+
 std::shared_ptr<DeviceContext> make_context(const std::vector<u64>& moduli, u32 n, const u64* psi) {
     if (moduli.empty()) throw Error(-1, "empty moduli list");
     if (n < 8 || (n & (n - 1))) throw Error(-1, "The degree is not a power of two larger or equal to 8");
@@ -70,11 +80,12 @@ std::shared_ptr<DeviceContext> make_context(const std::vector<u64>& moduli, u32 
         barrett128(p, mi.barrett_lo, mi.barrett_hi);
         mi.n_inv = invmod64(n % p, p);
         mi.n_inv_shoup = shoup64(mi.n_inv, p);
-        u64 inv = 1;  // p^-1 mod 2^64 by Newton iteration (p odd)
-        for (int it = 0; it < 6; it++) inv *= 2 - p * inv;
-        mi.pinv_neg = (u64)0 - inv;
+        u64 pinv = 1;  // p^-1 mod 2^64 by Newton iteration (p odd)
+        for (int it = 0; it < 6; it++) pinv *= 2 - p * pinv;
+        mi.pinv_neg = (u64)0 - pinv;
         mi.n_inv_r = (u64)(((u128)mi.n_inv << 64) % p);
         mi.n_inv_r_shoup = shoup64(mi.n_inv_r, p);
+        set_last_stage(mi, inv[(size_t)l * n + n - 2].x);
         c->h_mod.push_back(mi);
     }
     // q_last^-1 mod q_i (context.rs:67-74)
@@ -115,6 +126,7 @@ std::shared_ptr<DeviceContext> make_context(const std::vector<u64>& moduli, u32 
                 mi.n_inv_shoup = shoup64(1, p);
                 mi.n_inv_r = (u64)(((u128)1 << 64) % p);
                 mi.n_inv_r_shoup = shoup64(mi.n_inv_r, p);
+                set_last_stage(mi, I[nh - 2].x);
                 hc->h_mod.push_back(mi);
                 hc->moduli.push_back(p);
             }

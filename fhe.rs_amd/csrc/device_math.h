@@ -113,6 +113,10 @@ struct ModInfo {
     u64 pinv_neg;    // -p^-1 mod 2^64 (Montgomery)
     u64 n_inv_r;     // N^-1 * 2^64 mod p: undoes a Montgomery factor inside an inverse NTT
     u64 n_inv_r_shoup;
+    // zeta of the inverse transform's last stage times N^-1 (and times
+    // N^-1 2^64): the final scaling folded into that stage.
+    u64 last_zn, last_zn_shoup;
+    u64 last_zr, last_zr_shoup;
 };
 
 }  // namespace fhehip

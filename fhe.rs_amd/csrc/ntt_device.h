@@ -28,6 +28,16 @@ struct NttShape {
     static constexpr int LDS_WORDS = N + (N >> 4) + 1;
 };
 
+// Workgroups are dispatched round-robin over the chip's 8 XCDs, each with its
+// own L2: workgroup b of a 1-D grid runs on XCD b % 8. This returns the
+// logical index of workgroup b in an order that gives each XCD a contiguous
+// range, so consecutive logical indices (workgroups reading the same rows)
+// run on one XCD at about the same time and share its L2.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned total) {
+    const unsigned xcd = b & 7, slot = b >> 3, base = total >> 3, rem = total & 7;
+    return xcd * base + (xcd < rem ? xcd : rem) + slot;
+}
+
 // LDS word of element j: one pad word every 16 spreads the strided exchange
 // patterns over the banks.
 __device__ __forceinline__ int lds_slot(int j) { return j + (j >> 4); }
@@ -46,6 +56,12 @@ __device__ __forceinline__ void inv_bfly(u64& x, u64& y, u64 z, u64 zs, u64 p, u
     x = reduce1(y + t, p2);
     y = lazy_mul_shoup(t + p2 - y, z, zs, p);
 }
+
+// Final scaling of an inverse transform folded into its last stage:
+// x' = (x + y) s and y' = (x - y) zeta s, with zs = zeta s; outputs reduced.
+struct InvScale {
+    u64 s, ss, zs, zss;
+};
 
 // Element index owned by thread t in slot k for a forward pass whose
 // (virtual) first stage is S0V: stage s pairs bit LOGN-1-s.
@@ -95,9 +111,10 @@ __device__ __forceinline__ void fwd_pass(u64 (&x)[(NttShape<LOGN, R>::E)], int t
 
 // Inverse stages pairing bits B0V .. B0V+RE-1, skipping bits below VMIN.
 // tw: interleaved (zeta_inv, zeta_inv_shoup) pairs of the row's modulus.
-template <int LOGN, int B0V, int VMIN, int R = 4>
+// With SCALE, the last stage (bit LOGN-1, one group) applies `sc`.
+template <int LOGN, int B0V, int VMIN, int R = 4, bool SCALE = false>
 __device__ __forceinline__ void inv_pass(u64 (&x)[(NttShape<LOGN, R>::E)], int t, const ulonglong2* __restrict__ tw,
-                                         u64 p, u64 p2) {
+                                         u64 p, u64 p2, const InvScale& sc) {
     constexpr int RE = NttShape<LOGN, R>::RE;
     constexpr int N = 1 << LOGN;
     const int hi = t >> B0V;
@@ -106,6 +123,15 @@ __device__ __forceinline__ void inv_pass(u64 (&x)[(NttShape<LOGN, R>::E)], int t
         const int v = B0V + u;
         if (v < VMIN) continue;
         const int half = 1 << u;
+        if (SCALE && v == LOGN - 1) {
+#pragma unroll
+            for (int q = 0; q < half; q++) {
+                const u64 a = x[q], b = x[q + half];
+                x[q] = mul_shoup(a + b, sc.s, sc.ss, p);
+                x[q + half] = mul_shoup(a + p2 - b, sc.zs, sc.zss, p);
+            }
+            continue;
+        }
         const int offset = N - (N >> v);
 #pragma unroll
         for (int g = 0; g < (1 << (RE - 1 - u)); g++) {
@@ -166,9 +192,9 @@ __device__ __forceinline__ void fwd_rest(u64* lds, u64 (&x)[(NttShape<LOGN, R>::
     }
 }
 
-template <int LOGN, int B0V, int R = 4>
+template <int LOGN, int B0V, int R = 4, bool SCALE = false>
 __device__ __forceinline__ void inv_rest(u64* lds, u64 (&x)[(NttShape<LOGN, R>::E)], int t, const ulonglong2* tw,
-                                         u64 p, u64 p2) {
+                                         u64 p, u64 p2, const InvScale& sc) {
     constexpr int RE = NttShape<LOGN, R>::RE;
     if constexpr (B0V + RE < LOGN) {
         constexpr int NEXT = (B0V + 2 * RE <= LOGN) ? B0V + RE : LOGN - RE;
@@ -177,8 +203,8 @@ __device__ __forceinline__ void inv_rest(u64* lds, u64 (&x)[(NttShape<LOGN, R>::
         __syncthreads();
         lds_get_inv<LOGN, NEXT, R>(lds, x, t);
         __syncthreads();
-        inv_pass<LOGN, NEXT, VMIN, R>(x, t, tw, p, p2);
-        inv_rest<LOGN, NEXT, R>(lds, x, t, tw, p, p2);
+        inv_pass<LOGN, NEXT, VMIN, R, SCALE>(x, t, tw, p, p2, sc);
+        inv_rest<LOGN, NEXT, R, SCALE>(lds, x, t, tw, p, p2, sc);
     }
 }
 
@@ -193,15 +219,24 @@ __device__ __forceinline__ void ntt_fwd_regs(u64* lds, u64 (&x)[(NttShape<LOGN, 
     fwd_rest<LOGN, 0, R>(lds, x, t, tw, p, p2);
 }
 
-// Full inverse transform on registers (without the N^-1 factor): input order
-// inv_index<LOGN,0> (E consecutive values per thread), output order
-// inv_index<LOGN, last_start<LOGN>()>, values < 2p.
+// Full inverse transform on registers: input order inv_index<LOGN,0> (E
+// consecutive values per thread), output order inv_index<LOGN,
+// last_start<LOGN>()>. The final multiplication (by N^-1, or by N^-1 2^64
+// after Montgomery products) is folded into the last stage through `sc`;
+// the outputs are reduced.
 template <int LOGN, int R = 4>
 __device__ __forceinline__ void ntt_inv_regs(u64* lds, u64 (&x)[(NttShape<LOGN, R>::E)], int t, const ulonglong2* tw,
-                                             u64 p) {
+                                             u64 p, const InvScale& sc) {
     const u64 p2 = 2 * p;
-    inv_pass<LOGN, 0, 0, R>(x, t, tw, p, p2);
-    inv_rest<LOGN, 0, R>(lds, x, t, tw, p, p2);
+    inv_pass<LOGN, 0, 0, R, true>(x, t, tw, p, p2, sc);
+    inv_rest<LOGN, 0, R, true>(lds, x, t, tw, p, p2, sc);
+}
+
+__device__ __forceinline__ InvScale inv_scale_n(const ModInfo& mi) {
+    return InvScale{mi.n_inv, mi.n_inv_shoup, mi.last_zn, mi.last_zn_shoup};
+}
+__device__ __forceinline__ InvScale inv_scale_nr(const ModInfo& mi) {
+    return InvScale{mi.n_inv_r, mi.n_inv_r_shoup, mi.last_zr, mi.last_zr_shoup};
 }
 
 // Coalesced global row -> registers in the inverse-transform input order

@@ -46,10 +46,9 @@ __global__ __launch_bounds__((NttShape<LOGN, R>::T)) void ntt_inv_kernel(RowMap 
     const ModInfo mi = mods[limb];
     u64 x[S::E];
     load_inv_order<LOGN, 4, R>(lds, x, t, [&](int j) { return row[j]; });
-    ntt_inv_regs<LOGN, R>(lds, x, t, tw, mi.p);
+    ntt_inv_regs<LOGN, R>(lds, x, t, tw, mi.p, inv_scale_n(mi));
 #pragma unroll
-    for (int k = 0; k < S::E; k++)
-        row[inv_index<LOGN, last_start<LOGN, R>(), R>(t, k)] = mul_shoup(x[k], mi.n_inv, mi.n_inv_shoup, mi.p);
+    for (int k = 0; k < S::E; k++) row[inv_index<LOGN, last_start<LOGN, R>(), R>(t, k)] = x[k];
 }
 
 // Elements per thread of the plain NTT kernels: 2^4 by default; the
