@@ -188,6 +188,25 @@ Ksk::Ksk(std::shared_ptr<DeviceContext> ct, std::shared_ptr<DeviceContext> key, 
     }
     sd_consts.reset(sd.size() * sizeof(SdConst));
     HIPCHK(hipMemcpy(sd_consts.ptr, sd.data(), sd_consts.bytes, hipMemcpyHostToDevice));
+    if (extra <= 1 && fused(*ct_ctx)) {
+        // key_d,i * q_l^-1 mod q_i: the switch-down factor of limb i applied to
+        // the key instead of to every key-switch output.
+        std::vector<u64> f0((size_t)digits * L * n), f1(f0.size());
+        for (u32 d = 0; d < digits; d++)
+            for (u32 i = 0; i < L; i++) {
+                const u64 q = key_ctx->moduli[i];
+                const u64 inv = extra ? sd[i].inv : 1;
+                const size_t src = ((size_t)d * Lk + i) * n, dst = ((size_t)d * L + i) * n;
+                for (u32 j = 0; j < n; j++) {
+                    f0[dst + j] = mulmod64(c0[src + j], inv, q);
+                    f1[dst + j] = mulmod64(c1[src + j], inv, q);
+                }
+            }
+        fin0.reset(f0.size() * 8);
+        fin1.reset(f1.size() * 8);
+        HIPCHK(hipMemcpy(fin0.ptr, f0.data(), fin0.bytes, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(fin1.ptr, f1.data(), fin1.bytes, hipMemcpyHostToDevice));
+    }
 }
 
 KeySwitchArgs Ksk::args(const u64* src, size_t src_inst_stride) const {
@@ -277,7 +296,7 @@ void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* out, u3
             launch_ks_special(key_ctx->logn, args(src, src_inst_stride), y, key_ctx->inv_tw.as<ulonglong2>(), special,
                               inst, s);
         }
-        KsFinishArgs f{a, y, key0.as<ulonglong2>(), key1.as<ulonglong2>(), digits, (u32)Lk, special};
+        KsFinishArgs f{a, y, fin0.as<u64>(), fin1.as<u64>(), digits, (u32)Lk, special};
         launch_ks_finish(ct_ctx->logn, f, inst, s);
         return;
     }

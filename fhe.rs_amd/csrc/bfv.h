@@ -58,6 +58,10 @@ struct Ksk {
     std::shared_ptr<DeviceContext> ct_ctx, key_ctx;
     u32 digits = 0, extra = 0, sd_stride = 0;
     DevBuf key0, key1;  // [digits][L'][N] (value, shoup)
+    // Rows of the key limbs below the dropped one with the switch-down factor
+    // q_l^-1 folded in (plain values, [digits][L][N]); used by the fused
+    // finish when the key is at most one level above the ciphertext.
+    DevBuf fin0, fin1;
     DevBuf extra_mods, sd_consts;
     Ksk(std::shared_ptr<DeviceContext> ct, std::shared_ptr<DeviceContext> key, const u64* c0, const u64* c1,
         u32 digits);

@@ -159,7 +159,7 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void tensor_intt_kernel(TensorAr
 // Digit lift and lazy forward NTT of key switching (rq/mod.rs:518-540 inside
 // key_switching_key.rs:227-239): Y[b][i][j] = forward_vt_lazy(
 // lazy_reduce_{q'_j}(p_i)) for ciphertext limb (digit) i and key limb j,
-// values < 4 q'_j. Each transformed digit is written once; the key inner
+// stored < 2 q'_j. Each transformed digit is written once; the key inner
 // product reads it back in the kernels below. 1-D grid over
 // (instance, digit, key limb), key limb fastest.
 // ---------------------------------------------------------------------------
@@ -181,7 +181,8 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void lift_ntt_kernel(KeySwitchAr
         x[e] = lazy_reduce_u64(src[fwd_index<LOGN, 0>(t, e)], mi.barrett_lo, mi.barrett_hi, p);
     ntt_fwd_regs<LOGN>(lds, x, t, a.key_fwd_tw + ((size_t)j << LOGN), p);
     u64* dst = y + ((((size_t)b * a.digits + i) * a.key_limbs + j) << LOGN);
-    store_fwd_order<LOGN>(lds, x, t, [&](int k, u64 v) { dst[k] = v; });
+    const u64 p2 = 2 * p;
+    store_fwd_order<LOGN>(lds, x, t, [&](int k, u64 v) { dst[k] = reduce1(v, p2); });
 }
 
 // Key inner product (key_switching_key.rs:227-239) of one key limb j at
@@ -307,91 +308,97 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void sd_add_ntt_kernel(SdNttArgs
 // which is linear in x_i. Only the dropped limb l must be in power basis
 // (`special`, from ks_special_kernel); the other limbs stay in the NTT domain:
 //   out_i = NTT(add_pb_i - q_l^-1 [y]_i) + q_l^-1 x_i  (+ NTT-domain addend),
-// where x_i = sum_d Y[d][i] * key[d][i] is the key inner product, computed
-// here while the output is stored. With no extra modulus,
-// out_i = NTT(add_pb_i) + x_i. One workgroup per (limb, instance) handles
-// both parts so that every Y row is read once: part 0's pass stores part 1's
-// inner product (plus its addend) into the part 1 output, which part 1's pass
-// completes. grid = (instances, L)
+// where x_i = sum_d Y[d][i] * key[d][i] is the key inner product. With no
+// extra modulus, out_i = NTT(add_pb_i) + x_i. Two kernels: ks_mac_kernel
+// writes q_l^-1 x_i (+ addend) into `out`, elementwise; ks_finish_kernel
+// adds the transformed power-basis part.
 // ---------------------------------------------------------------------------
+
+// One thread per coefficient of limb i, both parts: the switch-down factor
+// q_l^-1 is folded into the key rows (fin0, fin1), Y < 2q and the key rows
+// < q, so four products stay below 2^127 and one Barrett reduction per four
+// terms gives the exact residue. grid = (N / 256, L, instances)
+__global__ __launch_bounds__(256) void ks_mac_kernel(KsFinishArgs f, u32 logn) {
+    const SdNttArgs& a = f.sd;
+    const u32 j = blockIdx.x * 256 + threadIdx.x;
+    if (j >> logn) return;
+    const u32 i = blockIdx.y, b = blockIdx.z;
+    const ModInfo mi = a.mods[i];
+    const u64 p = mi.p;
+    const size_t ystride = (size_t)f.key_limbs << logn, kstride = (size_t)a.L << logn;
+    const u64* yb = f.y + (((size_t)b * f.digits * f.key_limbs + i) << logn) + j;
+    const u64* k0 = f.fin0 + ((size_t)i << logn) + j;
+    const u64* k1 = f.fin1 + ((size_t)i << logn) + j;
+    u64 m0 = 0, m1 = 0;
+    for (u32 d0 = 0; d0 < f.digits; d0 += 4) {
+        const u32 dn = f.digits - d0 < 4 ? f.digits - d0 : 4;
+        u64 l0 = 0, h0 = 0, l1 = 0, h1 = 0;
+#pragma unroll
+        for (u32 d = 0; d < 4; d++) {
+            if (d < dn) {
+                const u64 yv = yb[(d0 + d) * ystride];
+                u64 lo, hi;
+                mul_wide(yv, k0[(d0 + d) * kstride], lo, hi);
+                add128(l0, h0, lo, hi);
+                mul_wide(yv, k1[(d0 + d) * kstride], lo, hi);
+                add128(l1, h1, lo, hi);
+            }
+        }
+        m0 = add_mod(m0, reduce_u128(l0, h0, mi.barrett_lo, mi.barrett_hi, p), p);
+        m1 = add_mod(m1, reduce_u128(l1, h1, mi.barrett_lo, mi.barrett_hi, p), p);
+    }
+    const u32 src = a.perm ? a.perm[j] : j;
+    u64* out = a.out + (size_t)b * a.out_inst_stride + ((size_t)i << logn) + j;
+    u64 m[2] = {m0, m1};
+#pragma unroll
+    for (u32 part = 0; part < 2; part++) {
+        if (a.add_ntt && part < a.add_ntt_parts)
+            m[part] = add_mod(m[part], a.add_ntt[(size_t)b * a.add_ntt_inst_stride + (size_t)part * a.add_ntt_part_stride +
+                                                 ((size_t)i << logn) + src],
+                              p);
+        out[(size_t)part * a.out_part_stride] = m[part];
+    }
+}
+
+// out_i += NTT(add_pb_i - q_l^-1 [y]_i) (or NTT(add_pb_i) without an extra
+// modulus). 1-D grid over (instance, part, limb), limb fastest: the limbs of
+// one (instance, part) read the same dropped-limb row and run on one XCD.
 template <int LOGN>
 __global__ __launch_bounds__(NttShape<LOGN>::T) void ks_finish_kernel(KsFinishArgs f) {
     using S = NttShape<LOGN>;
     __shared__ u64 lds[S::LDS_WORDS];
     const SdNttArgs& a = f.sd;
     const int t = threadIdx.x;
-    const u32 b = blockIdx.x, i = blockIdx.y;
+    const u32 l = xcd_block(blockIdx.x, gridDim.x);
+    const u32 i = l % a.L, part = (l / a.L) & 1, b = l / (2 * a.L);
     const ModInfo mi = a.mods[i];
     const u64 p = mi.p, p2 = 2 * p;
-    const bool extra = a.extra != 0;
-    const SdConst cst = extra ? a.sd[i] : SdConst{};
-    const u64 ql = extra ? a.extra_mods[0] : 0;
-    const size_t stride = (size_t)f.key_limbs << LOGN;
-    const u64* yb = f.y + (((size_t)b * f.digits * f.key_limbs + i) << LOGN);
-    const ulonglong2* k0 = f.key0 + ((size_t)i << LOGN);
-    const ulonglong2* k1 = f.key1 + ((size_t)i << LOGN);
-    const u32 digits = f.digits;
-    const u32* perm = a.perm;
-    u64* out[2];
-    const u64* g[2];
+    const u64* ad = a.add_pb ? a.add_pb + (size_t)b * a.add_inst_stride + (size_t)part * a.add_part_stride +
+                                   ((size_t)i << LOGN)
+                             : nullptr;
+    u64 x[S::E];
+    if (a.extra) {
+        const SdConst cst = a.sd[i];
+        const u64 ql = a.extra_mods[0];
+        const u64 half = ql >> 1;
+        const u64* lrow = f.special + (((size_t)b * 2 + part) << LOGN);
 #pragma unroll
-    for (int part = 0; part < 2; part++) {
-        out[part] = a.out + (size_t)b * a.out_inst_stride + (size_t)part * a.out_part_stride + ((size_t)i << LOGN);
-        g[part] = (a.add_ntt && (u32)part < a.add_ntt_parts)
-                      ? a.add_ntt + (size_t)b * a.add_ntt_inst_stride + (size_t)part * a.add_ntt_part_stride +
-                            ((size_t)i << LOGN)
-                      : nullptr;
-    }
-    for (int part = 0; part < 2; part++) {
-        const u64* ad = a.add_pb ? a.add_pb + (size_t)b * a.add_inst_stride + (size_t)part * a.add_part_stride +
-                                       ((size_t)i << LOGN)
-                                 : nullptr;
-        u64 x[S::E];
-        if (extra) {
-            const u64 half = ql >> 1;
-            const u64* lrow = f.special + (((size_t)b * 2 + part) << LOGN);
-#pragma unroll
-            for (int e = 0; e < S::E; e++) {
-                const int j = fwd_index<LOGN, 0>(t, e);
-                u64 yv = lrow[j] + half;
-                yv = yv >= ql ? yv - ql : yv;
-                const u64 tmp = lazy_reduce_u64(yv, mi.barrett_lo, mi.barrett_hi, p) + cst.half_mod;
-                u64 z = mul_shoup(3 * p - tmp, cst.inv, cst.inv_shoup, p);
-                if (ad) z = add_mod(z, ad[j], p);
-                x[e] = z;
-            }
-        } else {
-#pragma unroll
-            for (int e = 0; e < S::E; e++) x[e] = ad ? ad[fwd_index<LOGN, 0>(t, e)] : 0;
+        for (int e = 0; e < S::E; e++) {
+            const int j = fwd_index<LOGN, 0>(t, e);
+            u64 yv = lrow[j] + half;
+            yv = yv >= ql ? yv - ql : yv;
+            const u64 tmp = lazy_reduce_u64(yv, mi.barrett_lo, mi.barrett_hi, p) + cst.half_mod;
+            u64 z = mul_shoup(3 * p - tmp, cst.inv, cst.inv_shoup, p);
+            if (ad) z = add_mod(z, ad[j], p);
+            x[e] = z;
         }
-        ntt_fwd_regs<LOGN>(lds, x, t, a.fwd_tw + ((size_t)i << LOGN), p);
-        if (part == 0) {
-            store_fwd_order<LOGN, 4, 2>(lds, x, t, [&](int j, u64 v) {
-                u64 m0 = 0, m1 = 0;
-                for (u32 d = 0; d < digits; d++) {
-                    const u64 yv = yb[d * stride + j];
-                    const ulonglong2 w0 = k0[d * stride + j], w1 = k1[d * stride + j];
-                    m0 = add_mod(m0, mul_shoup(yv, w0.x, w0.y, p), p);
-                    m1 = add_mod(m1, mul_shoup(yv, w1.x, w1.y, p), p);
-                }
-                if (extra) {
-                    m0 = mul_shoup(m0, cst.inv, cst.inv_shoup, p);
-                    m1 = mul_shoup(m1, cst.inv, cst.inv_shoup, p);
-                }
-                const u32 src = perm ? perm[j] : (u32)j;
-                u64 r = add_mod(reduce1(reduce1(v, p2), p), m0, p);
-                if (g[0]) r = add_mod(r, g[0][src], p);
-                if (g[1]) m1 = add_mod(m1, g[1][src], p);
-                out[0][j] = r;
-                out[1][j] = m1;
-            });
-        } else {
-            store_fwd_order<LOGN>(lds, x, t, [&](int j, u64 v) {
-                out[1][j] = add_mod(reduce1(reduce1(v, p2), p), out[1][j], p);
-            });
-        }
-        __syncthreads();
+    } else {
+#pragma unroll
+        for (int e = 0; e < S::E; e++) x[e] = ad ? ad[fwd_index<LOGN, 0>(t, e)] : 0;
     }
+    ntt_fwd_regs<LOGN>(lds, x, t, a.fwd_tw + ((size_t)i << LOGN), p);
+    u64* dst = a.out + (size_t)b * a.out_inst_stride + (size_t)part * a.out_part_stride + ((size_t)i << LOGN);
+    store_fwd_order<LOGN>(lds, x, t, [&](int j, u64 v) { dst[j] = add_mod(reduce1(reduce1(v, p2), p), dst[j], p); });
 }
 
 // ---------------------------------------------------------------------------
@@ -470,8 +477,14 @@ void launch_ks_special(int logn, const KeySwitchArgs& a, const u64* y, const ulo
 
 void launch_ks_finish(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s) {
     if (!f.sd.L || !inst) return;
+    {
+        ProfScope ps("ks_mac", logn, s);
+        hipLaunchKernelGGL(ks_mac_kernel, dim3(((1u << logn) + 255) / 256, f.sd.L, inst), dim3(256), 0, s, f,
+                           (u32)logn);
+        HIPCHK(hipGetLastError());
+    }
     ProfScope ps("ks_finish", logn, s);
-    dim3 grid(inst, f.sd.L);
+    dim3 grid(inst * 2 * f.sd.L);
     FHEHIP_LOGN_DISPATCH(logn, ks_finish_kernel, grid, 0, s, f);
 }
 

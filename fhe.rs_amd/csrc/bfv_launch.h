@@ -72,9 +72,9 @@ struct SdNttArgs {
 // dropped limb in power basis [inst][2][N] when sd.extra == 1.
 struct KsFinishArgs {
     SdNttArgs sd;
-    const u64* y;  // [inst][digits][key_limbs][N]
-    const ulonglong2* key0;
-    const ulonglong2* key1;
+    const u64* y;     // [inst][digits][key_limbs][N], values < 2q
+    const u64* fin0;  // [digits][L][N] key rows times q_l^-1 (or plain without extra)
+    const u64* fin1;
     u32 digits, key_limbs;
     const u64* special;
 };

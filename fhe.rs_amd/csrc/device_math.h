@@ -81,18 +81,17 @@ __device__ __forceinline__ void mul_wide(u64 a, u64 b, u64& lo, u64& hi) { mul_w
 
 // (lo, hi) += (alo, ahi)
 __device__ __forceinline__ void add128(u64& lo, u64& hi, u64 alo, u64 ahi) {
-    lo += alo;
-    hi += ahi + (lo < alo ? 1 : 0);
+    unsigned long long c;
+    lo = __builtin_addcll(lo, alo, 0, &c);
+    hi = hi + ahi + c;
 }
 
 // (lo, hi, top) += (alo, ahi): 192-bit accumulation of 128-bit terms.
 __device__ __forceinline__ void add192(u64& lo, u64& hi, u64& top, u64 alo, u64 ahi) {
-    lo += alo;
-    const u64 c0 = lo < alo ? 1 : 0;
-    const u64 h = hi + ahi;
-    const u64 c1 = h < hi ? 1 : 0;
-    hi = h + c0;
-    top += c1 + (hi < c0 ? 1 : 0);
+    unsigned long long c0, c1;
+    lo = __builtin_addcll(lo, alo, 0, &c0);
+    hi = __builtin_addcll(hi, ahi, c0, &c1);
+    top += c1;
 }
 
 // Montgomery reduction with R = 2^64 of t = hi * 2^64 + lo < p * 2^63:
