@@ -86,6 +86,21 @@ def load_traffic(kernel):
     return None
 
 
+def load_valu(kernel):
+    """(VALUBusy %, wave-instructions per CU-cycle) of `kernel` from the
+    committed rocprofv3 VALU counter summary, when present."""
+    path = os.path.join(ROOT, "profiles", "pmc_valu_mul_relin_n16384_b512.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 6 and parts[0] == kernel:
+                    return float(parts[1]), float(parts[5])
+    except (OSError, ValueError):
+        pass
+    return None, None
+
+
 # Single-instance configuration: N = 8192, generate_moduli([62] * 3, 8192), t = 65537,
 # relinearization key at the ciphertext level.
 CFG1_N = 8192
@@ -527,9 +542,11 @@ def main():
     abytes = kernel_bytes(dname, B, L, Lk, M)
     achieved = abytes / avg_s / 1e9 if abytes else None
     traffic = load_traffic(dname.split("/")[0])
+    valu_busy, valu_rate = load_valu(dname.split("/")[0])
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
-                "kernel": dname, "avg_launch_ms": avg_s * 1e3, "algorithmic_bytes": abytes}
+                "kernel": dname, "avg_launch_ms": avg_s * 1e3, "algorithmic_bytes": abytes,
+                "valu_busy_pct": valu_busy, "valu_wave_instructions_per_cu_cycle": valu_rate}
 
     extras = {}
     if not args.no_extras:

@@ -15,7 +15,12 @@ typedef uint64_t u64;
 typedef uint32_t u32;
 typedef unsigned __int128 u128;
 
-__device__ __forceinline__ u64 reduce1(u64 x, u64 p) { return x >= p ? x - p : x; }
+// x mod p for x < 2p: the borrow of x - p selects the result.
+__device__ __forceinline__ u64 reduce1(u64 x, u64 p) {
+    unsigned long long borrow;
+    const u64 d = __builtin_subcll(x, p, 0, &borrow);
+    return borrow ? x : d;
+}
 
 // Full 64 x 64 -> 128-bit product from four 32 x 32 -> 64 partial products.
 __device__ __forceinline__ void mul_wide_fwd(u64 a, u64 b, u64& lo, u64& hi) {
