@@ -188,7 +188,7 @@ Ksk::Ksk(std::shared_ptr<DeviceContext> ct, std::shared_ptr<DeviceContext> key, 
     }
     sd_consts.reset(sd.size() * sizeof(SdConst));
     HIPCHK(hipMemcpy(sd_consts.ptr, sd.data(), sd_consts.bytes, hipMemcpyHostToDevice));
-    if (extra <= 1 && fused(*ct_ctx)) {
+    if (extra <= 1) {
         // key_d,i * q_l^-1 mod q_i: the switch-down factor of limb i applied to
         // the key instead of to every key-switch output.
         std::vector<u64> f0((size_t)digits * L * n), f1(f0.size());
@@ -231,8 +231,7 @@ u64* Ksk::lift(const u64* src, size_t src_inst_stride, u32 inst, hipStream_t s) 
     if (fused(*key_ctx)) {
         launch_lift_ntt(key_ctx->logn, a, y, inst, s);
     } else {
-        launch_lift_ew(src, src_inst_stride, y, a.key_mods, digits, Lk, key_ctx->logn, inst, s);
-        launch_ntt_forward(*key_ctx, RowMap{y, Lk, Lk, 0, 0}, inst * digits * Lk, true, s);
+        launch_lift_ntt_split(*key_ctx, src, src_inst_stride, y, digits, inst, s);
     }
     return y;
 }
@@ -241,7 +240,7 @@ void Ksk::key_switch_pb(const u64* src, size_t src_inst_stride, u64* acc, u32 in
                         hipStream_t s) const {
     const u32 Lk = key_ctx->L;
     const u64* y = lift(src, src_inst_stride, inst, s);
-    launch_mac_ew(y, digits, Lk, key0.as<ulonglong2>(), key1.as<ulonglong2>(), key_ctx->mod.as<ModInfo>(), acc,
+    launch_mac_ew(y, digits, Lk, 0, Lk, key0.as<ulonglong2>(), key1.as<ulonglong2>(), key_ctx->mod.as<ModInfo>(), acc,
                   key_ctx->logn, inst, s);
     if (to_power_basis) launch_ntt_backward(*key_ctx, RowMap{acc, Lk, Lk, 0, 0}, inst * 2 * Lk, s);
 }
@@ -300,17 +299,23 @@ void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* out, u3
         launch_ks_finish(ct_ctx->logn, f, inst, s);
         return;
     }
-    // Unfused: inner product of every key limb, inverse NTT of the dropped
-    // limb, power-basis part, forward NTT, NTT-domain combination.
-    u64* acc = scratch(Scratch::SwitchDown).get(s, (size_t)inst * 2 * (Lk + L) * n * 8).as<u64>();
-    u64* x = acc + (size_t)inst * 2 * Lk * n;
-    launch_mac_ew(y, digits, (u32)Lk, key0.as<ulonglong2>(), key1.as<ulonglong2>(), key_ctx->mod.as<ModInfo>(), acc,
-                  key_ctx->logn, inst, s);
-    if (extra == 1) launch_ntt_backward(*key_ctx, RowMap{acc, 1, (u32)Lk, (u32)(Lk - 1), (u32)(Lk - 1)}, inst * 2, s);
-    a.ks = acc;
-    launch_sd_prepare(ct_ctx->logn, a, x, inst, s);
+    // Unfused (degree 32768): inner product of the dropped limb and its
+    // inverse NTT, inner products of the other limbs into `out`, then the
+    // transformed power-basis part added in.
+    u64* base = scratch(Scratch::SwitchDown).get(s, (size_t)inst * 2 * (1 + L) * n * 8).as<u64>();
+    u64* special = nullptr;
+    u64* x = base + (size_t)inst * 2 * n;
+    if (extra == 1) {
+        special = base;
+        launch_mac_ew(y, digits, (u32)Lk, (u32)(Lk - 1), 1, key0.as<ulonglong2>(), key1.as<ulonglong2>(),
+                      key_ctx->mod.as<ModInfo>(), special, key_ctx->logn, inst, s);
+        launch_ntt_backward(*key_ctx, RowMap{special, 1, 1, 0, (u32)(Lk - 1)}, inst * 2, s);
+    }
+    KsFinishArgs f{a, y, fin0.as<u64>(), fin1.as<u64>(), digits, (u32)Lk, special};
+    launch_ks_mac(ct_ctx->logn, f, inst, s);
+    launch_sd_z(ct_ctx->logn, f, x, inst, s);
     launch_ntt_forward(*ct_ctx, RowMap{x, (u32)L, (u32)L, 0, 0}, inst * 2 * (u32)L, false, s);
-    launch_sd_combine(ct_ctx->logn, a, x, inst, s);
+    launch_add_rows(out, x, out, ct_ctx->mod.as<ModInfo>(), (u32)L, ct_ctx->logn, (size_t)inst * 2 * L, s);
 }
 
 // KeySwitchingKey::key_switch (key_switching_key.rs:214-241): power-basis p in

@@ -69,38 +69,18 @@ void launch_tensor_ew(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_
     HIPCHK(hipGetLastError());
 }
 
-// Digit lift (rq/mod.rs:518-540) before the lazy NTT: Y[b][i][j] =
-// lazy_reduce_{q'_j}(src[b][i]). grid = (N/256, digits * L', instances)
-__global__ __launch_bounds__(256) void lift_ew_kernel(const u64* __restrict__ src, size_t src_inst,
-                                                      u64* __restrict__ y, const ModInfo* __restrict__ key_mods,
-                                                      u32 key_limbs, u32 logn) {
-    const u32 e = blockIdx.x * 256 + threadIdx.x;
-    if (e >> logn) return;
-    const u32 j = blockIdx.y % key_limbs, i = blockIdx.y / key_limbs, b = blockIdx.z;
-    const ModInfo mi = key_mods[j];
-    const u64 v = src[b * src_inst + ((size_t)i << logn) + e];
-    y[(((size_t)b * gridDim.y + blockIdx.y) << logn) + e] = lazy_reduce_u64(v, mi.barrett_lo, mi.barrett_hi, mi.p);
-}
-
-void launch_lift_ew(const u64* src, size_t src_inst, u64* y, const ModInfo* key_mods, u32 digits, u32 key_limbs,
-                    int logn, u32 inst, hipStream_t s) {
-    if (!digits || !inst) return;
-    ProfScope ps("lift_ew", logn, s);
-    hipLaunchKernelGGL(lift_ew_kernel, ew_grid(1u << logn, digits * key_limbs, inst), dim3(256), 0, s, src, src_inst,
-                       y, key_mods, key_limbs, (u32)logn);
-    HIPCHK(hipGetLastError());
-}
-
-// Key inner product (key_switching_key.rs:227-239): acc[b][p][j] =
-// sum_i Y[b][i][j] * key_p[i][j] (Shoup, exact). grid = (N/256, L' * 2, instances)
-__global__ __launch_bounds__(256) void mac_ew_kernel(const u64* __restrict__ y, u32 digits, u32 key_limbs,
-                                                     const ulonglong2* __restrict__ key0,
+// Key inner product (key_switching_key.rs:227-239) of key limbs first ..
+// first + count - 1: acc[b][p][j - first] = sum_i Y[b][i][j] * key_p[i][j]
+// (Shoup, exact). grid = (N/256, count * 2, instances)
+__global__ __launch_bounds__(256) void mac_ew_kernel(const u64* __restrict__ y, u32 digits, u32 key_limbs, u32 first,
+                                                     u32 count, const ulonglong2* __restrict__ key0,
                                                      const ulonglong2* __restrict__ key1,
                                                      const ModInfo* __restrict__ key_mods, u64* __restrict__ acc,
                                                      u32 logn) {
     const u32 e = blockIdx.x * 256 + threadIdx.x;
     if (e >> logn) return;
-    const u32 j = blockIdx.y % key_limbs, part = blockIdx.y / key_limbs, b = blockIdx.z;
+    const u32 jj = blockIdx.y % count, part = blockIdx.y / count, b = blockIdx.z;
+    const u32 j = first + jj;
     const u64 p = key_mods[j].p;
     const ulonglong2* key = part ? key1 : key0;
     u64 r = 0;
@@ -109,33 +89,34 @@ __global__ __launch_bounds__(256) void mac_ew_kernel(const u64* __restrict__ y, 
         const ulonglong2 w = key[(((size_t)i * key_limbs + j) << logn) + e];
         r = add_mod(r, mul_shoup(v, w.x, w.y, p), p);
     }
-    acc[((((size_t)b * 2 + part) * key_limbs + j) << logn) + e] = r;
+    acc[((((size_t)b * 2 + part) * count + jj) << logn) + e] = r;
 }
 
-void launch_mac_ew(const u64* y, u32 digits, u32 key_limbs, const ulonglong2* key0, const ulonglong2* key1,
-                   const ModInfo* key_mods, u64* acc, int logn, u32 inst, hipStream_t s) {
-    if (!key_limbs || !inst) return;
+void launch_mac_ew(const u64* y, u32 digits, u32 key_limbs, u32 first, u32 count, const ulonglong2* key0,
+                   const ulonglong2* key1, const ModInfo* key_mods, u64* acc, int logn, u32 inst, hipStream_t s) {
+    if (!count || !inst) return;
     ProfScope ps("mac_ew", logn, s);
-    hipLaunchKernelGGL(mac_ew_kernel, ew_grid(1u << logn, 2 * key_limbs, inst), dim3(256), 0, s, y, digits,
-                       key_limbs, key0, key1, key_mods, acc, (u32)logn);
+    hipLaunchKernelGGL(mac_ew_kernel, ew_grid(1u << logn, 2 * count, inst), dim3(256), 0, s, y, digits, key_limbs,
+                       first, count, key0, key1, key_mods, acc, (u32)logn);
     HIPCHK(hipGetLastError());
 }
 
-// Power-basis part of the switch-down finish (see ks_finish_kernel):
-// x[b][p][i] = add_pb - q_l^-1 [y]_i, y from the power-basis dropped limb.
-// grid = (N/256, L * 2, instances)
-__global__ __launch_bounds__(256) void sd_prepare_kernel(SdNttArgs a, u64* __restrict__ x, u32 logn) {
+// Power-basis part of the key-switch finish (the z of ks_finish_kernel):
+// x[b][p][i] = add_pb - q_l^-1 [y]_i with y from the dropped limb in
+// power basis (`special` [inst][2][N]), or add_pb alone without an extra
+// modulus. grid = (N/256, L * 2, instances)
+__global__ __launch_bounds__(256) void sd_z_kernel(KsFinishArgs f, u64* __restrict__ x, u32 logn) {
+    const SdNttArgs& a = f.sd;
     const u32 e = blockIdx.x * 256 + threadIdx.x;
     if (e >> logn) return;
     const u32 i = blockIdx.y % a.L, part = blockIdx.y / a.L, b = blockIdx.z;
     const ModInfo mi = a.mods[i];
     const u64 p = mi.p;
-    const u64* ks = a.ks + (size_t)b * a.ks_inst_stride + (size_t)part * a.ks_part_stride;
     u64 z = 0;
     if (a.extra) {
         const SdConst cst = a.sd[i];
         const u64 ql = a.extra_mods[0];
-        u64 yv = ks[((size_t)a.L << logn) + e] + (ql >> 1);
+        u64 yv = f.special[(((size_t)b * 2 + part) << logn) + e] + (ql >> 1);
         yv = yv >= ql ? yv - ql : yv;
         const u64 tmp = lazy_reduce_u64(yv, mi.barrett_lo, mi.barrett_hi, p) + cst.half_mod;
         z = mul_shoup(3 * p - tmp, cst.inv, cst.inv_shoup, p);
@@ -146,41 +127,10 @@ __global__ __launch_bounds__(256) void sd_prepare_kernel(SdNttArgs a, u64* __res
     x[((((size_t)b * 2 + part) * a.L + i) << logn) + e] = z;
 }
 
-// out = NTT(x) + q_l^-1 ks_i (or ks_i without an extra modulus) + NTT-domain
-// addend. grid = (N/256, L * 2, instances)
-__global__ __launch_bounds__(256) void sd_combine_kernel(SdNttArgs a, const u64* __restrict__ xn, u32 logn) {
-    const u32 e = blockIdx.x * 256 + threadIdx.x;
-    if (e >> logn) return;
-    const u32 i = blockIdx.y % a.L, part = blockIdx.y / a.L, b = blockIdx.z;
-    const u64 p = a.mods[i].p;
-    const u64* ks = a.ks + (size_t)b * a.ks_inst_stride + (size_t)part * a.ks_part_stride;
-    const u64 kv = ks[((size_t)i << logn) + e];
-    u64 r = xn[((((size_t)b * 2 + part) * a.L + i) << logn) + e];
-    if (a.extra) {
-        const SdConst cst = a.sd[i];
-        r = add_mod(r, mul_shoup(kv, cst.inv, cst.inv_shoup, p), p);
-    } else {
-        r = add_mod(r, kv, p);
-    }
-    if (a.add_ntt && part < a.add_ntt_parts) {
-        const u64* g = a.add_ntt + (size_t)b * a.add_ntt_inst_stride + (size_t)part * a.add_ntt_part_stride +
-                       ((size_t)i << logn);
-        r = add_mod(r, g[a.perm ? a.perm[e] : e], p);
-    }
-    a.out[(size_t)b * a.out_inst_stride + (size_t)part * a.out_part_stride + ((size_t)i << logn) + e] = r;
-}
-
-void launch_sd_prepare(int logn, const SdNttArgs& a, u64* x, u32 inst, hipStream_t s) {
+void launch_sd_z(int logn, const KsFinishArgs& f, u64* x, u32 inst, hipStream_t s) {
     if (!inst) return;
-    ProfScope ps("sd_prepare", logn, s);
-    hipLaunchKernelGGL(sd_prepare_kernel, ew_grid(1u << logn, 2 * a.L, inst), dim3(256), 0, s, a, x, (u32)logn);
-    HIPCHK(hipGetLastError());
-}
-
-void launch_sd_combine(int logn, const SdNttArgs& a, const u64* xn, u32 inst, hipStream_t s) {
-    if (!inst) return;
-    ProfScope ps("sd_combine", logn, s);
-    hipLaunchKernelGGL(sd_combine_kernel, ew_grid(1u << logn, 2 * a.L, inst), dim3(256), 0, s, a, xn, (u32)logn);
+    ProfScope ps("sd_z", logn, s);
+    hipLaunchKernelGGL(sd_z_kernel, ew_grid(1u << logn, 2 * f.sd.L, inst), dim3(256), 0, s, f, x, (u32)logn);
     HIPCHK(hipGetLastError());
 }
 

@@ -475,14 +475,16 @@ void launch_ks_special(int logn, const KeySwitchArgs& a, const u64* y, const ulo
     FHEHIP_LOGN_DISPATCH(logn, ks_special_kernel, grid, 0, s, a, y, key_inv_tw, special);
 }
 
+void launch_ks_mac(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s) {
+    if (!f.sd.L || !inst) return;
+    ProfScope ps("ks_mac", logn, s);
+    hipLaunchKernelGGL(ks_mac_kernel, dim3(((1u << logn) + 255) / 256, f.sd.L, inst), dim3(256), 0, s, f, (u32)logn);
+    HIPCHK(hipGetLastError());
+}
+
 void launch_ks_finish(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s) {
     if (!f.sd.L || !inst) return;
-    {
-        ProfScope ps("ks_mac", logn, s);
-        hipLaunchKernelGGL(ks_mac_kernel, dim3(((1u << logn) + 255) / 256, f.sd.L, inst), dim3(256), 0, s, f,
-                           (u32)logn);
-        HIPCHK(hipGetLastError());
-    }
+    launch_ks_mac(logn, f, inst, s);
     ProfScope ps("ks_finish", logn, s);
     dim3 grid(inst * 2 * f.sd.L);
     FHEHIP_LOGN_DISPATCH(logn, ks_finish_kernel, grid, 0, s, f);

@@ -99,12 +99,13 @@ void launch_add_rows(const u64* a, const u64* b, u64* out, const ModInfo* mods, 
 void launch_gather_rows(const u64* src, size_t src_inst, u64* dst, size_t dst_inst, const u32* perm, int logn,
                         u32 rows, u32 inst, hipStream_t s);
 void launch_tensor_ew(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_t s);
-void launch_lift_ew(const u64* src, size_t src_inst, u64* y, const ModInfo* key_mods, u32 digits, u32 key_limbs,
-                    int logn, u32 inst, hipStream_t s);
-void launch_mac_ew(const u64* y, u32 digits, u32 key_limbs, const ulonglong2* key0, const ulonglong2* key1,
-                   const ModInfo* key_mods, u64* acc, int logn, u32 inst, hipStream_t s);
-void launch_sd_prepare(int logn, const SdNttArgs& a, u64* x, u32 inst, hipStream_t s);
-void launch_sd_combine(int logn, const SdNttArgs& a, const u64* xn, u32 inst, hipStream_t s);
+void launch_mac_ew(const u64* y, u32 digits, u32 key_limbs, u32 first, u32 count, const ulonglong2* key0,
+                   const ulonglong2* key1, const ModInfo* key_mods, u64* acc, int logn, u32 inst, hipStream_t s);
+// Key inner product of the limbs below the dropped one (+ NTT-domain addend)
+// into sd.out, any degree (ks_mac_kernel).
+void launch_ks_mac(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s);
+// Power-basis switch-down part x [inst][2][L][N] of the finish (unfused path).
+void launch_sd_z(int logn, const KsFinishArgs& f, u64* x, u32 inst, hipStream_t s);
 // Copies `count` slots of `elems` words; `table` is a device array of
 // (src, dst) pointer pairs.
 void launch_copy_slots(const u64* const* table, u32 count, size_t elems, hipStream_t s);

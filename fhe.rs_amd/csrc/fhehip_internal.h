@@ -77,5 +77,9 @@ struct RowMap {
 // NTT launchers (ntt_kernels.hip). `rows` = total rows in the launch.
 void launch_ntt_forward(const DeviceContext& c, RowMap m, u32 rows, bool lazy, hipStream_t s);
 void launch_ntt_backward(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s);
+// Degree 32768: y[b][i][j] = forward_vt_lazy(lazy_reduce_{q'_j}(src[b][i]))
+// for every digit i < digits and key limb j of `key`.
+void launch_lift_ntt_split(const DeviceContext& key, const u64* src, size_t src_inst, u64* y, u32 digits, u32 inst,
+                           hipStream_t s);
 
 }  // namespace fhehip

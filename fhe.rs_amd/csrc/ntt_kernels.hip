@@ -135,6 +135,45 @@ __global__ __launch_bounds__(256) void ntt_split_inv_kernel(RowMap m, const ulon
     row[j + 16384] = mul_shoup(x + 2 * p - y, z.x, z.y, p);
 }
 
+// N = 32768 digit lift (rq/mod.rs:518-540) fused with the first forward stage:
+// for digit i of instance b and key limb j, the two halves of
+// lazy_reduce_{q'_j}(src_i) go through the stage-0 butterfly into
+// y[b][i][j] (values < 4 q'_j); the half transforms follow.
+// grid = (16384 / 256, digits * key_limbs, instances)
+__global__ __launch_bounds__(256) void lift_split_kernel(const u64* __restrict__ src, size_t src_inst,
+                                                         u64* __restrict__ y, const ulonglong2* __restrict__ split,
+                                                         const ModInfo* __restrict__ mods, u32 key_limbs) {
+    const u32 k = blockIdx.x * 256 + threadIdx.x;
+    const u32 j = blockIdx.y % key_limbs, i = blockIdx.y / key_limbs, b = blockIdx.z;
+    const ModInfo mi = mods[j];
+    const u64 p = mi.p;
+    const u64* row = src + b * src_inst + ((size_t)i << 15);
+    u64 x = lazy_reduce_u64(row[k], mi.barrett_lo, mi.barrett_hi, p);
+    u64 z = lazy_reduce_u64(row[k + 16384], mi.barrett_lo, mi.barrett_hi, p);
+    const ulonglong2 w = split[3 * j];
+    fwd_bfly(x, z, w.x, w.y, p, 2 * p);
+    u64* out = y + (((size_t)b * gridDim.y + blockIdx.y) << 15);
+    out[k] = x;
+    out[k + 16384] = z;
+}
+
+void launch_lift_ntt_split(const DeviceContext& key, const u64* src, size_t src_inst, u64* y, u32 digits, u32 inst,
+                           hipStream_t s) {
+    if (key.logn != 15) throw Error(-2, "the fused lift and split stage is for degree 32768");
+    if (!digits || !inst) return;
+    {
+        ProfScope ps("lift_split", 15, s);
+        hipLaunchKernelGGL(lift_split_kernel, dim3(16384 / 256, digits * key.L, inst), dim3(256), 0, s, src, src_inst,
+                           y, key.split.as<ulonglong2>(), key.mod.as<ModInfo>(), key.L);
+        HIPCHK(hipGetLastError());
+    }
+    ProfScope ps("ntt_forward_lazy", 15, s);
+    RowMap m{y, key.L, key.L, 0, 0};
+    fwd_launch<14>(*key.half, RowMap{m.data, 2 * m.rows_per_poly, 2 * m.limbs_per_poly, 0, 0}, 2 * inst * digits * key.L,
+                   true, s);
+    HIPCHK(hipGetLastError());
+}
+
 static RowMap half_rows(RowMap m) {
     return RowMap{m.data, 2 * m.rows_per_poly, 2 * m.limbs_per_poly, 2 * m.first_limb, 2 * m.table_offset};
 }
