@@ -158,6 +158,11 @@ def ntt_microbench(fhehip, torch, reps):
     ctx.ntt_backward(d, count, stream=stream)
     torch.cuda.synchronize()
     exact = bool(torch.equal(d, ref))
+    # steady state: warm up for about as long as the timed region
+    for _ in range(reps):
+        ctx.ntt_forward(d, count, stream=stream)
+        ctx.ntt_backward(d, count, stream=stream)
+    torch.cuda.synchronize()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
     ev[0].record(stream)
     for _ in range(reps):
@@ -520,6 +525,7 @@ def main():
 
     # Per-call GPU drop-in: one operation per call, into its own output buffer.
     call_ms, per_call_value = [], None
+    per_call_graph = None
     if args.per_call_ops > 0:
         one_o = torch.empty(per, dtype=torch.int64, device=dev)
         for _ in range(3):
@@ -534,6 +540,21 @@ def main():
         torch.cuda.synchronize()
         call_ms = [pe[k].elapsed_time(pe[k + 1]) for k in range(args.per_call_ops)]
         per_call_value = args.per_call_ops / (pe[0].elapsed_time(pe[-1]) / 1e3)
+        # the same single-instance call captured once as a HIP graph and replayed
+        graph = fhehip.Graph(mult, lhs[:per], rhs[:per], one_o, 1)
+        for _ in range(3):
+            graph.launch(stream)
+        torch.cuda.synchronize()
+        ge = [torch.cuda.Event(enable_timing=True) for _ in range(args.per_call_ops + 1)]
+        ge[0].record(stream)
+        for k in range(args.per_call_ops):
+            graph.launch(stream)
+            ge[k + 1].record(stream)
+        torch.cuda.synchronize()
+        graph_ms = [ge[k].elapsed_time(ge[k + 1]) for k in range(args.per_call_ops)]
+        per_call_graph = {"value": args.per_call_ops / (ge[0].elapsed_time(ge[-1]) / 1e3), "unit": UNIT,
+                          "p50_ms": percentile(graph_ms, 50), "p99_ms": percentile(graph_ms, 99)}
+        graph.close()
 
     # Roofline of the dominant kernel.
     dom = max(kernels.items(), key=lambda kv: kv[1][1])
@@ -551,7 +572,7 @@ def main():
     extras = {}
     if not args.no_extras:
         extras["single_instance"] = single_instance_latency(fhehip, torch, 200)
-        extras["ntt_microbench"] = ntt_microbench(fhehip, torch, 10)
+        extras["ntt_microbench"] = ntt_microbench(fhehip, torch, 50)
         extras["uneven_batch"] = uneven_batch(fhehip, torch)
         extras["serving"] = serving(fhehip, torch)
     if args.copy_calibration:
@@ -589,6 +610,7 @@ def main():
         "p99_step_ms": percentile(step_ms, 99),
         "per_call": {"value": per_call_value, "unit": UNIT, "p50_ms": percentile(call_ms, 50),
                      "p99_ms": percentile(call_ms, 99), "ops": args.per_call_ops},
+        "per_call_graph": per_call_graph,
         "batched_over_per_call": (value / per_call_value) if per_call_value else None,
         "bit_exact_vs_oracle": bit_exact,
         "kernels_ms": {k: {"launches": c, "avg_ms": ms / c} for k, (c, ms) in sorted(kernels.items())},

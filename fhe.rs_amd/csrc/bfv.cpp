@@ -528,6 +528,15 @@ StreamScratch& scratch(Scratch k) {
     return pools[(int)k];
 }
 
+void release_scratch(hipStream_t s) {
+    for (int k = 0; k < (int)Scratch::Count; k++) scratch((Scratch)k).release(s);
+}
+
+void StreamScratch::release(hipStream_t s) const {
+    std::lock_guard<std::mutex> g(m_);
+    ws_.erase(s);
+}
+
 Workspace& StreamScratch::get(hipStream_t s, size_t bytes) const {
     std::lock_guard<std::mutex> g(m_);
     auto& w = ws_[s];

@@ -25,6 +25,9 @@ class StreamScratch {
     // Buffer of at least `bytes` for work on stream `s`.
     Workspace& get(hipStream_t s, size_t bytes) const;
 
+    // Frees the buffer of stream `s` (the stream's work must be complete).
+    void release(hipStream_t s) const;
+
   private:
     mutable std::mutex m_;
     mutable std::map<hipStream_t, std::unique_ptr<Workspace>> ws_;
@@ -36,6 +39,8 @@ class StreamScratch {
 // different pools.
 enum class Scratch { Scale, RelinC2, Lift, SwitchDown, Mul, GaloisC2, EvalTmp, Count };
 StreamScratch& scratch(Scratch k);
+// Frees every pool's buffer of stream `s`.
+void release_scratch(hipStream_t s);
 
 std::vector<u64> scaler_blob(const ScalerConsts& c, const std::vector<ModInfo>& to_mods, u32 start);
 

@@ -6,6 +6,7 @@
 
 #include "bfv.h"
 #include "capi_internal.h"
+#include "graph.h"
 #include "jobs.h"
 #include "server.h"
 
@@ -28,6 +29,9 @@ struct fhehip_evaluation_key {
 };
 struct fhehip_server {
     std::unique_ptr<Server> s;
+};
+struct fhehip_graph {
+    std::unique_ptr<Graph> g;
 };
 
 static BigUint factor_num(const fhehip_scaling_factor* f) {
@@ -232,6 +236,56 @@ int fhehip_ciphertext_add(const fhehip_context* ctx, const uint64_t* d_a, const 
     launch_add_rows(d_a, d_b, d_out, c.mod.as<ModInfo>(), c.L, c.logn, (size_t)num_polys * c.L, S(stream));
     FHEHIP_API_END
 }
+
+int fhehip_multiply_graph(fhehip_multiplicator* m, const uint64_t* d_lhs, const uint64_t* d_rhs, uint64_t* d_out,
+                          uint32_t num, fhehip_graph** out) {
+    FHEHIP_API_BEGIN
+    if (!m || !out || !d_lhs || !d_rhs || !d_out || !num) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    Multiplicator* mm = m->m.get();
+    auto g = capture_graph(std::unique_ptr<Graph>(new Graph),
+                           [&](hipStream_t s) { mm->multiply(d_lhs, d_rhs, d_out, num, s); });
+    *out = new fhehip_graph{std::move(g)};
+    FHEHIP_API_END
+}
+
+int fhehip_multiply_sum_graph(fhehip_multiplicator* m, fhehip_evaluation_key* ek, uint32_t log2_slots, int rows,
+                              const uint64_t* d_lhs, const uint64_t* d_rhs, uint64_t* d_out, uint32_t num,
+                              fhehip_graph** out) {
+    FHEHIP_API_BEGIN
+    if (!m || !ek || !out || !d_lhs || !d_rhs || !d_out || !num) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (!m->m->rk) return fail(FHEHIP_ERR_INVALID_ARG, "the multiplicator has no relinearization key");
+    EvaluationKey* e = ek->ek.get();
+    const auto& b = *m->m->base;
+    if (e->degree != b.N || e->keys[0]->ksk->ct_ctx->moduli != b.moduli)
+        return fail(FHEHIP_ERR_CTX_MISMATCH, "evaluation key and multiplicator use different ciphertext contexts");
+    if ((1ull << log2_slots) > b.N / 2) return fail(FHEHIP_ERR_INVALID_ARG, "log2_slots too large");
+    std::vector<u32> ex;
+    for (u32 k = 0; k < log2_slots; k++) ex.push_back(e->column_exponent(1u << k));
+    if (rows) ex.push_back(2 * e->degree - 1);
+    for (u32 x : ex)
+        if (!e->find(x)) return fail(FHEHIP_ERR_INVALID_ARG, "the evaluation key lacks a rotation of the sum");
+    std::unique_ptr<Graph> g(new Graph);
+    g->tmp.reset((size_t)num * 2 * b.L * b.N * 8);
+    u64* prod = g->tmp.as<u64>();
+    Multiplicator* mm = m->m.get();
+    g = capture_graph(std::move(g), [&](hipStream_t s) {
+        mm->multiply(d_lhs, d_rhs, prod, num, s);
+        e->rotate_and_sum(ex, prod, d_out, num, s);
+    });
+    *out = new fhehip_graph{std::move(g)};
+    FHEHIP_API_END
+}
+
+int fhehip_graph_launch(fhehip_graph* g, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!g) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    g->g->launch(S(stream));
+    FHEHIP_API_END
+}
+
+void fhehip_graph_destroy(fhehip_graph* g) { delete g; }
 
 int fhehip_submit(const fhehip_job* jobs, uint32_t n, void* stream) {
     FHEHIP_API_BEGIN

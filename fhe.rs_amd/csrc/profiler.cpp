@@ -42,7 +42,12 @@ void drain_locked() {
 }
 }  // namespace
 
-bool profiler_on() { return g_on.load(std::memory_order_relaxed); }
+thread_local int t_capturing = 0;
+
+CaptureGuard::CaptureGuard() { t_capturing++; }
+CaptureGuard::~CaptureGuard() { t_capturing--; }
+
+bool profiler_on() { return !t_capturing && g_on.load(std::memory_order_relaxed); }
 
 hipEvent_t profiler_event() {
     std::lock_guard<std::mutex> lk(g_mu);

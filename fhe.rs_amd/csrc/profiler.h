@@ -8,6 +8,13 @@
 namespace fhehip {
 
 bool profiler_on();
+
+// While a CaptureGuard lives on a thread, launches on that thread record no
+// profiling events (they are being captured into a HIP graph).
+struct CaptureGuard {
+    CaptureGuard();
+    ~CaptureGuard();
+};
 void profiler_record(const char* name, int logn, hipEvent_t start, hipEvent_t stop);
 hipEvent_t profiler_event();
 

@@ -57,6 +57,10 @@ SIGNATURES = {
     "fhehip_inner_sum": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp]),
     "fhehip_rotate_and_sum": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_int, C.c_uint32, _vp]),
     "fhehip_ciphertext_add": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_multiply_graph": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vpp]),
+    "fhehip_multiply_sum_graph": (C.c_int, [_vp, _vp, C.c_uint32, C.c_int, _vp, _vp, _vp, C.c_uint32, _vpp]),
+    "fhehip_graph_launch": (C.c_int, [_vp, _vp]),
+    "fhehip_graph_destroy": (None, [_vp]),
     "fhehip_submit": (C.c_int, [_vp, C.c_uint32, _vp]),
     "fhehip_generate_moduli": (C.c_int, [_u32p, C.c_uint32, C.c_uint32, _u64p]),
     "fhehip_server_create": (C.c_int, [_vp, _vpp]),
@@ -314,6 +318,34 @@ class EvaluationKey:
 
 def ciphertext_add(ctx, d_a, d_b, d_out, num_polys, stream=None):
     check(lib().fhehip_ciphertext_add(ctx.h, dptr(d_a), dptr(d_b), dptr(d_out), num_polys, stream_ptr(stream)))
+
+
+class Graph:
+    """A captured HIP graph: multiply (and optionally the slot sum) on fixed buffers."""
+
+    def __init__(self, mult, d_lhs, d_rhs, d_out, num, ek=None, log2_slots=None, rows=False):
+        h = C.c_void_p()
+        if ek is None:
+            check(lib().fhehip_multiply_graph(mult.h, dptr(d_lhs), dptr(d_rhs), dptr(d_out), num, C.byref(h)))
+        else:
+            check(lib().fhehip_multiply_sum_graph(mult.h, ek.h, log2_slots, 1 if rows else 0, dptr(d_lhs),
+                                                  dptr(d_rhs), dptr(d_out), num, C.byref(h)))
+        self.h = h
+        self._keep = (mult, ek, d_lhs, d_rhs, d_out)
+
+    def launch(self, stream=None):
+        check(lib().fhehip_graph_launch(self.h, stream_ptr(stream)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fhehip_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 JOB_MAX_ROTATIONS = 4

@@ -171,6 +171,25 @@ int fhehip_rotate_and_sum(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint6
 int fhehip_ciphertext_add(const fhehip_context* ctx, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out,
                           uint32_t num_polys, void* stream);
 
+/* ---- HIP graphs ---------------------------------------------------------------
+ * No fhe.rs counterpart: a fixed sequence of launches captured once and
+ * replayed with one call. Capture runs the sequence once eagerly on a
+ * private stream (sizing its scratch), then records it. Buffers and handles
+ * are fixed at capture and must outlive the graph; replays of one graph must
+ * not overlap each other. */
+typedef struct fhehip_graph fhehip_graph;
+/* fhehip_multiply(m, lhs, rhs, out, num) as a graph. */
+int fhehip_multiply_graph(fhehip_multiplicator* m, const uint64_t* d_lhs, const uint64_t* d_rhs, uint64_t* d_out,
+                          uint32_t num, fhehip_graph** out);
+/* fhehip_multiply followed by fhehip_rotate_and_sum(ek, .., log2_slots, rows)
+ * (the request chain of the server) as a graph. */
+int fhehip_multiply_sum_graph(fhehip_multiplicator* m, fhehip_evaluation_key* ek, uint32_t log2_slots, int rows,
+                              const uint64_t* d_lhs, const uint64_t* d_rhs, uint64_t* d_out, uint32_t num,
+                              fhehip_graph** out);
+int fhehip_graph_launch(fhehip_graph* g, void* stream);
+/* Waits for the device to be idle, then frees the graph and its scratch. */
+void fhehip_graph_destroy(fhehip_graph* g);
+
 /* ---- Batches of mixed parameters ----------------------------------------------
  * No single fhe.rs call: independent evaluations, each Multiplicator::multiply
  * (mul.rs:158-227) with relinearization followed by up to four
