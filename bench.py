@@ -485,6 +485,12 @@ def main():
     ksk = fhehip.KeySwitchingKey(cctx, kctx, c0, c1)
     mult = fhehip.Multiplicator(rk=ksk, plaintext=T_PLAIN)
     M = len(mult.extended_basis())
+    # Autotuned shape of the plain NTT kernels at this degree (16 or 32 values
+    # per thread, per direction), timed on 2048 rows of the multiplication context.
+    tune_f, tune_i, tune_ms = fhehip.Context(mult.extended_basis(), N).ntt_autotune(2048)
+    ntt_tuning = {"forward_log2_values_per_thread": tune_f, "inverse_log2_values_per_thread": tune_i,
+                  "ms_per_launch": {"fwd16": tune_ms[0], "fwd32": tune_ms[1], "inv16": tune_ms[2],
+                                    "inv32": tune_ms[3]}, "rows": 2048}
 
     # Seeded uniform ciphertext residues, generated on the device.
     g = torch.Generator(device=dev)
@@ -614,6 +620,7 @@ def main():
         "batched_over_per_call": (value / per_call_value) if per_call_value else None,
         "bit_exact_vs_oracle": bit_exact,
         "kernels_ms": {k: {"launches": c, "avg_ms": ms / c} for k, (c, ms) in sorted(kernels.items())},
+        "ntt_autotune": ntt_tuning,
         "roofline": roofline,
         "cpu_baseline": cpu,
         **extras,

@@ -67,4 +67,24 @@ int fhehip_ntt_backward(const fhehip_context* ctx, uint64_t* d_polys, uint32_t n
     FHEHIP_API_END
 }
 
+int fhehip_ntt_set_shape(uint32_t degree, uint32_t fwd, uint32_t inv) {
+    FHEHIP_API_BEGIN
+    if (degree < 8 || (degree & (degree - 1))) return fail(FHEHIP_ERR_INVALID_ARG, "degree is not a power of two >= 8");
+    set_ntt_shape(31 - __builtin_clz(degree), (int)fwd, (int)inv);
+    FHEHIP_API_END
+}
+
+int fhehip_ntt_autotune(const fhehip_context* ctx, uint32_t rows, uint32_t* fwd, uint32_t* inv, float* ms_out) {
+    FHEHIP_API_BEGIN
+    if (!ctx || !fwd || !inv || !rows) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    int rf = 4, ri = 4;
+    float ms[4];
+    autotune_ntt(*ctx->c, rows, rf, ri, ms);
+    *fwd = (uint32_t)rf;
+    *inv = (uint32_t)ri;
+    if (ms_out)
+        for (int k = 0; k < 4; k++) ms_out[k] = ms[k];
+    FHEHIP_API_END
+}
+
 }  // extern "C"

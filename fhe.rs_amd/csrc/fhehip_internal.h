@@ -77,6 +77,10 @@ struct RowMap {
 // NTT launchers (ntt_kernels.hip). `rows` = total rows in the launch.
 void launch_ntt_forward(const DeviceContext& c, RowMap m, u32 rows, bool lazy, hipStream_t s);
 void launch_ntt_backward(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s);
+// Picks the faster of 16 and 32 values per thread for this degree's plain
+// NTT kernels by timing `rows` rows each way (ms: fwd16, fwd32, inv16, inv32).
+void autotune_ntt(const DeviceContext& c, u32 rows, int& re_fwd, int& re_inv, float (&ms)[4]);
+void set_ntt_shape(int logn, int re_fwd, int re_inv);
 // Degree 32768: y[b][i][j] = forward_vt_lazy(lazy_reduce_{q'_j}(src[b][i]))
 // for every digit i < digits and key limb j of `key`.
 void launch_lift_ntt_split(const DeviceContext& key, const u64* src, size_t src_inst, u64* y, u32 digits, u32 inst,

@@ -1,4 +1,6 @@
 // Batched negacyclic NTT kernels: one workgroup per polynomial row.
+#include <atomic>
+
 #include "fhehip_internal.h"
 #include "ntt_device.h"
 #include "profiler.h"
@@ -51,14 +53,23 @@ __global__ __launch_bounds__((NttShape<LOGN, R>::T)) void ntt_inv_kernel(RowMap 
     for (int k = 0; k < S::E; k++) row[inv_index<LOGN, last_start<LOGN, R>(), R>(t, k)] = x[k];
 }
 
-// Elements per thread of the plain NTT kernels: 2^4 by default; the
-// environment variable FHEHIP_NTT_RE=5 selects 2^5 (half the threads).
-static int ntt_re() {
+// Values per thread (log2) of the plain NTT kernels, per degree and
+// direction: 4 (16 values, N/16 threads) until fhehip_ntt_autotune picks the
+// faster of 4 and 5 by timing both; the environment variable FHEHIP_NTT_RE=5
+// forces 5.
+static std::atomic<int> g_re_fwd[16], g_re_inv[16];
+
+static int forced_re() {
     static int re = [] {
         const char* e = getenv("FHEHIP_NTT_RE");
-        return (e && atoi(e) == 5) ? 5 : 4;
+        return (e && atoi(e) == 5) ? 5 : 0;
     }();
     return re;
+}
+static int ntt_re(int logn, bool forward) {
+    if (forced_re()) return forced_re();
+    const int r = (forward ? g_re_fwd : g_re_inv)[logn].load(std::memory_order_relaxed);
+    return r ? r : 4;
 }
 
 template <int LOGN>
@@ -66,7 +77,7 @@ static void fwd_launch(const DeviceContext& c, RowMap m, u32 rows, bool lazy, hi
     dim3 grid(rows);
     const ulonglong2* tw = c.fwd_tw.as<ulonglong2>();
     const ModInfo* md = c.mod.as<ModInfo>();
-    if (ntt_re() == 5) {
+    if (ntt_re(LOGN, true) == 5) {
         dim3 block(NttShape<LOGN, 5>::T);
         if (lazy) hipLaunchKernelGGL((ntt_fwd_kernel<LOGN, true, 5>), grid, block, 0, s, m, tw, md);
         else hipLaunchKernelGGL((ntt_fwd_kernel<LOGN, false, 5>), grid, block, 0, s, m, tw, md);
@@ -82,7 +93,7 @@ static void inv_launch(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s
     dim3 grid(rows);
     const ulonglong2* tw = c.inv_tw.as<ulonglong2>();
     const ModInfo* md = c.mod.as<ModInfo>();
-    if (ntt_re() == 5)
+    if (ntt_re(LOGN, false) == 5)
         hipLaunchKernelGGL((ntt_inv_kernel<LOGN, 5>), grid, dim3(NttShape<LOGN, 5>::T), 0, s, m, tw, md);
     else
         hipLaunchKernelGGL((ntt_inv_kernel<LOGN, 4>), grid, dim3(NttShape<LOGN, 4>::T), 0, s, m, tw, md);
@@ -204,6 +215,80 @@ void launch_ntt_backward(const DeviceContext& c, RowMap m, u32 rows, hipStream_t
         FHEHIP_LOGN_SWITCH(c.logn, inv_launch, c, m, rows, s);
     }
     HIPCHK(hipGetLastError());
+}
+
+// Times forward and inverse transforms of `rows` rows of `c` (the half
+// transforms at degree 32768) with 16 and 32 values per thread and keeps the
+// faster shape per direction for this degree. Returns the chosen log2 values
+// per thread and the measured milliseconds per launch [fwd16, fwd32, inv16, inv32].
+void autotune_ntt(const DeviceContext& c, u32 rows, int& re_fwd, int& re_inv, float (&ms)[4]) {
+    const DeviceContext& t = c.logn == 15 ? *c.half : c;
+    const u32 trows = c.logn == 15 ? 2 * rows : rows;
+    const int logn = t.logn;
+    if (logn < 5) {  // one shape only
+        re_fwd = re_inv = 4;
+        for (float& m : ms) m = 0;
+        return;
+    }
+    DevBuf buf((size_t)trows * t.N * 8);
+    hipStream_t s;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    try {
+        HIPCHK(hipMemsetAsync(buf.ptr, 0, buf.bytes, s));  // zero rows: valid residues
+        const RowMap m{buf.as<u64>(), t.L, t.L, 0, 0};
+        const int reps = 5;
+        for (int dir = 0; dir < 2; dir++) {
+            for (int k = 0; k < 2; k++) {
+                const int re = 4 + k;
+                auto& slot = (dir == 0 ? g_re_fwd : g_re_inv)[logn];
+                const int saved = slot.load();
+                slot.store(re);
+                auto run = [&] {
+                    if (dir == 0) {
+                        FHEHIP_LOGN_SWITCH(logn, fwd_launch, t, m, trows, false, s);
+                    } else {
+                        FHEHIP_LOGN_SWITCH(logn, inv_launch, t, m, trows, s);
+                    }
+                };
+                run();
+                run();
+                HIPCHK(hipEventRecord(e0, s));
+                for (int r = 0; r < reps; r++) run();
+                HIPCHK(hipEventRecord(e1, s));
+                HIPCHK(hipEventSynchronize(e1));
+                HIPCHK(hipGetLastError());
+                float el = 0;
+                HIPCHK(hipEventElapsedTime(&el, e0, e1));
+                ms[2 * dir + k] = el / reps;
+                slot.store(saved);
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+        throw;
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+    re_fwd = ms[1] < ms[0] ? 5 : 4;
+    re_inv = ms[3] < ms[2] ? 5 : 4;
+    g_re_fwd[logn].store(re_fwd);
+    g_re_inv[logn].store(re_inv);
+}
+
+void set_ntt_shape(int logn, int re_fwd, int re_inv) {
+    if (logn == 15) logn = 14;  // the half transforms
+    if (logn < 3 || logn > 14 || (re_fwd != 4 && re_fwd != 5) || (re_inv != 4 && re_inv != 5))
+        throw Error(-1, "NTT shape: degree 8..32768, 4 or 5 log2 values per thread");
+    g_re_fwd[logn].store(re_fwd);
+    g_re_inv[logn].store(re_inv);
 }
 
 }  // namespace fhehip

@@ -34,6 +34,8 @@ SIGNATURES = {
     "fhehip_context_psi": (C.c_int, [_vp, _u64p]),
     "fhehip_ntt_forward": (C.c_int, [_vp, _vp, C.c_uint32, C.c_int, _vp]),
     "fhehip_ntt_backward": (C.c_int, [_vp, _vp, C.c_uint32, _vp]),
+    "fhehip_ntt_autotune": (C.c_int, [_vp, C.c_uint32, _u32p, _u32p, C.POINTER(C.c_float)]),
+    "fhehip_ntt_set_shape": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32]),
     "fhehip_scaler_create": (C.c_int, [_vp, _vp, _vp, _vpp]),
     "fhehip_scaler_destroy": (None, [_vp]),
     "fhehip_scaler_scale": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_int, _vp]),
@@ -170,6 +172,14 @@ class Context:
 
     def ntt_forward(self, polys, num_polys, lazy=False, stream=None):
         check(lib().fhehip_ntt_forward(self.h, dptr(polys), num_polys, 1 if lazy else 0, stream_ptr(stream)))
+
+    def ntt_autotune(self, rows):
+        """Times both NTT shapes for this degree and keeps the faster per
+        direction; returns (fwd_log2_values, inv_log2_values, [ms fwd16, fwd32, inv16, inv32])."""
+        f, i = C.c_uint32(), C.c_uint32()
+        ms = (C.c_float * 4)()
+        check(lib().fhehip_ntt_autotune(self.h, rows, C.byref(f), C.byref(i), ms))
+        return f.value, i.value, [float(x) for x in ms]
 
     def ntt_backward(self, polys, num_polys, stream=None):
         check(lib().fhehip_ntt_backward(self.h, dptr(polys), num_polys, stream_ptr(stream)))
@@ -314,6 +324,11 @@ class EvaluationKey:
     def rotate_and_sum(self, d_ct, d_out, log2_slots, rows, num, stream=None):
         check(lib().fhehip_rotate_and_sum(self.h, dptr(d_ct), dptr(d_out), log2_slots, 1 if rows else 0, num,
                                           stream_ptr(stream)))
+
+
+def ntt_set_shape(degree, fwd, inv):
+    """log2 values per thread (4 or 5) of the plain NTT kernels for `degree`."""
+    check(lib().fhehip_ntt_set_shape(degree, fwd, inv))
 
 
 def ciphertext_add(ctx, d_a, d_b, d_out, num_polys, stream=None):

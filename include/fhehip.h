@@ -73,6 +73,15 @@ int fhehip_ntt_forward(const fhehip_context* ctx, uint64_t* d_polys, uint32_t nu
 /* In-place inverse NTT (Poly::into_power_basis, rq/mod.rs:305-313;
  * NttOperator::backward, native.rs:106-132). Inputs must be < 2p. */
 int fhehip_ntt_backward(const fhehip_context* ctx, uint64_t* d_polys, uint32_t num_polys, void* stream);
+/* No fhe.rs counterpart: times this degree's forward and inverse transforms
+ * of `rows` rows with 16 and 32 values per thread and keeps the faster shape
+ * per direction for all later transforms of the degree (process-wide).
+ * log2 values per thread chosen -> *fwd, *inv; ms_out (may be NULL) receives
+ * milliseconds per launch {fwd16, fwd32, inv16, inv32}. Synchronous. */
+int fhehip_ntt_autotune(const fhehip_context* ctx, uint32_t rows, uint32_t* fwd, uint32_t* inv, float* ms_out);
+/* Sets the shape (4 or 5 log2 values per thread) of the plain NTT kernels for
+ * one degree directly, as the autotuner does. */
+int fhehip_ntt_set_shape(uint32_t degree, uint32_t fwd, uint32_t inv);
 
 /* ---- Scaling (RNS base extension and t/Q rescaling) ----------------------
  * A scaling factor numerator/denominator as little-endian 32-bit words

@@ -109,3 +109,27 @@ def test_ntt_32768_split_rows(fh, oracle):
     assert np.array_equal(fh.to_host(d), octx.forward(a))
     ctx.ntt_backward(d, 2)
     assert np.array_equal(fh.to_host(d), a)
+
+
+@pytest.mark.parametrize("n", [1024, 16384, 32768])
+def test_ntt_shapes_and_autotune(fh, oracle, n):
+    """Both kernel shapes (16 and 32 values per thread) are bit-exact; the
+    autotuner picks one of them per direction."""
+    moduli = oracle.generate_moduli([62, 60], n)
+    ctx = fh.Context(moduli, n)
+    octx = oracle.Context(moduli, n)
+    a = oracle.random_polys(moduli, n, 3, seed=n + 1)
+    ref = octx.forward(a)
+    for fwd, inv in [(4, 5), (5, 4), (5, 5), (4, 4)]:
+        fh.ntt_set_shape(n, fwd, inv)
+        d = fh.to_device(a)
+        ctx.ntt_forward(d, 3)
+        assert np.array_equal(fh.to_host(d), ref), (fwd, inv)
+        ctx.ntt_backward(d, 3)
+        assert np.array_equal(fh.to_host(d), a), (fwd, inv)
+    f, i, ms = ctx.ntt_autotune(64)
+    assert f in (4, 5) and i in (4, 5) and all(x >= 0 for x in ms)
+    d = fh.to_device(a)
+    ctx.ntt_forward(d, 3)
+    assert np.array_equal(fh.to_host(d), ref)
+    fh.ntt_set_shape(n, 4, 4)
