@@ -450,7 +450,8 @@ def main():
     ap.add_argument("--per-call-ops", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-extras", action="store_true", help="skip the single-instance and NTT measurements")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the NTT autotuning and the secondary workloads (single instance, NTT, uneven, serving)")
     ap.add_argument("--only", choices=["uneven", "serve"], default=None,
                     help="run only this secondary workload and print its JSON object")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -487,10 +488,13 @@ def main():
     M = len(mult.extended_basis())
     # Autotuned shape of the plain NTT kernels at this degree (16 or 32 values
     # per thread, per direction), timed on 2048 rows of the multiplication context.
-    tune_f, tune_i, tune_ms = fhehip.Context(mult.extended_basis(), N).ntt_autotune(2048)
-    ntt_tuning = {"forward_log2_values_per_thread": tune_f, "inverse_log2_values_per_thread": tune_i,
-                  "ms_per_launch": {"fwd16": tune_ms[0], "fwd32": tune_ms[1], "inv16": tune_ms[2],
-                                    "inv32": tune_ms[3]}, "rows": 2048}
+    # (Skipped with --no-extras, so that profiled runs hold only the pipeline's launches.)
+    ntt_tuning = None
+    if not args.no_extras:
+        tune_f, tune_i, tune_ms = fhehip.Context(mult.extended_basis(), N).ntt_autotune(2048)
+        ntt_tuning = {"forward_log2_values_per_thread": tune_f, "inverse_log2_values_per_thread": tune_i,
+                      "ms_per_launch": {"fwd16": tune_ms[0], "fwd32": tune_ms[1], "inv16": tune_ms[2],
+                                        "inv32": tune_ms[3]}, "rows": 2048}
 
     # Seeded uniform ciphertext residues, generated on the device.
     g = torch.Generator(device=dev)
