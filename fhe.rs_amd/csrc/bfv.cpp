@@ -51,6 +51,12 @@ std::vector<u64> scaler_blob(const ScalerConsts& c, const std::vector<ModInfo>& 
 }
 
 // ---------------------------------------------------------------------------
+// Batched launches index instances with the third grid dimension.
+static void check_batch(u32 inst) {
+    if (inst > 65535) throw Error(-1, "at most 65535 instances (or polynomials) per call");
+}
+
+// ---------------------------------------------------------------------------
 // Rows of up to 16384 coefficients are transformed inside one workgroup's LDS
 // together with the elementwise work around the transform; longer rows
 // (N = 32768) run the unfused pipeline: elementwise kernels around the
@@ -103,6 +109,7 @@ PolyScaler::PolyScaler(std::shared_ptr<DeviceContext> f, std::shared_ptr<DeviceC
 
 // rq::Scaler::scale (math/src/rq/scaler.rs:52-126) on `polys` polynomials.
 void PolyScaler::scale(const u64* in, u64* out, u32 polys, bool ntt_repr, hipStream_t s) {
+    check_batch(polys);
     const size_t n = from->N;
     const size_t isz = (size_t)from->L * n, osz = (size_t)to->L * n;
     if (!polys) return;
@@ -190,13 +197,14 @@ Ksk::Ksk(std::shared_ptr<DeviceContext> ct, std::shared_ptr<DeviceContext> key, 
     HIPCHK(hipMemcpy(sd_consts.ptr, sd.data(), sd_consts.bytes, hipMemcpyHostToDevice));
     if (extra <= 1) {
         // key_d,i * q_l^-1 mod q_i: the switch-down factor of limb i applied to
-        // the key instead of to every key-switch output.
-        std::vector<u64> f0((size_t)digits * L * n), f1(f0.size());
+        // the key instead of to every key-switch output; the dropped limb's
+        // rows (i == L) stay plain.
+        std::vector<u64> f0((size_t)digits * Lk * n), f1(f0.size());
         for (u32 d = 0; d < digits; d++)
-            for (u32 i = 0; i < L; i++) {
+            for (u32 i = 0; i < Lk; i++) {
                 const u64 q = key_ctx->moduli[i];
-                const u64 inv = extra ? sd[i].inv : 1;
-                const size_t src = ((size_t)d * Lk + i) * n, dst = ((size_t)d * L + i) * n;
+                const u64 inv = i < L && extra ? sd[i].inv : 1;
+                const size_t src = ((size_t)d * Lk + i) * n, dst = src;
                 for (u32 j = 0; j < n; j++) {
                     f0[dst + j] = mulmod64(c0[src + j], inv, q);
                     f1[dst + j] = mulmod64(c1[src + j], inv, q);
@@ -238,6 +246,7 @@ u64* Ksk::lift(const u64* src, size_t src_inst_stride, u32 inst, hipStream_t s) 
 
 void Ksk::key_switch_pb(const u64* src, size_t src_inst_stride, u64* acc, u32 inst, bool to_power_basis,
                         hipStream_t s) const {
+    check_batch(inst);
     const u32 Lk = key_ctx->L;
     const u64* y = lift(src, src_inst_stride, inst, s);
     launch_mac_ew(y, digits, Lk, 0, Lk, key0.as<ulonglong2>(), key1.as<ulonglong2>(), key_ctx->mod.as<ModInfo>(), acc,
@@ -275,6 +284,7 @@ SdNttArgs Ksk::sd_args(u64* out, const u64* add_pb, size_t add_inst, size_t add_
 void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* out, u32 inst, const u64* add_pb,
                             size_t add_inst, size_t add_part, const u64* add_ntt, size_t add_ntt_inst,
                             size_t add_ntt_part, u32 add_ntt_parts, const u32* perm, hipStream_t s) const {
+    check_batch(inst);
     const size_t n = ct_ctx->N, Lk = key_ctx->L, L = ct_ctx->L;
     SdNttArgs a = sd_args(out, add_pb, add_inst, add_part, add_ntt, add_ntt_inst, add_ntt_part, add_ntt_parts, perm);
     if (extra > 1) {
@@ -287,32 +297,22 @@ void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* out, u3
         return;
     }
     const u64* y = lift(src, src_inst_stride, inst, s);
-    if (fused(*ct_ctx)) {
-        u64* special = nullptr;
-        if (extra == 1) {
-            // Only the dropped limb goes back to power basis.
-            special = scratch(Scratch::SwitchDown).get(s, (size_t)inst * 2 * n * 8).as<u64>();
-            launch_ks_special(key_ctx->logn, args(src, src_inst_stride), y, key_ctx->inv_tw.as<ulonglong2>(), special,
-                              inst, s);
-        }
-        KsFinishArgs f{a, y, fin0.as<u64>(), fin1.as<u64>(), digits, (u32)Lk, special};
+    // Inner products of every key limb: the limbs below the dropped one into
+    // `out` (NTT domain), the dropped limb into `special`, which alone goes
+    // back to power basis; then the transformed power-basis part is added in.
+    const bool fuse = fused(*ct_ctx);
+    u64* base = scratch(Scratch::SwitchDown).get(s, (size_t)inst * 2 * (extra + (fuse ? 0 : L)) * n * 8).as<u64>();
+    u64* special = extra ? base : nullptr;
+    KsFinishArgs f{a, y, fin0.as<u64>(), fin1.as<u64>(), digits, (u32)Lk, special,
+                   key_ctx->mod.as<ModInfo>() + (Lk - 1)};
+    launch_ks_mac(ct_ctx->logn, f, inst, s);
+    if (extra) launch_ntt_backward(*key_ctx, RowMap{special, 1, 1, 0, (u32)(Lk - 1)}, inst * 2, s);
+    if (fuse) {
         launch_ks_finish(ct_ctx->logn, f, inst, s);
         return;
     }
-    // Unfused (degree 32768): inner product of the dropped limb and its
-    // inverse NTT, inner products of the other limbs into `out`, then the
-    // transformed power-basis part added in.
-    u64* base = scratch(Scratch::SwitchDown).get(s, (size_t)inst * 2 * (1 + L) * n * 8).as<u64>();
-    u64* special = nullptr;
-    u64* x = base + (size_t)inst * 2 * n;
-    if (extra == 1) {
-        special = base;
-        launch_mac_ew(y, digits, (u32)Lk, (u32)(Lk - 1), 1, key0.as<ulonglong2>(), key1.as<ulonglong2>(),
-                      key_ctx->mod.as<ModInfo>(), special, key_ctx->logn, inst, s);
-        launch_ntt_backward(*key_ctx, RowMap{special, 1, 1, 0, (u32)(Lk - 1)}, inst * 2, s);
-    }
-    KsFinishArgs f{a, y, fin0.as<u64>(), fin1.as<u64>(), digits, (u32)Lk, special};
-    launch_ks_mac(ct_ctx->logn, f, inst, s);
+    // Degree 32768: the power-basis part z, its NTT, then out += NTT(z).
+    u64* x = base + (size_t)inst * 2 * extra * n;
     launch_sd_z(ct_ctx->logn, f, x, inst, s);
     launch_ntt_forward(*ct_ctx, RowMap{x, (u32)L, (u32)L, 0, 0}, inst * 2 * (u32)L, false, s);
     launch_add_rows(out, x, out, ct_ctx->mod.as<ModInfo>(), (u32)L, ct_ctx->logn, (size_t)inst * 2 * L, s);
@@ -327,6 +327,7 @@ void Ksk::key_switch(const u64* p, u64* out, u32 inst, hipStream_t s) const {
 // RelinearizationKey::relinearizes (relinearization_key.rs:71-98) on
 // [inst][3][L][N] NTT ciphertexts -> [inst][2][L][N].
 void Ksk::relinearize(const u64* ct3, u64* out, u32 inst, hipStream_t s) {
+    check_batch(inst);
     const size_t n = ct_ctx->N, L = ct_ctx->L;
     u64* c2 = scratch(Scratch::RelinC2).get(s, (size_t)inst * L * n * 8).as<u64>();
     InttRowsArgs a{};
@@ -378,6 +379,7 @@ std::unique_ptr<Multiplicator> Multiplicator::make_default(std::shared_ptr<Ksk> 
 
 // Multiplicator::multiply (fhe/src/bfv/ops/mul.rs:158-227) on a batch.
 void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out, u32 inst, hipStream_t s) {
+    check_batch(inst);
     const size_t n = base->N, L = base->L, M = mul->L;
     const u32 cl = ext_lhs->common, cr = ext_rhs->common;
     const size_t xr = M - std::min(cl, cr);  // extension rows per input polynomial
@@ -474,6 +476,7 @@ Galois::Galois(std::shared_ptr<Ksk> k, u32 exp) : ksk(std::move(k)) {
 
 // GaloisKey::relinearize_into (fhe/src/bfv/keys/galois_key.rs:90-124).
 void Galois::apply(const u64* ct, u64* out, u32 inst, hipStream_t s) {
+    check_batch(inst);
     const size_t n = ksk->ct_ctx->N, L = ksk->ct_ctx->L;
     const size_t C = (size_t)inst * L * n;
     u64* c2 = scratch(Scratch::GaloisC2).get(s, C * 8).as<u64>();
@@ -541,6 +544,10 @@ Workspace& StreamScratch::get(hipStream_t s, size_t bytes) const {
     std::lock_guard<std::mutex> g(m_);
     auto& w = ws_[s];
     if (!w) w.reset(new Workspace);
+    if (bytes > w->buf.bytes && w->buf.ptr) {
+        // Work already queued on this stream may still use the old buffer.
+        HIPCHK(hipStreamSynchronize(s));
+    }
     w->ensure(bytes);
     return *w;
 }

@@ -129,7 +129,7 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void tensor_intt_kernel(TensorAr
             mul_wide(ra0[j], rb1[j], lo, hi);
             mul_wide(ra1[j], rb0[j], l2, h2);
             add128(lo, hi, l2, h2);
-            return reduce1(mont_redc(lo, hi, p, pn), p);
+            return mont_redc(lo, hi, p, pn);  // < 2p: a valid inverse-transform input
         });
     } else {
         const u64* ra = rows[c == 0 ? 0 : 1];
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void tensor_intt_kernel(TensorAr
         load_inv_order<LOGN>(lds, x, t, [&](int j) {
             u64 lo, hi;
             mul_wide(ra[j], rb[j], lo, hi);
-            return reduce1(mont_redc(lo, hi, p, pn), p);
+            return mont_redc(lo, hi, p, pn);
         });
     }
     const ulonglong2* tw = a.inv_tw + ((size_t)m << LOGN);
@@ -183,45 +183,6 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void lift_ntt_kernel(KeySwitchAr
     u64* dst = y + ((((size_t)b * a.digits + i) * a.key_limbs + j) << LOGN);
     const u64 p2 = 2 * p;
     store_fwd_order<LOGN>(lds, x, t, [&](int k, u64 v) { dst[k] = reduce1(v, p2); });
-}
-
-// Key inner product (key_switching_key.rs:227-239) of one key limb j at
-// element k: sum_i Y[i][j][k] * key[i][j][k] mod q'_j (Shoup, exact).
-__device__ __forceinline__ u64 ks_mac(const u64* __restrict__ yb, const ulonglong2* __restrict__ key, size_t stride,
-                                      u32 digits, u32 k, u64 p) {
-    u64 r = 0;
-    for (u32 i = 0; i < digits; i++) {
-        const ulonglong2 w = key[i * stride + k];
-        r = add_mod(r, mul_shoup(yb[i * stride + k], w.x, w.y, p), p);
-    }
-    return r;
-}
-
-// Key inner product of the last key limb (the modulus that switch_down drops)
-// followed by its inverse NTT, for both output parts:
-// special[b][part] = INTT(sum_i Y[b][i][L'-1] * key_part[i][L'-1]).
-// grid = (instances, 2)
-template <int LOGN>
-__global__ __launch_bounds__(NttShape<LOGN>::T) void ks_special_kernel(KeySwitchArgs a, const u64* __restrict__ y,
-                                                                       const ulonglong2* __restrict__ key_inv_tw,
-                                                                       u64* __restrict__ special) {
-    using S = NttShape<LOGN>;
-    __shared__ u64 lds[S::LDS_WORDS];
-    const int t = threadIdx.x;
-    const u32 b = blockIdx.x, part = blockIdx.y;
-    const u32 j = a.key_limbs - 1;
-    const ModInfo mi = a.key_mods[j];
-    const u64 p = mi.p;
-    const size_t stride = (size_t)a.key_limbs << LOGN;
-    const ulonglong2* key = (part ? a.key1 : a.key0) + ((size_t)j << LOGN);
-    const u64* yb = y + (((size_t)b * a.digits * a.key_limbs + j) << LOGN);
-    const u32 digits = a.digits;
-    u64 x[S::E];
-    load_inv_order<LOGN, 2>(lds, x, t, [&](int k) { return ks_mac(yb, key, stride, digits, (u32)k, p); });
-    ntt_inv_regs<LOGN>(lds, x, t, key_inv_tw + ((size_t)j << LOGN), p, inv_scale_n(mi));
-    u64* dst = special + (((size_t)b * 2 + part) << LOGN);
-#pragma unroll
-    for (int e = 0; e < S::E; e++) dst[inv_index<LOGN, last_start<LOGN>()>(t, e)] = x[e];
 }
 
 // ---------------------------------------------------------------------------
@@ -306,7 +267,8 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void sd_add_ntt_kernel(SdNttArgs
 //   q_l^-1 (x_i - [y]_i) mod q_i,  y = (x_l + floor(q_l/2)) mod q_l,
 // [y]_i = lazy_reduce(y) - floor(q_l/2) mod q_i,
 // which is linear in x_i. Only the dropped limb l must be in power basis
-// (`special`, from ks_special_kernel); the other limbs stay in the NTT domain:
+// (`special`: ks_mac_kernel, then an inverse NTT); the other limbs stay in
+// the NTT domain:
 //   out_i = NTT(add_pb_i - q_l^-1 [y]_i) + q_l^-1 x_i  (+ NTT-domain addend),
 // where x_i = sum_d Y[d][i] * key[d][i] is the key inner product. With no
 // extra modulus, out_i = NTT(add_pb_i) + x_i. Two kernels: ks_mac_kernel
@@ -314,18 +276,21 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void sd_add_ntt_kernel(SdNttArgs
 // adds the transformed power-basis part.
 // ---------------------------------------------------------------------------
 
-// One thread per coefficient of limb i, both parts: the switch-down factor
-// q_l^-1 is folded into the key rows (fin0, fin1), Y < 2q and the key rows
-// < q, so four products stay below 2^127 and one Barrett reduction per four
-// terms gives the exact residue. grid = (N / 256, L, instances)
+// One thread per coefficient of key limb i, both parts (key_switching_key.rs
+// :227-239): the switch-down factor q_l^-1 is folded into the key rows of
+// the limbs below the dropped one (fin0, fin1), Y < 2q and the key rows < q,
+// so four products stay below 2^127 and one Barrett reduction per four terms
+// gives the exact residue. Limb i == L (the dropped one, when extra) goes to
+// `special`. grid = (N / 256, L + extra, instances)
 __global__ __launch_bounds__(256) void ks_mac_kernel(KsFinishArgs f, u32 logn) {
     const SdNttArgs& a = f.sd;
     const u32 j = blockIdx.x * 256 + threadIdx.x;
     if (j >> logn) return;
     const u32 i = blockIdx.y, b = blockIdx.z;
-    const ModInfo mi = a.mods[i];
+    const bool dropped = i == a.L;
+    const ModInfo mi = dropped ? *f.special_mod : a.mods[i];
     const u64 p = mi.p;
-    const size_t ystride = (size_t)f.key_limbs << logn, kstride = (size_t)a.L << logn;
+    const size_t ystride = (size_t)f.key_limbs << logn, kstride = (size_t)(a.L + a.extra) << logn;
     const u64* yb = f.y + (((size_t)b * f.digits * f.key_limbs + i) << logn) + j;
     const u64* k0 = f.fin0 + ((size_t)i << logn) + j;
     const u64* k1 = f.fin1 + ((size_t)i << logn) + j;
@@ -346,6 +311,11 @@ __global__ __launch_bounds__(256) void ks_mac_kernel(KsFinishArgs f, u32 logn) {
         }
         m0 = add_mod(m0, reduce_u128(l0, h0, mi.barrett_lo, mi.barrett_hi, p), p);
         m1 = add_mod(m1, reduce_u128(l1, h1, mi.barrett_lo, mi.barrett_hi, p), p);
+    }
+    if (dropped) {
+        f.special[(((size_t)b * 2) << logn) + j] = m0;
+        f.special[(((size_t)b * 2 + 1) << logn) + j] = m1;
+        return;
     }
     const u32 src = a.perm ? a.perm[j] : j;
     u64* out = a.out + (size_t)b * a.out_inst_stride + ((size_t)i << logn) + j;
@@ -467,24 +437,16 @@ void launch_lift_ntt(int logn, const KeySwitchArgs& a, u64* y, u32 inst, hipStre
     FHEHIP_LOGN_DISPATCH(logn, lift_ntt_kernel, grid, 0, s, a, y);
 }
 
-void launch_ks_special(int logn, const KeySwitchArgs& a, const u64* y, const ulonglong2* key_inv_tw, u64* special,
-                       u32 inst, hipStream_t s) {
-    if (!inst) return;
-    ProfScope ps("ks_special", logn, s);
-    dim3 grid(inst, 2);
-    FHEHIP_LOGN_DISPATCH(logn, ks_special_kernel, grid, 0, s, a, y, key_inv_tw, special);
-}
-
 void launch_ks_mac(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s) {
     if (!f.sd.L || !inst) return;
     ProfScope ps("ks_mac", logn, s);
-    hipLaunchKernelGGL(ks_mac_kernel, dim3(((1u << logn) + 255) / 256, f.sd.L, inst), dim3(256), 0, s, f, (u32)logn);
+    hipLaunchKernelGGL(ks_mac_kernel, dim3(((1u << logn) + 255) / 256, f.sd.L + f.sd.extra, inst), dim3(256), 0, s, f,
+                       (u32)logn);
     HIPCHK(hipGetLastError());
 }
 
 void launch_ks_finish(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s) {
     if (!f.sd.L || !inst) return;
-    launch_ks_mac(logn, f, inst, s);
     ProfScope ps("ks_finish", logn, s);
     dim3 grid(inst * 2 * f.sd.L);
     FHEHIP_LOGN_DISPATCH(logn, ks_finish_kernel, grid, 0, s, f);

@@ -73,10 +73,11 @@ struct SdNttArgs {
 struct KsFinishArgs {
     SdNttArgs sd;
     const u64* y;     // [inst][digits][key_limbs][N], values < 2q
-    const u64* fin0;  // [digits][L][N] key rows times q_l^-1 (or plain without extra)
+    const u64* fin0;  // [digits][L + extra][N]: rows i < L times q_l^-1, row L plain
     const u64* fin1;
     u32 digits, key_limbs;
-    const u64* special;
+    u64* special;               // [inst][2][N] dropped limb (extra == 1)
+    const ModInfo* special_mod; // modulus of the dropped limb
 };
 
 void launch_rns_scale(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst, size_t out_part,
@@ -86,10 +87,9 @@ void launch_intt_rows(int logn, const InttRowsArgs& a, u32 limbs, u32 parts, u32
 void launch_tensor_intt(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_t s);
 // Lifted, lazily transformed digits Y [inst][digits][key_limbs][N].
 void launch_lift_ntt(int logn, const KeySwitchArgs& a, u64* y, u32 inst, hipStream_t s);
-// Inner product of the last key limb + inverse NTT -> special [inst][2][N].
-void launch_ks_special(int logn, const KeySwitchArgs& a, const u64* y, const ulonglong2* key_inv_tw, u64* special,
-                       u32 inst, hipStream_t s);
 void launch_sd_add_ntt(int logn, const SdNttArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s);
+// out += NTT(power-basis switch-down part), degree <= 16384 (after ks_mac and
+// the inverse NTT of `special`).
 void launch_ks_finish(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s);
 void launch_add_rows(const u64* a, const u64* b, u64* out, const ModInfo* mods, u32 L, int logn, size_t rows,
                      hipStream_t s);
@@ -101,8 +101,9 @@ void launch_gather_rows(const u64* src, size_t src_inst, u64* dst, size_t dst_in
 void launch_tensor_ew(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_t s);
 void launch_mac_ew(const u64* y, u32 digits, u32 key_limbs, u32 first, u32 count, const ulonglong2* key0,
                    const ulonglong2* key1, const ModInfo* key_mods, u64* acc, int logn, u32 inst, hipStream_t s);
-// Key inner product of the limbs below the dropped one (+ NTT-domain addend)
-// into sd.out, any degree (ks_mac_kernel).
+// Key inner products, any degree (ks_mac_kernel): limbs below the dropped
+// one (+ NTT-domain addend) into sd.out, the dropped limb (when sd.extra) into
+// `special`, still in the NTT domain.
 void launch_ks_mac(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s);
 // Power-basis switch-down part x [inst][2][L][N] of the finish (unfused path).
 void launch_sd_z(int logn, const KsFinishArgs& f, u64* x, u32 inst, hipStream_t s);

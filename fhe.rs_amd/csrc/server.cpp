@@ -16,8 +16,8 @@ double now_s() {
 }  // namespace
 
 Server::Server(const ServerConfig& cfg) : cfg_(cfg) {
-    if (cfg_.max_batch == 0 || cfg_.lanes == 0 || cfg_.lanes > 16)
-        throw Error(-1, "server needs max_batch >= 1 and 1..16 lanes");
+    if (cfg_.max_batch == 0 || cfg_.max_batch > 65535 || cfg_.lanes == 0 || cfg_.lanes > 16)
+        throw Error(-1, "server needs 1..65535 requests per batch and 1..16 lanes");
     HIPCHK(hipGetDevice(&device_));
     for (u32 k = 0; k < cfg_.lanes; k++) {
         lanes_.emplace_back(new Lane);
