@@ -77,9 +77,9 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void intt_rows_kernel(InttRowsAr
     u64 x[S::E];
     if (a.perm) {
         const u32* perm = a.perm;
-        load_inv_order<LOGN>(lds, x, t, [&](int j) { return src[perm[j]]; });
+        load_inv_order<LOGN, 16>(lds, x, t, [&](int j) { return src[perm[j]]; });
     } else {
-        load_inv_order<LOGN>(lds, x, t, [&](int j) { return src[j]; });
+        load_inv_order<LOGN, 16>(lds, x, t, [&](int j) { return src[j]; });
     }
     ntt_inv_regs<LOGN>(lds, x, t, tw, mi.p, inv_scale_n(mi));
 #pragma unroll

@@ -88,12 +88,6 @@ std::shared_ptr<DeviceContext> make_context(const std::vector<u64>& moduli, u32 
         set_last_stage(mi, inv[(size_t)l * n + n - 2].x);
         c->h_mod.push_back(mi);
     }
-    // q_last^-1 mod q_i (context.rs:67-74)
-    std::vector<ulonglong2> il;
-    for (u32 l = 0; l + 1 < c->L; l++) {
-        u64 v = invmod64(moduli.back() % moduli[l], moduli[l]);
-        il.push_back(make_ulonglong2(v, shoup64(v, moduli[l])));
-    }
     if (n == 32768) {
         // Half tables (see DeviceContext::half). Forward: local index k = 2^s + g
         // of a 16384-point transform at stage s maps to the global index
@@ -152,10 +146,6 @@ std::shared_ptr<DeviceContext> make_context(const std::vector<u64>& moduli, u32 
     HIPCHK(hipMemcpy(c->fwd_tw.ptr, fwd.data(), c->fwd_tw.bytes, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->inv_tw.ptr, inv.data(), c->inv_tw.bytes, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->mod.ptr, c->h_mod.data(), c->mod.bytes, hipMemcpyHostToDevice));
-    if (!il.empty()) {
-        c->inv_last.reset(il.size() * sizeof(ulonglong2));
-        HIPCHK(hipMemcpy(c->inv_last.ptr, il.data(), c->inv_last.bytes, hipMemcpyHostToDevice));
-    }
     return c;
 }
 

@@ -48,7 +48,6 @@ struct DeviceContext {
     DevBuf fwd_tw;   // [L][N] ulonglong2 (omega, omega_shoup)
     DevBuf inv_tw;   // [L][N] ulonglong2 (zeta_inv, zeta_inv_shoup)
     DevBuf mod;      // [L] ModInfo
-    DevBuf inv_last; // [L-1] ulonglong2 (q_last^-1 mod q_i, shoup)
     RnsBasis rns;    // host-side CRT constants
     // N = 32768 only: the forward transform's first stage and the inverse's
     // last stage pair the two halves of a row; each half is then an

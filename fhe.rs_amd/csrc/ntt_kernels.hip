@@ -47,7 +47,7 @@ __global__ __launch_bounds__((NttShape<LOGN, R>::T)) void ntt_inv_kernel(RowMap 
     const ulonglong2* tw = tw_all + ((size_t)limb << LOGN);
     const ModInfo mi = mods[limb];
     u64 x[S::E];
-    load_inv_order<LOGN, 4, R>(lds, x, t, [&](int j) { return row[j]; });
+    load_inv_order<LOGN, 16, R>(lds, x, t, [&](int j) { return row[j]; });
     ntt_inv_regs<LOGN, R>(lds, x, t, tw, mi.p, inv_scale_n(mi));
 #pragma unroll
     for (int k = 0; k < S::E; k++) row[inv_index<LOGN, last_start<LOGN, R>(), R>(t, k)] = x[k];

@@ -159,7 +159,10 @@ class Context:
         self.h = h
 
     def __del__(self):
-        _destroy("fhehip_context_destroy", getattr(self, "h", None))
+        try:
+            _destroy("fhehip_context_destroy", getattr(self, "h", None))
+        except Exception:  # module globals already cleared at interpreter exit
+            pass
 
     @property
     def L(self):
@@ -222,7 +225,10 @@ class Scaler:
         self.h = h
 
     def __del__(self):
-        _destroy("fhehip_scaler_destroy", getattr(self, "h", None))
+        try:
+            _destroy("fhehip_scaler_destroy", getattr(self, "h", None))
+        except Exception:  # module globals already cleared at interpreter exit
+            pass
 
     def scale(self, d_in, d_out, num_polys, ntt_repr, stream=None):
         check(lib().fhehip_scaler_scale(self.h, dptr(d_in), dptr(d_out), num_polys, 1 if ntt_repr else 0,
@@ -244,7 +250,10 @@ class KeySwitchingKey:
         self.ct_ctx, self.key_ctx = ct_ctx, key_ctx
 
     def __del__(self):
-        _destroy("fhehip_ksk_destroy", getattr(self, "h", None))
+        try:
+            _destroy("fhehip_ksk_destroy", getattr(self, "h", None))
+        except Exception:  # module globals already cleared at interpreter exit
+            pass
 
     def key_switch(self, d_p, d_out, num, stream=None):
         check(lib().fhehip_key_switch(self.h, dptr(d_p), dptr(d_out), num, stream_ptr(stream)))
@@ -272,7 +281,10 @@ class Multiplicator:
         self.h = h
 
     def __del__(self):
-        _destroy("fhehip_multiplicator_destroy", getattr(self, "h", None))
+        try:
+            _destroy("fhehip_multiplicator_destroy", getattr(self, "h", None))
+        except Exception:  # module globals already cleared at interpreter exit
+            pass
 
     def extended_basis(self):
         n = C.c_uint32(64)
@@ -293,7 +305,10 @@ class GaloisKey:
         self.exponent = exponent
 
     def __del__(self):
-        _destroy("fhehip_galois_destroy", getattr(self, "h", None))
+        try:
+            _destroy("fhehip_galois_destroy", getattr(self, "h", None))
+        except Exception:  # module globals already cleared at interpreter exit
+            pass
 
     def relinearize(self, d_ct, d_out, num, stream=None):
         check(lib().fhehip_galois_relinearize(self.h, dptr(d_ct), dptr(d_out), num, stream_ptr(stream)))
@@ -310,7 +325,10 @@ class EvaluationKey:
         self.keys = list(galois_keys)
 
     def __del__(self):
-        _destroy("fhehip_evaluation_key_destroy", getattr(self, "h", None))
+        try:
+            _destroy("fhehip_evaluation_key_destroy", getattr(self, "h", None))
+        except Exception:  # module globals already cleared at interpreter exit
+            pass
 
     def rotate_columns(self, d_ct, d_out, step, num, stream=None):
         check(lib().fhehip_rotate_columns(self.h, dptr(d_ct), dptr(d_out), step, num, stream_ptr(stream)))

@@ -43,6 +43,50 @@ def test_ntt_matches_oracle(fh, oracle, n, L):
         assert np.array_equal(lz[:, l] % np.uint64(p), ref[:, l])
 
 
+def _smallest_prime_above(oracle, lo, n):
+    p = lo + (1 - lo) % (2 * n)  # first value = 1 mod 2n from lo on
+    if p <= lo:
+        p += 2 * n
+    for _ in range(100000):
+        if oracle.is_prime(p):
+            return p
+        p += 2 * n
+    raise AssertionError("no prime found")
+
+
+@pytest.mark.parametrize("n", [8, 1024, 16384, 32768])
+def test_ntt_moduli_around_2_60(fh, oracle, n):
+    """Moduli on both sides of 2^60 (the largest prime below it, whose lazy
+    operands reach just under 2^62, and the smallest above it), a 62-bit and
+    small moduli, with all-(p-1) and delta rows, in both kernel shapes."""
+    moduli = oracle.generate_moduli([60, 62, 20, 40], n) + [_smallest_prime_above(oracle, 1 << 60, n)]
+    assert moduli[0] < 1 << 60 <= moduli[4]
+    L = len(moduli)
+    print(f"ntt moduli around 2^60 n={n}", flush=True)
+    ctx = fh.Context(moduli, n)
+    octx = oracle.Context(moduli, n)
+    a = oracle.random_polys(moduli, n, 3, seed=n + 7).reshape(3, L, n)
+    a[1] = np.array(moduli, np.uint64)[:, None] - np.uint64(1)
+    a[2] = 0
+    a[2, :, 0] = 1
+    a = a.reshape(-1)
+    ref = octx.forward(a)
+    for shape in (4, 5):
+        fh.ntt_set_shape(n, shape, shape)
+        d = fh.to_device(a)
+        ctx.ntt_forward(d, 3)
+        assert np.array_equal(fh.to_host(d), ref), shape
+        ctx.ntt_backward(d, 3)
+        assert np.array_equal(fh.to_host(d), a), shape
+        d2 = fh.to_device(a)
+        ctx.ntt_forward(d2, 3, lazy=True)
+        lz = fh.to_host(d2).reshape(3, L, n)
+        for l, p in enumerate(moduli):
+            assert (lz[:, l] < np.uint64(4 * p)).all()
+            assert np.array_equal(lz[:, l] % np.uint64(p), ref.reshape(3, L, n)[:, l])
+    fh.ntt_set_shape(n, 4, 4)
+
+
 def test_ntt_edge_values(fh, oracle):
     n = 4096
     moduli = [4611686018427322369, 1152921504606748673]
