@@ -82,8 +82,7 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void intt_rows_kernel(InttRowsAr
         load_inv_order<LOGN, 16>(lds, x, t, [&](int j) { return src[j]; });
     }
     ntt_inv_regs<LOGN>(lds, x, t, tw, mi.p, inv_scale_n(mi));
-#pragma unroll
-    for (int e = 0; e < S::E; e++) dst[inv_index<LOGN, last_start<LOGN>()>(t, e)] = x[e];
+    store_inv_order<LOGN>(dst, x, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -143,8 +142,7 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void tensor_intt_kernel(TensorAr
     const ulonglong2* tw = a.inv_tw + ((size_t)m << LOGN);
     ntt_inv_regs<LOGN>(lds, x, t, tw, mi.p, inv_scale_nr(mi));
     u64* dst = a.out + (size_t)b * a.out_inst_stride + (size_t)c * a.out_part_stride + ((size_t)m << LOGN);
-#pragma unroll
-    for (int e = 0; e < S::E; e++) dst[inv_index<LOGN, last_start<LOGN>()>(t, e)] = x[e];
+    store_inv_order<LOGN>(dst, x, t);
 }
 
 // ---------------------------------------------------------------------------

@@ -42,19 +42,17 @@ __device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned total) {
 // patterns over the banks.
 __device__ __forceinline__ int lds_slot(int j) { return j + (j >> 4); }
 
-// Forward butterfly (native.rs:256-269): x, y < 4p -> x', y' < 4p.
+// Forward butterfly (native.rs:256-269): x, y < 4p -> x', y' < 4p. UT: the
+// twiddle is the same for the whole wave.
+template <bool UT = false>
 __device__ __forceinline__ void fwd_bfly(u64& x, u64& y, u64 w, u64 ws, u64 p, u64 p2) {
-    u64 xr = reduce1(x, p2);
-    u64 t = lazy_mul_shoup(y, w, ws, p);
-    y = xr + p2 - t;
-    x = xr + t;
+    fwd_bfly_asm<UT>(x, y, w, ws, p, p2);
 }
 
 // Inverse butterfly (native.rs:288-300): x, y < 2p -> x', y' < 2p.
+template <bool UT = false>
 __device__ __forceinline__ void inv_bfly(u64& x, u64& y, u64 z, u64 zs, u64 p, u64 p2) {
-    u64 t = x;
-    x = reduce1(y + t, p2);
-    y = lazy_mul_shoup(t + p2 - y, z, zs, p);
+    inv_bfly_asm<UT>(x, y, z, zs, p, p2);
 }
 
 // Final scaling of an inverse transform folded into its last stage:
@@ -100,12 +98,12 @@ __device__ __forceinline__ void fwd_pass(u64 (&x)[(NttShape<LOGN, R>::E)], int t
 #pragma unroll
             for (int q = 0; q < half; q++) {
                 const int k0 = (g << (RE - u)) + q;
-                fwd_bfly(x[k0], x[k0 + half], w.x, w.y, p, p2);
+                fwd_bfly<S0V == 0>(x[k0], x[k0 + half], w.x, w.y, p, p2);
             }
         }
         // With 32 values per thread, keep the next stage's twiddle loads from
         // being hoisted above this one (bounds the live registers).
-        if constexpr (RE >= 5) asm volatile("" ::: "memory");
+        if constexpr (RE >= 4) asm volatile("" ::: "memory");
     }
 }
 
@@ -127,8 +125,8 @@ __device__ __forceinline__ void inv_pass(u64 (&x)[(NttShape<LOGN, R>::E)], int t
 #pragma unroll
             for (int q = 0; q < half; q++) {
                 const u64 a = x[q], b = x[q + half];
-                x[q] = mul_shoup(a + b, sc.s, sc.ss, p);
-                x[q + half] = mul_shoup(a + p2 - b, sc.zs, sc.zss, p);
+                x[q] = mul_shoup_u(a + b, sc.s, sc.ss, p);
+                x[q + half] = mul_shoup_u(a + p2 - b, sc.zs, sc.zss, p);
             }
             continue;
         }
@@ -139,10 +137,11 @@ __device__ __forceinline__ void inv_pass(u64 (&x)[(NttShape<LOGN, R>::E)], int t
 #pragma unroll
             for (int q = 0; q < half; q++) {
                 const int k0 = (g << (u + 1)) + q;
-                inv_bfly(x[k0], x[k0 + half], z.x, z.y, p, p2);
+                // In the last pass (B0V = LOGN - RE) hi is 0: shared twiddles.
+                inv_bfly<B0V == LOGN - RE>(x[k0], x[k0 + half], z.x, z.y, p, p2);
             }
         }
-        if constexpr (RE >= 5) asm volatile("" ::: "memory");
+        if constexpr (RE >= 4) asm volatile("" ::: "memory");
     }
 }
 
@@ -167,6 +166,39 @@ __device__ __forceinline__ void lds_get_inv(const u64* lds, u64 (&x)[(NttShape<L
     for (int k = 0; k < NttShape<LOGN, R>::E; k++) x[k] = lds[lds_slot(inv_index<LOGN, B0V, R>(t, k))];
 }
 
+// Exchange layouts in which each wave only touches the block of 64 * E
+// consecutive elements it owns (elements j with j >> (6 + RE) equal to its
+// wave index t >> 6). Moving data between two such layouts needs no
+// workgroup barrier, only ordering within the wave: LDS instructions of one
+// wave execute in program order, so a compiler fence at wavefront scope
+// suffices.
+template <int LOGN, int S0V, int R = 4>
+constexpr bool fwd_local() {
+    return LOGN - S0V - NttShape<LOGN, R>::RE <= 6;
+}
+template <int B0V>
+constexpr bool inv_local() {
+    return B0V <= 6;
+}
+
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Ordering between an exchange's LDS writes and reads (or reads and the next
+// writes): a workgroup barrier unless every wave stays inside its own block.
+template <bool LOCAL>
+__device__ __forceinline__ void exchange_sync() {
+    if constexpr (LOCAL)
+        wave_sync();
+    else
+        __syncthreads();
+}
+
+constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
+
 // The last pass of either direction always starts at LOGN - RE, so that its
 // element order is fwd_index<LOGN, LOGN-RE> (16 consecutive values per thread)
 // or inv_index<LOGN, LOGN-RE> (value k of thread t at k * T + t).
@@ -183,10 +215,13 @@ __device__ __forceinline__ void fwd_rest(u64* lds, u64 (&x)[(NttShape<LOGN, R>::
     if constexpr (S0V + RE < LOGN) {
         constexpr int NEXT = (S0V + 2 * RE <= LOGN) ? S0V + RE : LOGN - RE;
         constexpr int SMIN = S0V + RE;
+        // The written layout is S0V's, the read one NEXT's; the next writes
+        // (the following exchange, or store_fwd_order) use NEXT's again.
+        constexpr bool A = fwd_local<LOGN, S0V, R>(), B = fwd_local<LOGN, NEXT, R>();
         lds_put_fwd<LOGN, S0V, R>(lds, x, t);
-        __syncthreads();
+        exchange_sync<A && B>();
         lds_get_fwd<LOGN, NEXT, R>(lds, x, t);
-        __syncthreads();
+        exchange_sync<B>();
         fwd_pass<LOGN, NEXT, SMIN, R>(x, t, tw, p, p2);
         fwd_rest<LOGN, NEXT, R>(lds, x, t, tw, p, p2);
     }
@@ -197,12 +232,21 @@ __device__ __forceinline__ void inv_rest(u64* lds, u64 (&x)[(NttShape<LOGN, R>::
                                          u64 p, u64 p2, const InvScale& sc) {
     constexpr int RE = NttShape<LOGN, R>::RE;
     if constexpr (B0V + RE < LOGN) {
-        constexpr int NEXT = (B0V + 2 * RE <= LOGN) ? B0V + RE : LOGN - RE;
+        // Bits below LW are paired inside one wave's block. When the next
+        // full pass would straddle LW, a shorter wave-local pass up to LW
+        // (layout LW - RE, stages from VMIN) keeps that exchange barrier-free,
+        // provided it costs no extra pass.
+        constexpr int LW = 6 + RE;
+        constexpr int PLAIN = (B0V + 2 * RE <= LOGN) ? B0V + RE : LOGN - RE;
+        constexpr bool SPLIT = B0V + RE < LW && LW < B0V + 2 * RE && LW - RE > B0V && LOGN > LW &&
+                               1 + cdiv(LOGN - LW, RE) <= cdiv(LOGN - B0V - RE, RE);
+        constexpr int NEXT = SPLIT ? LW - RE : PLAIN;
         constexpr int VMIN = B0V + RE;
+        constexpr bool A = inv_local<B0V>(), B = inv_local<NEXT>();
         lds_put_inv<LOGN, B0V, R>(lds, x, t);
-        __syncthreads();
+        exchange_sync<A && B>();
         lds_get_inv<LOGN, NEXT, R>(lds, x, t);
-        __syncthreads();
+        exchange_sync<B>();
         inv_pass<LOGN, NEXT, VMIN, R, SCALE>(x, t, tw, p, p2, sc);
         inv_rest<LOGN, NEXT, R, SCALE>(lds, x, t, tw, p, p2, sc);
     }
@@ -241,8 +285,8 @@ __device__ __forceinline__ InvScale inv_scale_nr(const ModInfo& mi) {
 
 // Coalesced global row -> registers in the inverse-transform input order
 // (value e of thread t is element t*E + e): the row is read with consecutive
-// lanes on consecutive elements and transposed through LDS. Ends with a
-// barrier, so the LDS is free for the transform.
+// lanes on consecutive elements and transposed through LDS. Must be followed
+// by the inverse transform (its first LDS writes stay in each wave's block).
 template <int LOGN, int UNROLL = 4, int R = 4, class Load>
 __device__ __forceinline__ void load_inv_order(u64* lds, u64 (&x)[(NttShape<LOGN, R>::E)], int t, Load load) {
     using S = NttShape<LOGN, R>;
@@ -255,7 +299,22 @@ __device__ __forceinline__ void load_inv_order(u64* lds, u64 (&x)[(NttShape<LOGN
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < S::E; e++) x[e] = lds[lds_slot((t << S::RE) + e)];
-    __syncthreads();
+    // The reads are in the inverse transform's first layout, whose first
+    // exchange writes the same wave-local block.
+    exchange_sync<inv_local<0>()>();
+}
+
+// Registers in the inverse-transform output order -> row stores: value e of
+// thread t goes to element inv_index<LOGN, last_start>(t, e). The thread
+// index passes through an empty asm so that the addresses are formed here
+// and not hoisted to the kernel start, where they would stay live in
+// registers across the whole transform.
+template <int LOGN, int R = 4>
+__device__ __forceinline__ void store_inv_order(u64* __restrict__ dst, const u64 (&x)[(NttShape<LOGN, R>::E)], int t) {
+    int tl = t;
+    asm volatile("" : "+v"(tl));
+#pragma unroll
+    for (int e = 0; e < NttShape<LOGN, R>::E; e++) dst[inv_index<LOGN, last_start<LOGN, R>(), R>(tl, e)] = x[e];
 }
 
 // Registers in the forward-transform output order (value e of thread t is

@@ -49,8 +49,7 @@ __global__ __launch_bounds__((NttShape<LOGN, R>::T)) void ntt_inv_kernel(RowMap 
     u64 x[S::E];
     load_inv_order<LOGN, 16, R>(lds, x, t, [&](int j) { return row[j]; });
     ntt_inv_regs<LOGN, R>(lds, x, t, tw, mi.p, inv_scale_n(mi));
-#pragma unroll
-    for (int k = 0; k < S::E; k++) row[inv_index<LOGN, last_start<LOGN, R>(), R>(t, k)] = x[k];
+    store_inv_order<LOGN, R>(row, x, t);
 }
 
 // Values per thread (log2) of the plain NTT kernels, per degree and
@@ -126,7 +125,7 @@ __global__ __launch_bounds__(256) void ntt_split_fwd_kernel(RowMap m, const ulon
     const ulonglong2 w = split[3 * limb];
     const u64 p = mods[limb].p;
     u64 x = row[j], y = row[j + 16384];
-    fwd_bfly(x, y, w.x, w.y, p, 2 * p);
+    fwd_bfly<true>(x, y, w.x, w.y, p, 2 * p);
     row[j] = x;
     row[j + 16384] = y;
 }
@@ -142,8 +141,8 @@ __global__ __launch_bounds__(256) void ntt_split_inv_kernel(RowMap m, const ulon
     const ulonglong2 z = split[3 * limb + 1], ni = split[3 * limb + 2];
     const u64 p = mods[limb].p;
     const u64 x = row[j], y = row[j + 16384];
-    row[j] = mul_shoup(x + y, ni.x, ni.y, p);
-    row[j + 16384] = mul_shoup(x + 2 * p - y, z.x, z.y, p);
+    row[j] = mul_shoup_u(x + y, ni.x, ni.y, p);
+    row[j + 16384] = mul_shoup_u(x + 2 * p - y, z.x, z.y, p);
 }
 
 // N = 32768 digit lift (rq/mod.rs:518-540) fused with the first forward stage:
@@ -162,7 +161,7 @@ __global__ __launch_bounds__(256) void lift_split_kernel(const u64* __restrict__
     u64 x = lazy_reduce_u64(row[k], mi.barrett_lo, mi.barrett_hi, p);
     u64 z = lazy_reduce_u64(row[k + 16384], mi.barrett_lo, mi.barrett_hi, p);
     const ulonglong2 w = split[3 * j];
-    fwd_bfly(x, z, w.x, w.y, p, 2 * p);
+    fwd_bfly<true>(x, z, w.x, w.y, p, 2 * p);
     u64* out = y + (((size_t)b * gridDim.y + blockIdx.y) << 15);
     out[k] = x;
     out[k + 16384] = z;
