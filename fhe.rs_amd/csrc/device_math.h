@@ -88,9 +88,9 @@ __device__ __forceinline__ u64 lazy_mul_shoup_u(u64 a, u64 w, u64 ws, u64 p) {
 
 // Harvey forward butterfly (ntt/native.rs:256-269) on x, y < 4p:
 // xr = x - 2p if x >= 2p, t = lazy_mul_shoup(y), x' = xr + t, y' = xr + 2p - t.
-// Fixed temporaries v0..v7 as in lazy_mul_shoup. The high word of t is
-// accumulated as the upper half of the pair (0, K) = v[6:7], and both outputs
-// are written by 64-bit adds straight into x and y after the last input read.
+// Fixed temporaries v0..v7 as in lazy_mul_shoup (t ends in v[2:3]); both
+// outputs are written by 64-bit adds straight into x and y after the last
+// input read.
 // p is uniform across the wave; UT: the twiddle (w, ws) is too.
 #define FHEHIP_FWD_BFLY_ASM                                \
     "v_mov_b32 v1, 0\n\t"                                  \
@@ -111,13 +111,12 @@ __device__ __forceinline__ u64 lazy_mul_shoup_u(u64 a, u64 w, u64 ws, u64 p) {
     "v_mul_lo_u32 v0, v0, %[n1]\n\t"                       \
     "v_mul_lo_u32 v1, v1, %[n0]\n\t"                       \
     "v_add3_u32 v7, v7, v6, v0\n\t"                        \
-    "v_sub_co_u32 v0, vcc, %[P0], v2\n\t"                  \
-    "v_mov_b32 v6, 0\n\t"                                  \
-    "v_add_u32 v7, v7, v1\n\t"                             \
-    "v_lshl_add_u64 v[2:3], v[2:3], 0, v[6:7]\n\t"         \
-    "v_subb_co_u32 v1, vcc, %[P1], v3, vcc\n\t"            \
+    "v_sub_co_u32 v6, vcc, %[P0], v2\n\t"                  \
+    "v_add3_u32 v3, v3, v7, v1\n\t"                        \
+    "s_nop 0\n\t"                                          \
+    "v_subb_co_u32 v7, vcc, %[P1], v3, vcc\n\t"            \
     "v_lshl_add_u64 %[xo], v[4:5], 0, v[2:3]\n\t"          \
-    "v_lshl_add_u64 %[yo], v[4:5], 0, v[0:1]"
+    "v_lshl_add_u64 %[yo], v[4:5], 0, v[6:7]"
 
 #define FHEHIP_FWD_BFLY_IN(C)                                                                                  \
     [x0] "v"((u32)x), [x1] "v"((u32)(x >> 32)), [a0] "v"((u32)y), [a1] "v"((u32)(y >> 32)), [s0] C((u32)ws), \
@@ -138,7 +137,7 @@ __device__ __forceinline__ void fwd_bfly_asm(u64& x, u64& y, u64 w, u64 ws, u64 
 
 // Gentleman-Sande inverse butterfly (ntt/native.rs:288-300) on x, y < 2p:
 // x' = (x + y) mod 2p, y' = lazy_mul_shoup(x + 2p - y). The reduction adds
-// (u < 2p ? 0 : -2p) to u = x + y; y' = G + (0, K) as in the forward one.
+// (u < 2p ? 0 : -2p) to u = x + y.
 #define FHEHIP_INV_BFLY_ASM                                \
     "v_sub_co_u32 v6, %[k], %[x0], %[y0]\n\t"              \
     "v_mov_b32 v1, 0\n\t"                                  \
@@ -160,11 +159,10 @@ __device__ __forceinline__ void fwd_bfly_asm(u64& x, u64& y, u64 w, u64 ws, u64 
     "v_mul_lo_u32 v0, v0, %[n1]\n\t"                       \
     "v_mul_lo_u32 v1, v1, %[n0]\n\t"                       \
     "v_add3_u32 v7, v7, v6, v0\n\t"                        \
-    "v_mov_b32 v6, 0\n\t"                                  \
-    "v_add_u32 v7, v7, v1\n\t"                             \
+    "v_add3_u32 v3, v3, v7, v1\n\t"                        \
     "v_cndmask_b32 v0, %[m0], 0, vcc\n\t"                  \
     "v_cndmask_b32 v1, %[m1], 0, vcc\n\t"                  \
-    "v_lshl_add_u64 %[yo], v[2:3], 0, v[6:7]\n\t"          \
+    "v_mov_b64 %[yo], v[2:3]\n\t"                          \
     "v_lshl_add_u64 %[xo], v[4:5], 0, v[0:1]"
 
 #define FHEHIP_INV_BFLY_IN(C)                                                                                  \
