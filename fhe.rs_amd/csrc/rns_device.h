@@ -95,6 +95,60 @@ __device__ __forceinline__ u64 sc_reduce_u128(u64 a_lo, u64 a_hi, u64 bl, u64 bh
     return reduce1(a_lo - q * p, p);
 }
 
+// Exact sums of 64 x 64-bit products in column form. With r = r1 2^32 + r0
+// and w = w1 2^32 + w0, r w = r0 w0 + 2^32 (r0 w1 + r1 w0) + 2^64 r1 w1; the
+// three column sums are kept in 64-bit words a, b, c whose carries out are
+// counted in ka, kb, kc, so that
+//   S = a + 2^32 b + 2^64 c + 2^64 ka + 2^96 kb + 2^128 kc
+// exactly. One product costs four v_mad_u64_u32 (the carry out of each add
+// goes to an SGPR pair) and four v_addc_co_u32 counting those carries; every
+// carry is read at least two instructions after the VALU wrote it.
+struct Acc3 {
+    u64 a, b, c;
+    u32 ka, kb, kc;
+};
+
+__device__ __forceinline__ void acc3_init(Acc3& s, u64 a0) {
+    s.a = a0;
+    s.b = s.c = 0;
+    s.ka = s.kb = s.kc = 0;
+}
+
+#define FHEHIP_ACC3_ASM                                    \
+    "v_mad_u64_u32 %[a], %[k0], %[r0], %[w0], %[a]\n\t"    \
+    "v_mad_u64_u32 %[b], %[k1], %[r0], %[w1], %[b]\n\t"    \
+    "v_mad_u64_u32 %[c], %[k2], %[r1], %[w1], %[c]\n\t"    \
+    "v_addc_co_u32 %[ka], %[d], %[ka], 0, %[k0]\n\t"       \
+    "v_mad_u64_u32 %[b], %[k0], %[r1], %[w0], %[b]\n\t"    \
+    "v_addc_co_u32 %[kb], %[d], %[kb], 0, %[k1]\n\t"       \
+    "v_addc_co_u32 %[kc], %[d], %[kc], 0, %[k2]\n\t"       \
+    "v_addc_co_u32 %[kb], %[d], %[kb], 0, %[k0]"
+
+// s += r * w. SW: w is uniform across the wave (passed in SGPRs).
+template <bool SW>
+__device__ __forceinline__ void acc3_mul(Acc3& s, u64 r, u64 w) {
+    u64 k0, k1, k2, d;
+    if constexpr (SW)
+        asm(FHEHIP_ACC3_ASM
+            : [a] "+v"(s.a), [b] "+v"(s.b), [c] "+v"(s.c), [ka] "+v"(s.ka), [kb] "+v"(s.kb), [kc] "+v"(s.kc),
+              [k0] "=&s"(k0), [k1] "=&s"(k1), [k2] "=&s"(k2), [d] "=&s"(d)
+            : [r0] "v"((u32)r), [r1] "v"((u32)(r >> 32)), [w0] "s"((u32)w), [w1] "s"((u32)(w >> 32)));
+    else
+        asm(FHEHIP_ACC3_ASM
+            : [a] "+v"(s.a), [b] "+v"(s.b), [c] "+v"(s.c), [ka] "+v"(s.ka), [kb] "+v"(s.kb), [kc] "+v"(s.kc),
+              [k0] "=&s"(k0), [k1] "=&s"(k1), [k2] "=&s"(k2), [d] "=&s"(d)
+            : [r0] "v"((u32)r), [r1] "v"((u32)(r >> 32)), [w0] "v"((u32)w), [w1] "v"((u32)(w >> 32)));
+}
+
+// The 192-bit value of an accumulator (it is below 2^192).
+__device__ __forceinline__ void acc3_value(const Acc3& s, u64& lo, u64& hi, u64& top) {
+    unsigned long long c1, c2, c3;
+    lo = __builtin_addcll(s.a, s.b << 32, 0, &c1);
+    hi = __builtin_addcll(s.c, (s.b >> 32) | ((u64)s.kb << 32), c1, &c2);
+    hi = __builtin_addcll(hi, (u64)s.ka, 0, &c3);
+    top = (u64)s.kc + c2 + c3;
+}
+
 struct U256 {
     u64 w[4];
 };
@@ -157,15 +211,17 @@ __device__ __forceinline__ void rns_scale_column(const u64* __restrict__ sc, Res
     // an exact 192-bit sum (no term exceeds 2^128), combined once.
     U256 sum;
     {
-        u64 a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
+        Acc3 sa, sb;
+        acc3_init(sa, 0);
+        acc3_init(sb, 0);
 #pragma unroll
         for (int j = 0; j < nf; j++) {
-            u64 l, h;
-            sc_mul_wide(rests(j), tg[2 * j], l, h);
-            sc_add192(a0, a1, a2, l, h);
-            sc_mul_wide(rests(j), tg[2 * j + 1], l, h);
-            sc_add192(b0, b1, b2, l, h);
+            acc3_mul<true>(sa, rests(j), tg[2 * j]);
+            acc3_mul<true>(sb, rests(j), tg[2 * j + 1]);
         }
+        u64 a0, a1, a2, b0, b1, b2;
+        acc3_value(sa, a0, a1, a2);
+        acc3_value(sb, b0, b1, b2);
         u256_combine(sum, a0, a1, a2, b0, b1, b2);
     }
     u128 vv = u256_shr_lo128(sum, shift - 1);
@@ -177,21 +233,26 @@ __device__ __forceinline__ void rns_scale_column(const u64* __restrict__ sc, Res
         // sum_j +-r_j theta_omega_j with the signs of the constants: the
         // positive and negative terms are summed apart (the signs are uniform
         // over the columns) and subtracted modulo 2^256.
-        u64 p0 = 0, p1 = 0, p2 = 0, q0 = 0, q1 = 0, q2 = 0;  // positive lo / hi halves
-        u64 n0 = 0, n1 = 0, n2 = 0, m0 = 0, m1 = 0, m2 = 0;  // negative lo / hi halves
+        Acc3 pa, pb, na, nb;  // positive and negative terms, lo and hi halves of theta
+        acc3_init(pa, 0);
+        acc3_init(pb, 0);
+        acc3_init(na, 0);
+        acc3_init(nb, 0);
 #pragma unroll
         for (int j = 0; j < nf; j++) {
-            u64 l, h, l2, h2;
-            sc_mul_wide(rests(j), tom[3 * j], l, h);
-            sc_mul_wide(rests(j), tom[3 * j + 1], l2, h2);
             if (tom[3 * j + 2]) {
-                sc_add192(n0, n1, n2, l, h);
-                sc_add192(m0, m1, m2, l2, h2);
+                acc3_mul<true>(na, rests(j), tom[3 * j]);
+                acc3_mul<true>(nb, rests(j), tom[3 * j + 1]);
             } else {
-                sc_add192(p0, p1, p2, l, h);
-                sc_add192(q0, q1, q2, l2, h2);
+                acc3_mul<true>(pa, rests(j), tom[3 * j]);
+                acc3_mul<true>(pb, rests(j), tom[3 * j + 1]);
             }
         }
+        u64 p0, p1, p2, q0, q1, q2, n0, n1, n2, m0, m1, m2;
+        acc3_value(pa, p0, p1, p2);
+        acc3_value(pb, q0, q1, q2);
+        acc3_value(na, n0, n1, n2);
+        acc3_value(nb, m0, m1, m2);
         U256 so, neg;
         u256_combine(so, p0, p1, p2, q0, q1, q2);
         u256_combine(neg, n0, n1, n2, m0, m1, m2);
@@ -241,39 +302,31 @@ __device__ __forceinline__ void rns_scale_column(const u64* __restrict__ sc, Res
         const u64* o = outc + (size_t)i * ostride;
         const u64 q = o[0], bl = o[1], bh = o[2];
         // o[3] = q - gamma, o[4] = [2^64 (q - gamma)]_q, o[5] = [2^64]_q, o[6] = [2^128]_q
-        u64 lo, hi, top = 0;
-        sc_mul_wide(v0, o[3], lo, hi);
-        {
-            u64 l, h;
-            sc_mul_wide(v1, o[4], l, h);
-            sc_add192(lo, hi, top, l, h);
-        }
+        Acc3 acc;
+        acc3_init(acc, 0);
+        acc3_mul<true>(acc, v0, o[3]);
+        acc3_mul<true>(acc, v1, o[4]);
         if (!is_one) {
             // +w, or -w = w (q - 1) mod q, without a branch on the column's
             // sign: w0 c0 + w1 c1 with (c0, c1) = (1, [2^64]_q) or
             // (q - 1, q - [2^64]_q).
             const u64 c0 = w_sign ? q - 1 : 1;
             const u64 c1 = w_sign ? q - o[5] : o[5];
-            u64 l, h;
-            sc_mul_wide(w0, c0, l, h);
-            sc_add192(lo, hi, top, l, h);
-            sc_mul_wide(w1, c1, l, h);
-            sc_add192(lo, hi, top, l, h);
+            acc3_mul<false>(acc, w0, c0);
+            acc3_mul<false>(acc, w1, c1);
         }
         const u64* om = o + 7;
 #pragma unroll
-        for (int j = 0; j < nf; j++) {
-            u64 l, h;
-            sc_mul_wide(rests(j), om[j], l, h);
-            sc_add192(lo, hi, top, l, h);
-        }
+        for (int j = 0; j < nf; j++) acc3_mul<true>(acc, rests(j), om[j]);
         // lo + 2^64 hi + 2^128 top  ->  lo + hi [2^64]_q + top [2^128]_q  (< 2^127) -> Barrett
-        u64 l2, h2;
-        sc_mul_wide(hi, o[5], l2, h2);
-        sc_add128(l2, h2, lo, 0);
-        u64 l3, h3;
-        sc_mul_wide(top, o[6], l3, h3);
-        sc_add128(l2, h2, l3, h3);
+        u64 lo, hi, top;
+        acc3_value(acc, lo, hi, top);
+        Acc3 fold;
+        acc3_init(fold, lo);
+        acc3_mul<true>(fold, hi, o[5]);
+        acc3_mul<true>(fold, top, o[6]);
+        u64 l2, h2, t2;
+        acc3_value(fold, l2, h2, t2);
         emit(i, sc_reduce_u128(l2, h2, bl, bh, q));
     }
 }
