@@ -119,7 +119,7 @@ __global__ __launch_bounds__(256) void sd_z_kernel(KsFinishArgs f, u64* __restri
         u64 yv = f.special[(((size_t)b * 2 + part) << logn) + e] + (ql >> 1);
         yv = yv >= ql ? yv - ql : yv;
         const u64 tmp = lazy_reduce_u64(yv, mi.barrett_lo, mi.barrett_hi, p) + cst.half_mod;
-        z = mul_shoup(3 * p - tmp, cst.inv, cst.inv_shoup, p);
+        z = mul_shoup_u(3 * p - tmp, cst.inv, cst.inv_shoup, p);
     }
     if (a.add_pb)
         z = add_mod(z, a.add_pb[(size_t)b * a.add_inst_stride + (size_t)part * a.add_part_stride + ((size_t)i << logn) + e],

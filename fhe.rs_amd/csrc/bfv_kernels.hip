@@ -226,7 +226,7 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void sd_add_ntt_kernel(SdNttArgs
                 z = z + half2;
                 z = z >= ql2 ? z - ql2 : z;
                 u64 tmp = lazy_reduce_u64(z, cst.bl, cst.bh, cst.q) + cst.half_mod;
-                y[e] = mul_shoup(y[e] + 3 * cst.q - tmp, cst.inv, cst.inv_shoup, cst.q);
+                y[e] = mul_shoup_u(y[e] + 3 * cst.q - tmp, cst.inv, cst.inv_shoup, cst.q);
             }
         }
         const SdConst cst = a.sd[s * a.sd_stride + i];
@@ -235,7 +235,7 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void sd_add_ntt_kernel(SdNttArgs
             u64 z = y[e] + half;
             z = z >= ql ? z - ql : z;
             u64 tmp = lazy_reduce_u64(z, mi.barrett_lo, mi.barrett_hi, p) + cst.half_mod;
-            x[e] = mul_shoup(x[e] + 3 * p - tmp, cst.inv, cst.inv_shoup, p);
+            x[e] = mul_shoup_u(x[e] + 3 * p - tmp, cst.inv, cst.inv_shoup, p);
         }
     }
     if (a.add_pb) {
@@ -274,57 +274,83 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void sd_add_ntt_kernel(SdNttArgs
 // adds the transformed power-basis part.
 // ---------------------------------------------------------------------------
 
-// One thread per coefficient of key limb i, both parts (key_switching_key.rs
-// :227-239): the switch-down factor q_l^-1 is folded into the key rows of
-// the limbs below the dropped one (fin0, fin1), Y < 2q and the key rows < q,
-// so four products stay below 2^127 and one Barrett reduction per four terms
-// gives the exact residue. Limb i == L (the dropped one, when extra) goes to
-// `special`. grid = (N / 256, L + extra, instances)
-__global__ __launch_bounds__(256) void ks_mac_kernel(KsFinishArgs f, u32 logn) {
+// One thread per coefficient of key limb i, both parts, for KS_MAC_INST
+// instances (key_switching_key.rs:227-239): the key values of a digit are
+// loaded once and used for every instance of the group. The switch-down
+// factor q_l^-1 is folded into the key rows of the limbs below the dropped
+// one (fin0, fin1), Y < 2q and the key rows < q, so four products stay below
+// 2^127; their exact sum (column accumulator) gets one Barrett reduction.
+// Limb i == L (the dropped one, when extra) goes to `special`.
+// grid = (N / 256, L + extra, ceil(instances / KS_MAC_INST))
+constexpr u32 KS_MAC_INST = 4;
+
+__global__ __launch_bounds__(256) void ks_mac_kernel(KsFinishArgs f, u32 logn, u32 inst) {
     const SdNttArgs& a = f.sd;
     const u32 j = blockIdx.x * 256 + threadIdx.x;
     if (j >> logn) return;
-    const u32 i = blockIdx.y, b = blockIdx.z;
+    const u32 i = blockIdx.y, b0 = blockIdx.z * KS_MAC_INST;
+    const u32 nb = inst - b0 < KS_MAC_INST ? inst - b0 : KS_MAC_INST;
     const bool dropped = i == a.L;
     const ModInfo mi = dropped ? *f.special_mod : a.mods[i];
     const u64 p = mi.p;
     const size_t ystride = (size_t)f.key_limbs << logn, kstride = (size_t)(a.L + a.extra) << logn;
-    const u64* yb = f.y + (((size_t)b * f.digits * f.key_limbs + i) << logn) + j;
+    const size_t yinst = (size_t)f.digits * f.key_limbs << logn;
+    const u64* yb = f.y + (((size_t)b0 * f.digits * f.key_limbs + i) << logn) + j;
     const u64* k0 = f.fin0 + ((size_t)i << logn) + j;
     const u64* k1 = f.fin1 + ((size_t)i << logn) + j;
-    u64 m0 = 0, m1 = 0;
+    u64 m0[KS_MAC_INST], m1[KS_MAC_INST];
+#pragma unroll
+    for (u32 q = 0; q < KS_MAC_INST; q++) m0[q] = m1[q] = 0;
     for (u32 d0 = 0; d0 < f.digits; d0 += 4) {
         const u32 dn = f.digits - d0 < 4 ? f.digits - d0 : 4;
-        u64 l0 = 0, h0 = 0, l1 = 0, h1 = 0;
+        u64 kv0[4], kv1[4];
 #pragma unroll
         for (u32 d = 0; d < 4; d++) {
-            if (d < dn) {
-                const u64 yv = yb[(d0 + d) * ystride];
-                u64 lo, hi;
-                mul_wide(yv, k0[(d0 + d) * kstride], lo, hi);
-                add128(l0, h0, lo, hi);
-                mul_wide(yv, k1[(d0 + d) * kstride], lo, hi);
-                add128(l1, h1, lo, hi);
+            kv0[d] = d < dn ? k0[(d0 + d) * kstride] : 0;
+            kv1[d] = d < dn ? k1[(d0 + d) * kstride] : 0;
+        }
+#pragma unroll
+        for (u32 q = 0; q < KS_MAC_INST; q++) {
+            if (q < nb) {
+                Acc3 s0, s1;
+                acc3_init(s0, 0);
+                acc3_init(s1, 0);
+#pragma unroll
+                for (u32 d = 0; d < 4; d++) {
+                    if (d < dn) {
+                        const u64 yv = yb[q * yinst + (d0 + d) * ystride];
+                        acc3_mul<false>(s0, yv, kv0[d]);
+                        acc3_mul<false>(s1, yv, kv1[d]);
+                    }
+                }
+                u64 l0, h0, t0, l1, h1, t1;
+                acc3_value(s0, l0, h0, t0);
+                acc3_value(s1, l1, h1, t1);
+                m0[q] = add_mod(m0[q], reduce_u128(l0, h0, mi.barrett_lo, mi.barrett_hi, p), p);
+                m1[q] = add_mod(m1[q], reduce_u128(l1, h1, mi.barrett_lo, mi.barrett_hi, p), p);
             }
         }
-        m0 = add_mod(m0, reduce_u128(l0, h0, mi.barrett_lo, mi.barrett_hi, p), p);
-        m1 = add_mod(m1, reduce_u128(l1, h1, mi.barrett_lo, mi.barrett_hi, p), p);
-    }
-    if (dropped) {
-        f.special[(((size_t)b * 2) << logn) + j] = m0;
-        f.special[(((size_t)b * 2 + 1) << logn) + j] = m1;
-        return;
     }
     const u32 src = a.perm ? a.perm[j] : j;
-    u64* out = a.out + (size_t)b * a.out_inst_stride + ((size_t)i << logn) + j;
-    u64 m[2] = {m0, m1};
 #pragma unroll
-    for (u32 part = 0; part < 2; part++) {
-        if (a.add_ntt && part < a.add_ntt_parts)
-            m[part] = add_mod(m[part], a.add_ntt[(size_t)b * a.add_ntt_inst_stride + (size_t)part * a.add_ntt_part_stride +
-                                                 ((size_t)i << logn) + src],
-                              p);
-        out[(size_t)part * a.out_part_stride] = m[part];
+    for (u32 q = 0; q < KS_MAC_INST; q++) {
+        if (q >= nb) break;
+        const u32 b = b0 + q;
+        if (dropped) {
+            f.special[(((size_t)b * 2) << logn) + j] = m0[q];
+            f.special[(((size_t)b * 2 + 1) << logn) + j] = m1[q];
+            continue;
+        }
+        u64* out = a.out + (size_t)b * a.out_inst_stride + ((size_t)i << logn) + j;
+        u64 m[2] = {m0[q], m1[q]};
+#pragma unroll
+        for (u32 part = 0; part < 2; part++) {
+            if (a.add_ntt && part < a.add_ntt_parts)
+                m[part] = add_mod(m[part], a.add_ntt[(size_t)b * a.add_ntt_inst_stride +
+                                                     (size_t)part * a.add_ntt_part_stride + ((size_t)i << logn) + src],
+                                  p);
+            out[(size_t)part * a.out_part_stride] = m[part];
+        }
     }
 }
 
@@ -356,7 +382,7 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void ks_finish_kernel(KsFinishAr
             u64 yv = lrow[j] + half;
             yv = yv >= ql ? yv - ql : yv;
             const u64 tmp = lazy_reduce_u64(yv, mi.barrett_lo, mi.barrett_hi, p) + cst.half_mod;
-            u64 z = mul_shoup(3 * p - tmp, cst.inv, cst.inv_shoup, p);
+            u64 z = mul_shoup_u(3 * p - tmp, cst.inv, cst.inv_shoup, p);
             if (ad) z = add_mod(z, ad[j], p);
             x[e] = z;
         }
@@ -438,8 +464,9 @@ void launch_lift_ntt(int logn, const KeySwitchArgs& a, u64* y, u32 inst, hipStre
 void launch_ks_mac(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s) {
     if (!f.sd.L || !inst) return;
     ProfScope ps("ks_mac", logn, s);
-    hipLaunchKernelGGL(ks_mac_kernel, dim3(((1u << logn) + 255) / 256, f.sd.L + f.sd.extra, inst), dim3(256), 0, s, f,
-                       (u32)logn);
+    hipLaunchKernelGGL(ks_mac_kernel,
+                       dim3(((1u << logn) + 255) / 256, f.sd.L + f.sd.extra, (inst + KS_MAC_INST - 1) / KS_MAC_INST),
+                       dim3(256), 0, s, f, (u32)logn, inst);
     HIPCHK(hipGetLastError());
 }
 

@@ -283,25 +283,40 @@ __device__ __forceinline__ InvScale inv_scale_nr(const ModInfo& mi) {
     return InvScale{mi.n_inv_r, mi.n_inv_r_shoup, mi.last_zr, mi.last_zr_shoup};
 }
 
+// Lanes per wave that hold part of the row (a row of fewer than 64 * E
+// values lives in one partial wave), and the first element of the block of
+// LANES * E consecutive elements that the wave of thread t owns.
+template <int LOGN, int R = 4>
+struct WaveBlock {
+    static constexpr int T = NttShape<LOGN, R>::T;
+    static constexpr int LB = T < 64 ? LOGN - NttShape<LOGN, R>::RE : 6;  // log2 lanes
+    static constexpr int RE = NttShape<LOGN, R>::RE;
+    __device__ static int base(int t) { return (t >> LB) << (LB + RE); }
+    __device__ static int lane(int t) { return t & ((1 << LB) - 1); }
+};
+
 // Coalesced global row -> registers in the inverse-transform input order
-// (value e of thread t is element t*E + e): the row is read with consecutive
-// lanes on consecutive elements and transposed through LDS. Must be followed
-// by the inverse transform (its first LDS writes stay in each wave's block).
+// (value e of thread t is element t*E + e). Each wave reads its own block of
+// 64 * E elements with consecutive lanes on consecutive elements and
+// transposes it through its own part of the LDS, so no wave waits for
+// another: waves that have their data start transforming while others are
+// still loading. Must be followed by the inverse transform (whose first
+// exchange writes the same block).
 template <int LOGN, int UNROLL = 4, int R = 4, class Load>
 __device__ __forceinline__ void load_inv_order(u64* lds, u64 (&x)[(NttShape<LOGN, R>::E)], int t, Load load) {
     using S = NttShape<LOGN, R>;
+    using W = WaveBlock<LOGN, R>;
+    const int base = W::base(t), lane = W::lane(t);
     // Partial unrolling bounds the number of loads in flight (and registers).
 #pragma unroll UNROLL
     for (int e = 0; e < S::E; e++) {
-        const int j = e * S::T + t;
+        const int j = base + (e << W::LB) + lane;
         lds[lds_slot(j)] = load(j);
     }
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int e = 0; e < S::E; e++) x[e] = lds[lds_slot((t << S::RE) + e)];
-    // The reads are in the inverse transform's first layout, whose first
-    // exchange writes the same wave-local block.
-    exchange_sync<inv_local<0>()>();
+    wave_sync();
 }
 
 // Registers in the inverse-transform output order -> row stores: value e of
@@ -318,17 +333,20 @@ __device__ __forceinline__ void store_inv_order(u64* __restrict__ dst, const u64
 }
 
 // Registers in the forward-transform output order (value e of thread t is
-// element t*E + e) -> coalesced `store(j, value)`. Must follow a forward
-// transform (whose last LDS access is fenced by a barrier).
+// element t*E + e) -> coalesced `store(j, value)`, transposed per wave
+// through the wave's own block of the LDS (as in load_inv_order). Must follow
+// a forward transform, whose last exchange reads the same block.
 template <int LOGN, int R = 4, int UNROLL = 4, class Store>
 __device__ __forceinline__ void store_fwd_order(u64* lds, const u64 (&x)[(NttShape<LOGN, R>::E)], int t, Store store) {
     using S = NttShape<LOGN, R>;
+    using W = WaveBlock<LOGN, R>;
 #pragma unroll
     for (int e = 0; e < S::E; e++) lds[lds_slot((t << S::RE) + e)] = x[e];
-    __syncthreads();
+    wave_sync();
+    const int base = W::base(t), lane = W::lane(t);
 #pragma unroll UNROLL
     for (int e = 0; e < S::E; e++) {
-        const int j = e * S::T + t;
+        const int j = base + (e << W::LB) + lane;
         store(j, lds[lds_slot(j)]);
     }
 }
