@@ -251,7 +251,7 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void sd_add_ntt_kernel(SdNttArgs
         const u64* g = a.add_ntt + (size_t)b * a.add_ntt_inst_stride + (size_t)part * a.add_ntt_part_stride +
                        ((size_t)i << LOGN);
         const u32* perm = a.perm;
-        store_fwd_order<LOGN>(lds, x, t, [&](int j, u64 v) {
+        store_fwd_order<LOGN, 4, 16>(lds, x, t, [&](int j, u64 v) {
             dst[j] = add_mod(reduce1(reduce1(v, p2), p), g[perm ? perm[j] : (u32)j], p);
         });
     } else {
@@ -392,7 +392,7 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void ks_finish_kernel(KsFinishAr
     }
     ntt_fwd_regs<LOGN>(lds, x, t, a.fwd_tw + ((size_t)i << LOGN), p);
     u64* dst = a.out + (size_t)b * a.out_inst_stride + (size_t)part * a.out_part_stride + ((size_t)i << LOGN);
-    store_fwd_order<LOGN>(lds, x, t, [&](int j, u64 v) { dst[j] = add_mod(reduce1(reduce1(v, p2), p), dst[j], p); });
+    store_fwd_order<LOGN, 4, 16>(lds, x, t, [&](int j, u64 v) { dst[j] = add_mod(reduce1(reduce1(v, p2), p), dst[j], p); });
 }
 
 // ---------------------------------------------------------------------------

@@ -101,8 +101,8 @@ __device__ __forceinline__ void fwd_pass(u64 (&x)[(NttShape<LOGN, R>::E)], int t
                 fwd_bfly<S0V == 0>(x[k0], x[k0 + half], w.x, w.y, p, p2);
             }
         }
-        // With 32 values per thread, keep the next stage's twiddle loads from
-        // being hoisted above this one (bounds the live registers).
+        // Keep the next stage's twiddle loads from being hoisted above this
+        // one (bounds the live registers).
         if constexpr (RE >= 4) asm volatile("" ::: "memory");
     }
 }
@@ -246,7 +246,9 @@ __device__ __forceinline__ void inv_rest(u64* lds, u64 (&x)[(NttShape<LOGN, R>::
         lds_put_inv<LOGN, B0V, R>(lds, x, t);
         exchange_sync<A && B>();
         lds_get_inv<LOGN, NEXT, R>(lds, x, t);
-        exchange_sync<B>();
+        // After the last exchange the LDS is not written again (the callers
+        // store the result from registers), so its reads need no fence.
+        if constexpr (NEXT + RE < LOGN) exchange_sync<B>();
         inv_pass<LOGN, NEXT, VMIN, R, SCALE>(x, t, tw, p, p2, sc);
         inv_rest<LOGN, NEXT, R, SCALE>(lds, x, t, tw, p, p2, sc);
     }
@@ -265,7 +267,8 @@ __device__ __forceinline__ void ntt_fwd_regs(u64* lds, u64 (&x)[(NttShape<LOGN, 
 
 // Full inverse transform on registers: input order inv_index<LOGN,0> (E
 // consecutive values per thread), output order inv_index<LOGN,
-// last_start<LOGN>()>. The final multiplication (by N^-1, or by N^-1 2^64
+// last_start<LOGN>()>. Its last LDS reads are not fenced: the caller must
+// not write the LDS afterwards. The final multiplication (by N^-1, or by N^-1 2^64
 // after Montgomery products) is folded into the last stage through `sc`;
 // the outputs are reduced.
 template <int LOGN, int R = 4>
