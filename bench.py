@@ -63,10 +63,12 @@ def kernel_bytes(name, B, L, Lk, M):
         return B * 3 * (M + L) * row
     if kind == "lift_ntt":       # read c2 (L rows), write the lifted, transformed digits (L x Lk rows)
         return B * (L + L * Lk) * row
-    if kind == "ks_special":     # read the digits of the special limb, write 2 power-basis rows; key rows once
-        return B * (L + 2) * row + 2 * L * 2 * row
-    if kind == "ks_finish":      # read digits of limbs < L, 2 special rows, 2L c0/c1 rows; write 2L rows; keys once
-        return B * (L * L + 2 + 4 * L) * row + 2 * L * L * 2 * row
+    if kind == "ks_mac":         # read the L x Lk transformed digits, write 2 (L + 1) rows; key rows once
+        return B * (L * Lk + 2 * Lk) * row + 2 * L * Lk * row
+    if kind == "ntt_backward":   # the dropped limb's two rows, in place
+        return B * 2 * row * 2
+    if kind == "ks_finish":      # read 2 dropped-limb rows, 2L c0/c1 rows and 2L outputs; write 2L rows
+        return B * (2 + 6 * L) * row
     return None
 
 
@@ -579,6 +581,18 @@ def main():
                 "kernel": dname, "avg_launch_ms": avg_s * 1e3, "algorithmic_bytes": abytes,
                 "valu_busy_pct": valu_busy, "valu_wave_instructions_per_cu_cycle": valu_rate}
 
+    # The same figures for every kernel of the step (HBM-bound accounting; the
+    # VALU columns come from the committed counter summary).
+    kernel_roofline = {}
+    for kname, (kc, kms) in sorted(kernels.items()):
+        kb = kernel_bytes(kname, B, L, Lk, M)
+        ks = kms / kc / 1e3
+        vb, vr = load_valu(kname.split("/")[0])
+        kernel_roofline[kname] = {"avg_launch_ms": ks * 1e3, "algorithmic_bytes": kb,
+                                  "achieved_GBps": kb / ks / 1e9 if kb else None,
+                                  "hbm_frac": kb / ks / 1e9 / HBM_PEAK_GBS if kb else None,
+                                  "traffic": load_traffic(kname.split("/")[0]), "valu_busy_pct": vb}
+
     extras = {}
     if not args.no_extras:
         extras["single_instance"] = single_instance_latency(fhehip, torch, 200)
@@ -626,6 +640,7 @@ def main():
         "kernels_ms": {k: {"launches": c, "avg_ms": ms / c} for k, (c, ms) in sorted(kernels.items())},
         "ntt_autotune": ntt_tuning,
         "roofline": roofline,
+        "kernel_roofline": kernel_roofline,
         "cpu_baseline": cpu,
         **extras,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,

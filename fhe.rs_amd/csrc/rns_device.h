@@ -140,6 +140,32 @@ __device__ __forceinline__ void acc3_mul(Acc3& s, u64 r, u64 w) {
             : [r0] "v"((u32)r), [r1] "v"((u32)(r >> 32)), [w0] "v"((u32)w), [w1] "v"((u32)(w >> 32)));
 }
 
+// s = r * w (starts an accumulator without zeroing it first): the only
+// carry is the one of the middle column.
+#define FHEHIP_ACC3_FIRST_ASM                              \
+    "v_mad_u64_u32 %[b], %[k0], %[r0], %[w1], 0\n\t"      \
+    "v_mad_u64_u32 %[a], %[k1], %[r0], %[w0], 0\n\t"      \
+    "v_mad_u64_u32 %[c], %[k1], %[r1], %[w1], 0\n\t"      \
+    "v_mad_u64_u32 %[b], %[k0], %[r1], %[w0], %[b]\n\t"   \
+    "v_mov_b32 %[ka], 0\n\t"                              \
+    "v_mov_b32 %[kc], 0\n\t"                              \
+    "v_addc_co_u32 %[kb], %[k1], 0, 0, %[k0]"
+
+template <bool SW>
+__device__ __forceinline__ void acc3_first(Acc3& s, u64 r, u64 w) {
+    u64 k0, k1;
+    if constexpr (SW)
+        asm(FHEHIP_ACC3_FIRST_ASM
+            : [a] "=&v"(s.a), [b] "=&v"(s.b), [c] "=&v"(s.c), [ka] "=&v"(s.ka), [kb] "=&v"(s.kb), [kc] "=&v"(s.kc),
+              [k0] "=&s"(k0), [k1] "=&s"(k1)
+            : [r0] "v"((u32)r), [r1] "v"((u32)(r >> 32)), [w0] "s"((u32)w), [w1] "s"((u32)(w >> 32)));
+    else
+        asm(FHEHIP_ACC3_FIRST_ASM
+            : [a] "=&v"(s.a), [b] "=&v"(s.b), [c] "=&v"(s.c), [ka] "=&v"(s.ka), [kb] "=&v"(s.kb), [kc] "=&v"(s.kc),
+              [k0] "=&s"(k0), [k1] "=&s"(k1)
+            : [r0] "v"((u32)r), [r1] "v"((u32)(r >> 32)), [w0] "v"((u32)w), [w1] "v"((u32)(w >> 32)));
+}
+
 // The 192-bit value of an accumulator (it is below 2^192).
 __device__ __forceinline__ void acc3_value(const Acc3& s, u64& lo, u64& hi, u64& top) {
     unsigned long long c1, c2, c3;
@@ -212,10 +238,10 @@ __device__ __forceinline__ void rns_scale_column(const u64* __restrict__ sc, Res
     U256 sum;
     {
         Acc3 sa, sb;
-        acc3_init(sa, 0);
-        acc3_init(sb, 0);
+        acc3_first<true>(sa, rests(0), tg[0]);
+        acc3_first<true>(sb, rests(0), tg[1]);
 #pragma unroll
-        for (int j = 0; j < nf; j++) {
+        for (int j = 1; j < nf; j++) {
             acc3_mul<true>(sa, rests(j), tg[2 * j]);
             acc3_mul<true>(sb, rests(j), tg[2 * j + 1]);
         }
@@ -230,33 +256,38 @@ __device__ __forceinline__ void rns_scale_column(const u64* __restrict__ sc, Res
     bool w_sign = false;
     u128 w = 0;
     if (!is_one) {
-        // sum_j +-r_j theta_omega_j with the signs of the constants: the
-        // positive and negative terms are summed apart (the signs are uniform
-        // over the columns) and subtracted modulo 2^256.
-        Acc3 pa, pb, na, nb;  // positive and negative terms, lo and hi halves of theta
-        acc3_init(pa, 0);
-        acc3_init(pb, 0);
-        acc3_init(na, 0);
-        acc3_init(nb, 0);
+        // sum_j +-r_j theta_omega_j with the signs of the constants, without
+        // a branch on the sign: a negative theta is taken as its 128-bit
+        // complement 2^128 - theta, so that
+        //   sum = sum_j r_j theta'_j - 2^128 sum_{negative j} r_j
+        // exactly (the signs and constants are uniform, so theta' and the
+        // mask are scalar values).
+        Acc3 wa, wb;
+        u64 rn0 = 0, rn1 = 0;
 #pragma unroll
         for (int j = 0; j < nf; j++) {
-            if (tom[3 * j + 2]) {
-                acc3_mul<true>(na, rests(j), tom[3 * j]);
-                acc3_mul<true>(nb, rests(j), tom[3 * j + 1]);
+            const u64 lo = tom[3 * j], hi = tom[3 * j + 1];
+            const bool neg = tom[3 * j + 2] != 0 && (lo | hi) != 0;
+            const u64 clo = neg ? 0 - lo : lo;
+            const u64 chi = neg ? ~hi + (lo == 0 ? 1 : 0) : hi;
+            if (j == 0) {
+                acc3_first<true>(wa, rests(j), clo);
+                acc3_first<true>(wb, rests(j), chi);
             } else {
-                acc3_mul<true>(pa, rests(j), tom[3 * j]);
-                acc3_mul<true>(pb, rests(j), tom[3 * j + 1]);
+                acc3_mul<true>(wa, rests(j), clo);
+                acc3_mul<true>(wb, rests(j), chi);
             }
+            add128(rn0, rn1, rests(j) & (neg ? ~0ull : 0ull), 0);
         }
-        u64 p0, p1, p2, q0, q1, q2, n0, n1, n2, m0, m1, m2;
-        acc3_value(pa, p0, p1, p2);
-        acc3_value(pb, q0, q1, q2);
-        acc3_value(na, n0, n1, n2);
-        acc3_value(nb, m0, m1, m2);
-        U256 so, neg;
+        u64 p0, p1, p2, q0, q1, q2;
+        acc3_value(wa, p0, p1, p2);
+        acc3_value(wb, q0, q1, q2);
+        U256 so, corr;
         u256_combine(so, p0, p1, p2, q0, q1, q2);
-        u256_combine(neg, n0, n1, n2, m0, m1, m2);
-        u256_sub(so, neg);
+        corr.w[0] = corr.w[1] = 0;
+        corr.w[2] = rn0;
+        corr.w[3] = rn1;
+        u256_sub(so, corr);
         // v * theta_gamma (128 x 128 -> 256)
         const u64 tgl = sc[SC_TG_LO], tgh = sc[SC_TG_HI];
         U256 vt;
@@ -303,8 +334,7 @@ __device__ __forceinline__ void rns_scale_column(const u64* __restrict__ sc, Res
         const u64 q = o[0], bl = o[1], bh = o[2];
         // o[3] = q - gamma, o[4] = [2^64 (q - gamma)]_q, o[5] = [2^64]_q, o[6] = [2^128]_q
         Acc3 acc;
-        acc3_init(acc, 0);
-        acc3_mul<true>(acc, v0, o[3]);
+        acc3_first<true>(acc, v0, o[3]);
         acc3_mul<true>(acc, v1, o[4]);
         if (!is_one) {
             // +w, or -w = w (q - 1) mod q, without a branch on the column's
