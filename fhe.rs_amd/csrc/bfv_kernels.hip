@@ -63,10 +63,11 @@ void launch_rns_scale(const u64* in, size_t in_inst, size_t in_part, u64* out, s
 // (Poly::substitute in the Ntt representation, math/src/rq/mod.rs:324-346,
 // as q[m] = p[perm[m]]). grid = (limbs, parts, instances).
 // ---------------------------------------------------------------------------
-template <int LOGN>
-__global__ __launch_bounds__(NttShape<LOGN>::T) void intt_rows_kernel(InttRowsArgs a) {
+// The gather and the plain copy are separate kernels (PERM), so that each
+// gets its own register allocation.
+template <int LOGN, bool PERM>
+__device__ __forceinline__ void intt_rows_body(u64* lds, const InttRowsArgs& a) {
     using S = NttShape<LOGN>;
-    __shared__ u64 lds[S::LDS_WORDS];
     const int t = threadIdx.x;
     const u32 l = blockIdx.x, k = blockIdx.y, b = blockIdx.z;
     const u64* src = a.src[k] + (size_t)b * a.src_inst_stride + ((size_t)l << LOGN);
@@ -75,7 +76,7 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void intt_rows_kernel(InttRowsAr
     const ulonglong2* tw = a.inv_tw + ((size_t)limb << LOGN);
     const ModInfo mi = a.mods[limb];
     u64 x[S::E];
-    if (a.perm) {
+    if constexpr (PERM) {
         const u32* perm = a.perm;
         load_inv_order<LOGN, 16>(lds, x, t, [&](int j) { return src[perm[j]]; });
     } else {
@@ -83,6 +84,18 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void intt_rows_kernel(InttRowsAr
     }
     ntt_inv_regs<LOGN>(lds, x, t, tw, mi.p, inv_scale_n(mi));
     store_inv_order<LOGN>(dst, x, t);
+}
+
+template <int LOGN>
+__global__ __launch_bounds__(NttShape<LOGN>::T) void intt_rows_kernel(InttRowsArgs a) {
+    __shared__ u64 lds[NttShape<LOGN>::LDS_WORDS];
+    intt_rows_body<LOGN, false>(lds, a);
+}
+
+template <int LOGN>
+__global__ __launch_bounds__(NttShape<LOGN>::T) void intt_rows_gather_kernel(InttRowsArgs a) {
+    __shared__ u64 lds[NttShape<LOGN>::LDS_WORDS];
+    intt_rows_body<LOGN, true>(lds, a);
 }
 
 // ---------------------------------------------------------------------------
@@ -442,7 +455,11 @@ void launch_intt_rows(int logn, const InttRowsArgs& a, u32 limbs, u32 parts, u32
     if (!limbs || !parts || !inst) return;
     ProfScope ps(a.perm ? "intt_rows_gather" : "intt_rows", logn, s);
     dim3 grid(limbs, parts, inst);
-    FHEHIP_LOGN_DISPATCH(logn, intt_rows_kernel, grid, 0, s, a);
+    if (a.perm) {
+        FHEHIP_LOGN_DISPATCH(logn, intt_rows_gather_kernel, grid, 0, s, a);
+    } else {
+        FHEHIP_LOGN_DISPATCH(logn, intt_rows_kernel, grid, 0, s, a);
+    }
 }
 
 void launch_tensor_intt(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_t s) {

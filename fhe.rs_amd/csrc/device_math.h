@@ -137,7 +137,8 @@ __device__ __forceinline__ void fwd_bfly_asm(u64& x, u64& y, u64 w, u64 ws, u64 
 
 // Gentleman-Sande inverse butterfly (ntt/native.rs:288-300) on x, y < 2p:
 // x' = (x + y) mod 2p, y' = lazy_mul_shoup(x + 2p - y). The reduction adds
-// (u < 2p ? 0 : -2p) to u = x + y.
+// (u < 2p ? 0 : -2p) to u = x + y, with -2p masked by a lane mask so that it
+// can stay in SGPRs.
 #define FHEHIP_INV_BFLY_ASM                                \
     "v_sub_co_u32 v6, %[k], %[x0], %[y0]\n\t"              \
     "v_mov_b32 v1, 0\n\t"                                  \
@@ -160,15 +161,16 @@ __device__ __forceinline__ void fwd_bfly_asm(u64& x, u64& y, u64 w, u64 ws, u64 
     "v_mul_lo_u32 v1, v1, %[n0]\n\t"                       \
     "v_add3_u32 v7, v7, v6, v0\n\t"                        \
     "v_add3_u32 v3, v3, v7, v1\n\t"                        \
-    "v_cndmask_b32 v0, %[m0], 0, vcc\n\t"                  \
-    "v_cndmask_b32 v1, %[m1], 0, vcc\n\t"                  \
+    "v_cndmask_b32 v1, -1, 0, vcc\n\t"                     \
+    "v_and_b32 v0, %[m0], v1\n\t"                          \
+    "v_and_b32 v1, %[m1], v1\n\t"                          \
     "v_mov_b64 %[yo], v[2:3]\n\t"                          \
     "v_lshl_add_u64 %[xo], v[4:5], 0, v[0:1]"
 
 #define FHEHIP_INV_BFLY_IN(C)                                                                                  \
     [x0] "v"((u32)x), [x1] "v"((u32)(x >> 32)), [y0] "v"((u32)y), [y1] "v"((u32)(y >> 32)), [x] "v"(x),       \
         [y] "v"(y), [s0] C((u32)zs), [s1] C((u32)(zs >> 32)), [w0] C((u32)z), [w1] C((u32)(z >> 32)),        \
-        [n0] "s"((u32)n), [n1] "s"((u32)(n >> 32)), [P] "s"(p2), [m0] "v"((u32)m2), [m1] "v"((u32)(m2 >> 32))
+        [n0] "s"((u32)n), [n1] "s"((u32)(n >> 32)), [P] "s"(p2), [m0] "s"((u32)m2), [m1] "s"((u32)(m2 >> 32))
 
 template <bool UT = false>
 __device__ __forceinline__ void inv_bfly_asm(u64& x, u64& y, u64 z, u64 zs, u64 p, u64 p2) {
