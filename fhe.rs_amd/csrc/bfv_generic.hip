@@ -118,7 +118,9 @@ __global__ __launch_bounds__(256) void sd_z_kernel(KsFinishArgs f, u64* __restri
         const u64 ql = a.extra_mods[0];
         u64 yv = f.special[(((size_t)b * 2 + part) << logn) + e] + (ql >> 1);
         yv = yv >= ql ? yv - ql : yv;
-        const u64 tmp = lazy_reduce_u64(yv, mi.barrett_lo, mi.barrett_hi, p) + cst.half_mod;
+        // y < q_l <= 2 q_i is already a valid lazy [y]_i (see ks_finish_kernel)
+        const u64 yr = ql <= 2 * p ? yv : lazy_reduce_u64(yv, mi.barrett_lo, mi.barrett_hi, p);
+        const u64 tmp = yr + cst.half_mod;
         z = mul_shoup_u(3 * p - tmp, cst.inv, cst.inv_shoup, p);
     }
     if (a.add_pb)

@@ -1,6 +1,8 @@
 // Kernels of the BFV evaluation path: RNS scaling columns, INTT with optional
 // Galois gather, fused tensor + INTT, key switching (digit lift + lazy NTT +
 // Shoup multiply-accumulate), and fused switch-down + add + NTT.
+#include <type_traits>
+
 #include "bfv_launch.h"
 #include "ntt_device.h"
 #include "profiler.h"
@@ -187,9 +189,18 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void lift_ntt_kernel(KeySwitchAr
     const u64 p = mi.p;
     const u64* src = a.src + (size_t)b * a.src_inst_stride + ((size_t)i << LOGN);
     u64 x[S::E];
+    // Digit i is below q_i (the ciphertext moduli are the first key moduli).
+    // Below 4 q'_j it is already a valid input of the lazy forward transform:
+    // the stored outputs are then other representatives (< 2 q'_j) of the
+    // same residues, which the exact key inner product reduces.
+    if (a.key_mods[i].p <= 4 * p) {
 #pragma unroll
-    for (int e = 0; e < S::E; e++)
-        x[e] = lazy_reduce_u64(src[fwd_index<LOGN, 0>(t, e)], mi.barrett_lo, mi.barrett_hi, p);
+        for (int e = 0; e < S::E; e++) x[e] = src[fwd_index<LOGN, 0>(t, e)];
+    } else {
+#pragma unroll
+        for (int e = 0; e < S::E; e++)
+            x[e] = lazy_reduce_u64(src[fwd_index<LOGN, 0>(t, e)], mi.barrett_lo, mi.barrett_hi, p);
+    }
     ntt_fwd_regs<LOGN>(lds, x, t, a.key_fwd_tw + ((size_t)j << LOGN), p);
     u64* dst = y + ((((size_t)b * a.digits + i) * a.key_limbs + j) << LOGN);
     const u64 p2 = 2 * p;
@@ -389,16 +400,26 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void ks_finish_kernel(KsFinishAr
         const u64 ql = a.extra_mods[0];
         const u64 half = ql >> 1;
         const u64* lrow = f.special + (((size_t)b * 2 + part) << LOGN);
+        auto body = [&](auto small) {
 #pragma unroll
-        for (int e = 0; e < S::E; e++) {
-            const int j = fwd_index<LOGN, 0>(t, e);
-            u64 yv = lrow[j] + half;
-            yv = yv >= ql ? yv - ql : yv;
-            const u64 tmp = lazy_reduce_u64(yv, mi.barrett_lo, mi.barrett_hi, p) + cst.half_mod;
-            u64 z = mul_shoup_u(3 * p - tmp, cst.inv, cst.inv_shoup, p);
-            if (ad) z = add_mod(z, ad[j], p);
-            x[e] = z;
-        }
+            for (int e = 0; e < S::E; e++) {
+                const int j = fwd_index<LOGN, 0>(t, e);
+                u64 yv = lrow[j] + half;
+                yv = yv >= ql ? yv - ql : yv;
+                // [y]_i only has to be < 2 q_i and congruent to y (the rest
+                // is exact modulo q_i and the output is reduced): y < q_l <=
+                // 2 q_i already is.
+                const u64 yr = decltype(small)::value ? yv : lazy_reduce_u64(yv, mi.barrett_lo, mi.barrett_hi, p);
+                const u64 tmp = yr + cst.half_mod;
+                u64 z = mul_shoup_u(3 * p - tmp, cst.inv, cst.inv_shoup, p);
+                if (ad) z = add_mod(z, ad[j], p);
+                x[e] = z;
+            }
+        };
+        if (ql <= 2 * p)
+            body(std::true_type{});
+        else
+            body(std::false_type{});
     } else {
 #pragma unroll
         for (int e = 0; e < S::E; e++) x[e] = ad ? ad[fwd_index<LOGN, 0>(t, e)] : 0;
