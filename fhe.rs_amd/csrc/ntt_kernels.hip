@@ -158,8 +158,13 @@ __global__ __launch_bounds__(256) void lift_split_kernel(const u64* __restrict__
     const ModInfo mi = mods[j];
     const u64 p = mi.p;
     const u64* row = src + b * src_inst + ((size_t)i << 15);
-    u64 x = lazy_reduce_u64(row[k], mi.barrett_lo, mi.barrett_hi, p);
-    u64 z = lazy_reduce_u64(row[k + 16384], mi.barrett_lo, mi.barrett_hi, p);
+    u64 x = row[k], z = row[k + 16384];
+    // Digit i is below q_i (a prefix of the key moduli); below 4 q'_j it is
+    // already a valid lazy transform input (as in lift_ntt_kernel).
+    if (mods[i].p > 4 * p) {
+        x = lazy_reduce_u64(x, mi.barrett_lo, mi.barrett_hi, p);
+        z = lazy_reduce_u64(z, mi.barrett_lo, mi.barrett_hi, p);
+    }
     const ulonglong2 w = split[3 * j];
     fwd_bfly<true>(x, z, w.x, w.y, p, 2 * p);
     u64* out = y + (((size_t)b * gridDim.y + blockIdx.y) << 15);
