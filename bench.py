@@ -538,6 +538,7 @@ def main():
     # Per-call GPU drop-in: one operation per call, into its own output buffer.
     call_ms, per_call_value = [], None
     per_call_graph = None
+    per_call_host = None
     if args.per_call_ops > 0:
         one_o = torch.empty(per, dtype=torch.int64, device=dev)
         for _ in range(3):
@@ -567,6 +568,31 @@ def main():
         per_call_graph = {"value": args.per_call_ops / (ge[0].elapsed_time(ge[-1]) / 1e3), "unit": UNIT,
                           "p50_ms": percentile(graph_ms, 50), "p99_ms": percentile(graph_ms, 99)}
         graph.close()
+        # The same call as a drop-in behind an API that holds ciphertexts in
+        # host memory (as fhe.rs does): pinned host inputs copied in, the
+        # product copied back, and the call waited on. Reported beside the
+        # device-resident figures; `value` and batched_over_per_call use those.
+        h_l = torch.empty(per, dtype=torch.int64, pin_memory=True)
+        h_r = torch.empty(per, dtype=torch.int64, pin_memory=True)
+        h_o = torch.empty(per, dtype=torch.int64, pin_memory=True)
+        d_l = torch.empty(per, dtype=torch.int64, device=dev)
+        d_r = torch.empty(per, dtype=torch.int64, device=dev)
+        h_l.copy_(lhs[:per])
+        h_r.copy_(rhs[:per])
+        host_ms = []
+        for k in range(args.per_call_ops + 3):
+            t1 = time.perf_counter()
+            d_l.copy_(h_l, non_blocking=True)
+            d_r.copy_(h_r, non_blocking=True)
+            mult.multiply(d_l, d_r, one_o, 1, stream)
+            h_o.copy_(one_o, non_blocking=True)
+            torch.cuda.synchronize()
+            if k >= 3:
+                host_ms.append((time.perf_counter() - t1) * 1e3)
+        per_call_host = {"value": len(host_ms) / (sum(host_ms) / 1e3), "unit": UNIT,
+                         "p50_ms": percentile(host_ms, 50), "p99_ms": percentile(host_ms, 99),
+                         "bytes_per_call_pcie": (3 * per) * 8,
+                         "bit_exact_vs_device_resident": bool(torch.equal(h_o, one_o.cpu()))}
 
     # Roofline of the dominant kernel.
     dom = max(kernels.items(), key=lambda kv: kv[1][1])
@@ -635,6 +661,8 @@ def main():
         "per_call": {"value": per_call_value, "unit": UNIT, "p50_ms": percentile(call_ms, 50),
                      "p99_ms": percentile(call_ms, 99), "ops": args.per_call_ops},
         "per_call_graph": per_call_graph,
+        "per_call_host_buffers": per_call_host,
+        "batched_over_per_call_host_buffers": (value / per_call_host["value"]) if per_call_host else None,
         "batched_over_per_call": (value / per_call_value) if per_call_value else None,
         "bit_exact_vs_oracle": bit_exact,
         "kernels_ms": {k: {"launches": c, "avg_ms": ms / c} for k, (c, ms) in sorted(kernels.items())},

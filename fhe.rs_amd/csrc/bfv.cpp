@@ -377,8 +377,53 @@ std::unique_ptr<Multiplicator> Multiplicator::make_default(std::shared_ptr<Ksk> 
     return m;
 }
 
+// ---------------------------------------------------------------------------
+Switcher::Switcher(std::shared_ptr<DeviceContext> f, std::shared_ptr<DeviceContext> t)
+    : from(std::move(f)), to(std::move(t)) {
+    if (from->N != to->N) throw Error(-4, "Incompatible degrees");
+    if (from->L < 2) throw Error(-1, "a single-modulus context cannot be switched down");
+    if (to->L != from->L - 1 || !std::equal(to->moduli.begin(), to->moduli.end(), from->moduli.begin()))
+        throw Error(-4, "the target context must be the source context without its last modulus");
+    // switch_down constants for dropping q_l (rq/mod.rs:390-449)
+    const u64 ql = from->moduli.back();
+    std::vector<SdConst> sd(to->L);
+    for (u32 m = 0; m < to->L; m++) {
+        const u64 q = to->moduli[m];
+        SdConst& c = sd[m];
+        c.q = q;
+        c.bl = to->h_mod[m].barrett_lo;
+        c.bh = to->h_mod[m].barrett_hi;
+        c.half_mod = q - ((ql >> 1) % q);
+        c.inv = invmod64(ql % q, q);
+        c.inv_shoup = shoup64(c.inv, q);
+    }
+    consts.reset(sd.size() * sizeof(SdConst));
+    HIPCHK(hipMemcpy(consts.ptr, sd.data(), consts.bytes, hipMemcpyHostToDevice));
+}
+
+// Ciphertext::switch_down (ciphertext.rs:86-97) as fhe.rs runs it: every
+// limb to power basis, Poly::switch_down, the remaining limbs back to the
+// NTT domain.
+void Switcher::switch_down(const u64* in, u64* out, u32 polys, hipStream_t s) const {
+    check_batch(polys);
+    if (!polys) return;
+    const size_t n = from->N, L = from->L;
+    u64* pb = scratch(Scratch::ModSwitch).get(s, (size_t)polys * L * n * 8).as<u64>();
+    InttRowsArgs ia{};
+    ia.src[0] = in;
+    ia.src_inst_stride = L * n;
+    ia.dst = pb;
+    ia.dst_inst_stride = L * n;
+    ia.dst_part_stride = L * n;
+    ia.inv_tw = from->inv_tw.as<ulonglong2>();
+    ia.mods = from->mod.as<ModInfo>();
+    intt_rows(*from, ia, (u32)L, 1, polys, s);
+    launch_switch_down_pb(pb, out, consts.as<SdConst>(), from->moduli.back(), (u32)L, from->logn, polys, s);
+    launch_ntt_forward(*to, RowMap{out, (u32)(L - 1), (u32)(L - 1), 0, 0}, polys * (u32)(L - 1), false, s);
+}
+
 // Multiplicator::multiply (fhe/src/bfv/ops/mul.rs:158-227) on a batch.
-void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out, u32 inst, hipStream_t s) {
+void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out_final, u32 inst, hipStream_t s) {
     check_batch(inst);
     const size_t n = base->N, L = base->L, M = mul->L;
     const u32 cl = ext_lhs->common, cr = ext_rhs->common;
@@ -392,6 +437,10 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out, u32 inst,
     u64* wX = wP + P;
     u64* wT = wX + X;
     u64* wD = wT + T;
+    // With modulus switching the product lands in wP (free after the tensor
+    // product; [inst][3][L][N] fits in its [inst][4][L][N]) and is switched
+    // down into the caller's buffer at the end (mul.rs:221-223).
+    u64* out = mod_switch ? wP : out_final;
     // 1. inverse NTT of the four input polynomials (rq/scaler.rs:68-78)
     InttRowsArgs ia{};
     ia.src[0] = lhs;
@@ -447,12 +496,15 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out, u32 inst,
     if (!rk) {
         RowMap m{out, (u32)L, (u32)L, 0, 0};
         launch_ntt_forward(*base, m, inst * 3 * (u32)L, false, s);
+        if (mod_switch) mod_switch->switch_down(out, out_final, inst * 3, s);
         return;
     }
     // 6. key switch of c2 (power basis already) (relinearization_key.rs:101-106), then
     // 7. switch down to the ciphertext context, add c0/c1, forward NTT (mul.rs:194-210)
     rk->switch_and_finish(wD + 2 * L * n, 3 * L * n, out, inst, wD, 3 * L * n, L * n, nullptr, 0, 0, 0, nullptr,
                           s);
+    // 8. Ciphertext::switch_down when modulus switching is enabled (mul.rs:221-223)
+    if (mod_switch) mod_switch->switch_down(out, out_final, inst * 2, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -37,7 +37,7 @@ class StreamScratch {
 // every handle shares the pool of each purpose per stream; purposes that are
 // live at the same time (a multiplication and the key switch it calls) use
 // different pools.
-enum class Scratch { Scale, RelinC2, Lift, SwitchDown, Mul, GaloisC2, EvalTmp, Count };
+enum class Scratch { Scale, RelinC2, Lift, SwitchDown, Mul, GaloisC2, EvalTmp, ModSwitch, Count };
 StreamScratch& scratch(Scratch k);
 // Frees every pool's buffer of stream `s`.
 void release_scratch(hipStream_t s);
@@ -91,10 +91,24 @@ struct Ksk {
 };
 
 // Multiplicator (fhe/src/bfv/ops/mul.rs:22-227).
+// Ciphertext::switch_down (fhe/src/bfv/ciphertext.rs:86-97): NTT-domain
+// polynomials of `from` rounded into `to`, the first L - 1 moduli of `from`
+// (Poly::switch_down, math/src/rq/mod.rs:390-449), back in the NTT domain.
+struct Switcher {
+    std::shared_ptr<DeviceContext> from, to;
+    DevBuf consts;  // SdConst of the dropped modulus for each remaining one
+    Switcher(std::shared_ptr<DeviceContext> from, std::shared_ptr<DeviceContext> to);
+    // [polys][L][N] -> [polys][L - 1][N]; out must not alias in.
+    void switch_down(const u64* in, u64* out, u32 polys, hipStream_t s) const;
+};
+
 struct Multiplicator {
     std::shared_ptr<DeviceContext> base, mul;
     std::unique_ptr<PolyScaler> ext_lhs, ext_rhs, down;
     std::shared_ptr<Ksk> rk;
+    // Multiplicator::enable_mod_switching (mul.rs:146-155): the product is
+    // switched down to the next context.
+    std::unique_ptr<Switcher> mod_switch;
     Multiplicator(std::shared_ptr<DeviceContext> base, const std::vector<u64>& extended_basis, const BigUint& lhs_num,
                   const BigUint& lhs_den, const BigUint& rhs_num, const BigUint& rhs_den, const BigUint& post_num,
                   const BigUint& post_den);

@@ -101,6 +101,36 @@ void launch_mac_ew(const u64* y, u32 digits, u32 key_limbs, u32 first, u32 count
     HIPCHK(hipGetLastError());
 }
 
+// Poly::switch_down (math/src/rq/mod.rs:390-449) in power basis: with
+// y = (x_l + floor(q_l / 2)) mod q_l, x_i <- (x_i - y + floor(q_l / 2)) q_l^-1
+// mod q_i for each remaining limb i. grid = (N / 256, L - 1, polys)
+__global__ __launch_bounds__(256) void switch_down_pb_kernel(const u64* __restrict__ in, u64* __restrict__ out,
+                                                             const SdConst* __restrict__ consts, u64 ql, u32 L,
+                                                             u32 logn) {
+    const u32 e = blockIdx.x * 256 + threadIdx.x;
+    if (e >> logn) return;
+    const u32 i = blockIdx.y, b = blockIdx.z;
+    const u64* src = in + ((size_t)b * L << logn);
+    u64 y = src[((size_t)(L - 1) << logn) + e] + (ql >> 1);
+    y = y >= ql ? y - ql : y;
+    const SdConst cst = consts[i];
+    const u64 q = cst.q;
+    // [y]_i only has to be < 2 q_i and congruent to y (ks_finish_kernel)
+    const u64 yr = ql <= 2 * q ? y : lazy_reduce_u64(y, cst.bl, cst.bh, q);
+    const u64 tmp = yr + cst.half_mod;
+    out[(((size_t)b * (L - 1) + i) << logn) + e] =
+        mul_shoup_u(src[((size_t)i << logn) + e] + 3 * q - tmp, cst.inv, cst.inv_shoup, q);
+}
+
+void launch_switch_down_pb(const u64* in, u64* out, const SdConst* consts, u64 ql, u32 L, int logn, u32 polys,
+                           hipStream_t s) {
+    if (L < 2 || !polys) return;
+    ProfScope ps("switch_down", logn, s);
+    hipLaunchKernelGGL(switch_down_pb_kernel, ew_grid(1u << logn, L - 1, polys), dim3(256), 0, s, in, out, consts,
+                       ql, L, (u32)logn);
+    HIPCHK(hipGetLastError());
+}
+
 // Power-basis part of the key-switch finish (the z of ks_finish_kernel):
 // x[b][p][i] = add_pb - q_l^-1 [y]_i with y from the dropped limb in
 // power basis (`special` [inst][2][N]), or add_pb alone without an extra

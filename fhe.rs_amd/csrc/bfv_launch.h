@@ -105,6 +105,10 @@ void launch_mac_ew(const u64* y, u32 digits, u32 key_limbs, u32 first, u32 count
 // one (+ NTT-domain addend) into sd.out, the dropped limb (when sd.extra) into
 // `special`, still in the NTT domain.
 void launch_ks_mac(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s);
+// Poly::switch_down (math/src/rq/mod.rs:390-449) of power-basis polynomials
+// [polys][L][N] into [polys][L - 1][N] (power basis), dropping modulus q_l.
+void launch_switch_down_pb(const u64* in, u64* out, const SdConst* consts, u64 ql, u32 L, int logn, u32 polys,
+                           hipStream_t s);
 // Power-basis switch-down part x [inst][2][L][N] of the finish (unfused path).
 void launch_sd_z(int logn, const KsFinishArgs& f, u64* x, u32 inst, hipStream_t s);
 // Copies `count` slots of `elems` words; `table` is a device array of

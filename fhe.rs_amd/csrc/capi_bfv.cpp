@@ -33,6 +33,9 @@ struct fhehip_server {
 struct fhehip_graph {
     std::unique_ptr<Graph> g;
 };
+struct fhehip_switcher {
+    std::unique_ptr<Switcher> s;
+};
 
 static BigUint factor_num(const fhehip_scaling_factor* f) {
     if (!f || !f->num) return BigUint(1);
@@ -127,7 +130,36 @@ int fhehip_multiplicator_enable_relinearization(fhehip_multiplicator* m, const f
     FHEHIP_API_END
 }
 
+int fhehip_multiplicator_enable_mod_switching(fhehip_multiplicator* m, const fhehip_context* next) {
+    FHEHIP_API_BEGIN
+    if (!m || !next) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    if (m->m->base->L < 2)
+        return fail(FHEHIP_ERR_INVALID_ARG, "Cannot modulo switch as this is already the last level");
+    m->m->mod_switch.reset(new Switcher(m->m->base, next->c));
+    FHEHIP_API_END
+}
+
 void fhehip_multiplicator_destroy(fhehip_multiplicator* m) { delete m; }
+
+int fhehip_switcher_create(const fhehip_context* from, const fhehip_context* to, fhehip_switcher** out) {
+    FHEHIP_API_BEGIN
+    if (!from || !to || !out) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    std::unique_ptr<Switcher> sw(new Switcher(from->c, to->c));
+    *out = new fhehip_switcher{std::move(sw)};
+    FHEHIP_API_END
+}
+
+void fhehip_switcher_destroy(fhehip_switcher* s) { delete s; }
+
+int fhehip_switch_down(const fhehip_switcher* s, const uint64_t* d_in, uint64_t* d_out, uint32_t num_polys,
+                       void* stream) {
+    FHEHIP_API_BEGIN
+    if (!s || ((!d_in || !d_out) && num_polys)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    if (num_polys && d_in == d_out) return fail(FHEHIP_ERR_INVALID_ARG, "the output must not alias the input");
+    s->s->switch_down(d_in, d_out, num_polys, S(stream));
+    FHEHIP_API_END
+}
 
 int fhehip_multiplicator_extended_basis(const fhehip_multiplicator* m, uint64_t* out, uint32_t* len) {
     FHEHIP_API_BEGIN
@@ -256,6 +288,7 @@ int fhehip_multiply_sum_graph(fhehip_multiplicator* m, fhehip_evaluation_key* ek
     if (!m || !ek || !out || !d_lhs || !d_rhs || !d_out || !num) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
     if (!m->m->rk) return fail(FHEHIP_ERR_INVALID_ARG, "the multiplicator has no relinearization key");
+    if (m->m->mod_switch) return fail(FHEHIP_ERR_UNSUPPORTED, "modulus switching in a multiply-and-sum graph");
     EvaluationKey* e = ek->ek.get();
     const auto& b = *m->m->base;
     if (e->degree != b.N || e->keys[0]->ksk->ct_ctx->moduli != b.moduli)
@@ -294,6 +327,7 @@ int fhehip_submit(const fhehip_job* jobs, uint32_t n, void* stream) {
     for (uint32_t k = 0; k < n; k++) {
         const fhehip_job& j = jobs[k];
         if (!j.mult) return fail(FHEHIP_ERR_INVALID_ARG, "null multiplicator in a job");
+        if (j.mult->m->mod_switch) return fail(FHEHIP_ERR_UNSUPPORTED, "modulus switching in a job");
         js[k].mult = j.mult->m.get();
         js[k].ek = j.ek ? j.ek->ek.get() : nullptr;
         js[k].lhs = j.d_lhs;
@@ -324,6 +358,7 @@ int fhehip_server_add_params(fhehip_server* s, fhehip_multiplicator* mult, fhehi
                              uint32_t log2_slots, uint32_t* id, uint64_t* footprint_bytes) {
     FHEHIP_API_BEGIN
     if (!s || !mult || !ek || !id) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    if (mult->m->mod_switch) return fail(FHEHIP_ERR_UNSUPPORTED, "modulus switching in a served parameter set");
     *id = s->s->add_params(mult->m.get(), ek->ek.get(), log2_slots);
     if (footprint_bytes) *footprint_bytes = s->s->params(*id).footprint;
     FHEHIP_API_END

@@ -46,6 +46,10 @@ SIGNATURES = {
     "fhehip_multiplicator_default": (C.c_int, [_vp, C.c_uint64, _vpp]),
     "fhehip_multiplicator_create": (C.c_int, [_vp, _u64p, C.c_uint32, _vp, _vp, _vp, _vpp]),
     "fhehip_multiplicator_enable_relinearization": (C.c_int, [_vp, _vp]),
+    "fhehip_multiplicator_enable_mod_switching": (C.c_int, [_vp, _vp]),
+    "fhehip_switcher_create": (C.c_int, [_vp, _vp, _vpp]),
+    "fhehip_switcher_destroy": (None, [_vp]),
+    "fhehip_switch_down": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp]),
     "fhehip_multiplicator_destroy": (None, [_vp]),
     "fhehip_multiplicator_extended_basis": (C.c_int, [_vp, _u64p, _u32p]),
     "fhehip_multiply": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
@@ -294,6 +298,30 @@ class Multiplicator:
 
     def multiply(self, d_lhs, d_rhs, d_out, num, stream=None):
         check(lib().fhehip_multiply(self.h, dptr(d_lhs), dptr(d_rhs), dptr(d_out), num, stream_ptr(stream)))
+
+    def enable_mod_switching(self, next_ctx):
+        """Multiplicator::enable_mod_switching: products land in next_ctx."""
+        self.next_ctx = next_ctx
+        check(lib().fhehip_multiplicator_enable_mod_switching(self.h, next_ctx.h))
+
+
+class Switcher:
+    """Ciphertext::switch_down from `from_ctx` to `to_ctx` (its first L - 1 moduli)."""
+
+    def __init__(self, from_ctx, to_ctx):
+        self.from_ctx, self.to_ctx = from_ctx, to_ctx
+        h = C.c_void_p()
+        check(lib().fhehip_switcher_create(from_ctx.h, to_ctx.h, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            _destroy("fhehip_switcher_destroy", getattr(self, "h", None))
+        except Exception:  # module globals already cleared at interpreter exit
+            pass
+
+    def switch_down(self, d_in, d_out, num_polys, stream=None):
+        check(lib().fhehip_switch_down(self.h, dptr(d_in), dptr(d_out), num_polys, stream_ptr(stream)))
 
 
 class GaloisKey:

@@ -43,6 +43,7 @@ typedef struct fhehip_ksk fhehip_ksk;
 typedef struct fhehip_multiplicator fhehip_multiplicator;
 typedef struct fhehip_galois fhehip_galois;
 typedef struct fhehip_evaluation_key fhehip_evaluation_key;
+typedef struct fhehip_switcher fhehip_switcher;
 
 /* Text of the last error raised on the calling thread. */
 const char* fhehip_last_error(void);
@@ -136,6 +137,15 @@ int fhehip_multiplicator_create(const fhehip_context* base_ctx, const uint64_t* 
                                 const fhehip_scaling_factor* post, fhehip_multiplicator** out);
 /* Multiplicator::enable_relinearization (mul.rs:133-142). */
 int fhehip_multiplicator_enable_relinearization(fhehip_multiplicator* m, const fhehip_ksk* rk);
+/* Multiplicator::enable_mod_switching (mul.rs:146-155): every product is then
+ * switched down (Ciphertext::switch_down, mul.rs:221-223) into `next`, the
+ * base context without its last modulus; fhehip_multiply writes
+ * [num][2][L-1][N] (or [num][3][L-1][N] without relinearization). Refused
+ * (FHEHIP_ERR_CTX_MISMATCH) when `next` is not that context, and
+ * (FHEHIP_ERR_INVALID_ARG) when the base context has one modulus only. Job
+ * batches, the request server and the multiply-and-sum graph do not take
+ * such a Multiplicator. */
+int fhehip_multiplicator_enable_mod_switching(fhehip_multiplicator* m, const fhehip_context* next);
 void fhehip_multiplicator_destroy(fhehip_multiplicator* m);
 /* Moduli of the multiplication context; *len is in/out (capacity / count). */
 int fhehip_multiplicator_extended_basis(const fhehip_multiplicator* m, uint64_t* out, uint32_t* len);
@@ -174,6 +184,18 @@ int fhehip_inner_sum(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint64_t* 
  * the computes_inner_sum loop truncated. */
 int fhehip_rotate_and_sum(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint64_t* d_out, uint32_t log2_slots,
                           int rows, uint32_t num, void* stream);
+/* ---- Leveled operations ------------------------------------------------------
+ * Ciphertext::switch_down (fhe/src/bfv/ciphertext.rs:86-97) and
+ * Ciphertext::switch_to_level (:99-119, by repeated calls): the NTT-domain
+ * polynomials of `from` are taken to power basis, rounded by the last
+ * modulus (Poly::switch_down, math/src/rq/mod.rs:390-449) and transformed
+ * back in `to`, which must be `from` without its last modulus
+ * (FHEHIP_ERR_CTX_MISMATCH otherwise). */
+int fhehip_switcher_create(const fhehip_context* from, const fhehip_context* to, fhehip_switcher** out);
+void fhehip_switcher_destroy(fhehip_switcher* s);
+/* [num_polys][L][N] -> [num_polys][L-1][N]; out must not alias in. */
+int fhehip_switch_down(const fhehip_switcher* s, const uint64_t* d_in, uint64_t* d_out, uint32_t num_polys,
+                       void* stream);
 /* Ciphertext / polynomial addition (fhe/src/bfv/ops/mod.rs:15-69, Poly
  * AddAssign math/src/rq/ops.rs:92-118) of `num_polys` polynomials of the
  * context: out = a + b (out may alias a or b). */

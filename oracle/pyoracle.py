@@ -294,6 +294,15 @@ class Context:
         return random_polys(self.moduli, self.n, count, seed)
 
 
+def ct_switch_down(ctx_from: Context, ctx_to: Context, ct):
+    """Ciphertext::switch_down (fhe/src/bfv/ciphertext.rs:86-97): every
+    NTT-domain polynomial [polys][L][N] of ctx_from to power basis,
+    Poly::switch_down (rq/mod.rs:390-449), and into the NTT domain of ctx_to
+    (ctx_from without its last modulus)."""
+    assert ctx_to.moduli == ctx_from.moduli[:-1] and ctx_to.n == ctx_from.n
+    return ctx_to.forward(ctx_from.switch_down(ctx_from.backward(ct)))
+
+
 class PolyScaler:
     def __init__(self, from_ctx: Context, to_ctx: Context, num=None, den=None):
         self.f, self.t = from_ctx, to_ctx

@@ -187,3 +187,28 @@ def test_mul_relin_decrypts_cfg1(oracle):
                                   4611686018424733697]
     ct = m.multiply(c1, c2)
     assert [int(x) for x in bfv.decrypt(ct)] == [(a * b) % t for a, b in zip(v1, v2)]
+
+
+@pytest.mark.parametrize("L", [2, 3, 4])
+def test_ciphertext_switch_down_decrypts(oracle, L):
+    """Ciphertext::switch_down (ciphertext.rs:86-97) composed from the oracle's
+    pinned NTT and Poly::switch_down: the ciphertext decrypts to the same
+    slots one level down, including after a multiplication with modulus
+    switching (mul.rs:221-223), for every level it can reach."""
+    n, t = 16, 1153
+    bfv = oracle.Bfv(n, t, oracle.generate_moduli([62] * L, n), sk_seed=50 + L)
+    rng = random.Random(L)
+    v1, v2 = _slots(rng, n, t), _slots(rng, n, t)
+    ct = bfv.encrypt(v1, level=0, seed=61)
+    for level in range(L - 1):
+        cf = oracle.Context(bfv.level_moduli(level), n)
+        ctn = oracle.Context(bfv.level_moduli(level + 1), n)
+        ct = oracle.ct_switch_down(cf, ctn, ct)
+        assert [int(x) for x in bfv.decrypt(ct, level=level + 1)] == v1
+    c1 = bfv.encrypt(v1, level=0, seed=62)
+    c2 = bfv.encrypt(v2, level=0, seed=63)
+    rk = bfv.relin_key(ct_level=0, key_level=0, seed=64)
+    prod = oracle.Multiplicator(bfv, rk, level=0).multiply(c1, c2)
+    low = oracle.ct_switch_down(oracle.Context(bfv.level_moduli(0), n), oracle.Context(bfv.level_moduli(1), n),
+                                prod)
+    assert [int(x) for x in bfv.decrypt(low, level=1)] == [(a * b) % t for a, b in zip(v1, v2)]
