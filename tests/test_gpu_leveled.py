@@ -140,3 +140,35 @@ def test_leveled_errors(fh, oracle):
     gm.enable_mod_switching(fh.Context(m[:2], n))
     with pytest.raises(fh.FheHipError):
         fh.submit([(gm, None, buf, buf, buf, [])])
+
+
+def test_multiply_graph_with_mod_switching(fh, oracle):
+    """fhehip_multiply_graph of a mod-switching Multiplicator: every replay
+    equals the eager call, with eager work and synchronizations between the
+    replays."""
+    n, batch = 4096, 3
+    print(f"graph with mod switching n={n} batch={batch}", flush=True)
+    bfv = oracle.Bfv(n, T, oracle.generate_moduli([60] * 5, n), sk_seed=21)
+    rk = bfv.relin_key(ct_level=0, key_level=0, seed=22)
+    c0, c1 = rk.export()
+    cctx = fh.Context(bfv.level_moduli(0), n)
+    gm = fh.Multiplicator(rk=fh.KeySwitchingKey(cctx, cctx, c0, c1), plaintext=T)
+    gm.enable_mod_switching(fh.Context(bfv.level_moduli(1), n))
+    lhs = fh.to_device(oracle.random_polys(bfv.level_moduli(0), n, 2 * batch, seed=5))
+    rhs = fh.to_device(oracle.random_polys(bfv.level_moduli(0), n, 2 * batch, seed=6))
+    size = batch * 2 * 4 * n
+    eager, out = fh.empty(size), fh.empty(size)
+    gm.multiply(lhs, rhs, eager, batch)
+    _sync()
+    want = fh.to_host(eager)
+    g = fh.Graph(gm, lhs, rhs, out, batch)
+    try:
+        for r in range(3):
+            out.zero_()
+            g.launch()
+            _sync()
+            assert np.array_equal(fh.to_host(out), want), r
+            gm.multiply(rhs, lhs, eager, batch)
+            _sync()
+    finally:
+        g.close()
