@@ -576,6 +576,106 @@ void EvaluationKey::rotate_and_sum(const std::vector<u32>& exponents, const u64*
     }
 }
 
+// The expansion levels a key supports (EvaluationKey::supports_expansion,
+// evaluation_key.rs:134-147) and their monomials -X^(N - 2^l)
+// (EvaluationKeyBuilder::build, evaluation_key.rs:355-363), transformed by
+// the context's own forward NTT.
+void EvaluationKey::prepare_expansion() {
+    expansion_levels = 0;
+    if (keys.empty()) return;
+    const DeviceContext& c = *keys[0]->ksk->ct_ctx;
+    while (expansion_levels < (u32)c.logn && find((degree >> expansion_levels) + 1)) expansion_levels++;
+    if (!expansion_levels) return;
+    const size_t n = c.N, L = c.L, rows = (size_t)expansion_levels * L;
+    std::vector<u64> h(rows * n, 0);
+    for (u32 l = 0; l < expansion_levels; l++)
+        for (size_t m = 0; m < L; m++) h[(l * L + m) * n + n - ((size_t)1 << l)] = c.moduli[m] - 1;
+    DevBuf tmp(rows * n * 8);
+    HIPCHK(hipMemcpy(tmp.ptr, h.data(), tmp.bytes, hipMemcpyHostToDevice));
+    launch_ntt_forward(c, RowMap{tmp.as<u64>(), (u32)L, (u32)L, 0, 0}, (u32)rows, false, nullptr);
+    HIPCHK(hipStreamSynchronize(nullptr));
+    HIPCHK(hipMemcpy(h.data(), tmp.ptr, tmp.bytes, hipMemcpyDeviceToHost));
+    std::vector<ulonglong2> hm(rows * n);
+    for (size_t r = 0; r < rows; r++) {
+        const u64 q = c.moduli[r % L];
+        for (size_t j = 0; j < n; j++) {
+            const u64 v = h[r * n + j];
+            hm[r * n + j] = ulonglong2{v, shoup64(v, q)};
+        }
+    }
+    monomials.reset(hm.size() * sizeof(ulonglong2));
+    HIPCHK(hipMemcpy(monomials.ptr, hm.data(), monomials.bytes, hipMemcpyHostToDevice));
+}
+
+// EvaluationKey::expands (evaluation_key.rs:153-193): level l relinearizes
+// out[0 .. 2^l) with the key of exponent (N >> l) + 1 as one batch, then one
+// elementwise pass forms out[2^l + i] and updates out[i].
+void EvaluationKey::expand(const u64* ct, u32 size, u64* out, hipStream_t s) const {
+    if (keys.empty()) throw Error(-1, "empty evaluation key");
+    u32 level = 0;
+    while (((u64)1 << level) < size) level++;
+    if (level > expansion_levels) throw Error(-1, "This key does not support expansion at this level");
+    const DeviceContext& c = *keys[0]->ksk->ct_ctx;
+    const size_t sz = (size_t)2 * c.L * c.N;
+    HIPCHK(hipMemcpyAsync(out, ct, sz * 8, hipMemcpyDeviceToDevice, s));
+    if (!level) return;
+    u64* sub = scratch(Scratch::EvalTmp).get(s, ((size_t)1 << (level - 1)) * sz * 8).as<u64>();
+    for (u32 l = 0; l < level; l++) {
+        const u32 step = 1u << l;
+        find((degree >> l) + 1)->apply(out, sub, step, s);
+        launch_expand_step(out, sub, monomials.as<ulonglong2>() + (size_t)l * c.L * c.N, c.mod.as<ModInfo>(), c.L,
+                           c.logn, step, size, s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// dot_product_scalar (fhe/src/bfv/ops/dot_product.rs:56-143).
+static void check_dot(u32 count, u32 parts) {
+    if (!count) throw Error(-1, "At least one iterator is empty");
+    if (!parts) throw Error(-1, "ciphertexts without parts");
+    if (count > 0x7fffffffu) throw Error(-1, "at most 2^31 - 1 terms per dot product");
+}
+
+void dot_product_scalar(const DeviceContext& c, const u64* ct, size_t ct_stride, const u64* pt, size_t pt_stride,
+                        u32 count, u32 parts, u64* out, hipStream_t s) {
+    check_dot(count, parts);
+    DotArgs a{};
+    a.ct = ct;
+    a.ct_stride = ct_stride;
+    a.pt = pt;
+    a.pt_row_stride = pt_stride;
+    a.out = out;
+    a.mods = c.mod.as<ModInfo>();
+    a.count = count;
+    a.columns = 1;
+    a.words = c.L * c.N;
+    a.logn = (u32)c.logn;
+    launch_dot_product(a, parts, s);
+}
+
+// The column products of a PIR server (examples/mulpir.rs:170-179): column
+// j is the plaintexts db[r][j] (database.iter().skip(j).step_by(columns)).
+void dot_product_columns(const DeviceContext& c, const u64* ct, u32 rows, u32 parts, const u64* db, u32 columns,
+                         u64* out, hipStream_t s) {
+    check_dot(rows, parts);
+    if (!columns) return;
+    const size_t words = (size_t)c.L * c.N;
+    DotArgs a{};
+    a.ct = ct;
+    a.ct_stride = parts * words;
+    a.pt = db;
+    a.pt_row_stride = columns * words;
+    a.pt_col_stride = words;
+    a.out = out;
+    a.out_col_stride = parts * words;
+    a.mods = c.mod.as<ModInfo>();
+    a.count = rows;
+    a.columns = columns;
+    a.words = (u32)words;
+    a.logn = (u32)c.logn;
+    launch_dot_product(a, parts, s);
+}
+
 // ---------------------------------------------------------------------------
 StreamScratch& scratch(Scratch k) {
     // Never destroyed: device memory is released when the process ends.

@@ -130,6 +130,15 @@ struct Galois {
 struct EvaluationKey {
     std::vector<std::shared_ptr<Galois>> keys;
     u32 degree = 0;
+    // Oblivious expansion (evaluation_key.rs:153-193): levels l < expansion_levels
+    // have the key of exponent (N >> l) + 1; monomials [level][L][N] holds
+    // -X^(N - 2^l) in the NTT domain as (value, Shoup) pairs.
+    u32 expansion_levels = 0;
+    DevBuf monomials;
+    // Finds the expansion levels and builds their monomials (synchronous).
+    void prepare_expansion();
+    // EvaluationKey::expands: ct [2][L][N] -> out [max(size, 1)][2][L][N].
+    void expand(const u64* ct, u32 size, u64* out, hipStream_t s) const;
     std::shared_ptr<Galois> find(u32 exponent) const;
     // 3^i mod 2N (evaluation_key.rs:195-203)
     u32 column_exponent(u32 i) const;
@@ -137,5 +146,15 @@ struct EvaluationKey {
     // out = ct; for each listed exponent: out += rotate(out)
     void rotate_and_sum(const std::vector<u32>& exponents, const u64* ct, u64* out, u32 inst, hipStream_t s);
 };
+
+// dot_product_scalar (fhe/src/bfv/ops/dot_product.rs:56-143) of `count`
+// ciphertexts [parts][L][N] (ct_stride words apart) with NTT-domain plaintext
+// polynomials [L][N] (pt_stride words apart) of context c, into out [parts][L][N].
+void dot_product_scalar(const DeviceContext& c, const u64* ct, size_t ct_stride, const u64* pt, size_t pt_stride,
+                        u32 count, u32 parts, u64* out, hipStream_t s);
+// Every column of a [rows][columns][L][N] database against rows ciphertexts
+// [rows][parts][L][N]: out [columns][parts][L][N].
+void dot_product_columns(const DeviceContext& c, const u64* ct, u32 rows, u32 parts, const u64* db, u32 columns,
+                         u64* out, hipStream_t s);
 
 }  // namespace fhehip

@@ -109,6 +109,25 @@ void launch_ks_mac(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s);
 // [polys][L][N] into [polys][L - 1][N] (power basis), dropping modulus q_l.
 void launch_switch_down_pb(const u64* in, u64* out, const SdConst* consts, u64 ql, u32 L, int logn, u32 polys,
                            hipStream_t s);
+// Dot products of ciphertexts with NTT-domain plaintext polynomials
+// (pir_kernels.hip). Row r of column c: ciphertext at ct + r * ct_stride
+// ([parts][L][N], part stride `words` = L N), plaintext at pt + r *
+// pt_row_stride + c * pt_col_stride; out[c] at out + c * out_col_stride.
+struct DotArgs {
+    const u64* ct;
+    size_t ct_stride;
+    const u64* pt;
+    size_t pt_row_stride, pt_col_stride;
+    u64* out;
+    size_t out_col_stride;
+    const ModInfo* mods;
+    u32 count, columns, colgroups, words, logn;
+};
+void launch_dot_product(DotArgs a, u32 parts, hipStream_t s);
+// One level of the oblivious expansion (EvaluationKey::expands); `mono` is
+// the level's monomial [L][N] as (value, Shoup) pairs.
+void launch_expand_step(u64* out, const u64* sub, const ulonglong2* mono, const ModInfo* mods, u32 L, int logn,
+                        u32 step, u32 size, hipStream_t s);
 // Power-basis switch-down part x [inst][2][L][N] of the finish (unfused path).
 void launch_sd_z(int logn, const KsFinishArgs& f, u64* x, u32 inst, hipStream_t s);
 // Copies `count` slots of `elems` words; `table` is a device array of

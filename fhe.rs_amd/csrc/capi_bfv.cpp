@@ -214,6 +214,7 @@ int fhehip_evaluation_key_create(fhehip_galois* const* keys, uint32_t count, fhe
         ek->keys.push_back(g);
     }
     ek->degree = ek->keys[0]->ksk->ct_ctx->N;
+    ek->prepare_expansion();
     *out = new fhehip_evaluation_key{std::move(ek)};
     FHEHIP_API_END
 }
@@ -227,6 +228,38 @@ int fhehip_rotate_columns(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint6
     if (d_ct == d_out && num) return fail(FHEHIP_ERR_INVALID_ARG, "out must not alias ct");
     if (step == 0 || step >= ek->ek->degree / 2) return fail(FHEHIP_ERR_INVALID_ARG, "rotation step out of range");
     ek->ek->rotate(ek->ek->column_exponent(step), d_ct, d_out, num, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_expand(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint32_t size, uint64_t* d_out, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!ek || !d_ct || !d_out) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    if (d_ct == d_out) return fail(FHEHIP_ERR_INVALID_ARG, "out must not alias ct");
+    ek->ek->expand(d_ct, size, d_out, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_expansion_levels(const fhehip_evaluation_key* ek, uint32_t* levels) {
+    FHEHIP_API_BEGIN
+    if (!ek || !levels) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *levels = ek->ek->expansion_levels;
+    FHEHIP_API_END
+}
+
+int fhehip_dot_product_scalar(const fhehip_context* ctx, const uint64_t* d_ct, uint64_t ct_stride,
+                              const uint64_t* d_pt, uint64_t pt_stride, uint32_t count, uint32_t parts,
+                              uint64_t* d_out, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!ctx || ((!d_ct || !d_pt || !d_out) && count)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    dot_product_scalar(*ctx->c, d_ct, ct_stride, d_pt, pt_stride, count, parts, d_out, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_dot_product_columns(const fhehip_context* ctx, const uint64_t* d_ct, uint32_t rows, uint32_t parts,
+                               const uint64_t* d_db, uint32_t columns, uint64_t* d_out, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!ctx || ((!d_ct || !d_db || !d_out) && rows && columns)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    dot_product_columns(*ctx->c, d_ct, rows, parts, d_db, columns, d_out, S(stream));
     FHEHIP_API_END
 }
 

@@ -1,0 +1,126 @@
+// Kernels of the PIR server path: plaintext-ciphertext dot products over a
+// database resident in HBM, and the elementwise step of the oblivious
+// expansion of a query ciphertext.
+#include "bfv_launch.h"
+#include "ntt_device.h"
+#include "profiler.h"
+#include "rns_device.h"
+
+namespace fhehip {
+
+// dot_product_scalar (fhe/src/bfv/ops/dot_product.rs:56-143) for CG columns
+// per thread: out[c][p][k] = sum_r ct[r][p][k] * pt[r][c][k] mod q, k = one
+// word of [L][N]. The sums are exact (column accumulators) and reduced once,
+// so the result is the reference's whichever of its two accumulation
+// schedules it takes. Each thread loads the ciphertext words of a row once
+// for its CG columns; blocks that share a coefficient chunk get consecutive
+// logical indices on one XCD, so the other column groups read those words
+// from L2. 1-D grid of chunks * colgroups blocks of 256 threads.
+template <int PARTS, int CG>
+__global__ __launch_bounds__(256) void dot_product_kernel(DotArgs a) {
+    const u32 w = xcd_block(blockIdx.x, gridDim.x);
+    const u32 chunk = w / a.colgroups, cg = w % a.colgroups;
+    const u32 k = chunk * 256 + threadIdx.x;
+    if (k >= a.words) return;
+    const u32 c0 = cg * CG;
+    const u32 ncol = a.columns - c0 < (u32)CG ? a.columns - c0 : (u32)CG;
+    Acc3 acc[CG][PARTS];
+#pragma unroll
+    for (int c = 0; c < CG; c++)
+#pragma unroll
+        for (int p = 0; p < PARTS; p++) acc3_init(acc[c][p], 0);
+    const u64* ct = a.ct + k;
+    const u64* pt = a.pt + (size_t)c0 * a.pt_col_stride + k;
+    for (u32 r = 0; r < a.count; r++) {
+        u64 x[PARTS], y[CG];
+#pragma unroll
+        for (int p = 0; p < PARTS; p++) x[p] = ct[(size_t)p * a.words];
+#pragma unroll
+        for (int c = 0; c < CG; c++) y[c] = (u32)c < ncol ? pt[(size_t)c * a.pt_col_stride] : 0;
+#pragma unroll
+        for (int c = 0; c < CG; c++)
+#pragma unroll
+            for (int p = 0; p < PARTS; p++) acc3_mul<false>(acc[c][p], x[p], y[c]);
+        ct += a.ct_stride;
+        pt += a.pt_row_stride;
+    }
+    const ModInfo mi = a.mods[k >> a.logn];
+#pragma unroll
+    for (int c = 0; c < CG; c++) {
+        if ((u32)c >= ncol) break;
+#pragma unroll
+        for (int p = 0; p < PARTS; p++) {
+            // lo + 2^64 hi + 2^128 top: (top 2^64 + hi) mod q, then (that 2^64 + lo) mod q;
+            // top < count < 2^31 and q < 2^62 keep both Barrett inputs below 2^127.
+            u64 lo, hi, top;
+            acc3_value(acc[c][p], lo, hi, top);
+            const u64 r1 = reduce_u128(hi, top, mi.barrett_lo, mi.barrett_hi, mi.p);
+            a.out[(size_t)(c0 + c) * a.out_col_stride + (size_t)p * a.words + k] =
+                reduce_u128(lo, r1, mi.barrett_lo, mi.barrett_hi, mi.p);
+        }
+    }
+}
+
+template <int PARTS>
+static void launch_dot_parts(const DotArgs& a, int cg, u32 blocks, hipStream_t s) {
+    if (cg == 4)
+        hipLaunchKernelGGL((dot_product_kernel<PARTS, 4>), dim3(blocks), dim3(256), 0, s, a);
+    else if (cg == 2)
+        hipLaunchKernelGGL((dot_product_kernel<PARTS, 2>), dim3(blocks), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((dot_product_kernel<PARTS, 1>), dim3(blocks), dim3(256), 0, s, a);
+}
+
+void launch_dot_product(DotArgs a, u32 parts, hipStream_t s) {
+    if (!a.count || !a.columns || !a.words) return;
+    // Columns per thread: the ciphertext words of a row are loaded once for
+    // them; the accumulators take 9 VGPRs per (column, part).
+    const int cg = a.columns >= 4 && parts <= 2 ? 4 : (a.columns >= 2 ? 2 : 1);
+    a.colgroups = (a.columns + cg - 1) / cg;
+    const u32 chunks = (a.words + 255) / 256;
+    const u64 blocks = (u64)chunks * a.colgroups;
+    if (blocks > 0x7fffffffu) throw Error(-1, "dot product grid too large");
+    ProfScope ps("dot_product", a.logn, s);
+    switch (parts) {
+        case 1: launch_dot_parts<1>(a, cg, (u32)blocks, s); break;
+        case 2: launch_dot_parts<2>(a, cg, (u32)blocks, s); break;
+        case 3: launch_dot_parts<3>(a, cg, (u32)blocks, s); break;
+        case 4: launch_dot_parts<4>(a, cg, (u32)blocks, s); break;
+        default: throw Error(-2, "dot products of ciphertexts with more than 4 parts are not supported");
+    }
+    HIPCHK(hipGetLastError());
+}
+
+// One level of EvaluationKey::expands (fhe/src/bfv/keys/evaluation_key.rs:
+// 168-185) for i < step, with sub[i] = GaloisKey::relinearize(out[i]):
+//   out[step + i] = (out[i] - sub[i]) * monomial   (when step + i < size)
+//   out[i]       += sub[i]
+// All NTT-domain and exactly reduced. grid = (N/256, 2 L, step)
+__global__ __launch_bounds__(256) void expand_step_kernel(u64* __restrict__ out, const u64* __restrict__ sub,
+                                                          const ulonglong2* __restrict__ mono,
+                                                          const ModInfo* __restrict__ mods, u32 L, u32 logn,
+                                                          u32 step, u32 size) {
+    const u32 e = blockIdx.x * 256 + threadIdx.x;
+    if (e >> logn) return;
+    const u32 m = blockIdx.y % L, i = blockIdx.z;
+    const size_t sz = (size_t)2 * L << logn;
+    const size_t w = ((size_t)blockIdx.y << logn) + e;
+    const u64 q = mods[m].p;
+    const u64 lo = out[i * sz + w], sb = sub[i * sz + w];
+    if (step + i < size) {
+        const ulonglong2 mw = mono[((size_t)m << logn) + e];
+        out[(step + i) * sz + w] = mul_shoup(sub_mod(lo, sb, q), mw.x, mw.y, q);
+    }
+    out[i * sz + w] = add_mod(lo, sb, q);
+}
+
+void launch_expand_step(u64* out, const u64* sub, const ulonglong2* mono, const ModInfo* mods, u32 L, int logn,
+                        u32 step, u32 size, hipStream_t s) {
+    if (!step) return;
+    ProfScope ps("expand_step", logn, s);
+    hipLaunchKernelGGL(expand_step_kernel, dim3(((1u << logn) + 255) / 256, 2 * L, step), dim3(256), 0, s, out, sub,
+                       mono, mods, L, (u32)logn, step, size);
+    HIPCHK(hipGetLastError());
+}
+
+}  // namespace fhehip

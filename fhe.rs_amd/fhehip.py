@@ -63,6 +63,11 @@ SIGNATURES = {
     "fhehip_inner_sum": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp]),
     "fhehip_rotate_and_sum": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_int, C.c_uint32, _vp]),
     "fhehip_ciphertext_add": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_expand": (C.c_int, [_vp, _vp, C.c_uint32, _vp, _vp]),
+    "fhehip_expansion_levels": (C.c_int, [_vp, _u32p]),
+    "fhehip_dot_product_scalar": (C.c_int, [_vp, _vp, C.c_uint64, _vp, C.c_uint64, C.c_uint32, C.c_uint32, _vp,
+                                            _vp]),
+    "fhehip_dot_product_columns": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint32, _vp, C.c_uint32, _vp, _vp]),
     "fhehip_multiply_graph": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vpp]),
     "fhehip_multiply_sum_graph": (C.c_int, [_vp, _vp, C.c_uint32, C.c_int, _vp, _vp, _vp, C.c_uint32, _vpp]),
     "fhehip_graph_launch": (C.c_int, [_vp, _vp]),
@@ -371,6 +376,15 @@ class EvaluationKey:
         check(lib().fhehip_rotate_and_sum(self.h, dptr(d_ct), dptr(d_out), log2_slots, 1 if rows else 0, num,
                                           stream_ptr(stream)))
 
+    def expand(self, d_ct, size, d_out, stream=None):
+        """EvaluationKey::expands: d_out [max(size, 1)][2][L][N]."""
+        check(lib().fhehip_expand(self.h, dptr(d_ct), size, dptr(d_out), stream_ptr(stream)))
+
+    def expansion_levels(self):
+        v = C.c_uint32()
+        check(lib().fhehip_expansion_levels(self.h, C.byref(v)))
+        return v.value
+
 
 def ntt_set_shape(degree, fwd, inv):
     """log2 values per thread (4 or 5) of the plain NTT kernels for `degree`."""
@@ -379,6 +393,18 @@ def ntt_set_shape(degree, fwd, inv):
 
 def ciphertext_add(ctx, d_a, d_b, d_out, num_polys, stream=None):
     check(lib().fhehip_ciphertext_add(ctx.h, dptr(d_a), dptr(d_b), dptr(d_out), num_polys, stream_ptr(stream)))
+
+
+def dot_product_scalar(ctx, d_ct, ct_stride, d_pt, pt_stride, count, parts, d_out, stream=None):
+    """dot_product_scalar over `count` (ciphertext, plaintext) pairs; strides in words."""
+    check(lib().fhehip_dot_product_scalar(ctx.h, dptr(d_ct), ct_stride, dptr(d_pt), pt_stride, count, parts,
+                                          dptr(d_out), stream_ptr(stream)))
+
+
+def dot_product_columns(ctx, d_ct, rows, parts, d_db, columns, d_out, stream=None):
+    """Every column of a [rows][columns][L][N] database against `rows` ciphertexts."""
+    check(lib().fhehip_dot_product_columns(ctx.h, dptr(d_ct), rows, parts, dptr(d_db), columns, dptr(d_out),
+                                           stream_ptr(stream)))
 
 
 class Graph:

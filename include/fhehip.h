@@ -184,6 +184,34 @@ int fhehip_inner_sum(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint64_t* 
  * the computes_inner_sum loop truncated. */
 int fhehip_rotate_and_sum(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint64_t* d_out, uint32_t log2_slots,
                           int rows, uint32_t num, void* stream);
+/* ---- PIR server path ---------------------------------------------------------
+ * EvaluationKey::expands (fhe/src/bfv/keys/evaluation_key.rs:153-193): the
+ * oblivious expansion (eprint 2019/1483) of the ciphertext d_ct [2][L][N]
+ * into d_out [size][2][L][N], with the key's Galois keys of exponents
+ * (N >> l) + 1 for l < ceil(log2(size)). As in fhe.rs, size 0 gives one
+ * ciphertext (the input). FHEHIP_ERR_INVALID_ARG when a needed key is missing
+ * ("This key does not support expansion at this level") or out aliases ct. */
+int fhehip_expand(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint32_t size, uint64_t* d_out, void* stream);
+/* EvaluationKey::supports_expansion (evaluation_key.rs:134-147): the largest
+ * level (ceil(log2(size))) fhehip_expand accepts with this key. */
+int fhehip_expansion_levels(const fhehip_evaluation_key* ek, uint32_t* levels);
+/* dot_product_scalar (fhe/src/bfv/ops/dot_product.rs:56-143): d_out
+ * [parts][L][N] = sum over i < count of ciphertext i ([parts][L][N], NTT
+ * domain, at d_ct + i * ct_stride) times plaintext i (its NTT-domain
+ * polynomial Plaintext::poly_ntt [L][N] at the ciphertexts' level, at d_pt + i
+ * * pt_stride); strides in 64-bit words. Exact, fully reduced, for any count.
+ * count 0 -> FHEHIP_ERR_INVALID_ARG ("At least one iterator is empty"); at
+ * most 4 parts (FHEHIP_ERR_UNSUPPORTED above). */
+int fhehip_dot_product_scalar(const fhehip_context* ctx, const uint64_t* d_ct, uint64_t ct_stride,
+                              const uint64_t* d_pt, uint64_t pt_stride, uint32_t count, uint32_t parts,
+                              uint64_t* d_out, void* stream);
+/* The column products of a PIR server (examples/mulpir.rs:170-179,
+ * sealpir.rs:165-173) in one pass over the database: for each column j <
+ * columns, d_out[j] ([parts][L][N]) = dot_product_scalar of the d_ct rows
+ * ([rows][parts][L][N]) with plaintexts db[0][j], db[1][j], ... of d_db
+ * ([rows][columns][L][N], i.e. database.iter().skip(j).step_by(columns)). */
+int fhehip_dot_product_columns(const fhehip_context* ctx, const uint64_t* d_ct, uint32_t rows, uint32_t parts,
+                               const uint64_t* d_db, uint32_t columns, uint64_t* d_out, void* stream);
 /* ---- Leveled operations ------------------------------------------------------
  * Ciphertext::switch_down (fhe/src/bfv/ciphertext.rs:86-97) and
  * Ciphertext::switch_to_level (:99-119, by repeated calls): the NTT-domain

@@ -382,11 +382,15 @@ struct SecretKey {
 
 // SecretKey::encrypt_poly (secret_key.rs:102-137) with `a` and `e` drawn from the
 // seeded generator. Returns [b, a] in NTT domain.
+// `poly`: Encoding::poly_at_level (the values are the plaintext's
+// coefficients, plaintext.rs:71-98) instead of Encoding::simd.
 static inline PolyData encrypt(const BfvParams& par, const SecretKey& sk, size_t level,
-                               const std::vector<u64>& slots, Rng& rng) {
+                               const std::vector<u64>& slots, Rng& rng, bool poly = false) {
     const Context& c = par.ctx(level);
     size_t sz = c.L() * c.degree;
-    PolyData pt = plaintext_to_poly(par, level, encode_simd(par, slots));
+    std::vector<u64> v = slots;
+    if (poly) v.resize(par.degree, 0);
+    PolyData pt = plaintext_to_poly(par, level, poly ? v : encode_simd(par, slots));
     PolyData s = poly_from_i64(c, sk.coeffs);
     ntt_forward(c, s.data());
     PolyData a = random_poly(c, rng);
@@ -402,9 +406,10 @@ static inline PolyData encrypt(const BfvParams& par, const SecretKey& sk, size_t
     return ct;
 }
 
-// FheDecrypter::try_decrypt (secret_key.rs:200-282) followed by SIMD decode.
+// FheDecrypter::try_decrypt (secret_key.rs:200-282) followed by SIMD decode, or
+// by none with `poly` (Encoding::poly_at_level).
 static inline std::vector<u64> decrypt(const BfvParams& par, const SecretKey& sk, size_t level,
-                                       const u64* ct, size_t nparts) {
+                                       const u64* ct, size_t nparts, bool poly = false) {
     const Level& L = par.levels[level];
     const Context& c = *L.ctx;
     size_t sz = c.L() * c.degree, n = c.degree;
@@ -423,7 +428,7 @@ static inline std::vector<u64> decrypt(const BfvParams& par, const SecretKey& sk
     Modulus q0(par.moduli[0]);
     std::vector<u64> w(n);
     for (size_t j = 0; j < n; j++) w[j] = par.t_mod.reduce(q0.reduce(d[j] + par.t));
-    return decode_simd(par, w);
+    return poly ? w : decode_simd(par, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -579,6 +584,107 @@ static inline PolyData galois_apply(const BfvParams& par, const KeySwitchingKey&
     PolyData out(2 * sz);
     std::copy(o0.begin(), o0.end(), out.begin());
     std::copy(o1.begin(), o1.end(), out.begin() + sz);
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Plaintext-ciphertext dot product and oblivious expansion (the PIR server path)
+// ---------------------------------------------------------------------------
+// dot_product_scalar (fhe/src/bfv/ops/dot_product.rs:56-143). cts: `count`
+// ciphertexts [parts][L][N] every `ct_stride` words; pts: `count` NTT-domain
+// plaintext polynomials [L][N] (Plaintext::poly_ntt) every `pt_stride` words.
+// Up to min_q 2^(2 lz(q)) terms one u128 accumulator per coefficient takes the
+// exact sum (dot_product.rs:110-127); above that each part goes through
+// rq::dot_product (math/src/rq/ops.rs:448-550), which reduces a limb's
+// accumulators whenever its count of terms reaches 2^(2 lz(q)).
+static inline PolyData dot_product_scalar(const Context& c, const u64* cts, size_t ct_stride, const u64* pts,
+                                          size_t pt_stride, size_t count, size_t parts) {
+    if (count == 0) throw std::runtime_error("At least one iterator is empty");
+    const size_t n = c.degree, L = c.L();
+    std::vector<u128> max_acc(L);
+    u128 min_of_max = ~(u128)0;
+    for (size_t j = 0; j < L; j++) {
+        max_acc[j] = (u128)1 << (2 * lzcnt64(c.moduli[j]));
+        min_of_max = std::min(min_of_max, max_acc[j]);
+    }
+    PolyData out(parts * L * n);
+    std::vector<u128> acc(L * n);
+    if ((u128)count > min_of_max) {
+        for (size_t part = 0; part < parts; part++) {
+            std::fill(acc.begin(), acc.end(), 0);
+            std::vector<u128> num_acc(L, 1);
+            for (size_t i = 0; i < count; i++) {
+                const u64* x = cts + i * ct_stride + part * L * n;
+                const u64* y = pts + i * pt_stride;
+                for (size_t k = 0; k < L * n; k++) acc[k] += (u128)x[k] * y[k];
+                for (size_t j = 0; j < L; j++) {
+                    if (++num_acc[j] == max_acc[j]) {
+                        for (size_t k = j * n; k < (j + 1) * n; k++) acc[k] = c.q[j].reduce_u128(acc[k]);
+                        num_acc[j] = 1;
+                    }
+                }
+            }
+            for (size_t j = 0; j < L; j++)
+                for (size_t k = j * n; k < (j + 1) * n; k++) out[part * L * n + k] = c.q[j].reduce_u128(acc[k]);
+        }
+    } else {
+        for (size_t part = 0; part < parts; part++) {
+            std::fill(acc.begin(), acc.end(), 0);
+            for (size_t i = 0; i < count; i++) {
+                const u64* x = cts + i * ct_stride + part * L * n;
+                const u64* y = pts + i * pt_stride;
+                for (size_t k = 0; k < L * n; k++) acc[k] += (u128)x[k] * y[k];
+            }
+            for (size_t j = 0; j < L; j++)
+                for (size_t k = j * n; k < (j + 1) * n; k++) out[part * L * n + k] = c.q[j].reduce_u128(acc[k]);
+        }
+    }
+    return out;
+}
+
+// The expansion monomial of level l: -X^(N - 2^l) in the NTT domain
+// (EvaluationKeyBuilder::build, fhe/src/bfv/keys/evaluation_key.rs:355-363).
+static inline PolyData expansion_monomial(const Context& c, size_t l) {
+    std::vector<i64> m(c.degree, 0);
+    m[c.degree - ((size_t)1 << l)] = -1;
+    PolyData p = poly_from_i64(c, m);
+    ntt_forward(c, p.data());
+    return p;
+}
+
+// EvaluationKey::expands (fhe/src/bfv/keys/evaluation_key.rs:153-193): the
+// oblivious expansion of a [2][L][N] ciphertext into `size` ciphertexts.
+// `gk(l)` returns the Galois key of exponent (N >> l) + 1 (or null).
+template <class KeyOf>
+static inline std::vector<PolyData> expands(const BfvParams& par, const Context& c, KeyOf gk, const u64* ct,
+                                            size_t size) {
+    size_t level = 0;
+    while (((size_t)1 << level) < size) level++;
+    const size_t sz = c.L() * c.degree;
+    if (level == 0) return {PolyData(ct, ct + 2 * sz)};
+    for (size_t l = 0; l < level; l++)
+        if (!gk(l)) throw std::runtime_error("This key does not support expansion at this level");
+    std::vector<PolyData> out((size_t)1 << level, PolyData(2 * sz, 0));
+    std::copy(ct, ct + 2 * sz, out[0].begin());
+    for (size_t l = 0; l < level; l++) {
+        const PolyData mono = expansion_monomial(c, l);
+        const size_t step = (size_t)1 << l, exponent = (c.degree >> l) + 1;
+        for (size_t i = 0; i < step; i++) {
+            PolyData sub = galois_apply(par, *gk(l), exponent, out[i].data());
+            const size_t j = step | i;
+            if (j < size) {
+                PolyData t = out[i];
+                poly_sub(c, t.data(), sub.data());
+                poly_sub(c, t.data() + sz, sub.data() + sz);
+                poly_mul(c, t.data(), mono.data());
+                poly_mul(c, t.data() + sz, mono.data());
+                out[j] = t;
+            }
+            poly_add(c, out[i].data(), sub.data());
+            poly_add(c, out[i].data() + sz, sub.data() + sz);
+        }
+    }
+    out.resize(size);
     return out;
 }
 

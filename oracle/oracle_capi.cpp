@@ -288,18 +288,20 @@ int or_bfv_secret(void* h, int64_t* out) {
         std::copy(s.begin(), s.end(), out);
     })
 }
-int or_bfv_encrypt(void* h, uint32_t level, const uint64_t* slots, uint32_t nslots, uint64_t seed, uint64_t* out) {
+// poly != 0: Encoding::poly_at_level instead of Encoding::simd.
+int or_bfv_encrypt(void* h, uint32_t level, const uint64_t* slots, uint32_t nslots, uint64_t seed, uint32_t poly,
+                   uint64_t* out) {
     OR_TRY({
         OrBfv& b = *(OrBfv*)h;
         Rng rng(seed);
-        PolyData ct = encrypt(*b.par, b.sk, level, std::vector<u64>(slots, slots + nslots), rng);
+        PolyData ct = encrypt(*b.par, b.sk, level, std::vector<u64>(slots, slots + nslots), rng, poly != 0);
         std::copy(ct.begin(), ct.end(), out);
     })
 }
-int or_bfv_decrypt(void* h, uint32_t level, const uint64_t* ct, uint32_t nparts, uint64_t* slots) {
+int or_bfv_decrypt(void* h, uint32_t level, const uint64_t* ct, uint32_t nparts, uint32_t poly, uint64_t* slots) {
     OR_TRY({
         OrBfv& b = *(OrBfv*)h;
-        std::vector<u64> s = decrypt(*b.par, b.sk, level, ct, nparts);
+        std::vector<u64> s = decrypt(*b.par, b.sk, level, ct, nparts, poly != 0);
         std::copy(s.begin(), s.end(), slots);
     })
 }
@@ -421,6 +423,31 @@ int or_galois(void* h, void* gk, uint32_t exponent, const uint64_t* ct, uint64_t
             PolyData r = galois_apply(*b.par, k, exponent, ct + i * sz);
             std::copy(r.begin(), r.end(), out + i * sz);
         }
+    })
+}
+// dot_product_scalar over `count` (ciphertext, plaintext) pairs; out [parts][L][N].
+int or_dot_product_scalar(void* ctx, const uint64_t* cts, uint64_t ct_stride, const uint64_t* pts,
+                          uint64_t pt_stride, uint32_t count, uint32_t parts, uint64_t* out) {
+    OR_TRY({
+        const Context& c = **(CtxPtr*)ctx;
+        PolyData r = dot_product_scalar(c, cts, ct_stride, pts, pt_stride, count, parts);
+        std::copy(r.begin(), r.end(), out);
+    })
+}
+// EvaluationKey::expands with the Galois keys `gks` of exponents `exps`;
+// out [size][2][L][N].
+int or_expands(void* h, void* const* gks, const uint32_t* exps, uint32_t ngk, uint32_t ct_level, const uint64_t* ct,
+               uint32_t size, uint64_t* out) {
+    OR_TRY({
+        OrBfv& b = *(OrBfv*)h;
+        const Context& c = b.par->ctx(ct_level);
+        auto key = [&](size_t l) -> const KeySwitchingKey* {
+            for (uint32_t k = 0; k < ngk; k++)
+                if (exps[k] == (c.degree >> l) + 1) return (const KeySwitchingKey*)gks[k];
+            return nullptr;
+        };
+        std::vector<PolyData> r = expands(*b.par, c, key, ct, size);
+        for (auto& x : r) out = std::copy(x.begin(), x.end(), out);
     })
 }
 // Seeded uniform residues for [count][L][N] in the given moduli.

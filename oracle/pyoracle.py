@@ -60,8 +60,12 @@ def lib():
             "or_bfv_free": (None, [_vp]),
             "or_bfv_level_moduli": (C.c_int, [_vp, C.c_uint32, _u64p, _u32p]),
             "or_bfv_secret": (C.c_int, [_vp, _i64p]),
-            "or_bfv_encrypt": (C.c_int, [_vp, C.c_uint32, _u64p, C.c_uint32, C.c_uint64, _u64p]),
-            "or_bfv_decrypt": (C.c_int, [_vp, C.c_uint32, _u64p, C.c_uint32, _u64p]),
+            "or_bfv_encrypt": (C.c_int, [_vp, C.c_uint32, _u64p, C.c_uint32, C.c_uint64, C.c_uint32, _u64p]),
+            "or_bfv_decrypt": (C.c_int, [_vp, C.c_uint32, _u64p, C.c_uint32, C.c_uint32, _u64p]),
+            "or_dot_product_scalar": (C.c_int, [_vp, _u64p, C.c_uint64, _u64p, C.c_uint64, C.c_uint32, C.c_uint32,
+                                                _u64p]),
+            "or_expands": (C.c_int, [_vp, C.POINTER(C.c_void_p), _u32p, C.c_uint32, C.c_uint32, _u64p, C.c_uint32,
+                                     _u64p]),
             "or_ksk_new": (_vp, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]),
             "or_ksk_free": (None, [_vp]),
             "or_ksk_from_rows": (_vp, [_vp, C.c_uint32, C.c_uint32, _u64p, _u64p]),
@@ -329,6 +333,19 @@ class PolyScaler:
         return out
 
 
+def dot_product_scalar(ctx: Context, cts, pts, count, parts, ct_stride=None, pt_stride=None):
+    """dot_product_scalar (fhe/src/bfv/ops/dot_product.rs:56-143) of `count`
+    ciphertexts [parts][L][N] with NTT-domain plaintext polynomials [L][N],
+    each `ct_stride` / `pt_stride` words apart; returns [parts][L][N]."""
+    L, n = len(ctx.moduli), ctx.n
+    ct_stride = parts * L * n if ct_stride is None else ct_stride
+    pt_stride = L * n if pt_stride is None else pt_stride
+    cts, pts = u64(cts), u64(pts)
+    out = np.zeros(parts * L * n, np.uint64)
+    _chk(lib().or_dot_product_scalar(ctx.h, _p64(cts), ct_stride, _p64(pts), pt_stride, count, parts, _p64(out)))
+    return out
+
+
 def random_polys(moduli, n, count, seed):
     m = u64(moduli)
     out = np.zeros(count * len(moduli) * n, np.uint64)
@@ -363,19 +380,32 @@ class Bfv:
         _chk(lib().or_bfv_secret(self.h, _pi64(out)))
         return out
 
-    def encrypt(self, slots, level=0, seed=7):
+    def encrypt(self, slots, level=0, seed=7, poly=False):
+        """SIMD-encoded slots, or with `poly` the plaintext's coefficients
+        (Encoding::poly_at_level)."""
         s = u64(slots)
         L = len(self.level_moduli(level))
         out = np.zeros(2 * L * self.degree, np.uint64)
-        _chk(lib().or_bfv_encrypt(self.h, level, _p64(s), len(s), seed, _p64(out)))
+        _chk(lib().or_bfv_encrypt(self.h, level, _p64(s), len(s), seed, 1 if poly else 0, _p64(out)))
         return out
 
-    def decrypt(self, ct, level=0):
+    def decrypt(self, ct, level=0, poly=False):
         ct = u64(ct)
         L = len(self.level_moduli(level))
         parts = ct.size // (L * self.degree)
         out = np.zeros(self.degree, np.uint64)
-        _chk(lib().or_bfv_decrypt(self.h, level, _p64(ct), parts, _p64(out)))
+        _chk(lib().or_bfv_decrypt(self.h, level, _p64(ct), parts, 1 if poly else 0, _p64(out)))
+        return out
+
+    def expands(self, gks, ct, size, level=0):
+        """EvaluationKey::expands (fhe/src/bfv/keys/evaluation_key.rs:153-193)
+        with Galois keys `gks` (from galois_key); returns [size][2][L][N]."""
+        ct = u64(ct)
+        L = len(self.level_moduli(level))
+        arr = (C.c_void_p * max(len(gks), 1))(*[k.h for k in gks])
+        exps = np.array([k.exponent for k in gks] or [0], np.uint32)
+        out = np.zeros(max(size, 1) * 2 * L * self.degree, np.uint64)
+        _chk(lib().or_expands(self.h, arr, _p32(exps), len(gks), level, _p64(ct), size, _p64(out)))
         return out
 
     def relin_key(self, ct_level=0, key_level=0, seed=11):

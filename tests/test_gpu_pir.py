@@ -1,0 +1,186 @@
+"""The PIR server path on the GPU, bit for bit against the oracle:
+dot_product_scalar (fhe/src/bfv/ops/dot_product.rs:56-143), the column
+products of a database (examples/mulpir.rs:170-179) and
+EvaluationKey::expands (fhe/src/bfv/keys/evaluation_key.rs:153-193), plus the
+decryption of the expanded ciphertexts."""
+import random
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+# MulPIR's plaintext modulus and moduli (examples/mulpir.rs:35-37)
+T_MULPIR = (1 << 20) + (1 << 19) + (1 << 17) + (1 << 16) + (1 << 14) + 1
+
+
+@pytest.fixture(scope="module")
+def fh():
+    import torch
+    import fhehip
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    fhehip.lib()
+    return fhehip
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+DOT_CASES = [
+    # n, moduli sizes, count, parts, ct padding, pt padding (words)
+    (16, [62, 62], 17, 2, 0, 0),          # past the u128 accumulator's 16 terms
+    (16, [62, 62], 1, 2, 0, 0),
+    (1024, [62, 62, 62], 40, 3, 16, 0),
+    (8192, [50, 55], 58, 2, 0, 8192),     # MulPIR at level 1, padded plaintext rows
+    (4096, [36, 36, 37], 5, 1, 0, 0),
+    (32768, [60] * 4, 3, 4, 0, 0),
+]
+
+
+@pytest.mark.parametrize("n,sizes,count,parts,cpad,ppad", DOT_CASES)
+def test_dot_product_scalar_matches_oracle(fh, oracle, n, sizes, count, parts, cpad, ppad):
+    print(f"dot_product_scalar n={n} sizes={sizes} count={count} parts={parts}", flush=True)
+    moduli = oracle.generate_moduli(sizes, n)
+    L = len(moduli)
+    cs, ps = parts * L * n + cpad, L * n + ppad
+    cts = np.zeros(count * cs, np.uint64)
+    pts = np.zeros(count * ps, np.uint64)
+    rc = oracle.random_polys(moduli, n, count * parts, seed=count).reshape(count, parts * L * n)
+    rp = oracle.random_polys(moduli, n, count, seed=count + 1).reshape(count, L * n)
+    for i in range(count):
+        cts[i * cs:i * cs + parts * L * n] = rc[i]
+        pts[i * ps:i * ps + L * n] = rp[i]
+    ctx = fh.Context(moduli, n)
+    out = fh.empty(parts * L * n)
+    fh.dot_product_scalar(ctx, fh.to_device(cts), cs, fh.to_device(pts), ps, count, parts, out)
+    _sync()
+    want = oracle.dot_product_scalar(oracle.Context(moduli, n), cts, pts, count, parts, cs, ps)
+    assert np.array_equal(fh.to_host(out), want)
+
+
+def test_dot_product_edge_values(fh, oracle):
+    """Operands at q - 1 (the largest products) over many terms: the top word
+    of the accumulators is in use."""
+    n, count = 1024, 300
+    moduli = oracle.generate_moduli([62, 62, 62], n)
+    L = len(moduli)
+    row = np.concatenate([np.full(n, q - 1, np.uint64) for q in moduli])
+    cts = np.tile(row, 2 * count)
+    pts = np.tile(row, count)
+    ctx = fh.Context(moduli, n)
+    out = fh.empty(2 * L * n)
+    fh.dot_product_scalar(ctx, fh.to_device(cts), 2 * L * n, fh.to_device(pts), L * n, count, 2, out)
+    _sync()
+    assert np.array_equal(fh.to_host(out),
+                          oracle.dot_product_scalar(oracle.Context(moduli, n), cts, pts, count, 2))
+    # (q - 1)^2 = 1 mod q: every coefficient is count mod q
+    assert all(int(x) == count for x in fh.to_host(out))
+
+
+COLUMN_CASES = [
+    # n, moduli sizes, rows, columns, parts
+    (8192, [50, 55], 9, 7, 2),     # column count not a multiple of the column group
+    (1024, [62, 62], 20, 5, 3),
+    (4096, [36, 36, 37], 6, 1, 2),
+    (8192, [50, 55], 58, 16, 2),   # MulPIR's first dimension
+    (16, [62, 62], 3, 9, 2),       # blocks straddle limbs
+]
+
+
+@pytest.mark.parametrize("n,sizes,rows,columns,parts", COLUMN_CASES)
+def test_dot_product_columns_match_oracle(fh, oracle, n, sizes, rows, columns, parts):
+    print(f"dot_product_columns n={n} rows={rows} columns={columns} parts={parts}", flush=True)
+    moduli = oracle.generate_moduli(sizes, n)
+    L = len(moduli)
+    cts = oracle.random_polys(moduli, n, rows * parts, seed=rows)
+    db = oracle.random_polys(moduli, n, rows * columns, seed=columns + 100)
+    ctx = fh.Context(moduli, n)
+    out = fh.empty(columns * parts * L * n)
+    fh.dot_product_columns(ctx, fh.to_device(cts), rows, parts, fh.to_device(db), columns, out)
+    _sync()
+    got = fh.to_host(out).reshape(columns, parts * L * n)
+    octx = oracle.Context(moduli, n)
+    for j in range(columns):
+        want = oracle.dot_product_scalar(octx, cts, db[j * L * n:], rows, parts, pt_stride=columns * L * n)
+        assert np.array_equal(got[j], want), j
+
+
+def test_dot_product_errors(fh, oracle):
+    n = 1024
+    moduli = oracle.generate_moduli([62, 62], n)
+    ctx = fh.Context(moduli, n)
+    buf = fh.empty(5 * 2 * n)
+    with pytest.raises(fh.FheHipError):
+        fh.dot_product_scalar(ctx, buf, 4 * n, buf, 2 * n, 0, 2, buf)   # empty
+    with pytest.raises(fh.FheHipError) as e:
+        fh.dot_product_scalar(ctx, buf, 10 * n, buf, 2 * n, 1, 5, buf)  # five parts
+    assert e.value.code == -2
+    with pytest.raises(fh.FheHipError):
+        fh.dot_product_columns(ctx, buf, 0, 2, buf, 3, buf)
+
+
+def _expansion_key(fh, bfv, levels, ct_level, key_level):
+    n = bfv.degree
+    okeys, gkeys = [], []
+    cctx = fh.Context(bfv.level_moduli(ct_level), n)
+    kctx = fh.Context(bfv.level_moduli(key_level), n)
+    for l in range(levels):
+        e = (n >> l) + 1
+        k = bfv.galois_key(e, ct_level=ct_level, key_level=key_level, seed=80 + l)
+        c0, c1 = k.export()
+        gkeys.append(fh.GaloisKey(fh.KeySwitchingKey(cctx, kctx, c0, c1), e))
+        okeys.append(k)
+    return okeys, fh.EvaluationKey(gkeys)
+
+
+EXPAND_CASES = [
+    # n, moduli sizes, t, ct_level, key_level, sizes to expand to
+    (16, [62, 62], 1153, 0, 0, [1, 2, 3, 8, 16]),
+    (1024, [62, 62, 62], 65537, 1, 0, [5, 64]),
+    (8192, [50, 55, 55], T_MULPIR, 1, 0, [115]),     # MulPIR: dim1 + dim2 = 58 + 57
+    (16384, [60] * 7, 65537, 1, 0, [6]),
+    (32768, [60] * 4, 65537, 0, 0, [3]),
+]
+
+
+@pytest.mark.parametrize("n,sizes,t,ct_level,key_level,targets", EXPAND_CASES)
+def test_expand_matches_oracle(fh, oracle, n, sizes, t, ct_level, key_level, targets):
+    print(f"expand n={n} sizes={sizes} levels={ct_level},{key_level} targets={targets}", flush=True)
+    bfv = oracle.Bfv(n, t, oracle.generate_moduli(sizes, n), sk_seed=n + 3)
+    levels = max(s - 1 for s in targets).bit_length()
+    okeys, ek = _expansion_key(fh, bfv, levels, ct_level, key_level)
+    assert ek.expansion_levels() == levels
+    L = len(bfv.level_moduli(ct_level))
+    rng = random.Random(n)
+    for size in targets:
+        v = [rng.randrange(t) for _ in range(size)]
+        ct = bfv.encrypt(v, level=ct_level, seed=size, poly=True)
+        out = fh.empty(size * 2 * L * n)
+        ek.expand(fh.to_device(ct), size, out)
+        _sync()
+        got = fh.to_host(out)
+        want = bfv.expands(okeys, ct, size, level=ct_level)
+        assert np.array_equal(got, want), size
+        lev = (size - 1).bit_length()
+        g = got.reshape(size, 2 * L * n)
+        for j in [0, size // 2, size - 1]:
+            dec = [int(x) for x in bfv.decrypt(g[j], level=ct_level, poly=True)]
+            assert dec == [v[j] * (1 << lev) % t] + [0] * (n - 1), (size, j)
+
+
+def test_expand_errors(fh, oracle):
+    n = 1024
+    bfv = oracle.Bfv(n, 65537, oracle.generate_moduli([62, 62], n), sk_seed=9)
+    okeys, ek = _expansion_key(fh, bfv, 2, 0, 0)
+    assert ek.expansion_levels() == 2
+    ct = fh.to_device(bfv.encrypt([1, 2], seed=1, poly=True))
+    out = fh.empty(8 * 2 * 2 * n)
+    with pytest.raises(fh.FheHipError):
+        ek.expand(ct, 5, out)            # needs level 3
+    with pytest.raises(fh.FheHipError):
+        ek.expand(out, 2, out)           # aliasing
+    ek.expand(ct, 0, out)                # size 0 gives the input, as in fhe.rs
+    _sync()
+    assert np.array_equal(fh.to_host(out)[:2 * 2 * n], fh.to_host(ct))
