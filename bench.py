@@ -415,6 +415,149 @@ def serving(fhehip, torch, rate=SERVE_RATE, seconds=SERVE_SECONDS, seed=6, lanes
             "bit_exact_vs_direct_calls": exact, "setup_s": setup_s}
 
 
+# MulPIR (examples/mulpir.rs:35-37) over the PIR examples' default database
+# (examples/pir.rs: 65536 elements of 1024 bytes).
+PIR_N = 8192
+PIR_T = (1 << 20) + (1 << 19) + (1 << 17) + (1 << 16) + (1 << 14) + 1
+PIR_MODULI_SIZES = [50, 55, 55]
+PIR_ELEMENTS, PIR_ELEMENT_BYTES = 65536, 1024
+
+
+def pir_dims():
+    """encode_database (examples/util.rs:95-123): plaintexts of
+    floor(log2 t) N / (8 element_bytes) elements, as a dim1 x dim2 matrix."""
+    per_pt = (PIR_T.bit_length() - 1) * PIR_N // (PIR_ELEMENT_BYTES * 8)
+    rows = -(-PIR_ELEMENTS // per_pt)
+    d1 = 1
+    while d1 * d1 < rows:
+        d1 += 1
+    return d1, -(-rows // d1)
+
+
+def _random_rows(np, rng, ct_moduli, key_moduli, n):
+    """Key-switching key rows c0, c1 [digits = len(ct_moduli)][len(key_moduli)][N]."""
+    c = []
+    for _ in range(2):
+        rows = np.empty((len(ct_moduli), len(key_moduli), n), dtype=np.uint64)
+        for j, q in enumerate(key_moduli):
+            rows[:, j, :] = rng.integers(0, q, (len(ct_moduli), n), dtype=np.uint64)
+        c.append(rows)
+    return c
+
+
+def pir_server(fhehip, torch, reps=20, check_cpu=True):
+    """MulPIR's server response (examples/mulpir.rs:151-185) on the examples'
+    default database, a 58 x 57 matrix of N = 8192 plaintexts at level 1
+    (433 MiB in HBM): expansion of the query into dim1 + dim2 ciphertexts,
+    the dim2 column dot products over the database in one pass, the dim2
+    ct x ct products (no relinearization) summed, one relinearization and
+    the switch to the last level. Inputs and keys are seeded uniform rows;
+    with check_cpu the oracle recomputes the same response on one core (bit
+    comparison and CPU time)."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    n, t = PIR_N, PIR_T
+    d1, d2 = pir_dims()
+    moduli = fhehip.generate_moduli(PIR_MODULI_SIZES, n)
+    m1, m2 = moduli[:2], moduli[:1]
+    L = len(m1)
+    per, per3 = 2 * L * n, 3 * L * n
+    ctx0, ctx1, ctx2 = fhehip.Context(moduli, n), fhehip.Context(m1, n), fhehip.Context(m2, n)
+    rng = np.random.default_rng(11)
+    levels = (d1 + d2 - 1).bit_length()
+    gk_rows = [((n >> l) + 1, *_random_rows(np, rng, m1, moduli, n)) for l in range(levels)]
+    gks = [fhehip.GaloisKey(fhehip.KeySwitchingKey(ctx1, ctx0, c0, c1), e) for e, c0, c1 in gk_rows]
+    ek = fhehip.EvaluationKey(gks)
+    rk_rows = _random_rows(np, rng, m1, m1, n)
+    rk = fhehip.KeySwitchingKey(ctx1, ctx1, *rk_rows)
+    ext = fhehip.Multiplicator(rk=rk, plaintext=t).extended_basis()
+    mul = fhehip.Multiplicator(base_ctx=ctx1, extended_basis=ext, post=(t, m1[0] * m1[1]))
+    sw = fhehip.Switcher(ctx1, ctx2)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(12)
+    db = _uniform_rows(torch, dev, m1, (d1 * d2, L, n), gen).reshape(-1)
+    query = _uniform_rows(torch, dev, m1, (2, L, n), gen).reshape(-1)
+    expanded = torch.empty((d1 + d2) * per, dtype=torch.int64, device=dev)
+    dots = torch.empty(d2 * per, dtype=torch.int64, device=dev)
+    prods = torch.empty(d2 * per3, dtype=torch.int64, device=dev)
+    relin = torch.empty(per, dtype=torch.int64, device=dev)
+    resp = torch.empty(2 * n, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def respond():
+        ek.expand(query, d1 + d2, expanded, stream)
+        fhehip.dot_product_columns(ctx1, expanded, d1, 2, db, d2, dots, stream)
+        mul.multiply(dots, expanded[d1 * per:], prods, d2, stream)
+        cnt = d2   # out += dot_j * ct_j: pairwise sums of the products
+        while cnt > 1:
+            half = cnt // 2
+            fhehip.ciphertext_add(ctx1, prods, prods[(cnt - half) * per3:], prods, half * 3, stream)
+            cnt -= half
+        rk.relinearize(prods, relin, 1, stream)
+        sw.switch_down(relin, resp, 2, stream)
+
+    for _ in range(3):
+        respond()
+    torch.cuda.synchronize()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+    evs[0].record(stream)
+    for k in range(reps):
+        respond()
+        evs[k + 1].record(stream)
+    torch.cuda.synchronize()
+    ms = [evs[k].elapsed_time(evs[k + 1]) for k in range(reps)]
+    fhehip.profiler_enable(True)
+    respond()
+    torch.cuda.synchronize()
+    kern = fhehip.profiler_report()
+    fhehip.profiler_enable(False)
+    dot_name = f"dot_product/N={n}"
+    dot_ms = kern[dot_name][1] / kern[dot_name][0] if dot_name in kern else None
+    dot_bytes = (d1 * d2 * L * n + d1 * per + d2 * per) * 8
+    res = {"workload": f"MulPIR server response (examples/mulpir.rs:151-185): database of {PIR_ELEMENTS} x "
+                       f"{PIR_ELEMENT_BYTES}-byte elements as {d1} x {d2} plaintexts, N={n}, moduli "
+                       f"{PIR_MODULI_SIZES} at level 1, t={t}; expand to {d1 + d2}, {d2} column dot products, "
+                       f"{d2} ct x ct products summed, relinearize, switch to level 2",
+           "value": 1e3 / percentile(ms, 50), "unit": "responses/s", "p50_ms": percentile(ms, 50),
+           "p99_ms": percentile(ms, 99), "reps": reps, "database_bytes": d1 * d2 * L * n * 8,
+           "kernels_ms": {k: {"launches": c, "avg_ms": v / c} for k, (c, v) in sorted(kern.items())},
+           "dot_product_roofline": {"bound": "hbm", "algorithmic_bytes": dot_bytes, "avg_launch_ms": dot_ms,
+                                    "achieved_GBps": dot_bytes / dot_ms / 1e6 if dot_ms else None,
+                                    "frac": dot_bytes / dot_ms / 1e6 / HBM_PEAK_GBS if dot_ms else None}}
+    if check_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as oracle
+        db_h, q_h, got = fhehip.to_host(db), fhehip.to_host(query), fhehip.to_host(resp)
+        bfv = oracle.Bfv(n, t, moduli, sk_seed=1)
+        okeys = []
+        for e, c0, c1 in gk_rows:
+            k = bfv.key_from_rows(c0, c1, ct_level=1, key_level=0)
+            k.exponent = e
+            okeys.append(k)
+        ork = bfv.key_from_rows(*rk_rows, ct_level=1, key_level=1)
+        o1, o2 = oracle.Context(m1, n), oracle.Context(m2, n)
+        om = oracle.Multiplicator(bfv, None, level=1)
+        qv = np.array(m1, dtype=np.uint64).repeat(n)
+        t0 = time.perf_counter()
+        ex = bfv.expands(okeys, q_h, d1 + d2, level=1).reshape(d1 + d2, per)
+        acc = None
+        for j in range(d2):
+            dj = oracle.dot_product_scalar(o1, ex[:d1].reshape(-1), db_h[j * L * n:], d1, 2, pt_stride=d2 * L * n)
+            pj = om.multiply(dj, ex[d1 + j]).reshape(3, L * n)
+            if acc is None:
+                acc = pj
+            else:
+                acc = acc + pj
+                acc = np.where(acc >= qv, acc - qv, acc)
+        want = oracle.ct_switch_down(o1, o2, bfv.relinearize(ork, acc.reshape(-1)))
+        cpu_s = time.perf_counter() - t0
+        res["bit_exact_vs_oracle"] = bool(np.array_equal(got, want))
+        res["cpu_reference"] = {"seconds_per_response": cpu_s, "cores": 1, "kind": "port",
+                                "sample": "one response by the oracle (C++ restatement, -O3, one thread)"}
+        res["gpu_over_cpu"] = cpu_s * 1e3 / percentile(ms, 50)
+    return res
+
+
 def cpu_baseline(lhs_h, rhs_h, c0, c1, gpu_out0, budget_s):
     """Oracle (C++ restatement of the fhe.rs CPU path) on the host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -454,7 +597,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the NTT autotuning and the secondary workloads (single instance, NTT, uneven, serving)")
-    ap.add_argument("--only", choices=["uneven", "serve"], default=None,
+    ap.add_argument("--only", choices=["uneven", "serve", "pir"], default=None,
                     help="run only this secondary workload and print its JSON object")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="run one device copy of this many MiB (reference for rocprofv3 byte counters)")
@@ -473,6 +616,9 @@ def main():
         return
     if args.only == "serve":
         print(json.dumps(serving(fhehip, torch)))
+        return
+    if args.only == "pir":
+        print(json.dumps(pir_server(fhehip, torch, check_cpu=not args.no_cpu)))
         return
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     dev = torch.device("cuda:0")
@@ -625,6 +771,7 @@ def main():
         extras["ntt_microbench"] = ntt_microbench(fhehip, torch, 50)
         extras["uneven_batch"] = uneven_batch(fhehip, torch)
         extras["serving"] = serving(fhehip, torch)
+        extras["pir_server"] = pir_server(fhehip, torch, check_cpu=not args.no_cpu)
     if args.copy_calibration:
         src = torch.empty(args.copy_calibration << 17, dtype=torch.int64, device=dev)
         dst = torch.empty_like(src)

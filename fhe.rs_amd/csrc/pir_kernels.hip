@@ -6,6 +6,11 @@
 #include "profiler.h"
 #include "rns_device.h"
 
+// Most columns one thread accumulates (4 or 2).
+#ifndef FHEHIP_DOT_CG_MAX
+#define FHEHIP_DOT_CG_MAX 4
+#endif
+
 namespace fhehip {
 
 // dot_product_scalar (fhe/src/bfv/ops/dot_product.rs:56-143) for CG columns
@@ -17,6 +22,23 @@ namespace fhehip {
 // logical indices on one XCD, so the other column groups read those words
 // from L2. 1-D grid of chunks * colgroups blocks of 256 threads.
 template <int PARTS, int CG>
+__device__ __forceinline__ void dot_load(const u64* ct, const u64* pt, const size_t (&coff)[CG], u32 words,
+                                         u64 (&x)[PARTS], u64 (&y)[CG]) {
+#pragma unroll
+    for (int p = 0; p < PARTS; p++) x[p] = ct[(size_t)p * words];
+#pragma unroll
+    for (int c = 0; c < CG; c++) y[c] = pt[coff[c]];
+}
+
+template <int PARTS, int CG>
+__device__ __forceinline__ void dot_mac(Acc3 (&acc)[CG][PARTS], const u64 (&x)[PARTS], const u64 (&y)[CG]) {
+#pragma unroll
+    for (int c = 0; c < CG; c++)
+#pragma unroll
+        for (int p = 0; p < PARTS; p++) acc3_mul<false>(acc[c][p], x[p], y[c]);
+}
+
+template <int PARTS, int CG>
 __global__ __launch_bounds__(256) void dot_product_kernel(DotArgs a) {
     const u32 w = xcd_block(blockIdx.x, gridDim.x);
     const u32 chunk = w / a.colgroups, cg = w % a.colgroups;
@@ -24,6 +46,11 @@ __global__ __launch_bounds__(256) void dot_product_kernel(DotArgs a) {
     if (k >= a.words) return;
     const u32 c0 = cg * CG;
     const u32 ncol = a.columns - c0 < (u32)CG ? a.columns - c0 : (u32)CG;
+    // Columns past the last one re-read the last column (in bounds, no
+    // branch in the loop); their sums are not stored.
+    size_t coff[CG];
+#pragma unroll
+    for (int c = 0; c < CG; c++) coff[c] = (size_t)((u32)c < ncol ? (u32)c : ncol - 1) * a.pt_col_stride;
     Acc3 acc[CG][PARTS];
 #pragma unroll
     for (int c = 0; c < CG; c++)
@@ -31,19 +58,21 @@ __global__ __launch_bounds__(256) void dot_product_kernel(DotArgs a) {
         for (int p = 0; p < PARTS; p++) acc3_init(acc[c][p], 0);
     const u64* ct = a.ct + k;
     const u64* pt = a.pt + (size_t)c0 * a.pt_col_stride + k;
-    for (u32 r = 0; r < a.count; r++) {
-        u64 x[PARTS], y[CG];
-#pragma unroll
-        for (int p = 0; p < PARTS; p++) x[p] = ct[(size_t)p * a.words];
-#pragma unroll
-        for (int c = 0; c < CG; c++) y[c] = (u32)c < ncol ? pt[(size_t)c * a.pt_col_stride] : 0;
-#pragma unroll
-        for (int c = 0; c < CG; c++)
-#pragma unroll
-            for (int p = 0; p < PARTS; p++) acc3_mul<false>(acc[c][p], x[p], y[c]);
+    u64 x[PARTS], y[CG];
+    dot_load<PARTS, CG>(ct, pt, coff, a.words, x, y);
+    // The next row's words are in flight while this row's products accumulate.
+    for (u32 r = 1; r < a.count; r++) {
         ct += a.ct_stride;
         pt += a.pt_row_stride;
+        u64 xn[PARTS], yn[CG];
+        dot_load<PARTS, CG>(ct, pt, coff, a.words, xn, yn);
+        dot_mac<PARTS, CG>(acc, x, y);
+#pragma unroll
+        for (int p = 0; p < PARTS; p++) x[p] = xn[p];
+#pragma unroll
+        for (int c = 0; c < CG; c++) y[c] = yn[c];
     }
+    dot_mac<PARTS, CG>(acc, x, y);
     const ModInfo mi = a.mods[k >> a.logn];
 #pragma unroll
     for (int c = 0; c < CG; c++) {
@@ -75,7 +104,7 @@ void launch_dot_product(DotArgs a, u32 parts, hipStream_t s) {
     if (!a.count || !a.columns || !a.words) return;
     // Columns per thread: the ciphertext words of a row are loaded once for
     // them; the accumulators take 9 VGPRs per (column, part).
-    const int cg = a.columns >= 4 && parts <= 2 ? 4 : (a.columns >= 2 ? 2 : 1);
+    const int cg = a.columns >= 4 && parts <= 2 && FHEHIP_DOT_CG_MAX >= 4 ? 4 : (a.columns >= 2 ? 2 : 1);
     a.colgroups = (a.columns + cg - 1) / cg;
     const u32 chunks = (a.words + 255) / 256;
     const u64 blocks = (u64)chunks * a.colgroups;
