@@ -629,6 +629,121 @@ void EvaluationKey::expand(const u64* ct, u32 size, u64* out, hipStream_t s) con
 }
 
 // ---------------------------------------------------------------------------
+static CodecConst codec_const(u64 q) {
+    CodecConst c{};
+    c.q = q;
+    barrett128(q, c.bl, c.bh);
+    return c;
+}
+
+Codec::Codec(std::shared_ptr<DeviceContext> c, std::shared_ptr<DeviceContext> p, u64 t_, const u64* psi_t,
+             const int64_t* sk_coeffs)
+    : ctx(std::move(c)), plain(std::move(p)), t(t_) {
+    if (ctx->N != plain->N) throw Error(-4, "Incompatible degrees");
+    if (t < 2 || (t >> 62)) throw Error(-1, "plaintext modulus out of range");
+    if (plain->moduli[0] != ctx->moduli[0])
+        throw Error(-4, "the plaintext context must start with the first ciphertext modulus");
+    const u32 n = ctx->N, L = ctx->L;
+    // SIMD needs an NTT modulo t (parameters.rs:497-503); coefficient encoding does not.
+    if (supports_ntt64(t, n)) tctx = make_context({t}, n, psi_t);
+    // delta_i = (-t)^-1 mod q_i (parameters.rs:512-525); q_mod_t = Q mod t (:536-537)
+    std::vector<CodecConst> cc(L);
+    q_mod_t = 1 % t;
+    for (u32 i = 0; i < L; i++) {
+        const u64 q = ctx->moduli[i];
+        cc[i] = codec_const(q);
+        const u64 tq = t % q;
+        if (!tq) throw Error(-1, "Inverse failed");
+        cc[i].delta = invmod64(q - tq, q);
+        cc[i].delta_shoup = shoup64(cc[i].delta, q);
+        q_mod_t = mulmod64(q_mod_t, q % t, t);
+    }
+    q_mod_t_shoup = shoup64(q_mod_t, t);
+    t_const = codec_const(t);
+    q0_const = codec_const(plain->moduli[0]);
+    consts.reset(cc.size() * sizeof(CodecConst));
+    HIPCHK(hipMemcpy(consts.ptr, cc.data(), consts.bytes, hipMemcpyHostToDevice));
+    // matrix_reps_index_map (parameters.rs:614-629): slot i of the first row
+    // and row_size | i of the second are the NTT positions of 3^i and -3^i.
+    std::vector<u32> map(n);
+    const u64 two_n = 2 * (u64)n;
+    u64 pos = 1;
+    for (u32 i = 0; i < n / 2; i++) {
+        map[i] = bit_reverse((u32)((pos - 1) >> 1), ctx->logn);
+        map[n / 2 + i] = bit_reverse((u32)((two_n - pos - 1) >> 1), ctx->logn);
+        pos = (pos * 3) & (two_n - 1);
+    }
+    index_map.reset(n * sizeof(u32));
+    HIPCHK(hipMemcpy(index_map.ptr, map.data(), index_map.bytes, hipMemcpyHostToDevice));
+    BigUint Q(1);
+    for (u64 q : ctx->moduli) Q = Q * BigUint(q);
+    dec.reset(new PolyScaler(ctx, plain, BigUint(t), Q));
+    if (sk_coeffs) {
+        std::vector<u64> h((size_t)L * n);
+        for (u32 i = 0; i < L; i++) {
+            const u64 q = ctx->moduli[i];
+            for (u32 j = 0; j < n; j++) {
+                const int64_t v = sk_coeffs[j];
+                const u64 r = (v < 0 ? 0 - (u64)v : (u64)v) % q;
+                h[(size_t)i * n + j] = v < 0 && r ? q - r : r;
+            }
+        }
+        sk.reset(h.size() * 8);
+        HIPCHK(hipMemcpy(sk.ptr, h.data(), sk.bytes, hipMemcpyHostToDevice));
+        launch_ntt_forward(*ctx, RowMap{sk.as<u64>(), L, L, 0, 0}, L, false, nullptr);
+        HIPCHK(hipStreamSynchronize(nullptr));
+    }
+}
+
+void Codec::encode(const u64* values, u64* out, u32 num, bool simd, bool scaled, hipStream_t s) const {
+    check_batch(num);
+    if (!num) return;
+    const u32 n = ctx->N, L = ctx->L;
+    const u64* m = values;
+    if (simd) {
+        if (!tctx) throw Error(-1, "The plaintext modulus does not support SIMD encoding");
+        u64* rows = scratch(Scratch::Codec).get(s, (size_t)num * n * 8).as<u64>();
+        launch_encode_gather(values, index_map.as<u32>(), rows, ctx->logn, num, s);
+        launch_ntt_backward(*tctx, RowMap{rows, 1, 1, 0, 0}, num, s);
+        m = rows;
+    }
+    launch_pt_lift(m, out, consts.as<CodecConst>(), t, q_mod_t, q_mod_t_shoup, L, ctx->logn, num, scaled, s);
+    launch_ntt_forward(*ctx, RowMap{out, L, L, 0, 0}, num * L, false, s);
+}
+
+void Codec::encrypt(const u64* pt, const u64* a, const int64_t* e, u64* ct, u32 num, hipStream_t s) const {
+    check_batch(num);
+    if (!sk.ptr) throw Error(-1, "no secret key");
+    if (!num) return;
+    const u32 n = ctx->N, L = ctx->L;
+    u64* en = scratch(Scratch::Codec).get(s, (size_t)num * L * n * 8).as<u64>();
+    launch_small_lift(e, en, consts.as<CodecConst>(), L, ctx->logn, num, s);
+    launch_ntt_forward(*ctx, RowMap{en, L, L, 0, 0}, num * L, false, s);
+    launch_encrypt_sk(pt, a, en, sk.as<u64>(), ct, consts.as<CodecConst>(), L, ctx->logn, num, s);
+}
+
+void Codec::decrypt(const u64* ct, u32 parts, u64* values, u32 num, bool simd, hipStream_t s) const {
+    check_batch(num);
+    if (!sk.ptr) throw Error(-1, "no secret key");
+    if (!parts) throw Error(-1, "ciphertexts without parts");
+    if (simd && !tctx) throw Error(-1, "The plaintext modulus does not support SIMD encoding");
+    if (!num) return;
+    const u32 n = ctx->N, L = ctx->L, k = plain->L;
+    const size_t a = (size_t)num * L * n, b = (size_t)num * k * n, c = simd ? (size_t)num * n : 0;
+    u64* ph = scratch(Scratch::Codec).get(s, (a + b + c) * 8).as<u64>();
+    u64* d = ph + a;
+    launch_phase(ct, parts, sk.as<u64>(), ph, consts.as<CodecConst>(), L, ctx->logn, num, s);
+    launch_ntt_backward(*ctx, RowMap{ph, L, L, 0, 0}, num * L, s);
+    dec->scale(ph, d, num, false, s);
+    u64* w = simd ? d + b : values;
+    launch_decode_reduce(d, k, w, q0_const, t_const, ctx->logn, num, s);
+    if (simd) {
+        launch_ntt_forward(*tctx, RowMap{w, 1, 1, 0, 0}, num, false, s);
+        launch_decode_gather(w, index_map.as<u32>(), values, ctx->logn, num, s);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // dot_product_scalar (fhe/src/bfv/ops/dot_product.rs:56-143).
 static void check_dot(u32 count, u32 parts) {
     if (!count) throw Error(-1, "At least one iterator is empty");

@@ -37,7 +37,7 @@ class StreamScratch {
 // every handle shares the pool of each purpose per stream; purposes that are
 // live at the same time (a multiplication and the key switch it calls) use
 // different pools.
-enum class Scratch { Scale, RelinC2, Lift, SwitchDown, Mul, GaloisC2, EvalTmp, ModSwitch, Count };
+enum class Scratch { Scale, RelinC2, Lift, SwitchDown, Mul, GaloisC2, EvalTmp, ModSwitch, Codec, Count };
 StreamScratch& scratch(Scratch k);
 // Frees every pool's buffer of stream `s`.
 void release_scratch(hipStream_t s);
@@ -145,6 +145,30 @@ struct EvaluationKey {
     void rotate(u32 exponent, const u64* ct, u64* out, u32 inst, hipStream_t s) const;
     // out = ct; for each listed exponent: out += rotate(out)
     void rotate_and_sum(const std::vector<u32>& exponents, const u64* ct, u64* out, u32 inst, hipStream_t s);
+};
+
+// BfvParameters at one level for the steps around the evaluation path
+// (parameters.rs:508-560, 614-629) and an optional SecretKey: SIMD /
+// coefficient encoding (Plaintext::try_encode, plaintext_vec.rs:37-101, and
+// Plaintext::to_poly, plaintext.rs:71-98), SecretKey::encrypt_poly
+// (secret_key.rs:102-137) with caller-drawn samples, and try_decrypt
+// (secret_key.rs:200-282) with decoding.
+struct Codec {
+    std::shared_ptr<DeviceContext> ctx, plain, tctx;  // level, plaintext context, {t} (null: no SIMD)
+    std::unique_ptr<PolyScaler> dec;                   // ctx -> plain by t / Q (cipher_plain_context.scaler)
+    u64 t = 0, q_mod_t = 0, q_mod_t_shoup = 0;
+    CodecConst t_const{}, q0_const{};
+    DevBuf consts;     // [L] CodecConst of the level's moduli
+    DevBuf index_map;  // [N] u32 (matrix_reps_index_map)
+    DevBuf sk;         // [L][N] NTT-domain secret (empty without a key)
+    Codec(std::shared_ptr<DeviceContext> ctx, std::shared_ptr<DeviceContext> plain, u64 t, const u64* psi_t,
+          const int64_t* sk_coeffs);
+    // values [num][N] (< t) -> [num][L][N]: `scaled` the to_poly polynomial, else poly_ntt.
+    void encode(const u64* values, u64* out, u32 num, bool simd, bool scaled, hipStream_t s) const;
+    // ct [num][2][L][N] = [e - a s + pt, a]; pt, a [num][L][N] NTT; e [num][N] signed.
+    void encrypt(const u64* pt, const u64* a, const int64_t* e, u64* ct, u32 num, hipStream_t s) const;
+    // ct [num][parts][L][N] -> values [num][N].
+    void decrypt(const u64* ct, u32 parts, u64* values, u32 num, bool simd, hipStream_t s) const;
 };
 
 // dot_product_scalar (fhe/src/bfv/ops/dot_product.rs:56-143) of `count`

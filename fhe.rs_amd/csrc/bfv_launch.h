@@ -128,6 +128,23 @@ void launch_dot_product(DotArgs a, u32 parts, hipStream_t s);
 // the level's monomial [L][N] as (value, Shoup) pairs.
 void launch_expand_step(u64* out, const u64* sub, const ulonglong2* mono, const ModInfo* mods, u32 L, int logn,
                         u32 step, u32 size, hipStream_t s);
+// Per-modulus constants of the encoding, encryption and decryption kernels
+// (codec_kernels.hip): the modulus, floor(2^128 / q) and the scaling
+// constant delta_i of Plaintext::to_poly with its Shoup companion.
+struct CodecConst {
+    u64 q, bl, bh, delta, delta_shoup;
+};
+void launch_encode_gather(const u64* v, const u32* map, u64* rows, int logn, u32 num, hipStream_t s);
+void launch_pt_lift(const u64* m, u64* out, const CodecConst* cc, u64 t, u64 qmt, u64 qmt_shoup, u32 L, int logn,
+                    u32 num, bool scaled, hipStream_t s);
+void launch_small_lift(const int64_t* e, u64* out, const CodecConst* cc, u32 L, int logn, u32 num, hipStream_t s);
+void launch_encrypt_sk(const u64* pt, const u64* a, const u64* e, const u64* sk, u64* ct, const CodecConst* cc,
+                       u32 L, int logn, u32 num, hipStream_t s);
+void launch_phase(const u64* ct, u32 parts, const u64* sk, u64* out, const CodecConst* cc, u32 L, int logn, u32 num,
+                  hipStream_t s);
+void launch_decode_reduce(const u64* d, u32 k, u64* w, const CodecConst& q0, const CodecConst& tc, int logn,
+                          u32 num, hipStream_t s);
+void launch_decode_gather(const u64* rows, const u32* map, u64* out, int logn, u32 num, hipStream_t s);
 // Power-basis switch-down part x [inst][2][L][N] of the finish (unfused path).
 void launch_sd_z(int logn, const KsFinishArgs& f, u64* x, u32 inst, hipStream_t s);
 // Copies `count` slots of `elems` words; `table` is a device array of

@@ -36,6 +36,9 @@ struct fhehip_graph {
 struct fhehip_switcher {
     std::unique_ptr<Switcher> s;
 };
+struct fhehip_codec {
+    std::unique_ptr<Codec> c;
+};
 
 static BigUint factor_num(const fhehip_scaling_factor* f) {
     if (!f || !f->num) return BigUint(1);
@@ -140,6 +143,42 @@ int fhehip_multiplicator_enable_mod_switching(fhehip_multiplicator* m, const fhe
 }
 
 void fhehip_multiplicator_destroy(fhehip_multiplicator* m) { delete m; }
+
+int fhehip_codec_create(const fhehip_context* ctx, const fhehip_context* plain_ctx, uint64_t t, const uint64_t* psi_t,
+                        const int64_t* sk_coeffs, fhehip_codec** out) {
+    FHEHIP_API_BEGIN
+    if (!ctx || !plain_ctx || !out) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    std::unique_ptr<Codec> c(new Codec(ctx->c, plain_ctx->c, t, psi_t, sk_coeffs));
+    *out = new fhehip_codec{std::move(c)};
+    FHEHIP_API_END
+}
+
+void fhehip_codec_destroy(fhehip_codec* c) { delete c; }
+
+int fhehip_encode(const fhehip_codec* c, const uint64_t* d_values, uint32_t num, int simd, int scaled,
+                  uint64_t* d_out, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!c || ((!d_values || !d_out) && num)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    c->c->encode(d_values, d_out, num, simd != 0, scaled != 0, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_encrypt_sk(const fhehip_codec* c, const uint64_t* d_pt, const uint64_t* d_a, const int64_t* d_e,
+                      uint64_t* d_ct, uint32_t num, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!c || ((!d_pt || !d_a || !d_e || !d_ct) && num)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    c->c->encrypt(d_pt, d_a, d_e, d_ct, num, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_decrypt(const fhehip_codec* c, const uint64_t* d_ct, uint32_t parts, uint32_t num, int simd,
+                   uint64_t* d_values, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!c || ((!d_ct || !d_values) && num)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    c->c->decrypt(d_ct, parts, d_values, num, simd != 0, S(stream));
+    FHEHIP_API_END
+}
 
 int fhehip_switcher_create(const fhehip_context* from, const fhehip_context* to, fhehip_switcher** out) {
     FHEHIP_API_BEGIN

@@ -63,6 +63,11 @@ SIGNATURES = {
     "fhehip_inner_sum": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp]),
     "fhehip_rotate_and_sum": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_int, C.c_uint32, _vp]),
     "fhehip_ciphertext_add": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_codec_create": (C.c_int, [_vp, _vp, C.c_uint64, _u64p, C.POINTER(C.c_int64), _vpp]),
+    "fhehip_codec_destroy": (None, [_vp]),
+    "fhehip_encode": (C.c_int, [_vp, _vp, C.c_uint32, C.c_int, C.c_int, _vp, _vp]),
+    "fhehip_encrypt_sk": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_decrypt": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint32, C.c_int, _vp, _vp]),
     "fhehip_expand": (C.c_int, [_vp, _vp, C.c_uint32, _vp, _vp]),
     "fhehip_expansion_levels": (C.c_int, [_vp, _u32p]),
     "fhehip_dot_product_scalar": (C.c_int, [_vp, _vp, C.c_uint64, _vp, C.c_uint64, C.c_uint32, C.c_uint32, _vp,
@@ -327,6 +332,39 @@ class Switcher:
 
     def switch_down(self, d_in, d_out, num_polys, stream=None):
         check(lib().fhehip_switch_down(self.h, dptr(d_in), dptr(d_out), num_polys, stream_ptr(stream)))
+
+
+class Codec:
+    """Encoding, secret-key encryption and decryption at the level of `ctx`
+    (fhehip_codec_*); sk: the SecretKey's signed coefficients or None."""
+
+    def __init__(self, ctx, plain_ctx, t, sk=None, psi_t=None):
+        import numpy as np
+        h = C.c_void_p()
+        self._sk = None if sk is None else np.ascontiguousarray(sk, dtype=np.int64)
+        skp = None if self._sk is None else self._sk.ctypes.data_as(C.POINTER(C.c_int64))
+        psi = None if psi_t is None else _arr64([psi_t])
+        check(lib().fhehip_codec_create(ctx.h, plain_ctx.h, t, psi, skp, C.byref(h)))
+        self.h = h
+        self.ctx, self.plain_ctx = ctx, plain_ctx
+
+    def __del__(self):
+        try:
+            _destroy("fhehip_codec_destroy", getattr(self, "h", None))
+        except Exception:  # module globals already cleared at interpreter exit
+            pass
+
+    def encode(self, d_values, num, d_out, simd=True, scaled=True, stream=None):
+        check(lib().fhehip_encode(self.h, dptr(d_values), num, 1 if simd else 0, 1 if scaled else 0, dptr(d_out),
+                                  stream_ptr(stream)))
+
+    def encrypt(self, d_pt, d_a, d_e, d_ct, num, stream=None):
+        check(lib().fhehip_encrypt_sk(self.h, dptr(d_pt), dptr(d_a), dptr(d_e), dptr(d_ct), num,
+                                      stream_ptr(stream)))
+
+    def decrypt(self, d_ct, parts, num, d_values, simd=True, stream=None):
+        check(lib().fhehip_decrypt(self.h, dptr(d_ct), parts, num, 1 if simd else 0, dptr(d_values),
+                                   stream_ptr(stream)))
 
 
 class GaloisKey:

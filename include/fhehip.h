@@ -44,6 +44,7 @@ typedef struct fhehip_multiplicator fhehip_multiplicator;
 typedef struct fhehip_galois fhehip_galois;
 typedef struct fhehip_evaluation_key fhehip_evaluation_key;
 typedef struct fhehip_switcher fhehip_switcher;
+typedef struct fhehip_codec fhehip_codec;
 
 /* Text of the last error raised on the calling thread. */
 const char* fhehip_last_error(void);
@@ -184,6 +185,38 @@ int fhehip_inner_sum(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint64_t* 
  * the computes_inner_sum loop truncated. */
 int fhehip_rotate_and_sum(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint64_t* d_out, uint32_t log2_slots,
                           int rows, uint32_t num, void* stream);
+/* ---- Encoding, encryption, decryption ------------------------------------------
+ * The steps on either side of the evaluation path, for the BfvParameters
+ * level whose ciphertext context is `ctx`: `plain_ctx` is the parameters'
+ * plaintext context (their first moduli whose sizes sum to at least
+ * log2(t) + 60, parameters.rs:479-494), t the plaintext modulus, psi_t the
+ * primitive 2N-th root of unity modulo t of the parameters' NttOperator
+ * (NULL: the library's root; it fixes the slot order of SIMD encoding) and
+ * sk_coeffs the SecretKey's N signed coefficients (NULL: encoding only).
+ * SIMD encoding needs t = 1 mod 2N. */
+int fhehip_codec_create(const fhehip_context* ctx, const fhehip_context* plain_ctx, uint64_t t, const uint64_t* psi_t,
+                        const int64_t* sk_coeffs, fhehip_codec** out);
+void fhehip_codec_destroy(fhehip_codec* c);
+/* Plaintext::try_encode (plaintext_vec.rs:37-101) of num x N values (< t),
+ * as SIMD slots (simd != 0) or coefficients, into d_out [num][L][N] (NTT
+ * domain): with scaled != 0 the polynomial encryption adds (Plaintext::to_poly,
+ * plaintext.rs:71-98), else Plaintext::poly_ntt (the operand of
+ * ciphertext-plaintext products and dot_product_scalar). */
+int fhehip_encode(const fhehip_codec* c, const uint64_t* d_values, uint32_t num, int simd, int scaled,
+                  uint64_t* d_out, void* stream);
+/* SecretKey::encrypt_poly (secret_key.rs:102-137) with caller-drawn samples:
+ * d_ct [num][2][L][N] = [e - a s + pt, a] for the scaled plaintexts d_pt
+ * [num][L][N], the uniform NTT-domain polynomials d_a [num][L][N] (fhe.rs draws
+ * them with Poly::random_from_seed) and the error coefficients d_e [num][N]
+ * (Poly::small). */
+int fhehip_encrypt_sk(const fhehip_codec* c, const uint64_t* d_pt, const uint64_t* d_a, const int64_t* d_e,
+                      uint64_t* d_ct, uint32_t num, void* stream);
+/* FheDecrypter::try_decrypt (secret_key.rs:200-282) of num ciphertexts of
+ * `parts` polynomials, then decoding as SIMD slots (simd != 0) or
+ * coefficients: d_values [num][N]. */
+int fhehip_decrypt(const fhehip_codec* c, const uint64_t* d_ct, uint32_t parts, uint32_t num, int simd,
+                   uint64_t* d_values, void* stream);
+
 /* ---- PIR server path ---------------------------------------------------------
  * EvaluationKey::expands (fhe/src/bfv/keys/evaluation_key.rs:153-193): the
  * oblivious expansion (eprint 2019/1483) of the ciphertext d_ct [2][L][N]

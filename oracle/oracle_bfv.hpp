@@ -380,6 +380,27 @@ struct SecretKey {
     std::vector<i64> coeffs;
 };
 
+// SecretKey::encrypt_poly (secret_key.rs:102-137) of the NTT-domain plaintext
+// polynomial pt with the given uniform NTT-domain `a` and error coefficients
+// `e`: [e - a s + pt, a].
+static inline PolyData encrypt_with(const BfvParams& par, const SecretKey& sk, size_t level, const u64* pt,
+                                    const u64* a, const std::vector<i64>& e) {
+    const Context& c = par.ctx(level);
+    size_t sz = c.L() * c.degree;
+    PolyData s = poly_from_i64(c, sk.coeffs);
+    ntt_forward(c, s.data());
+    PolyData as(a, a + sz);
+    poly_mul(c, as.data(), s.data());
+    PolyData b = poly_from_i64(c, e);
+    ntt_forward(c, b.data());
+    poly_sub(c, b.data(), as.data());
+    poly_add(c, b.data(), pt);
+    PolyData ct(2 * sz);
+    std::copy(b.begin(), b.end(), ct.begin());
+    std::copy(a, a + sz, ct.begin() + sz);
+    return ct;
+}
+
 // SecretKey::encrypt_poly (secret_key.rs:102-137) with `a` and `e` drawn from the
 // seeded generator. Returns [b, a] in NTT domain.
 // `poly`: Encoding::poly_at_level (the values are the plaintext's
@@ -696,7 +717,19 @@ struct Multiplicator {
     const KeySwitchingKey* rk;  // may be null
     size_t level;
     CtxPtr base_ctx, mul_ctx;
-    std::unique_ptr<PolyScaler> extender, down_scaler;
+    std::unique_ptr<PolyScaler> extender, extender_rhs, down_scaler;  // extender_rhs null: same as lhs
+
+    // Multiplicator::new_leveled (mul.rs:37-98): an explicit extended basis and
+    // the lhs / rhs extension and post-multiplication scaling factors.
+    Multiplicator(const BfvParams& p, const KeySwitchingKey* rk_, size_t lvl, const std::vector<u64>& ext,
+                  const ScalingFactor& lhs, const ScalingFactor& rhs, const ScalingFactor& post)
+        : par(&p), rk(rk_), level(lvl) {
+        base_ctx = p.levels[lvl].ctx;
+        mul_ctx = std::make_shared<Context>(ext, p.degree);
+        extender.reset(new PolyScaler(base_ctx, mul_ctx, lhs));
+        extender_rhs.reset(new PolyScaler(base_ctx, mul_ctx, rhs));
+        down_scaler.reset(new PolyScaler(mul_ctx, base_ctx, post));
+    }
 
     // Multiplicator::default (mul.rs:101-130)
     Multiplicator(const BfvParams& p, const KeySwitchingKey* rk_, size_t lvl) : par(&p), rk(rk_), level(lvl) {
@@ -724,7 +757,8 @@ struct Multiplicator {
         const Context& m = *mul_ctx;
         size_t sz = b.L() * b.degree, msz = m.L() * m.degree;
         PolyData c00 = extender->scale(lhs, true), c01 = extender->scale(lhs + sz, true);
-        PolyData c10 = extender->scale(rhs, true), c11 = extender->scale(rhs + sz, true);
+        const PolyScaler& er = extender_rhs ? *extender_rhs : *extender;
+        PolyData c10 = er.scale(rhs, true), c11 = er.scale(rhs + sz, true);
         PolyData c0 = c00;
         poly_mul(m, c0.data(), c10.data());
         PolyData c1 = c00;

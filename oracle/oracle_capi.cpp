@@ -298,6 +298,36 @@ int or_bfv_encrypt(void* h, uint32_t level, const uint64_t* slots, uint32_t nslo
         std::copy(ct.begin(), ct.end(), out);
     })
 }
+// Plaintext::try_encode (plaintext_vec.rs:37-101): `count` values (< t) as SIMD
+// slots (poly == 0) or coefficients -> [L][N] NTT: with `scaled` the
+// polynomial Plaintext::to_poly adds in encryption (plaintext.rs:71-98), else
+// Plaintext::poly_ntt (the values reduced per modulus, plaintext_vec.rs:75).
+int or_bfv_encode(void* h, uint32_t level, const uint64_t* values, uint32_t count, uint32_t poly, uint32_t scaled,
+                  uint64_t* out) {
+    OR_TRY({
+        OrBfv& b = *(OrBfv*)h;
+        std::vector<u64> v(values, values + count);
+        v.resize(b.par->degree, 0);
+        if (!poly) v = encode_simd(*b.par, std::vector<u64>(values, values + count));
+        PolyData p;
+        if (scaled) {
+            p = plaintext_to_poly(*b.par, level, v);
+        } else {
+            p = poly_from_u64(b.par->ctx(level), v);
+            ntt_forward(b.par->ctx(level), p.data());
+        }
+        std::copy(p.begin(), p.end(), out);
+    })
+}
+// SecretKey::encrypt_poly with the given a ([L][N], NTT) and e (N signed coefficients).
+int or_bfv_encrypt_with(void* h, uint32_t level, const uint64_t* pt, const uint64_t* a, const int64_t* e,
+                        uint64_t* out) {
+    OR_TRY({
+        OrBfv& b = *(OrBfv*)h;
+        PolyData ct = encrypt_with(*b.par, b.sk, level, pt, a, std::vector<i64>(e, e + b.par->degree));
+        std::copy(ct.begin(), ct.end(), out);
+    })
+}
 int or_bfv_decrypt(void* h, uint32_t level, const uint64_t* ct, uint32_t nparts, uint32_t poly, uint64_t* slots) {
     OR_TRY({
         OrBfv& b = *(OrBfv*)h;
@@ -356,6 +386,23 @@ void* or_mult_new(void* h, void* rk, uint32_t level) {
     try {
         OrBfv& b = *(OrBfv*)h;
         return new Multiplicator(*b.par, (KeySwitchingKey*)rk, level);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+// Multiplicator::new_leveled with factors given as 32-bit words (n = 0: one).
+void* or_mult_new_general(void* h, void* rk, uint32_t level, const uint64_t* ext, uint32_t n_ext,
+                          const uint32_t* const* words, const uint32_t* counts) {
+    try {
+        OrBfv& b = *(OrBfv*)h;
+        ScalingFactor f[3];
+        for (int i = 0; i < 3; i++)
+            if (counts[2 * i])
+                f[i] = ScalingFactor(OrBig::from_words(words[2 * i], counts[2 * i]),
+                                     OrBig::from_words(words[2 * i + 1], counts[2 * i + 1]));
+        return new Multiplicator(*b.par, (KeySwitchingKey*)rk, level, std::vector<u64>(ext, ext + n_ext), f[0], f[1],
+                                 f[2]);
     } catch (const std::exception& e) {
         g_err = e.what();
         return nullptr;

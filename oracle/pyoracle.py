@@ -64,6 +64,9 @@ def lib():
             "or_bfv_decrypt": (C.c_int, [_vp, C.c_uint32, _u64p, C.c_uint32, C.c_uint32, _u64p]),
             "or_dot_product_scalar": (C.c_int, [_vp, _u64p, C.c_uint64, _u64p, C.c_uint64, C.c_uint32, C.c_uint32,
                                                 _u64p]),
+            "or_mult_new_general": (_vp, [_vp, _vp, C.c_uint32, _u64p, C.c_uint32, C.POINTER(_u32p), _u32p]),
+            "or_bfv_encode": (C.c_int, [_vp, C.c_uint32, _u64p, C.c_uint32, C.c_uint32, C.c_uint32, _u64p]),
+            "or_bfv_encrypt_with": (C.c_int, [_vp, C.c_uint32, _u64p, _u64p, _i64p, _u64p]),
             "or_expands": (C.c_int, [_vp, C.POINTER(C.c_void_p), _u32p, C.c_uint32, C.c_uint32, _u64p, C.c_uint32,
                                      _u64p]),
             "or_ksk_new": (_vp, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]),
@@ -397,6 +400,23 @@ class Bfv:
         _chk(lib().or_bfv_decrypt(self.h, level, _p64(ct), parts, 1 if poly else 0, _p64(out)))
         return out
 
+    def encode(self, values, level=0, poly=False, scaled=True):
+        """Plaintext::try_encode: the NTT-domain polynomial [L][N] that
+        encryption adds (to_poly, scaled) or Plaintext::poly_ntt."""
+        v = u64(values)
+        L = len(self.level_moduli(level))
+        out = np.zeros(L * self.degree, np.uint64)
+        _chk(lib().or_bfv_encode(self.h, level, _p64(v), len(v), 1 if poly else 0, 1 if scaled else 0, _p64(out)))
+        return out
+
+    def encrypt_with(self, pt, a, e, level=0):
+        """SecretKey::encrypt_poly of pt with the given a (NTT) and e (signed)."""
+        L = len(self.level_moduli(level))
+        out = np.zeros(2 * L * self.degree, np.uint64)
+        e = np.ascontiguousarray(e, dtype=np.int64)
+        _chk(lib().or_bfv_encrypt_with(self.h, level, _p64(u64(pt)), _p64(u64(a)), _pi64(e), _p64(out)))
+        return out
+
     def expands(self, gks, ct, size, level=0):
         """EvaluationKey::expands (fhe/src/bfv/keys/evaluation_key.rs:153-193)
         with Galois keys `gks` (from galois_key); returns [size][2][L][N]."""
@@ -483,9 +503,25 @@ class KeySwitchingKey:
 class Multiplicator:
     """Multiplicator::default(rk) (or without relinearization when rk is None)."""
 
-    def __init__(self, bfv, rk=None, level=0):
+    def __init__(self, bfv, rk=None, level=0, extended_basis=None, lhs=None, rhs=None, post=None):
+        """With extended_basis: Multiplicator::new_leveled (mul.rs:37-98) with
+        the (num, den) factors lhs, rhs, post (None: one)."""
         self.bfv, self.rk, self.level = bfv, rk, level
-        self.h = _chkp(lib().or_mult_new(bfv.h, rk.h if rk is not None else None, level))
+        rkh = rk.h if rk is not None else None
+        if extended_basis is None:
+            self.h = _chkp(lib().or_mult_new(bfv.h, rkh, level))
+            return
+        ext = u64(extended_basis)
+        arrs, counts = [], []
+        for f in (lhs, rhs, post):
+            for x in (f if f is not None else (None, None)):
+                w = big_words(x) if x is not None else np.zeros(1, np.uint32)
+                arrs.append(w)
+                counts.append(len(w) if x is not None else 0)
+        ptrs = (_u32p * 6)(*[_p32(a) for a in arrs])
+        cnt = np.array(counts, np.uint32)
+        self._keep = arrs
+        self.h = _chkp(lib().or_mult_new_general(bfv.h, rkh, level, _p64(ext), len(ext), ptrs, _p32(cnt)))
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

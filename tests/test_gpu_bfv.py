@@ -254,3 +254,32 @@ def test_degree_32768_rejects_two_extra_key_levels(fh, oracle):
     with pytest.raises(fh.FheHipError) as e:
         _gpu_key(fh, bfv, rk, 2, 0)
     assert e.value.code == -2
+
+
+@pytest.mark.parametrize("n,sizes", [(16, [62, 62, 62]), (4096, [50, 55, 55]), (16384, [62] * 6)])
+def test_mul_and_relin_2_matches_oracle(fh, oracle, n, sizes):
+    """The second multiplication strategy of fhe/benches/bfv.rs:257-286:
+    Multiplicator::new(1, P/Q, basis, t/P) with relinearization."""
+    from helpers import mul_and_relin_2_basis
+    print(f"mul_and_relin_2 n={n} sizes={sizes}", flush=True)
+    t = 65537
+    bfv = _bfv(oracle, n, sizes, t=t, seed=6)
+    ext, P, Q = mul_and_relin_2_basis(oracle, bfv.level_moduli(0), n)
+    rk = bfv.relin_key(seed=7)
+    gk, cctx, _ = _gpu_key(fh, bfv, rk, 0, 0)
+    om = oracle.Multiplicator(bfv, rk, 0, extended_basis=ext, rhs=(P, Q), post=(t, P))
+    gm = fh.Multiplicator(rk=gk, base_ctx=cctx, extended_basis=ext, rhs=(P, Q), post=(t, P))
+    assert gm.extended_basis() == ext
+    rng = random.Random(n)
+    batch = 2
+    vs = [[rng.randrange(t) for _ in range(n)] for _ in range(2 * batch)]
+    lhs = np.concatenate([bfv.encrypt(vs[2 * i], seed=10 + i) for i in range(batch)])
+    rhs = np.concatenate([bfv.encrypt(vs[2 * i + 1], seed=20 + i) for i in range(batch)])
+    out = fh.empty(lhs.size)
+    gm.multiply(fh.to_device(lhs), fh.to_device(rhs), out, batch)
+    _sync()
+    got = fh.to_host(out)
+    assert np.array_equal(got, om.multiply(lhs, rhs))
+    for i in range(batch):
+        g = got[i * 2 * len(sizes) * n:(i + 1) * 2 * len(sizes) * n]
+        assert [int(x) for x in bfv.decrypt(g)] == [(a * b) % t for a, b in zip(vs[2 * i], vs[2 * i + 1])]

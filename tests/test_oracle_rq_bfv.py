@@ -6,6 +6,8 @@ import random
 import numpy as np
 import pytest
 
+from helpers import mul_and_relin_2_basis
+
 # math/src/rq/mod.rs test moduli
 MODULI = [1153, 4611686018326724609, 4611686018309947393, 4611686018232352769, 4611686018171535361,
           4611686018106523649]
@@ -212,3 +214,60 @@ def test_ciphertext_switch_down_decrypts(oracle, L):
     low = oracle.ct_switch_down(oracle.Context(bfv.level_moduli(0), n), oracle.Context(bfv.level_moduli(1), n),
                                 prod)
     assert [int(x) for x in bfv.decrypt(low, level=1)] == [(a * b) % t for a, b in zip(v1, v2)]
+
+
+@pytest.mark.parametrize("n,sizes", [(16, [62, 62, 62]), (64, [50, 55, 55])])
+def test_mul_and_relin_2_decrypts(oracle, n, sizes):
+    """Multiplicator::new(1, P/Q, basis, t/P) + relinearization (the
+    mul_and_relin_2 strategy) decrypts to the slot products."""
+    t = 1153
+    bfv = oracle.Bfv(n, t, oracle.generate_moduli(sizes, n), sk_seed=17)
+    ext, P, Q = mul_and_relin_2_basis(oracle, bfv.level_moduli(0), n)
+    rk = bfv.relin_key(seed=18)
+    om = oracle.Multiplicator(bfv, rk, 0, extended_basis=ext, rhs=(P, Q), post=(t, P))
+    assert om.ext_moduli() == ext
+    rng = random.Random(n)
+    v1, v2 = _slots(rng, n, t), _slots(rng, n, t)
+    prod = om.multiply(bfv.encrypt(v1, seed=1), bfv.encrypt(v2, seed=2))
+    assert [int(x) for x in bfv.decrypt(prod)] == [(a * b) % t for a, b in zip(v1, v2)]
+
+
+def _small(rng, n, bound=16):
+    return np.array([rng.randint(-bound, bound) for _ in range(n)], np.int64)
+
+
+@pytest.mark.parametrize("L,level", [(3, 0), (3, 1)])
+def test_encode_encrypt_with_decrypt(oracle, L, level):
+    """Plaintext::try_encode + to_poly, SecretKey::encrypt_poly with explicit
+    samples, try_decrypt: the round trip returns the values, SIMD and
+    coefficient encodings."""
+    n, t = 16, 1153
+    bfv = oracle.Bfv(n, t, oracle.generate_moduli([62] * L, n), sk_seed=4)
+    moduli = bfv.level_moduli(level)
+    rng = random.Random(L + level)
+    for poly in (False, True):
+        v = _slots(rng, n, t)
+        pt = bfv.encode(v, level=level, poly=poly)
+        a = oracle.random_polys(moduli, n, 1, seed=3)
+        ct = bfv.encrypt_with(pt, a, _small(rng, n), level=level)
+        assert np.array_equal(ct[len(moduli) * n:], a)
+        assert [int(x) for x in bfv.decrypt(ct, level=level, poly=poly)] == v
+
+
+def test_plaintext_poly_ntt_is_the_product_operand(oracle):
+    """Plaintext::poly_ntt (the unscaled encoding) multiplies a ciphertext
+    slot-wise (Mul<&Plaintext> for &Ciphertext): both parts times it decrypt
+    to the slot products."""
+    n, t = 16, 1153
+    bfv = oracle.Bfv(n, t, oracle.generate_moduli([62] * 3, n), sk_seed=8)
+    moduli = bfv.level_moduli(0)
+    rng = random.Random(5)
+    v1, v2 = _slots(rng, n, t), _slots(rng, n, t)
+    ct = bfv.encrypt(v1, seed=9)
+    p = bfv.encode(v2, scaled=False)
+    prod = ct.copy()
+    for part in range(2):
+        for i, q in enumerate(moduli):
+            o = (part * len(moduli) + i) * n
+            prod[o:o + n] = [int(x) * int(y) % q for x, y in zip(ct[o:o + n], p[i * n:(i + 1) * n])]
+    assert [int(x) for x in bfv.decrypt(prod)] == [(x * y) % t for x, y in zip(v1, v2)]
