@@ -558,6 +558,54 @@ def pir_server(fhehip, torch, reps=20, check_cpu=True):
     return res
 
 
+def codec_throughput(fhehip, torch, batch=512, reps=10):
+    """The steps either side of the evaluation path at the headline shape
+    (N=16384, the six 60-bit ciphertext moduli, t=65537), batched: SIMD
+    encoding with Plaintext::to_poly plus SecretKey::encrypt_poly (samples
+    a, e generated on the device), and try_decrypt with SIMD decoding."""
+    import numpy as np
+    dev = torch.device("cuda:0")
+    L = len(MODULI) - 1
+    ctx = fhehip.Context(MODULI[:L], N)
+    plain = fhehip.Context(MODULI[:2], N)   # 120 bits >= log2(t) + 60 (parameters.rs:479-494)
+    rng = np.random.default_rng(3)
+    sk = rng.integers(-1, 2, N).astype(np.int64)
+    codec = fhehip.Codec(ctx, plain, T_PLAIN, sk=sk)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(9)
+    vals = torch.randint(0, T_PLAIN, (batch * N,), dtype=torch.int64, device=dev, generator=gen)
+    a = _uniform_rows(torch, dev, MODULI[:L], (batch, L, N), gen).reshape(-1)
+    e = torch.randint(-16, 17, (batch * N,), dtype=torch.int64, device=dev, generator=gen)
+    pt = torch.empty(batch * L * N, dtype=torch.int64, device=dev)
+    ct = torch.empty(batch * 2 * L * N, dtype=torch.int64, device=dev)
+    dec = torch.empty(batch * N, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream()
+
+    def enc():
+        codec.encode(vals, batch, pt, simd=True, scaled=True, stream=stream)
+        codec.encrypt(pt, a, e, ct, batch, stream)
+
+    def dcr():
+        codec.decrypt(ct, 2, batch, dec, simd=True, stream=stream)
+
+    out = {"workload": f"{batch} x (SIMD encode + secret-key encryption) and {batch} x (decryption + SIMD decode), "
+                       f"N={N}, {L} x 60-bit moduli, t={T_PLAIN}", "batch": batch}
+    for name, fn in (("encode_encrypt", enc), ("decrypt_decode", dcr)):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = {"ms_per_batch": ms, "value": batch / (ms / 1e3), "unit": "ciphertexts/s"}
+    out["round_trip_exact"] = bool(torch.equal(dec, vals))
+    return out
+
+
 def cpu_baseline(lhs_h, rhs_h, c0, c1, gpu_out0, budget_s):
     """Oracle (C++ restatement of the fhe.rs CPU path) on the host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -597,7 +645,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the NTT autotuning and the secondary workloads (single instance, NTT, uneven, serving)")
-    ap.add_argument("--only", choices=["uneven", "serve", "pir"], default=None,
+    ap.add_argument("--only", choices=["uneven", "serve", "pir", "codec"], default=None,
                     help="run only this secondary workload and print its JSON object")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="run one device copy of this many MiB (reference for rocprofv3 byte counters)")
@@ -616,6 +664,9 @@ def main():
         return
     if args.only == "serve":
         print(json.dumps(serving(fhehip, torch)))
+        return
+    if args.only == "codec":
+        print(json.dumps(codec_throughput(fhehip, torch)))
         return
     if args.only == "pir":
         print(json.dumps(pir_server(fhehip, torch, check_cpu=not args.no_cpu)))
@@ -772,6 +823,7 @@ def main():
         extras["uneven_batch"] = uneven_batch(fhehip, torch)
         extras["serving"] = serving(fhehip, torch)
         extras["pir_server"] = pir_server(fhehip, torch, check_cpu=not args.no_cpu)
+        extras["codec"] = codec_throughput(fhehip, torch)
     if args.copy_calibration:
         src = torch.empty(args.copy_calibration << 17, dtype=torch.int64, device=dev)
         dst = torch.empty_like(src)

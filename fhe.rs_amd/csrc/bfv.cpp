@@ -722,6 +722,21 @@ void Codec::encrypt(const u64* pt, const u64* a, const int64_t* e, u64* ct, u32 
     launch_encrypt_sk(pt, a, en, sk.as<u64>(), ct, consts.as<CodecConst>(), L, ctx->logn, num, s);
 }
 
+// PublicKey::try_encrypt (public_key.rs:47-87) with caller-drawn u, e1, e2.
+void Codec::encrypt_pk(const u64* pk, const u64* pt, const int64_t* u, const int64_t* e1, const int64_t* e2, u64* ct,
+                       u32 num, hipStream_t s) const {
+    check_batch(num);
+    if (!num) return;
+    const u32 n = ctx->N, L = ctx->L;
+    const size_t stride = (size_t)num * L * n;
+    u64* sm = scratch(Scratch::Codec).get(s, 3 * stride * 8).as<u64>();
+    const int64_t* src[3] = {u, e1, e2};
+    for (int k = 0; k < 3; k++)
+        launch_small_lift(src[k], sm + k * stride, consts.as<CodecConst>(), L, ctx->logn, num, s);
+    launch_ntt_forward(*ctx, RowMap{sm, L, L, 0, 0}, 3 * num * L, false, s);
+    launch_encrypt_pk(pt, pk, sm, stride, ct, consts.as<CodecConst>(), L, ctx->logn, num, s);
+}
+
 void Codec::decrypt(const u64* ct, u32 parts, u64* values, u32 num, bool simd, hipStream_t s) const {
     check_batch(num);
     if (!sk.ptr) throw Error(-1, "no secret key");

@@ -167,6 +167,10 @@ struct Codec {
     void encode(const u64* values, u64* out, u32 num, bool simd, bool scaled, hipStream_t s) const;
     // ct [num][2][L][N] = [e - a s + pt, a]; pt, a [num][L][N] NTT; e [num][N] signed.
     void encrypt(const u64* pt, const u64* a, const int64_t* e, u64* ct, u32 num, hipStream_t s) const;
+    // ct [num][2][L][N] = [u pk0 + e1 + pt, u pk1 + e2]; pk [2][L][N] at this
+    // level; u, e1, e2 [num][N] signed.
+    void encrypt_pk(const u64* pk, const u64* pt, const int64_t* u, const int64_t* e1, const int64_t* e2, u64* ct,
+                    u32 num, hipStream_t s) const;
     // ct [num][parts][L][N] -> values [num][N].
     void decrypt(const u64* ct, u32 parts, u64* values, u32 num, bool simd, hipStream_t s) const;
 };

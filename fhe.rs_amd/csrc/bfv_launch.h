@@ -140,6 +140,8 @@ void launch_pt_lift(const u64* m, u64* out, const CodecConst* cc, u64 t, u64 qmt
 void launch_small_lift(const int64_t* e, u64* out, const CodecConst* cc, u32 L, int logn, u32 num, hipStream_t s);
 void launch_encrypt_sk(const u64* pt, const u64* a, const u64* e, const u64* sk, u64* ct, const CodecConst* cc,
                        u32 L, int logn, u32 num, hipStream_t s);
+void launch_encrypt_pk(const u64* pt, const u64* pk, const u64* sm, size_t sm_stride, u64* ct, const CodecConst* cc,
+                       u32 L, int logn, u32 num, hipStream_t s);
 void launch_phase(const u64* ct, u32 parts, const u64* sk, u64* out, const CodecConst* cc, u32 L, int logn, u32 num,
                   hipStream_t s);
 void launch_decode_reduce(const u64* d, u32 k, u64* w, const CodecConst& q0, const CodecConst& tc, int logn,

@@ -172,6 +172,15 @@ int fhehip_encrypt_sk(const fhehip_codec* c, const uint64_t* d_pt, const uint64_
     FHEHIP_API_END
 }
 
+int fhehip_encrypt_pk(const fhehip_codec* c, const uint64_t* d_pk, const uint64_t* d_pt, const int64_t* d_u,
+                      const int64_t* d_e1, const int64_t* d_e2, uint64_t* d_ct, uint32_t num, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!c || ((!d_pk || !d_pt || !d_u || !d_e1 || !d_e2 || !d_ct) && num))
+        return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    c->c->encrypt_pk(d_pk, d_pt, d_u, d_e1, d_e2, d_ct, num, S(stream));
+    FHEHIP_API_END
+}
+
 int fhehip_decrypt(const fhehip_codec* c, const uint64_t* d_ct, uint32_t parts, uint32_t num, int simd,
                    uint64_t* d_values, void* stream) {
     FHEHIP_API_BEGIN

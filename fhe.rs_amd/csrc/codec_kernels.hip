@@ -74,6 +74,35 @@ __global__ __launch_bounds__(256) void encrypt_sk_kernel(const u64* __restrict__
     ct[o + ((size_t)L << logn)] = av;
 }
 
+// PublicKey::try_encrypt (public_key.rs:47-87): ct = [u pk0 + e1 + pt,
+// u pk1 + e2] with u, e1, e2 = sm[0..3][b] in the NTT domain. grid = (N/256, L, num)
+__global__ __launch_bounds__(256) void encrypt_pk_kernel(const u64* __restrict__ pt, const u64* __restrict__ pk,
+                                                         const u64* __restrict__ sm, size_t sm_stride,
+                                                         u64* __restrict__ ct, const CodecConst* __restrict__ cc,
+                                                         u32 L, u32 logn) {
+    const u32 j = blockIdx.x * 256 + threadIdx.x;
+    if (j >> logn) return;
+    const u32 i = blockIdx.y, b = blockIdx.z;
+    const CodecConst c = cc[i];
+    const size_t w = (((size_t)b * L + i) << logn) + j;
+    const size_t k = ((size_t)i << logn) + j, row = (size_t)L << logn;
+    const u64 u = sm[w];
+    const u64 c0 = add_mod(add_mod(mul_mod(u, pk[k], c.bl, c.bh, c.q), sm[sm_stride + w], c.q), pt[w], c.q);
+    const u64 c1 = add_mod(mul_mod(u, pk[row + k], c.bl, c.bh, c.q), sm[2 * sm_stride + w], c.q);
+    const size_t o = ((((size_t)b * 2) * L + i) << logn) + j;
+    ct[o] = c0;
+    ct[o + row] = c1;
+}
+
+void launch_encrypt_pk(const u64* pt, const u64* pk, const u64* sm, size_t sm_stride, u64* ct, const CodecConst* cc,
+                       u32 L, int logn, u32 num, hipStream_t s) {
+    if (!num) return;
+    ProfScope ps("encrypt_pk", logn, s);
+    hipLaunchKernelGGL(encrypt_pk_kernel, cgrid(logn, L, num), dim3(256), 0, s, pt, pk, sm, sm_stride, ct, cc, L,
+                       (u32)logn);
+    HIPCHK(hipGetLastError());
+}
+
 // The phase c0 + c1 s + c2 s^2 + ... of try_decrypt (secret_key.rs:211-223),
 // NTT domain. grid = (N/256, L, num)
 __global__ __launch_bounds__(256) void phase_kernel(const u64* __restrict__ ct, u32 parts, const u64* __restrict__ s,

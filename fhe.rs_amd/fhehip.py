@@ -67,6 +67,7 @@ SIGNATURES = {
     "fhehip_codec_destroy": (None, [_vp]),
     "fhehip_encode": (C.c_int, [_vp, _vp, C.c_uint32, C.c_int, C.c_int, _vp, _vp]),
     "fhehip_encrypt_sk": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_encrypt_pk": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_uint32, _vp]),
     "fhehip_decrypt": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint32, C.c_int, _vp, _vp]),
     "fhehip_expand": (C.c_int, [_vp, _vp, C.c_uint32, _vp, _vp]),
     "fhehip_expansion_levels": (C.c_int, [_vp, _u32p]),
@@ -361,6 +362,10 @@ class Codec:
     def encrypt(self, d_pt, d_a, d_e, d_ct, num, stream=None):
         check(lib().fhehip_encrypt_sk(self.h, dptr(d_pt), dptr(d_a), dptr(d_e), dptr(d_ct), num,
                                       stream_ptr(stream)))
+
+    def encrypt_pk(self, d_pk, d_pt, d_u, d_e1, d_e2, d_ct, num, stream=None):
+        check(lib().fhehip_encrypt_pk(self.h, dptr(d_pk), dptr(d_pt), dptr(d_u), dptr(d_e1), dptr(d_e2), dptr(d_ct),
+                                      num, stream_ptr(stream)))
 
     def decrypt(self, d_ct, parts, num, d_values, simd=True, stream=None):
         check(lib().fhehip_decrypt(self.h, dptr(d_ct), parts, num, 1 if simd else 0, dptr(d_values),

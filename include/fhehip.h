@@ -211,6 +211,13 @@ int fhehip_encode(const fhehip_codec* c, const uint64_t* d_values, uint32_t num,
  * (Poly::small). */
 int fhehip_encrypt_sk(const fhehip_codec* c, const uint64_t* d_pt, const uint64_t* d_a, const int64_t* d_e,
                       uint64_t* d_ct, uint32_t num, void* stream);
+/* PublicKey::try_encrypt (public_key.rs:47-87) with caller-drawn samples:
+ * d_ct [num][2][L][N] = [u pk0 + e1 + pt, u pk1 + e2] for the public key d_pk
+ * [2][L][N] at the codec's level (switch it down first if it is above),
+ * the scaled plaintexts d_pt [num][L][N] and the small coefficients d_u,
+ * d_e1, d_e2 [num][N] (Poly::small). No secret key needed. */
+int fhehip_encrypt_pk(const fhehip_codec* c, const uint64_t* d_pk, const uint64_t* d_pt, const int64_t* d_u,
+                      const int64_t* d_e1, const int64_t* d_e2, uint64_t* d_ct, uint32_t num, void* stream);
 /* FheDecrypter::try_decrypt (secret_key.rs:200-282) of num ciphertexts of
  * `parts` polynomials, then decoding as SIMD slots (simd != 0) or
  * coefficients: d_values [num][N]. */

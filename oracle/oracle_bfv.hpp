@@ -401,6 +401,30 @@ static inline PolyData encrypt_with(const BfvParams& par, const SecretKey& sk, s
     return ct;
 }
 
+// PublicKey::try_encrypt (public_key.rs:47-87) of the NTT-domain plaintext
+// polynomial pt with the public key pk [2][L][N] at the same level and the
+// given small coefficients u, e1, e2: [u pk0 + e1 + pt, u pk1 + e2].
+static inline PolyData encrypt_pk_with(const BfvParams& par, size_t level, const u64* pk, const u64* pt,
+                                       const std::vector<i64>& u, const std::vector<i64>& e1,
+                                       const std::vector<i64>& e2) {
+    const Context& c = par.ctx(level);
+    size_t sz = c.L() * c.degree;
+    PolyData un = poly_from_i64(c, u), e1n = poly_from_i64(c, e1), e2n = poly_from_i64(c, e2);
+    ntt_forward(c, un.data());
+    ntt_forward(c, e1n.data());
+    ntt_forward(c, e2n.data());
+    PolyData c0(pk, pk + sz), c1(pk + sz, pk + 2 * sz);
+    poly_mul(c, c0.data(), un.data());
+    poly_add(c, c0.data(), e1n.data());
+    poly_add(c, c0.data(), pt);
+    poly_mul(c, c1.data(), un.data());
+    poly_add(c, c1.data(), e2n.data());
+    PolyData ct(2 * sz);
+    std::copy(c0.begin(), c0.end(), ct.begin());
+    std::copy(c1.begin(), c1.end(), ct.begin() + sz);
+    return ct;
+}
+
 // SecretKey::encrypt_poly (secret_key.rs:102-137) with `a` and `e` drawn from the
 // seeded generator. Returns [b, a] in NTT domain.
 // `poly`: Encoding::poly_at_level (the values are the plaintext's

@@ -328,6 +328,17 @@ int or_bfv_encrypt_with(void* h, uint32_t level, const uint64_t* pt, const uint6
         std::copy(ct.begin(), ct.end(), out);
     })
 }
+// PublicKey::try_encrypt with the given u, e1, e2 (N signed coefficients each).
+int or_bfv_encrypt_pk_with(void* h, uint32_t level, const uint64_t* pk, const uint64_t* pt, const int64_t* u,
+                           const int64_t* e1, const int64_t* e2, uint64_t* out) {
+    OR_TRY({
+        OrBfv& b = *(OrBfv*)h;
+        size_t n = b.par->degree;
+        PolyData ct = encrypt_pk_with(*b.par, level, pk, pt, std::vector<i64>(u, u + n), std::vector<i64>(e1, e1 + n),
+                                      std::vector<i64>(e2, e2 + n));
+        std::copy(ct.begin(), ct.end(), out);
+    })
+}
 int or_bfv_decrypt(void* h, uint32_t level, const uint64_t* ct, uint32_t nparts, uint32_t poly, uint64_t* slots) {
     OR_TRY({
         OrBfv& b = *(OrBfv*)h;

@@ -67,6 +67,7 @@ def lib():
             "or_mult_new_general": (_vp, [_vp, _vp, C.c_uint32, _u64p, C.c_uint32, C.POINTER(_u32p), _u32p]),
             "or_bfv_encode": (C.c_int, [_vp, C.c_uint32, _u64p, C.c_uint32, C.c_uint32, C.c_uint32, _u64p]),
             "or_bfv_encrypt_with": (C.c_int, [_vp, C.c_uint32, _u64p, _u64p, _i64p, _u64p]),
+            "or_bfv_encrypt_pk_with": (C.c_int, [_vp, C.c_uint32, _u64p, _u64p, _i64p, _i64p, _i64p, _u64p]),
             "or_expands": (C.c_int, [_vp, C.POINTER(C.c_void_p), _u32p, C.c_uint32, C.c_uint32, _u64p, C.c_uint32,
                                      _u64p]),
             "or_ksk_new": (_vp, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]),
@@ -415,6 +416,15 @@ class Bfv:
         out = np.zeros(2 * L * self.degree, np.uint64)
         e = np.ascontiguousarray(e, dtype=np.int64)
         _chk(lib().or_bfv_encrypt_with(self.h, level, _p64(u64(pt)), _p64(u64(a)), _pi64(e), _p64(out)))
+        return out
+
+    def encrypt_pk_with(self, pk, pt, u, e1, e2, level=0):
+        """PublicKey::try_encrypt of pt with the public key pk [2][L][N] and the given u, e1, e2."""
+        L = len(self.level_moduli(level))
+        out = np.zeros(2 * L * self.degree, np.uint64)
+        u, e1, e2 = [np.ascontiguousarray(x, dtype=np.int64) for x in (u, e1, e2)]
+        _chk(lib().or_bfv_encrypt_pk_with(self.h, level, _p64(u64(pk)), _p64(u64(pt)), _pi64(u), _pi64(e1),
+                                          _pi64(e2), _p64(out)))
         return out
 
     def expands(self, gks, ct, size, level=0):

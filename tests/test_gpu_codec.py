@@ -118,3 +118,26 @@ def test_codec_errors(fh, oracle):
         c.decrypt(buf, 2, 1, buf)
     with pytest.raises(fh.FheHipError):
         fh.Codec(ctx, fh.Context(moduli[1:], n), 65537)   # plaintext context not a prefix
+
+
+@pytest.mark.parametrize("n,sizes,level", [(16, [62, 62, 62], 1), (16384, [60] * 7, 1)])
+def test_public_key_encryption_matches_oracle(fh, oracle, n, sizes, level):
+    print(f"encrypt_pk n={n} level={level}", flush=True)
+    t = 65537 if n > 16 else 1153
+    bfv = oracle.Bfv(n, t, oracle.generate_moduli(sizes, n), sk_seed=21)
+    moduli = bfv.level_moduli(level)
+    rng = random.Random(n)
+    small = lambda b: np.array([rng.randint(-b, b) for _ in range(n)], np.int64)
+    pk = bfv.encrypt_with(bfv.encode([0] * n, level=level, poly=True), oracle.random_polys(moduli, n, 1, seed=2),
+                          small(16), level=level)
+    codec = _codec(fh, bfv, level, with_sk=False)
+    v = [rng.randrange(t) for _ in range(n)]
+    pt = bfv.encode(v, level=level)
+    u, e1, e2 = small(1), small(16), small(16)
+    ct = fh.empty(2 * len(moduli) * n)
+    codec.encrypt_pk(fh.to_device(pk), fh.to_device(pt), fh.to_device(u.view(np.uint64)),
+                     fh.to_device(e1.view(np.uint64)), fh.to_device(e2.view(np.uint64)), ct, 1)
+    _sync()
+    got = fh.to_host(ct)
+    assert np.array_equal(got, bfv.encrypt_pk_with(pk, pt, u, e1, e2, level=level))
+    assert [int(x) for x in bfv.decrypt(got, level=level)] == v

@@ -271,3 +271,16 @@ def test_plaintext_poly_ntt_is_the_product_operand(oracle):
             o = (part * len(moduli) + i) * n
             prod[o:o + n] = [int(x) * int(y) % q for x, y in zip(ct[o:o + n], p[i * n:(i + 1) * n])]
     assert [int(x) for x in bfv.decrypt(prod)] == [(x * y) % t for x, y in zip(v1, v2)]
+
+
+def test_public_key_encryption_decrypts(oracle):
+    """PublicKey::new (a secret-key encryption of zero, public_key.rs:25-37)
+    and PublicKey::try_encrypt with explicit u, e1, e2: decrypts to the slots."""
+    n, t = 16, 1153
+    bfv = oracle.Bfv(n, t, oracle.generate_moduli([62] * 3, n), sk_seed=12)
+    moduli = bfv.level_moduli(0)
+    rng = random.Random(12)
+    pk = bfv.encrypt_with(bfv.encode([0] * n, poly=True), oracle.random_polys(moduli, n, 1, seed=4), _small(rng, n))
+    v = _slots(rng, n, t)
+    ct = bfv.encrypt_pk_with(pk, bfv.encode(v), _small(rng, n, 2), _small(rng, n), _small(rng, n))
+    assert [int(x) for x in bfv.decrypt(ct)] == v
