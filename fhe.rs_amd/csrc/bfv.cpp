@@ -759,6 +759,32 @@ void Codec::decrypt(const u64* ct, u32 parts, u64* values, u32 num, bool simd, h
 }
 
 // ---------------------------------------------------------------------------
+void rgsw_external_product(const Ksk& k0, const Ksk& k1, const u64* ct, u64* out, u32 inst, hipStream_t s) {
+    check_batch(inst);
+    const DeviceContext& c = *k0.ct_ctx;
+    for (const Ksk* k : {&k0, &k1})
+        if (k->ct_ctx->moduli != c.moduli || k->key_ctx->moduli != c.moduli || k->ct_ctx->N != c.N)
+            throw Error(-4, "Inconsistent key switching levels");
+    if (!inst) return;
+    const size_t n = c.N, L = c.L, part = L * n;
+    u64* pb = scratch(Scratch::Rgsw).get(s, (size_t)inst * 4 * part * 8).as<u64>();
+    u64* tmp = pb + (size_t)inst * 2 * part;
+    InttRowsArgs ia{};
+    ia.src[0] = ct;
+    ia.src[1] = ct + part;
+    ia.src_inst_stride = 2 * part;
+    ia.dst = pb;
+    ia.dst_inst_stride = 2 * part;
+    ia.dst_part_stride = part;
+    ia.inv_tw = c.inv_tw.as<ulonglong2>();
+    ia.mods = c.mod.as<ModInfo>();
+    intt_rows(c, ia, (u32)L, 2, inst, s);
+    k0.key_switch_pb(pb, 2 * part, out, inst, false, s);
+    k1.key_switch_pb(pb + part, 2 * part, tmp, inst, false, s);
+    launch_add_rows(out, tmp, out, c.mod.as<ModInfo>(), (u32)L, c.logn, (size_t)inst * 2 * L, s);
+}
+
+// ---------------------------------------------------------------------------
 // dot_product_scalar (fhe/src/bfv/ops/dot_product.rs:56-143).
 static void check_dot(u32 count, u32 parts) {
     if (!count) throw Error(-1, "At least one iterator is empty");

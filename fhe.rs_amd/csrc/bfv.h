@@ -37,7 +37,7 @@ class StreamScratch {
 // every handle shares the pool of each purpose per stream; purposes that are
 // live at the same time (a multiplication and the key switch it calls) use
 // different pools.
-enum class Scratch { Scale, RelinC2, Lift, SwitchDown, Mul, GaloisC2, EvalTmp, ModSwitch, Codec, Count };
+enum class Scratch { Scale, RelinC2, Lift, SwitchDown, Mul, GaloisC2, EvalTmp, ModSwitch, Codec, Rgsw, Count };
 StreamScratch& scratch(Scratch k);
 // Frees every pool's buffer of stream `s`.
 void release_scratch(hipStream_t s);
@@ -174,6 +174,11 @@ struct Codec {
     // ct [num][parts][L][N] -> values [num][N].
     void decrypt(const u64* ct, u32 parts, u64* values, u32 num, bool simd, hipStream_t s) const;
 };
+
+// Ciphertext x RGSWCiphertext (fhe/src/bfv/rgsw_ciphertext.rs:123-160):
+// ct [inst][2][L][N] NTT -> out [inst][2][L][N] = key_switch(c0, k0) +
+// key_switch(c1, k1); both keys at the ciphertext's level.
+void rgsw_external_product(const Ksk& k0, const Ksk& k1, const u64* ct, u64* out, u32 inst, hipStream_t s);
 
 // dot_product_scalar (fhe/src/bfv/ops/dot_product.rs:56-143) of `count`
 // ciphertexts [parts][L][N] (ct_stride words apart) with NTT-domain plaintext

@@ -94,6 +94,15 @@ int fhehip_key_switch(const fhehip_ksk* k, const uint64_t* d_p, uint64_t* d_out,
     FHEHIP_API_END
 }
 
+int fhehip_rgsw_external_product(const fhehip_ksk* ksk0, const fhehip_ksk* ksk1, const uint64_t* d_ct,
+                                 uint64_t* d_out, uint32_t num, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!ksk0 || !ksk1 || ((!d_ct || !d_out) && num)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    if (d_ct == d_out && num) return fail(FHEHIP_ERR_INVALID_ARG, "out must not alias ct");
+    rgsw_external_product(*ksk0->k, *ksk1->k, d_ct, d_out, num, S(stream));
+    FHEHIP_API_END
+}
+
 int fhehip_relinearize(fhehip_ksk* rk, const uint64_t* d_ct3, uint64_t* d_out, uint32_t num, void* stream) {
     FHEHIP_API_BEGIN
     if (!rk || ((!d_ct3 || !d_out) && num)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");

@@ -63,6 +63,7 @@ SIGNATURES = {
     "fhehip_inner_sum": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp]),
     "fhehip_rotate_and_sum": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_int, C.c_uint32, _vp]),
     "fhehip_ciphertext_add": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_rgsw_external_product": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
     "fhehip_codec_create": (C.c_int, [_vp, _vp, C.c_uint64, _u64p, C.POINTER(C.c_int64), _vpp]),
     "fhehip_codec_destroy": (None, [_vp]),
     "fhehip_encode": (C.c_int, [_vp, _vp, C.c_uint32, C.c_int, C.c_int, _vp, _vp]),
@@ -333,6 +334,11 @@ class Switcher:
 
     def switch_down(self, d_in, d_out, num_polys, stream=None):
         check(lib().fhehip_switch_down(self.h, dptr(d_in), dptr(d_out), num_polys, stream_ptr(stream)))
+
+
+def rgsw_external_product(ksk0, ksk1, d_ct, d_out, num, stream=None):
+    """Ciphertext x RGSWCiphertext with the RGSW ciphertext's two KeySwitchingKeys."""
+    check(lib().fhehip_rgsw_external_product(ksk0.h, ksk1.h, dptr(d_ct), dptr(d_out), num, stream_ptr(stream)))
 
 
 class Codec:

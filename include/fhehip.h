@@ -185,6 +185,15 @@ int fhehip_inner_sum(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint64_t* 
  * the computes_inner_sum loop truncated. */
 int fhehip_rotate_and_sum(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint64_t* d_out, uint32_t log2_slots,
                           int rows, uint32_t num, void* stream);
+/* Ciphertext x RGSWCiphertext (fhe/src/bfv/rgsw_ciphertext.rs:123-160): an
+ * RGSW ciphertext is the pair of KeySwitchingKeys (ksk0 from m, ksk1 from
+ * m s) created with fhehip_ksk_create at the ciphertext's level (key context
+ * = ciphertext context). d_out [num][2][L][N] = key_switch(c0, ksk0) +
+ * key_switch(c1, ksk1) for d_ct [num][2][L][N]; FHEHIP_ERR_CTX_MISMATCH for
+ * keys at other levels ("Inconsistent key switching levels"). */
+int fhehip_rgsw_external_product(const fhehip_ksk* ksk0, const fhehip_ksk* ksk1, const uint64_t* d_ct,
+                                 uint64_t* d_out, uint32_t num, void* stream);
+
 /* ---- Encoding, encryption, decryption ------------------------------------------
  * The steps on either side of the evaluation path, for the BfvParameters
  * level whose ciphertext context is `ctx`: `plain_ctx` is the parameters'

@@ -734,6 +734,44 @@ static inline std::vector<PolyData> expands(const BfvParams& par, const Context&
 }
 
 // ---------------------------------------------------------------------------
+// RGSWCiphertext (fhe/src/bfv/rgsw_ciphertext.rs)
+// ---------------------------------------------------------------------------
+// SecretKey::try_encrypt into an RGSW ciphertext (rgsw_ciphertext.rs:97-117):
+// key-switching keys at the plaintext's level from m (which == 0) or from m s
+// (which == 1), m = the plaintext's poly_ntt [L][N].
+static inline KeySwitchingKey* rgsw_key_new(const BfvParams& par, const SecretKey& sk, size_t level,
+                                            const u64* pt_ntt, int which, Rng& rng) {
+    const Context& c = par.ctx(level);
+    PolyData from(pt_ntt, pt_ntt + c.L() * c.degree);
+    if (which) {
+        PolyData s = poly_from_i64(c, sk.coeffs);
+        ntt_forward(c, s.data());
+        poly_mul(c, from.data(), s.data());
+    }
+    ntt_backward(c, from.data());
+    return new KeySwitchingKey(par, sk, from, level, level, rng);
+}
+
+// Ciphertext x RGSWCiphertext (rgsw_ciphertext.rs:123-160): both parts to
+// power basis, key switched by ksk0 and ksk1, the results added.
+static inline PolyData rgsw_external_product(const KeySwitchingKey& k0, const KeySwitchingKey& k1, const u64* ct) {
+    const Context& c = *k0.ctx_ct;
+    size_t sz = c.L() * c.degree;
+    PolyData c0(ct, ct + sz), c1(ct + sz, ct + 2 * sz);
+    ntt_backward(c, c0.data());
+    ntt_backward(c, c1.data());
+    PolyData a0, a1, b0, b1;
+    k0.key_switch(c0.data(), a0, a1);
+    k1.key_switch(c1.data(), b0, b1);
+    const Context& k = *k0.ctx_ksk;
+    poly_add(k, a0.data(), b0.data());
+    poly_add(k, a1.data(), b1.data());
+    PolyData out(a0);
+    out.insert(out.end(), a1.begin(), a1.end());
+    return out;
+}
+
+// ---------------------------------------------------------------------------
 // Multiplicator — fhe/src/bfv/ops/mul.rs:22-227
 // ---------------------------------------------------------------------------
 struct Multiplicator {

@@ -358,6 +358,28 @@ void* or_ksk_new(void* h, uint32_t kind, uint32_t exponent, uint32_t ct_level, u
         return nullptr;
     }
 }
+// One key of an RGSW encryption of the plaintext whose poly_ntt is pt_ntt.
+void* or_rgsw_key_new(void* h, uint32_t level, const uint64_t* pt_ntt, uint32_t which, uint64_t seed) {
+    try {
+        OrBfv& b = *(OrBfv*)h;
+        Rng rng(seed);
+        return rgsw_key_new(*b.par, b.sk, level, pt_ntt, which, rng);
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+int or_rgsw_external_product(void* k0, void* k1, const uint64_t* ct, uint64_t* out, uint32_t count) {
+    OR_TRY({
+        const KeySwitchingKey& a = *(KeySwitchingKey*)k0;
+        const KeySwitchingKey& b = *(KeySwitchingKey*)k1;
+        size_t sz = 2 * a.ctx_ct->L() * a.ctx_ct->degree;
+        for (uint32_t i = 0; i < count; i++) {
+            PolyData r = rgsw_external_product(a, b, ct + i * sz);
+            std::copy(r.begin(), r.end(), out + i * sz);
+        }
+    })
+}
 void or_ksk_free(void* k) { delete (KeySwitchingKey*)k; }
 void* or_ksk_from_rows(void* h, uint32_t ct_level, uint32_t key_level, const uint64_t* c0, const uint64_t* c1) {
     try {

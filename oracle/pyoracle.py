@@ -68,6 +68,8 @@ def lib():
             "or_bfv_encode": (C.c_int, [_vp, C.c_uint32, _u64p, C.c_uint32, C.c_uint32, C.c_uint32, _u64p]),
             "or_bfv_encrypt_with": (C.c_int, [_vp, C.c_uint32, _u64p, _u64p, _i64p, _u64p]),
             "or_bfv_encrypt_pk_with": (C.c_int, [_vp, C.c_uint32, _u64p, _u64p, _i64p, _i64p, _i64p, _u64p]),
+            "or_rgsw_key_new": (_vp, [_vp, C.c_uint32, _u64p, C.c_uint32, C.c_uint64]),
+            "or_rgsw_external_product": (C.c_int, [_vp, _vp, _u64p, _u64p, C.c_uint32]),
             "or_expands": (C.c_int, [_vp, C.POINTER(C.c_void_p), _u32p, C.c_uint32, C.c_uint32, _u64p, C.c_uint32,
                                      _u64p]),
             "or_ksk_new": (_vp, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]),
@@ -416,6 +418,24 @@ class Bfv:
         out = np.zeros(2 * L * self.degree, np.uint64)
         e = np.ascontiguousarray(e, dtype=np.int64)
         _chk(lib().or_bfv_encrypt_with(self.h, level, _p64(u64(pt)), _p64(u64(a)), _pi64(e), _p64(out)))
+        return out
+
+    def rgsw_encrypt(self, values, level=0, seed=31):
+        """SecretKey::try_encrypt into an RGSWCiphertext (rgsw_ciphertext.rs:97-117)
+        of SIMD-encoded values: its two KeySwitchingKeys."""
+        pt = self.encode(values, level=level, scaled=False)
+        ks = []
+        for which in (0, 1):
+            h = lib().or_rgsw_key_new(self.h, level, _p64(pt), which, seed + which)
+            ks.append(KeySwitchingKey(self, h, level, level))
+        return ks
+
+    def rgsw_product(self, rgsw, ct):
+        """Ciphertext x RGSWCiphertext (rgsw_ciphertext.rs:123-160)."""
+        ct = u64(ct)
+        out = np.zeros_like(ct)
+        L = len(self.level_moduli(rgsw[0].ct_level))
+        _chk(lib().or_rgsw_external_product(rgsw[0].h, rgsw[1].h, _p64(ct), _p64(out), ct.size // (2 * L * self.degree)))
         return out
 
     def encrypt_pk_with(self, pk, pt, u, e1, e2, level=0):

@@ -283,3 +283,28 @@ def test_mul_and_relin_2_matches_oracle(fh, oracle, n, sizes):
     for i in range(batch):
         g = got[i * 2 * len(sizes) * n:(i + 1) * 2 * len(sizes) * n]
         assert [int(x) for x in bfv.decrypt(g)] == [(a * b) % t for a, b in zip(vs[2 * i], vs[2 * i + 1])]
+
+
+@pytest.mark.parametrize("n,L,level,batch", [(16, 2, 0, 3), (4096, 3, 1, 2), (16384, 6, 0, 2), (32768, 3, 0, 1)])
+def test_rgsw_external_product_matches_oracle(fh, oracle, n, L, level, batch):
+    """Ciphertext x RGSWCiphertext (fhe/src/bfv/rgsw_ciphertext.rs:123-160)."""
+    print(f"rgsw n={n} L={L} level={level} batch={batch}", flush=True)
+    t = 65537 if n > 16 else 1153
+    bfv = _bfv(oracle, n, [60] * L if n > 16 else [62] * L, t=t, seed=9)
+    rng = random.Random(n)
+    v2 = [rng.randrange(t) for _ in range(n)]
+    rgsw = bfv.rgsw_encrypt(v2, level=level)
+    keys = [_gpu_key(fh, bfv, k, level, level)[0] for k in rgsw]
+    vs = [[rng.randrange(t) for _ in range(n)] for _ in range(batch)]
+    cts = np.concatenate([bfv.encrypt(v, level=level, seed=40 + i) for i, v in enumerate(vs)])
+    out = fh.empty(cts.size)
+    fh.rgsw_external_product(keys[0], keys[1], fh.to_device(cts), out, batch)
+    _sync()
+    got = fh.to_host(out)
+    assert np.array_equal(got, bfv.rgsw_product(rgsw, cts))
+    step = cts.size // batch
+    for i, v in enumerate(vs):
+        dec = [int(x) for x in bfv.decrypt(got[i * step:(i + 1) * step], level=level)]
+        assert dec == [(a * b) % t for a, b in zip(v, v2)]
+    with pytest.raises(fh.FheHipError):
+        fh.rgsw_external_product(keys[0], keys[1], out, out, batch)

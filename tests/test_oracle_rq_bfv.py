@@ -284,3 +284,17 @@ def test_public_key_encryption_decrypts(oracle):
     v = _slots(rng, n, t)
     ct = bfv.encrypt_pk_with(pk, bfv.encode(v), _small(rng, n, 2), _small(rng, n), _small(rng, n))
     assert [int(x) for x in bfv.decrypt(ct)] == v
+
+
+@pytest.mark.parametrize("L,level", [(2, 0), (3, 1)])
+def test_rgsw_external_product_decrypts(oracle, L, level):
+    """The reference's external_product test (rgsw_ciphertext.rs:176-210):
+    ct1 x RGSW(m2) decrypts like ct1 x ct2, to the slot products."""
+    n, t = 16, 1153
+    bfv = oracle.Bfv(n, t, oracle.generate_moduli([62] * L, n), sk_seed=14)
+    rng = random.Random(L)
+    v1, v2 = _slots(rng, n, t), _slots(rng, n, t)
+    ct1 = bfv.encrypt(v1, level=level, seed=5)
+    rgsw = bfv.rgsw_encrypt(v2, level=level)
+    prod = bfv.rgsw_product(rgsw, ct1)
+    assert [int(x) for x in bfv.decrypt(prod, level=level)] == [(a * b) % t for a, b in zip(v1, v2)]
