@@ -603,6 +603,24 @@ def codec_throughput(fhehip, torch, batch=512, reps=10):
         ms = e0.elapsed_time(e1) / reps
         out[name] = {"ms_per_batch": ms, "value": batch / (ms / 1e3), "unit": "ciphertexts/s"}
     out["round_trip_exact"] = bool(torch.equal(dec, vals))
+    # Rq wire format of the ciphertexts (both polynomials, NTT domain -> bytes and back)
+    per = fhehip.serialization_length(ctx)
+    wire = torch.empty(batch * 2 * per, dtype=torch.uint8, device=dev)
+    back = torch.empty_like(ct)
+    for name, fn in (("serialize", lambda: fhehip.serialize(ctx, ct, 2 * batch, wire, True, stream)),
+                     ("deserialize", lambda: fhehip.deserialize(ctx, wire, 2 * batch, back, True, stream))):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        out[name] = {"ms_per_batch": ms, "value": batch / (ms / 1e3), "unit": "ciphertexts/s",
+                     "wire_bytes_per_ciphertext": 2 * per}
+    out["wire_round_trip_exact"] = bool(torch.equal(back, ct))
     return out
 
 

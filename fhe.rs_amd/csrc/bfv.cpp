@@ -759,6 +759,59 @@ void Codec::decrypt(const u64* ct, u32 parts, u64* values, u32 num, bool simd, h
 }
 
 // ---------------------------------------------------------------------------
+// Modulus::serialization_length (zq/mod.rs:773-777) per limb: N bits(q_i - 1) / 8.
+static WireTable wire_table(const DeviceContext& c, size_t& per_poly) {
+    if (c.L > kWireMaxLimbs) throw Error(-2, "serialization supports at most 64 moduli");
+    WireTable t{};
+    t.L = c.L;
+    per_poly = 0;
+    for (u32 i = 0; i < c.L; i++) {
+        t.nbits[i] = (u32)(64 - __builtin_clzll(c.moduli[i] - 1));
+        t.offset[i] = (u32)per_poly;
+        per_poly += (size_t)t.nbits[i] * c.N / 8;
+    }
+    return t;
+}
+
+size_t serialization_length(const DeviceContext& c) {
+    size_t per = 0;
+    wire_table(c, per);
+    return per;
+}
+
+void serialize_polys(const DeviceContext& c, const u64* in, u32 num, bool ntt, uint8_t* out, hipStream_t s) {
+    check_batch(num);
+    size_t per = 0;
+    const WireTable t = wire_table(c, per);
+    if (!num) return;
+    const u64* src = in;
+    if (ntt) {
+        const size_t rows = (size_t)c.L * c.N;
+        u64* pb = scratch(Scratch::Wire).get(s, (size_t)num * rows * 8).as<u64>();
+        InttRowsArgs ia{};
+        ia.src[0] = in;
+        ia.src_inst_stride = rows;
+        ia.dst = pb;
+        ia.dst_inst_stride = rows;
+        ia.dst_part_stride = rows;
+        ia.inv_tw = c.inv_tw.as<ulonglong2>();
+        ia.mods = c.mod.as<ModInfo>();
+        intt_rows(c, ia, c.L, 1, num, s);
+        src = pb;
+    }
+    launch_pack(src, out, t, c.logn, (u32)per, num, s);
+}
+
+void deserialize_polys(const DeviceContext& c, const uint8_t* in, u32 num, bool ntt, u64* out, hipStream_t s) {
+    check_batch(num);
+    size_t per = 0;
+    const WireTable t = wire_table(c, per);
+    if (!num) return;
+    launch_unpack(in, out, t, c.logn, (u32)per, num, s);
+    if (ntt) launch_ntt_forward(c, RowMap{out, c.L, c.L, 0, 0}, num * c.L, false, s);
+}
+
+// ---------------------------------------------------------------------------
 void rgsw_external_product(const Ksk& k0, const Ksk& k1, const u64* ct, u64* out, u32 inst, hipStream_t s) {
     check_batch(inst);
     const DeviceContext& c = *k0.ct_ctx;

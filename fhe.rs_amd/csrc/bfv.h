@@ -37,7 +37,7 @@ class StreamScratch {
 // every handle shares the pool of each purpose per stream; purposes that are
 // live at the same time (a multiplication and the key switch it calls) use
 // different pools.
-enum class Scratch { Scale, RelinC2, Lift, SwitchDown, Mul, GaloisC2, EvalTmp, ModSwitch, Codec, Rgsw, Count };
+enum class Scratch { Scale, RelinC2, Lift, SwitchDown, Mul, GaloisC2, EvalTmp, ModSwitch, Codec, Rgsw, Wire, Count };
 StreamScratch& scratch(Scratch k);
 // Frees every pool's buffer of stream `s`.
 void release_scratch(hipStream_t s);
@@ -174,6 +174,13 @@ struct Codec {
     // ct [num][parts][L][N] -> values [num][N].
     void decrypt(const u64* ct, u32 parts, u64* values, u32 num, bool simd, hipStream_t s) const;
 };
+
+// Rq wire format (math/src/rq/convert.rs:15-87): bytes of one polynomial's
+// coefficients, and num polynomials [num][L][N] (NTT domain when ntt, which
+// are serialized in power basis) to / from their payloads back to back.
+size_t serialization_length(const DeviceContext& c);
+void serialize_polys(const DeviceContext& c, const u64* in, u32 num, bool ntt, uint8_t* out, hipStream_t s);
+void deserialize_polys(const DeviceContext& c, const uint8_t* in, u32 num, bool ntt, u64* out, hipStream_t s);
 
 // Ciphertext x RGSWCiphertext (fhe/src/bfv/rgsw_ciphertext.rs:123-160):
 // ct [inst][2][L][N] NTT -> out [inst][2][L][N] = key_switch(c0, k0) +

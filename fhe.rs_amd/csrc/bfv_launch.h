@@ -147,6 +147,16 @@ void launch_phase(const u64* ct, u32 parts, const u64* sk, u64* out, const Codec
 void launch_decode_reduce(const u64* d, u32 k, u64* w, const CodecConst& q0, const CodecConst& tc, int logn,
                           u32 num, hipStream_t s);
 void launch_decode_gather(const u64* rows, const u32* map, u64* out, int logn, u32 num, hipStream_t s);
+// Byte layout of one polynomial's Rq coefficients (wire_kernels.hip): limb i
+// starts at byte offset[i] and packs N values of nbits[i] bits.
+constexpr u32 kWireMaxLimbs = 64;
+struct WireTable {
+    u32 L;
+    u32 nbits[kWireMaxLimbs];
+    u32 offset[kWireMaxLimbs];
+};
+void launch_pack(const u64* in, uint8_t* out, const WireTable& t, int logn, u32 per_poly, u32 num, hipStream_t s);
+void launch_unpack(const uint8_t* in, u64* out, const WireTable& t, int logn, u32 per_poly, u32 num, hipStream_t s);
 // Power-basis switch-down part x [inst][2][L][N] of the finish (unfused path).
 void launch_sd_z(int logn, const KsFinishArgs& f, u64* x, u32 inst, hipStream_t s);
 // Copies `count` slots of `elems` words; `table` is a device array of

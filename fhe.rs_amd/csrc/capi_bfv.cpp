@@ -94,6 +94,29 @@ int fhehip_key_switch(const fhehip_ksk* k, const uint64_t* d_p, uint64_t* d_out,
     FHEHIP_API_END
 }
 
+int fhehip_serialization_length(const fhehip_context* ctx, uint64_t* bytes_per_poly) {
+    FHEHIP_API_BEGIN
+    if (!ctx || !bytes_per_poly) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *bytes_per_poly = serialization_length(*ctx->c);
+    FHEHIP_API_END
+}
+
+int fhehip_serialize(const fhehip_context* ctx, const uint64_t* d_polys, uint32_t num, int ntt, uint8_t* d_bytes,
+                     void* stream) {
+    FHEHIP_API_BEGIN
+    if (!ctx || ((!d_polys || !d_bytes) && num)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    serialize_polys(*ctx->c, d_polys, num, ntt != 0, d_bytes, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_deserialize(const fhehip_context* ctx, const uint8_t* d_bytes, uint32_t num, int ntt, uint64_t* d_polys,
+                       void* stream) {
+    FHEHIP_API_BEGIN
+    if (!ctx || ((!d_polys || !d_bytes) && num)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    deserialize_polys(*ctx->c, d_bytes, num, ntt != 0, d_polys, S(stream));
+    FHEHIP_API_END
+}
+
 int fhehip_rgsw_external_product(const fhehip_ksk* ksk0, const fhehip_ksk* ksk1, const uint64_t* d_ct,
                                  uint64_t* d_out, uint32_t num, void* stream) {
     FHEHIP_API_BEGIN

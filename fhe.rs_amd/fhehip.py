@@ -63,6 +63,9 @@ SIGNATURES = {
     "fhehip_inner_sum": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp]),
     "fhehip_rotate_and_sum": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_int, C.c_uint32, _vp]),
     "fhehip_ciphertext_add": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_serialization_length": (C.c_int, [_vp, _u64p]),
+    "fhehip_serialize": (C.c_int, [_vp, _vp, C.c_uint32, C.c_int, _vp, _vp]),
+    "fhehip_deserialize": (C.c_int, [_vp, _vp, C.c_uint32, C.c_int, _vp, _vp]),
     "fhehip_rgsw_external_product": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
     "fhehip_codec_create": (C.c_int, [_vp, _vp, C.c_uint64, _u64p, C.POINTER(C.c_int64), _vpp]),
     "fhehip_codec_destroy": (None, [_vp]),
@@ -334,6 +337,21 @@ class Switcher:
 
     def switch_down(self, d_in, d_out, num_polys, stream=None):
         check(lib().fhehip_switch_down(self.h, dptr(d_in), dptr(d_out), num_polys, stream_ptr(stream)))
+
+
+def serialization_length(ctx):
+    v = C.c_uint64()
+    check(lib().fhehip_serialization_length(ctx.h, C.byref(v)))
+    return v.value
+
+
+def serialize(ctx, d_polys, num, d_bytes, ntt=True, stream=None):
+    """Rq coefficient payloads of num polynomials (d_bytes: a uint8 device buffer)."""
+    check(lib().fhehip_serialize(ctx.h, dptr(d_polys), num, 1 if ntt else 0, dptr(d_bytes), stream_ptr(stream)))
+
+
+def deserialize(ctx, d_bytes, num, d_polys, ntt=True, stream=None):
+    check(lib().fhehip_deserialize(ctx.h, dptr(d_bytes), num, 1 if ntt else 0, dptr(d_polys), stream_ptr(stream)))
 
 
 def rgsw_external_product(ksk0, ksk1, d_ct, d_out, num, stream=None):

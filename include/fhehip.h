@@ -194,6 +194,24 @@ int fhehip_rotate_and_sum(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint6
 int fhehip_rgsw_external_product(const fhehip_ksk* ksk0, const fhehip_ksk* ksk1, const uint64_t* d_ct,
                                  uint64_t* d_out, uint32_t num, void* stream);
 
+/* ---- Wire format ------------------------------------------------------------------
+ * The coefficients field of the Rq protobuf message (math/src/rq/convert.rs:
+ * 15-87, rq.proto), i.e. the bulk of a serialized Ciphertext, KeySwitchingKey
+ * or Plaintext: each limb's power-basis coefficients packed least significant
+ * bit first with the bit width of q_i - 1 (transcode_to_bytes,
+ * fhe-util/src/lib.rs:60-97), limbs back to back. The message framing
+ * (representation, degree, seeds) stays with the caller's protobuf codec. */
+/* Bytes of one polynomial of the context (Modulus::serialization_length summed). */
+int fhehip_serialization_length(const fhehip_context* ctx, uint64_t* bytes_per_poly);
+/* num polynomials [num][L][N] (NTT domain when ntt != 0; they are serialized
+ * in power basis, as fhe.rs does) -> num payloads back to back in d_bytes. */
+int fhehip_serialize(const fhehip_context* ctx, const uint64_t* d_polys, uint32_t num, int ntt, uint8_t* d_bytes,
+                     void* stream);
+/* The inverse (Poly::try_convert_from(&Rq), convert.rs:42-120): payloads ->
+ * [num][L][N], transformed to the NTT domain when ntt != 0. */
+int fhehip_deserialize(const fhehip_context* ctx, const uint8_t* d_bytes, uint32_t num, int ntt, uint64_t* d_polys,
+                       void* stream);
+
 /* ---- Encoding, encryption, decryption ------------------------------------------
  * The steps on either side of the evaluation path, for the BfvParameters
  * level whose ciphertext context is `ctx`: `plain_ctx` is the parameters'

@@ -734,6 +734,78 @@ static inline std::vector<PolyData> expands(const BfvParams& par, const Context&
 }
 
 // ---------------------------------------------------------------------------
+// Wire format of Rq coefficients (math/src/rq/convert.rs:15-39, :42-87)
+// ---------------------------------------------------------------------------
+// transcode_to_bytes / transcode_from_bytes (fhe-util/src/lib.rs:60-133):
+// values of nbits bits packed least significant bit first.
+static inline std::vector<uint8_t> transcode_to_bytes(const u64* a, size_t n, size_t nbits) {
+    const u128 mask = (u128)(~0ULL >> (64 - nbits));
+    std::vector<uint8_t> out;
+    u128 cur = 0;
+    size_t have = 0, i = 0;
+    while (i < n) {
+        if (have < 8) {
+            cur |= ((u128)a[i] & mask) << have;
+            have += nbits;
+            i++;
+        }
+        while (have >= 8) {
+            out.push_back((uint8_t)cur);
+            cur >>= 8;
+            have -= 8;
+        }
+    }
+    if (have > 0) out.push_back((uint8_t)cur);
+    return out;
+}
+static inline std::vector<u64> transcode_from_bytes(const uint8_t* b, size_t nb, size_t nbits) {
+    const u128 mask = (u128)(~0ULL >> (64 - nbits));
+    std::vector<u64> out;
+    u128 cur = 0;
+    size_t have = 0, i = 0;
+    while (i < nb) {
+        if (have < nbits) {
+            cur |= (u128)b[i] << have;
+            have += 8;
+            i++;
+        }
+        while (have >= nbits) {
+            out.push_back((u64)(cur & mask));
+            cur >>= nbits;
+            have -= nbits;
+        }
+    }
+    if (have > 0) out.push_back((u64)cur);
+    return out;
+}
+// Modulus::serialization_length's bit width (zq/mod.rs:773-793): bits of p - 1.
+static inline size_t coeff_bits(u64 p) { return (size_t)(64 - lzcnt64(p - 1)); }
+
+// The Rq coefficients payload of a polynomial [L][N] (NTT polynomials are
+// serialized in power basis): each limb's coefficients packed with its width.
+static inline std::vector<uint8_t> serialize_poly(const Context& c, const u64* p, bool ntt) {
+    PolyData pb(p, p + c.L() * c.degree);
+    if (ntt) ntt_backward(c, pb.data());
+    std::vector<uint8_t> out;
+    for (size_t i = 0; i < c.L(); i++) {
+        std::vector<uint8_t> v = transcode_to_bytes(pb.data() + i * c.degree, c.degree, coeff_bits(c.moduli[i]));
+        out.insert(out.end(), v.begin(), v.end());
+    }
+    return out;
+}
+static inline PolyData deserialize_poly(const Context& c, const uint8_t* b, bool ntt) {
+    PolyData p;
+    for (size_t i = 0; i < c.L(); i++) {
+        const size_t nbits = coeff_bits(c.moduli[i]), size = nbits * c.degree / 8;
+        std::vector<u64> v = transcode_from_bytes(b, size, nbits);
+        p.insert(p.end(), v.begin(), v.end());
+        b += size;
+    }
+    if (ntt) ntt_forward(c, p.data());
+    return p;
+}
+
+// ---------------------------------------------------------------------------
 // RGSWCiphertext (fhe/src/bfv/rgsw_ciphertext.rs)
 // ---------------------------------------------------------------------------
 // SecretKey::try_encrypt into an RGSW ciphertext (rgsw_ciphertext.rs:97-117):

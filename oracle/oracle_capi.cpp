@@ -530,6 +530,27 @@ int or_expands(void* h, void* const* gks, const uint32_t* exps, uint32_t ngk, ui
         for (auto& x : r) out = std::copy(x.begin(), x.end(), out);
     })
 }
+// Rq coefficient payloads of `count` polynomials [L][N] of ctx, back to back.
+int or_poly_serialize(void* ctx, const uint64_t* polys, uint32_t count, uint32_t ntt, uint8_t* out) {
+    OR_TRY({
+        const Context& c = **(CtxPtr*)ctx;
+        for (uint32_t k = 0; k < count; k++) {
+            std::vector<uint8_t> v = serialize_poly(c, polys + (size_t)k * c.L() * c.degree, ntt != 0);
+            out = std::copy(v.begin(), v.end(), out);
+        }
+    })
+}
+int or_poly_deserialize(void* ctx, const uint8_t* bytes, uint32_t count, uint32_t ntt, uint64_t* out) {
+    OR_TRY({
+        const Context& c = **(CtxPtr*)ctx;
+        size_t per = 0;
+        for (u64 q : c.moduli) per += coeff_bits(q) * c.degree / 8;
+        for (uint32_t k = 0; k < count; k++) {
+            PolyData p = deserialize_poly(c, bytes + k * per, ntt != 0);
+            out = std::copy(p.begin(), p.end(), out);
+        }
+    })
+}
 // Seeded uniform residues for [count][L][N] in the given moduli.
 int or_random_polys(const uint64_t* moduli, uint32_t L, uint32_t n, uint32_t count, uint64_t seed, uint64_t* out) {
     OR_TRY({

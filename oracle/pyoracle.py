@@ -70,6 +70,8 @@ def lib():
             "or_bfv_encrypt_pk_with": (C.c_int, [_vp, C.c_uint32, _u64p, _u64p, _i64p, _i64p, _i64p, _u64p]),
             "or_rgsw_key_new": (_vp, [_vp, C.c_uint32, _u64p, C.c_uint32, C.c_uint64]),
             "or_rgsw_external_product": (C.c_int, [_vp, _vp, _u64p, _u64p, C.c_uint32]),
+            "or_poly_serialize": (C.c_int, [_vp, _u64p, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint8)]),
+            "or_poly_deserialize": (C.c_int, [_vp, C.POINTER(C.c_uint8), C.c_uint32, C.c_uint32, _u64p]),
             "or_expands": (C.c_int, [_vp, C.POINTER(C.c_void_p), _u32p, C.c_uint32, C.c_uint32, _u64p, C.c_uint32,
                                      _u64p]),
             "or_ksk_new": (_vp, [_vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64]),
@@ -349,6 +351,27 @@ def dot_product_scalar(ctx: Context, cts, pts, count, parts, ct_stride=None, pt_
     cts, pts = u64(cts), u64(pts)
     out = np.zeros(parts * L * n, np.uint64)
     _chk(lib().or_dot_product_scalar(ctx.h, _p64(cts), ct_stride, _p64(pts), pt_stride, count, parts, _p64(out)))
+    return out
+
+
+def serialization_length(moduli, n):
+    """Bytes of one polynomial's Rq coefficients (Modulus::serialization_length per limb)."""
+    return sum((int(q) - 1).bit_length() * n // 8 for q in moduli)
+
+
+def poly_serialize(ctx: Context, polys, count, ntt):
+    """Rq coefficient payloads (rq/convert.rs:15-39) of `count` polynomials, back to back."""
+    out = np.zeros(count * serialization_length(ctx.moduli, ctx.n), np.uint8)
+    _chk(lib().or_poly_serialize(ctx.h, _p64(u64(polys)), count, 1 if ntt else 0,
+                                 out.ctypes.data_as(C.POINTER(C.c_uint8))))
+    return out
+
+
+def poly_deserialize(ctx: Context, data, count, ntt):
+    data = np.ascontiguousarray(data, dtype=np.uint8)
+    out = np.zeros(count * len(ctx.moduli) * ctx.n, np.uint64)
+    _chk(lib().or_poly_deserialize(ctx.h, data.ctypes.data_as(C.POINTER(C.c_uint8)), count, 1 if ntt else 0,
+                                   _p64(out)))
     return out
 
 
