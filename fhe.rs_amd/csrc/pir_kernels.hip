@@ -20,14 +20,16 @@ namespace fhehip {
 // schedules it takes. Each thread loads the ciphertext words of a row once
 // for its CG columns; blocks that share a coefficient chunk get consecutive
 // logical indices on one XCD, so the other column groups read those words
-// from L2. 1-D grid of chunks * colgroups blocks of 256 threads.
+// from L2. 1-D grid of (64-word chunk, column group) blocks of 256 threads.
+// The plaintext (database) words are read once: non-temporal loads keep the
+// ciphertext rows, which every column group re-reads, in L2.
 template <int PARTS, int CG>
 __device__ __forceinline__ void dot_load(const u64* ct, const u64* pt, const size_t (&coff)[CG], u32 words,
                                          u64 (&x)[PARTS], u64 (&y)[CG]) {
 #pragma unroll
     for (int p = 0; p < PARTS; p++) x[p] = ct[(size_t)p * words];
 #pragma unroll
-    for (int c = 0; c < CG; c++) y[c] = pt[coff[c]];
+    for (int c = 0; c < CG; c++) y[c] = __builtin_nontemporal_load(pt + coff[c]);
 }
 
 template <int PARTS, int CG>
@@ -38,12 +40,22 @@ __device__ __forceinline__ void dot_mac(Acc3 (&acc)[CG][PARTS], const u64 (&x)[P
         for (int p = 0; p < PARTS; p++) acc3_mul<false>(acc[c][p], x[p], y[c]);
 }
 
+// Rows are split over the four waves of a workgroup (wave g takes rows g,
+// g + 4, ...), each lane one word of a 64-word chunk: four times the waves of
+// one thread per (word, column group), so the grid is several rounds of
+// resident waves instead of one and a fraction. Each wave reduces its sums
+// to residues; the four residues of an output are added through LDS.
+constexpr u32 kDotGroups = 4;
+
 template <int PARTS, int CG>
 __global__ __launch_bounds__(256) void dot_product_kernel(DotArgs a) {
+    __shared__ u64 part[kDotGroups][CG * PARTS][64];
     const u32 w = xcd_block(blockIdx.x, gridDim.x);
     const u32 chunk = w / a.colgroups, cg = w % a.colgroups;
-    const u32 k = chunk * 256 + threadIdx.x;
-    if (k >= a.words) return;
+    const u32 lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const u32 k = chunk * 64 + lane;
+    const bool live = k < a.words;
+    const u32 kk = live ? k : a.words - 1;   // idle lanes read a valid word and store nothing
     const u32 c0 = cg * CG;
     const u32 ncol = a.columns - c0 < (u32)CG ? a.columns - c0 : (u32)CG;
     // Columns past the last one re-read the last column (in bounds, no
@@ -56,27 +68,29 @@ __global__ __launch_bounds__(256) void dot_product_kernel(DotArgs a) {
     for (int c = 0; c < CG; c++)
 #pragma unroll
         for (int p = 0; p < PARTS; p++) acc3_init(acc[c][p], 0);
-    const u64* ct = a.ct + k;
-    const u64* pt = a.pt + (size_t)c0 * a.pt_col_stride + k;
-    u64 x[PARTS], y[CG];
-    dot_load<PARTS, CG>(ct, pt, coff, a.words, x, y);
-    // The next row's words are in flight while this row's products accumulate.
-    for (u32 r = 1; r < a.count; r++) {
-        ct += a.ct_stride;
-        pt += a.pt_row_stride;
-        u64 xn[PARTS], yn[CG];
-        dot_load<PARTS, CG>(ct, pt, coff, a.words, xn, yn);
+    if (g < a.count) {
+        const u64* ct = a.ct + (size_t)g * a.ct_stride + kk;
+        const u64* pt = a.pt + (size_t)g * a.pt_row_stride + (size_t)c0 * a.pt_col_stride + kk;
+        const size_t cstep = kDotGroups * a.ct_stride, pstep = kDotGroups * a.pt_row_stride;
+        u64 x[PARTS], y[CG];
+        dot_load<PARTS, CG>(ct, pt, coff, a.words, x, y);
+        // The next row's words are in flight while this row's products accumulate.
+        for (u32 r = g + kDotGroups; r < a.count; r += kDotGroups) {
+            ct += cstep;
+            pt += pstep;
+            u64 xn[PARTS], yn[CG];
+            dot_load<PARTS, CG>(ct, pt, coff, a.words, xn, yn);
+            dot_mac<PARTS, CG>(acc, x, y);
+#pragma unroll
+            for (int p = 0; p < PARTS; p++) x[p] = xn[p];
+#pragma unroll
+            for (int c = 0; c < CG; c++) y[c] = yn[c];
+        }
         dot_mac<PARTS, CG>(acc, x, y);
-#pragma unroll
-        for (int p = 0; p < PARTS; p++) x[p] = xn[p];
-#pragma unroll
-        for (int c = 0; c < CG; c++) y[c] = yn[c];
     }
-    dot_mac<PARTS, CG>(acc, x, y);
-    const ModInfo mi = a.mods[k >> a.logn];
+    const ModInfo mi = a.mods[kk >> a.logn];
 #pragma unroll
-    for (int c = 0; c < CG; c++) {
-        if ((u32)c >= ncol) break;
+    for (int c = 0; c < CG; c++)
 #pragma unroll
         for (int p = 0; p < PARTS; p++) {
             // lo + 2^64 hi + 2^128 top: (top 2^64 + hi) mod q, then (that 2^64 + lo) mod q;
@@ -84,9 +98,15 @@ __global__ __launch_bounds__(256) void dot_product_kernel(DotArgs a) {
             u64 lo, hi, top;
             acc3_value(acc[c][p], lo, hi, top);
             const u64 r1 = reduce_u128(hi, top, mi.barrett_lo, mi.barrett_hi, mi.p);
-            a.out[(size_t)(c0 + c) * a.out_col_stride + (size_t)p * a.words + k] =
-                reduce_u128(lo, r1, mi.barrett_lo, mi.barrett_hi, mi.p);
+            part[g][c * PARTS + p][lane] = reduce_u128(lo, r1, mi.barrett_lo, mi.barrett_hi, mi.p);
         }
+    __syncthreads();
+    for (u32 j = g; j < (u32)(CG * PARTS); j += kDotGroups) {
+        const u32 c = j / PARTS, p = j % PARTS;
+        u64 v = part[0][j][lane];
+#pragma unroll
+        for (u32 h = 1; h < kDotGroups; h++) v = add_mod(v, part[h][j][lane], mi.p);
+        if (live && c < ncol) a.out[(size_t)(c0 + c) * a.out_col_stride + (size_t)p * a.words + k] = v;
     }
 }
 
@@ -106,7 +126,7 @@ void launch_dot_product(DotArgs a, u32 parts, hipStream_t s) {
     // them; the accumulators take 9 VGPRs per (column, part).
     const int cg = a.columns >= 4 && parts <= 2 && FHEHIP_DOT_CG_MAX >= 4 ? 4 : (a.columns >= 2 ? 2 : 1);
     a.colgroups = (a.columns + cg - 1) / cg;
-    const u32 chunks = (a.words + 255) / 256;
+    const u32 chunks = (a.words + 63) / 64;
     const u64 blocks = (u64)chunks * a.colgroups;
     if (blocks > 0x7fffffffu) throw Error(-1, "dot product grid too large");
     ProfScope ps("dot_product", a.logn, s);
