@@ -445,7 +445,7 @@ def _random_rows(np, rng, ct_moduli, key_moduli, n):
     return c
 
 
-def pir_server(fhehip, torch, reps=20, check_cpu=True):
+def pir_server(fhehip, torch, reps=20, check_cpu=True, batches=(16, 64)):
     """MulPIR's server response (examples/mulpir.rs:151-185) on the examples'
     default database, a 58 x 57 matrix of N = 8192 plaintexts at level 1
     (433 MiB in HBM): expansion of the query into dim1 + dim2 ciphertexts,
@@ -453,7 +453,9 @@ def pir_server(fhehip, torch, reps=20, check_cpu=True):
     ct x ct products (no relinearization) summed, one relinearization and
     the switch to the last level. Inputs and keys are seeded uniform rows;
     with check_cpu the oracle recomputes the same response on one core (bit
-    comparison and CPU time)."""
+    comparison and CPU time). Then `batches` queries at once (expansions
+    query-minor, so every stage is one batch and one database pass serves
+    all of them); query 0's batched response is compared with the single one."""
     import numpy as np
     dev = torch.device("cuda:0")
     n, t = PIR_N, PIR_T
@@ -513,6 +515,49 @@ def pir_server(fhehip, torch, reps=20, check_cpu=True):
     fhehip.profiler_enable(False)
     dot_name = f"dot_product/N={n}"
     dot_ms = kern[dot_name][1] / kern[dot_name][0] if dot_name in kern else None
+    single = fhehip.to_host(resp).copy()
+
+    def batched(nq):
+        """The same response for nq queries (query 0 = the single query)."""
+        qs = torch.cat([query, _uniform_rows(torch, dev, m1, (nq - 1, 2, L, n), gen).reshape(-1)])
+        ex = torch.empty((d1 + d2) * nq * per, dtype=torch.int64, device=dev)
+        dt = torch.empty(d2 * nq * per, dtype=torch.int64, device=dev)
+        pr = torch.empty(d2 * nq * per3, dtype=torch.int64, device=dev)
+        rl = torch.empty(nq * per, dtype=torch.int64, device=dev)
+        rs = torch.empty(nq * 2 * n, dtype=torch.int64, device=dev)
+
+        def run():
+            ek.expand_batch(qs, nq, d1 + d2, ex, stream)
+            fhehip.dot_product_columns(ctx1, ex, d1, 2 * nq, db, d2, dt, stream)
+            mul.multiply(dt, ex[d1 * nq * per:], pr, d2 * nq, stream)
+            cnt = d2
+            while cnt > 1:
+                half = cnt // 2
+                fhehip.ciphertext_add(ctx1, pr, pr[(cnt - half) * nq * per3:], pr, half * nq * 3, stream)
+                cnt -= half
+            rk.relinearize(pr, rl, nq, stream)
+            sw.switch_down(rl, rs, 2 * nq, stream)
+
+        run()
+        torch.cuda.synchronize()
+        b0, b1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        k = max(2, reps // 4)
+        b0.record(stream)
+        for _ in range(k):
+            run()
+        b1.record(stream)
+        torch.cuda.synchronize()
+        bms = b0.elapsed_time(b1) / k
+        fhehip.profiler_enable(True)
+        run()
+        torch.cuda.synchronize()
+        kb = fhehip.profiler_report()
+        fhehip.profiler_enable(False)
+        d_ms = kb[dot_name][1] / kb[dot_name][0] if dot_name in kb else None
+        d_bytes = (d1 * d2 * L * n + (d1 + d2) * nq * per) * 8
+        return {"queries": nq, "ms_per_batch": bms, "value": nq * 1e3 / bms, "unit": "responses/s",
+                "dot_product_ms": d_ms, "dot_product_GBps": d_bytes / d_ms / 1e6 if d_ms else None,
+                "query0_equals_single": bool(np.array_equal(fhehip.to_host(rs)[:2 * n], single))}
     dot_bytes = (d1 * d2 * L * n + d1 * per + d2 * per) * 8
     res = {"workload": f"MulPIR server response (examples/mulpir.rs:151-185): database of {PIR_ELEMENTS} x "
                        f"{PIR_ELEMENT_BYTES}-byte elements as {d1} x {d2} plaintexts, N={n}, moduli "
@@ -523,7 +568,8 @@ def pir_server(fhehip, torch, reps=20, check_cpu=True):
            "kernels_ms": {k: {"launches": c, "avg_ms": v / c} for k, (c, v) in sorted(kern.items())},
            "dot_product_roofline": {"bound": "hbm", "algorithmic_bytes": dot_bytes, "avg_launch_ms": dot_ms,
                                     "achieved_GBps": dot_bytes / dot_ms / 1e6 if dot_ms else None,
-                                    "frac": dot_bytes / dot_ms / 1e6 / HBM_PEAK_GBS if dot_ms else None}}
+                                    "frac": dot_bytes / dot_ms / 1e6 / HBM_PEAK_GBS if dot_ms else None},
+           "batched": [batched(nq) for nq in batches]}
     if check_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle as oracle

@@ -610,21 +610,27 @@ void EvaluationKey::prepare_expansion() {
 // EvaluationKey::expands (evaluation_key.rs:153-193): level l relinearizes
 // out[0 .. 2^l) with the key of exponent (N >> l) + 1 as one batch, then one
 // elementwise pass forms out[2^l + i] and updates out[i].
-void EvaluationKey::expand(const u64* ct, u32 size, u64* out, hipStream_t s) const {
+// With outputs stored query-minor ([j][query]), the ciphertexts 0 .. 2^l of
+// every query are contiguous, and the level step of query q, output i is
+// flat index i num + q of a step of 2^l num.
+void EvaluationKey::expand(const u64* ct, u32 size, u32 num, u64* out, hipStream_t s) const {
     if (keys.empty()) throw Error(-1, "empty evaluation key");
     u32 level = 0;
     while (((u64)1 << level) < size) level++;
     if (level > expansion_levels) throw Error(-1, "This key does not support expansion at this level");
+    if (!num) return;
+    if (((u64)num << (level ? level - 1 : 0)) > 65535 || (u64)num * std::max<u32>(size, 1) > 0xffffffffu)
+        throw Error(-1, "too many queries for this expansion size");
     const DeviceContext& c = *keys[0]->ksk->ct_ctx;
     const size_t sz = (size_t)2 * c.L * c.N;
-    HIPCHK(hipMemcpyAsync(out, ct, sz * 8, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(out, ct, (size_t)num * sz * 8, hipMemcpyDeviceToDevice, s));
     if (!level) return;
-    u64* sub = scratch(Scratch::EvalTmp).get(s, ((size_t)1 << (level - 1)) * sz * 8).as<u64>();
+    u64* sub = scratch(Scratch::EvalTmp).get(s, ((size_t)num << (level - 1)) * sz * 8).as<u64>();
     for (u32 l = 0; l < level; l++) {
-        const u32 step = 1u << l;
+        const u32 step = num << l;
         find((degree >> l) + 1)->apply(out, sub, step, s);
         launch_expand_step(out, sub, monomials.as<ulonglong2>() + (size_t)l * c.L * c.N, c.mod.as<ModInfo>(), c.L,
-                           c.logn, step, size, s);
+                           c.logn, step, size * num, s);
     }
 }
 

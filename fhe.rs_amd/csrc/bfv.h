@@ -137,8 +137,9 @@ struct EvaluationKey {
     DevBuf monomials;
     // Finds the expansion levels and builds their monomials (synchronous).
     void prepare_expansion();
-    // EvaluationKey::expands: ct [2][L][N] -> out [max(size, 1)][2][L][N].
-    void expand(const u64* ct, u32 size, u64* out, hipStream_t s) const;
+    // EvaluationKey::expands of num queries: ct [num][2][L][N] -> out
+    // [max(size, 1)][num][2][L][N] (output j of every query together).
+    void expand(const u64* ct, u32 size, u32 num, u64* out, hipStream_t s) const;
     std::shared_ptr<Galois> find(u32 exponent) const;
     // 3^i mod 2N (evaluation_key.rs:195-203)
     u32 column_exponent(u32 i) const;

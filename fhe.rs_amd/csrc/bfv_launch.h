@@ -122,6 +122,7 @@ struct DotArgs {
     size_t out_col_stride;
     const ModInfo* mods;
     u32 count, columns, colgroups, words, logn;
+    u32 parts, partgroups;  // parts of a row; groups of the kernel's PARTS parts (set by the launcher)
 };
 void launch_dot_product(DotArgs a, u32 parts, hipStream_t s);
 // One level of the oblivious expansion (EvaluationKey::expands); `mono` is

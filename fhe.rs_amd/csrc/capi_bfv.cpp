@@ -315,7 +315,16 @@ int fhehip_expand(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint32_t size
     FHEHIP_API_BEGIN
     if (!ek || !d_ct || !d_out) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
     if (d_ct == d_out) return fail(FHEHIP_ERR_INVALID_ARG, "out must not alias ct");
-    ek->ek->expand(d_ct, size, d_out, S(stream));
+    ek->ek->expand(d_ct, size, 1, d_out, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_expand_batch(fhehip_evaluation_key* ek, const uint64_t* d_cts, uint32_t num, uint32_t size,
+                        uint64_t* d_out, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!ek || ((!d_cts || !d_out) && num)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    if (d_cts == d_out && num) return fail(FHEHIP_ERR_INVALID_ARG, "out must not alias ct");
+    ek->ek->expand(d_cts, size, num, d_out, S(stream));
     FHEHIP_API_END
 }
 

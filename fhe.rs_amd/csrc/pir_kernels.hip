@@ -24,10 +24,10 @@ namespace fhehip {
 // The plaintext (database) words are read once: non-temporal loads keep the
 // ciphertext rows, which every column group re-reads, in L2.
 template <int PARTS, int CG>
-__device__ __forceinline__ void dot_load(const u64* ct, const u64* pt, const size_t (&coff)[CG], u32 words,
-                                         u64 (&x)[PARTS], u64 (&y)[CG]) {
+__device__ __forceinline__ void dot_load(const u64* ct, const u64* pt, const size_t (&poff)[PARTS],
+                                         const size_t (&coff)[CG], u64 (&x)[PARTS], u64 (&y)[CG]) {
 #pragma unroll
-    for (int p = 0; p < PARTS; p++) x[p] = ct[(size_t)p * words];
+    for (int p = 0; p < PARTS; p++) x[p] = ct[poff[p]];
 #pragma unroll
     for (int c = 0; c < CG; c++) y[c] = __builtin_nontemporal_load(pt + coff[c]);
 }
@@ -50,8 +50,12 @@ constexpr u32 kDotGroups = 4;
 template <int PARTS, int CG>
 __global__ __launch_bounds__(256) void dot_product_kernel(DotArgs a) {
     __shared__ u64 part[kDotGroups][CG * PARTS][64];
+    // Consecutive logical blocks (one XCD) share a coefficient chunk: part
+    // groups vary fastest (they share the database words), then column groups
+    // (they share the ciphertext words).
     const u32 w = xcd_block(blockIdx.x, gridDim.x);
-    const u32 chunk = w / a.colgroups, cg = w % a.colgroups;
+    const u32 pg = w % a.partgroups, rest = w / a.partgroups;
+    const u32 chunk = rest / a.colgroups, cg = rest % a.colgroups;
     const u32 lane = threadIdx.x & 63, g = threadIdx.x >> 6;
     const u32 k = chunk * 64 + lane;
     const bool live = k < a.words;
@@ -63,6 +67,12 @@ __global__ __launch_bounds__(256) void dot_product_kernel(DotArgs a) {
     size_t coff[CG];
 #pragma unroll
     for (int c = 0; c < CG; c++) coff[c] = (size_t)((u32)c < ncol ? (u32)c : ncol - 1) * a.pt_col_stride;
+    // Likewise for the parts past the last one of a part group.
+    const u32 p0 = pg * PARTS;
+    const u32 npart = a.parts - p0 < (u32)PARTS ? a.parts - p0 : (u32)PARTS;
+    size_t poff[PARTS];
+#pragma unroll
+    for (int p = 0; p < PARTS; p++) poff[p] = (size_t)(p0 + ((u32)p < npart ? (u32)p : npart - 1)) * a.words;
     Acc3 acc[CG][PARTS];
 #pragma unroll
     for (int c = 0; c < CG; c++)
@@ -73,13 +83,13 @@ __global__ __launch_bounds__(256) void dot_product_kernel(DotArgs a) {
         const u64* pt = a.pt + (size_t)g * a.pt_row_stride + (size_t)c0 * a.pt_col_stride + kk;
         const size_t cstep = kDotGroups * a.ct_stride, pstep = kDotGroups * a.pt_row_stride;
         u64 x[PARTS], y[CG];
-        dot_load<PARTS, CG>(ct, pt, coff, a.words, x, y);
+        dot_load<PARTS, CG>(ct, pt, poff, coff, x, y);
         // The next row's words are in flight while this row's products accumulate.
         for (u32 r = g + kDotGroups; r < a.count; r += kDotGroups) {
             ct += cstep;
             pt += pstep;
             u64 xn[PARTS], yn[CG];
-            dot_load<PARTS, CG>(ct, pt, coff, a.words, xn, yn);
+            dot_load<PARTS, CG>(ct, pt, poff, coff, xn, yn);
             dot_mac<PARTS, CG>(acc, x, y);
 #pragma unroll
             for (int p = 0; p < PARTS; p++) x[p] = xn[p];
@@ -106,37 +116,45 @@ __global__ __launch_bounds__(256) void dot_product_kernel(DotArgs a) {
         u64 v = part[0][j][lane];
 #pragma unroll
         for (u32 h = 1; h < kDotGroups; h++) v = add_mod(v, part[h][j][lane], mi.p);
-        if (live && c < ncol) a.out[(size_t)(c0 + c) * a.out_col_stride + (size_t)p * a.words + k] = v;
+        if (live && c < ncol && p < npart)
+            a.out[(size_t)(c0 + c) * a.out_col_stride + (size_t)(p0 + p) * a.words + k] = v;
     }
 }
 
-template <int PARTS>
-static void launch_dot_parts(const DotArgs& a, int cg, u32 blocks, hipStream_t s) {
-    if (cg == 4)
-        hipLaunchKernelGGL((dot_product_kernel<PARTS, 4>), dim3(blocks), dim3(256), 0, s, a);
-    else if (cg == 2)
-        hipLaunchKernelGGL((dot_product_kernel<PARTS, 2>), dim3(blocks), dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL((dot_product_kernel<PARTS, 1>), dim3(blocks), dim3(256), 0, s, a);
+template <int PARTS, int CG>
+static void launch_dot(const DotArgs& a, u32 blocks, hipStream_t s) {
+    hipLaunchKernelGGL((dot_product_kernel<PARTS, CG>), dim3(blocks), dim3(256), 0, s, a);
 }
 
 void launch_dot_product(DotArgs a, u32 parts, hipStream_t s) {
-    if (!a.count || !a.columns || !a.words) return;
-    // Columns per thread: the ciphertext words of a row are loaded once for
-    // them; the accumulators take 9 VGPRs per (column, part).
-    const int cg = a.columns >= 4 && parts <= 2 && FHEHIP_DOT_CG_MAX >= 4 ? 4 : (a.columns >= 2 ? 2 : 1);
+    if (!a.count || !a.columns || !a.words || !parts) return;
+    // Parts and columns per thread: the ciphertext words of a row are loaded
+    // once for CG columns and the database words once for PARTS parts; the
+    // accumulators take 9 VGPRs per (column, part), at most 8 of them.
+    int pt, cg;
+    if (parts <= 2) {
+        pt = (int)parts;
+        cg = a.columns >= 4 && FHEHIP_DOT_CG_MAX >= 4 ? 4 : (a.columns >= 2 ? 2 : 1);
+    } else {
+        pt = parts == 3 ? 3 : 4;
+        cg = a.columns >= 2 ? 2 : 1;
+    }
+    a.parts = parts;
+    a.partgroups = (parts + pt - 1) / pt;
     a.colgroups = (a.columns + cg - 1) / cg;
     const u32 chunks = (a.words + 63) / 64;
-    const u64 blocks = (u64)chunks * a.colgroups;
+    const u64 blocks = (u64)chunks * a.colgroups * a.partgroups;
     if (blocks > 0x7fffffffu) throw Error(-1, "dot product grid too large");
     ProfScope ps("dot_product", a.logn, s);
-    switch (parts) {
-        case 1: launch_dot_parts<1>(a, cg, (u32)blocks, s); break;
-        case 2: launch_dot_parts<2>(a, cg, (u32)blocks, s); break;
-        case 3: launch_dot_parts<3>(a, cg, (u32)blocks, s); break;
-        case 4: launch_dot_parts<4>(a, cg, (u32)blocks, s); break;
-        default: throw Error(-2, "dot products of ciphertexts with more than 4 parts are not supported");
-    }
+    const u32 nb = (u32)blocks;
+    if (pt == 1)
+        cg == 4 ? launch_dot<1, 4>(a, nb, s) : cg == 2 ? launch_dot<1, 2>(a, nb, s) : launch_dot<1, 1>(a, nb, s);
+    else if (pt == 2)
+        cg == 4 ? launch_dot<2, 4>(a, nb, s) : cg == 2 ? launch_dot<2, 2>(a, nb, s) : launch_dot<2, 1>(a, nb, s);
+    else if (pt == 3)
+        cg == 2 ? launch_dot<3, 2>(a, nb, s) : launch_dot<3, 1>(a, nb, s);
+    else
+        cg == 2 ? launch_dot<4, 2>(a, nb, s) : launch_dot<4, 1>(a, nb, s);
     HIPCHK(hipGetLastError());
 }
 

@@ -75,6 +75,7 @@ SIGNATURES = {
     "fhehip_decrypt": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint32, C.c_int, _vp, _vp]),
     "fhehip_expand": (C.c_int, [_vp, _vp, C.c_uint32, _vp, _vp]),
     "fhehip_expansion_levels": (C.c_int, [_vp, _u32p]),
+    "fhehip_expand_batch": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint32, _vp, _vp]),
     "fhehip_dot_product_scalar": (C.c_int, [_vp, _vp, C.c_uint64, _vp, C.c_uint64, C.c_uint32, C.c_uint32, _vp,
                                             _vp]),
     "fhehip_dot_product_columns": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint32, _vp, C.c_uint32, _vp, _vp]),
@@ -446,6 +447,10 @@ class EvaluationKey:
     def expand(self, d_ct, size, d_out, stream=None):
         """EvaluationKey::expands: d_out [max(size, 1)][2][L][N]."""
         check(lib().fhehip_expand(self.h, dptr(d_ct), size, dptr(d_out), stream_ptr(stream)))
+
+    def expand_batch(self, d_cts, num, size, d_out, stream=None):
+        """fhehip_expand_batch: d_out [max(size, 1)][num][2][L][N]."""
+        check(lib().fhehip_expand_batch(self.h, dptr(d_cts), num, size, dptr(d_out), stream_ptr(stream)))
 
     def expansion_levels(self):
         v = C.c_uint32()

@@ -259,6 +259,14 @@ int fhehip_decrypt(const fhehip_codec* c, const uint64_t* d_ct, uint32_t parts, 
  * ciphertext (the input). FHEHIP_ERR_INVALID_ARG when a needed key is missing
  * ("This key does not support expansion at this level") or out aliases ct. */
 int fhehip_expand(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint32_t size, uint64_t* d_out, void* stream);
+/* fhehip_expand of `num` queries d_cts [num][2][L][N] at once, into d_out
+ * [max(size, 1)][num][2][L][N] (output j of every query together), so that
+ * each level relinearizes 2^l num ciphertexts as one batch and row j of the
+ * expanded queries is one [num x 2]-part operand of fhehip_dot_product_columns
+ * (a database pass serves all num queries). No fhe.rs counterpart: fhe.rs
+ * expands one query per call. */
+int fhehip_expand_batch(fhehip_evaluation_key* ek, const uint64_t* d_cts, uint32_t num, uint32_t size,
+                        uint64_t* d_out, void* stream);
 /* EvaluationKey::supports_expansion (evaluation_key.rs:134-147): the largest
  * level (ceil(log2(size))) fhehip_expand accepts with this key. */
 int fhehip_expansion_levels(const fhehip_evaluation_key* ek, uint32_t* levels);
@@ -267,8 +275,7 @@ int fhehip_expansion_levels(const fhehip_evaluation_key* ek, uint32_t* levels);
  * domain, at d_ct + i * ct_stride) times plaintext i (its NTT-domain
  * polynomial Plaintext::poly_ntt [L][N] at the ciphertexts' level, at d_pt + i
  * * pt_stride); strides in 64-bit words. Exact, fully reduced, for any count.
- * count 0 -> FHEHIP_ERR_INVALID_ARG ("At least one iterator is empty"); at
- * most 4 parts (FHEHIP_ERR_UNSUPPORTED above). */
+ * count 0 -> FHEHIP_ERR_INVALID_ARG ("At least one iterator is empty"). */
 int fhehip_dot_product_scalar(const fhehip_context* ctx, const uint64_t* d_ct, uint64_t ct_stride,
                               const uint64_t* d_pt, uint64_t pt_stride, uint32_t count, uint32_t parts,
                               uint64_t* d_out, void* stream);
