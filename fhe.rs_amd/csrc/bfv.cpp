@@ -381,33 +381,41 @@ std::unique_ptr<Multiplicator> Multiplicator::make_default(std::shared_ptr<Ksk> 
 Switcher::Switcher(std::shared_ptr<DeviceContext> f, std::shared_ptr<DeviceContext> t)
     : from(std::move(f)), to(std::move(t)) {
     if (from->N != to->N) throw Error(-4, "Incompatible degrees");
-    if (from->L < 2) throw Error(-1, "a single-modulus context cannot be switched down");
-    if (to->L != from->L - 1 || !std::equal(to->moduli.begin(), to->moduli.end(), from->moduli.begin()))
-        throw Error(-4, "the target context must be the source context without its last modulus");
-    // switch_down constants for dropping q_l (rq/mod.rs:390-449)
-    const u64 ql = from->moduli.back();
-    std::vector<SdConst> sd(to->L);
-    for (u32 m = 0; m < to->L; m++) {
-        const u64 q = to->moduli[m];
-        SdConst& c = sd[m];
-        c.q = q;
-        c.bl = to->h_mod[m].barrett_lo;
-        c.bh = to->h_mod[m].barrett_hi;
-        c.half_mod = q - ((ql >> 1) % q);
-        c.inv = invmod64(ql % q, q);
-        c.inv_shoup = shoup64(c.inv, q);
+    if (to->L > from->L || !std::equal(to->moduli.begin(), to->moduli.end(), from->moduli.begin()))
+        throw Error(-4, "the target context must be the source context without its last moduli");
+    // switch_down constants of each step (rq/mod.rs:390-449): step s drops
+    // q_l = modulus L_from - 1 - s from the limbs below it
+    const u32 L = from->L, k = L - to->L;
+    if (!k) return;
+    std::vector<SdConst> sd((size_t)k * (L - 1));
+    for (u32 st = 0; st < k; st++) {
+        const u64 ql = from->moduli[L - 1 - st];
+        for (u32 m = 0; m + 1 + st < L; m++) {
+            const u64 q = from->moduli[m];
+            SdConst& c = sd[(size_t)st * (L - 1) + m];
+            c.q = q;
+            c.bl = from->h_mod[m].barrett_lo;
+            c.bh = from->h_mod[m].barrett_hi;
+            c.half_mod = q - ((ql >> 1) % q);
+            c.inv = invmod64(ql % q, q);
+            c.inv_shoup = shoup64(c.inv, q);
+        }
     }
     consts.reset(sd.size() * sizeof(SdConst));
     HIPCHK(hipMemcpy(consts.ptr, sd.data(), consts.bytes, hipMemcpyHostToDevice));
 }
 
-// Ciphertext::switch_down (ciphertext.rs:86-97) as fhe.rs runs it: every
-// limb to power basis, Poly::switch_down, the remaining limbs back to the
-// NTT domain.
+// Ciphertext::switch_down, once per dropped modulus (ciphertext.rs:86-119):
+// every limb to power basis, the steps, the remaining limbs back to the NTT
+// domain. Steps before the last run in place on the power-basis rows.
 void Switcher::switch_down(const u64* in, u64* out, u32 polys, hipStream_t s) const {
     check_batch(polys);
     if (!polys) return;
-    const size_t n = from->N, L = from->L;
+    const size_t n = from->N, L = from->L, k = L - to->L;
+    if (!k) {
+        HIPCHK(hipMemcpyAsync(out, in, (size_t)polys * L * n * 8, hipMemcpyDeviceToDevice, s));
+        return;
+    }
     u64* pb = scratch(Scratch::ModSwitch).get(s, (size_t)polys * L * n * 8).as<u64>();
     InttRowsArgs ia{};
     ia.src[0] = in;
@@ -418,8 +426,13 @@ void Switcher::switch_down(const u64* in, u64* out, u32 polys, hipStream_t s) co
     ia.inv_tw = from->inv_tw.as<ulonglong2>();
     ia.mods = from->mod.as<ModInfo>();
     intt_rows(*from, ia, (u32)L, 1, polys, s);
-    launch_switch_down_pb(pb, out, consts.as<SdConst>(), from->moduli.back(), (u32)L, from->logn, polys, s);
-    launch_ntt_forward(*to, RowMap{out, (u32)(L - 1), (u32)(L - 1), 0, 0}, polys * (u32)(L - 1), false, s);
+    for (size_t st = 0; st < k; st++) {
+        const u32 cur = (u32)(L - st);
+        const bool last = st + 1 == k;
+        launch_switch_down_pb(pb, last ? out : pb, consts.as<SdConst>() + st * (L - 1), from->moduli[cur - 1], cur,
+                              (u32)L, last ? (u32)to->L : (u32)L, from->logn, polys, s);
+    }
+    launch_ntt_forward(*to, RowMap{out, to->L, to->L, 0, 0}, polys * to->L, false, s);
 }
 
 // Multiplicator::multiply (fhe/src/bfv/ops/mul.rs:158-227) on a batch.

@@ -91,14 +91,17 @@ struct Ksk {
 };
 
 // Multiplicator (fhe/src/bfv/ops/mul.rs:22-227).
-// Ciphertext::switch_down (fhe/src/bfv/ciphertext.rs:86-97): NTT-domain
-// polynomials of `from` rounded into `to`, the first L - 1 moduli of `from`
-// (Poly::switch_down, math/src/rq/mod.rs:390-449), back in the NTT domain.
+// Ciphertext::switch_down / switch_to_level (fhe/src/bfv/ciphertext.rs:86-119):
+// NTT-domain polynomials of `from` rounded into `to`, the first L_to moduli
+// of `from`, by one Poly::switch_down (math/src/rq/mod.rs:390-449) per
+// dropped modulus. The NTT round trips between fhe.rs's steps are exact, so
+// one inverse transform, the steps in power basis and one forward transform
+// give the same result.
 struct Switcher {
     std::shared_ptr<DeviceContext> from, to;
-    DevBuf consts;  // SdConst of the dropped modulus for each remaining one
+    DevBuf consts;  // [dropped][L_from - 1] SdConst: step s drops modulus L_from - 1 - s
     Switcher(std::shared_ptr<DeviceContext> from, std::shared_ptr<DeviceContext> to);
-    // [polys][L][N] -> [polys][L - 1][N]; out must not alias in.
+    // [polys][L_from][N] -> [polys][L_to][N]; out must not alias in.
     void switch_down(const u64* in, u64* out, u32 polys, hipStream_t s) const;
 };
 

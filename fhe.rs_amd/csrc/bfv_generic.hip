@@ -103,14 +103,16 @@ void launch_mac_ew(const u64* y, u32 digits, u32 key_limbs, u32 first, u32 count
 
 // Poly::switch_down (math/src/rq/mod.rs:390-449) in power basis: with
 // y = (x_l + floor(q_l / 2)) mod q_l, x_i <- (x_i - y + floor(q_l / 2)) q_l^-1
-// mod q_i for each remaining limb i. grid = (N / 256, L - 1, polys)
-__global__ __launch_bounds__(256) void switch_down_pb_kernel(const u64* __restrict__ in, u64* __restrict__ out,
-                                                             const SdConst* __restrict__ consts, u64 ql, u32 L,
-                                                             u32 logn) {
+// mod q_i for each remaining limb i < L - 1. Polynomial b reads limbs at
+// in + b in_stride rows and writes at out + b out_stride rows; out may be in
+// (each thread rewrites only the limb it read, and limb L - 1 is not
+// written). grid = (N / 256, L - 1, polys)
+__global__ __launch_bounds__(256) void switch_down_pb_kernel(const u64* in, u64* out, const SdConst* __restrict__ consts,
+                                                             u64 ql, u32 L, u32 in_stride, u32 out_stride, u32 logn) {
     const u32 e = blockIdx.x * 256 + threadIdx.x;
     if (e >> logn) return;
     const u32 i = blockIdx.y, b = blockIdx.z;
-    const u64* src = in + ((size_t)b * L << logn);
+    const u64* src = in + ((size_t)b * in_stride << logn);
     u64 y = src[((size_t)(L - 1) << logn) + e] + (ql >> 1);
     y = y >= ql ? y - ql : y;
     const SdConst cst = consts[i];
@@ -118,16 +120,16 @@ __global__ __launch_bounds__(256) void switch_down_pb_kernel(const u64* __restri
     // [y]_i only has to be < 2 q_i and congruent to y (ks_finish_kernel)
     const u64 yr = ql <= 2 * q ? y : lazy_reduce_u64(y, cst.bl, cst.bh, q);
     const u64 tmp = yr + cst.half_mod;
-    out[(((size_t)b * (L - 1) + i) << logn) + e] =
+    out[(((size_t)b * out_stride + i) << logn) + e] =
         mul_shoup_u(src[((size_t)i << logn) + e] + 3 * q - tmp, cst.inv, cst.inv_shoup, q);
 }
 
-void launch_switch_down_pb(const u64* in, u64* out, const SdConst* consts, u64 ql, u32 L, int logn, u32 polys,
-                           hipStream_t s) {
+void launch_switch_down_pb(const u64* in, u64* out, const SdConst* consts, u64 ql, u32 L, u32 in_stride,
+                           u32 out_stride, int logn, u32 polys, hipStream_t s) {
     if (L < 2 || !polys) return;
     ProfScope ps("switch_down", logn, s);
     hipLaunchKernelGGL(switch_down_pb_kernel, ew_grid(1u << logn, L - 1, polys), dim3(256), 0, s, in, out, consts,
-                       ql, L, (u32)logn);
+                       ql, L, in_stride, out_stride, (u32)logn);
     HIPCHK(hipGetLastError());
 }
 

@@ -106,9 +106,10 @@ void launch_mac_ew(const u64* y, u32 digits, u32 key_limbs, u32 first, u32 count
 // `special`, still in the NTT domain.
 void launch_ks_mac(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s);
 // Poly::switch_down (math/src/rq/mod.rs:390-449) of power-basis polynomials
-// [polys][L][N] into [polys][L - 1][N] (power basis), dropping modulus q_l.
-void launch_switch_down_pb(const u64* in, u64* out, const SdConst* consts, u64 ql, u32 L, int logn, u32 polys,
-                           hipStream_t s);
+// of L limbs (in_stride limbs apart) into L - 1 limbs (out_stride apart),
+// dropping limb L - 1 (modulus q_l); out may equal in.
+void launch_switch_down_pb(const u64* in, u64* out, const SdConst* consts, u64 ql, u32 L, u32 in_stride,
+                           u32 out_stride, int logn, u32 polys, hipStream_t s);
 // Dot products of ciphertexts with NTT-domain plaintext polynomials
 // (pir_kernels.hip). Row r of column c: ciphertext at ct + r * ct_stride
 // ([parts][L][N], part stride `words` = L N), plaintext at pt + r *

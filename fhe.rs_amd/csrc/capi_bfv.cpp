@@ -170,6 +170,8 @@ int fhehip_multiplicator_enable_mod_switching(fhehip_multiplicator* m, const fhe
     if (!m || !next) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
     if (m->m->base->L < 2)
         return fail(FHEHIP_ERR_INVALID_ARG, "Cannot modulo switch as this is already the last level");
+    if (next->c->L + 1 != m->m->base->L)
+        return fail(FHEHIP_ERR_CTX_MISMATCH, "the next context must drop exactly the last modulus");
     m->m->mod_switch.reset(new Switcher(m->m->base, next->c));
     FHEHIP_API_END
 }

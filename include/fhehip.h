@@ -288,14 +288,15 @@ int fhehip_dot_product_columns(const fhehip_context* ctx, const uint64_t* d_ct, 
                                const uint64_t* d_db, uint32_t columns, uint64_t* d_out, void* stream);
 /* ---- Leveled operations ------------------------------------------------------
  * Ciphertext::switch_down (fhe/src/bfv/ciphertext.rs:86-97) and
- * Ciphertext::switch_to_level (:99-119, by repeated calls): the NTT-domain
- * polynomials of `from` are taken to power basis, rounded by the last
- * modulus (Poly::switch_down, math/src/rq/mod.rs:390-449) and transformed
- * back in `to`, which must be `from` without its last modulus
- * (FHEHIP_ERR_CTX_MISMATCH otherwise). */
+ * Ciphertext::switch_to_level (:99-119): the NTT-domain polynomials of `from`
+ * are taken to power basis, rounded by each dropped modulus in turn, last
+ * first (Poly::switch_down, math/src/rq/mod.rs:390-449), and transformed back
+ * in `to`, which must hold the first moduli of `from`
+ * (FHEHIP_ERR_CTX_MISMATCH otherwise): `from` without its last modulus is
+ * switch_down, a shorter prefix switch_to_level, `from` itself a copy. */
 int fhehip_switcher_create(const fhehip_context* from, const fhehip_context* to, fhehip_switcher** out);
 void fhehip_switcher_destroy(fhehip_switcher* s);
-/* [num_polys][L][N] -> [num_polys][L-1][N]; out must not alias in. */
+/* [num_polys][L_from][N] -> [num_polys][L_to][N]; out must not alias in. */
 int fhehip_switch_down(const fhehip_switcher* s, const uint64_t* d_in, uint64_t* d_out, uint32_t num_polys,
                        void* stream);
 /* Ciphertext / polynomial addition (fhe/src/bfv/ops/mod.rs:15-69, Poly

@@ -124,9 +124,7 @@ def test_leveled_errors(fh, oracle):
     with pytest.raises(fh.FheHipError):
         fh.Switcher(a, fh.Context(m[1:], n))          # not a prefix
     with pytest.raises(fh.FheHipError):
-        fh.Switcher(a, fh.Context(m[:2], n))          # two moduli down
-    with pytest.raises(fh.FheHipError):
-        fh.Switcher(fh.Context(m[:1], n), fh.Context(m[:1], n))
+        fh.Switcher(b, a)                              # upwards
     sw = fh.Switcher(a, b)
     buf = fh.empty(2 * 4 * n)
     with pytest.raises(fh.FheHipError):
@@ -137,6 +135,8 @@ def test_leveled_errors(fh, oracle):
     gm = fh.Multiplicator(rk=fh.KeySwitchingKey(b, a, c0, c1), plaintext=T)
     with pytest.raises(fh.FheHipError):
         gm.enable_mod_switching(fh.Context(m[1:3], n))  # not the next level
+    with pytest.raises(fh.FheHipError):
+        gm.enable_mod_switching(fh.Context(m[:1], n))   # two levels down
     gm.enable_mod_switching(fh.Context(m[:2], n))
     with pytest.raises(fh.FheHipError):
         fh.submit([(gm, None, buf, buf, buf, [])])
@@ -172,3 +172,31 @@ def test_multiply_graph_with_mod_switching(fh, oracle):
             _sync()
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("n,sizes,level,target,cts", [(1024, [60] * 5, 0, 3, 2), (16, [62] * 4, 1, 3, 1),
+                                                     (16384, [60] * 7, 0, 6, 1), (32768, [60] * 4, 0, 2, 1),
+                                                     (64, [62] * 3, 1, 1, 2)])
+def test_switch_to_level_matches_oracle(fh, oracle, n, sizes, level, target, cts):
+    """Ciphertext::switch_to_level (ciphertext.rs:99-119): one inverse NTT,
+    the switch-down steps and one forward NTT equal repeated switch_down;
+    the ciphertexts decrypt at the target level (a target equal to the
+    level is a copy)."""
+    print(f"switch_to_level n={n} sizes={sizes} {level}->{target}", flush=True)
+    t = T if n > 16 else 1153
+    bfv = oracle.Bfv(n, t, oracle.generate_moduli(sizes, n), sk_seed=5)
+    vals = [[i + 1, 2, 3] for i in range(cts)]
+    src = np.concatenate([bfv.encrypt(v, level=level, seed=i) for i, v in enumerate(vals)])
+    want = src
+    for lv in range(level, target):
+        want = oracle.ct_switch_down(oracle.Context(bfv.level_moduli(lv), n),
+                                     oracle.Context(bfv.level_moduli(lv + 1), n), want)
+    sw = fh.Switcher(fh.Context(bfv.level_moduli(level), n), fh.Context(bfv.level_moduli(target), n))
+    step = 2 * len(bfv.level_moduli(target)) * n
+    out = fh.empty(cts * step)
+    sw.switch_down(fh.to_device(src), out, 2 * cts)
+    _sync()
+    got = fh.to_host(out)
+    assert np.array_equal(got, want)
+    for i, v in enumerate(vals):
+        assert [int(x) for x in bfv.decrypt(got[i * step:(i + 1) * step], level=target)][:3] == v
