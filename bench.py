@@ -451,11 +451,11 @@ def pir_server(fhehip, torch, reps=20, check_cpu=True, batches=(16, 64)):
     (433 MiB in HBM): expansion of the query into dim1 + dim2 ciphertexts,
     the dim2 column dot products over the database in one pass, the dim2
     ct x ct products (no relinearization) summed, one relinearization and
-    the switch to the last level. Inputs and keys are seeded uniform rows;
-    with check_cpu the oracle recomputes the same response on one core (bit
-    comparison and CPU time). Then `batches` queries at once (expansions
-    query-minor, so every stage is one batch and one database pass serves
-    all of them); query 0's batched response is compared with the single one."""
+    the switch to the last level. Inputs and keys are seeded uniform rows.
+    Then `batches` queries at once (expansions query-minor, so every stage is
+    one batch and one database pass serves all of them); query 0's batched
+    response is compared with the single one. Returns the result and, with
+    check_cpu, the host copies pir_cpu_baseline recomputes the response from."""
     import numpy as np
     dev = torch.device("cuda:0")
     n, t = PIR_N, PIR_T
@@ -570,38 +570,11 @@ def pir_server(fhehip, torch, reps=20, check_cpu=True, batches=(16, 64)):
                                     "achieved_GBps": dot_bytes / dot_ms / 1e6 if dot_ms else None,
                                     "frac": dot_bytes / dot_ms / 1e6 / HBM_PEAK_GBS if dot_ms else None},
            "batched": [batched(nq) for nq in batches]}
-    if check_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import pyoracle as oracle
-        db_h, q_h, got = fhehip.to_host(db), fhehip.to_host(query), fhehip.to_host(resp)
-        bfv = oracle.Bfv(n, t, moduli, sk_seed=1)
-        okeys = []
-        for e, c0, c1 in gk_rows:
-            k = bfv.key_from_rows(c0, c1, ct_level=1, key_level=0)
-            k.exponent = e
-            okeys.append(k)
-        ork = bfv.key_from_rows(*rk_rows, ct_level=1, key_level=1)
-        o1, o2 = oracle.Context(m1, n), oracle.Context(m2, n)
-        om = oracle.Multiplicator(bfv, None, level=1)
-        qv = np.array(m1, dtype=np.uint64).repeat(n)
-        t0 = time.perf_counter()
-        ex = bfv.expands(okeys, q_h, d1 + d2, level=1).reshape(d1 + d2, per)
-        acc = None
-        for j in range(d2):
-            dj = oracle.dot_product_scalar(o1, ex[:d1].reshape(-1), db_h[j * L * n:], d1, 2, pt_stride=d2 * L * n)
-            pj = om.multiply(dj, ex[d1 + j]).reshape(3, L * n)
-            if acc is None:
-                acc = pj
-            else:
-                acc = acc + pj
-                acc = np.where(acc >= qv, acc - qv, acc)
-        want = oracle.ct_switch_down(o1, o2, bfv.relinearize(ork, acc.reshape(-1)))
-        cpu_s = time.perf_counter() - t0
-        res["bit_exact_vs_oracle"] = bool(np.array_equal(got, want))
-        res["cpu_reference"] = {"seconds_per_response": cpu_s, "cores": 1, "kind": "port",
-                                "sample": "one response by the oracle (C++ restatement, -O3, one thread)"}
-        res["gpu_over_cpu"] = cpu_s * 1e3 / percentile(ms, 50)
-    return res
+    # host copies for the CPU baseline leg (pir_cpu_baseline)
+    handover = {"moduli": moduli, "n": n, "t": t, "d1": d1, "d2": d2, "gk_rows": gk_rows, "rk_rows": rk_rows,
+                "db": fhehip.to_host(db), "query": fhehip.to_host(query), "resp": fhehip.to_host(resp),
+                "gpu_ms": percentile(ms, 50)} if check_cpu else None
+    return res, handover
 
 
 def codec_throughput(fhehip, torch, batch=512, reps=10):
@@ -670,6 +643,47 @@ def codec_throughput(fhehip, torch, batch=512, reps=10):
     return out
 
 
+def pir_cpu_baseline(h):
+    """CPU baseline of the PIR leg: the oracle (C++ restatement of the fhe.rs
+    CPU path) recomputes the single-query response on one core, for a bit
+    comparison with the GPU's and its time."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import pyoracle as oracle
+    moduli, n, t, d1, d2 = h["moduli"], h["n"], h["t"], h["d1"], h["d2"]
+    m1, m2 = moduli[:2], moduli[:1]
+    L = len(m1)
+    per = 2 * L * n
+    bfv = oracle.Bfv(n, t, moduli, sk_seed=1)
+    okeys = []
+    for e, c0, c1 in h["gk_rows"]:
+        k = bfv.key_from_rows(c0, c1, ct_level=1, key_level=0)
+        k.exponent = e
+        okeys.append(k)
+    ork = bfv.key_from_rows(*h["rk_rows"], ct_level=1, key_level=1)
+    o1, o2 = oracle.Context(m1, n), oracle.Context(m2, n)
+    om = oracle.Multiplicator(bfv, None, level=1)
+    qv = np.array(m1, dtype=np.uint64).repeat(n)
+    db_h = h["db"]
+    t0 = time.perf_counter()
+    ex = bfv.expands(okeys, h["query"], d1 + d2, level=1).reshape(d1 + d2, per)
+    acc = None
+    for j in range(d2):
+        dj = oracle.dot_product_scalar(o1, ex[:d1].reshape(-1), db_h[j * L * n:], d1, 2, pt_stride=d2 * L * n)
+        pj = om.multiply(dj, ex[d1 + j]).reshape(3, L * n)
+        if acc is None:
+            acc = pj
+        else:
+            acc = acc + pj
+            acc = np.where(acc >= qv, acc - qv, acc)
+    want = oracle.ct_switch_down(o1, o2, bfv.relinearize(ork, acc.reshape(-1)))
+    cpu_s = time.perf_counter() - t0
+    return {"bit_exact_vs_oracle": bool(np.array_equal(h["resp"], want)),
+            "cpu_reference": {"seconds_per_response": cpu_s, "cores": 1, "kind": "port",
+                              "sample": "one response by the oracle (C++ restatement, -O3, one thread)"},
+            "gpu_over_cpu": cpu_s * 1e3 / h["gpu_ms"]}
+
+
 def cpu_baseline(lhs_h, rhs_h, c0, c1, gpu_out0, budget_s):
     """Oracle (C++ restatement of the fhe.rs CPU path) on the host cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -733,7 +747,10 @@ def main():
         print(json.dumps(codec_throughput(fhehip, torch)))
         return
     if args.only == "pir":
-        print(json.dumps(pir_server(fhehip, torch, check_cpu=not args.no_cpu)))
+        res, handover = pir_server(fhehip, torch, check_cpu=not args.no_cpu)
+        if handover is not None:
+            res.update(pir_cpu_baseline(handover))
+        print(json.dumps(res))
         return
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     dev = torch.device("cuda:0")
@@ -881,12 +898,13 @@ def main():
                                   "traffic": load_traffic(kname.split("/")[0]), "valu_busy_pct": vb}
 
     extras = {}
+    pir_handover = None
     if not args.no_extras:
         extras["single_instance"] = single_instance_latency(fhehip, torch, 200)
         extras["ntt_microbench"] = ntt_microbench(fhehip, torch, 50)
         extras["uneven_batch"] = uneven_batch(fhehip, torch)
         extras["serving"] = serving(fhehip, torch)
-        extras["pir_server"] = pir_server(fhehip, torch, check_cpu=not args.no_cpu)
+        extras["pir_server"], pir_handover = pir_server(fhehip, torch, check_cpu=not args.no_cpu)
         extras["codec"] = codec_throughput(fhehip, torch)
     if args.copy_calibration:
         src = torch.empty(args.copy_calibration << 17, dtype=torch.int64, device=dev)
@@ -897,6 +915,8 @@ def main():
 
     cpu = None
     bit_exact = None
+    if not args.no_cpu and not args.no_extras and pir_handover is not None:
+        extras["pir_server"].update(pir_cpu_baseline(pir_handover))
     if not args.no_cpu:
         out0 = fhehip.to_host(out[:per]).copy()
         cpu, bit_exact = cpu_baseline(fhehip.to_host(lhs[:per]).copy(), fhehip.to_host(rhs[:per]).copy(),
