@@ -141,3 +141,36 @@ def test_public_key_encryption_matches_oracle(fh, oracle, n, sizes, level):
     got = fh.to_host(ct)
     assert np.array_equal(got, bfv.encrypt_pk_with(pk, pt, u, e1, e2, level=level))
     assert [int(x) for x in bfv.decrypt(got, level=level)] == v
+
+
+def test_codec_explicit_root_modulo_t(fh, oracle):
+    """psi_t (the root of the parameters' NttOperator modulo t) is honoured:
+    another primitive 2N-th root changes the slot layout of the encoding,
+    and encryption and decryption under it still return the values."""
+    n, t = 1024, 65537
+    bfv = oracle.Bfv(n, t, oracle.generate_moduli([62, 62, 62], n), sk_seed=8)
+    L = 3
+    psi = fh.Context([t], n).psi()[0]
+    psi3 = pow(psi, 3, t)   # an odd power of a primitive 2N-th root is one
+    sk = bfv.secret()
+    plain = fh.Context(bfv.moduli[:2], n)
+    ctx = fh.Context(bfv.moduli, n)
+    base, other = fh.Codec(ctx, plain, t, sk=sk), fh.Codec(ctx, plain, t, sk=sk, psi_t=psi3)
+    same = fh.Codec(ctx, plain, t, sk=sk, psi_t=psi)
+    rng = np.random.default_rng(1)
+    v = rng.integers(0, t, n).astype(np.uint64)
+    dv = fh.to_device(v)
+    pts = [fh.empty(L * n) for _ in range(3)]
+    for c, out in zip((base, other, same), pts):
+        c.encode(dv, 1, out)
+    _sync()
+    assert np.array_equal(fh.to_host(pts[0]), fh.to_host(pts[2]))
+    assert not np.array_equal(fh.to_host(pts[0]), fh.to_host(pts[1]))
+    a = fh.to_device(oracle.random_polys(bfv.moduli, n, 1, seed=3))
+    e = fh.to_device(np.zeros(n, np.int64).view(np.uint64))
+    ct = fh.empty(2 * L * n)
+    other.encrypt(pts[1], a, e, ct, 1)
+    dec = fh.empty(n)
+    other.decrypt(ct, 2, 1, dec)
+    _sync()
+    assert np.array_equal(fh.to_host(dec), v)
