@@ -136,6 +136,12 @@ void launch_expand_step(u64* out, const u64* sub, const ulonglong2* mono, const 
 struct CodecConst {
     u64 q, bl, bh, delta, delta_shoup;
 };
+// out = a - b, or -a with neg (a, b, out may alias), over rows of modulus r % L.
+void launch_rows_sub_neg(const u64* a, const u64* b, u64* out, const ModInfo* mods, u32 L, int logn, size_t rows,
+                         bool neg, hipStream_t s);
+// out [num][parts][L][N] = ct times the plaintext row block at pt + b pt_stride (words).
+void launch_mul_plain(const u64* ct, u32 parts, const u64* pt, size_t pt_stride, u64* out, const ModInfo* mods, u32 L,
+                      int logn, u32 num, hipStream_t s);
 void launch_encode_gather(const u64* v, const u32* map, u64* rows, int logn, u32 num, hipStream_t s);
 void launch_pt_lift(const u64* m, u64* out, const CodecConst* cc, u64 t, u64 qmt, u64 qmt_shoup, u32 L, int logn,
                     u32 num, bool scaled, hipStream_t s);

@@ -393,6 +393,36 @@ int fhehip_ciphertext_add(const fhehip_context* ctx, const uint64_t* d_a, const 
     FHEHIP_API_END
 }
 
+int fhehip_ciphertext_sub(const fhehip_context* ctx, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out,
+                          uint32_t num_polys, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!ctx || ((!d_a || !d_b || !d_out) && num_polys)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    const auto& c = *ctx->c;
+    launch_rows_sub_neg(d_a, d_b, d_out, c.mod.as<ModInfo>(), c.L, c.logn, (size_t)num_polys * c.L, false,
+                        S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_ciphertext_neg(const fhehip_context* ctx, const uint64_t* d_a, uint64_t* d_out, uint32_t num_polys,
+                          void* stream) {
+    FHEHIP_API_BEGIN
+    if (!ctx || ((!d_a || !d_out) && num_polys)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    const auto& c = *ctx->c;
+    launch_rows_sub_neg(d_a, d_a, d_out, c.mod.as<ModInfo>(), c.L, c.logn, (size_t)num_polys * c.L, true, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_ciphertext_mul_plain(const fhehip_context* ctx, const uint64_t* d_ct, uint32_t parts, uint32_t num,
+                                const uint64_t* d_pt, uint64_t pt_stride, uint64_t* d_out, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!ctx || ((!d_ct || !d_pt || !d_out) && num && parts)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    if (num > 65535) return fail(FHEHIP_ERR_INVALID_ARG, "at most 65535 ciphertexts per call");
+    const auto& c = *ctx->c;
+    if ((size_t)parts * c.L > 65535) return fail(FHEHIP_ERR_INVALID_ARG, "too many parts");
+    launch_mul_plain(d_ct, parts, d_pt, pt_stride, d_out, c.mod.as<ModInfo>(), c.L, c.logn, num, S(stream));
+    FHEHIP_API_END
+}
+
 int fhehip_multiply_graph(fhehip_multiplicator* m, const uint64_t* d_lhs, const uint64_t* d_rhs, uint64_t* d_out,
                           uint32_t num, fhehip_graph** out) {
     FHEHIP_API_BEGIN

@@ -144,6 +144,51 @@ __global__ __launch_bounds__(256) void decode_gather_kernel(const u64* __restric
     out[b + i] = rows[b + map[i]];
 }
 
+// Sub / Neg for &Ciphertext (fhe/src/bfv/ops/mod.rs:110-227 over Poly
+// SubAssign / Neg, math/src/rq/ops.rs): out = a - b (op 0) or -a (op 1),
+// elementwise over rows where row r has modulus r % L.
+__global__ __launch_bounds__(256) void rows_sub_neg_kernel(const u64* a, const u64* b, u64* out,
+                                                           const ModInfo* __restrict__ mods, u32 L, u32 logn,
+                                                           size_t total, u32 op) {
+    const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= total) return;
+    const u64 p = mods[(i >> logn) % L].p;
+    out[i] = op == 0 ? sub_mod(a[i], b[i], p) : (a[i] ? p - a[i] : 0);
+}
+
+// MulAssign<&Plaintext> for Ciphertext (ops/mod.rs:229-258): every part of
+// ciphertext b times the NTT-domain plaintext polynomial at pt + b pt_stride.
+// grid = (N/256, parts L, num)
+__global__ __launch_bounds__(256) void mul_plain_kernel(const u64* ct, u32 parts, const u64* __restrict__ pt,
+                                                        size_t pt_stride, u64* out, const ModInfo* __restrict__ mods,
+                                                        u32 L, u32 logn) {
+    const u32 e = blockIdx.x * 256 + threadIdx.x;
+    if (e >> logn) return;
+    const u32 i = blockIdx.y % L, b = blockIdx.z;
+    const ModInfo mi = mods[i];
+    const size_t w = (((size_t)b * parts * L + blockIdx.y) << logn) + e;
+    out[w] = mul_mod(ct[w], pt[(size_t)b * pt_stride + ((size_t)i << logn) + e], mi.barrett_lo, mi.barrett_hi, mi.p);
+}
+
+void launch_rows_sub_neg(const u64* a, const u64* b, u64* out, const ModInfo* mods, u32 L, int logn, size_t rows,
+                         bool neg, hipStream_t s) {
+    const size_t total = rows << logn;
+    if (!total) return;
+    ProfScope ps(neg ? "neg_rows" : "sub_rows", logn, s);
+    hipLaunchKernelGGL(rows_sub_neg_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a, b, out, mods,
+                       L, (u32)logn, total, neg ? 1u : 0u);
+    HIPCHK(hipGetLastError());
+}
+
+void launch_mul_plain(const u64* ct, u32 parts, const u64* pt, size_t pt_stride, u64* out, const ModInfo* mods, u32 L,
+                      int logn, u32 num, hipStream_t s) {
+    if (!num || !parts) return;
+    ProfScope ps("mul_plain", logn, s);
+    hipLaunchKernelGGL(mul_plain_kernel, cgrid(logn, parts * L, num), dim3(256), 0, s, ct, parts, pt, pt_stride, out,
+                       mods, L, (u32)logn);
+    HIPCHK(hipGetLastError());
+}
+
 void launch_encode_gather(const u64* v, const u32* map, u64* rows, int logn, u32 num, hipStream_t s) {
     if (!num) return;
     ProfScope ps("encode_gather", logn, s);

@@ -63,6 +63,9 @@ SIGNATURES = {
     "fhehip_inner_sum": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp]),
     "fhehip_rotate_and_sum": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_int, C.c_uint32, _vp]),
     "fhehip_ciphertext_add": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_ciphertext_sub": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_ciphertext_neg": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_ciphertext_mul_plain": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint32, _vp, C.c_uint64, _vp, _vp]),
     "fhehip_serialization_length": (C.c_int, [_vp, _u64p]),
     "fhehip_serialize": (C.c_int, [_vp, _vp, C.c_uint32, C.c_int, _vp, _vp]),
     "fhehip_deserialize": (C.c_int, [_vp, _vp, C.c_uint32, C.c_int, _vp, _vp]),
@@ -465,6 +468,19 @@ def ntt_set_shape(degree, fwd, inv):
 
 def ciphertext_add(ctx, d_a, d_b, d_out, num_polys, stream=None):
     check(lib().fhehip_ciphertext_add(ctx.h, dptr(d_a), dptr(d_b), dptr(d_out), num_polys, stream_ptr(stream)))
+
+
+def ciphertext_sub(ctx, d_a, d_b, d_out, num_polys, stream=None):
+    check(lib().fhehip_ciphertext_sub(ctx.h, dptr(d_a), dptr(d_b), dptr(d_out), num_polys, stream_ptr(stream)))
+
+
+def ciphertext_neg(ctx, d_a, d_out, num_polys, stream=None):
+    check(lib().fhehip_ciphertext_neg(ctx.h, dptr(d_a), dptr(d_out), num_polys, stream_ptr(stream)))
+
+
+def ciphertext_mul_plain(ctx, d_ct, parts, num, d_pt, pt_stride, d_out, stream=None):
+    check(lib().fhehip_ciphertext_mul_plain(ctx.h, dptr(d_ct), parts, num, dptr(d_pt), pt_stride, dptr(d_out),
+                                            stream_ptr(stream)))
 
 
 def dot_product_scalar(ctx, d_ct, ct_stride, d_pt, pt_stride, count, parts, d_out, stream=None):

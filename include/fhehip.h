@@ -304,6 +304,20 @@ int fhehip_switch_down(const fhehip_switcher* s, const uint64_t* d_in, uint64_t*
  * context: out = a + b (out may alias a or b). */
 int fhehip_ciphertext_add(const fhehip_context* ctx, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out,
                           uint32_t num_polys, void* stream);
+/* Sub for &Ciphertext (ops/mod.rs:110-164): out = a - b (out may alias a or
+ * b). Adding or subtracting a plaintext (:71-107, :166-203) is this or
+ * fhehip_ciphertext_add on part 0 with its fhehip_encode(scaled) polynomial. */
+int fhehip_ciphertext_sub(const fhehip_context* ctx, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_out,
+                          uint32_t num_polys, void* stream);
+/* Neg for &Ciphertext (ops/mod.rs:205-227): out = -a (out may alias a). */
+int fhehip_ciphertext_neg(const fhehip_context* ctx, const uint64_t* d_a, uint64_t* d_out, uint32_t num_polys,
+                          void* stream);
+/* Mul<&Plaintext> for &Ciphertext (ops/mod.rs:229-258): each of the `parts`
+ * polynomials of ciphertext i (d_ct [num][parts][L][N], NTT domain) times the
+ * plaintext's poly_ntt at d_pt + i * pt_stride (words; 0: one plaintext for
+ * all) into d_out [num][parts][L][N] (may alias d_ct). */
+int fhehip_ciphertext_mul_plain(const fhehip_context* ctx, const uint64_t* d_ct, uint32_t parts, uint32_t num,
+                                const uint64_t* d_pt, uint64_t pt_stride, uint64_t* d_out, void* stream);
 
 /* ---- HIP graphs ---------------------------------------------------------------
  * No fhe.rs counterpart: a fixed sequence of launches captured once and

@@ -308,3 +308,48 @@ def test_rgsw_external_product_matches_oracle(fh, oracle, n, L, level, batch):
         assert dec == [(a * b) % t for a, b in zip(v, v2)]
     with pytest.raises(fh.FheHipError):
         fh.rgsw_external_product(keys[0], keys[1], out, out, batch)
+
+
+@pytest.mark.parametrize("n,L", [(16, 2), (4096, 3), (32768, 4)])
+def test_ciphertext_sub_neg_mul_plain(fh, oracle, n, L):
+    """Sub, Neg and Mul<&Plaintext> for &Ciphertext (fhe/src/bfv/ops/mod.rs:
+    110-258) against the oracle's modular operations, limb by limb, and
+    ct x poly_ntt(v2) decrypting to the slot products."""
+    print(f"ciphertext sub/neg/mul_plain n={n} L={L}", flush=True)
+    t = 65537 if n > 16 else 1153
+    bfv = _bfv(oracle, n, [60] * L if n > 16 else [62] * L, t=t, seed=11)
+    moduli = bfv.level_moduli(0)
+    ctx = fh.Context(moduli, n)
+    num, parts = 3, 2
+    a = oracle.random_polys(moduli, n, num * parts, seed=1)
+    b = oracle.random_polys(moduli, n, num * parts, seed=2)
+    a[:n] = 0   # neg of zero stays zero
+    pts = oracle.random_polys(moduli, n, num, seed=3)
+
+    def limbwise(op, x, y=None, rows=num * parts * L):
+        out = np.empty_like(x)
+        for r in range(rows):
+            q = moduli[r % L]
+            sl = slice(r * n, (r + 1) * n)
+            out[sl] = oracle.modulus_op(q, op, x[sl], None if y is None else y[sl])
+        return out
+
+    out = fh.empty(a.size)
+    fh.ciphertext_sub(ctx, fh.to_device(a), fh.to_device(b), out, num * parts)
+    _sync()
+    assert np.array_equal(fh.to_host(out), limbwise("sub", a, b))
+    fh.ciphertext_neg(ctx, fh.to_device(a), out, num * parts)
+    _sync()
+    assert np.array_equal(fh.to_host(out), limbwise("neg", a))
+    for stride in (L * n, 0):
+        fh.ciphertext_mul_plain(ctx, fh.to_device(a), parts, num, fh.to_device(pts), stride, out)
+        _sync()
+        per_ct = [pts[(i if stride else 0) * L * n:((i if stride else 0) + 1) * L * n] for i in range(num)]
+        want = limbwise("mul", a, np.concatenate([np.tile(p, parts) for p in per_ct]))
+        assert np.array_equal(fh.to_host(out), want), stride
+    rng = random.Random(n)
+    v1, v2 = [rng.randrange(t) for _ in range(n)], [rng.randrange(t) for _ in range(n)]
+    ct = fh.to_device(bfv.encrypt(v1, seed=4))
+    fh.ciphertext_mul_plain(ctx, ct, 2, 1, fh.to_device(bfv.encode(v2, scaled=False)), 0, ct)   # in place
+    _sync()
+    assert [int(x) for x in bfv.decrypt(fh.to_host(ct))] == [(x * y) % t for x, y in zip(v1, v2)]
