@@ -3,8 +3,12 @@
  *
  * A drop-in for the hot path of fhe.rs (crates fhe-math and fhe): negacyclic
  * NTTs, RNS scaling / base extension, key switching, ciphertext multiplication
- * with relinearization, and Galois rotations. Every entry point is extern "C",
- * takes plain pointers and sizes, never aborts, and returns a status code
+ * with relinearization, and Galois rotations; around it modulus switching,
+ * the PIR server path (oblivious expansion, plaintext dot products), the
+ * RGSW external product, ciphertext arithmetic, encoding / encryption /
+ * decryption and the wire format's coefficient payloads. Every entry point
+ * is extern "C", takes plain pointers and sizes, never aborts, and returns a
+ * status code
  * (FHEHIP_OK == 0). All outputs are exact residues, identical to those of the
  * fhe.rs CPU path for the same inputs and NTT roots.
  *
