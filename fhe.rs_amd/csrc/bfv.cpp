@@ -778,6 +778,38 @@ void Codec::decrypt(const u64* ct, u32 parts, u64* values, u32 num, bool simd, h
 }
 
 // ---------------------------------------------------------------------------
+void pir_respond(const EvaluationKey& ek, Multiplicator& mul, Ksk& rk, const Switcher& sw, const u64* db, u32 dim1,
+                 u32 dim2, const u64* queries, u32 num, u64* responses, hipStream_t s) {
+    if (ek.keys.empty()) throw Error(-1, "empty evaluation key");
+    const DeviceContext& c = *ek.keys[0]->ksk->ct_ctx;
+    if (mul.base->moduli != c.moduli || rk.ct_ctx->moduli != c.moduli || sw.from->moduli != c.moduli)
+        throw Error(-4, "the expansion key, multiplicator, relinearization key and switcher must share the query context");
+    if (mul.rk || mul.mod_switch) throw Error(-1, "the multiplicator must not relinearize or switch");
+    if (!dim1 || !dim2) throw Error(-1, "empty database");
+    check_batch(num);
+    if ((u64)dim2 * num > 65535) throw Error(-1, "at most 65535 products (dim2 x queries) per call");
+    if (!num) return;
+    const size_t per = (size_t)2 * c.L * c.N, per3 = (size_t)3 * c.L * c.N, size = (size_t)dim1 + dim2;
+    const size_t ex_w = size * num * per, dt_w = (size_t)dim2 * num * per, pr_w = (size_t)dim2 * num * per3;
+    u64* ex = scratch(Scratch::Pir).get(s, (ex_w + dt_w + pr_w + (size_t)num * per) * 8).as<u64>();
+    u64* dots = ex + ex_w;
+    u64* prods = dots + dt_w;
+    u64* relin = prods + pr_w;
+    ek.expand(queries, (u32)size, num, ex, s);
+    dot_product_columns(c, ex, dim1, 2 * num, db, dim2, dots, s);
+    mul.multiply(dots, ex + (size_t)dim1 * num * per, prods, dim2 * num, s);
+    // out += dot_j * ct_j (mulpir.rs:177-179): pairwise sums over the columns
+    for (u32 cnt = dim2; cnt > 1;) {
+        const u32 half = cnt / 2;
+        launch_add_rows(prods, prods + (size_t)(cnt - half) * num * per3, prods, c.mod.as<ModInfo>(), c.L, c.logn,
+                        (size_t)half * num * 3 * c.L, s);
+        cnt -= half;
+    }
+    rk.relinearize(prods, relin, num, s);
+    sw.switch_down(relin, responses, 2 * num, s);
+}
+
+// ---------------------------------------------------------------------------
 // Modulus::serialization_length (zq/mod.rs:773-777) per limb: N bits(q_i - 1) / 8.
 static WireTable wire_table(const DeviceContext& c, size_t& per_poly) {
     if (c.L > kWireMaxLimbs) throw Error(-2, "serialization supports at most 64 moduli");

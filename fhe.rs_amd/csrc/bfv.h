@@ -37,7 +37,7 @@ class StreamScratch {
 // every handle shares the pool of each purpose per stream; purposes that are
 // live at the same time (a multiplication and the key switch it calls) use
 // different pools.
-enum class Scratch { Scale, RelinC2, Lift, SwitchDown, Mul, GaloisC2, EvalTmp, ModSwitch, Codec, Rgsw, Wire, Count };
+enum class Scratch { Scale, RelinC2, Lift, SwitchDown, Mul, GaloisC2, EvalTmp, ModSwitch, Codec, Rgsw, Wire, Pir, Count };
 StreamScratch& scratch(Scratch k);
 // Frees every pool's buffer of stream `s`.
 void release_scratch(hipStream_t s);
@@ -185,6 +185,15 @@ struct Codec {
 size_t serialization_length(const DeviceContext& c);
 void serialize_polys(const DeviceContext& c, const u64* in, u32 num, bool ntt, uint8_t* out, hipStream_t s);
 void deserialize_polys(const DeviceContext& c, const uint8_t* in, u32 num, bool ntt, u64* out, hipStream_t s);
+
+// The PIR server response of examples/mulpir.rs:151-185 for `num` queries
+// [num][2][L][N] over a [dim1][dim2][L][N] database (NTT-domain plaintext
+// rows): expansion to dim1 + dim2 (query-minor), the dim2 column products,
+// their products with the second-dimension ciphertexts (`mul`: no
+// relinearization), the sum, relinearization by `rk` and `sw` (to the
+// response level) into responses [num][2][L_out][N].
+void pir_respond(const EvaluationKey& ek, Multiplicator& mul, Ksk& rk, const Switcher& sw, const u64* db, u32 dim1,
+                 u32 dim2, const u64* queries, u32 num, u64* responses, hipStream_t s);
 
 // Ciphertext x RGSWCiphertext (fhe/src/bfv/rgsw_ciphertext.rs:123-160):
 // ct [inst][2][L][N] NTT -> out [inst][2][L][N] = key_switch(c0, k0) +

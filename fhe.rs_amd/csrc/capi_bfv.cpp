@@ -330,6 +330,16 @@ int fhehip_expand_batch(fhehip_evaluation_key* ek, const uint64_t* d_cts, uint32
     FHEHIP_API_END
 }
 
+int fhehip_pir_respond(fhehip_evaluation_key* ek, fhehip_multiplicator* mul, fhehip_ksk* rk, const fhehip_switcher* sw,
+                       const uint64_t* d_db, uint32_t dim1, uint32_t dim2, const uint64_t* d_queries, uint32_t num,
+                       uint64_t* d_responses, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!ek || !mul || !rk || !sw || ((!d_db || !d_queries || !d_responses) && num))
+        return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    pir_respond(*ek->ek, *mul->m, *rk->k, *sw->s, d_db, dim1, dim2, d_queries, num, d_responses, S(stream));
+    FHEHIP_API_END
+}
+
 int fhehip_expansion_levels(const fhehip_evaluation_key* ek, uint32_t* levels) {
     FHEHIP_API_BEGIN
     if (!ek || !levels) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");

@@ -79,6 +79,7 @@ SIGNATURES = {
     "fhehip_expand": (C.c_int, [_vp, _vp, C.c_uint32, _vp, _vp]),
     "fhehip_expansion_levels": (C.c_int, [_vp, _u32p]),
     "fhehip_expand_batch": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint32, _vp, _vp]),
+    "fhehip_pir_respond": (C.c_int, [_vp, _vp, _vp, _vp, _vp, C.c_uint32, C.c_uint32, _vp, C.c_uint32, _vp, _vp]),
     "fhehip_dot_product_scalar": (C.c_int, [_vp, _vp, C.c_uint64, _vp, C.c_uint64, C.c_uint32, C.c_uint32, _vp,
                                             _vp]),
     "fhehip_dot_product_columns": (C.c_int, [_vp, _vp, C.c_uint32, C.c_uint32, _vp, C.c_uint32, _vp, _vp]),
@@ -356,6 +357,12 @@ def serialize(ctx, d_polys, num, d_bytes, ntt=True, stream=None):
 
 def deserialize(ctx, d_bytes, num, d_polys, ntt=True, stream=None):
     check(lib().fhehip_deserialize(ctx.h, dptr(d_bytes), num, 1 if ntt else 0, dptr(d_polys), stream_ptr(stream)))
+
+
+def pir_respond(ek, mult, rk, switcher, d_db, dim1, dim2, d_queries, num, d_responses, stream=None):
+    """fhehip_pir_respond: the MulPIR server response for num queries."""
+    check(lib().fhehip_pir_respond(ek.h, mult.h, rk.h, switcher.h, dptr(d_db), dim1, dim2, dptr(d_queries), num,
+                                   dptr(d_responses), stream_ptr(stream)))
 
 
 def rgsw_external_product(ksk0, ksk1, d_ct, d_out, num, stream=None):

@@ -271,6 +271,18 @@ int fhehip_expand(fhehip_evaluation_key* ek, const uint64_t* d_ct, uint32_t size
  * expands one query per call. */
 int fhehip_expand_batch(fhehip_evaluation_key* ek, const uint64_t* d_cts, uint32_t num, uint32_t size,
                         uint64_t* d_out, void* stream);
+/* The PIR server response of fhe.rs's MulPIR example (examples/mulpir.rs:
+ * 151-185) for num queries d_queries [num][2][L][N] (the query level's
+ * context) over the database d_db [dim1][dim2][L][N] (Plaintext::poly_ntt
+ * rows at that level): expansion to dim1 + dim2 ciphertexts, the dim2 column
+ * dot products (one pass for all queries), their products with the last dim2
+ * expanded ciphertexts by `mul` (a Multiplicator::new without
+ * relinearization), the sum, relinearization by `rk` and Ciphertext::
+ * switch_to_level by `sw` into d_responses [num][2][L_out][N]. The handles
+ * must share the query context. No single fhe.rs call. */
+int fhehip_pir_respond(fhehip_evaluation_key* ek, fhehip_multiplicator* mul, fhehip_ksk* rk, const fhehip_switcher* sw,
+                       const uint64_t* d_db, uint32_t dim1, uint32_t dim2, const uint64_t* d_queries, uint32_t num,
+                       uint64_t* d_responses, void* stream);
 /* EvaluationKey::supports_expansion (evaluation_key.rs:134-147): the largest
  * level (ceil(log2(size))) fhehip_expand accepts with this key. */
 int fhehip_expansion_levels(const fhehip_evaluation_key* ek, uint32_t* levels);

@@ -209,3 +209,54 @@ def test_expand_batch_matches_single_expansions(fh, oracle, n, num, size):
         _sync()
         assert np.array_equal(got[:, q, :].reshape(-1), fh.to_host(one)), q
     assert np.array_equal(got[:, 0, :].reshape(-1), bfv.expands(okeys, qs[:sz], size, level=1))
+
+
+def test_pir_respond_end_to_end(fh, oracle):
+    """A MulPIR round trip (examples/mulpir.rs:120-215) through
+    fhehip_pir_respond: selection queries encrypted by the oracle, a database
+    of poly-encoded plaintexts, and responses that decrypt to the selected
+    plaintexts; query 0's response equals the oracle's, bit for bit."""
+    n, t = 8192, T_MULPIR
+    print("pir_respond MulPIR n=8192 4x3 database, 2 queries", flush=True)
+    bfv = oracle.Bfv(n, t, oracle.generate_moduli([50, 55, 55], n), sk_seed=31)
+    m1, m2 = bfv.level_moduli(1), bfv.level_moduli(2)
+    dim1, dim2 = 4, 3
+    level = (dim1 + dim2 - 1).bit_length()
+    rng = np.random.default_rng(5)
+    vals = rng.integers(0, t, (dim1 * dim2, n)).astype(np.uint64)
+    db = np.concatenate([bfv.encode(vals[k], level=1, poly=True, scaled=False) for k in range(dim1 * dim2)])
+    inv = pow(1 << level, -1, t)
+    sel = [(1, 2), (3, 0)]
+    queries = []
+    for q, (i, j) in enumerate(sel):
+        pt = [0] * (dim1 + dim2)
+        pt[i] = pt[dim1 + j] = inv
+        queries.append(bfv.encrypt(pt, level=1, seed=10 + q, poly=True))
+    okeys, ek = _expansion_key(fh, bfv, level, 1, 0)
+    ork = bfv.relin_key(ct_level=1, key_level=1, seed=12)
+    c0, c1 = ork.export()
+    ctx1, ctx2 = fh.Context(m1, n), fh.Context(m2, n)
+    rk = fh.KeySwitchingKey(ctx1, ctx1, c0, c1)
+    om = oracle.Multiplicator(bfv, None, level=1)
+    mul = fh.Multiplicator(base_ctx=ctx1, extended_basis=om.ext_moduli(), post=(t, int(m1[0]) * int(m1[1])))
+    sw = fh.Switcher(ctx1, ctx2)
+    out = fh.empty(len(sel) * 2 * len(m2) * n)
+    fh.pir_respond(ek, mul, rk, sw, fh.to_device(db), dim1, dim2, fh.to_device(np.concatenate(queries)), len(sel),
+                   out)
+    _sync()
+    got = fh.to_host(out).reshape(len(sel), -1)
+    for q, (i, j) in enumerate(sel):
+        dec = bfv.decrypt(got[q], level=2, poly=True)
+        assert np.array_equal(dec, vals[i * dim2 + j]), q
+    # the same response by the oracle, step by step
+    L = len(m1)
+    o1 = oracle.Context(m1, n)
+    ex = bfv.expands(okeys, queries[0], dim1 + dim2, level=1).reshape(dim1 + dim2, 2 * L * n)
+    acc = None
+    qv = np.array(m1, dtype=np.uint64).repeat(n)
+    for c in range(dim2):
+        dc = oracle.dot_product_scalar(o1, ex[:dim1].reshape(-1), db[c * L * n:], dim1, 2, pt_stride=dim2 * L * n)
+        pc = om.multiply(dc, ex[dim1 + c]).reshape(3, L * n)
+        acc = pc if acc is None else np.where(acc + pc >= qv, acc + pc - qv, acc + pc)
+    want = oracle.ct_switch_down(o1, oracle.Context(m2, n), bfv.relinearize(ork, acc.reshape(-1)))
+    assert np.array_equal(got[0], want)
