@@ -451,7 +451,8 @@ def pir_server(fhehip, torch, reps=20, check_cpu=True, batches=(16, 64)):
     (433 MiB in HBM): expansion of the query into dim1 + dim2 ciphertexts,
     the dim2 column dot products over the database in one pass, the dim2
     ct x ct products (no relinearization) summed, one relinearization and
-    the switch to the last level. Inputs and keys are seeded uniform rows.
+    the switch to the last level, one fhehip_pir_respond call. Inputs and
+    keys are seeded uniform rows.
     Then `batches` queries at once (expansions query-minor, so every stage is
     one batch and one database pass serves all of them); query 0's batched
     response is compared with the single one. Returns the result and, with
@@ -463,7 +464,7 @@ def pir_server(fhehip, torch, reps=20, check_cpu=True, batches=(16, 64)):
     moduli = fhehip.generate_moduli(PIR_MODULI_SIZES, n)
     m1, m2 = moduli[:2], moduli[:1]
     L = len(m1)
-    per, per3 = 2 * L * n, 3 * L * n
+    per = 2 * L * n
     ctx0, ctx1, ctx2 = fhehip.Context(moduli, n), fhehip.Context(m1, n), fhehip.Context(m2, n)
     rng = np.random.default_rng(11)
     levels = (d1 + d2 - 1).bit_length()
@@ -479,24 +480,11 @@ def pir_server(fhehip, torch, reps=20, check_cpu=True, batches=(16, 64)):
     gen.manual_seed(12)
     db = _uniform_rows(torch, dev, m1, (d1 * d2, L, n), gen).reshape(-1)
     query = _uniform_rows(torch, dev, m1, (2, L, n), gen).reshape(-1)
-    expanded = torch.empty((d1 + d2) * per, dtype=torch.int64, device=dev)
-    dots = torch.empty(d2 * per, dtype=torch.int64, device=dev)
-    prods = torch.empty(d2 * per3, dtype=torch.int64, device=dev)
-    relin = torch.empty(per, dtype=torch.int64, device=dev)
     resp = torch.empty(2 * n, dtype=torch.int64, device=dev)
     stream = torch.cuda.current_stream()
 
     def respond():
-        ek.expand(query, d1 + d2, expanded, stream)
-        fhehip.dot_product_columns(ctx1, expanded, d1, 2, db, d2, dots, stream)
-        mul.multiply(dots, expanded[d1 * per:], prods, d2, stream)
-        cnt = d2   # out += dot_j * ct_j: pairwise sums of the products
-        while cnt > 1:
-            half = cnt // 2
-            fhehip.ciphertext_add(ctx1, prods, prods[(cnt - half) * per3:], prods, half * 3, stream)
-            cnt -= half
-        rk.relinearize(prods, relin, 1, stream)
-        sw.switch_down(relin, resp, 2, stream)
+        fhehip.pir_respond(ek, mul, rk, sw, db, d1, d2, query, 1, resp, stream)
 
     for _ in range(3):
         respond()
@@ -520,23 +508,10 @@ def pir_server(fhehip, torch, reps=20, check_cpu=True, batches=(16, 64)):
     def batched(nq):
         """The same response for nq queries (query 0 = the single query)."""
         qs = torch.cat([query, _uniform_rows(torch, dev, m1, (nq - 1, 2, L, n), gen).reshape(-1)])
-        ex = torch.empty((d1 + d2) * nq * per, dtype=torch.int64, device=dev)
-        dt = torch.empty(d2 * nq * per, dtype=torch.int64, device=dev)
-        pr = torch.empty(d2 * nq * per3, dtype=torch.int64, device=dev)
-        rl = torch.empty(nq * per, dtype=torch.int64, device=dev)
         rs = torch.empty(nq * 2 * n, dtype=torch.int64, device=dev)
 
         def run():
-            ek.expand_batch(qs, nq, d1 + d2, ex, stream)
-            fhehip.dot_product_columns(ctx1, ex, d1, 2 * nq, db, d2, dt, stream)
-            mul.multiply(dt, ex[d1 * nq * per:], pr, d2 * nq, stream)
-            cnt = d2
-            while cnt > 1:
-                half = cnt // 2
-                fhehip.ciphertext_add(ctx1, pr, pr[(cnt - half) * nq * per3:], pr, half * nq * 3, stream)
-                cnt -= half
-            rk.relinearize(pr, rl, nq, stream)
-            sw.switch_down(rl, rs, 2 * nq, stream)
+            fhehip.pir_respond(ek, mul, rk, sw, db, d1, d2, qs, nq, rs, stream)
 
         run()
         torch.cuda.synchronize()
