@@ -149,9 +149,6 @@ Ksk::Ksk(std::shared_ptr<DeviceContext> ct, std::shared_ptr<DeviceContext> key, 
     if (digits != ct_ctx->L) throw Error(-1, "one key row per ciphertext modulus is required");
     if (key_ctx->L == 1) throw Error(-2, "single-modulus key switching is not supported");
     extra = key_ctx->L - ct_ctx->L;
-    if (extra > 2) throw Error(-2, "key context more than two levels above the ciphertext context");
-    if (extra > 1 && !fused(*key_ctx))
-        throw Error(-2, "at degree 32768 the key context may be at most one level above the ciphertext context");
     const u32 Lk = key_ctx->L, n = key_ctx->N;
     const size_t rows = (size_t)digits * Lk;
     std::vector<ulonglong2> h0(rows * n), h1(rows * n);
@@ -288,12 +285,30 @@ void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* out, u3
     const size_t n = ct_ctx->N, Lk = key_ctx->L, L = ct_ctx->L;
     SdNttArgs a = sd_args(out, add_pb, add_inst, add_part, add_ntt, add_ntt_inst, add_ntt_part, add_ntt_parts, perm);
     if (extra > 1) {
-        // Two dropped moduli: key switch to power basis, then the switch-down
-        // chain on all limbs (degree <= 16384 only).
-        u64* acc = scratch(Scratch::SwitchDown).get(s, (size_t)inst * 2 * Lk * n * 8).as<u64>();
+        // Several dropped moduli (switch_down_to, math/src/rq/mod.rs:455-462):
+        // key switch to power basis, then Poly::switch_down once per dropped
+        // modulus, last first. The drops of limbs L' - 1 ... L + 1 run in
+        // place on the key-switch rows; the last one (limb L) is fused with
+        // the addends and the forward NTT (degree <= 16384) or runs like the
+        // others before two elementwise ends around the split NTT.
+        const bool fuse = fused(*ct_ctx);
+        u64* acc =
+            scratch(Scratch::SwitchDown).get(s, (size_t)inst * 2 * (Lk + (fuse ? 0 : L)) * n * 8).as<u64>();
         key_switch_pb(src, src_inst_stride, acc, inst, true, s);
+        const u32 keep = fuse ? (u32)L + 1 : (u32)L;  // limbs left by the in-place drops
+        for (u32 cur = (u32)Lk; cur > keep; cur--)
+            launch_switch_down_pb(acc, acc, sd_consts.as<SdConst>() + (size_t)(cur - 1 - L) * sd_stride,
+                                  key_ctx->moduli[cur - 1], cur, (u32)Lk, (u32)Lk, ct_ctx->logn, inst * 2, s);
         a.ks = acc;
-        launch_sd_add_ntt(ct_ctx->logn, a, ct_ctx->L, 2, inst, s);
+        if (fuse) {
+            a.extra = 1;
+            launch_sd_add_ntt(ct_ctx->logn, a, ct_ctx->L, 2, inst, s);
+            return;
+        }
+        u64* x = acc + (size_t)inst * 2 * Lk * n;
+        launch_sd_tail(ct_ctx->logn, a, x, false, inst, s);
+        launch_ntt_forward(*ct_ctx, RowMap{x, (u32)L, (u32)L, 0, 0}, inst * 2 * (u32)L, false, s);
+        launch_sd_tail(ct_ctx->logn, a, x, true, inst, s);
         return;
     }
     const u64* y = lift(src, src_inst_stride, inst, s);
@@ -438,6 +453,8 @@ void Switcher::switch_down(const u64* in, u64* out, u32 polys, hipStream_t s) co
 // Multiplicator::multiply (fhe/src/bfv/ops/mul.rs:158-227) on a batch.
 void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out_final, u32 inst, hipStream_t s) {
     check_batch(inst);
+    // the final switch_down runs on every part of every product
+    if (mod_switch) check_batch(inst * (rk ? 2 : 3));
     const size_t n = base->N, L = base->L, M = mul->L;
     const u32 cl = ext_lhs->common, cr = ext_rhs->common;
     const size_t xr = M - std::min(cl, cr);  // extension rows per input polynomial
@@ -788,6 +805,8 @@ void pir_respond(const EvaluationKey& ek, Multiplicator& mul, Ksk& rk, const Swi
     if (!dim1 || !dim2) throw Error(-1, "empty database");
     check_batch(num);
     if ((u64)dim2 * num > 65535) throw Error(-1, "at most 65535 products (dim2 x queries) per call");
+    // the final switch_to_level runs on both parts of every response
+    if ((u64)2 * num > 65535) throw Error(-1, "at most 32767 queries per call");
     if (!num) return;
     const size_t per = (size_t)2 * c.L * c.N, per3 = (size_t)3 * c.L * c.N, size = (size_t)dim1 + dim2;
     const size_t ex_w = size * num * per, dt_w = (size_t)dim2 * num * per, pr_w = (size_t)dim2 * num * per3;

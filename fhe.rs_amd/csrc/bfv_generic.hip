@@ -168,6 +168,50 @@ void launch_sd_z(int logn, const KsFinishArgs& f, u64* x, u32 inst, hipStream_t 
     HIPCHK(hipGetLastError());
 }
 
+// The two elementwise ends of the key-switch finish after a switch-down
+// chain of any length (key context any number of moduli above the
+// ciphertext context, relinearization_key.rs:84-91, galois_key.rs:70-77):
+// before the forward NTT, x[b][p][i] = ks[b][p][i] (the switched-down limbs,
+// in place in the key-switch rows) + add_pb; after it, out[b][p][i] =
+// x[b][p][i] + add_ntt[b][p][i][perm] for parts below add_ntt_parts.
+// x is contiguous [inst][2][L][N]. grid = (N/256, 2 L, instances)
+__global__ __launch_bounds__(256) void sd_tail_pre_kernel(SdNttArgs a, u64* __restrict__ x, u32 logn) {
+    const u32 e = blockIdx.x * 256 + threadIdx.x;
+    if (e >> logn) return;
+    const u32 i = blockIdx.y % a.L, part = blockIdx.y / a.L, b = blockIdx.z;
+    u64 z = a.ks[(size_t)b * a.ks_inst_stride + (size_t)part * a.ks_part_stride + ((size_t)i << logn) + e];
+    if (a.add_pb)
+        z = add_mod(z,
+                    a.add_pb[(size_t)b * a.add_inst_stride + (size_t)part * a.add_part_stride + ((size_t)i << logn) + e],
+                    a.mods[i].p);
+    x[((((size_t)b * 2 + part) * a.L + i) << logn) + e] = z;
+}
+
+__global__ __launch_bounds__(256) void sd_tail_post_kernel(SdNttArgs a, const u64* __restrict__ x, u32 logn) {
+    const u32 e = blockIdx.x * 256 + threadIdx.x;
+    if (e >> logn) return;
+    const u32 i = blockIdx.y % a.L, part = blockIdx.y / a.L, b = blockIdx.z;
+    u64 z = x[((((size_t)b * 2 + part) * a.L + i) << logn) + e];
+    if (a.add_ntt && part < a.add_ntt_parts) {
+        const u32 src = a.perm ? a.perm[e] : e;
+        z = add_mod(z,
+                    a.add_ntt[(size_t)b * a.add_ntt_inst_stride + (size_t)part * a.add_ntt_part_stride +
+                              ((size_t)i << logn) + src],
+                    a.mods[i].p);
+    }
+    a.out[(size_t)b * a.out_inst_stride + (size_t)part * a.out_part_stride + ((size_t)i << logn) + e] = z;
+}
+
+void launch_sd_tail(int logn, const SdNttArgs& a, u64* x, bool post, u32 inst, hipStream_t s) {
+    if (!inst || !a.L) return;
+    ProfScope ps(post ? "sd_tail_post" : "sd_tail_pre", logn, s);
+    if (post)
+        hipLaunchKernelGGL(sd_tail_post_kernel, ew_grid(1u << logn, 2 * a.L, inst), dim3(256), 0, s, a, x, (u32)logn);
+    else
+        hipLaunchKernelGGL(sd_tail_pre_kernel, ew_grid(1u << logn, 2 * a.L, inst), dim3(256), 0, s, a, x, (u32)logn);
+    HIPCHK(hipGetLastError());
+}
+
 }  // namespace fhehip
 
 namespace fhehip {

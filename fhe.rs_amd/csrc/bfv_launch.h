@@ -167,6 +167,10 @@ void launch_pack(const u64* in, uint8_t* out, const WireTable& t, int logn, u32 
 void launch_unpack(const uint8_t* in, u64* out, const WireTable& t, int logn, u32 per_poly, u32 num, hipStream_t s);
 // Power-basis switch-down part x [inst][2][L][N] of the finish (unfused path).
 void launch_sd_z(int logn, const KsFinishArgs& f, u64* x, u32 inst, hipStream_t s);
+// Ends of the finish after an in-place switch-down chain (any key depth,
+// unfused path): post == false writes x = ks + add_pb (x contiguous
+// [inst][2][L][N]); post == true writes out = x + add_ntt[perm].
+void launch_sd_tail(int logn, const SdNttArgs& a, u64* x, bool post, u32 inst, hipStream_t s);
 // Copies `count` slots of `elems` words; `table` is a device array of
 // (src, dst) pointer pairs.
 void launch_copy_slots(const u64* const* table, u32 count, size_t elems, hipStream_t s);

@@ -116,10 +116,12 @@ int fhehip_scaler_scale(fhehip_scaler* s, const uint64_t* d_in, uint64_t* d_out,
  * KeySwitchingKey (fhe/src/bfv/keys/key_switching_key.rs:22-45) from its
  * NTT-domain rows c0, c1 given as HOST arrays [digits][L_key][N]; digits must
  * equal the number of ciphertext moduli (RNS-limb decomposition,
- * key_switching_key.rs:89-104). The key context must extend the ciphertext
- * context by at most two moduli (one at degree 32768): the special primes
- * removed by switch_down_to. The rows are copied to the device with their Shoup
- * companions (NttShoup representation). */
+ * key_switching_key.rs:89-104). The key context must start with the
+ * ciphertext context's moduli and may extend it by any number of moduli (a
+ * key at any lower level, RelinearizationKey::new_leveled and GaloisKey::new,
+ * relinearization_key.rs:34-67, galois_key.rs:26-60): they are removed by
+ * switch_down_to after the switch (math/src/rq/mod.rs:455-462). The rows are
+ * copied to the device with their Shoup companions (NttShoup representation). */
 int fhehip_ksk_create(const fhehip_context* ct_ctx, const fhehip_context* key_ctx, const uint64_t* c0,
                       const uint64_t* c1, uint32_t digits, fhehip_ksk** out);
 void fhehip_ksk_destroy(fhehip_ksk* k);

@@ -245,17 +245,6 @@ def test_galois_matches_oracle(fh, oracle, n, L, ct_level, key_level):
         assert [int(x) for x in bfv.decrypt(got, level=ct_level)] == _rotate_columns(v, k)
 
 
-def test_degree_32768_rejects_two_extra_key_levels(fh, oracle):
-    """The unfused pipeline supports a key context at most one level above the
-    ciphertext context; deeper keys are refused with UNSUPPORTED."""
-    n = 32768
-    bfv = _bfv(oracle, n, [60] * 4, seed=1)
-    rk = bfv.relin_key(ct_level=2, key_level=0, seed=2)
-    with pytest.raises(fh.FheHipError) as e:
-        _gpu_key(fh, bfv, rk, 2, 0)
-    assert e.value.code == -2
-
-
 @pytest.mark.parametrize("n,sizes", [(16, [62, 62, 62]), (4096, [50, 55, 55]), (16384, [62] * 6)])
 def test_mul_and_relin_2_matches_oracle(fh, oracle, n, sizes):
     """The second multiplication strategy of fhe/benches/bfv.rs:257-286:
