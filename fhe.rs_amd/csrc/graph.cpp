@@ -23,7 +23,10 @@ std::unique_ptr<Graph> capture_graph(std::unique_ptr<Graph> g, const std::functi
     issue(g->cap);
     HIPCHK(hipStreamSynchronize(g->cap));
     CaptureGuard guard;
-    HIPCHK(hipStreamBeginCapture(g->cap, hipStreamCaptureModeGlobal));
+    // Thread-local capture: the library's own threads (the request server, the
+    // job runner) keep allocating and synchronizing while a graph is captured
+    // here; the eager run above sized every scratch buffer the capture uses.
+    HIPCHK(hipStreamBeginCapture(g->cap, hipStreamCaptureModeThreadLocal));
     try {
         issue(g->cap);
     } catch (...) {

@@ -103,3 +103,43 @@ def test_graph_rejects_missing_rotation(fh, oracle):
     d = fh.empty(2 * len(moduli) * 1024)
     with pytest.raises(fh.FheHipError):
         fh.Graph(mult, d, d, fh.empty(d.numel()), 1, ek=ek, log2_slots=2)
+
+
+def test_capture_while_server_runs(fh, oracle):
+    """A graph captured while the request server's threads allocate scratch
+    and synchronize (its first batches) leaves both intact: every served
+    request equals its direct calls, and the graph's replays the eager
+    product."""
+    n, log2_slots = 1024, 2
+    bfv, moduli, mult, ek, om, ctxs = _setup(fh, oracle, n, [60] * 4, log2_slots, seed=11)
+    L = len(moduli)
+    w = 2 * L * n
+    count = 48
+    a = fh.to_device(oracle.random_polys(moduli, n, 2 * count, seed=21))
+    b = fh.to_device(oracle.random_polys(moduli, n, 2 * count, seed=22))
+    served = fh.empty(count * w)
+    _sync()
+    srv = fh.Server(max_batch=4, lanes=3)
+    pid = srv.add_params(mult, ek, log2_slots)[0]
+    for k in range(count):
+        srv.submit(pid, a[k * w:(k + 1) * w], b[k * w:(k + 1) * w], served[k * w:(k + 1) * w])
+    gout = fh.empty(2 * w)
+    g = fh.Graph(mult, a[:2 * w], b[:2 * w], gout, 2)   # captured while the server works
+    srv.drain(120)
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == count and st["failed"] == 0
+    eager = fh.empty(2 * w)
+    mult.multiply(a[:2 * w], b[:2 * w], eager, 2)
+    prod, ref = fh.empty(count * w), fh.empty(count * w)
+    mult.multiply(a, b, prod, count)
+    ek.rotate_and_sum(prod, ref, log2_slots, False, count)
+    _sync()
+    assert np.array_equal(fh.to_host(served), fh.to_host(ref))
+    for rep in range(2):
+        gout.zero_()
+        _sync()
+        g.launch()
+        _sync()
+        assert np.array_equal(fh.to_host(gout), fh.to_host(eager)), rep
+    g.close()
