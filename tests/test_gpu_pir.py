@@ -211,52 +211,106 @@ def test_expand_batch_matches_single_expansions(fh, oracle, n, num, size):
     assert np.array_equal(got[:, 0, :].reshape(-1), bfv.expands(okeys, qs[:sz], size, level=1))
 
 
-def test_pir_respond_end_to_end(fh, oracle):
-    """A MulPIR round trip (examples/mulpir.rs:120-215) through
-    fhehip_pir_respond: selection queries encrypted by the oracle, a database
-    of poly-encoded plaintexts, and responses that decrypt to the selected
-    plaintexts; query 0's response equals the oracle's, bit for bit."""
-    n, t = 8192, T_MULPIR
-    print("pir_respond MulPIR n=8192 4x3 database, 2 queries", flush=True)
-    bfv = oracle.Bfv(n, t, oracle.generate_moduli([50, 55, 55], n), sk_seed=31)
+def _pir_setup(fh, oracle, n, dim1, dim2, t=None, seed=31):
+    t = t or T_MULPIR
+    bfv = oracle.Bfv(n, t, oracle.generate_moduli([50, 55, 55], n), sk_seed=seed)
     m1, m2 = bfv.level_moduli(1), bfv.level_moduli(2)
-    dim1, dim2 = 4, 3
     level = (dim1 + dim2 - 1).bit_length()
-    rng = np.random.default_rng(5)
-    vals = rng.integers(0, t, (dim1 * dim2, n)).astype(np.uint64)
-    db = np.concatenate([bfv.encode(vals[k], level=1, poly=True, scaled=False) for k in range(dim1 * dim2)])
-    inv = pow(1 << level, -1, t)
-    sel = [(1, 2), (3, 0)]
-    queries = []
-    for q, (i, j) in enumerate(sel):
-        pt = [0] * (dim1 + dim2)
-        pt[i] = pt[dim1 + j] = inv
-        queries.append(bfv.encrypt(pt, level=1, seed=10 + q, poly=True))
     okeys, ek = _expansion_key(fh, bfv, level, 1, 0)
-    ork = bfv.relin_key(ct_level=1, key_level=1, seed=12)
+    ork = bfv.relin_key(ct_level=1, key_level=1, seed=seed + 1)
     c0, c1 = ork.export()
     ctx1, ctx2 = fh.Context(m1, n), fh.Context(m2, n)
     rk = fh.KeySwitchingKey(ctx1, ctx1, c0, c1)
     om = oracle.Multiplicator(bfv, None, level=1)
     mul = fh.Multiplicator(base_ctx=ctx1, extended_basis=om.ext_moduli(), post=(t, int(m1[0]) * int(m1[1])))
     sw = fh.Switcher(ctx1, ctx2)
-    out = fh.empty(len(sel) * 2 * len(m2) * n)
-    fh.pir_respond(ek, mul, rk, sw, fh.to_device(db), dim1, dim2, fh.to_device(np.concatenate(queries)), len(sel),
-                   out)
-    _sync()
-    got = fh.to_host(out).reshape(len(sel), -1)
-    for q, (i, j) in enumerate(sel):
-        dec = bfv.decrypt(got[q], level=2, poly=True)
-        assert np.array_equal(dec, vals[i * dim2 + j]), q
-    # the same response by the oracle, step by step
+    return dict(bfv=bfv, t=t, m1=m1, m2=m2, level=level, okeys=okeys, ek=ek, ork=ork, rk=rk, om=om, mul=mul, sw=sw,
+                ctx1=ctx1, ctx2=ctx2)
+
+
+def _pir_oracle_response(oracle, S, db, query, dim1, dim2):
+    """The MulPIR server response (examples/mulpir.rs:151-185), step by step
+    in the oracle."""
+    bfv, m1, n = S["bfv"], S["m1"], S["bfv"].degree
     L = len(m1)
     o1 = oracle.Context(m1, n)
-    ex = bfv.expands(okeys, queries[0], dim1 + dim2, level=1).reshape(dim1 + dim2, 2 * L * n)
+    ex = bfv.expands(S["okeys"], query, dim1 + dim2, level=1).reshape(dim1 + dim2, 2 * L * n)
     acc = None
     qv = np.array(m1, dtype=np.uint64).repeat(n)
     for c in range(dim2):
         dc = oracle.dot_product_scalar(o1, ex[:dim1].reshape(-1), db[c * L * n:], dim1, 2, pt_stride=dim2 * L * n)
-        pc = om.multiply(dc, ex[dim1 + c]).reshape(3, L * n)
+        pc = S["om"].multiply(dc, ex[dim1 + c]).reshape(3, L * n)
         acc = pc if acc is None else np.where(acc + pc >= qv, acc + pc - qv, acc + pc)
-    want = oracle.ct_switch_down(o1, oracle.Context(m2, n), bfv.relinearize(ork, acc.reshape(-1)))
-    assert np.array_equal(got[0], want)
+    return oracle.ct_switch_down(o1, oracle.Context(S["m2"], n), bfv.relinearize(S["ork"], acc.reshape(-1)))
+
+
+@pytest.mark.parametrize("n,dim1,dim2,num", [(8192, 4, 3, 2), (8192, 5, 1, 5), (16384, 3, 4, 1), (1024, 7, 9, 16)])
+def test_pir_respond_end_to_end(fh, oracle, n, dim1, dim2, num):
+    """A MulPIR round trip (examples/mulpir.rs:120-215) through
+    fhehip_pir_respond: selection queries encrypted by the oracle, a database
+    of poly-encoded plaintexts, and responses that decrypt to the selected
+    plaintexts; every query's response equals the oracle's, bit for bit."""
+    print(f"pir_respond MulPIR n={n} {dim1}x{dim2} database, {num} queries", flush=True)
+    S = _pir_setup(fh, oracle, n, dim1, dim2)
+    bfv, t, m1, m2 = S["bfv"], S["t"], S["m1"], S["m2"]
+    rng = np.random.default_rng(n + num)
+    vals = rng.integers(0, t, (dim1 * dim2, n)).astype(np.uint64)
+    db = np.concatenate([bfv.encode(vals[k], level=1, poly=True, scaled=False) for k in range(dim1 * dim2)])
+    inv = pow(1 << S["level"], -1, t)
+    sel = [(int(rng.integers(dim1)), int(rng.integers(dim2))) for _ in range(num)]
+    queries = []
+    for q, (i, j) in enumerate(sel):
+        pt = [0] * (dim1 + dim2)
+        pt[i] = pt[dim1 + j] = inv
+        queries.append(bfv.encrypt(pt, level=1, seed=10 + q, poly=True))
+    out = fh.empty(num * 2 * len(m2) * n)
+    fh.pir_respond(S["ek"], S["mul"], S["rk"], S["sw"], fh.to_device(db), dim1, dim2,
+                   fh.to_device(np.concatenate(queries)), num, out)
+    _sync()
+    got = fh.to_host(out).reshape(num, -1)
+    for q, (i, j) in enumerate(sel):
+        dec = bfv.decrypt(got[q], level=2, poly=True)
+        assert np.array_equal(dec, vals[i * dim2 + j]), q
+        assert np.array_equal(got[q], _pir_oracle_response(oracle, S, db, queries[q], dim1, dim2)), q
+
+
+def test_pir_respond_rejects_bad_handles_and_sizes(fh, oracle):
+    """fhehip_pir_respond refuses handles of another context, a multiplicator
+    that relinearizes or switches moduli, and batches whose products or
+    switch-down exceed one launch, before any work."""
+    n, dim1, dim2 = 1024, 2, 2
+    S = _pir_setup(fh, oracle, n, dim1, dim2, seed=41)
+    L1, L2 = len(S["m1"]), len(S["m2"])
+    db = fh.empty(dim1 * dim2 * L1 * n)
+    q = fh.empty(2 * 2 * L1 * n)
+    out = fh.empty(2 * 2 * L2 * n)
+
+    def respond(ek=S["ek"], mul=S["mul"], rk=S["rk"], sw=S["sw"], d1=dim1, d2=dim2, num=2, queries=q):
+        fh.pir_respond(ek, mul, rk, sw, db, d1, d2, queries, num, out)
+
+    # another context: a switcher from the level-2 context
+    other_sw = fh.Switcher(S["ctx2"], S["ctx2"])
+    with pytest.raises(fh.FheHipError) as e:
+        respond(sw=other_sw)
+    assert e.value.code == fh.ERR_CTX_MISMATCH
+    # a relinearizing multiplicator
+    relin_mul = fh.Multiplicator(rk=S["rk"], plaintext=S["t"])
+    with pytest.raises(fh.FheHipError) as e:
+        respond(mul=relin_mul)
+    assert e.value.code == fh.ERR_INVALID_ARG
+    # a mod-switching multiplicator
+    ms = fh.Multiplicator(base_ctx=S["ctx1"], extended_basis=S["om"].ext_moduli(),
+                          post=(S["t"], int(S["m1"][0]) * int(S["m1"][1])))
+    ms.enable_mod_switching(S["ctx2"])
+    with pytest.raises(fh.FheHipError) as e:
+        respond(mul=ms)
+    assert e.value.code == fh.ERR_INVALID_ARG
+    # dim2 x queries above one launch, and more queries than one switch-down takes
+    with pytest.raises(fh.FheHipError) as e:
+        respond(d2=40000, num=2)
+    assert e.value.code == fh.ERR_INVALID_ARG
+    with pytest.raises(fh.FheHipError) as e:
+        respond(d2=1, num=40000)
+    assert e.value.code == fh.ERR_INVALID_ARG
+    with pytest.raises(fh.FheHipError):
+        respond(d1=0)
