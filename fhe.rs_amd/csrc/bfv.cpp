@@ -229,9 +229,19 @@ KeySwitchArgs Ksk::args(const u64* src, size_t src_inst_stride) const {
 
 // Digit lift + lazy forward NTT of every (digit, key limb) pair
 // (rq/mod.rs:518-540) into lift_work [inst][digits][L'][N].
+size_t Ksk::lift_bytes(u32 inst) const { return (size_t)inst * digits * key_ctx->L * key_ctx->N * 8; }
+
+// [inst][2][L'][N] power-basis key-switch rows (+ [inst][2][L][N] at degree
+// 32768) for a deep key; [inst][2][N] for the dropped limb (+ the same
+// [inst][2][L][N] at degree 32768) otherwise.
+size_t Ksk::finish_bytes(u32 inst) const {
+    const size_t n = ct_ctx->N, L = ct_ctx->L, Lk = key_ctx->L;
+    const size_t unfused = fused(*ct_ctx) ? 0 : L;
+    return (size_t)inst * 2 * ((extra > 1 ? Lk : extra) + unfused) * n * 8;
+}
+
 u64* Ksk::lift(const u64* src, size_t src_inst_stride, u32 inst, hipStream_t s) const {
-    const u32 Lk = key_ctx->L;
-    u64* y = scratch(Scratch::Lift).get(s, (size_t)inst * digits * Lk * key_ctx->N * 8).as<u64>();
+    u64* y = scratch(Scratch::Lift).get(s, lift_bytes(inst)).as<u64>();
     const KeySwitchArgs a = args(src, src_inst_stride);
     if (fused(*key_ctx)) {
         launch_lift_ntt(key_ctx->logn, a, y, inst, s);
@@ -292,8 +302,7 @@ void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* out, u3
         // the addends and the forward NTT (degree <= 16384) or runs like the
         // others before two elementwise ends around the split NTT.
         const bool fuse = fused(*ct_ctx);
-        u64* acc =
-            scratch(Scratch::SwitchDown).get(s, (size_t)inst * 2 * (Lk + (fuse ? 0 : L)) * n * 8).as<u64>();
+        u64* acc = scratch(Scratch::SwitchDown).get(s, finish_bytes(inst)).as<u64>();
         key_switch_pb(src, src_inst_stride, acc, inst, true, s);
         const u32 keep = fuse ? (u32)L + 1 : (u32)L;  // limbs left by the in-place drops
         for (u32 cur = (u32)Lk; cur > keep; cur--)
@@ -316,7 +325,7 @@ void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* out, u3
     // `out` (NTT domain), the dropped limb into `special`, which alone goes
     // back to power basis; then the transformed power-basis part is added in.
     const bool fuse = fused(*ct_ctx);
-    u64* base = scratch(Scratch::SwitchDown).get(s, (size_t)inst * 2 * (extra + (fuse ? 0 : L)) * n * 8).as<u64>();
+    u64* base = scratch(Scratch::SwitchDown).get(s, finish_bytes(inst)).as<u64>();
     u64* special = extra ? base : nullptr;
     KsFinishArgs f{a, y, fin0.as<u64>(), fin1.as<u64>(), digits, (u32)Lk, special,
                    key_ctx->mod.as<ModInfo>() + (Lk - 1)};
@@ -431,7 +440,7 @@ void Switcher::switch_down(const u64* in, u64* out, u32 polys, hipStream_t s) co
         HIPCHK(hipMemcpyAsync(out, in, (size_t)polys * L * n * 8, hipMemcpyDeviceToDevice, s));
         return;
     }
-    u64* pb = scratch(Scratch::ModSwitch).get(s, (size_t)polys * L * n * 8).as<u64>();
+    u64* pb = scratch(Scratch::ModSwitch).get(s, scratch_bytes(polys)).as<u64>();
     InttRowsArgs ia{};
     ia.src[0] = in;
     ia.src_inst_stride = L * n;
@@ -450,6 +459,25 @@ void Switcher::switch_down(const u64* in, u64* out, u32 polys, hipStream_t s) co
     launch_ntt_forward(*to, RowMap{out, to->L, to->L, 0, 0}, polys * to->L, false, s);
 }
 
+size_t Switcher::scratch_bytes(u32 polys) const {
+    return from->L == to->L ? 0 : (size_t)polys * from->L * from->N * 8;
+}
+
+// [inst][4][L][N] inverse-transformed inputs, [inst][4][xr][N] extension
+// rows, [inst][3][M][N] tensor rows and [inst][3][L][N] down-scaled rows.
+size_t Multiplicator::mul_bytes(u32 inst) const {
+    const size_t n = base->N, L = base->L, M = mul->L;
+    const size_t xr = M - std::min(ext_lhs->common, ext_rhs->common);
+    return (size_t)inst * (4 * L + 4 * xr + 3 * M + 3 * L) * n * 8;
+}
+
+size_t Multiplicator::scratch_bytes(u32 inst) const {
+    size_t b = FixedArena::aligned(mul_bytes(inst));
+    if (rk) b += FixedArena::aligned(rk->lift_bytes(inst)) + FixedArena::aligned(rk->finish_bytes(inst));
+    if (mod_switch) b += FixedArena::aligned(mod_switch->scratch_bytes(inst * (rk ? 2 : 3)));
+    return b;
+}
+
 // Multiplicator::multiply (fhe/src/bfv/ops/mul.rs:158-227) on a batch.
 void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out_final, u32 inst, hipStream_t s) {
     check_batch(inst);
@@ -462,8 +490,7 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out_final, u32
     const size_t P = (size_t)inst * 4 * L * n;
     const size_t X = (size_t)inst * 4 * xr * n;
     const size_t T = (size_t)inst * 3 * M * n;
-    const size_t D = (size_t)inst * 3 * L * n;
-    u64* wP = scratch(Scratch::Mul).get(s, (P + X + T + D) * 8).as<u64>();
+    u64* wP = scratch(Scratch::Mul).get(s, mul_bytes(inst)).as<u64>();
     u64* wX = wP + P;
     u64* wT = wX + X;
     u64* wD = wT + T;
@@ -971,7 +998,26 @@ void StreamScratch::release(hipStream_t s) const {
     ws_.erase(s);
 }
 
+namespace {
+thread_local FixedArena* t_arena = nullptr;
+}
+
+ArenaScope::ArenaScope(FixedArena& a) : prev_(t_arena) { t_arena = &a; }
+ArenaScope::~ArenaScope() { t_arena = prev_; }
+
+Workspace& FixedArena::carve(size_t bytes) {
+    if (views_ == kMaxViews) throw Error(-1, "too many scratch regions for the fixed scratch");
+    const size_t need = aligned(bytes);
+    if (need > size_ - used_)
+        throw Error(-5, "the fixed scratch region holds " + std::to_string(size_) + " bytes; this call needs more");
+    Workspace& w = view_[views_++];
+    w.view = base_ + used_;
+    used_ += need;
+    return w;
+}
+
 Workspace& StreamScratch::get(hipStream_t s, size_t bytes) const {
+    if (t_arena) return t_arena->carve(bytes);
     std::lock_guard<std::mutex> g(m_);
     auto& w = ws_[s];
     if (!w) w.reset(new Workspace);

@@ -9,12 +9,46 @@
 
 namespace fhehip {
 
-// Device scratch owned by a handle; grows to the largest batch seen.
+// Device scratch owned by a handle; grows to the largest batch seen. A view
+// (non-owning) is a region of a FixedArena instead.
 struct Workspace {
     DevBuf buf;
+    void* view = nullptr;
     void ensure(size_t bytes);
     template <class T>
-    T* as() const { return buf.as<T>(); }
+    T* as() const { return (T*)(view ? view : buf.ptr); }
+};
+
+// A fixed device region, owned by the caller, that every scratch request made
+// on the calling thread inside an ArenaScope is carved from, in request order
+// (each region 256-byte aligned), instead of the per-stream pools: no device
+// allocation happens. A request that does not fit throws before its kernels
+// are issued.
+class FixedArena {
+  public:
+    static constexpr size_t kAlign = 256;
+    static constexpr int kMaxViews = 16;
+    FixedArena(void* base, size_t bytes) : base_((uint8_t*)base), size_(bytes) {}
+    Workspace& carve(size_t bytes);
+    size_t used() const { return used_; }
+    static size_t aligned(size_t bytes) { return (bytes + kAlign - 1) / kAlign * kAlign; }
+
+  private:
+    uint8_t* base_;
+    size_t size_, used_ = 0;
+    int views_ = 0;
+    Workspace view_[kMaxViews];
+};
+
+// Routes the calling thread's scratch requests to `arena` while alive.
+struct ArenaScope {
+    explicit ArenaScope(FixedArena& a);
+    ~ArenaScope();
+    ArenaScope(const ArenaScope&) = delete;
+    ArenaScope& operator=(const ArenaScope&) = delete;
+
+  private:
+    FixedArena* prev_;
 };
 
 // Scratch kept per stream: calls issued on different streams never share a
@@ -71,6 +105,10 @@ struct Ksk {
     Ksk(std::shared_ptr<DeviceContext> ct, std::shared_ptr<DeviceContext> key, const u64* c0, const u64* c1,
         u32 digits);
     KeySwitchArgs args(const u64* src, size_t src_inst_stride) const;
+    // Scratch bytes of the lifted digits and of the switch-down rows of one
+    // switch_and_finish over `inst` instances.
+    size_t lift_bytes(u32 inst) const;
+    size_t finish_bytes(u32 inst) const;
     // Digit lift + lazy forward NTT into scratch [inst][digits][L'][N].
     u64* lift(const u64* src, size_t src_inst_stride, u32 inst, hipStream_t s) const;
     // Full key switch into acc [inst][2][L'][N] (NTT domain, or power basis
@@ -103,6 +141,8 @@ struct Switcher {
     Switcher(std::shared_ptr<DeviceContext> from, std::shared_ptr<DeviceContext> to);
     // [polys][L_from][N] -> [polys][L_to][N]; out must not alias in.
     void switch_down(const u64* in, u64* out, u32 polys, hipStream_t s) const;
+    // Scratch bytes of one switch_down of `polys` polynomials.
+    size_t scratch_bytes(u32 polys) const;
 };
 
 struct Multiplicator {
@@ -117,7 +157,16 @@ struct Multiplicator {
                   const BigUint& post_den);
     static std::unique_ptr<Multiplicator> make_default(std::shared_ptr<Ksk> rk, u64 plaintext);
     void multiply(const u64* lhs, const u64* rhs, u64* out, u32 inst, hipStream_t s);
+    // Bytes of the multiplication's own workspace, and of all scratch one
+    // multiply over `inst` instances requests (each region aligned as a
+    // FixedArena carves it).
+    size_t mul_bytes(u32 inst) const;
+    size_t scratch_bytes(u32 inst) const;
 };
+
+// fhehip_multiply with all scratch in the library's fixed device region
+// (no device allocation); refuses batches whose scratch does not fit.
+void multiply_noalloc(Multiplicator& m, const u64* lhs, const u64* rhs, u64* out, u32 inst, hipStream_t s);
 
 // GaloisKey (fhe/src/bfv/keys/galois_key.rs:15-124).
 struct Galois {

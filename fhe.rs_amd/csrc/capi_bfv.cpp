@@ -263,6 +263,14 @@ int fhehip_multiply(fhehip_multiplicator* m, const uint64_t* d_lhs, const uint64
     FHEHIP_API_END
 }
 
+int fhehip_multiply_noalloc(fhehip_multiplicator* m, const uint64_t* d_lhs, const uint64_t* d_rhs, uint64_t* d_out,
+                            uint32_t num, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!m || ((!d_lhs || !d_rhs || !d_out) && num)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    multiply_noalloc(*m->m, d_lhs, d_rhs, d_out, num, S(stream));
+    FHEHIP_API_END
+}
+
 int fhehip_galois_create(const fhehip_ksk* ksk, uint32_t exponent, fhehip_galois** out) {
     FHEHIP_API_BEGIN
     if (!ksk || !out) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");

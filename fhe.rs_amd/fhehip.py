@@ -53,6 +53,7 @@ SIGNATURES = {
     "fhehip_multiplicator_destroy": (None, [_vp]),
     "fhehip_multiplicator_extended_basis": (C.c_int, [_vp, _u64p, _u32p]),
     "fhehip_multiply": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_multiply_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
     "fhehip_galois_create": (C.c_int, [_vp, C.c_uint32, _vpp]),
     "fhehip_galois_destroy": (None, [_vp]),
     "fhehip_galois_relinearize": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp]),
@@ -318,6 +319,10 @@ class Multiplicator:
 
     def multiply(self, d_lhs, d_rhs, d_out, num, stream=None):
         check(lib().fhehip_multiply(self.h, dptr(d_lhs), dptr(d_rhs), dptr(d_out), num, stream_ptr(stream)))
+
+    def multiply_noalloc(self, d_lhs, d_rhs, d_out, num, stream=None):
+        """fhehip_multiply_noalloc: scratch in the library's fixed device region."""
+        check(lib().fhehip_multiply_noalloc(self.h, dptr(d_lhs), dptr(d_rhs), dptr(d_out), num, stream_ptr(stream)))
 
     def enable_mod_switching(self, next_ctx):
         """Multiplicator::enable_mod_switching: products land in next_ctx."""

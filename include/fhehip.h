@@ -163,6 +163,23 @@ int fhehip_multiplicator_extended_basis(const fhehip_multiplicator* m, uint64_t*
 int fhehip_multiply(fhehip_multiplicator* m, const uint64_t* d_lhs, const uint64_t* d_rhs, uint64_t* d_out,
                     uint32_t num, void* stream);
 
+/* No fhe.rs counterpart. For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): fhehip_multiply with every scratch buffer carved from one fixed
+ * device region of FHEHIP_NOALLOC_SCRATCH_BYTES, which each GPU the library
+ * runs on holds from the library's load until the process ends; the call
+ * allocates nothing. Its first call on a GPU looks the region up (make that
+ * call outside graph capture). A batch whose scratch does not fit is refused
+ * with FHEHIP_ERR_OOM before any work: at N = 16384 with 6 moduli and a key
+ * one modulus above them an instance needs 19.1 MiB (13 per call); at
+ * N = 8192 with 3 moduli and a key at their level 4.2 MiB (61 per call). Every call on one GPU shares
+ * the region, and the region keeps the last call's contents (no part of it
+ * has to start at zero): calls that overlap in time on one GPU race and must
+ * be serialized by the caller. Results equal fhehip_multiply's. */
+#define FHEHIP_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int fhehip_multiply_noalloc(fhehip_multiplicator* m, const uint64_t* d_lhs, const uint64_t* d_rhs, uint64_t* d_out,
+                            uint32_t num, void* stream);
+
 /* ---- Galois automorphisms -------------------------------------------------
  * GaloisKey (fhe/src/bfv/keys/galois_key.rs:15-60) from a key-switching key
  * built for s(x^exponent) and the exponent (odd modulo 2N). */
