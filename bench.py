@@ -72,35 +72,125 @@ def kernel_bytes(name, B, L, Lk, M):
     return None
 
 
-def load_traffic(kernel):
-    """HBM bytes per launch of `kernel` (its name without the degree suffix)
-    from the committed rocprofv3 counter summary, when present: the summary
-    holds a table whose rows read `<kernel> <fetch KiB> <write KiB> <bytes>`."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_mul_relin_n16384_b512.md")
+def kernel_sources_digest():
+    """SHA-256 (first 16 hex digits) of the library's kernel and host sources
+    (fhe.rs_amd/csrc, sorted by name): the counter summaries under profiles/
+    record the digest of the sources they were measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    d = os.path.join(ROOT, "fhe.rs_amd", "csrc")
+    for name in sorted(os.listdir(d)):
+        h.update(name.encode())
+        with open(os.path.join(d, name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def load_profile(name):
+    """(rows, digest) of a committed rocprofv3 counter summary: its table
+    rows split on whitespace, and the digest on its `Kernel sources:` line."""
+    rows, digest = [], None
     try:
-        with open(path) as f:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
             for line in f:
-                parts = line.split()
-                if len(parts) == 4 and parts[0] == kernel and parts[3].isdigit():
-                    return int(parts[3])
+                if line.startswith("Kernel sources:"):
+                    digest = line.split(":", 1)[1].strip().strip("`")
+                else:
+                    rows.append(line.split())
     except OSError:
         pass
+    return rows, digest
+
+
+TRAFFIC_PROFILE = "pmc_traffic_mul_relin_n16384_b512.md"
+VALU_PROFILE = "pmc_valu_mul_relin_n16384_b512.md"
+
+
+def load_traffic(kernel, digest):
+    """HBM bytes per launch of `kernel` (its name without the degree suffix)
+    from the committed counter summary (rows `<kernel> <fetch KiB> <write KiB>
+    <bytes>`), only when that summary was measured on the current sources;
+    otherwise None (a stale figure is never reported as this run's)."""
+    rows, d = load_profile(TRAFFIC_PROFILE)
+    if d != digest:
+        return None
+    for parts in rows:
+        if len(parts) == 4 and parts[0] == kernel and parts[3].isdigit():
+            return int(parts[3])
     return None
 
 
-def load_valu(kernel):
+def load_valu(kernel, digest):
     """(VALUBusy %, wave-instructions per CU-cycle) of `kernel` from the
-    committed rocprofv3 VALU counter summary, when present."""
-    path = os.path.join(ROOT, "profiles", "pmc_valu_mul_relin_n16384_b512.md")
+    committed VALU counter summary, when measured on the current sources."""
+    rows, d = load_profile(VALU_PROFILE)
+    if d != digest:
+        return None, None
+    for parts in rows:
+        if len(parts) >= 6 and parts[0] == kernel:
+            try:
+                return float(parts[1]), float(parts[5])
+            except ValueError:
+                return None, None
+    return None, None
+
+
+def profile_provenance(digest):
+    """Which counter summaries the roofline's traffic and VALU figures come
+    from, and whether they were measured on the sources of this run."""
+    out = {"kernel_sources": digest}
+    for key, name in (("traffic", TRAFFIC_PROFILE), ("valu", VALU_PROFILE)):
+        _, d = load_profile(name)
+        out[key] = {"file": "profiles/" + name, "measured_on": d, "current": d == digest}
+        if d != digest:
+            print(f"bench.py: profiles/{name} was measured on kernel sources {d}, not {digest}; "
+                  f"its figures are left out", file=sys.stderr)
+    return out
+
+
+def host_cpu():
+    """The host's CPU as the CPU baseline states it: usable cores, nproc and
+    lscpu's model fields."""
+    info = {"usable_cores": len(os.sched_getaffinity(0)), "nproc": os.cpu_count()}
     try:
-        with open(path) as f:
-            for line in f:
-                parts = line.split()
-                if len(parts) == 6 and parts[0] == kernel:
-                    return float(parts[1]), float(parts[5])
+        import subprocess
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k in ("Model name", "CPU family", "Model", "Vendor ID", "Thread(s) per core", "Core(s) per socket",
+                     "Socket(s)"):
+                info[k] = v.strip()
     except (OSError, ValueError):
         pass
-    return None, None
+    return info
+
+
+_ORACLE_NATIVE = None
+
+
+def oracle_module():
+    """The oracle (C++ restatement of the fhe.rs CPU path) for the CPU
+    baseline, compiled on this host with -O3 -march=native into a temporary
+    directory (the committed build is for a generic x86-64-v3 CPU); falls back
+    to that build if the compiler is unavailable. Returns (module, build)."""
+    global _ORACLE_NATIVE
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    if _ORACLE_NATIVE is None:
+        import subprocess
+        import tempfile
+        out = os.path.join(tempfile.mkdtemp(prefix="fhehip_oracle_"), "liboracle.so")
+        cmd = ["g++", "-O3", "-march=native", "-std=c++17", "-fPIC", "-shared", "-o", out,
+               os.path.join(ROOT, "oracle", "oracle_capi.cpp"), "-lpthread"]
+        try:
+            subprocess.run(cmd, check=True, timeout=600, capture_output=True)
+            pyoracle.LIB_PATH = out
+            _ORACLE_NATIVE = "g++ -O3 -march=native (built on this host)"
+        except (OSError, subprocess.SubprocessError):
+            _ORACLE_NATIVE = "g++ -O3 -march=x86-64-v3 (committed build; native build failed)"
+    pyoracle.lib()
+    return pyoracle, _ORACLE_NATIVE
 
 
 # Single-instance configuration: N = 8192, generate_moduli([62] * 3, 8192), t = 65537,
@@ -622,9 +712,8 @@ def pir_cpu_baseline(h):
     """CPU baseline of the PIR leg: the oracle (C++ restatement of the fhe.rs
     CPU path) recomputes the single-query response on one core, for a bit
     comparison with the GPU's and its time."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
-    import pyoracle as oracle
+    oracle, build = oracle_module()
     moduli, n, t, d1, d2 = h["moduli"], h["n"], h["t"], h["d1"], h["d2"]
     m1, m2 = moduli[:2], moduli[:1]
     L = len(m1)
@@ -655,15 +744,16 @@ def pir_cpu_baseline(h):
     cpu_s = time.perf_counter() - t0
     return {"bit_exact_vs_oracle": bool(np.array_equal(h["resp"], want)),
             "cpu_reference": {"seconds_per_response": cpu_s, "cores": 1, "kind": "port",
-                              "sample": "one response by the oracle (C++ restatement, -O3, one thread)"},
+                              "sample": f"one response by the oracle (C++ restatement, {build}, one thread)"},
             "gpu_over_cpu": cpu_s * 1e3 / h["gpu_ms"]}
 
 
 def cpu_baseline(lhs_h, rhs_h, c0, c1, gpu_out0, budget_s):
-    """Oracle (C++ restatement of the fhe.rs CPU path) on the host cores."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    """Oracle (C++ restatement of the fhe.rs CPU path) on every host core this
+    process may use, one independent Multiplicator::multiply per thread at a
+    time (fhe.rs itself is single-threaded per operation)."""
     import numpy as np
-    import pyoracle as oracle
+    oracle, build = oracle_module()
     bfv = oracle.Bfv(N, T_PLAIN, MODULI, sk_seed=1)
     rk = bfv.key_from_rows(c0, c1, ct_level=1, key_level=0)
     om = oracle.Multiplicator(bfv, rk, level=1)
@@ -673,7 +763,8 @@ def cpu_baseline(lhs_h, rhs_h, c0, c1, gpu_out0, budget_s):
     t0 = time.perf_counter()
     om.multiply(lhs_h, rhs_h)
     one = time.perf_counter() - t0
-    threads = max(1, min(16, os.cpu_count() or 1))
+    host = host_cpu()
+    threads = max(1, host["usable_cores"])
     reps = max(1, int(budget_s / max(one, 1e-3) / 1.5))
     wall, lat0 = om.bench(lhs_h, rhs_h, threads, reps)
     return {
@@ -681,9 +772,12 @@ def cpu_baseline(lhs_h, rhs_h, c0, c1, gpu_out0, budget_s):
         "unit": UNIT,
         "cores": threads,
         "kind": "port",
-        "sample": f"{threads} threads x {reps} Multiplicator::multiply each at the same shape "
-                  f"(oracle C++ restatement, -O3); single-op p50 {percentile(list(lat0 * 1e3), 50):.1f} ms",
+        "sample": f"{threads} threads (every core this process may use) x {reps} Multiplicator::multiply each "
+                  f"at the same shape (oracle C++ restatement, {build}); single-op p50 "
+                  f"{percentile(list(lat0 * 1e3), 50):.1f} ms",
         "p50_ms_1core": percentile(list(lat0 * 1e3), 50),
+        "build": build,
+        "host": host,
     }, bit_exact
 
 
@@ -698,7 +792,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the NTT autotuning and the secondary workloads (single instance, NTT, uneven, serving)")
-    ap.add_argument("--only", choices=["uneven", "serve", "pir", "codec"], default=None,
+    ap.add_argument("--only", choices=["uneven", "serve", "pir", "codec", "ntt"], default=None,
                     help="run only this secondary workload and print its JSON object")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="run one device copy of this many MiB (reference for rocprofv3 byte counters)")
@@ -712,6 +806,22 @@ def main():
 
     import fhehip
 
+    def copy_calibration():
+        """One plain device copy of --copy-calibration MiB, the reference
+        dispatch of the rocprofv3 byte counters (it runs last)."""
+        if not args.copy_calibration:
+            return None
+        src = torch.empty(args.copy_calibration << 17, dtype=torch.int64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        return args.copy_calibration << 20
+
+    if args.only == "ntt":
+        res = ntt_microbench(fhehip, torch, args.steps)
+        res["copy_calibration_bytes"] = copy_calibration()
+        print(json.dumps(res))
+        return
     if args.only == "uneven":
         print(json.dumps(uneven_batch(fhehip, torch)))
         return
@@ -853,12 +963,15 @@ def main():
     avg_s = dms / dcalls / 1e3
     abytes = kernel_bytes(dname, B, L, Lk, M)
     achieved = abytes / avg_s / 1e9 if abytes else None
-    traffic = load_traffic(dname.split("/")[0])
-    valu_busy, valu_rate = load_valu(dname.split("/")[0])
+    digest = kernel_sources_digest()
+    provenance = profile_provenance(digest)
+    traffic = load_traffic(dname.split("/")[0], digest)
+    valu_busy, valu_rate = load_valu(dname.split("/")[0], digest)
     roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                 "kernel": dname, "avg_launch_ms": avg_s * 1e3, "algorithmic_bytes": abytes,
-                "valu_busy_pct": valu_busy, "valu_wave_instructions_per_cu_cycle": valu_rate}
+                "valu_busy_pct": valu_busy, "valu_wave_instructions_per_cu_cycle": valu_rate,
+                "counter_profiles": provenance}
 
     # The same figures for every kernel of the step (HBM-bound accounting; the
     # VALU columns come from the committed counter summary).
@@ -866,11 +979,11 @@ def main():
     for kname, (kc, kms) in sorted(kernels.items()):
         kb = kernel_bytes(kname, B, L, Lk, M)
         ks = kms / kc / 1e3
-        vb, vr = load_valu(kname.split("/")[0])
+        vb, vr = load_valu(kname.split("/")[0], digest)
         kernel_roofline[kname] = {"avg_launch_ms": ks * 1e3, "algorithmic_bytes": kb,
                                   "achieved_GBps": kb / ks / 1e9 if kb else None,
                                   "hbm_frac": kb / ks / 1e9 / HBM_PEAK_GBS if kb else None,
-                                  "traffic": load_traffic(kname.split("/")[0]), "valu_busy_pct": vb}
+                                  "traffic": load_traffic(kname.split("/")[0], digest), "valu_busy_pct": vb}
 
     extras = {}
     pir_handover = None
@@ -882,11 +995,7 @@ def main():
         extras["pir_server"], pir_handover = pir_server(fhehip, torch, check_cpu=not args.no_cpu)
         extras["codec"] = codec_throughput(fhehip, torch)
     if args.copy_calibration:
-        src = torch.empty(args.copy_calibration << 17, dtype=torch.int64, device=dev)
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        extras["copy_calibration_bytes"] = args.copy_calibration << 20
+        extras["copy_calibration_bytes"] = copy_calibration()
 
     cpu = None
     bit_exact = None
