@@ -565,6 +565,76 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out_final, u32
 }
 
 // ---------------------------------------------------------------------------
+// BfvParametersBuilder::build (fhe/src/bfv/parameters.rs:431-641).
+Parameters::Parameters(u32 degree, u64 plaintext, const std::vector<u64>& m) : N(degree), t(plaintext), moduli(m) {
+    if (degree < 8 || (degree & (degree - 1))) throw Error(-1, "the degree must be a power of two, at least 8");
+    if (moduli.empty()) throw Error(-1, "no ciphertext moduli");
+    if (t < 2 || (t >> 62)) throw Error(-2, "plaintext modulus out of range (2 .. 2^62)");
+    for (u64 q : moduli)
+        if (t % q == 0) throw Error(-1, "the plaintext modulus is not invertible modulo a ciphertext modulus");
+    // plaintext context: the first moduli whose sizes reach log2(t) + 60 bits (parameters.rs:479-494)
+    const u32 t_bits = 64 - __builtin_clzll(t);
+    u32 count = 0, acc = 0;
+    for (u64 q : moduli) {
+        acc += 64 - __builtin_clzll(q);
+        count++;
+        if (acc >= t_bits + 60) break;
+    }
+    plain = make_context(std::vector<u64>(moduli.begin(), moduli.begin() + count), N, nullptr);
+    // L + 1 62-bit primes = 1 mod 2N, none of the moduli (parameters.rs:585-593)
+    u64 ub = (u64)1 << 62;
+    while (extension.size() != moduli.size() + 1) {
+        ub = generate_prime64(62, 2 * (u64)N, ub);
+        if (!ub) throw Error(-2, "not enough 62-bit primes for the multiplication basis");
+        if (std::find(extension.begin(), extension.end(), ub) == extension.end() &&
+            std::find(moduli.begin(), moduli.end(), ub) == moduli.end())
+            extension.push_back(ub);
+    }
+    for (size_t lvl = 0; lvl < moduli.size(); lvl++) {
+        std::vector<u64> lm(moduli.begin(), moduli.end() - lvl);
+        levels.push_back(make_context(lm, N, nullptr));
+        size_t bits = 0;
+        for (u64 q : lm) bits += 64 - __builtin_clzll(q);
+        std::vector<u64> mm = lm;
+        mm.insert(mm.end(), extension.begin(), extension.begin() + (bits + 60 + 61) / 62);
+        mul.emplace_back(new Multiplicator(levels.back(), mm, BigUint(1), BigUint(1), BigUint(1), BigUint(1),
+                                           BigUint(t), levels.back()->rns.product));
+    }
+}
+
+const Multiplicator& Parameters::at(u32 level) const {
+    if (level >= mul.size())
+        throw Error(-3, "invalid level " + std::to_string(level) + " (the maximum is " +
+                            std::to_string(mul.size() - 1) + ")");
+    return *mul[level];
+}
+
+void Parameters::ct_mul(u32 level, const u64* lhs, u32 lp, const u64* rhs, u32 rp, u64* out, u32 num,
+                        hipStream_t s) {
+    at(level);
+    Multiplicator& m = *mul[level];
+    if (!lp || !rp) throw Error(-1, "empty ciphertext");
+    if (lp + rp - 1 > 16) throw Error(-2, "products of more than 16 parts are not supported");
+    const u32 parts = lp + rp - 1;
+    check_batch(num * std::max(parts, std::max(lp, rp)));
+    if (!num) return;
+    if (lp == 2 && rp == 2) {
+        m.multiply(lhs, rhs, out, num, s);
+        return;
+    }
+    // every part extended (Poly::scale in the Ntt representation), products in
+    // the multiplication context, every product part scaled down
+    const size_t n = N, M = m.mul->L;
+    u64* wa = scratch(Scratch::Mul).get(s, (size_t)num * (lp + rp + parts) * M * n * 8).as<u64>();
+    u64* wb = wa + (size_t)num * lp * M * n;
+    u64* wc = wb + (size_t)num * rp * M * n;
+    m.ext_lhs->scale(lhs, wa, num * lp, true, s);
+    m.ext_rhs->scale(rhs, wb, num * rp, true, s);
+    launch_tensor_parts(wa, lp, wb, rp, wc, m.mul->mod.as<ModInfo>(), (u32)M, m.mul->logn, num, s);
+    m.down->scale(wc, out, num * parts, true, s);
+}
+
+// ---------------------------------------------------------------------------
 Galois::Galois(std::shared_ptr<Ksk> k, u32 exp) : ksk(std::move(k)) {
     const u32 n = ksk->ct_ctx->N;
     exponent = exp % (2 * n);

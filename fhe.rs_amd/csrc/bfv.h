@@ -164,6 +164,27 @@ struct Multiplicator {
     size_t scratch_bytes(u32 inst) const;
 };
 
+// BfvParameters (fhe/src/bfv/parameters.rs:431-641) on the device: the level
+// chain (context_at_level(l) holds the first L - l moduli), the plaintext
+// context and each level's MultiplicationParameters (parameters.rs:585-612,
+// 694-717): the level's moduli extended by the first ceil((bits + 60) / 62)
+// of L + 1 generated 62-bit primes, an extender of factor one and a t / Q_l
+// down-scaler, held as a Multiplicator without relinearization.
+struct Parameters {
+    u32 N = 0;
+    u64 t = 0;
+    std::vector<u64> moduli, extension;
+    std::vector<std::shared_ptr<DeviceContext>> levels;
+    std::shared_ptr<DeviceContext> plain;
+    std::vector<std::unique_ptr<Multiplicator>> mul;
+    Parameters(u32 degree, u64 plaintext, const std::vector<u64>& moduli);
+    const Multiplicator& at(u32 level) const;
+    // Mul<&Ciphertext> for &Ciphertext (fhe/src/bfv/ops/mod.rs:259-342) at
+    // `level` on num pairs: lhs [num][lp][L][N], rhs [num][rp][L][N] (NTT) ->
+    // out [num][lp + rp - 1][L][N].
+    void ct_mul(u32 level, const u64* lhs, u32 lp, const u64* rhs, u32 rp, u64* out, u32 num, hipStream_t s);
+};
+
 // fhehip_multiply with all scratch in the library's fixed device region
 // (no device allocation); refuses batches whose scratch does not fit.
 void multiply_noalloc(Multiplicator& m, const u64* lhs, const u64* rhs, u64* out, u32 inst, hipStream_t s);

@@ -62,6 +62,40 @@ __global__ __launch_bounds__(256) void tensor_ew_kernel(TensorArgs a, u32 M, u32
         reduce_u128(lo, hi, mi.barrett_lo, mi.barrett_hi, mi.p);
 }
 
+// grid = (N/256, M, num); each thread forms every output part of its
+// coefficient. At most 8 products meet in one part (lp + rp - 1 <= 16), each
+// below q^2 < 2^124, so the 128-bit sum is exact.
+__global__ __launch_bounds__(256) void tensor_parts_kernel(const u64* __restrict__ a, u32 lp, const u64* __restrict__ b,
+                                                           u32 rp, u64* __restrict__ c,
+                                                           const ModInfo* __restrict__ mods, u32 M, u32 logn) {
+    const u32 e = blockIdx.x * 256 + threadIdx.x;
+    if (e >> logn) return;
+    const u32 m = blockIdx.y, inst = blockIdx.z;
+    const ModInfo mi = mods[m];
+    const size_t row = (size_t)M << logn, off = ((size_t)m << logn) + e;
+    const u64* ab = a + (size_t)inst * lp * row + off;
+    const u64* bb = b + (size_t)inst * rp * row + off;
+    u64* cb = c + (size_t)inst * (lp + rp - 1) * row + off;
+    for (u32 k = 0; k + 1 < lp + rp; k++) {
+        u64 lo = 0, hi = 0;
+        for (u32 i = k + 1 > rp ? k + 1 - rp : 0; i < lp && i <= k; i++) {
+            u64 l2, h2;
+            mul_wide(ab[i * row], bb[(k - i) * row], l2, h2);
+            add128(lo, hi, l2, h2);
+        }
+        cb[k * row] = reduce_u128(lo, hi, mi.barrett_lo, mi.barrett_hi, mi.p);
+    }
+}
+
+void launch_tensor_parts(const u64* a, u32 lp, const u64* b, u32 rp, u64* c, const ModInfo* mods, u32 M, int logn,
+                         u32 num, hipStream_t s) {
+    if (!num || !M) return;
+    ProfScope ps("tensor_parts", logn, s);
+    hipLaunchKernelGGL(tensor_parts_kernel, ew_grid(1u << logn, M, num), dim3(256), 0, s, a, lp, b, rp, c, mods, M,
+                       (u32)logn);
+    HIPCHK(hipGetLastError());
+}
+
 void launch_tensor_ew(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_t s) {
     if (!M || !inst) return;
     ProfScope ps("tensor_ew", logn, s);

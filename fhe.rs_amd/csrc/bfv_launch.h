@@ -94,6 +94,11 @@ void launch_ks_finish(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s);
 void launch_add_rows(const u64* a, const u64* b, u64* out, const ModInfo* mods, u32 L, int logn, size_t rows,
                      hipStream_t s);
 
+// NTT-domain products of ciphertexts of any part counts in the
+// multiplication context: c[b][k] = sum over i + j = k of a[b][i] * b[b][j]
+// (exact), a [num][lp][M][N], b [num][rp][M][N], c [num][lp + rp - 1][M][N].
+void launch_tensor_parts(const u64* a, u32 lp, const u64* b, u32 rp, u64* c, const ModInfo* mods, u32 M, int logn,
+                         u32 num, hipStream_t s);
 // Elementwise kernels of the unfused pipeline (bfv_generic.hip), used where a
 // row does not fit one workgroup's LDS (N = 32768).
 void launch_gather_rows(const u64* src, size_t src_inst, u64* dst, size_t dst_inst, const u32* perm, int logn,

@@ -21,6 +21,9 @@ struct fhehip_ksk {
 struct fhehip_multiplicator {
     std::unique_ptr<Multiplicator> m;
 };
+struct fhehip_parameters {
+    std::unique_ptr<Parameters> p;
+};
 struct fhehip_galois {
     std::shared_ptr<Galois> g;
 };
@@ -260,6 +263,63 @@ int fhehip_multiply(fhehip_multiplicator* m, const uint64_t* d_lhs, const uint64
     FHEHIP_API_BEGIN
     if (!m || ((!d_lhs || !d_rhs || !d_out) && num)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
     m->m->multiply(d_lhs, d_rhs, d_out, num, S(stream));
+    FHEHIP_API_END
+}
+
+int fhehip_parameters_create(uint32_t degree, uint64_t plaintext, const uint64_t* moduli, uint32_t num_moduli,
+                             fhehip_parameters** out) {
+    FHEHIP_API_BEGIN
+    if (!moduli || !out || !num_moduli) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    std::unique_ptr<Parameters> p(new Parameters(degree, plaintext, std::vector<u64>(moduli, moduli + num_moduli)));
+    *out = new fhehip_parameters{std::move(p)};
+    FHEHIP_API_END
+}
+
+void fhehip_parameters_destroy(fhehip_parameters* p) { delete p; }
+
+int fhehip_parameters_info(const fhehip_parameters* p, uint32_t* max_level, uint32_t* degree, uint64_t* plaintext) {
+    FHEHIP_API_BEGIN
+    if (!p) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    if (max_level) *max_level = (uint32_t)p->p->levels.size() - 1;
+    if (degree) *degree = p->p->N;
+    if (plaintext) *plaintext = p->p->t;
+    FHEHIP_API_END
+}
+
+int fhehip_parameters_context(const fhehip_parameters* p, uint32_t level, fhehip_context** out) {
+    FHEHIP_API_BEGIN
+    if (!p || !out) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    p->p->at(level);
+    *out = new fhehip_context{p->p->levels[level]};
+    FHEHIP_API_END
+}
+
+int fhehip_parameters_plaintext_context(const fhehip_parameters* p, fhehip_context** out) {
+    FHEHIP_API_BEGIN
+    if (!p || !out) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *out = new fhehip_context{p->p->plain};
+    FHEHIP_API_END
+}
+
+int fhehip_parameters_mul_basis(const fhehip_parameters* p, uint32_t level, uint64_t* out, uint32_t* len) {
+    FHEHIP_API_BEGIN
+    if (!p || !len) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    const auto& mod = p->p->at(level).mul->moduli;
+    if (out) {
+        if (*len < mod.size()) return fail(FHEHIP_ERR_INVALID_ARG, "output too small");
+        std::memcpy(out, mod.data(), mod.size() * 8);
+    }
+    *len = (uint32_t)mod.size();
+    FHEHIP_API_END
+}
+
+int fhehip_ciphertext_mul(fhehip_parameters* p, uint32_t level, const uint64_t* d_lhs, uint32_t lhs_parts,
+                          const uint64_t* d_rhs, uint32_t rhs_parts, uint64_t* d_out, uint32_t num, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!p || ((!d_lhs || !d_rhs || !d_out) && num)) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    p->p->ct_mul(level, d_lhs, lhs_parts, d_rhs, rhs_parts, d_out, num, S(stream));
     FHEHIP_API_END
 }
 

@@ -54,6 +54,13 @@ SIGNATURES = {
     "fhehip_multiplicator_extended_basis": (C.c_int, [_vp, _u64p, _u32p]),
     "fhehip_multiply": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
     "fhehip_multiply_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_parameters_create": (C.c_int, [C.c_uint32, C.c_uint64, _u64p, C.c_uint32, _vpp]),
+    "fhehip_parameters_destroy": (None, [_vp]),
+    "fhehip_parameters_info": (C.c_int, [_vp, _u32p, _u32p, _u64p]),
+    "fhehip_parameters_context": (C.c_int, [_vp, C.c_uint32, _vpp]),
+    "fhehip_parameters_plaintext_context": (C.c_int, [_vp, _vpp]),
+    "fhehip_parameters_mul_basis": (C.c_int, [_vp, C.c_uint32, _u64p, _u32p]),
+    "fhehip_ciphertext_mul": (C.c_int, [_vp, C.c_uint32, _vp, C.c_uint32, _vp, C.c_uint32, _vp, C.c_uint32, _vp]),
     "fhehip_galois_create": (C.c_int, [_vp, C.c_uint32, _vpp]),
     "fhehip_galois_destroy": (None, [_vp]),
     "fhehip_galois_relinearize": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _vp]),
@@ -328,6 +335,53 @@ class Multiplicator:
         """Multiplicator::enable_mod_switching: products land in next_ctx."""
         self.next_ctx = next_ctx
         check(lib().fhehip_multiplicator_enable_mod_switching(self.h, next_ctx.h))
+
+
+class Parameters:
+    """BfvParameters: level chain and per-level multiplication parameters."""
+
+    def __init__(self, degree, plaintext, moduli):
+        self.moduli = [int(q) for q in moduli]
+        h = C.c_void_p()
+        check(lib().fhehip_parameters_create(degree, plaintext, _arr64(self.moduli), len(self.moduli), C.byref(h)))
+        self.h = h
+        mx, n, t = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        check(lib().fhehip_parameters_info(self.h, C.byref(mx), C.byref(n), C.byref(t)))
+        self.max_level, self.degree, self.plaintext = mx.value, n.value, t.value
+
+    def __del__(self):
+        _destroy("fhehip_parameters_destroy", getattr(self, "h", None))
+
+    def _ctx(self, h):
+        """A Context object over a handle the library returned (every context
+        of the parameters holds a prefix of their moduli)."""
+        c = Context.__new__(Context)
+        c.h = h
+        L, n = C.c_uint32(), C.c_uint32()
+        check(lib().fhehip_context_info(h, C.byref(L), C.byref(n)))
+        c.moduli = self.moduli[:L.value]
+        c.degree = n.value
+        return c
+
+    def context(self, level):
+        h = C.c_void_p()
+        check(lib().fhehip_parameters_context(self.h, level, C.byref(h)))
+        return self._ctx(h)
+
+    def plaintext_context(self):
+        h = C.c_void_p()
+        check(lib().fhehip_parameters_plaintext_context(self.h, C.byref(h)))
+        return self._ctx(h)
+
+    def mul_basis(self, level):
+        n = C.c_uint32(128)
+        out = (C.c_uint64 * 128)()
+        check(lib().fhehip_parameters_mul_basis(self.h, level, out, C.byref(n)))
+        return [int(out[i]) for i in range(n.value)]
+
+    def ct_mul(self, level, d_lhs, lhs_parts, d_rhs, rhs_parts, d_out, num, stream=None):
+        check(lib().fhehip_ciphertext_mul(self.h, level, dptr(d_lhs), lhs_parts, dptr(d_rhs), rhs_parts, dptr(d_out),
+                                          num, stream_ptr(stream)))
 
 
 class Switcher:

@@ -163,6 +163,39 @@ int fhehip_multiplicator_extended_basis(const fhehip_multiplicator* m, uint64_t*
 int fhehip_multiply(fhehip_multiplicator* m, const uint64_t* d_lhs, const uint64_t* d_rhs, uint64_t* d_out,
                     uint32_t num, void* stream);
 
+/* ---- Parameters ---------------------------------------------------------------
+ * BfvParameters (fhe/src/bfv/parameters.rs:431-641) built from the degree
+ * (a power of two, 8 .. 32768), a plaintext modulus t (2 <= t < 2^62,
+ * invertible modulo every ciphertext modulus) and the ciphertext moduli (see
+ * fhehip_generate_moduli for BfvParametersBuilder::set_moduli_sizes): the
+ * level chain (level l holds the first L - l moduli, max_level = L - 1), the
+ * plaintext context (the first moduli whose sizes reach log2(t) + 60 bits)
+ * and each level's MultiplicationParameters (parameters.rs:585-612): the
+ * level's moduli followed by the first ceil((bits + 60) / 62) of L + 1
+ * generated 62-bit primes, an extender of factor one and a t / Q_l
+ * down-scaler. */
+typedef struct fhehip_parameters fhehip_parameters;
+int fhehip_parameters_create(uint32_t degree, uint64_t plaintext, const uint64_t* moduli, uint32_t num_moduli,
+                             fhehip_parameters** out);
+void fhehip_parameters_destroy(fhehip_parameters* p);
+/* BfvParameters::max_level, degree and plaintext (parameters.rs:125-154). */
+int fhehip_parameters_info(const fhehip_parameters* p, uint32_t* max_level, uint32_t* degree, uint64_t* plaintext);
+/* BfvParameters::context_at_level (parameters.rs:155-178): a new context
+ * handle sharing the level's tables (destroy it with fhehip_context_destroy);
+ * FHEHIP_ERR_BAD_LEVEL above max_level. */
+int fhehip_parameters_context(const fhehip_parameters* p, uint32_t level, fhehip_context** out);
+/* The plaintext context (the codec's plain_ctx), a new handle likewise. */
+int fhehip_parameters_plaintext_context(const fhehip_parameters* p, fhehip_context** out);
+/* Moduli of the level's multiplication context; *len is in/out. */
+int fhehip_parameters_mul_basis(const fhehip_parameters* p, uint32_t level, uint64_t* out, uint32_t* len);
+/* Mul<&Ciphertext> for &Ciphertext (fhe/src/bfv/ops/mod.rs:259-342) at
+ * `level` with that level's MultiplicationParameters, on num pairs: d_lhs
+ * [num][lhs_parts][L][N] and d_rhs [num][rhs_parts][L][N] (NTT domain) ->
+ * d_out [num][lhs_parts + rhs_parts - 1][L][N], no relinearization; d_lhs
+ * may equal d_rhs (squaring). At most 16 output parts. */
+int fhehip_ciphertext_mul(fhehip_parameters* p, uint32_t level, const uint64_t* d_lhs, uint32_t lhs_parts,
+                          const uint64_t* d_rhs, uint32_t rhs_parts, uint64_t* d_out, uint32_t num, void* stream);
+
 /* No fhe.rs counterpart. For callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid
  * hipMalloc): fhehip_multiply with every scratch buffer carved from one fixed

@@ -486,6 +486,45 @@ int or_ct_mul(void* h, uint32_t level, const uint64_t* lhs, const uint64_t* rhs,
         std::copy(d2.begin(), d2.end(), out + 2 * sz);
     })
 }
+// Mul<&Ciphertext> for &Ciphertext (fhe/src/bfv/ops/mod.rs:259-342) for any
+// part counts: every part scaled by the level's extender (NTT representation),
+// c[i + j] += a_i * b_j in the multiplication context, every product part
+// scaled back by the down-scaler. out: [lp + rp - 1][L][N].
+int or_ct_mul_parts(void* h, uint32_t level, const uint64_t* lhs, uint32_t lp, const uint64_t* rhs, uint32_t rp,
+                    uint64_t* out) {
+    OR_TRY({
+        OrBfv& b = *(OrBfv*)h;
+        if (!lp || !rp) throw std::runtime_error("empty ciphertext");
+        const Level& L = b.par->levels.at(level);
+        const Context& c = *L.ctx;
+        const Context& m = *L.mp.to;
+        const size_t sz = c.L() * c.degree;
+        std::vector<PolyData> a, bb;
+        for (uint32_t i = 0; i < lp; i++) a.push_back(L.mp.extender->scale(lhs + i * sz, true));
+        for (uint32_t j = 0; j < rp; j++) bb.push_back(L.mp.extender->scale(rhs + j * sz, true));
+        std::vector<PolyData> cc(lp + rp - 1, PolyData(m.L() * m.degree, 0));
+        for (uint32_t i = 0; i < lp; i++)
+            for (uint32_t j = 0; j < rp; j++) {
+                PolyData t = a[i];
+                poly_mul(m, t.data(), bb[j].data());
+                poly_add(m, cc[i + j].data(), t.data());
+            }
+        for (size_t k = 0; k < cc.size(); k++) {
+            PolyData d = L.mp.down_scaler->scale(cc[k].data(), true);
+            std::copy(d.begin(), d.end(), out + k * sz);
+        }
+    })
+}
+// The multiplication context moduli of a level (MultiplicationParameters,
+// fhe/src/bfv/parameters.rs:585-612).
+int or_bfv_mul_moduli(void* h, uint32_t level, uint64_t* out, uint32_t* len) {
+    OR_TRY({
+        const Context& m = *((OrBfv*)h)->par->levels.at(level).mp.to;
+        if (*len < m.L()) throw std::runtime_error("buffer too small");
+        std::copy(m.moduli.begin(), m.moduli.end(), out);
+        *len = (uint32_t)m.L();
+    })
+}
 int or_relinearize(void* h, void* rk, const uint64_t* ct3, uint64_t* out) {
     OR_TRY({
         OrBfv& b = *(OrBfv*)h;

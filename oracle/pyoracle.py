@@ -85,6 +85,8 @@ def lib():
             "or_mult_multiply": (C.c_int, [_vp, _u64p, _u64p, _u64p, C.c_uint32]),
             "or_ct_mul": (C.c_int, [_vp, C.c_uint32, _u64p, _u64p, _u64p]),
             "or_relinearize": (C.c_int, [_vp, _vp, _u64p, _u64p]),
+            "or_ct_mul_parts": (C.c_int, [_vp, C.c_uint32, _u64p, C.c_uint32, _u64p, C.c_uint32, _u64p]),
+            "or_bfv_mul_moduli": (C.c_int, [_vp, C.c_uint32, _u64p, _u32p]),
             "or_galois": (C.c_int, [_vp, _vp, C.c_uint32, _u64p, _u64p, C.c_uint32]),
             "or_random_polys": (C.c_int, [_u64p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, _u64p]),
             "or_bench_mul_relin": (C.c_int, [_vp, _u64p, _u64p, C.c_uint32, C.c_uint32, _dblp, _dblp]),
@@ -502,6 +504,22 @@ class Bfv:
         out = np.zeros(3 * L * self.degree, np.uint64)
         _chk(lib().or_ct_mul(self.h, level, _p64(u64(lhs)), _p64(u64(rhs)), _p64(out)))
         return out
+
+    def ct_mul_parts(self, lhs, rhs, level=0):
+        """&ct * &ct (ops/mod.rs:259-342) for ciphertexts of any part counts."""
+        lhs, rhs = u64(lhs), u64(rhs)
+        w = len(self.level_moduli(level)) * self.degree
+        lp, rp = lhs.size // w, rhs.size // w
+        out = np.zeros((lp + rp - 1) * w, np.uint64)
+        _chk(lib().or_ct_mul_parts(self.h, level, _p64(lhs), lp, _p64(rhs), rp, _p64(out)))
+        return out
+
+    def mul_moduli(self, level=0):
+        """Moduli of the level's multiplication context (parameters.rs:585-612)."""
+        out = np.zeros(128, np.uint64)
+        n = np.array([128], np.uint32)
+        _chk(lib().or_bfv_mul_moduli(self.h, level, _p64(out), _p32(n)))
+        return [int(x) for x in out[:int(n[0])]]
 
     def relinearize(self, rk, ct3):
         L = len(self.level_moduli(rk.ct_level))

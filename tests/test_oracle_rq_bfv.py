@@ -298,3 +298,46 @@ def test_rgsw_external_product_decrypts(oracle, L, level):
     rgsw = bfv.rgsw_encrypt(v2, level=level)
     prod = bfv.rgsw_product(rgsw, ct1)
     assert [int(x) for x in bfv.decrypt(prod, level=level)] == [(a * b) % t for a, b in zip(v1, v2)]
+
+
+@pytest.mark.parametrize("L,level", [(4, 0), (4, 1)])
+def test_ct_mul_any_parts_decrypts(oracle, L, level):
+    """&ct * &ct for any part counts (fhe/src/bfv/ops/mod.rs:259-342): a
+    3-part product times a 2-part ciphertext (4 parts), a square, and a
+    1-part trivial encryption times a ciphertext decrypt to the slot
+    products; the 2 x 2 case equals the fixed-shape path."""
+    import random
+    n, t = 64, 257
+    bfv = oracle.Bfv(n, t, oracle.generate_moduli([62] * L, n), sk_seed=L + level)
+    rng = random.Random(level)
+    v1, v2, v3 = ([rng.randrange(t) for _ in range(n)] for _ in range(3))
+    c1, c2, c3 = (bfv.encrypt(v, level=level, seed=s) for v, s in ((v1, 1), (v2, 2), (v3, 3)))
+    p12 = bfv.ct_mul_parts(c1, c2, level)
+    assert np.array_equal(p12, bfv.ct_mul(c1, c2, level))
+    p123 = bfv.ct_mul_parts(p12, c3, level)
+    assert [int(x) for x in bfv.decrypt(p123, level=level)] == [(a * b * c) % t for a, b, c in zip(v1, v2, v3)]
+    sq = bfv.ct_mul_parts(c1, c1, level)
+    assert [int(x) for x in bfv.decrypt(sq, level=level)] == [(a * a) % t for a in v1]
+    triv = bfv.encode(v3, level=level, scaled=True)   # the one-part ciphertext (delta m)
+    assert [int(x) for x in bfv.decrypt(triv, level=level)] == v3
+    p = bfv.ct_mul_parts(triv, c2, level)
+    assert [int(x) for x in bfv.decrypt(p, level=level)] == [(a * b) % t for a, b in zip(v3, v2)]
+
+
+def test_level_multiplication_basis(oracle):
+    """MultiplicationParameters (fhe/src/bfv/parameters.rs:585-612): at level
+    l the multiplication context is the level's moduli followed by the first
+    ceil((bits of the level's moduli + 60) / 62) of L + 1 generated 62-bit
+    primes (= 1 mod 2N, none of the moduli); the same list at every level."""
+    n = 256
+    moduli = oracle.generate_moduli([62, 50, 62, 55], n)
+    bfv = oracle.Bfv(n, 65537, moduli, sk_seed=1)
+    ext, ub = [], 1 << 62
+    while len(ext) != len(moduli) + 1:
+        ub = oracle.generate_prime(62, 2 * n, ub)
+        if ub not in ext and ub not in moduli:
+            ext.append(ub)
+    for level in range(len(moduli)):
+        lm = moduli[:len(moduli) - level]
+        k = -(-(sum(int(q).bit_length() for q in lm) + 60) // 62)
+        assert bfv.mul_moduli(level) == [int(q) for q in lm] + ext[:k]
