@@ -69,13 +69,12 @@ static void intt_rows(const DeviceContext& c, const InttRowsArgs& a, u32 L, u32 
         launch_intt_rows(c.logn, a, L, parts, inst, s);
         return;
     }
+    // Degree 32768: half-row inverse transforms, then the stage pairing the halves.
     const size_t rows = (size_t)L * c.N;
     if (a.dst_inst_stride != parts * rows || (parts > 1 && a.dst_part_stride != rows))
-        throw Error(-1, "the unfused inverse NTT needs contiguous destination rows");
-    for (u32 k = 0; k < parts; k++)
-        launch_gather_rows(a.src[k], a.src_inst_stride, a.dst + k * a.dst_part_stride, a.dst_inst_stride, a.perm,
-                           c.logn, L, inst, s);
-    launch_ntt_backward(c, RowMap{a.dst, L, L, 0, a.table_offset}, inst * parts * L, s);
+        throw Error(-1, "the degree-32768 inverse NTT needs contiguous destination rows");
+    launch_intt_rows_half(c, a, L, parts, inst, s);
+    launch_ntt_split_inv(c, RowMap{a.dst, L, L, 0, a.table_offset}, inst * parts * L, s);
 }
 
 // Tensor product followed by the inverse NTT of the three output polynomials.
@@ -85,9 +84,9 @@ static void tensor_intt(const DeviceContext& mul, const TensorArgs& a, u32 M, u3
         return;
     }
     if (a.out_inst_stride != 3 * (size_t)M * mul.N || a.out_part_stride != (size_t)M * mul.N)
-        throw Error(-1, "the unfused tensor product needs contiguous output rows");
-    launch_tensor_ew(mul.logn, a, M, inst, s);
-    launch_ntt_backward(mul, RowMap{a.out, M, M, 0, 0}, inst * 3 * M, s);
+        throw Error(-1, "the degree-32768 tensor product needs contiguous output rows");
+    launch_tensor_intt_half(mul, a, M, inst, s);
+    launch_ntt_split_inv(mul, RowMap{a.out, M, M, 0, 0}, inst * 3 * M, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -232,12 +231,11 @@ KeySwitchArgs Ksk::args(const u64* src, size_t src_inst_stride) const {
 size_t Ksk::lift_bytes(u32 inst) const { return (size_t)inst * digits * key_ctx->L * key_ctx->N * 8; }
 
 // [inst][2][L'][N] power-basis key-switch rows (+ [inst][2][L][N] at degree
-// 32768) for a deep key; [inst][2][N] for the dropped limb (+ the same
-// [inst][2][L][N] at degree 32768) otherwise.
+// 32768) for a deep key; [inst][2][N] for the dropped limb otherwise.
 size_t Ksk::finish_bytes(u32 inst) const {
     const size_t n = ct_ctx->N, L = ct_ctx->L, Lk = key_ctx->L;
     const size_t unfused = fused(*ct_ctx) ? 0 : L;
-    return (size_t)inst * 2 * ((extra > 1 ? Lk : extra) + unfused) * n * 8;
+    return (size_t)inst * 2 * (extra > 1 ? Lk + unfused : extra) * n * 8;
 }
 
 u64* Ksk::lift(const u64* src, size_t src_inst_stride, u32 inst, hipStream_t s) const {
@@ -331,15 +329,10 @@ void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* out, u3
                    key_ctx->mod.as<ModInfo>() + (Lk - 1)};
     launch_ks_mac(ct_ctx->logn, f, inst, s);
     if (extra) launch_ntt_backward(*key_ctx, RowMap{special, 1, 1, 0, (u32)(Lk - 1)}, inst * 2, s);
-    if (fuse) {
+    if (fuse)
         launch_ks_finish(ct_ctx->logn, f, inst, s);
-        return;
-    }
-    // Degree 32768: the power-basis part z, its NTT, then out += NTT(z).
-    u64* x = base + (size_t)inst * 2 * extra * n;
-    launch_sd_z(ct_ctx->logn, f, x, inst, s);
-    launch_ntt_forward(*ct_ctx, RowMap{x, (u32)L, (u32)L, 0, 0}, inst * 2 * (u32)L, false, s);
-    launch_add_rows(out, x, out, ct_ctx->mod.as<ModInfo>(), (u32)L, ct_ctx->logn, (size_t)inst * 2 * L, s);
+    else
+        launch_ks_finish_half(f, *ct_ctx, inst, s);
 }
 
 // KeySwitchingKey::key_switch (key_switching_key.rs:214-241): power-basis p in

@@ -1,66 +1,12 @@
-// Elementwise kernels of the unfused BFV pipeline. The fused kernels keep a
-// whole row in one workgroup's LDS, which holds up to N = 16384; at N = 32768
-// the same operations run as these elementwise passes around the split NTT
-// (ntt_kernels.hip). One thread per coefficient; grid.y / grid.z index rows.
+// Elementwise kernels of the BFV path: products of any part counts, the key
+// inner product and switch-down steps of deep keys, and job staging. One
+// thread per coefficient; grid.y / grid.z index rows.
 #include "bfv_launch.h"
 #include "profiler.h"
 
 namespace fhehip {
 
 static inline dim3 ew_grid(u32 n, u32 y, u32 z) { return dim3((n + 255) / 256, y, z); }
-
-// dst[b][r][m] = src[b][r][perm[m]]  (Poly::substitute, NTT representation,
-// math/src/rq/mod.rs:324-346), or a plain copy without perm.
-// grid = (N/256, rows, instances)
-__global__ __launch_bounds__(256) void gather_rows_kernel(const u64* __restrict__ src, size_t src_inst,
-                                                          u64* __restrict__ dst, size_t dst_inst,
-                                                          const u32* __restrict__ perm, u32 logn) {
-    const u32 m = blockIdx.x * 256 + threadIdx.x;
-    if (m >> logn) return;
-    const size_t r = (size_t)blockIdx.y << logn;
-    dst[blockIdx.z * dst_inst + r + m] = src[blockIdx.z * src_inst + r + (perm ? perm[m] : m)];
-}
-
-void launch_gather_rows(const u64* src, size_t src_inst, u64* dst, size_t dst_inst, const u32* perm, int logn,
-                        u32 rows, u32 inst, hipStream_t s) {
-    if (!rows || !inst) return;
-    ProfScope ps("gather_rows", logn, s);
-    hipLaunchKernelGGL(gather_rows_kernel, ew_grid(1u << logn, rows, inst), dim3(256), 0, s, src, src_inst, dst,
-                       dst_inst, perm, (u32)logn);
-    HIPCHK(hipGetLastError());
-}
-
-// Exact NTT-domain tensor product (mul.rs:181-185) into [b][3][M][N].
-// grid = (N/256, M * 3, instances)
-__global__ __launch_bounds__(256) void tensor_ew_kernel(TensorArgs a, u32 M, u32 logn) {
-    const u32 e = blockIdx.x * 256 + threadIdx.x;
-    if (e >> logn) return;
-    const u32 m = blockIdx.y % M, c = blockIdx.y / M, b = blockIdx.z;
-    const ModInfo mi = a.mods[m];
-    const u64* rows[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const u32 common = k < 2 ? a.common_lhs : a.common_rhs;
-        if (m < common) {
-            const u64* base = k < 2 ? a.lhs : a.rhs;
-            rows[k] = base + (size_t)b * a.in_inst_stride + (size_t)(k & 1) * a.in_part_stride + ((size_t)m << logn);
-        } else {
-            rows[k] = a.ext + (size_t)b * a.ext_inst_stride + (size_t)k * a.ext_part_stride +
-                      ((size_t)(m - common) << logn);
-        }
-    }
-    u64 lo, hi;
-    if (c == 1) {
-        u64 l2, h2;
-        mul_wide(rows[0][e], rows[3][e], lo, hi);
-        mul_wide(rows[1][e], rows[2][e], l2, h2);
-        add128(lo, hi, l2, h2);
-    } else {
-        mul_wide(rows[c == 0 ? 0 : 1][e], rows[c == 0 ? 2 : 3][e], lo, hi);
-    }
-    a.out[(size_t)b * a.out_inst_stride + (size_t)c * a.out_part_stride + ((size_t)m << logn) + e] =
-        reduce_u128(lo, hi, mi.barrett_lo, mi.barrett_hi, mi.p);
-}
 
 // grid = (N/256, M, num); each thread forms every output part of its
 // coefficient. At most 8 products meet in one part (lp + rp - 1 <= 16), each
@@ -93,13 +39,6 @@ void launch_tensor_parts(const u64* a, u32 lp, const u64* b, u32 rp, u64* c, con
     ProfScope ps("tensor_parts", logn, s);
     hipLaunchKernelGGL(tensor_parts_kernel, ew_grid(1u << logn, M, num), dim3(256), 0, s, a, lp, b, rp, c, mods, M,
                        (u32)logn);
-    HIPCHK(hipGetLastError());
-}
-
-void launch_tensor_ew(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_t s) {
-    if (!M || !inst) return;
-    ProfScope ps("tensor_ew", logn, s);
-    hipLaunchKernelGGL(tensor_ew_kernel, ew_grid(1u << logn, 3 * M, inst), dim3(256), 0, s, a, M, (u32)logn);
     HIPCHK(hipGetLastError());
 }
 
@@ -164,41 +103,6 @@ void launch_switch_down_pb(const u64* in, u64* out, const SdConst* consts, u64 q
     ProfScope ps("switch_down", logn, s);
     hipLaunchKernelGGL(switch_down_pb_kernel, ew_grid(1u << logn, L - 1, polys), dim3(256), 0, s, in, out, consts,
                        ql, L, in_stride, out_stride, (u32)logn);
-    HIPCHK(hipGetLastError());
-}
-
-// Power-basis part of the key-switch finish (the z of ks_finish_kernel):
-// x[b][p][i] = add_pb - q_l^-1 [y]_i with y from the dropped limb in
-// power basis (`special` [inst][2][N]), or add_pb alone without an extra
-// modulus. grid = (N/256, L * 2, instances)
-__global__ __launch_bounds__(256) void sd_z_kernel(KsFinishArgs f, u64* __restrict__ x, u32 logn) {
-    const SdNttArgs& a = f.sd;
-    const u32 e = blockIdx.x * 256 + threadIdx.x;
-    if (e >> logn) return;
-    const u32 i = blockIdx.y % a.L, part = blockIdx.y / a.L, b = blockIdx.z;
-    const ModInfo mi = a.mods[i];
-    const u64 p = mi.p;
-    u64 z = 0;
-    if (a.extra) {
-        const SdConst cst = a.sd[i];
-        const u64 ql = a.extra_mods[0];
-        u64 yv = f.special[(((size_t)b * 2 + part) << logn) + e] + (ql >> 1);
-        yv = yv >= ql ? yv - ql : yv;
-        // y < q_l <= 2 q_i is already a valid lazy [y]_i (see ks_finish_kernel)
-        const u64 yr = ql <= 2 * p ? yv : lazy_reduce_u64(yv, mi.barrett_lo, mi.barrett_hi, p);
-        const u64 tmp = yr + cst.half_mod;
-        z = mul_shoup_u(3 * p - tmp, cst.inv, cst.inv_shoup, p);
-    }
-    if (a.add_pb)
-        z = add_mod(z, a.add_pb[(size_t)b * a.add_inst_stride + (size_t)part * a.add_part_stride + ((size_t)i << logn) + e],
-                    p);
-    x[((((size_t)b * 2 + part) * a.L + i) << logn) + e] = z;
-}
-
-void launch_sd_z(int logn, const KsFinishArgs& f, u64* x, u32 inst, hipStream_t s) {
-    if (!inst) return;
-    ProfScope ps("sd_z", logn, s);
-    hipLaunchKernelGGL(sd_z_kernel, ew_grid(1u << logn, 2 * f.sd.L, inst), dim3(256), 0, s, f, x, (u32)logn);
     HIPCHK(hipGetLastError());
 }
 

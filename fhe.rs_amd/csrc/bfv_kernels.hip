@@ -161,6 +161,82 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void tensor_intt_kernel(TensorAr
 }
 
 // ---------------------------------------------------------------------------
+// Degree 32768 (a row is two halves of 16384): the inverse transform of each
+// half with the Galois gather or the tensor products formed on load, using
+// the half tables (which leave out N^-1; the Montgomery factor 2^-64 is
+// undone by their last stage as in tensor_intt_kernel). The last inverse
+// stage, pairing the halves, follows as ntt_split_inv on the outputs.
+// grid = (2 limbs, parts, instances) for the rows: blockIdx.x = 2 limb + half.
+template <bool PERM>
+__global__ __launch_bounds__(NttShape<14>::T) void intt_rows_half_kernel(InttRowsArgs a) {
+    using S = NttShape<14>;
+    __shared__ u64 lds[S::LDS_WORDS];
+    const int t = threadIdx.x;
+    const u32 l = blockIdx.x >> 1, h = blockIdx.x & 1, k = blockIdx.y, b = blockIdx.z;
+    const u64* src = a.src[k] + (size_t)b * a.src_inst_stride + ((size_t)l << 15);
+    u64* dst = a.dst + (size_t)b * a.dst_inst_stride + (size_t)k * a.dst_part_stride + ((size_t)l << 15) +
+               ((size_t)h << 14);
+    const u32 row = 2 * (l + a.table_offset) + h;  // half-table row
+    const ModInfo mi = a.mods[row];
+    u64 x[S::E];
+    const u32 off = h << 14;
+    if constexpr (PERM) {
+        const u32* perm = a.perm;
+        load_inv_order<14, 16>(lds, x, t, [&](int j) { return src[perm[off + j]]; });
+    } else {
+        load_inv_order<14, 16>(lds, x, t, [&](int j) { return src[off + j]; });
+    }
+    ntt_inv_regs<14>(lds, x, t, a.inv_tw + ((size_t)row << 14), mi.p, inv_scale_n(mi));
+    store_inv_order<14>(dst, x, t);
+}
+
+// 1-D grid over (instance, limb, half, product), product fastest.
+__global__ __launch_bounds__(NttShape<14>::T) void tensor_intt_half_kernel(TensorArgs a) {
+    using S = NttShape<14>;
+    __shared__ u64 lds[S::LDS_WORDS];
+    const int t = threadIdx.x;
+    const u32 l = xcd_block(blockIdx.x, gridDim.x);
+    const u32 c = l % 3, h = (l / 3) & 1, m = (l / 6) % a.M, b = l / (6 * a.M);
+    const ModInfo mi = a.mods[2 * m + h];  // half-table constants
+    const u32 off = h << 14;
+    const u64* rows[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const u32 common = k < 2 ? a.common_lhs : a.common_rhs;
+        if (m < common) {
+            const u64* base = k < 2 ? a.lhs : a.rhs;
+            rows[k] = base + (size_t)b * a.in_inst_stride + (size_t)(k & 1) * a.in_part_stride + ((size_t)m << 15) + off;
+        } else {
+            rows[k] = a.ext + (size_t)b * a.ext_inst_stride + (size_t)k * a.ext_part_stride +
+                      ((size_t)(m - common) << 15) + off;
+        }
+    }
+    const u64 p = mi.p, pn = mi.pinv_neg;
+    u64 x[S::E];
+    if (c == 1) {
+        const u64 *ra0 = rows[0], *ra1 = rows[1], *rb0 = rows[2], *rb1 = rows[3];
+        load_inv_order<14, 2>(lds, x, t, [&](int j) {
+            u64 lo, hi, l2, h2;
+            mul_wide(ra0[j], rb1[j], lo, hi);
+            mul_wide(ra1[j], rb0[j], l2, h2);
+            add128(lo, hi, l2, h2);
+            return mont_redc(lo, hi, p, pn);
+        });
+    } else {
+        const u64* ra = rows[c == 0 ? 0 : 1];
+        const u64* rb = rows[c == 0 ? 2 : 3];
+        load_inv_order<14>(lds, x, t, [&](int j) {
+            u64 lo, hi;
+            mul_wide(ra[j], rb[j], lo, hi);
+            return mont_redc(lo, hi, p, pn);
+        });
+    }
+    ntt_inv_regs<14>(lds, x, t, a.inv_tw + ((size_t)(2 * m + h) << 14), p, inv_scale_nr(mi));
+    u64* dst = a.out + (size_t)b * a.out_inst_stride + (size_t)c * a.out_part_stride + ((size_t)m << 15) + off;
+    store_inv_order<14>(dst, x, t);
+}
+
+// ---------------------------------------------------------------------------
 // Key switching (KeySwitchingKey::key_switch, key_switching_key.rs:214-241) for
 // one key limb j of one instance: for every digit i, the power-basis row i of
 // the input is reduced modulo q'_j (lazy_reduce_vec), forward transformed
@@ -429,6 +505,56 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void ks_finish_kernel(KsFinishAr
     store_fwd_order<LOGN, 4, 16>(lds, x, t, [&](int j, u64 v) { dst[j] = add_mod(reduce1(reduce1(v, p2), p), dst[j], p); });
 }
 
+// The same finish at N = 32768, where a row is transformed as two halves:
+// the workgroup of half h forms the power-basis part z (as ks_finish_kernel)
+// at both coefficients j and j + 16384 of each pair it owns, takes half h of
+// the stage-0 butterfly on load (native.rs:256-269), runs the remaining 14
+// stages with the half tables and adds the reduced result into `out`. 1-D
+// grid over (instance, part, limb, half), half fastest.
+__global__ __launch_bounds__(NttShape<14>::T) void ks_finish_half_kernel(KsFinishArgs f,
+                                                                        const ulonglong2* __restrict__ half_tw,
+                                                                        const ulonglong2* __restrict__ split) {
+    using S = NttShape<14>;
+    __shared__ u64 lds[S::LDS_WORDS];
+    const SdNttArgs& a = f.sd;
+    const int t = threadIdx.x;
+    const u32 l = xcd_block(blockIdx.x, gridDim.x);
+    const u32 h = l & 1, r = l >> 1;
+    const u32 i = r % a.L, part = (r / a.L) & 1, b = r / (2 * a.L);
+    const ModInfo mi = a.mods[i];
+    const u64 p = mi.p, p2 = 2 * p;
+    const ulonglong2 w = split[3 * i];
+    const u64* ad = a.add_pb ? a.add_pb + (size_t)b * a.add_inst_stride + (size_t)part * a.add_part_stride +
+                                   ((size_t)i << 15)
+                             : nullptr;
+    const u64* lrow = a.extra ? f.special + (((size_t)b * 2 + part) << 15) : nullptr;
+    const SdConst cst = a.sd[i];
+    const u64 ql = a.extra ? a.extra_mods[0] : 0, half = ql >> 1;
+    // z = add_pb - q_l^-1 [y]_i (add_pb alone without an extra modulus); see
+    // ks_finish_kernel for the representatives.
+    auto z_at = [&](int k) {
+        u64 z = 0;
+        if (lrow) {
+            u64 yv = lrow[k] + half;
+            yv = yv >= ql ? yv - ql : yv;
+            const u64 yr = ql <= 2 * p ? yv : lazy_reduce_u64(yv, mi.barrett_lo, mi.barrett_hi, p);
+            z = mul_shoup_u(3 * p - (yr + cst.half_mod), cst.inv, cst.inv_shoup, p);
+        }
+        return ad ? add_mod(z, ad[k], p) : z;
+    };
+    u64 x[S::E];
+#pragma unroll
+    for (int e = 0; e < S::E; e++) {
+        const int k = fwd_index<14, 0>(t, e);
+        x[e] = split_half(z_at(k), z_at(k + 16384), h, w.x, w.y, p, p2);
+    }
+    ntt_fwd_regs<14>(lds, x, t, half_tw + ((size_t)(2 * i + h) << 14), p);
+    u64* dst = a.out + (size_t)b * a.out_inst_stride + (size_t)part * a.out_part_stride + ((size_t)i << 15) +
+               ((size_t)h << 14);
+    store_fwd_order<14, 4, 16>(lds, x, t,
+                               [&](int k, u64 v) { dst[k] = add_mod(reduce1(reduce1(v, p2), p), dst[k], p); });
+}
+
 // ---------------------------------------------------------------------------
 // Ciphertext addition (fhe/src/bfv/ops/mod.rs:15-69 -> Poly AddAssign,
 // math/src/rq/ops.rs:92-118): out = a + b mod q_limb, elementwise over
@@ -483,6 +609,34 @@ void launch_intt_rows(int logn, const InttRowsArgs& a, u32 limbs, u32 parts, u32
     }
 }
 
+void launch_intt_rows_half(const DeviceContext& c, const InttRowsArgs& a, u32 limbs, u32 parts, u32 inst,
+                           hipStream_t s) {
+    if (!limbs || !parts || !inst) return;
+    if (c.logn != 15) throw Error(-2, "the half-row inverse transforms are for degree 32768");
+    ProfScope ps(a.perm ? "intt_rows_gather" : "intt_rows", 15, s);
+    InttRowsArgs h = a;
+    h.inv_tw = c.half->inv_tw.as<ulonglong2>();
+    h.mods = c.half->mod.as<ModInfo>();
+    dim3 grid(2 * limbs, parts, inst);
+    if (a.perm)
+        hipLaunchKernelGGL(intt_rows_half_kernel<true>, grid, dim3(NttShape<14>::T), 0, s, h);
+    else
+        hipLaunchKernelGGL(intt_rows_half_kernel<false>, grid, dim3(NttShape<14>::T), 0, s, h);
+    HIPCHK(hipGetLastError());
+}
+
+void launch_tensor_intt_half(const DeviceContext& mul, const TensorArgs& a, u32 M, u32 inst, hipStream_t s) {
+    if (!M || !inst) return;
+    if (mul.logn != 15) throw Error(-2, "the half-row tensor product is for degree 32768");
+    ProfScope ps("tensor_intt", 15, s);
+    TensorArgs b = a;
+    b.M = M;
+    b.inv_tw = mul.half->inv_tw.as<ulonglong2>();
+    b.mods = mul.half->mod.as<ModInfo>();
+    hipLaunchKernelGGL(tensor_intt_half_kernel, dim3(M * 6 * inst), dim3(NttShape<14>::T), 0, s, b);
+    HIPCHK(hipGetLastError());
+}
+
 void launch_tensor_intt(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_t s) {
     if (!M || !inst) return;
     ProfScope ps("tensor_intt", logn, s);
@@ -513,6 +667,15 @@ void launch_ks_finish(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s) 
     ProfScope ps("ks_finish", logn, s);
     dim3 grid(inst * 2 * f.sd.L);
     FHEHIP_LOGN_DISPATCH(logn, ks_finish_kernel, grid, 0, s, f);
+}
+
+void launch_ks_finish_half(const KsFinishArgs& f, const DeviceContext& ct, u32 inst, hipStream_t s) {
+    if (!f.sd.L || !inst) return;
+    if (ct.logn != 15) throw Error(-2, "the half-row key-switch finish is for degree 32768");
+    ProfScope ps("ks_finish", 15, s);
+    hipLaunchKernelGGL(ks_finish_half_kernel, dim3(inst * 2 * f.sd.L * 2), dim3(NttShape<14>::T), 0, s, f,
+                       ct.half->fwd_tw.as<ulonglong2>(), ct.split.as<ulonglong2>());
+    HIPCHK(hipGetLastError());
 }
 
 void launch_sd_add_ntt(int logn, const SdNttArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s) {

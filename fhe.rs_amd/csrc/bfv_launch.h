@@ -85,12 +85,21 @@ void launch_rns_scale(const u64* in, size_t in_inst, size_t in_part, u64* out, s
                       const char* name = "rns_scale");
 void launch_intt_rows(int logn, const InttRowsArgs& a, u32 limbs, u32 parts, u32 inst, hipStream_t s);
 void launch_tensor_intt(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_t s);
+// Degree 32768 (`c` the context): the same with the two half-row inverse
+// transforms of every row; the last inverse stage (ntt_split_inv) follows
+// separately, on the contiguous destination rows.
+void launch_intt_rows_half(const DeviceContext& c, const InttRowsArgs& a, u32 limbs, u32 parts, u32 inst,
+                           hipStream_t s);
+void launch_tensor_intt_half(const DeviceContext& mul, const TensorArgs& a, u32 M, u32 inst, hipStream_t s);
 // Lifted, lazily transformed digits Y [inst][digits][key_limbs][N].
 void launch_lift_ntt(int logn, const KeySwitchArgs& a, u64* y, u32 inst, hipStream_t s);
 void launch_sd_add_ntt(int logn, const SdNttArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s);
 // out += NTT(power-basis switch-down part), degree <= 16384 (after ks_mac and
 // the inverse NTT of `special`).
 void launch_ks_finish(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s);
+// The same at degree 32768 (`ct` the ciphertext context): z on load, the
+// half-row transforms, out += result.
+void launch_ks_finish_half(const KsFinishArgs& f, const DeviceContext& ct, u32 inst, hipStream_t s);
 void launch_add_rows(const u64* a, const u64* b, u64* out, const ModInfo* mods, u32 L, int logn, size_t rows,
                      hipStream_t s);
 
@@ -99,11 +108,7 @@ void launch_add_rows(const u64* a, const u64* b, u64* out, const ModInfo* mods, 
 // (exact), a [num][lp][M][N], b [num][rp][M][N], c [num][lp + rp - 1][M][N].
 void launch_tensor_parts(const u64* a, u32 lp, const u64* b, u32 rp, u64* c, const ModInfo* mods, u32 M, int logn,
                          u32 num, hipStream_t s);
-// Elementwise kernels of the unfused pipeline (bfv_generic.hip), used where a
-// row does not fit one workgroup's LDS (N = 32768).
-void launch_gather_rows(const u64* src, size_t src_inst, u64* dst, size_t dst_inst, const u32* perm, int logn,
-                        u32 rows, u32 inst, hipStream_t s);
-void launch_tensor_ew(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_t s);
+// Elementwise kernels (bfv_generic.hip).
 void launch_mac_ew(const u64* y, u32 digits, u32 key_limbs, u32 first, u32 count, const ulonglong2* key0,
                    const ulonglong2* key1, const ModInfo* key_mods, u64* acc, int logn, u32 inst, hipStream_t s);
 // Key inner products, any degree (ks_mac_kernel): limbs below the dropped
@@ -170,8 +175,6 @@ struct WireTable {
 };
 void launch_pack(const u64* in, uint8_t* out, const WireTable& t, int logn, u32 per_poly, u32 num, hipStream_t s);
 void launch_unpack(const uint8_t* in, u64* out, const WireTable& t, int logn, u32 per_poly, u32 num, hipStream_t s);
-// Power-basis switch-down part x [inst][2][L][N] of the finish (unfused path).
-void launch_sd_z(int logn, const KsFinishArgs& f, u64* x, u32 inst, hipStream_t s);
 // Ends of the finish after an in-place switch-down chain (any key depth,
 // unfused path): post == false writes x = ks + add_pb (x contiguous
 // [inst][2][L][N]); post == true writes out = x + add_ntt[perm].

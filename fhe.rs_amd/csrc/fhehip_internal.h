@@ -84,5 +84,8 @@ void set_ntt_shape(int logn, int re_fwd, int re_inv);
 // for every digit i < digits and key limb j of `key`.
 void launch_lift_ntt_split(const DeviceContext& key, const u64* src, size_t src_inst, u64* y, u32 digits, u32 inst,
                            hipStream_t s);
+// Degree 32768: the last inverse stage of `rows` rows whose two halves went
+// through the half-row inverse transforms (N^-1 folded in).
+void launch_ntt_split_inv(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s);
 
 }  // namespace fhehip

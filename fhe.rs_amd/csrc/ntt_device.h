@@ -55,6 +55,17 @@ __device__ __forceinline__ void inv_bfly(u64& x, u64& y, u64 z, u64 zs, u64 p, u
     inv_bfly_asm<UT>(x, y, z, zs, p, p2);
 }
 
+// Half h of the N = 32768 stage-0 butterfly (native.rs:256-269) for one
+// coefficient pair (a, c) = (x_j, x_{j + 16384}), a, c < 4p: with
+// ar = a mod 2p and wc = lazy_mul_shoup(c) (< 2p), half 0 takes ar + wc and
+// half 1 ar + 2p - wc (both < 4p). A workgroup that transforms one half of
+// a row forms its own half of the first stage this way on load.
+__device__ __forceinline__ u64 split_half(u64 a, u64 c, u32 h, u64 w, u64 ws, u64 p, u64 p2) {
+    const u64 wc = lazy_mul_shoup_u(c, w, ws, p);
+    const u64 ar = reduce1(a, p2);
+    return h ? ar + p2 - wc : ar + wc;
+}
+
 // Final scaling of an inverse transform folded into its last stage:
 // x' = (x + y) s and y' = (x - y) zeta s, with zs = zeta s; outputs reduced.
 struct InvScale {

@@ -145,47 +145,60 @@ __global__ __launch_bounds__(256) void ntt_split_inv_kernel(RowMap m, const ulon
     row[j + 16384] = mul_shoup_u(x + 2 * p - y, z.x, z.y, p);
 }
 
-// N = 32768 digit lift (rq/mod.rs:518-540) fused with the first forward stage:
-// for digit i of instance b and key limb j, the two halves of
-// lazy_reduce_{q'_j}(src_i) go through the stage-0 butterfly into
-// y[b][i][j] (values < 4 q'_j); the half transforms follow.
-// grid = (16384 / 256, digits * key_limbs, instances)
-__global__ __launch_bounds__(256) void lift_split_kernel(const u64* __restrict__ src, size_t src_inst,
-                                                         u64* __restrict__ y, const ulonglong2* __restrict__ split,
-                                                         const ModInfo* __restrict__ mods, u32 key_limbs) {
-    const u32 k = blockIdx.x * 256 + threadIdx.x;
-    const u32 j = blockIdx.y % key_limbs, i = blockIdx.y / key_limbs, b = blockIdx.z;
+// N = 32768 digit lift (rq/mod.rs:518-540) with its whole lazy forward
+// transform in one kernel: the workgroup of half h of (instance b, digit i,
+// key limb j) reads both halves of lazy_reduce_{q'_j}(src_i), forms half h of
+// the stage-0 butterfly on load and runs the remaining 14 stages on it with
+// the half tables; outputs < 2 q'_j, as lift_ntt_kernel's. 1-D grid over
+// (instance, digit, key limb, half), half fastest: the two halves of one row
+// and the key limbs of one digit read the same source row and run on one XCD.
+__global__ __launch_bounds__(NttShape<14>::T) void lift_ntt_half_kernel(const u64* __restrict__ src, size_t src_inst,
+                                                                       u64* __restrict__ y,
+                                                                       const ulonglong2* __restrict__ half_tw,
+                                                                       const ModInfo* __restrict__ mods,
+                                                                       const ulonglong2* __restrict__ split,
+                                                                       u32 digits, u32 key_limbs) {
+    using S = NttShape<14>;
+    __shared__ u64 lds[S::LDS_WORDS];
+    const int t = threadIdx.x;
+    const u32 l = xcd_block(blockIdx.x, gridDim.x);
+    const u32 h = l & 1, r = l >> 1;
+    const u32 j = r % key_limbs, i = (r / key_limbs) % digits, b = r / (key_limbs * digits);
     const ModInfo mi = mods[j];
-    const u64 p = mi.p;
+    const u64 p = mi.p, p2 = 2 * p;
+    const ulonglong2 w = split[3 * j];
     const u64* row = src + b * src_inst + ((size_t)i << 15);
-    u64 x = row[k], z = row[k + 16384];
+    u64 x[S::E];
     // Digit i is below q_i (a prefix of the key moduli); below 4 q'_j it is
     // already a valid lazy transform input (as in lift_ntt_kernel).
-    if (mods[i].p > 4 * p) {
-        x = lazy_reduce_u64(x, mi.barrett_lo, mi.barrett_hi, p);
-        z = lazy_reduce_u64(z, mi.barrett_lo, mi.barrett_hi, p);
+    if (mods[i].p <= 4 * p) {
+#pragma unroll
+        for (int e = 0; e < S::E; e++) {
+            const int k = fwd_index<14, 0>(t, e);
+            x[e] = split_half(row[k], row[k + 16384], h, w.x, w.y, p, p2);
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < S::E; e++) {
+            const int k = fwd_index<14, 0>(t, e);
+            const u64 a = lazy_reduce_u64(row[k], mi.barrett_lo, mi.barrett_hi, p);
+            const u64 c = lazy_reduce_u64(row[k + 16384], mi.barrett_lo, mi.barrett_hi, p);
+            x[e] = split_half(a, c, h, w.x, w.y, p, p2);
+        }
     }
-    const ulonglong2 w = split[3 * j];
-    fwd_bfly<true>(x, z, w.x, w.y, p, 2 * p);
-    u64* out = y + (((size_t)b * gridDim.y + blockIdx.y) << 15);
-    out[k] = x;
-    out[k + 16384] = z;
+    ntt_fwd_regs<14>(lds, x, t, half_tw + ((size_t)(2 * j + h) << 14), p);
+    u64* dst = y + ((((size_t)b * digits + i) * key_limbs + j) << 15) + ((size_t)h << 14);
+    store_fwd_order<14>(lds, x, t, [&](int k, u64 v) { dst[k] = reduce1(v, p2); });
 }
 
 void launch_lift_ntt_split(const DeviceContext& key, const u64* src, size_t src_inst, u64* y, u32 digits, u32 inst,
                            hipStream_t s) {
     if (key.logn != 15) throw Error(-2, "the fused lift and split stage is for degree 32768");
     if (!digits || !inst) return;
-    {
-        ProfScope ps("lift_split", 15, s);
-        hipLaunchKernelGGL(lift_split_kernel, dim3(16384 / 256, digits * key.L, inst), dim3(256), 0, s, src, src_inst,
-                           y, key.split.as<ulonglong2>(), key.mod.as<ModInfo>(), key.L);
-        HIPCHK(hipGetLastError());
-    }
-    ProfScope ps("ntt_forward_lazy", 15, s);
-    RowMap m{y, key.L, key.L, 0, 0};
-    fwd_launch<14>(*key.half, RowMap{m.data, 2 * m.rows_per_poly, 2 * m.limbs_per_poly, 0, 0}, 2 * inst * digits * key.L,
-                   true, s);
+    ProfScope ps("lift_ntt", 15, s);
+    hipLaunchKernelGGL(lift_ntt_half_kernel, dim3(2 * inst * digits * key.L), dim3(NttShape<14>::T), 0, s, src,
+                       src_inst, y, key.half->fwd_tw.as<ulonglong2>(), key.mod.as<ModInfo>(),
+                       key.split.as<ulonglong2>(), digits, key.L);
     HIPCHK(hipGetLastError());
 }
 
@@ -204,6 +217,15 @@ void launch_ntt_forward(const DeviceContext& c, RowMap m, u32 rows, bool lazy, h
     } else {
         FHEHIP_LOGN_SWITCH(c.logn, fwd_launch, c, m, rows, lazy, s);
     }
+    HIPCHK(hipGetLastError());
+}
+
+void launch_ntt_split_inv(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s) {
+    if (rows == 0) return;
+    if (c.logn != 15) throw Error(-2, "the split inverse stage is for degree 32768");
+    ProfScope ps("ntt_split_inv", 15, s);
+    hipLaunchKernelGGL(ntt_split_inv_kernel, dim3(16384 / 256, rows), dim3(256), 0, s, m, c.split.as<ulonglong2>(),
+                       c.mod.as<ModInfo>());
     HIPCHK(hipGetLastError());
 }
 
