@@ -4,8 +4,10 @@
 Workload: 512 independent BFV ciphertext pairs at N=16384 with 6 x 60-bit
 ciphertext primes and one 60-bit special prime (key at the level above the
 ciphertexts), t = 65537, Multiplicator::default with a relinearization key.
-One step = one fhehip_multiply call over the whole batch (extension, tensor,
-t/Q down-scaling, key switch, switch-down), inputs already resident in HBM.
+One step = one replay of the HIP graph of one fhehip_multiply over the whole
+batch (extension, tensor, t/Q down-scaling, key switch, switch-down; the
+fused kernels' launch shapes autotuned first), inputs already resident in
+HBM; the same step as an eager call is reported beside it.
 
 Also measured: the per-call GPU drop-in (the same entry point called once per
 operation), per-kernel durations with HIP events on the launch stream for the
@@ -166,31 +168,72 @@ def host_cpu():
     return info
 
 
-_ORACLE_NATIVE = None
+_ORACLE_BUILD = None
+ORACLE_GENERIC = "g++ -O3 -march=x86-64-v3 (committed build)"
+ORACLE_NATIVE = "g++ -O3 -march=native (built on this host)"
+
+
+def oracle_native_path():
+    """The oracle compiled on this host with -O3 -march=native into a
+    temporary directory, or None if the compiler is unavailable."""
+    import subprocess
+    import tempfile
+    out = os.path.join(tempfile.mkdtemp(prefix="fhehip_oracle_"), "liboracle.so")
+    cmd = ["g++", "-O3", "-march=native", "-std=c++17", "-fPIC", "-shared", "-o", out,
+           os.path.join(ROOT, "oracle", "oracle_capi.cpp"), "-lpthread"]
+    try:
+        subprocess.run(cmd, check=True, timeout=600, capture_output=True)
+        return out
+    except (OSError, subprocess.SubprocessError):
+        return None
+
+
+def time_oracle_builds(budget_s=4.0):
+    """Single-thread time of one headline-shape Multiplicator::multiply with
+    each build of the oracle, each in its own process (a library loads once
+    per process): {build: ms}."""
+    import subprocess
+    script = (
+        "import sys, time, numpy as np; sys.path.insert(0, sys.argv[2]); import pyoracle as o; "
+        "o.LIB_PATH = sys.argv[1]; o.lib(); "
+        f"N = {N}; M = {MODULI!r}; b = o.Bfv(N, {T_PLAIN}, M, sk_seed=1); rk = b.relin_key(1, 0, seed=2); "
+        "m = o.Multiplicator(b, rk, level=1); x = o.random_polys(M[:-1], N, 2, seed=3); m.multiply(x, x); "
+        "ts = []\n"
+        "for _ in range(5):\n t0 = time.perf_counter(); m.multiply(x, x); ts.append(time.perf_counter() - t0)\n"
+        "print(sorted(ts)[2] * 1e3)")
+    out = {}
+    paths = {ORACLE_GENERIC: os.path.join(ROOT, "oracle", "lib", "liboracle.so")}
+    native = oracle_native_path()
+    if native:
+        paths[ORACLE_NATIVE] = native
+    for build, path in paths.items():
+        try:
+            r = subprocess.run([sys.executable, "-c", script, path, os.path.join(ROOT, "oracle")], capture_output=True,
+                               text=True, timeout=300, check=True)
+            out[build] = (float(r.stdout.strip().splitlines()[-1]), path)
+        except (OSError, subprocess.SubprocessError, ValueError, IndexError):
+            pass
+    return out
 
 
 def oracle_module():
     """The oracle (C++ restatement of the fhe.rs CPU path) for the CPU
-    baseline, compiled on this host with -O3 -march=native into a temporary
-    directory (the committed build is for a generic x86-64-v3 CPU); falls back
-    to that build if the compiler is unavailable. Returns (module, build)."""
-    global _ORACLE_NATIVE
+    baseline: of the committed x86-64-v3 build and a -march=native build made
+    on this host, the one faster on this host's cores. Returns (module, build,
+    {build: single-op ms})."""
+    global _ORACLE_BUILD
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    if _ORACLE_NATIVE is None:
-        import subprocess
-        import tempfile
-        out = os.path.join(tempfile.mkdtemp(prefix="fhehip_oracle_"), "liboracle.so")
-        cmd = ["g++", "-O3", "-march=native", "-std=c++17", "-fPIC", "-shared", "-o", out,
-               os.path.join(ROOT, "oracle", "oracle_capi.cpp"), "-lpthread"]
-        try:
-            subprocess.run(cmd, check=True, timeout=600, capture_output=True)
-            pyoracle.LIB_PATH = out
-            _ORACLE_NATIVE = "g++ -O3 -march=native (built on this host)"
-        except (OSError, subprocess.SubprocessError):
-            _ORACLE_NATIVE = "g++ -O3 -march=x86-64-v3 (committed build; native build failed)"
+    if _ORACLE_BUILD is None:
+        timed = time_oracle_builds()
+        if timed:
+            build = min(timed, key=lambda k: timed[k][0])
+            pyoracle.LIB_PATH = timed[build][1]
+            _ORACLE_BUILD = (build, {k: v[0] for k, v in timed.items()})
+        else:
+            _ORACLE_BUILD = (ORACLE_GENERIC, {})
     pyoracle.lib()
-    return pyoracle, _ORACLE_NATIVE
+    return pyoracle, _ORACLE_BUILD[0], _ORACLE_BUILD[1]
 
 
 # Single-instance configuration: N = 8192, generate_moduli([62] * 3, 8192), t = 65537,
@@ -350,11 +393,16 @@ def uneven_batch(fhehip, torch, instances=1024, reps=3, per_call_jobs=48):
     stream = torch.cuda.current_stream()
     fhehip.submit(jobs, stream)
     torch.cuda.synchronize()
+    # A half-precision fill on each side of the timed submits marks them in a
+    # kernel trace (it runs outside the timed region).
+    marker = torch.empty(4, dtype=torch.float16, device=dev)
+    marker.fill_(1.5)
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
     ev[0].record(stream)
     for k in range(reps):
         fhehip.submit(jobs, stream)
         ev[k + 1].record(stream)
+    marker.fill_(2.5)
     torch.cuda.synchronize()
     sub_ms = [ev[k].elapsed_time(ev[k + 1]) for k in range(reps)]
     total_ms = ev[0].elapsed_time(ev[-1])
@@ -713,7 +761,7 @@ def pir_cpu_baseline(h):
     CPU path) recomputes the single-query response on one core, for a bit
     comparison with the GPU's and its time."""
     import numpy as np
-    oracle, build = oracle_module()
+    oracle, build, _ = oracle_module()
     moduli, n, t, d1, d2 = h["moduli"], h["n"], h["t"], h["d1"], h["d2"]
     m1, m2 = moduli[:2], moduli[:1]
     L = len(m1)
@@ -753,7 +801,7 @@ def cpu_baseline(lhs_h, rhs_h, c0, c1, gpu_out0, budget_s):
     process may use, one independent Multiplicator::multiply per thread at a
     time (fhe.rs itself is single-threaded per operation)."""
     import numpy as np
-    oracle, build = oracle_module()
+    oracle, build, builds_ms = oracle_module()
     bfv = oracle.Bfv(N, T_PLAIN, MODULI, sk_seed=1)
     rk = bfv.key_from_rows(c0, c1, ct_level=1, key_level=0)
     om = oracle.Multiplicator(bfv, rk, level=1)
@@ -777,6 +825,7 @@ def cpu_baseline(lhs_h, rhs_h, c0, c1, gpu_out0, budget_s):
                   f"{percentile(list(lat0 * 1e3), 50):.1f} ms",
         "p50_ms_1core": percentile(list(lat0 * 1e3), 50),
         "build": build,
+        "single_op_ms_by_build": builds_ms,
         "host": host,
     }, bit_exact
 
@@ -823,7 +872,7 @@ def main():
         print(json.dumps(res))
         return
     if args.only == "uneven":
-        print(json.dumps(uneven_batch(fhehip, torch)))
+        print(json.dumps(uneven_batch(fhehip, torch, reps=args.steps)))
         return
     if args.only == "serve":
         print(json.dumps(serving(fhehip, torch)))
@@ -877,11 +926,19 @@ def main():
     out = torch.empty(B * per, dtype=torch.int64, device=dev)
     stream = torch.cuda.current_stream()
 
+    # Autotuned launch shapes of the fused kernels at this batch (skipped with
+    # --no-extras, so that profiled runs hold only the pipeline's launches).
+    fused_tuning = None
+    if not args.no_extras:
+        chosen, tms = fhehip.multiply_autotune(mult, B)
+        fused_tuning = {"chosen": chosen, "ms_per_multiply": tms, "instances": B}
+
     for _ in range(args.warmup):
         mult.multiply(lhs, rhs, out, B, stream)
     torch.cuda.synchronize()
 
-    # Timed region: K batched steps, per-step events, per-kernel events.
+    # Eager steps: K batched fhehip_multiply calls with per-step events and
+    # per-kernel events (the roofline figures).
     fhehip.profiler_enable(True)
     evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
     torch.cuda.synchronize()
@@ -894,8 +951,30 @@ def main():
     wall = time.perf_counter() - t0
     kernels = fhehip.profiler_report()
     fhehip.profiler_enable(False)
-    step_ms = [evs[k].elapsed_time(evs[k + 1]) for k in range(args.steps)]
-    total_ms = evs[0].elapsed_time(evs[-1])
+    eager_step_ms = [evs[k].elapsed_time(evs[k + 1]) for k in range(args.steps)]
+    eager_total_ms = evs[0].elapsed_time(evs[-1])
+    eager_value = B * args.steps / (eager_total_ms / 1e3)
+
+    # Timed steps: the same batched step captured once as a HIP graph (its
+    # ten launches) and replayed K times; each replay is checked against the
+    # eager output.
+    gout = torch.empty_like(out)
+    torch.cuda.synchronize()
+    graph = fhehip.Graph(mult, lhs, rhs, gout, B)
+    for _ in range(args.warmup):
+        graph.launch(stream)
+    torch.cuda.synchronize()
+    gev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    gev[0].record(stream)
+    for k in range(args.steps):
+        graph.launch(stream)
+        gev[k + 1].record(stream)
+    torch.cuda.synchronize()
+    step_ms = [gev[k].elapsed_time(gev[k + 1]) for k in range(args.steps)]
+    total_ms = gev[0].elapsed_time(gev[-1])
+    graph_equals_eager = bool(torch.equal(gout, out))
+    graph.close()
+    del gout
     value = B * args.steps / (total_ms / 1e3)
 
     # Per-call GPU drop-in: one operation per call, into its own output buffer.
@@ -1025,6 +1104,12 @@ def main():
                    "parallelism": "single GPU"},
         "p50_step_ms": percentile(step_ms, 50),
         "p99_step_ms": percentile(step_ms, 99),
+        "step": "one HIP-graph replay of the batched fhehip_multiply (captured once before the timed region)",
+        "graph_equals_eager": graph_equals_eager,
+        "eager": {"value": eager_value, "unit": UNIT, "ms_per_step": eager_total_ms / args.steps,
+                  "p50_step_ms": percentile(eager_step_ms, 50), "p99_step_ms": percentile(eager_step_ms, 99),
+                  "note": "fhehip_multiply called per step, with per-kernel timing events"},
+        "fused_autotune": fused_tuning,
         "per_call": {"value": per_call_value, "unit": UNIT, "p50_ms": percentile(call_ms, 50),
                      "p99_ms": percentile(call_ms, 99), "ops": args.per_call_ops},
         "per_call_graph": per_call_graph,

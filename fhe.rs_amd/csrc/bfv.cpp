@@ -558,6 +558,76 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out_final, u32
 }
 
 // ---------------------------------------------------------------------------
+FusedTuning autotune_multiply(Multiplicator& m, u32 inst, float (&ms)[11]) {
+    check_batch(inst);
+    if (!inst) throw Error(-1, "the autotuner needs at least one instance");
+    const int logn = m.base->logn;
+    const size_t n = m.base->N, L = m.base->L;
+    const size_t words_in = (size_t)inst * 2 * L * n;
+    const size_t words_out = (size_t)inst * (m.rk ? 2 : 3) * (m.mod_switch ? L - 1 : L) * n;
+    DevBuf in(words_in * 8), out(words_out * 8);
+    hipStream_t s;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const FusedTuning start = fused_tuning(logn);
+    FusedTuning best = start;
+    try {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipMemsetAsync(in.ptr, 0, in.bytes, s));  // zero residues are valid inputs
+        auto time_it = [&](const FusedTuning& t) {
+            set_fused_tuning(logn, t);
+            const int reps = 3;
+            m.multiply(in.as<u64>(), in.as<u64>(), out.as<u64>(), inst, s);
+            m.multiply(in.as<u64>(), in.as<u64>(), out.as<u64>(), inst, s);
+            HIPCHK(hipEventRecord(e0, s));
+            for (int r = 0; r < reps; r++) m.multiply(in.as<u64>(), in.as<u64>(), out.as<u64>(), inst, s);
+            HIPCHK(hipEventRecord(e1, s));
+            HIPCHK(hipEventSynchronize(e1));
+            float el = 0;
+            HIPCHK(hipEventElapsedTime(&el, e0, e1));
+            return el / reps;
+        };
+        int slot = 0;
+        auto knob = [&](int FusedTuning::*field, std::initializer_list<int> values) {
+            int pick = best.*field;
+            float fastest = 0;
+            for (int v : values) {
+                FusedTuning t = best;
+                t.*field = v;
+                const float el = time_it(t);
+                ms[slot++] = el;
+                if (fastest == 0 || el < fastest) {
+                    fastest = el;
+                    pick = v;
+                }
+            }
+            best.*field = pick;
+        };
+        knob(&FusedTuning::xcd, {0, 1});
+        knob(&FusedTuning::ks_mac_inst, {2, 4, 8});
+        knob(&FusedTuning::re_tensor, {4, 5});
+        knob(&FusedTuning::re_lift, {4, 5});
+        knob(&FusedTuning::re_finish, {4, 5});
+        set_fused_tuning(logn, best);
+    } catch (...) {
+        set_fused_tuning(logn, start);
+        (void)hipStreamSynchronize(s);
+        release_scratch(s);
+        (void)hipStreamDestroy(s);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        throw;
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    release_scratch(s);
+    (void)hipStreamDestroy(s);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return best;
+}
+
+// ---------------------------------------------------------------------------
 // BfvParametersBuilder::build (fhe/src/bfv/parameters.rs:431-641).
 Parameters::Parameters(u32 degree, u64 plaintext, const std::vector<u64>& m) : N(degree), t(plaintext), moduli(m) {
     if (degree < 8 || (degree & (degree - 1))) throw Error(-1, "the degree must be a power of two, at least 8");

@@ -3,6 +3,7 @@
 
 #include "bfv_launch.h"
 #include "rns_device.h"
+#include "tuning.h"
 
 #include <map>
 #include <mutex>
@@ -184,6 +185,13 @@ struct Parameters {
     // out [num][lp + rp - 1][L][N].
     void ct_mul(u32 level, const u64* lhs, u32 lp, const u64* rhs, u32 rp, u64* out, u32 num, hipStream_t s);
 };
+
+// Times m.multiply on `inst` instances (zero inputs on a private stream)
+// under each choice of the fused kernels' launch shapes, one knob at a time
+// starting from the current choice, keeps the fastest for the degree and
+// returns it. ms: milliseconds per multiply for xcd 0/1, ks_mac instances
+// 2/4/8, tensor 4/5, lift 4/5, finish 4/5 log2 values per thread.
+FusedTuning autotune_multiply(Multiplicator& m, u32 inst, float (&ms)[11]);
 
 // fhehip_multiply with all scratch in the library's fixed device region
 // (no device allocation); refuses batches whose scratch does not fit.

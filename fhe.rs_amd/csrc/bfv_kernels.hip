@@ -7,6 +7,7 @@
 #include "ntt_device.h"
 #include "profiler.h"
 #include "rns_device.h"
+#include "tuning.h"
 
 namespace fhehip {
 
@@ -106,14 +107,14 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void intt_rows_gather_kernel(Int
 // :68-78): c0 = a0 b0, c1 = a0 b1 + a1 b0, c2 = a1 b1 over the M limbs of the
 // multiplication context. 1-D grid over (instance, limb, product), product fastest.
 // ---------------------------------------------------------------------------
-template <int LOGN>
-__global__ __launch_bounds__(NttShape<LOGN>::T) void tensor_intt_kernel(TensorArgs a) {
-    using S = NttShape<LOGN>;
+template <int LOGN, int R>
+__global__ __launch_bounds__((NttShape<LOGN, R>::T)) void tensor_intt_kernel(TensorArgs a) {
+    using S = NttShape<LOGN, R>;
     __shared__ u64 lds[S::LDS_WORDS];
     const int t = threadIdx.x;
     // The three products of one (limb, instance) read the same rows: they are
     // consecutive logical indices, so they run on one XCD.
-    const u32 l = xcd_block(blockIdx.x, gridDim.x);
+    const u32 l = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const u32 c = l % 3, m = (l / 3) % a.M, b = l / (3 * a.M);
     const ModInfo mi = a.mods[m];
     // Row pointers of the four extended inputs a0, a1, b0, b1 at limb m.
@@ -138,7 +139,7 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void tensor_intt_kernel(TensorAr
     u64 x[S::E];
     if (c == 1) {
         const u64 *ra0 = rows[0], *ra1 = rows[1], *rb0 = rows[2], *rb1 = rows[3];
-        load_inv_order<LOGN, 2>(lds, x, t, [&](int j) {
+        load_inv_order<LOGN, 2, R>(lds, x, t, [&](int j) {
             u64 lo, hi, l2, h2;
             mul_wide(ra0[j], rb1[j], lo, hi);
             mul_wide(ra1[j], rb0[j], l2, h2);
@@ -148,16 +149,16 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void tensor_intt_kernel(TensorAr
     } else {
         const u64* ra = rows[c == 0 ? 0 : 1];
         const u64* rb = rows[c == 0 ? 2 : 3];
-        load_inv_order<LOGN>(lds, x, t, [&](int j) {
+        load_inv_order<LOGN, 4, R>(lds, x, t, [&](int j) {
             u64 lo, hi;
             mul_wide(ra[j], rb[j], lo, hi);
             return mont_redc(lo, hi, p, pn);
         });
     }
     const ulonglong2* tw = a.inv_tw + ((size_t)m << LOGN);
-    ntt_inv_regs<LOGN>(lds, x, t, tw, mi.p, inv_scale_nr(mi));
+    ntt_inv_regs<LOGN, R>(lds, x, t, tw, mi.p, inv_scale_nr(mi));
     u64* dst = a.out + (size_t)b * a.out_inst_stride + (size_t)c * a.out_part_stride + ((size_t)m << LOGN);
-    store_inv_order<LOGN>(dst, x, t);
+    store_inv_order<LOGN, R>(dst, x, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -252,14 +253,14 @@ __global__ __launch_bounds__(NttShape<14>::T) void tensor_intt_half_kernel(Tenso
 // product reads it back in the kernels below. 1-D grid over
 // (instance, digit, key limb), key limb fastest.
 // ---------------------------------------------------------------------------
-template <int LOGN>
-__global__ __launch_bounds__(NttShape<LOGN>::T) void lift_ntt_kernel(KeySwitchArgs a, u64* __restrict__ y) {
-    using S = NttShape<LOGN>;
+template <int LOGN, int R>
+__global__ __launch_bounds__((NttShape<LOGN, R>::T)) void lift_ntt_kernel(KeySwitchArgs a, u64* __restrict__ y) {
+    using S = NttShape<LOGN, R>;
     __shared__ u64 lds[S::LDS_WORDS];
     const int t = threadIdx.x;
     // The key limbs of one digit read the same source row: consecutive
     // logical indices, run on one XCD.
-    const u32 l = xcd_block(blockIdx.x, gridDim.x);
+    const u32 l = a.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const u32 j = l % a.key_limbs, i = (l / a.key_limbs) % a.digits, b = l / (a.key_limbs * a.digits);
     const ModInfo mi = a.key_mods[j];
     const u64 p = mi.p;
@@ -271,16 +272,16 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void lift_ntt_kernel(KeySwitchAr
     // same residues, which the exact key inner product reduces.
     if (a.key_mods[i].p <= 4 * p) {
 #pragma unroll
-        for (int e = 0; e < S::E; e++) x[e] = src[fwd_index<LOGN, 0>(t, e)];
+        for (int e = 0; e < S::E; e++) x[e] = src[fwd_index<LOGN, 0, R>(t, e)];
     } else {
 #pragma unroll
         for (int e = 0; e < S::E; e++)
-            x[e] = lazy_reduce_u64(src[fwd_index<LOGN, 0>(t, e)], mi.barrett_lo, mi.barrett_hi, p);
+            x[e] = lazy_reduce_u64(src[fwd_index<LOGN, 0, R>(t, e)], mi.barrett_lo, mi.barrett_hi, p);
     }
-    ntt_fwd_regs<LOGN>(lds, x, t, a.key_fwd_tw + ((size_t)j << LOGN), p);
+    ntt_fwd_regs<LOGN, R>(lds, x, t, a.key_fwd_tw + ((size_t)j << LOGN), p);
     u64* dst = y + ((((size_t)b * a.digits + i) * a.key_limbs + j) << LOGN);
     const u64 p2 = 2 * p;
-    store_fwd_order<LOGN>(lds, x, t, [&](int k, u64 v) { dst[k] = reduce1(v, p2); });
+    store_fwd_order<LOGN, R>(lds, x, t, [&](int k, u64 v) { dst[k] = reduce1(v, p2); });
 }
 
 // ---------------------------------------------------------------------------
@@ -375,15 +376,14 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void sd_add_ntt_kernel(SdNttArgs
 // ---------------------------------------------------------------------------
 
 // One thread per coefficient of key limb i, both parts, for KS_MAC_INST
-// instances (key_switching_key.rs:227-239): the key values of a digit are
+// (2, 4 or 8, chosen by the autotuner) instances (key_switching_key.rs:227-239): the key values of a digit are
 // loaded once and used for every instance of the group. The switch-down
 // factor q_l^-1 is folded into the key rows of the limbs below the dropped
 // one (fin0, fin1), Y < 2q and the key rows < q, so four products stay below
 // 2^127; their exact sum (column accumulator) gets one Barrett reduction.
 // Limb i == L (the dropped one, when extra) goes to `special`.
 // grid = (N / 256, L + extra, ceil(instances / KS_MAC_INST))
-constexpr u32 KS_MAC_INST = 4;
-
+template <u32 KS_MAC_INST>
 __global__ __launch_bounds__(256) void ks_mac_kernel(KsFinishArgs f, u32 logn, u32 inst) {
     const SdNttArgs& a = f.sd;
     const u32 j = blockIdx.x * 256 + threadIdx.x;
@@ -457,13 +457,13 @@ __global__ __launch_bounds__(256) void ks_mac_kernel(KsFinishArgs f, u32 logn, u
 // out_i += NTT(add_pb_i - q_l^-1 [y]_i) (or NTT(add_pb_i) without an extra
 // modulus). 1-D grid over (instance, part, limb), limb fastest: the limbs of
 // one (instance, part) read the same dropped-limb row and run on one XCD.
-template <int LOGN>
-__global__ __launch_bounds__(NttShape<LOGN>::T) void ks_finish_kernel(KsFinishArgs f) {
-    using S = NttShape<LOGN>;
+template <int LOGN, int R>
+__global__ __launch_bounds__((NttShape<LOGN, R>::T)) void ks_finish_kernel(KsFinishArgs f) {
+    using S = NttShape<LOGN, R>;
     __shared__ u64 lds[S::LDS_WORDS];
     const SdNttArgs& a = f.sd;
     const int t = threadIdx.x;
-    const u32 l = xcd_block(blockIdx.x, gridDim.x);
+    const u32 l = f.xcd ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
     const u32 i = l % a.L, part = (l / a.L) & 1, b = l / (2 * a.L);
     const ModInfo mi = a.mods[i];
     const u64 p = mi.p, p2 = 2 * p;
@@ -479,7 +479,7 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void ks_finish_kernel(KsFinishAr
         auto body = [&](auto small) {
 #pragma unroll
             for (int e = 0; e < S::E; e++) {
-                const int j = fwd_index<LOGN, 0>(t, e);
+                const int j = fwd_index<LOGN, 0, R>(t, e);
                 u64 yv = lrow[j] + half;
                 yv = yv >= ql ? yv - ql : yv;
                 // [y]_i only has to be < 2 q_i and congruent to y (the rest
@@ -498,11 +498,11 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void ks_finish_kernel(KsFinishAr
             body(std::false_type{});
     } else {
 #pragma unroll
-        for (int e = 0; e < S::E; e++) x[e] = ad ? ad[fwd_index<LOGN, 0>(t, e)] : 0;
+        for (int e = 0; e < S::E; e++) x[e] = ad ? ad[fwd_index<LOGN, 0, R>(t, e)] : 0;
     }
-    ntt_fwd_regs<LOGN>(lds, x, t, a.fwd_tw + ((size_t)i << LOGN), p);
+    ntt_fwd_regs<LOGN, R>(lds, x, t, a.fwd_tw + ((size_t)i << LOGN), p);
     u64* dst = a.out + (size_t)b * a.out_inst_stride + (size_t)part * a.out_part_stride + ((size_t)i << LOGN);
-    store_fwd_order<LOGN, 4, 16>(lds, x, t, [&](int j, u64 v) { dst[j] = add_mod(reduce1(reduce1(v, p2), p), dst[j], p); });
+    store_fwd_order<LOGN, R, 16>(lds, x, t, [&](int j, u64 v) { dst[j] = add_mod(reduce1(reduce1(v, p2), p), dst[j], p); });
 }
 
 // The same finish at N = 32768, where a row is transformed as two halves:
@@ -598,6 +598,44 @@ void launch_add_rows(const u64* a, const u64* b, u64* out, const ModInfo* mods, 
     }                                                                                        \
     HIPCHK(hipGetLastError());
 
+// The fused kernels with a tunable number of values per thread: RE = 5 (32
+// values, N / 32 threads) from degree 32 up, else RE = 4.
+#define FHEHIP_LOGN_R_CASE(L, R, KERNEL, grid, stream, ...) \
+    case L: hipLaunchKernelGGL((KERNEL<L, R>), grid, dim3(NttShape<L, R>::T), 0, stream, __VA_ARGS__); break;
+#define FHEHIP_LOGN_R_DISPATCH(logn, re, KERNEL, grid, stream, ...)                                   \
+    if ((re) == 5 && (logn) >= 5) {                                                                   \
+        switch (logn) {                                                                               \
+            FHEHIP_LOGN_R_CASE(5, 5, KERNEL, grid, stream, __VA_ARGS__)                               \
+            FHEHIP_LOGN_R_CASE(6, 5, KERNEL, grid, stream, __VA_ARGS__)                               \
+            FHEHIP_LOGN_R_CASE(7, 5, KERNEL, grid, stream, __VA_ARGS__)                               \
+            FHEHIP_LOGN_R_CASE(8, 5, KERNEL, grid, stream, __VA_ARGS__)                               \
+            FHEHIP_LOGN_R_CASE(9, 5, KERNEL, grid, stream, __VA_ARGS__)                               \
+            FHEHIP_LOGN_R_CASE(10, 5, KERNEL, grid, stream, __VA_ARGS__)                              \
+            FHEHIP_LOGN_R_CASE(11, 5, KERNEL, grid, stream, __VA_ARGS__)                              \
+            FHEHIP_LOGN_R_CASE(12, 5, KERNEL, grid, stream, __VA_ARGS__)                              \
+            FHEHIP_LOGN_R_CASE(13, 5, KERNEL, grid, stream, __VA_ARGS__)                              \
+            FHEHIP_LOGN_R_CASE(14, 5, KERNEL, grid, stream, __VA_ARGS__)                              \
+            default: throw Error(-2, "degree not supported");                                         \
+        }                                                                                             \
+    } else {                                                                                          \
+        switch (logn) {                                                                               \
+            FHEHIP_LOGN_R_CASE(3, 4, KERNEL, grid, stream, __VA_ARGS__)                               \
+            FHEHIP_LOGN_R_CASE(4, 4, KERNEL, grid, stream, __VA_ARGS__)                               \
+            FHEHIP_LOGN_R_CASE(5, 4, KERNEL, grid, stream, __VA_ARGS__)                               \
+            FHEHIP_LOGN_R_CASE(6, 4, KERNEL, grid, stream, __VA_ARGS__)                               \
+            FHEHIP_LOGN_R_CASE(7, 4, KERNEL, grid, stream, __VA_ARGS__)                               \
+            FHEHIP_LOGN_R_CASE(8, 4, KERNEL, grid, stream, __VA_ARGS__)                               \
+            FHEHIP_LOGN_R_CASE(9, 4, KERNEL, grid, stream, __VA_ARGS__)                               \
+            FHEHIP_LOGN_R_CASE(10, 4, KERNEL, grid, stream, __VA_ARGS__)                              \
+            FHEHIP_LOGN_R_CASE(11, 4, KERNEL, grid, stream, __VA_ARGS__)                              \
+            FHEHIP_LOGN_R_CASE(12, 4, KERNEL, grid, stream, __VA_ARGS__)                              \
+            FHEHIP_LOGN_R_CASE(13, 4, KERNEL, grid, stream, __VA_ARGS__)                              \
+            FHEHIP_LOGN_R_CASE(14, 4, KERNEL, grid, stream, __VA_ARGS__)                              \
+            default: throw Error(-2, "degree not supported");                                         \
+        }                                                                                             \
+    }                                                                                                 \
+    HIPCHK(hipGetLastError());
+
 void launch_intt_rows(int logn, const InttRowsArgs& a, u32 limbs, u32 parts, u32 inst, hipStream_t s) {
     if (!limbs || !parts || !inst) return;
     ProfScope ps(a.perm ? "intt_rows_gather" : "intt_rows", logn, s);
@@ -640,33 +678,46 @@ void launch_tensor_intt_half(const DeviceContext& mul, const TensorArgs& a, u32 
 void launch_tensor_intt(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_t s) {
     if (!M || !inst) return;
     ProfScope ps("tensor_intt", logn, s);
+    const FusedTuning tu = fused_tuning(logn);
     TensorArgs b = a;
     b.M = M;
+    b.xcd = tu.xcd;
     dim3 grid(M * 3 * inst);
-    FHEHIP_LOGN_DISPATCH(logn, tensor_intt_kernel, grid, 0, s, b);
+    FHEHIP_LOGN_R_DISPATCH(logn, tu.re_tensor, tensor_intt_kernel, grid, s, b);
 }
 
 void launch_lift_ntt(int logn, const KeySwitchArgs& a, u64* y, u32 inst, hipStream_t s) {
     if (!a.digits || !inst) return;
     ProfScope ps("lift_ntt", logn, s);
+    const FusedTuning tu = fused_tuning(logn);
+    KeySwitchArgs b = a;
+    b.xcd = tu.xcd;
     dim3 grid(a.key_limbs * a.digits * inst);
-    FHEHIP_LOGN_DISPATCH(logn, lift_ntt_kernel, grid, 0, s, a, y);
+    FHEHIP_LOGN_R_DISPATCH(logn, tu.re_lift, lift_ntt_kernel, grid, s, b, y);
 }
 
 void launch_ks_mac(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s) {
     if (!f.sd.L || !inst) return;
     ProfScope ps("ks_mac", logn, s);
-    hipLaunchKernelGGL(ks_mac_kernel,
-                       dim3(((1u << logn) + 255) / 256, f.sd.L + f.sd.extra, (inst + KS_MAC_INST - 1) / KS_MAC_INST),
-                       dim3(256), 0, s, f, (u32)logn, inst);
+    const u32 k = (u32)fused_tuning(logn).ks_mac_inst;
+    const dim3 grid(((1u << logn) + 255) / 256, f.sd.L + f.sd.extra, (inst + k - 1) / k);
+    if (k == 2)
+        hipLaunchKernelGGL(ks_mac_kernel<2>, grid, dim3(256), 0, s, f, (u32)logn, inst);
+    else if (k == 8)
+        hipLaunchKernelGGL(ks_mac_kernel<8>, grid, dim3(256), 0, s, f, (u32)logn, inst);
+    else
+        hipLaunchKernelGGL(ks_mac_kernel<4>, grid, dim3(256), 0, s, f, (u32)logn, inst);
     HIPCHK(hipGetLastError());
 }
 
 void launch_ks_finish(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s) {
     if (!f.sd.L || !inst) return;
     ProfScope ps("ks_finish", logn, s);
+    const FusedTuning tu = fused_tuning(logn);
+    KsFinishArgs g = f;
+    g.xcd = tu.xcd;
     dim3 grid(inst * 2 * f.sd.L);
-    FHEHIP_LOGN_DISPATCH(logn, ks_finish_kernel, grid, 0, s, f);
+    FHEHIP_LOGN_R_DISPATCH(logn, tu.re_finish, ks_finish_kernel, grid, s, g);
 }
 
 void launch_ks_finish_half(const KsFinishArgs& f, const DeviceContext& ct, u32 inst, hipStream_t s) {

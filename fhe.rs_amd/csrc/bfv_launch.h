@@ -29,6 +29,7 @@ struct TensorArgs {
     const ulonglong2* inv_tw;  // multiplication context
     const ModInfo* mods;
     u32 M;                     // limbs of the multiplication context (set by the launcher)
+    u32 xcd;                   // XCD-aware block order (set by the launcher)
 };
 
 struct KeySwitchArgs {
@@ -39,6 +40,7 @@ struct KeySwitchArgs {
     const ulonglong2* key1;
     const ulonglong2* key_fwd_tw;
     const ModInfo* key_mods;
+    u32 xcd;  // XCD-aware block order (set by the launcher)
 };
 
 // Constants of one switch-down step applied to one remaining limb.
@@ -78,6 +80,7 @@ struct KsFinishArgs {
     u32 digits, key_limbs;
     u64* special;               // [inst][2][N] dropped limb (extra == 1)
     const ModInfo* special_mod; // modulus of the dropped limb
+    u32 xcd;                    // XCD-aware block order (set by the launcher)
 };
 
 void launch_rns_scale(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst, size_t out_part,

@@ -266,6 +266,50 @@ int fhehip_multiply(fhehip_multiplicator* m, const uint64_t* d_lhs, const uint64
     FHEHIP_API_END
 }
 
+static FusedTuning from_c(const fhehip_tuning& t) {
+    FusedTuning f;
+    f.xcd = (int)t.xcd_order;
+    f.ks_mac_inst = (int)t.ks_mac_instances;
+    f.re_tensor = (int)t.tensor_log2_values;
+    f.re_lift = (int)t.lift_log2_values;
+    f.re_finish = (int)t.finish_log2_values;
+    return f;
+}
+
+static fhehip_tuning to_c(const FusedTuning& f) {
+    return fhehip_tuning{(uint32_t)f.xcd, (uint32_t)f.ks_mac_inst, (uint32_t)f.re_tensor, (uint32_t)f.re_lift,
+                         (uint32_t)f.re_finish};
+}
+
+static int degree_log2(uint32_t degree) {
+    if (degree < 8 || degree > 32768 || (degree & (degree - 1))) throw Error(-1, "degree must be 8 .. 32768, a power of two");
+    return 31 - __builtin_clz(degree);
+}
+
+int fhehip_multiply_autotune(fhehip_multiplicator* m, uint32_t num, fhehip_tuning* chosen, float* ms_out) {
+    FHEHIP_API_BEGIN
+    if (!m) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    float ms[11] = {};
+    const FusedTuning t = autotune_multiply(*m->m, num, ms);
+    if (chosen) *chosen = to_c(t);
+    if (ms_out) std::memcpy(ms_out, ms, sizeof ms);
+    FHEHIP_API_END
+}
+
+int fhehip_tuning_set(uint32_t degree, const fhehip_tuning* t) {
+    FHEHIP_API_BEGIN
+    if (!t) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    set_fused_tuning(degree_log2(degree), from_c(*t));
+    FHEHIP_API_END
+}
+
+int fhehip_tuning_get(uint32_t degree, fhehip_tuning* t) {
+    FHEHIP_API_BEGIN
+    if (!t) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *t = to_c(fused_tuning(degree_log2(degree)));
+    FHEHIP_API_END
+}
+
 int fhehip_parameters_create(uint32_t degree, uint64_t plaintext, const uint64_t* moduli, uint32_t num_moduli,
                              fhehip_parameters** out) {
     FHEHIP_API_BEGIN

@@ -54,6 +54,9 @@ SIGNATURES = {
     "fhehip_multiplicator_extended_basis": (C.c_int, [_vp, _u64p, _u32p]),
     "fhehip_multiply": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
     "fhehip_multiply_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, C.c_uint32, _vp]),
+    "fhehip_multiply_autotune": (C.c_int, [_vp, C.c_uint32, _vp, C.POINTER(C.c_float)]),
+    "fhehip_tuning_set": (C.c_int, [C.c_uint32, _vp]),
+    "fhehip_tuning_get": (C.c_int, [C.c_uint32, _vp]),
     "fhehip_parameters_create": (C.c_int, [C.c_uint32, C.c_uint64, _u64p, C.c_uint32, _vpp]),
     "fhehip_parameters_destroy": (None, [_vp]),
     "fhehip_parameters_info": (C.c_int, [_vp, _u32p, _u32p, _u64p]),
@@ -335,6 +338,38 @@ class Multiplicator:
         """Multiplicator::enable_mod_switching: products land in next_ctx."""
         self.next_ctx = next_ctx
         check(lib().fhehip_multiplicator_enable_mod_switching(self.h, next_ctx.h))
+
+
+class TuningC(C.Structure):
+    """fhehip_tuning of include/fhehip.h."""
+    _fields_ = [("xcd_order", C.c_uint32), ("ks_mac_instances", C.c_uint32), ("tensor_log2_values", C.c_uint32),
+                ("lift_log2_values", C.c_uint32), ("finish_log2_values", C.c_uint32)]
+
+
+TUNING_TIMES = ["xcd0", "xcd1", "ks_mac2", "ks_mac4", "ks_mac8", "tensor4", "tensor5", "lift4", "lift5", "finish4",
+                "finish5"]
+
+
+def tuning_get(degree):
+    t = TuningC()
+    check(lib().fhehip_tuning_get(degree, C.byref(t)))
+    return {k: getattr(t, k) for k, _ in TuningC._fields_}
+
+
+def tuning_set(degree, **kw):
+    """Sets the named knobs (the others keep their current value)."""
+    cur = tuning_get(degree)
+    cur.update(kw)
+    t = TuningC(**cur)
+    check(lib().fhehip_tuning_set(degree, C.byref(t)))
+
+
+def multiply_autotune(mult, num):
+    """fhehip_multiply_autotune: (chosen knobs, {candidate: ms per multiply})."""
+    t = TuningC()
+    ms = (C.c_float * 11)()
+    check(lib().fhehip_multiply_autotune(mult.h, num, C.byref(t), ms))
+    return {k: getattr(t, k) for k, _ in TuningC._fields_}, dict(zip(TUNING_TIMES, [float(x) for x in ms]))
 
 
 class Parameters:

@@ -163,6 +163,33 @@ int fhehip_multiplicator_extended_basis(const fhehip_multiplicator* m, uint64_t*
 int fhehip_multiply(fhehip_multiplicator* m, const uint64_t* d_lhs, const uint64_t* d_rhs, uint64_t* d_out,
                     uint32_t num, void* stream);
 
+/* ---- Autotuning of the multiplication's fused kernels ------------------------
+ * No fhe.rs counterpart. Launch shapes of the fused row kernels, per degree
+ * and process-wide: the XCD-aware workgroup order of the 1-D grids (1) or
+ * the plain order (0); the instances that share one load of the key rows in
+ * the key inner product (2, 4 or 8); the log2 values per thread (4: N/16
+ * threads, 5: N/32 threads) of the fused tensor product + inverse NTT, of
+ * the digit lift + forward NTT and of the key-switch finish. Every choice
+ * gives the same, exact results. */
+typedef struct fhehip_tuning {
+    uint32_t xcd_order;
+    uint32_t ks_mac_instances;
+    uint32_t tensor_log2_values;
+    uint32_t lift_log2_values;
+    uint32_t finish_log2_values;
+} fhehip_tuning;
+/* Times fhehip_multiply on `num` instances (zero inputs, a private stream)
+ * under every choice of each knob in turn, starting from the degree's
+ * current choice, keeps the fastest for the multiplicator's degree and
+ * writes it to *chosen; ms_out (may be NULL) receives 11 milliseconds per
+ * multiply: xcd_order 0, 1; ks_mac_instances 2, 4, 8; tensor 4, 5; lift 4,
+ * 5; finish 4, 5. Synchronous. Multiplications of the same degree running
+ * meanwhile on other threads may use the trial choices. */
+int fhehip_multiply_autotune(fhehip_multiplicator* m, uint32_t num, fhehip_tuning* chosen, float* ms_out);
+/* Sets or reads the choice for one degree directly. */
+int fhehip_tuning_set(uint32_t degree, const fhehip_tuning* t);
+int fhehip_tuning_get(uint32_t degree, fhehip_tuning* t);
+
 /* ---- Parameters ---------------------------------------------------------------
  * BfvParameters (fhe/src/bfv/parameters.rs:431-641) built from the degree
  * (a power of two, 8 .. 32768), a plaintext modulus t (2 <= t < 2^62,
