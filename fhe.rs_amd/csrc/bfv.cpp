@@ -145,8 +145,18 @@ Ksk::Ksk(std::shared_ptr<DeviceContext> ct, std::shared_ptr<DeviceContext> key, 
     for (u32 i = 0; i < ct_ctx->L; i++)
         if (ct_ctx->moduli[i] != key_ctx->moduli[i])
             throw Error(-4, "the ciphertext context is not a level of the key context");
-    if (digits != ct_ctx->L) throw Error(-1, "one key row per ciphertext modulus is required");
-    if (key_ctx->L == 1) throw Error(-2, "single-modulus key switching is not supported");
+    if (key_ctx->L == 1) {
+        // KeySwitchingKey::new with a one-modulus key context (key_switching_key
+        // .rs:70-86): log_base = log2(next_power_of_two(q)) / 2 bit digits.
+        u32 log_modulus = 0;
+        while (log_modulus < 64 && (((u64)1 << log_modulus) < key_ctx->moduli[0])) log_modulus++;
+        log_base = log_modulus / 2;
+        const u32 need = (log_modulus + log_base - 1) / log_base;
+        if (digits != need)
+            throw Error(-1, "a one-modulus key has ceil(log2(q) / log_base) rows of the power-of-two decomposition");
+    } else if (digits != ct_ctx->L) {
+        throw Error(-1, "one key row per ciphertext modulus is required");
+    }
     extra = key_ctx->L - ct_ctx->L;
     const u32 Lk = key_ctx->L, n = key_ctx->N;
     const size_t rows = (size_t)digits * Lk;
@@ -223,6 +233,7 @@ KeySwitchArgs Ksk::args(const u64* src, size_t src_inst_stride) const {
     a.key1 = key1.as<ulonglong2>();
     a.key_fwd_tw = key_ctx->fwd_tw.as<ulonglong2>();
     a.key_mods = key_ctx->mod.as<ModInfo>();
+    a.log_base = log_base;
     return a;
 }
 
@@ -244,7 +255,7 @@ u64* Ksk::lift(const u64* src, size_t src_inst_stride, u32 inst, hipStream_t s) 
     if (fused(*key_ctx)) {
         launch_lift_ntt(key_ctx->logn, a, y, inst, s);
     } else {
-        launch_lift_ntt_split(*key_ctx, src, src_inst_stride, y, digits, inst, s);
+        launch_lift_ntt_split(*key_ctx, src, src_inst_stride, y, digits, inst, log_base, s);
     }
     return y;
 }

@@ -97,6 +97,10 @@ struct PolyScaler {
 struct Ksk {
     std::shared_ptr<DeviceContext> ct_ctx, key_ctx;
     u32 digits = 0, extra = 0, sd_stride = 0;
+    // One-modulus key context: power-of-two decomposition into digits of
+    // log_base bits (key_switching_key.rs:70-86, 292-329); 0: one digit per
+    // ciphertext modulus.
+    u32 log_base = 0;
     DevBuf key0, key1;  // [digits][L'][N] (value, shoup)
     // Rows of the key limbs below the dropped one with the switch-down factor
     // q_l^-1 folded in (plain values, [digits][L][N]); used by the fused

@@ -264,13 +264,19 @@ __global__ __launch_bounds__((NttShape<LOGN, R>::T)) void lift_ntt_kernel(KeySwi
     const u32 j = l % a.key_limbs, i = (l / a.key_limbs) % a.digits, b = l / (a.key_limbs * a.digits);
     const ModInfo mi = a.key_mods[j];
     const u64 p = mi.p;
-    const u64* src = a.src + (size_t)b * a.src_inst_stride + ((size_t)i << LOGN);
+    const u64* src = a.src + (size_t)b * a.src_inst_stride + ((size_t)(a.log_base ? 0 : i) << LOGN);
     u64 x[S::E];
     // Digit i is below q_i (the ciphertext moduli are the first key moduli).
     // Below 4 q'_j it is already a valid input of the lazy forward transform:
     // the stored outputs are then other representatives (< 2 q'_j) of the
-    // same residues, which the exact key inner product reduces.
-    if (a.key_mods[i].p <= 4 * p) {
+    // same residues, which the exact key inner product reduces. A
+    // decomposition digit (below 2^31 < q'_j) always is.
+    if (a.log_base) {
+        const u32 sh = i * a.log_base;
+        const u64 mask = ((u64)1 << a.log_base) - 1;
+#pragma unroll
+        for (int e = 0; e < S::E; e++) x[e] = (src[fwd_index<LOGN, 0, R>(t, e)] >> sh) & mask;
+    } else if (a.key_mods[i].p <= 4 * p) {
 #pragma unroll
         for (int e = 0; e < S::E; e++) x[e] = src[fwd_index<LOGN, 0, R>(t, e)];
     } else {

@@ -40,7 +40,8 @@ struct KeySwitchArgs {
     const ulonglong2* key1;
     const ulonglong2* key_fwd_tw;
     const ModInfo* key_mods;
-    u32 xcd;  // XCD-aware block order (set by the launcher)
+    u32 xcd;       // XCD-aware block order (set by the launcher)
+    u32 log_base;  // power-of-two decomposition: digit i is bits [i log_base, (i + 1) log_base) of limb 0; 0: RNS digits
 };
 
 // Constants of one switch-down step applied to one remaining limb.

@@ -82,8 +82,9 @@ void autotune_ntt(const DeviceContext& c, u32 rows, int& re_fwd, int& re_inv, fl
 void set_ntt_shape(int logn, int re_fwd, int re_inv);
 // Degree 32768: y[b][i][j] = forward_vt_lazy(lazy_reduce_{q'_j}(src[b][i]))
 // for every digit i < digits and key limb j of `key`.
+// log_base != 0: power-of-two decomposition digits of limb 0 (one-modulus keys).
 void launch_lift_ntt_split(const DeviceContext& key, const u64* src, size_t src_inst, u64* y, u32 digits, u32 inst,
-                           hipStream_t s);
+                           u32 log_base, hipStream_t s);
 // Degree 32768: the last inverse stage of `rows` rows whose two halves went
 // through the half-row inverse transforms (N^-1 folded in).
 void launch_ntt_split_inv(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s);

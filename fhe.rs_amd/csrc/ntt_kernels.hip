@@ -157,7 +157,7 @@ __global__ __launch_bounds__(NttShape<14>::T) void lift_ntt_half_kernel(const u6
                                                                        const ulonglong2* __restrict__ half_tw,
                                                                        const ModInfo* __restrict__ mods,
                                                                        const ulonglong2* __restrict__ split,
-                                                                       u32 digits, u32 key_limbs) {
+                                                                       u32 digits, u32 key_limbs, u32 log_base) {
     using S = NttShape<14>;
     __shared__ u64 lds[S::LDS_WORDS];
     const int t = threadIdx.x;
@@ -167,11 +167,20 @@ __global__ __launch_bounds__(NttShape<14>::T) void lift_ntt_half_kernel(const u6
     const ModInfo mi = mods[j];
     const u64 p = mi.p, p2 = 2 * p;
     const ulonglong2 w = split[3 * j];
-    const u64* row = src + b * src_inst + ((size_t)i << 15);
+    const u64* row = src + b * src_inst + ((size_t)(log_base ? 0 : i) << 15);
     u64 x[S::E];
     // Digit i is below q_i (a prefix of the key moduli); below 4 q'_j it is
-    // already a valid lazy transform input (as in lift_ntt_kernel).
-    if (mods[i].p <= 4 * p) {
+    // already a valid lazy transform input (as in lift_ntt_kernel), and so is
+    // a power-of-two decomposition digit.
+    if (log_base) {
+        const u32 sh = i * log_base;
+        const u64 mask = ((u64)1 << log_base) - 1;
+#pragma unroll
+        for (int e = 0; e < S::E; e++) {
+            const int k = fwd_index<14, 0>(t, e);
+            x[e] = split_half((row[k] >> sh) & mask, (row[k + 16384] >> sh) & mask, h, w.x, w.y, p, p2);
+        }
+    } else if (mods[i].p <= 4 * p) {
 #pragma unroll
         for (int e = 0; e < S::E; e++) {
             const int k = fwd_index<14, 0>(t, e);
@@ -192,13 +201,13 @@ __global__ __launch_bounds__(NttShape<14>::T) void lift_ntt_half_kernel(const u6
 }
 
 void launch_lift_ntt_split(const DeviceContext& key, const u64* src, size_t src_inst, u64* y, u32 digits, u32 inst,
-                           hipStream_t s) {
+                           u32 log_base, hipStream_t s) {
     if (key.logn != 15) throw Error(-2, "the fused lift and split stage is for degree 32768");
     if (!digits || !inst) return;
     ProfScope ps("lift_ntt", 15, s);
     hipLaunchKernelGGL(lift_ntt_half_kernel, dim3(2 * inst * digits * key.L), dim3(NttShape<14>::T), 0, s, src,
                        src_inst, y, key.half->fwd_tw.as<ulonglong2>(), key.mod.as<ModInfo>(),
-                       key.split.as<ulonglong2>(), digits, key.L);
+                       key.split.as<ulonglong2>(), digits, key.L, log_base);
     HIPCHK(hipGetLastError());
 }
 

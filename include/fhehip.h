@@ -116,7 +116,11 @@ int fhehip_scaler_scale(fhehip_scaler* s, const uint64_t* d_in, uint64_t* d_out,
  * KeySwitchingKey (fhe/src/bfv/keys/key_switching_key.rs:22-45) from its
  * NTT-domain rows c0, c1 given as HOST arrays [digits][L_key][N]; digits must
  * equal the number of ciphertext moduli (RNS-limb decomposition,
- * key_switching_key.rs:89-104). The key context must start with the
+ * key_switching_key.rs:89-104), or, for a key context of one modulus q, the
+ * ceil(log_modulus / log_base) rows of the power-of-two decomposition with
+ * log_modulus = log2(next_power_of_two(q)) and log_base = log_modulus / 2
+ * (key_switching_key.rs:70-86, 292-329; Galois keys at the last level). The
+ * key context must start with the
  * ciphertext context's moduli and may extend it by any number of moduli (a
  * key at any lower level, RelinearizationKey::new_leveled and GaloisKey::new,
  * relinearization_key.rs:34-67, galois_key.rs:26-60): they are removed by

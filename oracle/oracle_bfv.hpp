@@ -481,20 +481,54 @@ static inline std::vector<u64> decrypt(const BfvParams& par, const SecretKey& sk
 // c0/c1 are NttShoup polynomials in the key context; the Shoup companions are
 // recomputed where needed.
 // ---------------------------------------------------------------------------
+// With a single-modulus key context the key uses the power-of-two
+// decomposition (key_switching_key.rs:70-86, 292-329): digits of log_base =
+// log_modulus / 2 bits, log_modulus = log2(next_power_of_two(q)).
+static inline size_t decomposition_log_modulus(u64 q) {
+    size_t b = 0;
+    while (b < 64 && (((u64)1 << b) < q)) b++;
+    return b;
+}
+
 struct KeySwitchingKey {
     CtxPtr ctx_ct, ctx_ksk;
     size_t ciphertext_level = 0, ksk_level = 0;
+    size_t log_base = 0;  // power-of-two decomposition (one-modulus key context), else 0
     std::vector<PolyData> c0, c1, c0s, c1s;  // values and Shoup companions
 
-    // key_switching_key.rs:50-105 and 126-169 (multi-modulus case)
+    // key_switching_key.rs:50-105 and 126-210
     KeySwitchingKey(const BfvParams& par, const SecretKey& sk, const PolyData& from_pb,
                     size_t ct_level, size_t key_level, Rng& rng)
         : ciphertext_level(ct_level), ksk_level(key_level) {
         ctx_ksk = par.levels[key_level].ctx;
         ctx_ct = par.levels[ct_level].ctx;
         const Context& k = *ctx_ksk;
-        if (k.L() == 1) throw std::runtime_error("single-modulus key switching not supported");
-        size_t size = ctx_ct->L(), n = k.degree, sz = k.L() * n;
+        if (k.L() == 1) {
+            // generate_c0_decomposition (key_switching_key.rs:172-210): c0_i =
+            // e - c1_i s + 2^(i log_base) from
+            const size_t log_modulus = decomposition_log_modulus(k.moduli[0]);
+            log_base = log_modulus / 2;
+            const size_t size = (log_modulus + log_base - 1) / log_base, n = k.degree;
+            PolyData s = poly_from_i64(k, sk.coeffs);
+            ntt_forward(k, s.data());
+            for (size_t i = 0; i < size; i++) {
+                PolyData c1i = random_poly(k, rng);
+                PolyData as = c1i;
+                poly_mul(k, as.data(), s.data());
+                ntt_backward(k, as.data());
+                PolyData b = poly_from_i64(k, sample_vec_cbd(n, par.variance, rng));
+                poly_sub(k, b.data(), as.data());
+                PolyData gf = from_pb;
+                poly_mul_big(k, gf.data(), Big((u64)1 << (i * log_base)));  // i log_base < 62
+                poly_add(k, b.data(), gf.data());
+                ntt_forward(k, b.data());
+                c0.push_back(b);
+                c1.push_back(c1i);
+            }
+            shoups();
+            return;
+        }
+        size_t size = ctx_ct->L(), n = k.degree;
         PolyData s = poly_from_i64(k, sk.coeffs);
         ntt_forward(k, s.data());
         RnsContext rns(std::vector<u64>(par.moduli.begin(), par.moduli.begin() + size));
@@ -512,7 +546,13 @@ struct KeySwitchingKey {
             c0.push_back(b);
             c1.push_back(c1i);
         }
-        for (size_t i = 0; i < size; i++) {
+        shoups();
+    }
+
+    void shoups() {
+        const Context& k = *ctx_ksk;
+        const size_t n = k.degree, sz = k.L() * n;
+        for (size_t i = 0; i < c0.size(); i++) {
             PolyData a(sz), b(sz);
             for (size_t l = 0; l < k.L(); l++)
                 for (size_t j = 0; j < n; j++) {
@@ -522,6 +562,14 @@ struct KeySwitchingKey {
             c0s.push_back(a);
             c1s.push_back(b);
         }
+    }
+
+    // Number of key rows: one per ciphertext modulus, or the decomposition's
+    // ceil(log_modulus / log_base) digits.
+    size_t digits() const {
+        if (!log_base) return ctx_ct->L();
+        const size_t lm = decomposition_log_modulus(ctx_ksk->moduli[0]);
+        return (lm + log_base - 1) / log_base;
     }
 
     // A key given by its NTT-domain rows c0, c1 [digits][L_key][N].
@@ -530,33 +578,34 @@ struct KeySwitchingKey {
         ctx_ksk = par.levels[key_level].ctx;
         ctx_ct = par.levels[ct_level].ctx;
         const Context& k = *ctx_ksk;
-        size_t n = k.degree, sz = k.L() * n;
-        for (size_t i = 0; i < ctx_ct->L(); i++) {
+        size_t sz = k.L() * k.degree;
+        if (k.L() == 1) log_base = decomposition_log_modulus(k.moduli[0]) / 2;
+        for (size_t i = 0; i < digits(); i++) {
             c0.emplace_back(r0 + i * sz, r0 + (i + 1) * sz);
             c1.emplace_back(r1 + i * sz, r1 + (i + 1) * sz);
-            PolyData a(sz), b(sz);
-            for (size_t l = 0; l < k.L(); l++)
-                for (size_t j = 0; j < n; j++) {
-                    a[l * n + j] = k.q[l].shoup(c0[i][l * n + j]);
-                    b[l * n + j] = k.q[l].shoup(c1[i][l * n + j]);
-                }
-            c0s.push_back(a);
-            c1s.push_back(b);
         }
+        shoups();
     }
 
-    // key_switching_key.rs:214-241 ; p is power basis in ctx_ct. Returns NTT polys.
+    // key_switching_key.rs:214-241 (and key_switch_decomposition, :292-329,
+    // with a one-modulus key context); p is power basis in ctx_ct. Returns
+    // NTT polys.
     void key_switch(const u64* p, PolyData& out0, PolyData& out1) const {
         const Context& k = *ctx_ksk;
         size_t n = k.degree, sz = k.L() * n;
         out0.assign(sz, 0);
         out1.assign(sz, 0);
         PolyData lift(sz);
-        for (size_t i = 0; i < ctx_ct->L(); i++) {
-            // rq/mod.rs:518-540 digit lift: copy, lazy reduce, forward_vt_lazy
+        const u64 mask = log_base ? ((u64)1 << log_base) - 1 : 0;
+        for (size_t i = 0; i < digits(); i++) {
+            // rq/mod.rs:518-540 digit lift: copy, lazy reduce, forward_vt_lazy;
+            // a decomposition digit is bits [i log_base, (i + 1) log_base) of p
             for (size_t l = 0; l < k.L(); l++) {
                 u64* row = lift.data() + l * n;
-                for (size_t j = 0; j < n; j++) row[j] = k.q[l].lazy_reduce_vec_elem(p[i * n + j]);
+                for (size_t j = 0; j < n; j++) {
+                    const u64 d = log_base ? (p[j] >> (i * log_base)) & mask : p[i * n + j];
+                    row[j] = k.q[l].lazy_reduce_vec_elem(d);
+                }
                 k.ops[l].forward_lazy(row);
             }
             for (size_t l = 0; l < k.L(); l++) {

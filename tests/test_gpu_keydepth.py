@@ -105,3 +105,42 @@ def test_deep_galois(fh, oracle, n, sizes, ct_level, key_level):
     got = fh.to_host(out)
     assert np.array_equal(got, bfv.galois(gkey, ct))
     assert [int(x) for x in bfv.decrypt(got, level=ct_level)] == _rotate_columns(v, k)
+
+
+# One-modulus key contexts: the power-of-two decomposition of
+# KeySwitchingKey (key_switching_key.rs:70-86, 172-210, 292-329), used by
+# Galois keys at the last level.
+@pytest.mark.parametrize("n,sizes", [(64, [62, 62]), (64, [37, 40]), (4096, [60, 60]), (16384, [62] * 3),
+                                     (32768, [60] * 3)])
+def test_single_modulus_galois_decomposition(fh, oracle, n, sizes):
+    print(f"decomposition galois n={n} sizes={sizes}", flush=True)
+    t = 257 if n == 64 else T
+    bfv = oracle.Bfv(n, t, oracle.generate_moduli(sizes, n), sk_seed=n % 23)
+    last = len(sizes) - 1
+    ctx = fh.Context(bfv.level_moduli(last), n)
+    rng = random.Random(n)
+    v = [rng.randrange(t) for _ in range(n)]
+    ct = bfv.encrypt(v, level=last, seed=4)
+    for k in (1, 2):
+        e = pow(3, k, 2 * n)
+        gkey = bfv.galois_key(e, ct_level=last, key_level=last, seed=k)
+        c0, c1 = gkey.export()
+        ksk = fh.KeySwitchingKey(ctx, ctx, c0, c1)
+        g = fh.GaloisKey(ksk, e)
+        out = fh.empty(ct.size)
+        g.relinearize(fh.to_device(ct), out, 1)
+        # the key switch itself on a batch of two power-basis polynomials
+        p = oracle.random_polys(bfv.level_moduli(last), n, 2, seed=k)
+        ks = fh.empty(2 * 2 * n)
+        ksk.key_switch(fh.to_device(p), ks, 2)
+        _sync()
+        got = fh.to_host(out)
+        assert np.array_equal(got, bfv.galois(gkey, ct)), k
+        assert [int(x) for x in bfv.decrypt(got, level=last)] == _rotate_columns(v, k)
+        kh = fh.to_host(ks).reshape(2, 2, n)
+        for i in range(2):
+            e0, e1 = gkey.key_switch(p[i * n:(i + 1) * n])
+            assert np.array_equal(kh[i, 0], e0) and np.array_equal(kh[i, 1], e1)
+    # a key with the wrong number of decomposition rows is refused
+    with pytest.raises(fh.FheHipError):
+        fh.KeySwitchingKey(ctx, ctx, c0[:1], c1[:1])

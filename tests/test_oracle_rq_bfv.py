@@ -341,3 +341,28 @@ def test_level_multiplication_basis(oracle):
         lm = moduli[:len(moduli) - level]
         k = -(-(sum(int(q).bit_length() for q in lm) + 60) // 62)
         assert bfv.mul_moduli(level) == [int(q) for q in lm] + ext[:k]
+
+
+@pytest.mark.parametrize("sizes", [[62, 62], [37, 40], [30, 20, 37]])
+def test_galois_rotation_decrypts_single_modulus_decomposition(oracle, sizes):
+    """GaloisKey at the last level, where the key context has one modulus and
+    KeySwitchingKey uses the power-of-two decomposition
+    (fhe/src/bfv/keys/key_switching_key.rs:70-86, 172-210, 292-329): digits
+    of ceil(log2 q / 2) bits; rotations still decrypt to the rotated slots,
+    and the key has ceil(log2 q / log_base) rows of one limb."""
+    n, t = 16, 97
+    bfv = oracle.Bfv(n, t, oracle.generate_moduli(sizes, n), sk_seed=41)
+    last = len(sizes) - 1
+    q = bfv.level_moduli(last)[0]
+    log_modulus = (q - 1).bit_length()
+    digits = -(-log_modulus // (log_modulus // 2))
+    rng = random.Random(len(sizes))
+    v = _slots(rng, n, t)
+    ct = bfv.encrypt(v, level=last, seed=3)
+    for k in [1, 3]:
+        e = pow(3, k, 2 * n)
+        gk = bfv.galois_key(e, ct_level=last, key_level=last, seed=9 + k)
+        c0, c1 = gk.export()
+        assert c0.shape == (digits, 1, n)
+        out = bfv.galois(gk, ct)
+        assert [int(x) for x in bfv.decrypt(out, level=last)] == _rotate_columns(v, k)
