@@ -57,34 +57,34 @@ static void check_batch(u32 inst) {
 }
 
 // ---------------------------------------------------------------------------
-// Rows of up to 16384 coefficients are transformed inside one workgroup's LDS
-// together with the elementwise work around the transform; longer rows
-// (N = 32768) run the unfused pipeline: elementwise kernels around the
-// two-pass NTT.
+// Rows of up to 16384 coefficients fit one workgroup's LDS, where they are
+// transformed together with the elementwise work around the transform.
+// Longer rows (N = 32768) are transformed as two halves, and so are the rows
+// of small launches from N = 4096 (split_rows), to spread them over more CUs.
 static bool fused(const DeviceContext& c) { return c.logn <= 14; }
 
 // Inverse NTT of `parts` polynomials per instance into contiguous rows.
 static void intt_rows(const DeviceContext& c, const InttRowsArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s) {
-    if (fused(c)) {
+    if (!split_rows(c, (size_t)L * parts * inst)) {
         launch_intt_rows(c.logn, a, L, parts, inst, s);
         return;
     }
-    // Degree 32768: half-row inverse transforms, then the stage pairing the halves.
+    // Half-row inverse transforms, then the stage pairing the halves.
     const size_t rows = (size_t)L * c.N;
     if (a.dst_inst_stride != parts * rows || (parts > 1 && a.dst_part_stride != rows))
-        throw Error(-1, "the degree-32768 inverse NTT needs contiguous destination rows");
+        throw Error(-1, "the half-row inverse NTT needs contiguous destination rows");
     launch_intt_rows_half(c, a, L, parts, inst, s);
     launch_ntt_split_inv(c, RowMap{a.dst, L, L, 0, a.table_offset}, inst * parts * L, s);
 }
 
 // Tensor product followed by the inverse NTT of the three output polynomials.
 static void tensor_intt(const DeviceContext& mul, const TensorArgs& a, u32 M, u32 inst, hipStream_t s) {
-    if (fused(mul)) {
+    if (!split_rows(mul, (size_t)3 * M * inst)) {
         launch_tensor_intt(mul.logn, a, M, inst, s);
         return;
     }
     if (a.out_inst_stride != 3 * (size_t)M * mul.N || a.out_part_stride != (size_t)M * mul.N)
-        throw Error(-1, "the degree-32768 tensor product needs contiguous output rows");
+        throw Error(-1, "the half-row tensor product needs contiguous output rows");
     launch_tensor_intt_half(mul, a, M, inst, s);
     launch_ntt_split_inv(mul, RowMap{a.out, M, M, 0, 0}, inst * 3 * M, s);
 }
@@ -252,7 +252,7 @@ size_t Ksk::finish_bytes(u32 inst) const {
 u64* Ksk::lift(const u64* src, size_t src_inst_stride, u32 inst, hipStream_t s) const {
     u64* y = scratch(Scratch::Lift).get(s, lift_bytes(inst)).as<u64>();
     const KeySwitchArgs a = args(src, src_inst_stride);
-    if (fused(*key_ctx)) {
+    if (!split_rows(*key_ctx, (size_t)digits * key_ctx->L * inst)) {
         launch_lift_ntt(key_ctx->logn, a, y, inst, s);
     } else {
         launch_lift_ntt_split(*key_ctx, src, src_inst_stride, y, digits, inst, log_base, s);
@@ -333,17 +333,16 @@ void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* out, u3
     // Inner products of every key limb: the limbs below the dropped one into
     // `out` (NTT domain), the dropped limb into `special`, which alone goes
     // back to power basis; then the transformed power-basis part is added in.
-    const bool fuse = fused(*ct_ctx);
     u64* base = scratch(Scratch::SwitchDown).get(s, finish_bytes(inst)).as<u64>();
     u64* special = extra ? base : nullptr;
     KsFinishArgs f{a, y, fin0.as<u64>(), fin1.as<u64>(), digits, (u32)Lk, special,
                    key_ctx->mod.as<ModInfo>() + (Lk - 1)};
     launch_ks_mac(ct_ctx->logn, f, inst, s);
     if (extra) launch_ntt_backward(*key_ctx, RowMap{special, 1, 1, 0, (u32)(Lk - 1)}, inst * 2, s);
-    if (fuse)
-        launch_ks_finish(ct_ctx->logn, f, inst, s);
-    else
+    if (split_rows(*ct_ctx, (size_t)2 * L * inst))
         launch_ks_finish_half(f, *ct_ctx, inst, s);
+    else
+        launch_ks_finish(ct_ctx->logn, f, inst, s);
 }
 
 // KeySwitchingKey::key_switch (key_switching_key.rs:214-241): power-basis p in

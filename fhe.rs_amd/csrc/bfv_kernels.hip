@@ -162,61 +162,64 @@ __global__ __launch_bounds__((NttShape<LOGN, R>::T)) void tensor_intt_kernel(Ten
 }
 
 // ---------------------------------------------------------------------------
-// Degree 32768 (a row is two halves of 16384): the inverse transform of each
-// half with the Galois gather or the tensor products formed on load, using
-// the half tables (which leave out N^-1; the Montgomery factor 2^-64 is
-// undone by their last stage as in tensor_intt_kernel). The last inverse
-// stage, pairing the halves, follows as ntt_split_inv on the outputs.
+// Rows transformed as two halves of 2^LOGH (degree 32768 always; smaller
+// degrees for small batches, so that a row spreads over two workgroups): the
+// inverse transform of each half with the Galois gather or the tensor
+// products formed on load, using the half tables (which leave out N^-1; the
+// Montgomery factor 2^-64 is undone by their last stage as in
+// tensor_intt_kernel). The last inverse stage, pairing the halves, follows
+// as ntt_split_inv on the outputs.
 // grid = (2 limbs, parts, instances) for the rows: blockIdx.x = 2 limb + half.
-template <bool PERM>
-__global__ __launch_bounds__(NttShape<14>::T) void intt_rows_half_kernel(InttRowsArgs a) {
-    using S = NttShape<14>;
+template <int LOGH, bool PERM>
+__global__ __launch_bounds__(NttShape<LOGH>::T) void intt_rows_half_kernel(InttRowsArgs a) {
+    using S = NttShape<LOGH>;
     __shared__ u64 lds[S::LDS_WORDS];
     const int t = threadIdx.x;
     const u32 l = blockIdx.x >> 1, h = blockIdx.x & 1, k = blockIdx.y, b = blockIdx.z;
-    const u64* src = a.src[k] + (size_t)b * a.src_inst_stride + ((size_t)l << 15);
-    u64* dst = a.dst + (size_t)b * a.dst_inst_stride + (size_t)k * a.dst_part_stride + ((size_t)l << 15) +
-               ((size_t)h << 14);
+    const u64* src = a.src[k] + (size_t)b * a.src_inst_stride + ((size_t)l << (LOGH + 1));
+    u64* dst = a.dst + (size_t)b * a.dst_inst_stride + (size_t)k * a.dst_part_stride + ((size_t)l << (LOGH + 1)) +
+               ((size_t)h << LOGH);
     const u32 row = 2 * (l + a.table_offset) + h;  // half-table row
     const ModInfo mi = a.mods[row];
     u64 x[S::E];
-    const u32 off = h << 14;
+    const u32 off = h << LOGH;
     if constexpr (PERM) {
         const u32* perm = a.perm;
-        load_inv_order<14, 16>(lds, x, t, [&](int j) { return src[perm[off + j]]; });
+        load_inv_order<LOGH, 16>(lds, x, t, [&](int j) { return src[perm[off + j]]; });
     } else {
-        load_inv_order<14, 16>(lds, x, t, [&](int j) { return src[off + j]; });
+        load_inv_order<LOGH, 16>(lds, x, t, [&](int j) { return src[off + j]; });
     }
-    ntt_inv_regs<14>(lds, x, t, a.inv_tw + ((size_t)row << 14), mi.p, inv_scale_n(mi));
-    store_inv_order<14>(dst, x, t);
+    ntt_inv_regs<LOGH>(lds, x, t, a.inv_tw + ((size_t)row << LOGH), mi.p, inv_scale_n(mi));
+    store_inv_order<LOGH>(dst, x, t);
 }
 
 // 1-D grid over (instance, limb, half, product), product fastest.
-__global__ __launch_bounds__(NttShape<14>::T) void tensor_intt_half_kernel(TensorArgs a) {
-    using S = NttShape<14>;
+template <int LOGH>
+__global__ __launch_bounds__(NttShape<LOGH>::T) void tensor_intt_half_kernel(TensorArgs a) {
+    using S = NttShape<LOGH>;
     __shared__ u64 lds[S::LDS_WORDS];
     const int t = threadIdx.x;
     const u32 l = xcd_block(blockIdx.x, gridDim.x);
     const u32 c = l % 3, h = (l / 3) & 1, m = (l / 6) % a.M, b = l / (6 * a.M);
     const ModInfo mi = a.mods[2 * m + h];  // half-table constants
-    const u32 off = h << 14;
+    const u32 off = h << LOGH;
     const u64* rows[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const u32 common = k < 2 ? a.common_lhs : a.common_rhs;
         if (m < common) {
             const u64* base = k < 2 ? a.lhs : a.rhs;
-            rows[k] = base + (size_t)b * a.in_inst_stride + (size_t)(k & 1) * a.in_part_stride + ((size_t)m << 15) + off;
+            rows[k] = base + (size_t)b * a.in_inst_stride + (size_t)(k & 1) * a.in_part_stride + ((size_t)m << (LOGH + 1)) + off;
         } else {
             rows[k] = a.ext + (size_t)b * a.ext_inst_stride + (size_t)k * a.ext_part_stride +
-                      ((size_t)(m - common) << 15) + off;
+                      ((size_t)(m - common) << (LOGH + 1)) + off;
         }
     }
     const u64 p = mi.p, pn = mi.pinv_neg;
     u64 x[S::E];
     if (c == 1) {
         const u64 *ra0 = rows[0], *ra1 = rows[1], *rb0 = rows[2], *rb1 = rows[3];
-        load_inv_order<14, 2>(lds, x, t, [&](int j) {
+        load_inv_order<LOGH, 2>(lds, x, t, [&](int j) {
             u64 lo, hi, l2, h2;
             mul_wide(ra0[j], rb1[j], lo, hi);
             mul_wide(ra1[j], rb0[j], l2, h2);
@@ -226,15 +229,15 @@ __global__ __launch_bounds__(NttShape<14>::T) void tensor_intt_half_kernel(Tenso
     } else {
         const u64* ra = rows[c == 0 ? 0 : 1];
         const u64* rb = rows[c == 0 ? 2 : 3];
-        load_inv_order<14>(lds, x, t, [&](int j) {
+        load_inv_order<LOGH>(lds, x, t, [&](int j) {
             u64 lo, hi;
             mul_wide(ra[j], rb[j], lo, hi);
             return mont_redc(lo, hi, p, pn);
         });
     }
-    ntt_inv_regs<14>(lds, x, t, a.inv_tw + ((size_t)(2 * m + h) << 14), p, inv_scale_nr(mi));
-    u64* dst = a.out + (size_t)b * a.out_inst_stride + (size_t)c * a.out_part_stride + ((size_t)m << 15) + off;
-    store_inv_order<14>(dst, x, t);
+    ntt_inv_regs<LOGH>(lds, x, t, a.inv_tw + ((size_t)(2 * m + h) << LOGH), p, inv_scale_nr(mi));
+    u64* dst = a.out + (size_t)b * a.out_inst_stride + (size_t)c * a.out_part_stride + ((size_t)m << (LOGH + 1)) + off;
+    store_inv_order<LOGH>(dst, x, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -511,16 +514,17 @@ __global__ __launch_bounds__((NttShape<LOGN, R>::T)) void ks_finish_kernel(KsFin
     store_fwd_order<LOGN, R, 16>(lds, x, t, [&](int j, u64 v) { dst[j] = add_mod(reduce1(reduce1(v, p2), p), dst[j], p); });
 }
 
-// The same finish at N = 32768, where a row is transformed as two halves:
-// the workgroup of half h forms the power-basis part z (as ks_finish_kernel)
-// at both coefficients j and j + 16384 of each pair it owns, takes half h of
+// The same finish with rows transformed as two halves of 2^LOGH: the
+// workgroup of half h forms the power-basis part z (as ks_finish_kernel)
+// at both coefficients j and j + 2^LOGH of each pair it owns, takes half h of
 // the stage-0 butterfly on load (native.rs:256-269), runs the remaining 14
 // stages with the half tables and adds the reduced result into `out`. 1-D
 // grid over (instance, part, limb, half), half fastest.
-__global__ __launch_bounds__(NttShape<14>::T) void ks_finish_half_kernel(KsFinishArgs f,
+template <int LOGH>
+__global__ __launch_bounds__(NttShape<LOGH>::T) void ks_finish_half_kernel(KsFinishArgs f,
                                                                         const ulonglong2* __restrict__ half_tw,
                                                                         const ulonglong2* __restrict__ split) {
-    using S = NttShape<14>;
+    using S = NttShape<LOGH>;
     __shared__ u64 lds[S::LDS_WORDS];
     const SdNttArgs& a = f.sd;
     const int t = threadIdx.x;
@@ -531,9 +535,9 @@ __global__ __launch_bounds__(NttShape<14>::T) void ks_finish_half_kernel(KsFinis
     const u64 p = mi.p, p2 = 2 * p;
     const ulonglong2 w = split[3 * i];
     const u64* ad = a.add_pb ? a.add_pb + (size_t)b * a.add_inst_stride + (size_t)part * a.add_part_stride +
-                                   ((size_t)i << 15)
+                                   ((size_t)i << (LOGH + 1))
                              : nullptr;
-    const u64* lrow = a.extra ? f.special + (((size_t)b * 2 + part) << 15) : nullptr;
+    const u64* lrow = a.extra ? f.special + (((size_t)b * 2 + part) << (LOGH + 1)) : nullptr;
     const SdConst cst = a.sd[i];
     const u64 ql = a.extra ? a.extra_mods[0] : 0, half = ql >> 1;
     // z = add_pb - q_l^-1 [y]_i (add_pb alone without an extra modulus); see
@@ -551,13 +555,13 @@ __global__ __launch_bounds__(NttShape<14>::T) void ks_finish_half_kernel(KsFinis
     u64 x[S::E];
 #pragma unroll
     for (int e = 0; e < S::E; e++) {
-        const int k = fwd_index<14, 0>(t, e);
-        x[e] = split_half(z_at(k), z_at(k + 16384), h, w.x, w.y, p, p2);
+        const int k = fwd_index<LOGH, 0>(t, e);
+        x[e] = split_half(z_at(k), z_at(k + (1 << LOGH)), h, w.x, w.y, p, p2);
     }
-    ntt_fwd_regs<14>(lds, x, t, half_tw + ((size_t)(2 * i + h) << 14), p);
-    u64* dst = a.out + (size_t)b * a.out_inst_stride + (size_t)part * a.out_part_stride + ((size_t)i << 15) +
-               ((size_t)h << 14);
-    store_fwd_order<14, 4, 16>(lds, x, t,
+    ntt_fwd_regs<LOGH>(lds, x, t, half_tw + ((size_t)(2 * i + h) << LOGH), p);
+    u64* dst = a.out + (size_t)b * a.out_inst_stride + (size_t)part * a.out_part_stride + ((size_t)i << (LOGH + 1)) +
+               ((size_t)h << LOGH);
+    store_fwd_order<LOGH, 4, 16>(lds, x, t,
                                [&](int k, u64 v) { dst[k] = add_mod(reduce1(reduce1(v, p2), p), dst[k], p); });
 }
 
@@ -653,31 +657,49 @@ void launch_intt_rows(int logn, const InttRowsArgs& a, u32 limbs, u32 parts, u32
     }
 }
 
+#define FHEHIP_LOGH_DISPATCH(logh, F, ...)                                     \
+    switch (logh) {                                                             \
+        case 11: F<11>(__VA_ARGS__); break;                                     \
+        case 12: F<12>(__VA_ARGS__); break;                                     \
+        case 13: F<13>(__VA_ARGS__); break;                                     \
+        case 14: F<14>(__VA_ARGS__); break;                                     \
+        default: throw Error(-2, "half-row transforms need a degree of 4096 .. 32768"); \
+    }
+
+template <int LOGH>
+static void intt_rows_half_go(const InttRowsArgs& h, dim3 grid, hipStream_t s) {
+    if (h.perm)
+        hipLaunchKernelGGL((intt_rows_half_kernel<LOGH, true>), grid, dim3(NttShape<LOGH>::T), 0, s, h);
+    else
+        hipLaunchKernelGGL((intt_rows_half_kernel<LOGH, false>), grid, dim3(NttShape<LOGH>::T), 0, s, h);
+}
+
 void launch_intt_rows_half(const DeviceContext& c, const InttRowsArgs& a, u32 limbs, u32 parts, u32 inst,
                            hipStream_t s) {
     if (!limbs || !parts || !inst) return;
-    if (c.logn != 15) throw Error(-2, "the half-row inverse transforms are for degree 32768");
-    ProfScope ps(a.perm ? "intt_rows_gather" : "intt_rows", 15, s);
+    if (!c.half) throw Error(-2, "this context has no half-row tables");
+    ProfScope ps(a.perm ? "intt_rows_gather" : "intt_rows", c.logn, s);
     InttRowsArgs h = a;
     h.inv_tw = c.half->inv_tw.as<ulonglong2>();
     h.mods = c.half->mod.as<ModInfo>();
-    dim3 grid(2 * limbs, parts, inst);
-    if (a.perm)
-        hipLaunchKernelGGL(intt_rows_half_kernel<true>, grid, dim3(NttShape<14>::T), 0, s, h);
-    else
-        hipLaunchKernelGGL(intt_rows_half_kernel<false>, grid, dim3(NttShape<14>::T), 0, s, h);
+    FHEHIP_LOGH_DISPATCH(c.logn - 1, intt_rows_half_go, h, dim3(2 * limbs, parts, inst), s);
     HIPCHK(hipGetLastError());
+}
+
+template <int LOGH>
+static void tensor_half_go(const TensorArgs& b, dim3 grid, hipStream_t s) {
+    hipLaunchKernelGGL(tensor_intt_half_kernel<LOGH>, grid, dim3(NttShape<LOGH>::T), 0, s, b);
 }
 
 void launch_tensor_intt_half(const DeviceContext& mul, const TensorArgs& a, u32 M, u32 inst, hipStream_t s) {
     if (!M || !inst) return;
-    if (mul.logn != 15) throw Error(-2, "the half-row tensor product is for degree 32768");
-    ProfScope ps("tensor_intt", 15, s);
+    if (!mul.half) throw Error(-2, "this context has no half-row tables");
+    ProfScope ps("tensor_intt", mul.logn, s);
     TensorArgs b = a;
     b.M = M;
     b.inv_tw = mul.half->inv_tw.as<ulonglong2>();
     b.mods = mul.half->mod.as<ModInfo>();
-    hipLaunchKernelGGL(tensor_intt_half_kernel, dim3(M * 6 * inst), dim3(NttShape<14>::T), 0, s, b);
+    FHEHIP_LOGH_DISPATCH(mul.logn - 1, tensor_half_go, b, dim3(M * 6 * inst), s);
     HIPCHK(hipGetLastError());
 }
 
@@ -726,12 +748,17 @@ void launch_ks_finish(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s) 
     FHEHIP_LOGN_R_DISPATCH(logn, tu.re_finish, ks_finish_kernel, grid, s, g);
 }
 
+template <int LOGH>
+static void ks_finish_half_go(const KsFinishArgs& f, const DeviceContext& ct, dim3 grid, hipStream_t s) {
+    hipLaunchKernelGGL(ks_finish_half_kernel<LOGH>, grid, dim3(NttShape<LOGH>::T), 0, s, f,
+                       ct.half->fwd_tw.as<ulonglong2>(), ct.split.as<ulonglong2>());
+}
+
 void launch_ks_finish_half(const KsFinishArgs& f, const DeviceContext& ct, u32 inst, hipStream_t s) {
     if (!f.sd.L || !inst) return;
-    if (ct.logn != 15) throw Error(-2, "the half-row key-switch finish is for degree 32768");
-    ProfScope ps("ks_finish", 15, s);
-    hipLaunchKernelGGL(ks_finish_half_kernel, dim3(inst * 2 * f.sd.L * 2), dim3(NttShape<14>::T), 0, s, f,
-                       ct.half->fwd_tw.as<ulonglong2>(), ct.split.as<ulonglong2>());
+    if (!ct.half) throw Error(-2, "this context has no half-row tables");
+    ProfScope ps("ks_finish", ct.logn, s);
+    FHEHIP_LOGH_DISPATCH(ct.logn - 1, ks_finish_half_go, f, ct, dim3(inst * 2 * f.sd.L * 2), s);
     HIPCHK(hipGetLastError());
 }
 

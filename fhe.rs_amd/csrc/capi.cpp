@@ -74,6 +74,13 @@ int fhehip_ntt_set_shape(uint32_t degree, uint32_t fwd, uint32_t inv) {
     FHEHIP_API_END
 }
 
+int fhehip_split_row_limit(uint32_t max_rows, uint32_t* previous) {
+    FHEHIP_API_BEGIN
+    if (previous) *previous = split_row_limit();
+    set_split_row_limit(max_rows);
+    FHEHIP_API_END
+}
+
 int fhehip_ntt_autotune(const fhehip_context* ctx, uint32_t rows, uint32_t* fwd, uint32_t* inv, float* ms_out) {
     FHEHIP_API_BEGIN
     if (!ctx || !fwd || !inv || !rows) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");

@@ -88,16 +88,17 @@ std::shared_ptr<DeviceContext> make_context(const std::vector<u64>& moduli, u32 
         set_last_stage(mi, inv[(size_t)l * n + n - 2].x);
         c->h_mod.push_back(mi);
     }
-    if (n == 32768) {
+    if (n >= 4096) {
         // Half tables (see DeviceContext::half). Forward: local index k = 2^s + g
-        // of a 16384-point transform at stage s maps to the global index
+        // of an N/2-point transform at stage s maps to the global index
         // 2^(s+1) + h 2^s + g. Inverse: local (N' - (N' >> v)) + g maps to
-        // (N - (N >> v)) + h 2^(13 - v) + g.
+        // (N - (N >> v)) + h 2^(logn - 2 - v) + g.
         const u32 nh = n / 2;
+        const u32 lh = (u32)c->logn - 1;
         auto hc = std::make_shared<DeviceContext>();
         hc->L = 2 * c->L;
         hc->N = nh;
-        hc->logn = 14;
+        hc->logn = (int)lh;
         std::vector<ulonglong2> hf((size_t)hc->L * nh), hi((size_t)hc->L * nh);
         std::vector<ulonglong2> sp;
         for (u32 l = 0; l < c->L; l++) {
@@ -110,9 +111,10 @@ std::shared_ptr<DeviceContext> make_context(const std::vector<u64>& moduli, u32 
                     const u32 sft = 31 - __builtin_clz(k);
                     F[k] = fwd[(size_t)l * n + k + ((1u + h) << sft)];
                 }
-                for (u32 v = 0; v < 14; v++) {
+                for (u32 v = 0; v < lh; v++) {
                     const u32 lo = nh - (nh >> v), go = n - (n >> v);
-                    for (u32 g = 0; g < (1u << (13 - v)); g++) I[lo + g] = inv[(size_t)l * n + go + (h << (13 - v)) + g];
+                    for (u32 g = 0; g < (1u << (lh - 1 - v)); g++)
+                        I[lo + g] = inv[(size_t)l * n + go + (h << (lh - 1 - v)) + g];
                 }
                 I[nh - 1] = inv[(size_t)l * n + n - 1];
                 ModInfo mi = c->h_mod[l];

@@ -49,11 +49,13 @@ struct DeviceContext {
     DevBuf inv_tw;   // [L][N] ulonglong2 (zeta_inv, zeta_inv_shoup)
     DevBuf mod;      // [L] ModInfo
     RnsBasis rns;    // host-side CRT constants
-    // N = 32768 only: the forward transform's first stage and the inverse's
-    // last stage pair the two halves of a row; each half is then an
-    // independent 16384-point transform. `half` is a context of 2L rows of
-    // degree 16384 whose tables are the full tables re-indexed for half h of
-    // limb l at row 2l + h (its inverse leaves out the N^-1 factor).
+    // N >= 4096: a row can be transformed as two halves. The forward
+    // transform's first stage and the inverse's last stage pair the two halves
+    // of a row; each half is then an independent N/2-point transform. `half`
+    // is a context of 2L rows of degree N/2 whose tables are the full tables
+    // re-indexed for half h of limb l at row 2l + h (its inverse leaves out the
+    // N^-1 factor). Degree 32768 always runs this way (a row does not fit one
+    // workgroup's LDS); smaller degrees do for small launches (split_rows).
     std::shared_ptr<DeviceContext> half;
     DevBuf split;    // [L] ulonglong2 pairs: (omega_1, shoup), (zeta_last * N^-1, shoup), (N^-1, shoup)
 
@@ -72,6 +74,15 @@ struct RowMap {
     u32 first_limb;
     u32 table_offset;  // context limb index of limb first_limb
 };
+
+// Whether a launch of `rows` rows of context c transforms each row as two
+// halves: always at degree 32768; at 4096 .. 16384 when the launch has at
+// most split_row_limit() rows (fewer workgroups than the chip has CUs, so
+// that two workgroups per row shorten the launch).
+bool split_rows(const DeviceContext& c, size_t rows);
+u32 split_row_limit();
+// Sets the limit (0: only degree 32768 splits); process-wide.
+void set_split_row_limit(u32 rows);
 
 // NTT launchers (ntt_kernels.hip). `rows` = total rows in the launch.
 void launch_ntt_forward(const DeviceContext& c, RowMap m, u32 rows, bool lazy, hipStream_t s);

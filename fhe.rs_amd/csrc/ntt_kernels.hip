@@ -115,34 +115,45 @@ static void inv_launch(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s
         default: throw Error(-2, "degree not supported by the NTT kernels"); \
     }
 
-// N = 32768: first forward stage (pairs j, j + N/2 with omega_1; Harvey
-// butterfly of native.rs:256-269, outputs < 4p). grid = (N / 512, rows)
+// Rows transformed as two halves: the first forward stage (pairs j,
+// j + N/2 with omega_1; Harvey butterfly of native.rs:256-269, outputs < 4p).
+// grid = (N / 512, rows)
 __global__ __launch_bounds__(256) void ntt_split_fwd_kernel(RowMap m, const ulonglong2* __restrict__ split,
-                                                            const ModInfo* __restrict__ mods) {
+                                                            const ModInfo* __restrict__ mods, u32 logn) {
     u32 limb;
-    u64* row = row_ptr(m, blockIdx.y, 15, limb);
-    const u32 j = blockIdx.x * 256 + threadIdx.x;
+    u64* row = row_ptr(m, blockIdx.y, logn, limb);
+    const u32 j = blockIdx.x * 256 + threadIdx.x, half = 1u << (logn - 1);
     const ulonglong2 w = split[3 * limb];
     const u64 p = mods[limb].p;
-    u64 x = row[j], y = row[j + 16384];
+    u64 x = row[j], y = row[j + half];
     fwd_bfly<true>(x, y, w.x, w.y, p, 2 * p);
     row[j] = x;
-    row[j + 16384] = y;
+    row[j + half] = y;
 }
 
-// N = 32768: last inverse stage on the two reduced half transforms, with the
-// N^-1 factor folded in: X = (x + y) N^-1, Y = (x - y) zeta N^-1 (native.rs
-// :288-300 and :130-131).
+// The last inverse stage on the two reduced half transforms, with the N^-1
+// factor folded in: X = (x + y) N^-1, Y = (x - y) zeta N^-1 (native.rs
+// :288-300 and :130-131). grid = (N / 512, rows)
 __global__ __launch_bounds__(256) void ntt_split_inv_kernel(RowMap m, const ulonglong2* __restrict__ split,
-                                                            const ModInfo* __restrict__ mods) {
+                                                            const ModInfo* __restrict__ mods, u32 logn) {
     u32 limb;
-    u64* row = row_ptr(m, blockIdx.y, 15, limb);
-    const u32 j = blockIdx.x * 256 + threadIdx.x;
+    u64* row = row_ptr(m, blockIdx.y, logn, limb);
+    const u32 j = blockIdx.x * 256 + threadIdx.x, half = 1u << (logn - 1);
     const ulonglong2 z = split[3 * limb + 1], ni = split[3 * limb + 2];
     const u64 p = mods[limb].p;
-    const u64 x = row[j], y = row[j + 16384];
+    const u64 x = row[j], y = row[j + half];
     row[j] = mul_shoup_u(x + y, ni.x, ni.y, p);
-    row[j + 16384] = mul_shoup_u(x + 2 * p - y, z.x, z.y, p);
+    row[j + half] = mul_shoup_u(x + 2 * p - y, z.x, z.y, p);
+}
+
+static std::atomic<u32> g_split_rows{128};
+
+u32 split_row_limit() { return g_split_rows.load(std::memory_order_relaxed); }
+void set_split_row_limit(u32 rows) { g_split_rows.store(rows, std::memory_order_relaxed); }
+
+bool split_rows(const DeviceContext& c, size_t rows) {
+    if (c.logn == 15) return true;
+    return c.logn >= 12 && c.half && rows <= split_row_limit();
 }
 
 // N = 32768 digit lift (rq/mod.rs:518-540) with its whole lazy forward
@@ -152,13 +163,14 @@ __global__ __launch_bounds__(256) void ntt_split_inv_kernel(RowMap m, const ulon
 // the half tables; outputs < 2 q'_j, as lift_ntt_kernel's. 1-D grid over
 // (instance, digit, key limb, half), half fastest: the two halves of one row
 // and the key limbs of one digit read the same source row and run on one XCD.
-__global__ __launch_bounds__(NttShape<14>::T) void lift_ntt_half_kernel(const u64* __restrict__ src, size_t src_inst,
+template <int LOGH>
+__global__ __launch_bounds__(NttShape<LOGH>::T) void lift_ntt_half_kernel(const u64* __restrict__ src, size_t src_inst,
                                                                        u64* __restrict__ y,
                                                                        const ulonglong2* __restrict__ half_tw,
                                                                        const ModInfo* __restrict__ mods,
                                                                        const ulonglong2* __restrict__ split,
                                                                        u32 digits, u32 key_limbs, u32 log_base) {
-    using S = NttShape<14>;
+    using S = NttShape<LOGH>;
     __shared__ u64 lds[S::LDS_WORDS];
     const int t = threadIdx.x;
     const u32 l = xcd_block(blockIdx.x, gridDim.x);
@@ -167,7 +179,7 @@ __global__ __launch_bounds__(NttShape<14>::T) void lift_ntt_half_kernel(const u6
     const ModInfo mi = mods[j];
     const u64 p = mi.p, p2 = 2 * p;
     const ulonglong2 w = split[3 * j];
-    const u64* row = src + b * src_inst + ((size_t)(log_base ? 0 : i) << 15);
+    const u64* row = src + b * src_inst + ((size_t)(log_base ? 0 : i) << (LOGH + 1));
     u64 x[S::E];
     // Digit i is below q_i (a prefix of the key moduli); below 4 q'_j it is
     // already a valid lazy transform input (as in lift_ntt_kernel), and so is
@@ -177,37 +189,48 @@ __global__ __launch_bounds__(NttShape<14>::T) void lift_ntt_half_kernel(const u6
         const u64 mask = ((u64)1 << log_base) - 1;
 #pragma unroll
         for (int e = 0; e < S::E; e++) {
-            const int k = fwd_index<14, 0>(t, e);
-            x[e] = split_half((row[k] >> sh) & mask, (row[k + 16384] >> sh) & mask, h, w.x, w.y, p, p2);
+            const int k = fwd_index<LOGH, 0>(t, e);
+            x[e] = split_half((row[k] >> sh) & mask, (row[k + (1 << LOGH)] >> sh) & mask, h, w.x, w.y, p, p2);
         }
     } else if (mods[i].p <= 4 * p) {
 #pragma unroll
         for (int e = 0; e < S::E; e++) {
-            const int k = fwd_index<14, 0>(t, e);
-            x[e] = split_half(row[k], row[k + 16384], h, w.x, w.y, p, p2);
+            const int k = fwd_index<LOGH, 0>(t, e);
+            x[e] = split_half(row[k], row[k + (1 << LOGH)], h, w.x, w.y, p, p2);
         }
     } else {
 #pragma unroll
         for (int e = 0; e < S::E; e++) {
-            const int k = fwd_index<14, 0>(t, e);
+            const int k = fwd_index<LOGH, 0>(t, e);
             const u64 a = lazy_reduce_u64(row[k], mi.barrett_lo, mi.barrett_hi, p);
-            const u64 c = lazy_reduce_u64(row[k + 16384], mi.barrett_lo, mi.barrett_hi, p);
+            const u64 c = lazy_reduce_u64(row[k + (1 << LOGH)], mi.barrett_lo, mi.barrett_hi, p);
             x[e] = split_half(a, c, h, w.x, w.y, p, p2);
         }
     }
-    ntt_fwd_regs<14>(lds, x, t, half_tw + ((size_t)(2 * j + h) << 14), p);
-    u64* dst = y + ((((size_t)b * digits + i) * key_limbs + j) << 15) + ((size_t)h << 14);
-    store_fwd_order<14>(lds, x, t, [&](int k, u64 v) { dst[k] = reduce1(v, p2); });
+    ntt_fwd_regs<LOGH>(lds, x, t, half_tw + ((size_t)(2 * j + h) << LOGH), p);
+    u64* dst = y + ((((size_t)b * digits + i) * key_limbs + j) << (LOGH + 1)) + ((size_t)h << LOGH);
+    store_fwd_order<LOGH>(lds, x, t, [&](int k, u64 v) { dst[k] = reduce1(v, p2); });
+}
+
+template <int LOGH>
+static void lift_half_go(dim3 grid, hipStream_t s, const u64* src, size_t src_inst, u64* y, const ulonglong2* tw,
+                         const ModInfo* mods, const ulonglong2* split, u32 digits, u32 key_limbs, u32 log_base) {
+    hipLaunchKernelGGL(lift_ntt_half_kernel<LOGH>, grid, dim3(NttShape<LOGH>::T), 0, s, src, src_inst, y, tw, mods,
+                       split, digits, key_limbs, log_base);
 }
 
 void launch_lift_ntt_split(const DeviceContext& key, const u64* src, size_t src_inst, u64* y, u32 digits, u32 inst,
                            u32 log_base, hipStream_t s) {
-    if (key.logn != 15) throw Error(-2, "the fused lift and split stage is for degree 32768");
+    if (!key.half) throw Error(-2, "this context has no half-row tables");
     if (!digits || !inst) return;
-    ProfScope ps("lift_ntt", 15, s);
-    hipLaunchKernelGGL(lift_ntt_half_kernel, dim3(2 * inst * digits * key.L), dim3(NttShape<14>::T), 0, s, src,
-                       src_inst, y, key.half->fwd_tw.as<ulonglong2>(), key.mod.as<ModInfo>(),
-                       key.split.as<ulonglong2>(), digits, key.L, log_base);
+    ProfScope ps("lift_ntt", key.logn, s);
+    switch (key.logn - 1) {
+        case 11: lift_half_go<11>(dim3(2 * inst * digits * key.L), s, src, src_inst, y, key.half->fwd_tw.as<ulonglong2>(), key.mod.as<ModInfo>(), key.split.as<ulonglong2>(), digits, key.L, log_base); break;
+        case 12: lift_half_go<12>(dim3(2 * inst * digits * key.L), s, src, src_inst, y, key.half->fwd_tw.as<ulonglong2>(), key.mod.as<ModInfo>(), key.split.as<ulonglong2>(), digits, key.L, log_base); break;
+        case 13: lift_half_go<13>(dim3(2 * inst * digits * key.L), s, src, src_inst, y, key.half->fwd_tw.as<ulonglong2>(), key.mod.as<ModInfo>(), key.split.as<ulonglong2>(), digits, key.L, log_base); break;
+        case 14: lift_half_go<14>(dim3(2 * inst * digits * key.L), s, src, src_inst, y, key.half->fwd_tw.as<ulonglong2>(), key.mod.as<ModInfo>(), key.split.as<ulonglong2>(), digits, key.L, log_base); break;
+        default: throw Error(-2, "half-row transforms need a degree of 4096 .. 32768");
+    }
     HIPCHK(hipGetLastError());
 }
 
@@ -218,11 +241,11 @@ static RowMap half_rows(RowMap m) {
 void launch_ntt_forward(const DeviceContext& c, RowMap m, u32 rows, bool lazy, hipStream_t s) {
     if (rows == 0) return;
     ProfScope ps(lazy ? "ntt_forward_lazy" : "ntt_forward", c.logn, s);
-    if (c.logn == 15) {
-        hipLaunchKernelGGL(ntt_split_fwd_kernel, dim3(16384 / 256, rows), dim3(256), 0, s, m,
-                           c.split.as<ulonglong2>(), c.mod.as<ModInfo>());
+    if (split_rows(c, rows)) {
+        hipLaunchKernelGGL(ntt_split_fwd_kernel, dim3((c.N / 2) / 256, rows), dim3(256), 0, s, m,
+                           c.split.as<ulonglong2>(), c.mod.as<ModInfo>(), (u32)c.logn);
         HIPCHK(hipGetLastError());
-        fwd_launch<14>(*c.half, half_rows(m), 2 * rows, lazy, s);
+        FHEHIP_LOGN_SWITCH(c.logn - 1, fwd_launch, *c.half, half_rows(m), 2 * rows, lazy, s);
     } else {
         FHEHIP_LOGN_SWITCH(c.logn, fwd_launch, c, m, rows, lazy, s);
     }
@@ -231,21 +254,21 @@ void launch_ntt_forward(const DeviceContext& c, RowMap m, u32 rows, bool lazy, h
 
 void launch_ntt_split_inv(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s) {
     if (rows == 0) return;
-    if (c.logn != 15) throw Error(-2, "the split inverse stage is for degree 32768");
-    ProfScope ps("ntt_split_inv", 15, s);
-    hipLaunchKernelGGL(ntt_split_inv_kernel, dim3(16384 / 256, rows), dim3(256), 0, s, m, c.split.as<ulonglong2>(),
-                       c.mod.as<ModInfo>());
+    if (!c.half) throw Error(-2, "this context has no half-row tables");
+    ProfScope ps("ntt_split_inv", c.logn, s);
+    hipLaunchKernelGGL(ntt_split_inv_kernel, dim3((c.N / 2) / 256, rows), dim3(256), 0, s, m,
+                       c.split.as<ulonglong2>(), c.mod.as<ModInfo>(), (u32)c.logn);
     HIPCHK(hipGetLastError());
 }
 
 void launch_ntt_backward(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s) {
     if (rows == 0) return;
     ProfScope ps("ntt_backward", c.logn, s);
-    if (c.logn == 15) {
-        inv_launch<14>(*c.half, half_rows(m), 2 * rows, s);
+    if (split_rows(c, rows)) {
+        FHEHIP_LOGN_SWITCH(c.logn - 1, inv_launch, *c.half, half_rows(m), 2 * rows, s);
         HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL(ntt_split_inv_kernel, dim3(16384 / 256, rows), dim3(256), 0, s, m,
-                           c.split.as<ulonglong2>(), c.mod.as<ModInfo>());
+        hipLaunchKernelGGL(ntt_split_inv_kernel, dim3((c.N / 2) / 256, rows), dim3(256), 0, s, m,
+                           c.split.as<ulonglong2>(), c.mod.as<ModInfo>(), (u32)c.logn);
     } else {
         FHEHIP_LOGN_SWITCH(c.logn, inv_launch, c, m, rows, s);
     }

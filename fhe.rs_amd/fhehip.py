@@ -36,6 +36,7 @@ SIGNATURES = {
     "fhehip_ntt_backward": (C.c_int, [_vp, _vp, C.c_uint32, _vp]),
     "fhehip_ntt_autotune": (C.c_int, [_vp, C.c_uint32, _u32p, _u32p, C.POINTER(C.c_float)]),
     "fhehip_ntt_set_shape": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint32]),
+    "fhehip_split_row_limit": (C.c_int, [C.c_uint32, _u32p]),
     "fhehip_scaler_create": (C.c_int, [_vp, _vp, _vp, _vpp]),
     "fhehip_scaler_destroy": (None, [_vp]),
     "fhehip_scaler_scale": (C.c_int, [_vp, _vp, _vp, C.c_uint32, C.c_int, _vp]),
@@ -560,6 +561,13 @@ class EvaluationKey:
         v = C.c_uint32()
         check(lib().fhehip_expansion_levels(self.h, C.byref(v)))
         return v.value
+
+
+def split_row_limit(max_rows):
+    """fhehip_split_row_limit: sets the half-row launch limit, returns the old one."""
+    prev = C.c_uint32()
+    check(lib().fhehip_split_row_limit(max_rows, C.byref(prev)))
+    return prev.value
 
 
 def ntt_set_shape(degree, fwd, inv):

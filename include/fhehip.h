@@ -88,6 +88,13 @@ int fhehip_ntt_autotune(const fhehip_context* ctx, uint32_t rows, uint32_t* fwd,
 /* Sets the shape (4 or 5 log2 values per thread) of the plain NTT kernels for
  * one degree directly, as the autotuner does. */
 int fhehip_ntt_set_shape(uint32_t degree, uint32_t fwd, uint32_t inv);
+/* No fhe.rs counterpart. Launches of at most `max_rows` polynomial rows at
+ * degrees 4096 .. 16384 transform each row as two halves of N/2 on two
+ * workgroups (degree 32768 always does), which shortens small launches
+ * (single calls, small serving batches); 0 keeps every row on one workgroup.
+ * Process-wide; the default is 128. *previous (may be NULL) receives the old
+ * value. Results are identical either way. */
+int fhehip_split_row_limit(uint32_t max_rows, uint32_t* previous);
 
 /* ---- Scaling (RNS base extension and t/Q rescaling) ----------------------
  * A scaling factor numerator/denominator as little-endian 32-bit words
