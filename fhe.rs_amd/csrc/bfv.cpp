@@ -64,29 +64,38 @@ static void check_batch(u32 inst) {
 static bool fused(const DeviceContext& c) { return c.logn <= 14; }
 
 // Inverse NTT of `parts` polynomials per instance into contiguous rows.
-static void intt_rows(const DeviceContext& c, const InttRowsArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s) {
+// Rows transformed as halves end with the stage pairing the halves, unless
+// `close` is false: then that stage is left to the consumer (a paired
+// scaling) and the function returns true.
+static bool intt_rows(const DeviceContext& c, const InttRowsArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s,
+                      bool close = true) {
     if (!split_rows(c, (size_t)L * parts * inst)) {
         launch_intt_rows(c.logn, a, L, parts, inst, s);
-        return;
+        return false;
     }
-    // Half-row inverse transforms, then the stage pairing the halves.
     const size_t rows = (size_t)L * c.N;
     if (a.dst_inst_stride != parts * rows || (parts > 1 && a.dst_part_stride != rows))
         throw Error(-1, "the half-row inverse NTT needs contiguous destination rows");
     launch_intt_rows_half(c, a, L, parts, inst, s);
+    if (!close) return true;
     launch_ntt_split_inv(c, RowMap{a.dst, L, L, 0, a.table_offset}, inst * parts * L, s);
+    return false;
 }
 
-// Tensor product followed by the inverse NTT of the three output polynomials.
-static void tensor_intt(const DeviceContext& mul, const TensorArgs& a, u32 M, u32 inst, hipStream_t s) {
+// Tensor product followed by the inverse NTT of the three output polynomials
+// (`close` as for intt_rows).
+static bool tensor_intt(const DeviceContext& mul, const TensorArgs& a, u32 M, u32 inst, hipStream_t s,
+                        bool close = true) {
     if (!split_rows(mul, (size_t)3 * M * inst)) {
         launch_tensor_intt(mul.logn, a, M, inst, s);
-        return;
+        return false;
     }
     if (a.out_inst_stride != 3 * (size_t)M * mul.N || a.out_part_stride != (size_t)M * mul.N)
         throw Error(-1, "the half-row tensor product needs contiguous output rows");
     launch_tensor_intt_half(mul, a, M, inst, s);
+    if (!close) return true;
     launch_ntt_split_inv(mul, RowMap{a.out, M, M, 0, 0}, inst * 3 * M, s);
+    return false;
 }
 
 // ---------------------------------------------------------------------------
@@ -104,6 +113,27 @@ PolyScaler::PolyScaler(std::shared_ptr<DeviceContext> f, std::shared_ptr<DeviceC
     std::vector<u64> blob = scaler_blob(consts, to->h_mod, common);
     d_consts.reset(blob.size() * sizeof(u64));
     HIPCHK(hipMemcpy(d_consts.ptr, blob.data(), d_consts.bytes, hipMemcpyHostToDevice));
+}
+
+ScalePairArgs PolyScaler::pair(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst,
+                               size_t out_part, bool in_open, bool out_fwd) const {
+    ScalePairArgs a{};
+    a.in = in;
+    a.in_inst = in_inst;
+    a.in_part = in_part;
+    a.out = out;
+    a.out_inst = out_inst;
+    a.out_part = out_part;
+    a.sc = d_consts.as<u64>();
+    a.sc_words = words();
+    a.count = to->L - common;
+    a.n = (u32)from->N;
+    if (in_open) {
+        a.in_split = from->split.as<ulonglong2>();
+        a.in_mods = from->mod.as<ModInfo>();
+    }
+    if (out_fwd) a.out_split = to->split.as<ulonglong2>() + 3 * (size_t)common;
+    return a;
 }
 
 // rq::Scaler::scale (math/src/rq/scaler.rs:52-126) on `polys` polynomials.
@@ -129,7 +159,8 @@ void PolyScaler::scale(const u64* in, u64* out, u32 polys, bool ntt_repr, hipStr
         intt_rows(*from, a, from->L, 1, polys, s);
         src = tmp;
     }
-    launch_rns_scale(src, isz, 0, out + (size_t)common * n, osz, 0, d_consts.as<u64>(), from->L, (u32)n, 1, polys, s);
+    launch_rns_scale(src, isz, 0, out + (size_t)common * n, osz, 0, d_consts.as<u64>(), words(), from->L, (u32)n, 1,
+                     polys, s);
     if (ntt_repr) {
         RowMap m{out, to->L - common, to->L, common, common};
         launch_ntt_forward(*to, m, polys * (to->L - common), false, s);
@@ -242,11 +273,23 @@ KeySwitchArgs Ksk::args(const u64* src, size_t src_inst_stride) const {
 size_t Ksk::lift_bytes(u32 inst) const { return (size_t)inst * digits * key_ctx->L * key_ctx->N * 8; }
 
 // [inst][2][L'][N] power-basis key-switch rows (+ [inst][2][L][N] at degree
-// 32768) for a deep key; [inst][2][N] for the dropped limb otherwise.
+// 32768) for a deep key; [inst][2][N] for the dropped limb otherwise, +
+// [inst][2][L][N] for the paired finish.
 size_t Ksk::finish_bytes(u32 inst) const {
     const size_t n = ct_ctx->N, L = ct_ctx->L, Lk = key_ctx->L;
-    const size_t unfused = fused(*ct_ctx) ? 0 : L;
-    return (size_t)inst * 2 * (extra > 1 ? Lk + unfused : extra) * n * 8;
+    if (extra > 1) return (size_t)inst * 2 * (Lk + (fused(*ct_ctx) ? 0 : L)) * n * 8;
+    return (size_t)inst * 2 * (extra + (paired_finish(inst) ? L : 0)) * n * 8;
+}
+
+// Small launches of rows transformed as halves (extra <= 1) finish in two
+// steps: one elementwise pass over column pairs forms z and the first
+// forward stage of every row, and the half transforms only load it. Larger
+// launches form z on load of each half (twice per coefficient, but no
+// extra pass over the rows).
+bool Ksk::paired_finish(u32 inst) const {
+    const size_t L = ct_ctx->L, rows = (size_t)2 * L * inst;
+    return extra <= 1 && rows <= split_row_limit() && split_rows(*ct_ctx, rows) &&
+           (!extra || split_rows(*key_ctx, (size_t)2 * inst));
 }
 
 u64* Ksk::lift(const u64* src, size_t src_inst_stride, u32 inst, hipStream_t s) const {
@@ -333,11 +376,19 @@ void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* out, u3
     // Inner products of every key limb: the limbs below the dropped one into
     // `out` (NTT domain), the dropped limb into `special`, which alone goes
     // back to power basis; then the transformed power-basis part is added in.
+    const bool paired = paired_finish(inst);
     u64* base = scratch(Scratch::SwitchDown).get(s, finish_bytes(inst)).as<u64>();
     u64* special = extra ? base : nullptr;
     KsFinishArgs f{a, y, fin0.as<u64>(), fin1.as<u64>(), digits, (u32)Lk, special,
                    key_ctx->mod.as<ModInfo>() + (Lk - 1)};
     launch_ks_mac(ct_ctx->logn, f, inst, s);
+    if (paired) {
+        u64* pre = base + (size_t)inst * 2 * extra * n;
+        if (extra) launch_ntt_backward_halves(*key_ctx, RowMap{special, 1, 1, 0, (u32)(Lk - 1)}, inst * 2, s);
+        launch_ks_z_pair(f, *ct_ctx, *key_ctx, pre, inst, s);
+        launch_ks_finish_half(f, *ct_ctx, inst, s, pre);
+        return;
+    }
     if (extra) launch_ntt_backward(*key_ctx, RowMap{special, 1, 1, 0, (u32)(Lk - 1)}, inst * 2, s);
     if (split_rows(*ct_ctx, (size_t)2 * L * inst))
         launch_ks_finish_half(f, *ct_ctx, inst, s);
@@ -392,6 +443,7 @@ Multiplicator::Multiplicator(std::shared_ptr<DeviceContext> b, const std::vector
     mul = make_context(extended_basis, base->N, nullptr);
     ext_lhs.reset(new PolyScaler(base, mul, lhs_num, lhs_den));
     ext_rhs.reset(new PolyScaler(base, mul, rhs_num, rhs_den));
+    ext_same = lhs_num == rhs_num && lhs_den == rhs_den;
     down.reset(new PolyScaler(mul, base, post_num, post_den));
 }
 
@@ -513,18 +565,34 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out_final, u32
     ia.dst_part_stride = L * n;
     ia.inv_tw = base->inv_tw.as<ulonglong2>();
     ia.mods = base->mod.as<ModInfo>();
-    intt_rows(*base, ia, (u32)L, 4, inst, s);
+    // Rows transformed as halves leave the stage pairing the halves to the
+    // scalers, which also form the first stage of the forward transform of
+    // the new limbs when those are transformed as halves.
+    const bool in_open = intt_rows(*base, ia, (u32)L, 4, inst, s, false);
+    const bool out_fwd = cl == cr && cl < M && split_rows(*mul, (size_t)inst * 4 * (M - cl));
     // 2. base extension of the power-basis columns (rns/scaler.rs:249-352)
-    if (cl < M)
-        launch_rns_scale(wP, 4 * L * n, L * n, wX, 4 * xr * n, xr * n, ext_lhs->d_consts.as<u64>(), (u32)L, (u32)n, 2,
-                         inst, s, "rns_extend");
-    if (cr < M)
-        launch_rns_scale(wP + 2 * L * n, 4 * L * n, L * n, wX + 2 * xr * n, 4 * xr * n, xr * n,
-                         ext_rhs->d_consts.as<u64>(), (u32)L, (u32)n, 2, inst, s, "rns_extend");
+    auto extend = [&](const PolyScaler& sc, u32 k0, u32 parts) {
+        if (sc.common >= M) return;
+        const u64* in = wP + k0 * L * n;
+        u64* o = wX + k0 * xr * n;
+        if (in_open || out_fwd)
+            launch_scale_pair(sc.pair(in, 4 * L * n, L * n, o, 4 * xr * n, xr * n, in_open, out_fwd), (u32)L, parts,
+                              inst, s, "rns_extend");
+        else
+            launch_rns_scale(in, 4 * L * n, L * n, o, 4 * xr * n, xr * n, sc.d_consts.as<u64>(), sc.words(), (u32)L,
+                             (u32)n, parts, inst, s, "rns_extend");
+    };
+    if (ext_same) {
+        extend(*ext_lhs, 0, 4);
+    } else {
+        extend(*ext_lhs, 0, 2);
+        extend(*ext_rhs, 2, 2);
+    }
     // 3. forward NTT of the new limbs (rq/scaler.rs:96-114)
     if (cl == cr) {
         RowMap m{wX, (u32)(M - cl), (u32)xr, 0, cl};
-        launch_ntt_forward(*mul, m, inst * 4 * (u32)(M - cl), false, s);
+        if (out_fwd) launch_ntt_forward_halves(*mul, m, inst * 4 * (u32)(M - cl), false, s);
+        else launch_ntt_forward(*mul, m, inst * 4 * (u32)(M - cl), false, s);
     } else {
         for (int k = 0; k < 4; k++) {
             const u32 c = k < 2 ? cl : cr;
@@ -548,14 +616,20 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out_final, u32
     ta.out_part_stride = M * n;
     ta.inv_tw = mul->inv_tw.as<ulonglong2>();
     ta.mods = mul->mod.as<ModInfo>();
-    tensor_intt(*mul, ta, (u32)M, inst, s);
+    const bool t_open = tensor_intt(*mul, ta, (u32)M, inst, s, false);
     // 5. t/Q down-scaling of the three polynomials (mul.rs:187-190)
     u64* dst = rk ? wD : out;
-    launch_rns_scale(wT, 3 * M * n, M * n, dst, 3 * L * n, L * n, down->d_consts.as<u64>(), (u32)M, (u32)n, 3, inst,
-                     s, "rns_down");
+    const bool d_fwd = !rk && split_rows(*base, (size_t)inst * 3 * L);
+    if (t_open || d_fwd)
+        launch_scale_pair(down->pair(wT, 3 * M * n, M * n, dst, 3 * L * n, L * n, t_open, d_fwd), (u32)M, 3, inst, s,
+                          "rns_down");
+    else
+        launch_rns_scale(wT, 3 * M * n, M * n, dst, 3 * L * n, L * n, down->d_consts.as<u64>(), down->words(),
+                         (u32)M, (u32)n, 3, inst, s, "rns_down");
     if (!rk) {
         RowMap m{out, (u32)L, (u32)L, 0, 0};
-        launch_ntt_forward(*base, m, inst * 3 * (u32)L, false, s);
+        if (d_fwd) launch_ntt_forward_halves(*base, m, inst * 3 * (u32)L, false, s);
+        else launch_ntt_forward(*base, m, inst * 3 * (u32)L, false, s);
         if (mod_switch) mod_switch->switch_down(out, out_final, inst * 3, s);
         return;
     }

@@ -86,6 +86,12 @@ struct PolyScaler {
     u32 common = 0;  // number_common_moduli
     ScalerConsts consts;
     DevBuf d_consts;  // outputs start .. to->L
+    u32 words() const { return (u32)(d_consts.bytes / sizeof(u64)); }
+    // A paired launch (launch_scale_pair) of these constants; in_open: the
+    // inputs are half-row inverse transforms, out_fwd: the outputs get the
+    // first forward stage of the destination context.
+    ScalePairArgs pair(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst, size_t out_part,
+                       bool in_open, bool out_fwd) const;
     PolyScaler(std::shared_ptr<DeviceContext> f, std::shared_ptr<DeviceContext> t, const BigUint& num,
                const BigUint& den);
     void scale(const u64* in, u64* out, u32 polys, bool ntt_repr, hipStream_t s);
@@ -114,6 +120,9 @@ struct Ksk {
     // switch_and_finish over `inst` instances.
     size_t lift_bytes(u32 inst) const;
     size_t finish_bytes(u32 inst) const;
+    // Whether a finish over `inst` instances takes the paired form
+    // (launch_ks_z_pair, then the half transforms).
+    bool paired_finish(u32 inst) const;
     // Digit lift + lazy forward NTT into scratch [inst][digits][L'][N].
     u64* lift(const u64* src, size_t src_inst_stride, u32 inst, hipStream_t s) const;
     // Full key switch into acc [inst][2][L'][N] (NTT domain, or power basis
@@ -153,6 +162,7 @@ struct Switcher {
 struct Multiplicator {
     std::shared_ptr<DeviceContext> base, mul;
     std::unique_ptr<PolyScaler> ext_lhs, ext_rhs, down;
+    bool ext_same = false;  // both operands are extended with the same factor (one launch for both)
     std::shared_ptr<Ksk> rk;
     // Multiplicator::enable_mod_switching (mul.rs:146-155): the product is
     // switched down to the next context.

@@ -16,37 +16,57 @@ namespace fhehip {
 // loop, math/src/rq/scaler.rs:84-93). Power-basis rows of part k of instance b
 // start at in + b * in_inst + k * in_part; outputs likewise.
 // grid = (N / 256, parts, instances)
+//
+// LDS: the workgroup first copies the constant block (sc_words words) into
+// LDS. A column reads about 2 + count constants per source modulus; through
+// the scalar cache these arrive in dependent rounds of a few loads each,
+// which a launch of a few waves per SIMD cannot hide (single calls). Large
+// launches keep the scalar-cache form, whose constants cost no VGPRs.
 // ---------------------------------------------------------------------------
-template <int NF>
+template <int NF, bool LDS>
 __global__ __launch_bounds__(256) void rns_scale_kernel(const u64* __restrict__ in, size_t in_inst, size_t in_part,
                                                         u64* __restrict__ out, size_t out_inst, size_t out_part,
-                                                        const u64* __restrict__ sc, u32 n) {
+                                                        const u64* __restrict__ sc, u32 n, u32 sc_words) {
+    extern __shared__ u64 sc_lds[];
     const u32 col = blockIdx.x * 256 + threadIdx.x;
-    if (col >= n) return;
+    const bool live = col < n;
     const u64* src = in + blockIdx.z * in_inst + blockIdx.y * in_part + col;
     u64* dst = out + blockIdx.z * out_inst + blockIdx.y * out_part + col;
+    // the column's residues are in flight while the constants are staged
     u64 r[NF];
+    if (live) {
 #pragma unroll
-    for (int j = 0; j < NF; j++) r[j] = src[(size_t)j * n];
-    rns_scale_column<NF>(
-        sc, [&](int j) { return r[j]; }, [&](int i, u64 v) { dst[(size_t)i * n] = v; });
+        for (int j = 0; j < NF; j++) r[j] = src[(size_t)j * n];
+    }
+    if constexpr (LDS) {
+        for (u32 i = threadIdx.x; i < sc_words; i += 256) sc_lds[i] = sc[i];
+        __syncthreads();
+    }
+    if (!live) return;
+    rns_scale_column<NF, !LDS>(
+        LDS ? sc_lds : sc, [&](int j) { return r[j]; }, [&](int i, u64 v) { dst[(size_t)i * n] = v; });
 }
 
 template <int NF>
-static void rns_scale_launch(dim3 grid, hipStream_t s, const u64* in, size_t in_inst, size_t in_part, u64* out,
-                             size_t out_inst, size_t out_part, const u64* d_sc, u32 n) {
-    hipLaunchKernelGGL(rns_scale_kernel<NF>, grid, dim3(256), 0, s, in, in_inst, in_part, out, out_inst, out_part,
-                       d_sc, n);
+static void rns_scale_launch(dim3 grid, bool lds, hipStream_t s, const u64* in, size_t in_inst, size_t in_part,
+                             u64* out, size_t out_inst, size_t out_part, const u64* d_sc, u32 n, u32 sc_words) {
+    if (lds)
+        hipLaunchKernelGGL((rns_scale_kernel<NF, true>), grid, dim3(256), (size_t)sc_words * 8, s, in, in_inst, in_part,
+                           out, out_inst, out_part, d_sc, n, sc_words);
+    else
+        hipLaunchKernelGGL((rns_scale_kernel<NF, false>), grid, dim3(256), 0, s, in, in_inst, in_part, out, out_inst,
+                           out_part, d_sc, n, sc_words);
 }
 
 void launch_rns_scale(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst, size_t out_part,
-                      const u64* d_sc, u32 nf, u32 n, u32 parts, u32 inst, hipStream_t s,
+                      const u64* d_sc, u32 sc_words, u32 nf, u32 n, u32 parts, u32 inst, hipStream_t s,
                       const char* name) {
     if (!parts || !inst) return;
     ProfScope ps(name, 31 - __builtin_clz(n), s);
     dim3 grid((n + 255) / 256, parts, inst);
+    const bool lds = (size_t)grid.x * parts * inst <= kScaleLdsMaxBlocks && sc_words * 8 <= 64 * 1024;
 #define FHEHIP_SC_CASE(K) \
-    case K: rns_scale_launch<K>(grid, s, in, in_inst, in_part, out, out_inst, out_part, d_sc, n); break;
+    case K: rns_scale_launch<K>(grid, lds, s, in, in_inst, in_part, out, out_inst, out_part, d_sc, n, sc_words); break;
     switch (nf) {
         FHEHIP_SC_CASE(1) FHEHIP_SC_CASE(2) FHEHIP_SC_CASE(3) FHEHIP_SC_CASE(4) FHEHIP_SC_CASE(5)
         FHEHIP_SC_CASE(6) FHEHIP_SC_CASE(7) FHEHIP_SC_CASE(8) FHEHIP_SC_CASE(9) FHEHIP_SC_CASE(10)
@@ -520,10 +540,14 @@ __global__ __launch_bounds__((NttShape<LOGN, R>::T)) void ks_finish_kernel(KsFin
 // the stage-0 butterfly on load (native.rs:256-269), runs the remaining 14
 // stages with the half tables and adds the reduced result into `out`. 1-D
 // grid over (instance, part, limb, half), half fastest.
+//
+// pre != null: the rows after their first stage come ready from
+// ks_z_pair_kernel as pre [inst][2][L][N], and are only loaded.
 template <int LOGH>
 __global__ __launch_bounds__(NttShape<LOGH>::T) void ks_finish_half_kernel(KsFinishArgs f,
                                                                         const ulonglong2* __restrict__ half_tw,
-                                                                        const ulonglong2* __restrict__ split) {
+                                                                        const ulonglong2* __restrict__ split,
+                                                                        const u64* __restrict__ pre) {
     using S = NttShape<LOGH>;
     __shared__ u64 lds[S::LDS_WORDS];
     const SdNttArgs& a = f.sd;
@@ -553,10 +577,16 @@ __global__ __launch_bounds__(NttShape<LOGH>::T) void ks_finish_half_kernel(KsFin
         return ad ? add_mod(z, ad[k], p) : z;
     };
     u64 x[S::E];
+    if (pre) {
+        const u64* xr = pre + ((((size_t)b * 2 + part) * a.L + i) << (LOGH + 1)) + ((size_t)h << LOGH);
 #pragma unroll
-    for (int e = 0; e < S::E; e++) {
-        const int k = fwd_index<LOGH, 0>(t, e);
-        x[e] = split_half(z_at(k), z_at(k + (1 << LOGH)), h, w.x, w.y, p, p2);
+        for (int e = 0; e < S::E; e++) x[e] = xr[fwd_index<LOGH, 0>(t, e)];
+    } else {
+#pragma unroll
+        for (int e = 0; e < S::E; e++) {
+            const int k = fwd_index<LOGH, 0>(t, e);
+            x[e] = split_half(z_at(k), z_at(k + (1 << LOGH)), h, w.x, w.y, p, p2);
+        }
     }
     ntt_fwd_regs<LOGH>(lds, x, t, half_tw + ((size_t)(2 * i + h) << LOGH), p);
     u64* dst = a.out + (size_t)b * a.out_inst_stride + (size_t)part * a.out_part_stride + ((size_t)i << (LOGH + 1)) +
@@ -749,16 +779,75 @@ void launch_ks_finish(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s) 
 }
 
 template <int LOGH>
-static void ks_finish_half_go(const KsFinishArgs& f, const DeviceContext& ct, dim3 grid, hipStream_t s) {
+static void ks_finish_half_go(const KsFinishArgs& f, const DeviceContext& ct, const u64* pre, dim3 grid,
+                              hipStream_t s) {
     hipLaunchKernelGGL(ks_finish_half_kernel<LOGH>, grid, dim3(NttShape<LOGH>::T), 0, s, f,
-                       ct.half->fwd_tw.as<ulonglong2>(), ct.split.as<ulonglong2>());
+                       ct.half->fwd_tw.as<ulonglong2>(), ct.split.as<ulonglong2>(), pre);
 }
 
-void launch_ks_finish_half(const KsFinishArgs& f, const DeviceContext& ct, u32 inst, hipStream_t s) {
+void launch_ks_finish_half(const KsFinishArgs& f, const DeviceContext& ct, u32 inst, hipStream_t s, const u64* pre) {
     if (!f.sd.L || !inst) return;
     if (!ct.half) throw Error(-2, "this context has no half-row tables");
     ProfScope ps("ks_finish", ct.logn, s);
-    FHEHIP_LOGH_DISPATCH(ct.logn - 1, ks_finish_half_go, f, ct, dim3(inst * 2 * f.sd.L * 2), s);
+    FHEHIP_LOGH_DISPATCH(ct.logn - 1, ks_finish_half_go, f, ct, pre, dim3(inst * 2 * f.sd.L * 2), s);
+    HIPCHK(hipGetLastError());
+}
+
+// The first half of the key-switch finish of rows transformed as halves, as
+// one elementwise pass over column pairs (c, c + N/2): the last inverse
+// stage of the dropped limb's two half transforms (as ntt_split_inv_kernel,
+// with the key context's constants of that limb), then for every limb i of
+// the ciphertext context z = add_pb - q_l^-1 [y]_i at both columns (as
+// ks_finish_kernel) and the first forward stage of the pair (Harvey
+// butterfly, native.rs:256-269, outputs < 4p) into pre [inst][2][L][N].
+// grid = (N / 512, 2, instances)
+__global__ __launch_bounds__(256) void ks_z_pair_kernel(KsFinishArgs f, const ulonglong2* __restrict__ ct_split,
+                                                        const ulonglong2* __restrict__ drop_split,
+                                                        u64* __restrict__ pre, u32 logn) {
+    const SdNttArgs& a = f.sd;
+    const u32 half = 1u << (logn - 1);
+    const u32 c = blockIdx.x * 256 + threadIdx.x;
+    const u32 part = blockIdx.y, b = blockIdx.z;
+    const u64 ql = a.extra ? a.extra_mods[0] : 0, hq = ql >> 1;
+    u64 y0 = 0, y1 = 0;  // (y + floor(q_l / 2)) mod q_l at c and c + N/2
+    if (a.extra) {
+        const u64* lrow = f.special + (((size_t)b * 2 + part) << logn);
+        const u64 s0 = lrow[c], s1 = lrow[c + half];
+        const ulonglong2 z = drop_split[1], ni = drop_split[2];
+        y0 = mul_shoup_u(s0 + s1, ni.x, ni.y, ql) + hq;
+        y1 = mul_shoup_u(s0 + 2 * ql - s1, z.x, z.y, ql) + hq;
+        y0 = y0 >= ql ? y0 - ql : y0;
+        y1 = y1 >= ql ? y1 - ql : y1;
+    }
+    u64* xb = pre + (((size_t)b * 2 + part) * a.L << logn);
+    const u64* ad = a.add_pb ? a.add_pb + (size_t)b * a.add_inst_stride + (size_t)part * a.add_part_stride : nullptr;
+    for (u32 i = 0; i < a.L; i++) {
+        const ModInfo mi = a.mods[i];
+        const u64 p = mi.p;
+        const SdConst cst = a.sd[i];
+        auto z_of = [&](u64 yv) {
+            const u64 yr = ql <= 2 * p ? yv : lazy_reduce_u64(yv, mi.barrett_lo, mi.barrett_hi, p);
+            return mul_shoup_u(3 * p - (yr + cst.half_mod), cst.inv, cst.inv_shoup, p);
+        };
+        u64 z0 = a.extra ? z_of(y0) : 0, z1 = a.extra ? z_of(y1) : 0;
+        if (ad) {
+            z0 = add_mod(z0, ad[((size_t)i << logn) + c], p);
+            z1 = add_mod(z1, ad[((size_t)i << logn) + c + half], p);
+        }
+        const ulonglong2 w = ct_split[3 * i];
+        const u64 wc = lazy_mul_shoup_u(z1, w.x, w.y, p);
+        xb[((size_t)i << logn) + c] = z0 + wc;
+        xb[((size_t)i << logn) + c + half] = z0 + 2 * p - wc;
+    }
+}
+
+void launch_ks_z_pair(const KsFinishArgs& f, const DeviceContext& ct, const DeviceContext& key, u64* pre, u32 inst,
+                      hipStream_t s) {
+    if (!f.sd.L || !inst) return;
+    if (ct.N < 512 || key.N != ct.N) throw Error(-2, "paired key-switch finish needs one degree of at least 512");
+    ProfScope ps("ks_z_pair", ct.logn, s);
+    hipLaunchKernelGGL(ks_z_pair_kernel, dim3(ct.N / 512, 2, inst), dim3(256), 0, s, f, ct.split.as<ulonglong2>(),
+                       key.split.as<ulonglong2>() + 3 * (size_t)(key.L - 1), pre, (u32)ct.logn);
     HIPCHK(hipGetLastError());
 }
 

@@ -84,9 +84,32 @@ struct KsFinishArgs {
     u32 xcd;                    // XCD-aware block order (set by the launcher)
 };
 
+// Scaling launches of at most this many workgroups (4 per CU) copy their
+// constants to LDS first; larger ones read them through the scalar cache.
+constexpr size_t kScaleLdsMaxBlocks = 1024;
+// d_sc: the constant block of sc_words words (rns_device.h).
 void launch_rns_scale(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst, size_t out_part,
-                      const u64* d_sc, u32 nf, u32 n, u32 parts, u32 inst, hipStream_t s,
+                      const u64* d_sc, u32 sc_words, u32 nf, u32 n, u32 parts, u32 inst, hipStream_t s,
                       const char* name = "rns_scale");
+// Scaling of rows transformed as two halves (scale_pair.hip): rows of part k
+// of instance b at in + b in_inst + k in_part (likewise out), degree n >= 512.
+struct ScalePairArgs {
+    const u64* in;
+    size_t in_inst, in_part;
+    u64* out;
+    size_t out_inst, out_part;
+    const u64* sc;  // constant block (rns_device.h) of sc_words words, `count` outputs
+    u32 sc_words, count, n;
+    // Optional: the inputs are the two half inverse transforms of each row;
+    // the split table of the source context (3 entries per limb) and its
+    // moduli give the last inverse stage.
+    const ulonglong2* in_split;
+    const ModInfo* in_mods;
+    // Optional: the split table entries of the output limbs (3 per limb,
+    // from the first output limb); outputs get the first forward stage.
+    const ulonglong2* out_split;
+};
+void launch_scale_pair(const ScalePairArgs& a, u32 nf, u32 parts, u32 inst, hipStream_t s, const char* name);
 void launch_intt_rows(int logn, const InttRowsArgs& a, u32 limbs, u32 parts, u32 inst, hipStream_t s);
 void launch_tensor_intt(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_t s);
 // Degree 32768 (`c` the context): the same with the two half-row inverse
@@ -103,7 +126,15 @@ void launch_sd_add_ntt(int logn, const SdNttArgs& a, u32 L, u32 parts, u32 inst,
 void launch_ks_finish(int logn, const KsFinishArgs& f, u32 inst, hipStream_t s);
 // The same at degree 32768 (`ct` the ciphertext context): z on load, the
 // half-row transforms, out += result.
-void launch_ks_finish_half(const KsFinishArgs& f, const DeviceContext& ct, u32 inst, hipStream_t s);
+// pre != null: the rows after their first stage, from launch_ks_z_pair.
+void launch_ks_finish_half(const KsFinishArgs& f, const DeviceContext& ct, u32 inst, hipStream_t s,
+                           const u64* pre = nullptr);
+// The elementwise half of that finish over column pairs: the last inverse
+// stage of the dropped limb (`special` after launch_ntt_backward_halves in
+// the key context), z of every limb and the first forward stage, into pre
+// [inst][2][L][N].
+void launch_ks_z_pair(const KsFinishArgs& f, const DeviceContext& ct, const DeviceContext& key, u64* pre, u32 inst,
+                      hipStream_t s);
 void launch_add_rows(const u64* a, const u64* b, u64* out, const ModInfo* mods, u32 L, int logn, size_t rows,
                      hipStream_t s);
 

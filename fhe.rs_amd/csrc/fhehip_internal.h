@@ -96,8 +96,14 @@ void set_ntt_shape(int logn, int re_fwd, int re_inv);
 // log_base != 0: power-of-two decomposition digits of limb 0 (one-modulus keys).
 void launch_lift_ntt_split(const DeviceContext& key, const u64* src, size_t src_inst, u64* y, u32 digits, u32 inst,
                            u32 log_base, hipStream_t s);
-// Degree 32768: the last inverse stage of `rows` rows whose two halves went
-// through the half-row inverse transforms (N^-1 folded in).
+// The last inverse stage of `rows` rows whose two halves went through the
+// half-row inverse transforms (N^-1 folded in).
 void launch_ntt_split_inv(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s);
+// The forward transform of rows whose first stage (pairing the halves) the
+// producer already formed: the two half transforms of every row only.
+void launch_ntt_forward_halves(const DeviceContext& c, RowMap m, u32 rows, bool lazy, hipStream_t s);
+// The inverse transform of rows as two half transforms each, without the
+// stage pairing the halves (left to the consumer, as launch_ntt_split_inv).
+void launch_ntt_backward_halves(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s);
 
 }  // namespace fhehip

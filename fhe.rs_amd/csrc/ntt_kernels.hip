@@ -252,6 +252,22 @@ void launch_ntt_forward(const DeviceContext& c, RowMap m, u32 rows, bool lazy, h
     HIPCHK(hipGetLastError());
 }
 
+void launch_ntt_forward_halves(const DeviceContext& c, RowMap m, u32 rows, bool lazy, hipStream_t s) {
+    if (rows == 0) return;
+    if (!c.half) throw Error(-2, "this context has no half-row tables");
+    ProfScope ps(lazy ? "ntt_forward_lazy" : "ntt_forward", c.logn, s);
+    FHEHIP_LOGN_SWITCH(c.logn - 1, fwd_launch, *c.half, half_rows(m), 2 * rows, lazy, s);
+    HIPCHK(hipGetLastError());
+}
+
+void launch_ntt_backward_halves(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s) {
+    if (rows == 0) return;
+    if (!c.half) throw Error(-2, "this context has no half-row tables");
+    ProfScope ps("ntt_backward", c.logn, s);
+    FHEHIP_LOGN_SWITCH(c.logn - 1, inv_launch, *c.half, half_rows(m), 2 * rows, s);
+    HIPCHK(hipGetLastError());
+}
+
 void launch_ntt_split_inv(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s) {
     if (rows == 0) return;
     if (!c.half) throw Error(-2, "this context has no half-row tables");

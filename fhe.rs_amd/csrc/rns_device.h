@@ -206,29 +206,40 @@ __device__ __forceinline__ void u256_sub(U256& a, const U256& b) {
     a.w[3] = __builtin_subcll(a.w[3], b.w[3], c2, &c3);
 }
 
-// Low 128 bits of (a >> s), 0 < s < 256.
+// Low 128 bits of (a >> s), 0 < s < 256. The words are picked by selects
+// (no indexing of a private array by the shift, which would go to scratch).
 __device__ __forceinline__ u128 u256_shr_lo128(const U256& a, unsigned s) {
-    unsigned ws = s >> 6, bs = s & 63;
-    u64 r[2];
-#pragma unroll
-    for (int i = 0; i < 2; i++) {
-        unsigned k = i + ws;
-        u64 lo = k < 4 ? a.w[k] : 0;
-        u64 hi = (k + 1) < 4 ? a.w[k + 1] : 0;
-        r[i] = bs ? ((lo >> bs) | (hi << (64 - bs))) : lo;
-    }
-    return ((u128)r[1] << 64) | r[0];
+    const unsigned ws = s >> 6, bs = s & 63;
+    auto word = [&](unsigned k) -> u64 {
+        return k == 0 ? a.w[0] : k == 1 ? a.w[1] : k == 2 ? a.w[2] : k == 3 ? a.w[3] : 0;
+    };
+    const u64 x0 = word(ws), x1 = word(ws + 1), x2 = word(ws + 2);
+    const u64 r0 = bs ? ((x0 >> bs) | (x1 << (64 - bs))) : x0;
+    const u64 r1 = bs ? ((x1 >> bs) | (x2 << (64 - bs))) : x1;
+    return ((u128)r1 << 64) | r0;
 }
 
-// One column of RnsScaler::scale with NF input residues. `rests(j)` yields
-// residue j of the column; `emit(i, value)` receives output i (0-based, output
-// modulus start + i).
-template <int NF, class Rests, class Emit>
-__device__ __forceinline__ void rns_scale_column(const u64* __restrict__ sc, Rests rests, Emit emit) {
+// A small header field of the constant block as a wave-uniform value: read
+// through the scalar cache (SW), or from a copy in LDS and moved to an SGPR.
+template <bool SW>
+__device__ __forceinline__ u32 sc_uniform(u64 v) {
+    if constexpr (SW) return (u32)v;
+    else return (u32)__builtin_amdgcn_readfirstlane((int)(u32)v);
+}
+
+// The per-column values of RnsScaler::scale that every output uses: v and
+// the signed correction w (zero when the factor is one), as 64-bit halves.
+struct ScColumn {
+    u64 v0, v1, w0, w1;
+    bool w_sign;
+};
+
+// v and w of one column with NF input residues; `rests(j)` yields residue j.
+// SW: `sc` is global memory and the constants enter the products from SGPRs;
+// otherwise `sc` is a copy in LDS and they enter from VGPRs.
+template <int NF, bool SW, class Rests>
+__device__ __forceinline__ ScColumn sc_column(const u64* __restrict__ sc, Rests rests, unsigned shift, bool is_one) {
     constexpr int nf = NF;
-    const int count = (int)sc[SC_COUNT];
-    const unsigned shift = (unsigned)sc[SC_SHIFT];
-    const bool is_one = sc[SC_IS_ONE] != 0;
     const u64* tg = sc + SC_HEADER;
     const u64* tom = tg + 2 * nf;
 
@@ -238,12 +249,12 @@ __device__ __forceinline__ void rns_scale_column(const u64* __restrict__ sc, Res
     U256 sum;
     {
         Acc3 sa, sb;
-        acc3_first<true>(sa, rests(0), tg[0]);
-        acc3_first<true>(sb, rests(0), tg[1]);
+        acc3_first<SW>(sa, rests(0), tg[0]);
+        acc3_first<SW>(sb, rests(0), tg[1]);
 #pragma unroll
         for (int j = 1; j < nf; j++) {
-            acc3_mul<true>(sa, rests(j), tg[2 * j]);
-            acc3_mul<true>(sb, rests(j), tg[2 * j + 1]);
+            acc3_mul<SW>(sa, rests(j), tg[2 * j]);
+            acc3_mul<SW>(sb, rests(j), tg[2 * j + 1]);
         }
         u64 a0, a1, a2, b0, b1, b2;
         acc3_value(sa, a0, a1, a2);
@@ -271,11 +282,11 @@ __device__ __forceinline__ void rns_scale_column(const u64* __restrict__ sc, Res
             const u64 clo = neg ? 0 - lo : lo;
             const u64 chi = neg ? ~hi + (lo == 0 ? 1 : 0) : hi;
             if (j == 0) {
-                acc3_first<true>(wa, rests(j), clo);
-                acc3_first<true>(wb, rests(j), chi);
+                acc3_first<SW>(wa, rests(j), clo);
+                acc3_first<SW>(wb, rests(j), chi);
             } else {
-                acc3_mul<true>(wa, rests(j), clo);
-                acc3_mul<true>(wb, rests(j), chi);
+                acc3_mul<SW>(wa, rests(j), clo);
+                acc3_mul<SW>(wb, rests(j), chi);
             }
             add128(rn0, rn1, rests(j) & (neg ? ~0ull : 0ull), 0);
         }
@@ -304,7 +315,7 @@ __device__ __forceinline__ void rns_scale_column(const u64* __restrict__ sc, Res
             vt.w[2] += l;
             vt.w[3] += h + (vt.w[2] < l ? 1 : 0);
         }
-        if (sc[SC_TG_SIGN]) u256_add(so, vt);
+        if (sc_uniform<SW>(sc[SC_TG_SIGN])) u256_add(so, vt);
         else u256_sub(so, vt);
         w_sign = (so.w[3] >> 63) != 0;
         if (w_sign) {
@@ -318,47 +329,69 @@ __device__ __forceinline__ void rns_scale_column(const u64* __restrict__ sc, Res
             w = (ww >> 1) + (ww & 1);
         }
     }
+    return ScColumn{(u64)v, (u64)(v >> 64), (u64)w, (u64)(w >> 64), w_sign};
+}
 
-    // Each output residue is the exact value of
-    //   2 q - [v]_q gamma (+/- [w]_q) + sum_j r_j omega_ij   mod q,
-    // which the reference reduces term by term. Here every term is an exact
-    // product added into a 192-bit accumulator (v and w enter through their
-    // 64-bit halves: v = v1 2^64 + v0 contributes v0 (q - gamma) +
-    // v1 [2^64 (q - gamma)]_q), and the sum is reduced once.
-    const u64* outc = sc + sc_out_base(nf);
-    const int ostride = sc_out_stride(nf);
-    const u64 v0 = (u64)v, v1 = (u64)(v >> 64);
-    const u64 w0 = (u64)w, w1 = (u64)(w >> 64);
-    for (int i = 0; i < count; i++) {
-        const u64* o = outc + (size_t)i * ostride;
-        const u64 q = o[0], bl = o[1], bh = o[2];
-        // o[3] = q - gamma, o[4] = [2^64 (q - gamma)]_q, o[5] = [2^64]_q, o[6] = [2^128]_q
-        Acc3 acc;
-        acc3_first<true>(acc, v0, o[3]);
-        acc3_mul<true>(acc, v1, o[4]);
-        if (!is_one) {
-            // +w, or -w = w (q - 1) mod q, without a branch on the column's
-            // sign: w0 c0 + w1 c1 with (c0, c1) = (1, [2^64]_q) or
-            // (q - 1, q - [2^64]_q).
-            const u64 c0 = w_sign ? q - 1 : 1;
-            const u64 c1 = w_sign ? q - o[5] : o[5];
-            acc3_mul<false>(acc, w0, c0);
-            acc3_mul<false>(acc, w1, c1);
-        }
-        const u64* om = o + 7;
-#pragma unroll
-        for (int j = 0; j < nf; j++) acc3_mul<true>(acc, rests(j), om[j]);
-        // lo + 2^64 hi + 2^128 top  ->  lo + hi [2^64]_q + top [2^128]_q  (< 2^127) -> Barrett
-        u64 lo, hi, top;
-        acc3_value(acc, lo, hi, top);
-        Acc3 fold;
-        acc3_init(fold, lo);
-        acc3_mul<true>(fold, hi, o[5]);
-        acc3_mul<true>(fold, top, o[6]);
-        u64 l2, h2, t2;
-        acc3_value(fold, l2, h2, t2);
-        emit(i, sc_reduce_u128(l2, h2, bl, bh, q));
+// Each output residue is the exact value of
+//   2 q - [v]_q gamma (+/- [w]_q) + sum_j r_j omega_ij   mod q,
+// which the reference reduces term by term. Here every term is an exact
+// product added into a 192-bit accumulator (v and w enter through their
+// 64-bit halves: v = v1 2^64 + v0 contributes v0 (q - gamma) +
+// v1 [2^64 (q - gamma)]_q), and the sum is reduced once.
+template <int NF, bool SW, class Rests>
+__device__ __forceinline__ u64 sc_output(const u64* __restrict__ o, Rests rests, const ScColumn& c, bool is_one) {
+    constexpr int nf = NF;
+    const u64 q = o[0], bl = o[1], bh = o[2];
+    // o[3] = q - gamma, o[4] = [2^64 (q - gamma)]_q, o[5] = [2^64]_q, o[6] = [2^128]_q
+    Acc3 acc;
+    acc3_first<SW>(acc, c.v0, o[3]);
+    acc3_mul<SW>(acc, c.v1, o[4]);
+    if (!is_one) {
+        // +w, or -w = w (q - 1) mod q, without a branch on the column's
+        // sign: w0 c0 + w1 c1 with (c0, c1) = (1, [2^64]_q) or
+        // (q - 1, q - [2^64]_q).
+        const u64 c0 = c.w_sign ? q - 1 : 1;
+        const u64 c1 = c.w_sign ? q - o[5] : o[5];
+        acc3_mul<false>(acc, c.w0, c0);
+        acc3_mul<false>(acc, c.w1, c1);
     }
+    const u64* om = o + 7;
+#pragma unroll
+    for (int j = 0; j < nf; j++) acc3_mul<SW>(acc, rests(j), om[j]);
+    // lo + 2^64 hi + 2^128 top  ->  lo + hi [2^64]_q + top [2^128]_q  (< 2^127) -> Barrett
+    u64 lo, hi, top;
+    acc3_value(acc, lo, hi, top);
+    Acc3 fold;
+    acc3_init(fold, lo);
+    acc3_mul<SW>(fold, hi, o[5]);
+    acc3_mul<SW>(fold, top, o[6]);
+    u64 l2, h2, t2;
+    acc3_value(fold, l2, h2, t2);
+    return sc_reduce_u128(l2, h2, bl, bh, q);
+}
+
+// Header fields of the constant block as wave-uniform values.
+template <bool SW>
+struct ScHeader {
+    int count;
+    unsigned shift;
+    bool is_one;
+    __device__ __forceinline__ explicit ScHeader(const u64* __restrict__ sc)
+        : count((int)sc_uniform<SW>(sc[SC_COUNT])),
+          shift(sc_uniform<SW>(sc[SC_SHIFT])),
+          is_one(sc_uniform<SW>(sc[SC_IS_ONE]) != 0) {}
+};
+
+// One column of RnsScaler::scale with NF input residues. `rests(j)` yields
+// residue j of the column; `emit(i, value)` receives output i (0-based, output
+// modulus start + i).
+template <int NF, bool SW = true, class Rests, class Emit>
+__device__ __forceinline__ void rns_scale_column(const u64* __restrict__ sc, Rests rests, Emit emit) {
+    const ScHeader<SW> h(sc);
+    const ScColumn c = sc_column<NF, SW>(sc, rests, h.shift, h.is_one);
+    const u64* outc = sc + sc_out_base(NF);
+    for (int i = 0; i < h.count; i++)
+        emit(i, sc_output<NF, SW>(outc + (size_t)i * sc_out_stride(NF), rests, c, h.is_one));
 }
 
 }  // namespace fhehip

@@ -83,3 +83,31 @@ def test_whole_and_half_rows_match_oracle(fh, oracle, restore_limit, n, limit):
     assert np.array_equal(fwd, oc.forward(polys))
     assert np.array_equal(fh.to_host(d), polys)
     assert np.array_equal(fh.to_host(sout), oracle.PolyScaler(oc, oracle.Context(bfv.level_moduli(0), n)).scale(polys, True))
+
+
+@pytest.mark.parametrize("polys", [1, 17])
+def test_scaler_launch_forms(fh, oracle, polys):
+    """Small launches (at most 1024 workgroups) stage the scaler's constants
+    in LDS, larger ones read them through the scalar cache: both equal the
+    oracle (one and 17 polynomials of degree 16384, 6 -> 13 moduli and the
+    t/Q scaling back)."""
+    n = 16384
+    print(f"scaler launch forms polys={polys}", flush=True)
+    qs = oracle.generate_moduli([60] * 6, n)
+    ms = qs + [m for m in oracle.generate_moduli([62] * 8, n) if m not in qs][:7]
+    cq, cm = fh.Context(qs, n), fh.Context(ms, n)
+    oq, om = oracle.Context(qs, n), oracle.Context(ms, n)
+    a = oracle.random_polys(qs, n, polys, seed=polys)
+    up = fh.Scaler(cq, cm)
+    out = fh.empty(polys * len(ms) * n)
+    up.scale(fh.to_device(a), out, polys, False)
+    b = oracle.random_polys(ms, n, polys, seed=polys + 1)
+    Q = 1
+    for q in qs:
+        Q *= int(q)
+    down = fh.Scaler(cm, cq, T, Q)
+    out2 = fh.empty(polys * len(qs) * n)
+    down.scale(fh.to_device(b), out2, polys, False)
+    _sync()
+    assert np.array_equal(fh.to_host(out), oracle.PolyScaler(oq, om).scale(a, False))
+    assert np.array_equal(fh.to_host(out2), oracle.PolyScaler(om, oq, T, Q).scale(b, False))
