@@ -292,13 +292,15 @@ bool Ksk::paired_finish(u32 inst) const {
            (!extra || split_rows(*key_ctx, (size_t)2 * inst));
 }
 
-u64* Ksk::lift(const u64* src, size_t src_inst_stride, u32 inst, hipStream_t s) const {
+u64* Ksk::lift(const u64* src, size_t src_inst_stride, u32 inst, hipStream_t s, const HalfFwdRows* rows) const {
     u64* y = scratch(Scratch::Lift).get(s, lift_bytes(inst)).as<u64>();
     const KeySwitchArgs a = args(src, src_inst_stride);
     if (!split_rows(*key_ctx, (size_t)digits * key_ctx->L * inst)) {
         launch_lift_ntt(key_ctx->logn, a, y, inst, s);
+        if (rows && rows->rows)
+            launch_ntt_forward_halves(*ct_ctx, RowMap{rows->x, rows->L, rows->L, 0, 0}, rows->rows, false, s);
     } else {
-        launch_lift_ntt_split(*key_ctx, src, src_inst_stride, y, digits, inst, log_base, s);
+        launch_lift_ntt_split(*key_ctx, src, src_inst_stride, y, digits, inst, log_base, s, rows);
     }
     return y;
 }
@@ -372,16 +374,40 @@ void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* out, u3
         launch_sd_tail(ct_ctx->logn, a, x, true, inst, s);
         return;
     }
+    const bool paired = paired_finish(inst);
+    if (paired && !extra && a.add_pb && !a.add_ntt) {
+        // No dropped modulus: out = NTT(add_pb) + inner products. The
+        // power-basis addend gets its first forward stage in one pass, the
+        // rest of its transform in the lift's launch, and ks_mac adds it as
+        // an NTT-domain addend (no finish kernels).
+        u64* pre = scratch(Scratch::SwitchDown).get(s, finish_bytes(inst)).as<u64>();
+        KsFinishArgs z{a, nullptr, nullptr, nullptr, digits, (u32)Lk, nullptr, key_ctx->mod.as<ModInfo>()};
+        launch_ks_z_pair(z, *ct_ctx, *key_ctx, pre, inst, s);
+        const HalfFwdRows rows{pre, (u32)(inst * 2 * L), (u32)L, ct_ctx->half->fwd_tw.as<ulonglong2>(),
+                               ct_ctx->mod.as<ModInfo>()};
+        const u64* y = lift(src, src_inst_stride, inst, s, &rows);
+        SdNttArgs m = a;
+        m.add_pb = nullptr;
+        m.add_ntt = pre;
+        m.add_ntt_inst_stride = 2 * L * n;
+        m.add_ntt_part_stride = L * n;
+        m.add_ntt_parts = 2;
+        m.perm = nullptr;
+        launch_ks_mac(ct_ctx->logn, KsFinishArgs{m, y, fin0.as<u64>(), fin1.as<u64>(), digits, (u32)Lk, nullptr,
+                                                 key_ctx->mod.as<ModInfo>() + (Lk - 1)},
+                      inst, s);
+        return;
+    }
     const u64* y = lift(src, src_inst_stride, inst, s);
     // Inner products of every key limb: the limbs below the dropped one into
     // `out` (NTT domain), the dropped limb into `special`, which alone goes
     // back to power basis; then the transformed power-basis part is added in.
-    const bool paired = paired_finish(inst);
     u64* base = scratch(Scratch::SwitchDown).get(s, finish_bytes(inst)).as<u64>();
     u64* special = extra ? base : nullptr;
     KsFinishArgs f{a, y, fin0.as<u64>(), fin1.as<u64>(), digits, (u32)Lk, special,
                    key_ctx->mod.as<ModInfo>() + (Lk - 1)};
     launch_ks_mac(ct_ctx->logn, f, inst, s);
+    if (!extra && !a.add_pb) return;  // the finish would add NTT(0)
     if (paired) {
         u64* pre = base + (size_t)inst * 2 * extra * n;
         if (extra) launch_ntt_backward_halves(*key_ctx, RowMap{special, 1, 1, 0, (u32)(Lk - 1)}, inst * 2, s);

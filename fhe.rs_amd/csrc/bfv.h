@@ -124,7 +124,10 @@ struct Ksk {
     // (launch_ks_z_pair, then the half transforms).
     bool paired_finish(u32 inst) const;
     // Digit lift + lazy forward NTT into scratch [inst][digits][L'][N].
-    u64* lift(const u64* src, size_t src_inst_stride, u32 inst, hipStream_t s) const;
+    // `rows` (optional): rows of the ciphertext context transformed alongside
+    // (launch_lift_ntt_split).
+    u64* lift(const u64* src, size_t src_inst_stride, u32 inst, hipStream_t s,
+              const HalfFwdRows* rows = nullptr) const;
     // Full key switch into acc [inst][2][L'][N] (NTT domain, or power basis
     // when to_power_basis).
     void key_switch_pb(const u64* src, size_t src_inst_stride, u64* acc, u32 inst, bool to_power_basis,

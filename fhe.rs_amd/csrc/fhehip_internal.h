@@ -91,11 +91,20 @@ void launch_ntt_backward(const DeviceContext& c, RowMap m, u32 rows, hipStream_t
 // NTT kernels by timing `rows` rows each way (ms: fwd16, fwd32, inv16, inv32).
 void autotune_ntt(const DeviceContext& c, u32 rows, int& re_fwd, int& re_inv, float (&ms)[4]);
 void set_ntt_shape(int logn, int re_fwd, int re_inv);
-// Degree 32768: y[b][i][j] = forward_vt_lazy(lazy_reduce_{q'_j}(src[b][i]))
-// for every digit i < digits and key limb j of `key`.
+// Rows of one context transformed in the same launch as the digit lift:
+// rows [rows][N] after their first forward stage (< 4p), row r of limb r % L,
+// transformed in place to reduced NTT values with the half tables half_tw.
+struct HalfFwdRows {
+    u64* x;
+    u32 rows, L;
+    const ulonglong2* half_tw;
+    const ModInfo* mods;
+};
+// Half-row transforms: y[b][i][j] = forward_vt_lazy(lazy_reduce_{q'_j}(src[b][i]))
+// for every digit i < digits and key limb j of `key`, and the `rows`.
 // log_base != 0: power-of-two decomposition digits of limb 0 (one-modulus keys).
 void launch_lift_ntt_split(const DeviceContext& key, const u64* src, size_t src_inst, u64* y, u32 digits, u32 inst,
-                           u32 log_base, hipStream_t s);
+                           u32 log_base, hipStream_t s, const HalfFwdRows* rows = nullptr);
 // The last inverse stage of `rows` rows whose two halves went through the
 // half-row inverse transforms (N^-1 folded in).
 void launch_ntt_split_inv(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s);

@@ -163,17 +163,39 @@ bool split_rows(const DeviceContext& c, size_t rows) {
 // the half tables; outputs < 2 q'_j, as lift_ntt_kernel's. 1-D grid over
 // (instance, digit, key limb, half), half fastest: the two halves of one row
 // and the key limbs of one digit read the same source row and run on one XCD.
+//
+// The last 2 fr.rows workgroups of the grid transform the rows of `fr`
+// instead (fwd_rows_half_body), so that independent rows share the launch.
+template <int LOGH>
+__device__ __forceinline__ void fwd_rows_half_body(u64* lds, const HalfFwdRows& fr, u32 l, int t) {
+    using S = NttShape<LOGH>;
+    const u32 h = l & 1, r = l >> 1, i = r % fr.L;
+    const u64 p = fr.mods[i].p, p2 = 2 * p;
+    u64* row = fr.x + ((size_t)r << (LOGH + 1)) + ((size_t)h << LOGH);
+    u64 x[S::E];
+#pragma unroll
+    for (int e = 0; e < S::E; e++) x[e] = row[fwd_index<LOGH, 0>(t, e)];
+    ntt_fwd_regs<LOGH>(lds, x, t, fr.half_tw + ((size_t)(2 * i + h) << LOGH), p);
+    store_fwd_order<LOGH>(lds, x, t, [&](int k, u64 v) { row[k] = reduce1(reduce1(v, p2), p); });
+}
+
 template <int LOGH>
 __global__ __launch_bounds__(NttShape<LOGH>::T) void lift_ntt_half_kernel(const u64* __restrict__ src, size_t src_inst,
                                                                        u64* __restrict__ y,
                                                                        const ulonglong2* __restrict__ half_tw,
                                                                        const ModInfo* __restrict__ mods,
                                                                        const ulonglong2* __restrict__ split,
-                                                                       u32 digits, u32 key_limbs, u32 log_base) {
+                                                                       u32 digits, u32 key_limbs, u32 log_base,
+                                                                       HalfFwdRows fr) {
     using S = NttShape<LOGH>;
     __shared__ u64 lds[S::LDS_WORDS];
     const int t = threadIdx.x;
     const u32 l = xcd_block(blockIdx.x, gridDim.x);
+    const u32 lift_blocks = gridDim.x - 2 * fr.rows;
+    if (l >= lift_blocks) {
+        fwd_rows_half_body<LOGH>(lds, fr, l - lift_blocks, t);
+        return;
+    }
     const u32 h = l & 1, r = l >> 1;
     const u32 j = r % key_limbs, i = (r / key_limbs) % digits, b = r / (key_limbs * digits);
     const ModInfo mi = mods[j];
@@ -214,21 +236,31 @@ __global__ __launch_bounds__(NttShape<LOGH>::T) void lift_ntt_half_kernel(const 
 
 template <int LOGH>
 static void lift_half_go(dim3 grid, hipStream_t s, const u64* src, size_t src_inst, u64* y, const ulonglong2* tw,
-                         const ModInfo* mods, const ulonglong2* split, u32 digits, u32 key_limbs, u32 log_base) {
+                         const ModInfo* mods, const ulonglong2* split, u32 digits, u32 key_limbs, u32 log_base,
+                         const HalfFwdRows& fr) {
     hipLaunchKernelGGL(lift_ntt_half_kernel<LOGH>, grid, dim3(NttShape<LOGH>::T), 0, s, src, src_inst, y, tw, mods,
-                       split, digits, key_limbs, log_base);
+                       split, digits, key_limbs, log_base, fr);
 }
 
 void launch_lift_ntt_split(const DeviceContext& key, const u64* src, size_t src_inst, u64* y, u32 digits, u32 inst,
-                           u32 log_base, hipStream_t s) {
+                           u32 log_base, hipStream_t s, const HalfFwdRows* rows) {
     if (!key.half) throw Error(-2, "this context has no half-row tables");
     if (!digits || !inst) return;
+    HalfFwdRows fr{};
+    if (rows && rows->rows) {
+        if (rows->L == 0) throw Error(-1, "rows to transform without a context");
+        fr = *rows;
+    }
     ProfScope ps("lift_ntt", key.logn, s);
+    const dim3 grid(2 * (inst * digits * key.L + fr.rows));
+    const ulonglong2* tw = key.half->fwd_tw.as<ulonglong2>();
+    const ModInfo* md = key.mod.as<ModInfo>();
+    const ulonglong2* sp = key.split.as<ulonglong2>();
     switch (key.logn - 1) {
-        case 11: lift_half_go<11>(dim3(2 * inst * digits * key.L), s, src, src_inst, y, key.half->fwd_tw.as<ulonglong2>(), key.mod.as<ModInfo>(), key.split.as<ulonglong2>(), digits, key.L, log_base); break;
-        case 12: lift_half_go<12>(dim3(2 * inst * digits * key.L), s, src, src_inst, y, key.half->fwd_tw.as<ulonglong2>(), key.mod.as<ModInfo>(), key.split.as<ulonglong2>(), digits, key.L, log_base); break;
-        case 13: lift_half_go<13>(dim3(2 * inst * digits * key.L), s, src, src_inst, y, key.half->fwd_tw.as<ulonglong2>(), key.mod.as<ModInfo>(), key.split.as<ulonglong2>(), digits, key.L, log_base); break;
-        case 14: lift_half_go<14>(dim3(2 * inst * digits * key.L), s, src, src_inst, y, key.half->fwd_tw.as<ulonglong2>(), key.mod.as<ModInfo>(), key.split.as<ulonglong2>(), digits, key.L, log_base); break;
+        case 11: lift_half_go<11>(grid, s, src, src_inst, y, tw, md, sp, digits, key.L, log_base, fr); break;
+        case 12: lift_half_go<12>(grid, s, src, src_inst, y, tw, md, sp, digits, key.L, log_base, fr); break;
+        case 13: lift_half_go<13>(grid, s, src, src_inst, y, tw, md, sp, digits, key.L, log_base, fr); break;
+        case 14: lift_half_go<14>(grid, s, src, src_inst, y, tw, md, sp, digits, key.L, log_base, fr); break;
         default: throw Error(-2, "half-row transforms need a degree of 4096 .. 32768");
     }
     HIPCHK(hipGetLastError());
