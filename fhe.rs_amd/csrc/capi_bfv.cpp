@@ -8,6 +8,7 @@
 #include "capi_internal.h"
 #include "graph.h"
 #include "jobs.h"
+#include "prng.h"
 #include "server.h"
 
 using namespace fhehip;
@@ -697,6 +698,48 @@ int fhehip_generate_moduli(const uint32_t* sizes, uint32_t count, uint32_t degre
         }
     }
     std::copy(chosen.begin(), chosen.end(), out);
+    FHEHIP_API_END
+}
+
+static bool ntt_friendly(u64 p, u32 degree) {
+    return degree >= 8 && !(degree & (degree - 1)) && p >= 2 && !(p >> 62) && supports_ntt64(p, degree);
+}
+
+int fhehip_default_psi(uint32_t degree, uint64_t modulus, uint64_t* psi) {
+    FHEHIP_API_BEGIN
+    if (!psi) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    if (!ntt_friendly(modulus, degree)) return fail(FHEHIP_ERR_UNSUPPORTED, "not an NTT-friendly prime for degree");
+    u64 r = fhers_primitive_root(degree, modulus);
+    *psi = r ? r : primitive_root64(degree, modulus);
+    FHEHIP_API_END
+}
+
+static int check_seeded(const uint64_t* moduli, uint32_t num_moduli, uint32_t degree, const uint8_t* seed,
+                        const uint64_t* out) {
+    if (!moduli || !seed || !out || !num_moduli) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    if (degree < 8 || (degree & (degree - 1))) return fail(FHEHIP_ERR_INVALID_ARG, "degree is not a power of two >= 8");
+    for (uint32_t i = 0; i < num_moduli; i++)
+        if (moduli[i] < 2 || (moduli[i] >> 62)) return fail(FHEHIP_ERR_INVALID_ARG, "InvalidModulus");
+    return FHEHIP_OK;
+}
+
+int fhehip_random_from_seed(const uint64_t* moduli, uint32_t num_moduli, uint32_t degree, const uint8_t* seed,
+                            uint64_t* out) {
+    FHEHIP_API_BEGIN
+    if (int rc = check_seeded(moduli, num_moduli, degree, seed, out)) return rc;
+    Seed32 s;
+    std::memcpy(s.data(), seed, 32);
+    random_from_seed(std::vector<u64>(moduli, moduli + num_moduli), degree, s, out);
+    FHEHIP_API_END
+}
+
+int fhehip_generate_c1(const uint64_t* moduli, uint32_t num_moduli, uint32_t degree, const uint8_t* seed,
+                       uint32_t count, uint64_t* out) {
+    FHEHIP_API_BEGIN
+    if (int rc = check_seeded(moduli, num_moduli, degree, seed, out)) return rc;
+    Seed32 s;
+    std::memcpy(s.data(), seed, 32);
+    generate_c1(std::vector<u64>(moduli, moduli + num_moduli), degree, s, count, out);
     FHEHIP_API_END
 }
 

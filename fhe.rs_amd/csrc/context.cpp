@@ -2,6 +2,7 @@
 #include <cstring>
 
 #include "fhehip_internal.h"
+#include "prng.h"
 
 namespace fhehip {
 
@@ -61,7 +62,10 @@ std::shared_ptr<DeviceContext> make_context(const std::vector<u64>& moduli, u32 
     std::vector<u64> pw(n), pwi(n);
     for (u32 l = 0; l < c->L; l++) {
         const u64 p = moduli[l];
-        u64 r = psi ? psi[l] : primitive_root64(n, p);
+        // NttOperator::new's root (native.rs:320-336); the scan of 2, 3, ... only
+        // if its 100 draws find none (fhe.rs fails there)
+        u64 r = psi ? psi[l] : fhers_primitive_root(n, p);
+        if (!r) r = primitive_root64(n, p);
         if (psi && (powmod64(r, n, p) != p - 1)) throw Error(-1, "psi is not a primitive 2N-th root of unity");
         c->psi.push_back(r);
         const u64 rinv = invmod64(r, p);

@@ -101,6 +101,9 @@ SIGNATURES = {
     "fhehip_graph_destroy": (None, [_vp]),
     "fhehip_submit": (C.c_int, [_vp, C.c_uint32, _vp]),
     "fhehip_generate_moduli": (C.c_int, [_u32p, C.c_uint32, C.c_uint32, _u64p]),
+    "fhehip_default_psi": (C.c_int, [C.c_uint32, C.c_uint64, _u64p]),
+    "fhehip_random_from_seed": (C.c_int, [_u64p, C.c_uint32, C.c_uint32, C.c_char_p, _u64p]),
+    "fhehip_generate_c1": (C.c_int, [_u64p, C.c_uint32, C.c_uint32, C.c_char_p, C.c_uint32, _u64p]),
     "fhehip_server_create": (C.c_int, [_vp, _vpp]),
     "fhehip_server_destroy": (None, [_vp]),
     "fhehip_server_add_params": (C.c_int, [_vp, _vp, _vp, C.c_uint32, _u32p, _u64p]),
@@ -732,6 +735,40 @@ def generate_moduli(sizes, degree):
     out = (C.c_uint64 * max(len(sizes), 1))()
     check(lib().fhehip_generate_moduli(sz, len(sizes), degree, out))
     return [int(out[i]) for i in range(len(sizes))]
+
+
+def default_psi(degree, modulus):
+    """fhehip_default_psi: NttOperator::new's root of `modulus` for `degree`."""
+    out = C.c_uint64(0)
+    check(lib().fhehip_default_psi(degree, modulus, C.byref(out)))
+    return int(out.value)
+
+
+def _moduli_arg(moduli):
+    m = (C.c_uint64 * len(moduli))(*[int(q) for q in moduli])
+    return C.cast(m, _u64p), m
+
+
+def random_from_seed(moduli, degree, seed: bytes):
+    """fhehip_random_from_seed -> numpy [L * degree] (host)."""
+    import numpy as np
+    if len(seed) != 32:
+        raise ValueError("a seed has 32 bytes")
+    out = np.zeros(len(moduli) * degree, dtype=np.uint64)
+    mp, _keep = _moduli_arg(moduli)
+    check(lib().fhehip_random_from_seed(mp, len(moduli), degree, seed, out.ctypes.data_as(_u64p)))
+    return out
+
+
+def generate_c1(moduli, degree, seed: bytes, count):
+    """fhehip_generate_c1 -> numpy [count * L * degree] (host)."""
+    import numpy as np
+    if len(seed) != 32:
+        raise ValueError("a seed has 32 bytes")
+    out = np.zeros(count * len(moduli) * degree, dtype=np.uint64)
+    mp, _keep = _moduli_arg(moduli)
+    check(lib().fhehip_generate_c1(mp, len(moduli), degree, seed, count, out.ctypes.data_as(_u64p)))
+    return out
 
 
 def profiler_enable(on=True):

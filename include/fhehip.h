@@ -59,10 +59,10 @@ const char* fhehip_version(void);
 /* ---- Contexts and NTT ---------------------------------------------------
  * fhehip_context mirrors rq::Context::new (math/src/rq/context.rs:42-93) with
  * one NttOperator::new (math/src/ntt/native.rs:35-73) per modulus. `psi` may
- * hold, per modulus, the primitive 2N-th root to use; NULL selects the
- * library's deterministic root (first g = 2,3,... with g^((p-1)/2N) of order
- * 2N). Returns FHEHIP_ERR_UNSUPPORTED when a modulus is not an NTT-friendly
- * prime for `degree` (the None case of NttOperator::new, ntt/mod.rs:19-23). */
+ * hold, per modulus, the primitive 2N-th root to use; NULL selects fhe.rs's
+ * own root (fhehip_default_psi). Returns FHEHIP_ERR_UNSUPPORTED when a
+ * modulus is not an NTT-friendly prime for `degree` (the None case of
+ * NttOperator::new, ntt/mod.rs:19-23). */
 int fhehip_context_create(const uint64_t* moduli, uint32_t num_moduli, uint32_t degree, const uint64_t* psi,
                           fhehip_context** out);
 void fhehip_context_destroy(fhehip_context* ctx);
@@ -513,6 +513,25 @@ int fhehip_server_stats(fhehip_server* s, fhehip_server_statistics* out);
  * order, the largest prime below 2^size congruent to 1 mod 2 * degree not
  * already chosen. out receives `count` moduli. */
 int fhehip_generate_moduli(const uint32_t* sizes, uint32_t count, uint32_t degree, uint64_t* out);
+
+/* ---- Seeded values (host) -------------------------------------------------
+ * The pseudo-random draws fhe.rs makes from seeds, on the host (no device
+ * needed): ChaCha8Rng (rand_chacha 0.9), SHA-256 (sha2 0.10) and rand 0.9's
+ * uniform integers.
+ * NttOperator::new's root (math/src/ntt/native.rs:320-336): the first of 100
+ * draws random_range(0..p) from ChaCha8Rng::seed_from_u64(0), raised to
+ * (p - 1) / 2N, of order 2N. Contexts created without `psi` use it. */
+int fhehip_default_psi(uint32_t degree, uint64_t modulus, uint64_t* psi);
+/* Poly::random_from_seed (math/src/rq/mod.rs:241-257): out[l][j], l <
+ * num_moduli, j < degree, uniform below moduli[l] from ChaCha8Rng seeded with
+ * SHA-256(seed); NTT-domain values. seed: 32 bytes. */
+int fhehip_random_from_seed(const uint64_t* moduli, uint32_t num_moduli, uint32_t degree, const uint8_t* seed,
+                            uint64_t* out);
+/* KeySwitchingKey::generate_c1 (fhe/src/bfv/keys/key_switching_key.rs:108-123):
+ * `count` polynomials [count][num_moduli][degree], each random_from_seed of a
+ * 32-byte seed filled from ChaCha8Rng(seed). */
+int fhehip_generate_c1(const uint64_t* moduli, uint32_t num_moduli, uint32_t degree, const uint8_t* seed,
+                       uint32_t count, uint64_t* out);
 
 /* ---- Measurement ------------------------------------------------------------
  * No counterpart in fhe.rs. When enabled, every kernel launch is bracketed by a

@@ -41,6 +41,47 @@ uint64_t or_primitive_root(uint32_t n, uint64_t p) {
     try { return primitive_root(n, Modulus(p)); } catch (...) { return 0; }
 }
 
+// ---- Pseudo-random streams (oracle_rng.hpp) ------------------------------------
+static std::array<uint8_t, 32> seed32(const uint8_t* s) {
+    std::array<uint8_t, 32> a;
+    std::memcpy(a.data(), s, 32);
+    return a;
+}
+void or_sha256(const uint8_t* data, size_t len, uint8_t* out) {
+    const auto h = oracle_rng::sha256(data, len);
+    std::memcpy(out, h.data(), 32);
+}
+void or_chacha_block(const uint32_t* in, int rounds, uint32_t* out) { oracle_rng::chacha_block(in, rounds, out); }
+// kind 0: n x next_u32 (u32 out), 1: n x next_u64, 2: n bytes of fill_bytes;
+// kind 3: n x next_u32 then next_u64 alternating (u64 out) for the buffer edges.
+void or_chacha8_stream(const uint8_t* seed, int kind, size_t n, void* out) {
+    oracle_rng::ChaCha8Rng r(seed32(seed));
+    if (kind == 0) for (size_t i = 0; i < n; i++) ((uint32_t*)out)[i] = r.next_u32();
+    else if (kind == 1) for (size_t i = 0; i < n; i++) ((uint64_t*)out)[i] = r.next_u64();
+    else if (kind == 2) r.fill_bytes((uint8_t*)out, n);
+    else for (size_t i = 0; i < n; i++) ((uint64_t*)out)[i] = (i & 1) ? r.next_u64() : r.next_u32();
+}
+void or_pcg32_seed(uint64_t state, uint8_t* out) {
+    const auto s = oracle_rng::pcg32_seed(state);
+    std::memcpy(out, s.data(), 32);
+}
+uint32_t or_pcg32_output(uint64_t state) { return oracle_rng::pcg32_output(state); }
+// n draws of Uniform::new(0, range).sample (single = 0) or random_range(low..high) (single = 1)
+void or_uniform(const uint8_t* seed, int single, uint64_t low, uint64_t high, size_t n, uint64_t* out) {
+    oracle_rng::ChaCha8Rng r(seed32(seed));
+    for (size_t i = 0; i < n; i++)
+        out[i] = single ? oracle_rng::uniform_single(r, low, high) : oracle_rng::uniform_sample(r, high);
+}
+void or_random_from_seed(const uint64_t* moduli, uint32_t L, uint32_t n, const uint8_t* seed, uint64_t* out) {
+    const auto v = oracle_rng::random_from_seed(std::vector<u64>(moduli, moduli + L), n, seed32(seed));
+    std::copy(v.begin(), v.end(), out);
+}
+void or_generate_c1(const uint64_t* moduli, uint32_t L, uint32_t n, const uint8_t* seed, uint32_t size,
+                    uint64_t* out) {
+    const auto v = oracle_rng::generate_c1(std::vector<u64>(moduli, moduli + L), n, seed32(seed), size);
+    std::copy(v.begin(), v.end(), out);
+}
+
 // Elementwise Modulus operations for the property tests.
 // op: 0 add, 1 sub, 2 mul, 3 mul_opt, 4 shoup(a), 5 mul_shoup(a,b), 6 lazy_mul_shoup,
 //     7 reduce(a), 8 lazy_reduce(a), 9 reduce_u128(a_lo | b_hi<<64), 10 lazy_reduce_u128,

@@ -13,8 +13,9 @@
 //   generate_moduli      vs KAT fhe/src/bfv/parameters.rs:740-773      -> pinned
 //   NttOperator          vs closed form A[i] = a(psi^(2 brv(i)+1)),
 //                        O(N^2) negacyclic brute force, round trip    -> pinned
-//   primitive_root       own deterministic rule (fhe.rs draws candidates
-//                        from ChaCha8Rng(0), absent here)              -> parity unpinned
+//   primitive_root       fhe.rs's draw from ChaCha8Rng::seed_from_u64(0)
+//                        (oracle_rng.hpp); ChaCha and PCG pinned, the
+//                        uniform draw (rand 0.9) unpinned              -> parity unpinned
 //                        for the NTT-domain representation only; every
 //                        power-basis result is independent of the root.
 //   RnsContext           vs KAT math/src/rns/mod.rs:205-242            -> pinned
@@ -35,6 +36,8 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+
+#include "oracle_rng.hpp"
 
 namespace oracle {
 
@@ -372,15 +375,15 @@ static inline bool supports_ntt(u64 p, size_t n) {
     return n >= 8 && (n & (n - 1)) == 0 && p % ((u64)n << 1) == 1 && is_prime(p);
 }
 
-// Primitive 2n-th root of unity mod p. fhe.rs (math/src/ntt/native.rs:320-336)
-// raises candidates drawn from ChaCha8Rng(seed 0) to the power (p-1)/(2n);
-// this restatement scans the candidates 2, 3, 4, ... instead. Any primitive
-// 2n-th root yields the same power-basis results.
+// Primitive 2n-th root of unity mod p (math/src/ntt/native.rs:320-336): up
+// to 100 candidates drawn with random_range(0..p) from
+// ChaCha8Rng::seed_from_u64(0), each raised to the power (p-1)/(2n), the
+// first of order 2n (native.rs:341-348) kept.
 static inline u64 primitive_root(size_t n, const Modulus& p) {
-    u64 lambda = (p.p - 1) / (2 * (u64)n);
-    for (u64 g = 2; g < p.p; g++) {
-        u64 root = p.pow(g, lambda);
-        // native.rs:341-348
+    const u64 lambda = (p.p - 1) / (2 * (u64)n);
+    oracle_rng::ChaCha8Rng rng(oracle_rng::pcg32_seed(0));
+    for (int i = 0; i < 100; i++) {
+        const u64 root = p.pow(oracle_rng::uniform_single(rng, 0, p.p), lambda);
         if (p.pow(root, 2 * (u64)n) == 1 && p.pow(root, (u64)n) != 1) return root;
     }
     throw std::runtime_error("no primitive root");

@@ -91,6 +91,14 @@ def lib():
             "or_random_polys": (C.c_int, [_u64p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, _u64p]),
             "or_bench_mul_relin": (C.c_int, [_vp, _u64p, _u64p, C.c_uint32, C.c_uint32, _dblp, _dblp]),
             "or_bench_ntt": (C.c_int, [C.c_uint64, C.c_uint32, _u64p, C.c_uint32, C.c_uint32, _dblp]),
+            "or_sha256": (None, [C.c_char_p, C.c_size_t, C.POINTER(C.c_uint8)]),
+            "or_chacha_block": (None, [_u32p, C.c_int, _u32p]),
+            "or_chacha8_stream": (None, [C.c_char_p, C.c_int, C.c_size_t, C.c_void_p]),
+            "or_pcg32_seed": (None, [C.c_uint64, C.POINTER(C.c_uint8)]),
+            "or_pcg32_output": (C.c_uint32, [C.c_uint64]),
+            "or_uniform": (None, [C.c_char_p, C.c_int, C.c_uint64, C.c_uint64, C.c_size_t, _u64p]),
+            "or_random_from_seed": (None, [_u64p, C.c_uint32, C.c_uint32, C.c_char_p, _u64p]),
+            "or_generate_c1": (None, [_u64p, C.c_uint32, C.c_uint32, C.c_char_p, C.c_uint32, _u64p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -169,6 +177,65 @@ def generate_moduli(sizes, degree):
 
 def primitive_root(n, p):
     return int(lib().or_primitive_root(n, p))
+
+
+# ---- pseudo-random streams (oracle_rng.hpp) ----------------------------------
+def sha256(data: bytes) -> bytes:
+    out = (C.c_uint8 * 32)()
+    lib().or_sha256(data, len(data), out)
+    return bytes(out)
+
+
+def chacha_block(state16, rounds):
+    a = np.asarray(state16, dtype=np.uint32)
+    out = np.zeros(16, dtype=np.uint32)
+    lib().or_chacha_block(_p32(a), rounds, _p32(out))
+    return out
+
+
+def chacha8_stream(seed: bytes, kind: str, n: int):
+    """ChaCha8Rng::from_seed(seed): n x next_u32 ("u32"), next_u64 ("u64"),
+    n bytes of fill_bytes ("bytes"), or next_u32 / next_u64 alternating ("mixed")."""
+    assert len(seed) == 32
+    k = {"u32": 0, "u64": 1, "bytes": 2, "mixed": 3}[kind]
+    if k == 0:
+        out = np.zeros(n, dtype=np.uint32)
+    elif k == 2:
+        out = np.zeros(n, dtype=np.uint8)
+    else:
+        out = np.zeros(n, dtype=np.uint64)
+    lib().or_chacha8_stream(seed, k, n, out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+def pcg32_seed(state: int) -> bytes:
+    out = (C.c_uint8 * 32)()
+    lib().or_pcg32_seed(state, out)
+    return bytes(out)
+
+
+def pcg32_output(state: int) -> int:
+    return int(lib().or_pcg32_output(state))
+
+
+def uniform(seed: bytes, n: int, high: int, low: int = 0, single: bool = False):
+    out = np.zeros(n, dtype=np.uint64)
+    lib().or_uniform(seed, 1 if single else 0, low, high, n, _p64(out))
+    return out
+
+
+def random_from_seed(moduli, n, seed: bytes):
+    m = u64(moduli)
+    out = np.zeros(len(moduli) * n, dtype=np.uint64)
+    lib().or_random_from_seed(_p64(m), len(moduli), n, seed, _p64(out))
+    return out
+
+
+def generate_c1(moduli, n, seed: bytes, size: int):
+    m = u64(moduli)
+    out = np.zeros(size * len(moduli) * n, dtype=np.uint64)
+    lib().or_generate_c1(_p64(m), len(moduli), n, seed, size, _p64(out))
+    return out
 
 
 MOD_OPS = {n: i for i, n in enumerate(
