@@ -107,6 +107,10 @@ struct Ksk {
     // log_base bits (key_switching_key.rs:70-86, 292-329); 0: one digit per
     // ciphertext modulus.
     u32 log_base = 0;
+    // The 32-byte seed the c1 rows were drawn from (KeySwitchingKey::seed,
+    // key_switching_key.rs:108-123) when the key came from a seeded message;
+    // empty otherwise.
+    std::vector<uint8_t> seed;
     DevBuf key0, key1;  // [digits][L'][N] (value, shoup)
     // Rows of the key limbs below the dropped one with the switch-down factor
     // q_l^-1 folded in (plain values, [digits][L][N]); used by the fused
@@ -191,6 +195,7 @@ struct Multiplicator {
 struct Parameters {
     u32 N = 0;
     u64 t = 0;
+    u32 variance = 10;  // the error variance of BfvParametersBuilder (parameters.rs:318-334)
     std::vector<u64> moduli, extension;
     std::vector<std::shared_ptr<DeviceContext>> levels;
     std::shared_ptr<DeviceContext> plain;

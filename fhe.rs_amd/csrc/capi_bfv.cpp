@@ -9,6 +9,7 @@
 #include "graph.h"
 #include "jobs.h"
 #include "prng.h"
+#include "proto.h"
 #include "server.h"
 
 using namespace fhehip;
@@ -698,6 +699,355 @@ int fhehip_generate_moduli(const uint32_t* sizes, uint32_t count, uint32_t degre
         }
     }
     std::copy(chosen.begin(), chosen.end(), out);
+    FHEHIP_API_END
+}
+
+}  // extern "C"
+
+// ---- Protobuf messages -------------------------------------------------------
+namespace {
+
+// Level of context c in the parameters' chain (its moduli are a prefix).
+u32 level_of(const Parameters& par, const DeviceContext& c) {
+    const u32 L0 = (u32)par.moduli.size();
+    if (c.N != par.N || c.L > L0 || !std::equal(c.moduli.begin(), c.moduli.end(), par.moduli.begin()))
+        throw Error(FHEHIP_ERR_CTX_MISMATCH, "the context is not a level of these parameters");
+    return L0 - c.L;
+}
+
+const std::shared_ptr<DeviceContext>& level_ctx(const Parameters& par, u32 level) {
+    if (level >= par.levels.size()) throw Error(FHEHIP_ERR_BAD_LEVEL, "level above the parameters' max_level");
+    return par.levels[level];
+}
+
+int copy_message(const std::vector<uint8_t>& msg, uint8_t* out, size_t cap, size_t* len) {
+    if (!len) return fail(FHEHIP_ERR_INVALID_ARG, "null length");
+    *len = msg.size();
+    if (!out && !cap) return FHEHIP_OK;  // size query
+    if (!out || cap < msg.size()) return fail(FHEHIP_ERR_INVALID_ARG, "output buffer too small (*len holds the size)");
+    std::memcpy(out, msg.data(), msg.size());
+    return FHEHIP_OK;
+}
+
+// Payloads of `num` NTT-domain polynomials of c (power basis, bit-packed),
+// on the host; waits for the stream.
+std::vector<uint8_t> payloads(const DeviceContext& c, const u64* d_polys, u32 num, hipStream_t s) {
+    const size_t per = serialization_length(c);
+    std::vector<uint8_t> h(per * num);
+    if (!num) return h;
+    DevBuf d(h.size());
+    serialize_polys(c, d_polys, num, true, d.as<uint8_t>(), s);
+    HIPCHK(hipMemcpyAsync(h.data(), d.ptr, h.size(), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return h;
+}
+
+// Poly::<R>::from_bytes (math/src/rq/serialize.rs:23-26, convert.rs:42-120)
+// of each message: representation `repr`, the context's degree and payload
+// length; into d_out [num][L][N] in the NTT domain. Waits for the stream.
+void rq_to_device(const DeviceContext& c, const std::vector<pb::Span>& msgs, uint32_t repr, u64* d_out, hipStream_t s) {
+    if (msgs.empty()) return;
+    const size_t per = serialization_length(c);
+    std::vector<uint8_t> h(per * msgs.size());
+    for (size_t i = 0; i < msgs.size(); i++) {
+        const pb::Rq r = pb::parse_rq(msgs[i]);
+        if (r.representation == pb::REPR_UNKNOWN || r.representation > pb::REPR_NTTSHOUP)
+            throw Error(-1, "Unknown representation");
+        if (r.representation != repr)
+            throw Error(-1, "The representation asked for does not match the representation in the serialization");
+        if (r.degree % 8 || r.degree < 8) throw Error(-1, "Invalid degree");
+        if (r.degree != c.N || r.coefficients.len != per) throw Error(-1, "Invalid coefficients");
+        std::memcpy(h.data() + i * per, r.coefficients.data, per);
+    }
+    DevBuf d(h.size());
+    HIPCHK(hipMemcpyAsync(d.ptr, h.data(), h.size(), hipMemcpyHostToDevice, s));
+    deserialize_polys(c, d.as<uint8_t>(), (u32)msgs.size(), true, d_out, s);
+    HIPCHK(hipStreamSynchronize(s));
+}
+
+std::vector<uint8_t> seed_of(pb::Span s) {
+    if (s.len != 32) throw Error(-1, "Invalid seed (32 bytes expected)");
+    return std::vector<uint8_t>(s.data, s.data + 32);
+}
+
+// KeySwitchingKey::try_convert_from (fhe/src/bfv/keys/key_switching_key.rs:354-426).
+std::shared_ptr<Ksk> ksk_from_message(const Parameters& par, pb::Span bytes) {
+    const pb::KeySwitchingKey m = pb::parse_ksk(bytes);
+    const auto& key_ctx = level_ctx(par, m.ksk_level);
+    const auto& ct_ctx = level_ctx(par, m.ciphertext_level);
+    size_t c0_size;
+    if (m.log_base) {
+        const u32 max_level = (u32)par.levels.size() - 1;
+        if (m.ksk_level != max_level || m.ciphertext_level != max_level)
+            throw Error(-1, "A decomposition size is specified but the levels are not maximal");
+        u32 log_modulus = 0;
+        while (log_modulus < 64 && ((u64)1 << log_modulus) < par.moduli[0]) log_modulus++;
+        if (m.log_base != log_modulus / 2)
+            throw Error(FHEHIP_ERR_UNSUPPORTED, "only the decomposition of KeySwitchingKey::new (log2(q) / 2 bits)");
+        c0_size = (log_modulus + m.log_base - 1) / m.log_base;
+    } else {
+        c0_size = ct_ctx->L;
+    }
+    if (m.c0.size() != c0_size) throw Error(-1, "Incorrect number of values in c0");
+    std::vector<uint8_t> seed;
+    if (m.seed.len == 0) {
+        if (m.c1.size() != c0_size) throw Error(-1, "Incorrect number of values in c1");
+    } else {
+        seed = seed_of(m.seed);
+    }
+    const size_t row = (size_t)key_ctx->L * key_ctx->N;
+    std::vector<u64> c0(c0_size * row), c1(c0_size * row);
+    hipStream_t s;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    try {
+        DevBuf d(c0.size() * 8);
+        rq_to_device(*key_ctx, m.c0, pb::REPR_NTTSHOUP, d.as<u64>(), s);
+        HIPCHK(hipMemcpy(c0.data(), d.ptr, d.bytes, hipMemcpyDeviceToHost));
+        if (seed.empty()) {
+            rq_to_device(*key_ctx, m.c1, pb::REPR_NTTSHOUP, d.as<u64>(), s);
+            HIPCHK(hipMemcpy(c1.data(), d.ptr, d.bytes, hipMemcpyDeviceToHost));
+        } else {
+            Seed32 sd;
+            std::memcpy(sd.data(), seed.data(), 32);
+            generate_c1(key_ctx->moduli, key_ctx->N, sd, (u32)c0_size, c1.data());
+        }
+    } catch (...) {
+        (void)hipStreamDestroy(s);
+        throw;
+    }
+    HIPCHK(hipStreamDestroy(s));
+    auto k = std::make_shared<Ksk>(ct_ctx, key_ctx, c0.data(), c1.data(), (u32)c0_size);
+    k->seed = std::move(seed);
+    return k;
+}
+
+// From<&KeySwitchingKey> for KeySwitchingKeyProto (key_switching_key.rs:332-352).
+std::vector<uint8_t> ksk_message(const Parameters& par, const Ksk& k, bool variable_time) {
+    const u32 ct_level = level_of(par, *k.ct_ctx), key_level = level_of(par, *k.key_ctx);
+    const DeviceContext& kc = *k.key_ctx;
+    const size_t row = (size_t)kc.L * kc.N, rows = (size_t)k.digits * row;
+    std::vector<ulonglong2> h(rows);
+    std::vector<u64> vals(rows);
+    hipStream_t s;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    std::vector<std::vector<uint8_t>> c0, c1;
+    try {
+        DevBuf d(rows * 8);
+        for (int part = 0; part < 2; part++) {
+            if (part == 1 && !k.seed.empty()) break;  // the seed stands for c1
+            HIPCHK(hipMemcpy(h.data(), (part ? k.key1 : k.key0).ptr, rows * sizeof(ulonglong2), hipMemcpyDeviceToHost));
+            for (size_t i = 0; i < rows; i++) vals[i] = h[i].x;
+            HIPCHK(hipMemcpy(d.ptr, vals.data(), rows * 8, hipMemcpyHostToDevice));
+            const std::vector<uint8_t> pl = payloads(kc, d.as<u64>(), k.digits, s);
+            const size_t per = pl.size() / k.digits;
+            for (u32 r = 0; r < k.digits; r++)
+                (part ? c1 : c0).push_back(pb::write_rq(pb::REPR_NTTSHOUP, kc.N, pl.data() + r * per, per, variable_time));
+        }
+    } catch (...) {
+        (void)hipStreamDestroy(s);
+        throw;
+    }
+    HIPCHK(hipStreamDestroy(s));
+    return pb::write_ksk(c0, c1, k.seed.empty() ? nullptr : k.seed.data(), ct_level, key_level, k.log_base);
+}
+
+std::vector<uint8_t> wrap_ksk(const std::vector<uint8_t>& ksk, uint32_t exponent) {
+    pb::Writer w;
+    w.bytes_field(1, ksk.data(), ksk.size());
+    w.uint_field(2, exponent);
+    return std::move(w.out);
+}
+
+}  // namespace
+
+extern "C" {
+
+int fhehip_ciphertext_to_bytes(const fhehip_parameters* p, uint32_t level, const uint64_t* d_ct, uint32_t parts,
+                               const uint8_t* seed, uint32_t variable_time_mask, uint8_t* out, size_t cap,
+                               size_t* len, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!p || !d_ct || !len) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    if (parts < (seed ? 2u : 1u)) return fail(FHEHIP_ERR_INVALID_ARG, "too few polynomials");
+    const auto& ctx = level_ctx(*p->p, level);
+    const u32 written = seed ? parts - 1 : parts;
+    const std::vector<uint8_t> pl = payloads(*ctx, d_ct, written, S(stream));
+    const size_t per = serialization_length(*ctx);
+    pb::Writer w;
+    for (u32 i = 0; i < written; i++) {
+        const bool vt = i < 32 && ((variable_time_mask >> i) & 1);
+        const std::vector<uint8_t> rq = pb::write_rq(pb::REPR_NTT, ctx->N, pl.data() + i * per, per, vt);
+        w.bytes_field(1, rq.data(), rq.size());
+    }
+    if (seed) w.bytes_field(2, seed, 32);
+    w.uint_field(3, level);
+    return copy_message(w.out, out, cap, len);
+    FHEHIP_API_END
+}
+
+int fhehip_ciphertext_from_bytes(const fhehip_parameters* p, const uint8_t* bytes, size_t len, uint64_t* d_out,
+                                 uint32_t max_parts, uint32_t* parts, uint32_t* level, uint8_t* seed_out,
+                                 int* seeded, void* stream) {
+    FHEHIP_API_BEGIN
+    if (!p || !bytes || !parts) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    const pb::Ciphertext m = pb::parse_ciphertext(pb::Span{bytes, len});
+    if (m.c.empty() || (m.c.size() == 1 && m.seed.len == 0))
+        return fail(FHEHIP_ERR_INVALID_ARG, "Not enough polynomials");
+    const auto& ctx = level_ctx(*p->p, m.level);
+    const u32 total = (u32)m.c.size() + (m.seed.len ? 1 : 0);
+    std::vector<uint8_t> seed;
+    if (m.seed.len) seed = seed_of(m.seed);
+    *parts = total;
+    if (level) *level = m.level;
+    if (seeded) *seeded = seed.empty() ? 0 : 1;
+    if (seed_out && !seed.empty()) std::memcpy(seed_out, seed.data(), 32);
+    if (!d_out && !max_parts) return FHEHIP_OK;  // size query
+    if (!d_out || total > max_parts) return fail(FHEHIP_ERR_INVALID_ARG, "more polynomials than max_parts");
+    rq_to_device(*ctx, m.c, pb::REPR_NTT, d_out, S(stream));
+    if (!seed.empty()) {
+        // Poly::<Ntt>::random_from_seed (ciphertext.rs:224-235)
+        const size_t row = (size_t)ctx->L * ctx->N;
+        std::vector<u64> c1(row);
+        Seed32 sd;
+        std::memcpy(sd.data(), seed.data(), 32);
+        random_from_seed(ctx->moduli, ctx->N, sd, c1.data());
+        HIPCHK(hipMemcpyAsync(d_out + m.c.size() * row, c1.data(), row * 8, hipMemcpyHostToDevice, S(stream)));
+        HIPCHK(hipStreamSynchronize(S(stream)));
+    }
+    FHEHIP_API_END
+}
+
+int fhehip_ksk_from_bytes(const fhehip_parameters* p, const uint8_t* bytes, size_t len, fhehip_ksk** out) {
+    FHEHIP_API_BEGIN
+    if (!p || !bytes || !out) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    auto k = ksk_from_message(*p->p, pb::Span{bytes, len});
+    *out = new fhehip_ksk{k};
+    FHEHIP_API_END
+}
+
+int fhehip_ksk_to_bytes(const fhehip_parameters* p, const fhehip_ksk* k, int variable_time, uint8_t* out, size_t cap,
+                        size_t* len) {
+    FHEHIP_API_BEGIN
+    if (!p || !k || !len) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    return copy_message(ksk_message(*p->p, *k->k, variable_time != 0), out, cap, len);
+    FHEHIP_API_END
+}
+
+int fhehip_relinearization_key_from_bytes(const fhehip_parameters* p, const uint8_t* bytes, size_t len,
+                                          fhehip_ksk** out) {
+    FHEHIP_API_BEGIN
+    if (!p || !bytes || !out) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    const pb::WrappedKsk w = pb::parse_wrapped_ksk(pb::Span{bytes, len});
+    if (!w.has_ksk) return fail(FHEHIP_ERR_INVALID_ARG, "Invalid serialization");
+    auto k = ksk_from_message(*p->p, w.ksk);
+    *out = new fhehip_ksk{k};
+    FHEHIP_API_END
+}
+
+int fhehip_relinearization_key_to_bytes(const fhehip_parameters* p, const fhehip_ksk* k, int variable_time,
+                                        uint8_t* out, size_t cap, size_t* len) {
+    FHEHIP_API_BEGIN
+    if (!p || !k || !len) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    return copy_message(wrap_ksk(ksk_message(*p->p, *k->k, variable_time != 0), 0), out, cap, len);
+    FHEHIP_API_END
+}
+
+int fhehip_galois_key_from_bytes(const fhehip_parameters* p, const uint8_t* bytes, size_t len, fhehip_galois** out) {
+    FHEHIP_API_BEGIN
+    if (!p || !bytes || !out) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    const pb::WrappedKsk w = pb::parse_wrapped_ksk(pb::Span{bytes, len});
+    if (!w.has_ksk) return fail(FHEHIP_ERR_INVALID_ARG, "Invalid serialization");
+    auto g = std::make_shared<Galois>(ksk_from_message(*p->p, w.ksk), w.exponent);
+    *out = new fhehip_galois{g};
+    FHEHIP_API_END
+}
+
+int fhehip_galois_key_to_bytes(const fhehip_parameters* p, const fhehip_galois* g, int variable_time, uint8_t* out,
+                               size_t cap, size_t* len) {
+    FHEHIP_API_BEGIN
+    if (!p || !g || !len) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    return copy_message(wrap_ksk(ksk_message(*p->p, *g->g->ksk, variable_time != 0), g->g->exponent), out, cap, len);
+    FHEHIP_API_END
+}
+
+int fhehip_evaluation_key_from_bytes(const fhehip_parameters* p, const uint8_t* bytes, size_t len,
+                                     fhehip_evaluation_key** out, uint32_t* ciphertext_level) {
+    FHEHIP_API_BEGIN
+    if (!p || !bytes || !out) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    const pb::EvaluationKey m = pb::parse_evaluation_key(pb::Span{bytes, len});
+    const auto& ct_ctx = level_ctx(*p->p, m.ciphertext_level);
+    level_ctx(*p->p, m.evaluation_key_level);
+    std::unique_ptr<EvaluationKey> ek(new EvaluationKey);
+    for (const pb::Span& gs : m.gk) {
+        const pb::WrappedKsk w = pb::parse_wrapped_ksk(gs);
+        if (!w.has_ksk) return fail(FHEHIP_ERR_INVALID_ARG, "Invalid serialization");
+        auto k = ksk_from_message(*p->p, w.ksk);
+        if (level_of(*p->p, *k->ct_ctx) != m.ciphertext_level)
+            return fail(FHEHIP_ERR_INVALID_ARG, "Galois key has incorrect ciphertext level");
+        if (level_of(*p->p, *k->key_ctx) != m.evaluation_key_level)
+            return fail(FHEHIP_ERR_INVALID_ARG, "Galois key has incorrect evaluation key level");
+        auto g = std::make_shared<Galois>(k, w.exponent);
+        // one key per exponent, the last one kept (a map in fhe.rs)
+        auto same = std::find_if(ek->keys.begin(), ek->keys.end(),
+                                 [&](const std::shared_ptr<Galois>& x) { return x->exponent == g->exponent; });
+        if (same != ek->keys.end()) *same = g;
+        else ek->keys.push_back(g);
+    }
+    ek->degree = ct_ctx->N;
+    if (!ek->keys.empty()) ek->prepare_expansion();
+    if (ciphertext_level) *ciphertext_level = m.ciphertext_level;
+    *out = new fhehip_evaluation_key{std::move(ek)};
+    FHEHIP_API_END
+}
+
+int fhehip_evaluation_key_to_bytes(const fhehip_parameters* p, const fhehip_evaluation_key* ek, int variable_time,
+                                   uint8_t* out, size_t cap, size_t* len) {
+    FHEHIP_API_BEGIN
+    if (!p || !ek || !len) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    if (ek->ek->keys.empty()) return fail(FHEHIP_ERR_INVALID_ARG, "an evaluation key without Galois keys");
+    pb::Writer w;
+    for (const auto& g : ek->ek->keys) {
+        const std::vector<uint8_t> gk = wrap_ksk(ksk_message(*p->p, *g->ksk, variable_time != 0), g->exponent);
+        w.bytes_field(2, gk.data(), gk.size());
+    }
+    const Ksk& k0 = *ek->ek->keys[0]->ksk;
+    w.uint_field(3, level_of(*p->p, *k0.ct_ctx));
+    w.uint_field(4, level_of(*p->p, *k0.key_ctx));
+    return copy_message(w.out, out, cap, len);
+    FHEHIP_API_END
+}
+
+int fhehip_parameters_from_bytes(const uint8_t* bytes, size_t len, fhehip_parameters** out) {
+    FHEHIP_API_BEGIN
+    if (!bytes || !out) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    const pb::Parameters m = pb::parse_parameters(pb::Span{bytes, len});
+    if (!m.plaintext_kind) return fail(FHEHIP_ERR_INVALID_ARG, "Missing required field: Parameters.plaintext_modulus");
+    u64 t = m.plaintext;
+    if (m.plaintext_kind == 2) {
+        // BigUint::from_bytes_le; only values of at most 62 bits are supported
+        t = 0;
+        for (size_t i = 0; i < m.plaintext_big.len; i++) {
+            const uint8_t b = m.plaintext_big.data[i];
+            if (b && i >= 8) return fail(FHEHIP_ERR_UNSUPPORTED, "plaintext modulus above 2^62");
+            if (i < 8) t |= (u64)b << (8 * i);
+        }
+    }
+    if (m.variance < 1 || m.variance > 16)
+        return fail(FHEHIP_ERR_INVALID_ARG, "The variance should be an integer between 1 and 16");
+    std::unique_ptr<Parameters> par(new Parameters(m.degree, t, m.moduli));
+    par->variance = m.variance;
+    *out = new fhehip_parameters{std::move(par)};
+    FHEHIP_API_END
+}
+
+int fhehip_parameters_to_bytes(const fhehip_parameters* p, uint8_t* out, size_t cap, size_t* len) {
+    FHEHIP_API_BEGIN
+    if (!p || !len) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
+    return copy_message(pb::write_parameters(p->p->N, p->p->moduli, p->p->t, p->p->variance), out, cap, len);
     FHEHIP_API_END
 }
 

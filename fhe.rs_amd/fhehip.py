@@ -21,6 +21,7 @@ ERR_DEVICE = -6
 
 _u64p = C.POINTER(C.c_uint64)
 _u32p = C.POINTER(C.c_uint32)
+_szp = C.POINTER(C.c_size_t)
 _vp = C.c_void_p
 _vpp = C.POINTER(C.c_void_p)
 
@@ -102,6 +103,20 @@ SIGNATURES = {
     "fhehip_submit": (C.c_int, [_vp, C.c_uint32, _vp]),
     "fhehip_generate_moduli": (C.c_int, [_u32p, C.c_uint32, C.c_uint32, _u64p]),
     "fhehip_default_psi": (C.c_int, [C.c_uint32, C.c_uint64, _u64p]),
+    "fhehip_ciphertext_to_bytes": (C.c_int, [_vp, C.c_uint32, _vp, C.c_uint32, C.c_char_p, C.c_uint32, C.c_void_p,
+                                             C.c_size_t, _szp, _vp]),
+    "fhehip_ciphertext_from_bytes": (C.c_int, [_vp, C.c_char_p, C.c_size_t, _vp, C.c_uint32, _u32p, _u32p,
+                                               C.c_void_p, C.POINTER(C.c_int), _vp]),
+    "fhehip_ksk_from_bytes": (C.c_int, [_vp, C.c_char_p, C.c_size_t, _vpp]),
+    "fhehip_ksk_to_bytes": (C.c_int, [_vp, _vp, C.c_int, C.c_void_p, C.c_size_t, _szp]),
+    "fhehip_relinearization_key_from_bytes": (C.c_int, [_vp, C.c_char_p, C.c_size_t, _vpp]),
+    "fhehip_relinearization_key_to_bytes": (C.c_int, [_vp, _vp, C.c_int, C.c_void_p, C.c_size_t, _szp]),
+    "fhehip_galois_key_from_bytes": (C.c_int, [_vp, C.c_char_p, C.c_size_t, _vpp]),
+    "fhehip_galois_key_to_bytes": (C.c_int, [_vp, _vp, C.c_int, C.c_void_p, C.c_size_t, _szp]),
+    "fhehip_evaluation_key_from_bytes": (C.c_int, [_vp, C.c_char_p, C.c_size_t, _vpp, _u32p]),
+    "fhehip_evaluation_key_to_bytes": (C.c_int, [_vp, _vp, C.c_int, C.c_void_p, C.c_size_t, _szp]),
+    "fhehip_parameters_from_bytes": (C.c_int, [C.c_char_p, C.c_size_t, _vpp]),
+    "fhehip_parameters_to_bytes": (C.c_int, [_vp, C.c_void_p, C.c_size_t, _szp]),
     "fhehip_random_from_seed": (C.c_int, [_u64p, C.c_uint32, C.c_uint32, C.c_char_p, _u64p]),
     "fhehip_generate_c1": (C.c_int, [_u64p, C.c_uint32, C.c_uint32, C.c_char_p, C.c_uint32, _u64p]),
     "fhehip_server_create": (C.c_int, [_vp, _vpp]),
@@ -238,6 +253,15 @@ def to_device(np_u64, device="cuda"):
 def to_host(t):
     import numpy as np
     return t.detach().cpu().numpy().view(np.uint64)
+
+
+def _message(call):
+    """Runs call(out, cap, len_ptr) once for the size, then into a buffer of that size."""
+    n = C.c_size_t(0)
+    check(call(None, 0, C.byref(n)))
+    buf = (C.c_uint8 * max(n.value, 1))()
+    check(call(buf, n.value, C.byref(n)))
+    return bytes(buf[:n.value])
 
 
 def empty(n, device="cuda"):
@@ -411,6 +435,85 @@ class Parameters:
         h = C.c_void_p()
         check(lib().fhehip_parameters_plaintext_context(self.h, C.byref(h)))
         return self._ctx(h)
+
+    @classmethod
+    def from_bytes(cls, data: bytes):
+        """BfvParameters::try_deserialize (fhehip_parameters_from_bytes)."""
+        h = C.c_void_p()
+        check(lib().fhehip_parameters_from_bytes(data, len(data), C.byref(h)))
+        par = cls.__new__(cls)
+        par.h = h
+        mx, n, t = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        check(lib().fhehip_parameters_info(h, C.byref(mx), C.byref(n), C.byref(t)))
+        par.max_level, par.degree, par.plaintext = mx.value, n.value, t.value
+        par.moduli = par.mul_basis(0)[:par.max_level + 1]  # the level-0 moduli lead the basis
+        return par
+
+    def to_bytes(self):
+        return _message(lambda out, cap, ln: lib().fhehip_parameters_to_bytes(self.h, out, cap, ln))
+
+    def ciphertext_to_bytes(self, level, d_ct, parts, seed=None, variable_time_mask=0, stream=None):
+        """Ciphertext::to_bytes of d_ct [parts][L][N] at `level`."""
+        if seed is not None and len(seed) != 32:
+            raise ValueError("a seed has 32 bytes")
+        return _message(lambda out, cap, ln: lib().fhehip_ciphertext_to_bytes(
+            self.h, level, dptr(d_ct), parts, seed, variable_time_mask, out, cap, ln, stream_ptr(stream)))
+
+    def ciphertext_from_bytes(self, data: bytes, stream=None):
+        """Ciphertext::from_bytes -> (device [parts][L][N], parts, level, seed or None)."""
+        parts, level, seeded = C.c_uint32(), C.c_uint32(), C.c_int()
+        seed = (C.c_uint8 * 32)()
+        check(lib().fhehip_ciphertext_from_bytes(self.h, data, len(data), None, 0, C.byref(parts), C.byref(level),
+                                                 seed, C.byref(seeded), None))
+        L = len(self.moduli) - level.value
+        out = empty(parts.value * L * self.degree)
+        check(lib().fhehip_ciphertext_from_bytes(self.h, data, len(data), dptr(out), parts.value, C.byref(parts),
+                                                 C.byref(level), None, None, stream_ptr(stream)))
+        return out, parts.value, level.value, (bytes(seed) if seeded.value else None)
+
+    def _ksk(self, fn, data):
+        h = C.c_void_p()
+        check(getattr(lib(), fn)(self.h, data, len(data), C.byref(h)))
+        k = KeySwitchingKey.__new__(KeySwitchingKey)
+        k.h = h
+        k.ct_ctx = k.key_ctx = None
+        return k
+
+    def ksk_from_bytes(self, data: bytes):
+        return self._ksk("fhehip_ksk_from_bytes", data)
+
+    def relinearization_key_from_bytes(self, data: bytes):
+        return self._ksk("fhehip_relinearization_key_from_bytes", data)
+
+    def ksk_to_bytes(self, ksk, variable_time=True):
+        return _message(lambda out, cap, ln: lib().fhehip_ksk_to_bytes(self.h, ksk.h, int(variable_time), out, cap, ln))
+
+    def relinearization_key_to_bytes(self, ksk, variable_time=True):
+        return _message(lambda out, cap, ln: lib().fhehip_relinearization_key_to_bytes(
+            self.h, ksk.h, int(variable_time), out, cap, ln))
+
+    def galois_key_from_bytes(self, data: bytes):
+        h = C.c_void_p()
+        check(lib().fhehip_galois_key_from_bytes(self.h, data, len(data), C.byref(h)))
+        g = GaloisKey.__new__(GaloisKey)
+        g.h, g.ksk, g.exponent = h, None, None
+        return g
+
+    def galois_key_to_bytes(self, g, variable_time=True):
+        return _message(lambda out, cap, ln: lib().fhehip_galois_key_to_bytes(
+            self.h, g.h, int(variable_time), out, cap, ln))
+
+    def evaluation_key_from_bytes(self, data: bytes):
+        h = C.c_void_p()
+        lvl = C.c_uint32()
+        check(lib().fhehip_evaluation_key_from_bytes(self.h, data, len(data), C.byref(h), C.byref(lvl)))
+        ek = EvaluationKey.__new__(EvaluationKey)
+        ek.h, ek.keys = h, []
+        return ek, lvl.value
+
+    def evaluation_key_to_bytes(self, ek, variable_time=True):
+        return _message(lambda out, cap, ln: lib().fhehip_evaluation_key_to_bytes(
+            self.h, ek.h, int(variable_time), out, cap, ln))
 
     def mul_basis(self, level):
         n = C.c_uint32(128)

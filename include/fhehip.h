@@ -306,6 +306,65 @@ int fhehip_serialize(const fhehip_context* ctx, const uint64_t* d_polys, uint32_
 int fhehip_deserialize(const fhehip_context* ctx, const uint8_t* d_bytes, uint32_t num, int ntt, uint64_t* d_polys,
                        void* stream);
 
+/* ---- Protobuf messages (host bytes) ---------------------------------------
+ * fhe.rs's serialized forms (prost encodings of fhe/src/proto/bfv.proto and
+ * math/src/proto/rq.proto) for the objects of a parameters handle. Each Rq
+ * polynomial carries its coefficients in power basis, bit-packed as
+ * fhehip_serialize does. Writers fill `out` (capacity `cap`) and set *len;
+ * out == NULL with cap == 0 only reports the size. Messages are read with
+ * fields in any order and unknown fields skipped. These calls wait for their
+ * device work; `stream` (may be NULL) orders it after the caller's.
+ *
+ * Ciphertext::to_bytes (fhe/src/bfv/ciphertext.rs:167-196): d_ct [parts][L][N]
+ * (NTT) at `level`. With a 32-byte `seed` the last polynomial is the one drawn
+ * from it (random_from_seed) and is replaced by the seed. Bit i of
+ * variable_time_mask is polynomial i's allow_variable_time flag. */
+int fhehip_ciphertext_to_bytes(const fhehip_parameters* p, uint32_t level, const uint64_t* d_ct, uint32_t parts,
+                               const uint8_t* seed, uint32_t variable_time_mask, uint8_t* out, size_t cap,
+                               size_t* len, void* stream);
+/* Ciphertext::from_bytes (ciphertext.rs:198-242): *parts (polynomials,
+ * including the one a seed stands for), *level and, for a seeded message,
+ * *seeded = 1 and the seed (seed_out, 32 bytes, may be NULL); d_out receives
+ * [parts][L][N] in the NTT domain (at most max_parts). d_out == NULL with
+ * max_parts == 0 only parses. Errors as fhe.rs: fewer than two polynomials,
+ * a level above max_level, a seed not of 32 bytes, a representation other
+ * than NTT, a degree or payload length not of the level's context. */
+int fhehip_ciphertext_from_bytes(const fhehip_parameters* p, const uint8_t* bytes, size_t len, uint64_t* d_out,
+                                 uint32_t max_parts, uint32_t* parts, uint32_t* level, uint8_t* seed_out,
+                                 int* seeded, void* stream);
+/* KeySwitchingKey::try_convert_from (fhe/src/bfv/keys/key_switching_key.rs
+ * :354-426): the key of a KeySwitchingKey message; c1 is drawn from the seed
+ * (generate_c1) when the message has one, and the key keeps the seed. */
+int fhehip_ksk_from_bytes(const fhehip_parameters* p, const uint8_t* bytes, size_t len, fhehip_ksk** out);
+/* From<&KeySwitchingKey> (key_switching_key.rs:332-352): c0 (and c1, or the
+ * seed the key came with) as NttShoup Rq messages whose allow_variable_time
+ * flag is `variable_time`; levels from the parameters. */
+int fhehip_ksk_to_bytes(const fhehip_parameters* p, const fhehip_ksk* k, int variable_time, uint8_t* out, size_t cap,
+                        size_t* len);
+/* RelinearizationKey { ksk } (fhe/src/bfv/keys/relinearization_key.rs:109-127). */
+int fhehip_relinearization_key_from_bytes(const fhehip_parameters* p, const uint8_t* bytes, size_t len,
+                                          fhehip_ksk** out);
+int fhehip_relinearization_key_to_bytes(const fhehip_parameters* p, const fhehip_ksk* k, int variable_time,
+                                        uint8_t* out, size_t cap, size_t* len);
+/* GaloisKey { ksk, exponent } (fhe/src/bfv/keys/galois_key.rs:127-150). */
+int fhehip_galois_key_from_bytes(const fhehip_parameters* p, const uint8_t* bytes, size_t len, fhehip_galois** out);
+int fhehip_galois_key_to_bytes(const fhehip_parameters* p, const fhehip_galois* g, int variable_time, uint8_t* out,
+                               size_t cap, size_t* len);
+/* EvaluationKey { gk, ciphertext_level, evaluation_key_level }
+ * (fhe/src/bfv/keys/evaluation_key.rs:382-431): every Galois key must have the
+ * message's two levels; one key per exponent (the last one). The keys are
+ * written in the handle's order (fhe.rs writes its map's order). */
+int fhehip_evaluation_key_from_bytes(const fhehip_parameters* p, const uint8_t* bytes, size_t len,
+                                     fhehip_evaluation_key** out, uint32_t* ciphertext_level);
+int fhehip_evaluation_key_to_bytes(const fhehip_parameters* p, const fhehip_evaluation_key* ek, int variable_time,
+                                   uint8_t* out, size_t cap, size_t* len);
+/* BfvParameters::try_deserialize / to_bytes (fhe/src/bfv/parameters.rs
+ * :644-691): degree, moduli, variance (1 .. 16; 10 for handles made by
+ * fhehip_parameters_create) and the plaintext modulus (a plaintext_big of
+ * more than 62 bits is FHEHIP_ERR_UNSUPPORTED). */
+int fhehip_parameters_from_bytes(const uint8_t* bytes, size_t len, fhehip_parameters** out);
+int fhehip_parameters_to_bytes(const fhehip_parameters* p, uint8_t* out, size_t cap, size_t* len);
+
 /* ---- Encoding, encryption, decryption ------------------------------------------
  * The steps on either side of the evaluation path, for the BfvParameters
  * level whose ciphertext context is `ctx`: `plain_ctx` is the parameters'
