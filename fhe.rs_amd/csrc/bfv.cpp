@@ -10,10 +10,31 @@ namespace fhehip {
 
 // ---------------------------------------------------------------------------
 // Scaler constants blob (layout documented in rns_device.h).
-std::vector<u64> scaler_blob(const ScalerConsts& c, const std::vector<ModInfo>& to_mods, u32 start) {
+//
+// SC_F_SUM128 is set when, for inputs below 2 q_j (reduced residues, or lazy
+// ones below 2 q_j), every output's exact sum
+//   v0 (q - gamma) + v1 [2^64 (q - gamma)]_q (+ w0 c0 + w1 c1) + sum_j r_j omega_ij
+// is below 2^128. With r_j <= 2 q_j - 1 and theta_garner_j <= 2^shift,
+// v <= sum_j r_j + 1; |theta_omega_j|, |theta_gamma| < 2^127 give
+// w <= sum_j r_j + v + 1; c0, c1 < q.
+namespace {
+struct Sat128 {
+    u128 v = 0;
+    bool over = false;
+    void add(u128 x) {
+        const u128 s = v + x;
+        if (s < v) over = true;
+        v = s;
+    }
+};
+}  // namespace
+
+std::vector<u64> scaler_blob(const ScalerConsts& c, const std::vector<ModInfo>& from_mods,
+                             const std::vector<ModInfo>& to_mods, u32 start) {
     const u32 nf = (u32)c.theta_garner_lo.size();
     const u32 nt = (u32)c.gamma.size();
     const u32 count = nt - start;
+    if (nf > 32) throw Error(-2, "more than 32 source moduli in one scaling");
     std::vector<u64> b(sc_out_base(nf) + (size_t)count * sc_out_stride(nf), 0);
     b[SC_NF] = nf;
     b[SC_NT] = nt;
@@ -26,13 +47,26 @@ std::vector<u64> scaler_blob(const ScalerConsts& c, const std::vector<ModInfo>& 
     b[SC_TG_SIGN] = c.theta_gamma_sign ? 1 : 0;
     u64* tg = b.data() + SC_HEADER;
     u64* tom = tg + 2 * nf;
+    u64 omega_mask = 0;
     for (u32 j = 0; j < nf; j++) {
         tg[2 * j] = c.theta_garner_lo[j];
         tg[2 * j + 1] = c.theta_garner_hi[j];
         tom[3 * j] = c.theta_omega_lo[j];
         tom[3 * j + 1] = c.theta_omega_hi[j];
         tom[3 * j + 2] = c.theta_omega_sign[j];
+        if (c.theta_omega_lo[j] | c.theta_omega_hi[j]) omega_mask |= (u64)1 << j;
     }
+    b[SC_OMEGA_MASK] = omega_mask;
+    // bounds of the inputs and of v, w (see above)
+    Sat128 rsum;
+    std::vector<u64> rmax(nf);
+    for (u32 j = 0; j < nf; j++) {
+        rmax[j] = 2 * from_mods[j].p - 1;
+        rsum.add(rmax[j]);
+    }
+    const u128 vmax = rsum.v + 1, wmax = rsum.v + vmax + 1;
+    const u64 v1max = (u64)(vmax >> 64), w1max = (u64)(wmax >> 64);
+    bool sum128 = !rsum.over && vmax > rsum.v && wmax > vmax;
     for (u32 i = 0; i < count; i++) {
         u64* o = b.data() + sc_out_base(nf) + (size_t)i * sc_out_stride(nf);
         const u32 ti = start + i;
@@ -45,8 +79,22 @@ std::vector<u64> scaler_blob(const ScalerConsts& c, const std::vector<ModInfo>& 
         o[4] = (u64)(((u128)gneg << 64) % q);
         o[5] = (u64)(((u128)1 << 64) % q);
         o[6] = (u64)(((u128)o[5] << 64) % q);
-        for (u32 j = 0; j < nf; j++) o[7 + j] = c.omega[(size_t)ti * nf + j];
+        o[7] = (u64)(((u128)o[5] << 64) / q);  // shoup([2^64]_q)
+        o[8] = (u64)(((u128)1 << 64) / q);     // shoup(1)
+        Sat128 bound;
+        bound.add((u128)~0ull * o[3]);
+        bound.add((u128)v1max * o[4]);
+        if (!c.is_one) {
+            bound.add((u128)~0ull * (q - 1));
+            bound.add((u128)w1max * (q - 1));
+        }
+        for (u32 j = 0; j < nf; j++) {
+            o[9 + j] = c.omega[(size_t)ti * nf + j];
+            bound.add((u128)rmax[j] * o[9 + j]);
+        }
+        if (bound.over) sum128 = false;
     }
+    b[SC_FLAGS] = (sum128 ? SC_F_SUM128 : 0u) | ((c.theta_gamma_lo | c.theta_gamma_hi) ? 0u : SC_F_TG_ZERO);
     return b;
 }
 
@@ -110,7 +158,7 @@ PolyScaler::PolyScaler(std::shared_ptr<DeviceContext> f, std::shared_ptr<DeviceC
         while (common < from->L && common < to->L && from->moduli[common] == to->moduli[common]) ++common;
     }
     consts.build(from->rns, to->rns, num, den);
-    std::vector<u64> blob = scaler_blob(consts, to->h_mod, common);
+    std::vector<u64> blob = scaler_blob(consts, from->h_mod, to->h_mod, common);
     d_consts.reset(blob.size() * sizeof(u64));
     HIPCHK(hipMemcpy(d_consts.ptr, blob.data(), d_consts.bytes, hipMemcpyHostToDevice));
 }

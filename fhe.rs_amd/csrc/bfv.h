@@ -77,7 +77,8 @@ StreamScratch& scratch(Scratch k);
 // Frees every pool's buffer of stream `s`.
 void release_scratch(hipStream_t s);
 
-std::vector<u64> scaler_blob(const ScalerConsts& c, const std::vector<ModInfo>& to_mods, u32 start);
+std::vector<u64> scaler_blob(const ScalerConsts& c, const std::vector<ModInfo>& from_mods,
+                             const std::vector<ModInfo>& to_mods, u32 start);
 
 // rq::scaler::Scaler (math/src/rq/scaler.rs:18-126).
 struct PolyScaler {

@@ -14,15 +14,23 @@
 namespace fhehip {
 
 // Flat constant block (u64 words) written by the host (bfv.cpp):
-//   header[16]: nf, nt, start, count, shift, is_one, tg_lo, tg_hi, tg_sign
+//   header[16]: nf, nt, start, count, shift, is_one, tg_lo, tg_hi, tg_sign,
+//               flags, omega_mask
 //   theta_garner[nf][2], theta_omega[nf][3] (lo, hi, sign)
 //   per output i < count: p, barrett_lo, barrett_hi, q - gamma,
-//                         [2^64 (q - gamma)]_q, [2^64]_q, [2^128]_q, omega[nf]
+//                         [2^64 (q - gamma)]_q, [2^64]_q, [2^128]_q,
+//                         shoup([2^64]_q), shoup(1) = floor(2^64 / q), omega[nf]
+// flags: SC_F_SUM128 when the host has shown that every output's exact sum
+// (below) stays under 2^128 for inputs below 2 q_j; SC_F_TG_ZERO when
+// theta_gamma is zero. omega_mask: bit j set when theta_omega_j is not zero
+// (t/Q scaling has zero theta_omega for every modulus of the extension part).
 enum : int {
-    SC_NF = 0, SC_NT, SC_START, SC_COUNT, SC_SHIFT, SC_IS_ONE, SC_TG_LO, SC_TG_HI, SC_TG_SIGN,
+    SC_NF = 0, SC_NT, SC_START, SC_COUNT, SC_SHIFT, SC_IS_ONE, SC_TG_LO, SC_TG_HI, SC_TG_SIGN, SC_FLAGS,
+    SC_OMEGA_MASK,
     SC_HEADER = 16
 };
-__host__ __device__ inline int sc_out_stride(int nf) { return 7 + nf; }
+enum : unsigned { SC_F_SUM128 = 1, SC_F_TG_ZERO = 2 };
+__host__ __device__ inline int sc_out_stride(int nf) { return 9 + nf; }
 __host__ __device__ inline int sc_out_base(int nf) { return SC_HEADER + 2 * nf + 3 * nf; }
 
 // Scaler arithmetic with explicit 32-bit add-with-carry chains. In these
@@ -175,6 +183,122 @@ __device__ __forceinline__ void acc3_value(const Acc3& s, u64& lo, u64& hi, u64&
     top = (u64)s.kc + c2 + c3;
 }
 
+// Column accumulator of a sum known to stay below 2^128: the same columns
+// as Acc3, but the top column c cannot carry out (its carry count would
+// weigh 2^128), so one product costs four v_mad_u64_u32 and three
+// v_addc_co_u32. Every carry is read at least three instructions after the
+// VALU wrote it.
+struct Acc2 {
+    u64 a, b, c;
+    u32 ka, kb;
+};
+
+#define FHEHIP_ACC2_ASM                                    \
+    "v_mad_u64_u32 %[a], %[k0], %[r0], %[w0], %[a]\n\t"    \
+    "v_mad_u64_u32 %[b], %[k1], %[r0], %[w1], %[b]\n\t"    \
+    "v_mad_u64_u32 %[b], %[k2], %[r1], %[w0], %[b]\n\t"    \
+    "v_addc_co_u32 %[ka], %[d], %[ka], 0, %[k0]\n\t"       \
+    "v_mad_u64_u32 %[c], %[d], %[r1], %[w1], %[c]\n\t"     \
+    "v_addc_co_u32 %[kb], %[d], %[kb], 0, %[k1]\n\t"       \
+    "v_addc_co_u32 %[kb], %[d], %[kb], 0, %[k2]"
+
+// s += r * w. SW: w is uniform across the wave (passed in SGPRs).
+template <bool SW>
+__device__ __forceinline__ void acc2_mul(Acc2& s, u64 r, u64 w) {
+    u64 k0, k1, k2, d;
+    if constexpr (SW)
+        asm(FHEHIP_ACC2_ASM
+            : [a] "+v"(s.a), [b] "+v"(s.b), [c] "+v"(s.c), [ka] "+v"(s.ka), [kb] "+v"(s.kb), [k0] "=&s"(k0),
+              [k1] "=&s"(k1), [k2] "=&s"(k2), [d] "=&s"(d)
+            : [r0] "v"((u32)r), [r1] "v"((u32)(r >> 32)), [w0] "s"((u32)w), [w1] "s"((u32)(w >> 32)));
+    else
+        asm(FHEHIP_ACC2_ASM
+            : [a] "+v"(s.a), [b] "+v"(s.b), [c] "+v"(s.c), [ka] "+v"(s.ka), [kb] "+v"(s.kb), [k0] "=&s"(k0),
+              [k1] "=&s"(k1), [k2] "=&s"(k2), [d] "=&s"(d)
+            : [r0] "v"((u32)r), [r1] "v"((u32)(r >> 32)), [w0] "v"((u32)w), [w1] "v"((u32)(w >> 32)));
+}
+
+// s = r * w: the first two products add to zero and cannot carry; the
+// carry of the second middle product is read three instructions later.
+#define FHEHIP_ACC2_FIRST_ASM                              \
+    "v_mad_u64_u32 %[b], %[k1], %[r0], %[w1], 0\n\t"      \
+    "v_mad_u64_u32 %[a], %[k1], %[r0], %[w0], 0\n\t"      \
+    "v_mad_u64_u32 %[b], %[k0], %[r1], %[w0], %[b]\n\t"   \
+    "v_mov_b32 %[ka], 0\n\t"                              \
+    "v_mad_u64_u32 %[c], %[k1], %[r1], %[w1], 0\n\t"      \
+    "v_addc_co_u32 %[kb], %[k1], 0, 0, %[k0]"
+
+template <bool SW>
+__device__ __forceinline__ void acc2_first(Acc2& s, u64 r, u64 w) {
+    u64 k0, k1;
+    if constexpr (SW)
+        asm(FHEHIP_ACC2_FIRST_ASM
+            : [a] "=&v"(s.a), [b] "=&v"(s.b), [c] "=&v"(s.c), [ka] "=&v"(s.ka), [kb] "=&v"(s.kb), [k0] "=&s"(k0),
+              [k1] "=&s"(k1)
+            : [r0] "v"((u32)r), [r1] "v"((u32)(r >> 32)), [w0] "s"((u32)w), [w1] "s"((u32)(w >> 32)));
+    else
+        asm(FHEHIP_ACC2_FIRST_ASM
+            : [a] "=&v"(s.a), [b] "=&v"(s.b), [c] "=&v"(s.c), [ka] "=&v"(s.ka), [kb] "=&v"(s.kb), [k0] "=&s"(k0),
+              [k1] "=&s"(k1)
+            : [r0] "v"((u32)r), [r1] "v"((u32)(r >> 32)), [w0] "v"((u32)w), [w1] "v"((u32)(w >> 32)));
+}
+
+// The 128-bit value of an Acc2 accumulator.
+__device__ __forceinline__ void acc2_value(const Acc2& s, u64& lo, u64& hi) {
+    unsigned long long c1, c2;
+    lo = __builtin_addcll(s.a, s.b << 32, 0, &c1);
+    hi = __builtin_addcll(s.c, (s.b >> 32) | ((u64)s.kb << 32), c1, &c2);
+    hi += (u64)s.ka;
+}
+
+// a - floor(a * ws / 2^64) * p (mod 2^64) with ws = floor(2^64 / p): a mod p
+// in [0, 2p) for any a < 2^64 (lazy_mul_shoup by one, math/src/zq/mod.rs
+// :224-234, without the product by the multiplier). Temporaries and hazard
+// spacing as FHEHIP_SHOUP_ASM.
+#define FHEHIP_SHOUP1_ASM                                   \
+    "v_mov_b32 v1, 0\n\t"                                   \
+    "v_mul_hi_u32 v0, %[a0], %[s0]\n\t"                     \
+    "v_mad_u64_u32 v[2:3], %[k], %[a0], %[s1], v[0:1]\n\t"  \
+    "v_mad_u64_u32 v[2:3], %[k], %[a1], %[s0], v[2:3]\n\t"  \
+    "v_mov_b32 v0, v3\n\t"                                  \
+    "s_nop 1\n\t"                                           \
+    "v_addc_co_u32 v1, %[k], 0, 0, %[k]\n\t"                \
+    "v_mad_u64_u32 v[0:1], %[k], %[a1], %[s1], v[0:1]\n\t"  \
+    "v_mad_u64_u32 v[2:3], %[k], v0, %[n0], %[a]\n\t"      \
+    "v_mul_lo_u32 v4, v0, %[n1]\n\t"                        \
+    "v_mul_lo_u32 v5, v1, %[n0]\n\t"                        \
+    "v_add3_u32 v3, v3, v4, v5"
+
+template <bool SW>
+__device__ __forceinline__ u64 lazy_reduce_shoup1(u64 a, u64 ws, u64 p) {
+    const u64 n = 0 - p;
+    u64 r, k;
+    if constexpr (SW)
+        asm(FHEHIP_SHOUP1_ASM
+            : "=&{v[2:3]}"(r), [k] "=&s"(k)
+            : [a] "v"(a), [a0] "v"((u32)a), [a1] "v"((u32)(a >> 32)), [s0] "s"((u32)ws), [s1] "s"((u32)(ws >> 32)),
+              [n0] "s"((u32)n), [n1] "s"((u32)(n >> 32))
+            : "v0", "v1", "v4", "v5");
+    else
+        asm(FHEHIP_SHOUP1_ASM
+            : "=&{v[2:3]}"(r), [k] "=&s"(k)
+            : [a] "v"(a), [a0] "v"((u32)a), [a1] "v"((u32)(a >> 32)), [s0] "v"((u32)ws), [s1] "v"((u32)(ws >> 32)),
+              [n0] "v"((u32)n), [n1] "v"((u32)(n >> 32))
+            : "v0", "v1", "v4", "v5");
+    return r;
+}
+
+// Exact (lo + 2^64 hi) mod p for p < 2^62: hi [2^64]_p by a lazy Shoup
+// product (r64, r64s = shoup([2^64]_p)) and lo by a lazy Shoup reduction
+// (w1s = floor(2^64 / p)), each in [0, 2p), then two conditional
+// subtractions. SW: the constants are uniform across the wave.
+template <bool SW>
+__device__ __forceinline__ u64 reduce128_shoup(u64 lo, u64 hi, u64 r64, u64 r64s, u64 w1s, u64 p) {
+    const u64 x = (SW ? lazy_mul_shoup_u(hi, r64, r64s, p) : lazy_mul_shoup(hi, r64, r64s, p)) +
+                  lazy_reduce_shoup1<SW>(lo, w1s, p);
+    return reduce1(reduce1(x, 2 * p), p);
+}
+
 struct U256 {
     u64 w[4];
 };
@@ -234,11 +358,27 @@ struct ScColumn {
     bool w_sign;
 };
 
+// Header fields of the constant block as wave-uniform values.
+template <bool SW>
+struct ScHeader {
+    int count;
+    unsigned shift;
+    bool is_one;
+    unsigned flags;
+    unsigned omega_mask;
+    __device__ __forceinline__ explicit ScHeader(const u64* __restrict__ sc)
+        : count((int)sc_uniform<SW>(sc[SC_COUNT])),
+          shift(sc_uniform<SW>(sc[SC_SHIFT])),
+          is_one(sc_uniform<SW>(sc[SC_IS_ONE]) != 0),
+          flags(sc_uniform<SW>(sc[SC_FLAGS])),
+          omega_mask(sc_uniform<SW>(sc[SC_OMEGA_MASK])) {}
+};
+
 // v and w of one column with NF input residues; `rests(j)` yields residue j.
 // SW: `sc` is global memory and the constants enter the products from SGPRs;
 // otherwise `sc` is a copy in LDS and they enter from VGPRs.
 template <int NF, bool SW, class Rests>
-__device__ __forceinline__ ScColumn sc_column(const u64* __restrict__ sc, Rests rests, unsigned shift, bool is_one) {
+__device__ __forceinline__ ScColumn sc_column(const u64* __restrict__ sc, Rests rests, const ScHeader<SW>& h) {
     constexpr int nf = NF;
     const u64* tg = sc + SC_HEADER;
     const u64* tom = tg + 2 * nf;
@@ -261,34 +401,33 @@ __device__ __forceinline__ ScColumn sc_column(const u64* __restrict__ sc, Rests 
         acc3_value(sb, b0, b1, b2);
         u256_combine(sum, a0, a1, a2, b0, b1, b2);
     }
-    u128 vv = u256_shr_lo128(sum, shift - 1);
+    u128 vv = u256_shr_lo128(sum, h.shift - 1);
     const u128 v = (vv >> 1) + (vv & 1);
 
     bool w_sign = false;
     u128 w = 0;
-    if (!is_one) {
+    if (!h.is_one) {
         // sum_j +-r_j theta_omega_j with the signs of the constants, without
         // a branch on the sign: a negative theta is taken as its 128-bit
         // complement 2^128 - theta, so that
         //   sum = sum_j r_j theta'_j - 2^128 sum_{negative j} r_j
         // exactly (the signs and constants are uniform, so theta' and the
-        // mask are scalar values).
+        // mask are scalar values). Zero thetas (omega_mask) add nothing and
+        // are skipped by uniform branches.
         Acc3 wa, wb;
+        acc3_init(wa, 0);
+        acc3_init(wb, 0);
         u64 rn0 = 0, rn1 = 0;
 #pragma unroll
         for (int j = 0; j < nf; j++) {
+            if (!((h.omega_mask >> j) & 1)) continue;
             const u64 lo = tom[3 * j], hi = tom[3 * j + 1];
-            const bool neg = tom[3 * j + 2] != 0 && (lo | hi) != 0;
+            const bool neg = tom[3 * j + 2] != 0;
             const u64 clo = neg ? 0 - lo : lo;
             const u64 chi = neg ? ~hi + (lo == 0 ? 1 : 0) : hi;
-            if (j == 0) {
-                acc3_first<SW>(wa, rests(j), clo);
-                acc3_first<SW>(wb, rests(j), chi);
-            } else {
-                acc3_mul<SW>(wa, rests(j), clo);
-                acc3_mul<SW>(wb, rests(j), chi);
-            }
-            add128(rn0, rn1, rests(j) & (neg ? ~0ull : 0ull), 0);
+            acc3_mul<SW>(wa, rests(j), clo);
+            acc3_mul<SW>(wb, rests(j), chi);
+            if (sc_uniform<SW>(tom[3 * j + 2])) add128(rn0, rn1, rests(j), 0);
         }
         u64 p0, p1, p2, q0, q1, q2;
         acc3_value(wa, p0, p1, p2);
@@ -299,24 +438,26 @@ __device__ __forceinline__ ScColumn sc_column(const u64* __restrict__ sc, Rests 
         corr.w[2] = rn0;
         corr.w[3] = rn1;
         u256_sub(so, corr);
-        // v * theta_gamma (128 x 128 -> 256)
-        const u64 tgl = sc[SC_TG_LO], tgh = sc[SC_TG_HI];
-        U256 vt;
-        {
-            u64 a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0, l, h;
-            sc_mul_wide((u64)v, tgl, l, h);
-            sc_add192(a0, a1, a2, l, h);
-            sc_mul_wide((u64)(v >> 64), tgl, l, h);
-            sc_add192(b0, b1, b2, l, h);
-            sc_mul_wide((u64)v, tgh, l, h);
-            sc_add192(b0, b1, b2, l, h);
-            u256_combine(vt, a0, a1, a2, b0, b1, b2);
-            sc_mul_wide((u64)(v >> 64), tgh, l, h);  // times 2^128
-            vt.w[2] += l;
-            vt.w[3] += h + (vt.w[2] < l ? 1 : 0);
+        if (!(h.flags & SC_F_TG_ZERO)) {
+            // v * theta_gamma (128 x 128 -> 256)
+            const u64 tgl = sc[SC_TG_LO], tgh = sc[SC_TG_HI];
+            U256 vt;
+            {
+                u64 a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0, l, h2;
+                sc_mul_wide((u64)v, tgl, l, h2);
+                sc_add192(a0, a1, a2, l, h2);
+                sc_mul_wide((u64)(v >> 64), tgl, l, h2);
+                sc_add192(b0, b1, b2, l, h2);
+                sc_mul_wide((u64)v, tgh, l, h2);
+                sc_add192(b0, b1, b2, l, h2);
+                u256_combine(vt, a0, a1, a2, b0, b1, b2);
+                sc_mul_wide((u64)(v >> 64), tgh, l, h2);  // times 2^128
+                vt.w[2] += l;
+                vt.w[3] += h2 + (vt.w[2] < l ? 1 : 0);
+            }
+            if (sc_uniform<SW>(sc[SC_TG_SIGN])) u256_add(so, vt);
+            else u256_sub(so, vt);
         }
-        if (sc_uniform<SW>(sc[SC_TG_SIGN])) u256_add(so, vt);
-        else u256_sub(so, vt);
         w_sign = (so.w[3] >> 63) != 0;
         if (w_sign) {
             U256 nso;
@@ -335,30 +476,49 @@ __device__ __forceinline__ ScColumn sc_column(const u64* __restrict__ sc, Rests 
 // Each output residue is the exact value of
 //   2 q - [v]_q gamma (+/- [w]_q) + sum_j r_j omega_ij   mod q,
 // which the reference reduces term by term. Here every term is an exact
-// product added into a 192-bit accumulator (v and w enter through their
+// product added into one column accumulator (v and w enter through their
 // 64-bit halves: v = v1 2^64 + v0 contributes v0 (q - gamma) +
-// v1 [2^64 (q - gamma)]_q), and the sum is reduced once.
+// v1 [2^64 (q - gamma)]_q), and the sum is reduced once:
+// - SC_F_SUM128 (the sum stays below 2^128): Acc2 and reduce128_shoup;
+// - otherwise an Acc3 sum lo + 2^64 hi + 2^128 top, folded to
+//   lo + hi [2^64]_q + top [2^128]_q (< 2^127) and reduced by Barrett.
+// +w, or -w = w (q - 1) mod q, enters without a branch on the column's sign
+// as w0 c0 + w1 c1 with (c0, c1) = (1, [2^64]_q) or (q - 1, q - [2^64]_q).
 template <int NF, bool SW, class Rests>
-__device__ __forceinline__ u64 sc_output(const u64* __restrict__ o, Rests rests, const ScColumn& c, bool is_one) {
+__device__ __forceinline__ u64 sc_output(const u64* __restrict__ o, Rests rests, const ScColumn& c,
+                                         const ScHeader<SW>& h) {
     constexpr int nf = NF;
-    const u64 q = o[0], bl = o[1], bh = o[2];
-    // o[3] = q - gamma, o[4] = [2^64 (q - gamma)]_q, o[5] = [2^64]_q, o[6] = [2^128]_q
+    const u64 q = o[0];
+    // o[3] = q - gamma, o[4] = [2^64 (q - gamma)]_q, o[5] = [2^64]_q, o[6] = [2^128]_q,
+    // o[7] = shoup([2^64]_q), o[8] = floor(2^64 / q)
+    const u64* om = o + 9;
+    if (h.flags & SC_F_SUM128) {
+        Acc2 acc;
+        acc2_first<SW>(acc, c.v0, o[3]);
+        acc2_mul<SW>(acc, c.v1, o[4]);
+        if (!h.is_one) {
+            const u64 c0 = c.w_sign ? q - 1 : 1;
+            const u64 c1 = c.w_sign ? q - o[5] : o[5];
+            acc2_mul<false>(acc, c.w0, c0);
+            acc2_mul<false>(acc, c.w1, c1);
+        }
+#pragma unroll
+        for (int j = 0; j < nf; j++) acc2_mul<SW>(acc, rests(j), om[j]);
+        u64 lo, hi;
+        acc2_value(acc, lo, hi);
+        return reduce128_shoup<SW>(lo, hi, o[5], o[7], o[8], q);
+    }
     Acc3 acc;
     acc3_first<SW>(acc, c.v0, o[3]);
     acc3_mul<SW>(acc, c.v1, o[4]);
-    if (!is_one) {
-        // +w, or -w = w (q - 1) mod q, without a branch on the column's
-        // sign: w0 c0 + w1 c1 with (c0, c1) = (1, [2^64]_q) or
-        // (q - 1, q - [2^64]_q).
+    if (!h.is_one) {
         const u64 c0 = c.w_sign ? q - 1 : 1;
         const u64 c1 = c.w_sign ? q - o[5] : o[5];
         acc3_mul<false>(acc, c.w0, c0);
         acc3_mul<false>(acc, c.w1, c1);
     }
-    const u64* om = o + 7;
 #pragma unroll
     for (int j = 0; j < nf; j++) acc3_mul<SW>(acc, rests(j), om[j]);
-    // lo + 2^64 hi + 2^128 top  ->  lo + hi [2^64]_q + top [2^128]_q  (< 2^127) -> Barrett
     u64 lo, hi, top;
     acc3_value(acc, lo, hi, top);
     Acc3 fold;
@@ -367,20 +527,8 @@ __device__ __forceinline__ u64 sc_output(const u64* __restrict__ o, Rests rests,
     acc3_mul<SW>(fold, top, o[6]);
     u64 l2, h2, t2;
     acc3_value(fold, l2, h2, t2);
-    return sc_reduce_u128(l2, h2, bl, bh, q);
+    return sc_reduce_u128(l2, h2, o[1], o[2], q);
 }
-
-// Header fields of the constant block as wave-uniform values.
-template <bool SW>
-struct ScHeader {
-    int count;
-    unsigned shift;
-    bool is_one;
-    __device__ __forceinline__ explicit ScHeader(const u64* __restrict__ sc)
-        : count((int)sc_uniform<SW>(sc[SC_COUNT])),
-          shift(sc_uniform<SW>(sc[SC_SHIFT])),
-          is_one(sc_uniform<SW>(sc[SC_IS_ONE]) != 0) {}
-};
 
 // One column of RnsScaler::scale with NF input residues. `rests(j)` yields
 // residue j of the column; `emit(i, value)` receives output i (0-based, output
@@ -388,10 +536,9 @@ struct ScHeader {
 template <int NF, bool SW = true, class Rests, class Emit>
 __device__ __forceinline__ void rns_scale_column(const u64* __restrict__ sc, Rests rests, Emit emit) {
     const ScHeader<SW> h(sc);
-    const ScColumn c = sc_column<NF, SW>(sc, rests, h.shift, h.is_one);
+    const ScColumn c = sc_column<NF, SW>(sc, rests, h);
     const u64* outc = sc + sc_out_base(NF);
-    for (int i = 0; i < h.count; i++)
-        emit(i, sc_output<NF, SW>(outc + (size_t)i * sc_out_stride(NF), rests, c, h.is_one));
+    for (int i = 0; i < h.count; i++) emit(i, sc_output<NF, SW>(outc + (size_t)i * sc_out_stride(NF), rests, c, h));
 }
 
 }  // namespace fhehip

@@ -71,11 +71,11 @@ __global__ __launch_bounds__(256) void scale_pair_kernel(ScalePairArgs a) {
     }
     const ScHeader<SW> h(sc);
     auto rests = [&](int j) { return r[j]; };
-    const ScColumn cv = sc_column<NF, SW>(sc, rests, h.shift, h.is_one);
+    const ScColumn cv = sc_column<NF, SW>(sc, rests, h);
     const u64* outc = sc + sc_out_base(NF);
     for (int i = 0; i < h.count; i++) {
         const u64* o = outc + (size_t)i * sc_out_stride(NF);
-        u64 y = sc_output<NF, SW>(o, rests, cv, h.is_one);
+        u64 y = sc_output<NF, SW>(o, rests, cv, h);
         if (osp) {
             // Harvey forward butterfly of reduced inputs (native.rs:256-269),
             // outputs < 4q: the lane at c + N/2 forms t = w y and sends it,
