@@ -111,13 +111,19 @@ static void check_batch(u32 inst) {
 // of small launches from N = 4096 (split_rows), to spread them over more CUs.
 static bool fused(const DeviceContext& c) { return c.logn <= 14; }
 
+// Whether launches of any size at this degree run `stage` (a HALVES_* bit of
+// the tuning) as half rows (N = 4096 .. 16384).
+static bool halves_any(const DeviceContext& c, int stage) {
+    return c.logn >= 12 && c.logn <= 14 && c.half && (fused_tuning(c.logn).halves & stage);
+}
+
 // Inverse NTT of `parts` polynomials per instance into contiguous rows.
 // Rows transformed as halves end with the stage pairing the halves, unless
 // `close` is false: then that stage is left to the consumer (a paired
 // scaling) and the function returns true.
 static bool intt_rows(const DeviceContext& c, const InttRowsArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s,
                       bool close = true) {
-    if (!split_rows(c, (size_t)L * parts * inst)) {
+    if (!split_rows(c, (size_t)L * parts * inst) && (close || !halves_any(c, HALVES_INPUT))) {
         launch_intt_rows(c.logn, a, L, parts, inst, s);
         return false;
     }
@@ -134,7 +140,7 @@ static bool intt_rows(const DeviceContext& c, const InttRowsArgs& a, u32 L, u32 
 // (`close` as for intt_rows).
 static bool tensor_intt(const DeviceContext& mul, const TensorArgs& a, u32 M, u32 inst, hipStream_t s,
                         bool close = true) {
-    if (!split_rows(mul, (size_t)3 * M * inst)) {
+    if (!split_rows(mul, (size_t)3 * M * inst) && (close || !halves_any(mul, HALVES_TENSOR))) {
         launch_tensor_intt(mul.logn, a, M, inst, s);
         return false;
     }
@@ -329,15 +335,16 @@ size_t Ksk::finish_bytes(u32 inst) const {
     return (size_t)inst * 2 * (extra + (paired_finish(inst) ? L : 0)) * n * 8;
 }
 
-// Small launches of rows transformed as halves (extra <= 1) finish in two
-// steps: one elementwise pass over column pairs forms z and the first
-// forward stage of every row, and the half transforms only load it. Larger
-// launches form z on load of each half (twice per coefficient, but no
-// extra pass over the rows).
+// Rows transformed as halves (extra <= 1) finish in two steps when the
+// launch is small or the finish bit of the halves knob is on: one elementwise pass over column
+// pairs forms z and the first forward stage of every row, and the half
+// transforms only load it. Other half-row launches (degree 32768) form z on
+// load of each half (twice per coefficient, but no extra pass over the rows).
 bool Ksk::paired_finish(u32 inst) const {
     const size_t L = ct_ctx->L, rows = (size_t)2 * L * inst;
-    return extra <= 1 && rows <= split_row_limit() && split_rows(*ct_ctx, rows) &&
-           (!extra || split_rows(*key_ctx, (size_t)2 * inst));
+    if (extra > 1) return false;
+    if (halves_any(*ct_ctx, HALVES_FINISH) && (!extra || halves_any(*key_ctx, HALVES_FINISH))) return true;
+    return rows <= split_row_limit() && split_rows(*ct_ctx, rows) && (!extra || split_rows(*key_ctx, (size_t)2 * inst));
 }
 
 u64* Ksk::lift(const u64* src, size_t src_inst_stride, u32 inst, hipStream_t s, const HalfFwdRows* rows) const {
@@ -643,7 +650,8 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out_final, u32
     // scalers, which also form the first stage of the forward transform of
     // the new limbs when those are transformed as halves.
     const bool in_open = intt_rows(*base, ia, (u32)L, 4, inst, s, false);
-    const bool out_fwd = cl == cr && cl < M && split_rows(*mul, (size_t)inst * 4 * (M - cl));
+    const bool out_fwd =
+        cl == cr && cl < M && (split_rows(*mul, (size_t)inst * 4 * (M - cl)) || halves_any(*mul, HALVES_INPUT));
     // 2. base extension of the power-basis columns (rns/scaler.rs:249-352)
     auto extend = [&](const PolyScaler& sc, u32 k0, u32 parts) {
         if (sc.common >= M) return;
@@ -716,7 +724,7 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out_final, u32
 }
 
 // ---------------------------------------------------------------------------
-FusedTuning autotune_multiply(Multiplicator& m, u32 inst, float (&ms)[11]) {
+FusedTuning autotune_multiply(Multiplicator& m, u32 inst, float (&ms)[17]) {
     check_batch(inst);
     if (!inst) throw Error(-1, "the autotuner needs at least one instance");
     const int logn = m.base->logn;
@@ -732,7 +740,12 @@ FusedTuning autotune_multiply(Multiplicator& m, u32 inst, float (&ms)[11]) {
     try {
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        HIPCHK(hipMemsetAsync(in.ptr, 0, in.bytes, s));  // zero residues are valid inputs
+        // Pseudorandom residues (below the smallest modulus), so that the
+        // timed work draws the power of real operands: all-zero operands let
+        // the chip clock higher than uniform residues do.
+        u64 qmin = ~0ull;
+        for (u64 q : m.base->moduli) qmin = std::min(qmin, q);
+        launch_fill_words(in.as<u64>(), words_in, (u32)__builtin_clzll(qmin) + 1, 0x5eed, s);
         auto time_it = [&](const FusedTuning& t) {
             set_fused_tuning(logn, t);
             const int reps = 3;
@@ -767,6 +780,17 @@ FusedTuning autotune_multiply(Multiplicator& m, u32 inst, float (&ms)[11]) {
         knob(&FusedTuning::re_tensor, {4, 5});
         knob(&FusedTuning::re_lift, {4, 5});
         knob(&FusedTuning::re_finish, {4, 5});
+        // each stage's half-row choice in turn
+        for (int bit : {HALVES_FINISH, HALVES_INPUT, HALVES_TENSOR}) {
+            float t[2];
+            for (int on = 0; on < 2; on++) {
+                FusedTuning c = best;
+                c.halves = on ? (best.halves | bit) : (best.halves & ~bit);
+                t[on] = time_it(c);
+                ms[slot++] = t[on];
+            }
+            best.halves = t[1] < t[0] ? (best.halves | bit) : (best.halves & ~bit);
+        }
         set_fused_tuning(logn, best);
     } catch (...) {
         set_fused_tuning(logn, start);

@@ -214,7 +214,7 @@ struct Parameters {
 // starting from the current choice, keeps the fastest for the degree and
 // returns it. ms: milliseconds per multiply for xcd 0/1, ks_mac instances
 // 2/4/8, tensor 4/5, lift 4/5, finish 4/5 log2 values per thread.
-FusedTuning autotune_multiply(Multiplicator& m, u32 inst, float (&ms)[11]);
+FusedTuning autotune_multiply(Multiplicator& m, u32 inst, float (&ms)[17]);
 
 // fhehip_multiply with all scratch in the library's fixed device region
 // (no device allocation); refuses batches whose scratch does not fit.

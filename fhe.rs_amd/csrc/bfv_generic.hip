@@ -167,6 +167,24 @@ __global__ __launch_bounds__(256) void copy_slots_kernel(const u64* const* __res
     }
 }
 
+// x[e] = splitmix64(seed + e) >> shift: pseudorandom words below 2^(64 - shift),
+// residues of every modulus of at least 65 - shift bits.
+__global__ __launch_bounds__(256) void fill_words_kernel(u64* __restrict__ x, size_t elems, u32 shift, u64 seed) {
+    const size_t e = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= elems) return;
+    u64 z = seed + e * 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    x[e] = (z ^ (z >> 31)) >> shift;
+}
+
+void launch_fill_words(u64* x, size_t elems, u32 shift, u64 seed, hipStream_t s) {
+    if (!elems) return;
+    hipLaunchKernelGGL(fill_words_kernel, dim3((unsigned)((elems + 255) / 256)), dim3(256), 0, s, x, elems, shift,
+                       seed);
+    HIPCHK(hipGetLastError());
+}
+
 void launch_copy_slots(const u64* const* table, u32 count, size_t elems, hipStream_t s) {
     if (!count || !elems) return;
     ProfScope ps("copy_slots", 0, s);

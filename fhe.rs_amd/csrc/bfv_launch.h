@@ -217,5 +217,7 @@ void launch_sd_tail(int logn, const SdNttArgs& a, u64* x, bool post, u32 inst, h
 // Copies `count` slots of `elems` words; `table` is a device array of
 // (src, dst) pointer pairs.
 void launch_copy_slots(const u64* const* table, u32 count, size_t elems, hipStream_t s);
+// x[e] = splitmix64(seed + e golden-ratio steps) >> shift (timing inputs).
+void launch_fill_words(u64* x, size_t elems, u32 shift, u64 seed, hipStream_t s);
 
 }  // namespace fhehip

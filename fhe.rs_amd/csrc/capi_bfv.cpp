@@ -275,12 +275,13 @@ static FusedTuning from_c(const fhehip_tuning& t) {
     f.re_tensor = (int)t.tensor_log2_values;
     f.re_lift = (int)t.lift_log2_values;
     f.re_finish = (int)t.finish_log2_values;
+    f.halves = (int)t.halves;
     return f;
 }
 
 static fhehip_tuning to_c(const FusedTuning& f) {
-    return fhehip_tuning{(uint32_t)f.xcd, (uint32_t)f.ks_mac_inst, (uint32_t)f.re_tensor, (uint32_t)f.re_lift,
-                         (uint32_t)f.re_finish};
+    return fhehip_tuning{(uint32_t)f.xcd,       (uint32_t)f.ks_mac_inst, (uint32_t)f.re_tensor,
+                         (uint32_t)f.re_lift,   (uint32_t)f.re_finish,   (uint32_t)f.halves};
 }
 
 static int degree_log2(uint32_t degree) {
@@ -291,7 +292,7 @@ static int degree_log2(uint32_t degree) {
 int fhehip_multiply_autotune(fhehip_multiplicator* m, uint32_t num, fhehip_tuning* chosen, float* ms_out) {
     FHEHIP_API_BEGIN
     if (!m) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
-    float ms[11] = {};
+    float ms[17] = {};
     const FusedTuning t = autotune_multiply(*m->m, num, ms);
     if (chosen) *chosen = to_c(t);
     if (ms_out) std::memcpy(ms_out, ms, sizeof ms);

@@ -91,13 +91,87 @@ __global__ __launch_bounds__(256) void scale_pair_kernel(ScalePairArgs a) {
     }
 }
 
+// Large launches: one thread takes both columns c and c + N/2 of a pair, so
+// every residue is loaded once and the two stages pairing the halves need
+// no exchange between lanes; the constants (scalar cache) serve both
+// columns. The columns are scaled one after the other: while column c is in
+// registers, the residues of column c + N/2 (and, with the forward stage,
+// column c's outputs) wait in the thread's own LDS slots, so the kernel holds
+// the registers of one column, as rns_scale_kernel does.
+// grid = (N / 512, parts, instances), 256 threads; LDS: (NF + count) * 256
+// words with the forward stage, NF * 256 without.
+// The pointers are separate __restrict__ parameters (not fields of the
+// argument struct), so that the constants are known not to be written by the
+// kernel and load through the scalar cache.
+template <int NF>
+__global__ __launch_bounds__(256) void scale_pair2_kernel(const u64* __restrict__ in, size_t in_inst, size_t in_part,
+                                                          u64* __restrict__ out, size_t out_inst, size_t out_part,
+                                                          const u64* __restrict__ sc, const u64* __restrict__ isp,
+                                                          const ModInfo* __restrict__ in_mods,
+                                                          const u64* __restrict__ osp, u32 n) {
+    // isp: limb j at words 6 j .. 6 j + 5 = (w, w', z, z', n^-1, n^-1');
+    // osp: output i at words 6 i, 6 i + 1 = (w, w')
+    extern __shared__ u64 stage[];
+    const u32 half = n >> 1, tid = threadIdx.x;
+    const u32 c = blockIdx.x * 256 + tid;
+    const u64* src = in + blockIdx.z * in_inst + blockIdx.y * in_part + c;
+    u64* dst = out + blockIdx.z * out_inst + blockIdx.y * out_part + c;
+    u64 r[NF];
+#pragma unroll
+    for (int j = 0; j < NF; j++) {
+        const u64 x = src[(size_t)j * n], y = src[(size_t)j * n + half];
+        if (isp) {
+            // X = (x + y) n^-1 at c, Y = (x - y) zeta n^-1 at c + N/2
+            // (native.rs:288-300, 130-131)
+            const u64 p = in_mods[j].p;
+            r[j] = mul_shoup_u(x + y, isp[6 * j + 4], isp[6 * j + 5], p);
+            stage[j * 256 + tid] = mul_shoup_u(x + 2 * p - y, isp[6 * j + 2], isp[6 * j + 3], p);
+        } else {
+            r[j] = x;
+            stage[j * 256 + tid] = y;
+        }
+    }
+    const ScHeader<true> h(sc);
+    auto rests = [&](int j) { return r[j]; };
+    const u64* outc = sc + sc_out_base(NF);
+    {
+        const ScColumn cv = sc_column<NF, true>(sc, rests, h);
+        for (int i = 0; i < h.count; i++) {
+            const u64 y = sc_output<NF, true>(outc + (size_t)i * sc_out_stride(NF), rests, cv, h);
+            if (osp) stage[(NF + i) * 256 + tid] = y;
+            else dst[(size_t)i * n] = y;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NF; j++) r[j] = stage[j * 256 + tid];
+    const ScColumn cv = sc_column<NF, true>(sc, rests, h);
+    for (int i = 0; i < h.count; i++) {
+        const u64* o = outc + (size_t)i * sc_out_stride(NF);
+        u64 y1 = sc_output<NF, true>(o, rests, cv, h);
+        if (osp) {
+            // Harvey forward butterfly of reduced inputs (native.rs:256-269),
+            // outputs < 4q.
+            const u64 qi = o[0];
+            const u64 y0 = stage[(NF + i) * 256 + tid];
+            const u64 t = lazy_mul_shoup_u(y1, osp[6 * i], osp[6 * i + 1], qi);
+            dst[(size_t)i * n] = y0 + t;
+            y1 = y0 + 2 * qi - t;
+        }
+        dst[(size_t)i * n + half] = y1;
+    }
+}
+
 template <int NF>
 static void scale_pair_go(const ScalePairArgs& a, dim3 grid, bool lds, hipStream_t s) {
     if (lds) {
         const size_t words = a.sc_words + (a.in_split ? 7 * (size_t)NF : 0) + (a.out_split ? 6 * (size_t)a.count : 0);
         hipLaunchKernelGGL((scale_pair_kernel<NF, true>), grid, dim3(256), words * 8, s, a);
     } else {
-        hipLaunchKernelGGL((scale_pair_kernel<NF, false>), grid, dim3(256), 0, s, a);
+        const dim3 g2(a.n / 512, grid.y, grid.z);
+        const size_t words = (size_t)(NF + (a.out_split ? a.count : 0)) * 256;
+        hipLaunchKernelGGL((scale_pair2_kernel<NF>), g2, dim3(256), words * 8, s, a.in, a.in_inst, a.in_part, a.out,
+                           a.out_inst, a.out_part, a.sc, (const u64*)a.in_split, a.in_mods, (const u64*)a.out_split,
+                           a.n);
     }
 }
 

@@ -371,11 +371,11 @@ class Multiplicator:
 class TuningC(C.Structure):
     """fhehip_tuning of include/fhehip.h."""
     _fields_ = [("xcd_order", C.c_uint32), ("ks_mac_instances", C.c_uint32), ("tensor_log2_values", C.c_uint32),
-                ("lift_log2_values", C.c_uint32), ("finish_log2_values", C.c_uint32)]
+                ("lift_log2_values", C.c_uint32), ("finish_log2_values", C.c_uint32), ("halves", C.c_uint32)]
 
 
 TUNING_TIMES = ["xcd0", "xcd1", "ks_mac2", "ks_mac4", "ks_mac8", "tensor4", "tensor5", "lift4", "lift5", "finish4",
-                "finish5"]
+                "finish5", "finish_rows", "finish_halves", "input_rows", "input_halves", "tensor_rows", "tensor_halves"]
 
 
 def tuning_get(degree):
@@ -395,7 +395,7 @@ def tuning_set(degree, **kw):
 def multiply_autotune(mult, num):
     """fhehip_multiply_autotune: (chosen knobs, {candidate: ms per multiply})."""
     t = TuningC()
-    ms = (C.c_float * 11)()
+    ms = (C.c_float * len(TUNING_TIMES))()
     check(lib().fhehip_multiply_autotune(mult.h, num, C.byref(t), ms))
     return {k: getattr(t, k) for k, _ in TuningC._fields_}, dict(zip(TUNING_TIMES, [float(x) for x in ms]))
 

@@ -180,21 +180,28 @@ int fhehip_multiply(fhehip_multiplicator* m, const uint64_t* d_lhs, const uint64
  * the plain order (0); the instances that share one load of the key rows in
  * the key inner product (2, 4 or 8); the log2 values per thread (4: N/16
  * threads, 5: N/32 threads) of the fused tensor product + inverse NTT, of
- * the digit lift + forward NTT and of the key-switch finish. Every choice
- * gives the same, exact results. */
+ * the digit lift + forward NTT and of the key-switch finish; at N = 4096 ..
+ * 16384, which stages run as half rows (two workgroups per CU) in launches
+ * of any size, a mask `halves` (without its bit, only launches of at most
+ * fhehip_split_row_limit rows do): 1 the key-switch finish, 2 the
+ * multiplication's input inverse transforms, base extension and forward
+ * transform, 4 the tensor product + inverse transform and the t/Q scaling
+ * (2 and 4 pair the halves inside the scalers). Every choice gives the same,
+ * exact results. */
 typedef struct fhehip_tuning {
     uint32_t xcd_order;
     uint32_t ks_mac_instances;
     uint32_t tensor_log2_values;
     uint32_t lift_log2_values;
     uint32_t finish_log2_values;
+    uint32_t halves;
 } fhehip_tuning;
-/* Times fhehip_multiply on `num` instances (zero inputs, a private stream)
+/* Times fhehip_multiply on `num` instances (pseudorandom residues, a private stream)
  * under every choice of each knob in turn, starting from the degree's
  * current choice, keeps the fastest for the multiplicator's degree and
- * writes it to *chosen; ms_out (may be NULL) receives 11 milliseconds per
+ * writes it to *chosen; ms_out (may be NULL) receives 17 milliseconds per
  * multiply: xcd_order 0, 1; ks_mac_instances 2, 4, 8; tensor 4, 5; lift 4,
- * 5; finish 4, 5. Synchronous. Multiplications of the same degree running
+ * 5; finish 4, 5; then halves bit 1 off, on; bit 2 off, on; bit 4 off, on. Synchronous. Multiplications of the same degree running
  * meanwhile on other threads may use the trial choices. */
 int fhehip_multiply_autotune(fhehip_multiplicator* m, uint32_t num, fhehip_tuning* chosen, float* ms_out);
 /* Sets or reads the choice for one degree directly. */

@@ -1,7 +1,8 @@
 """GPU tests of the multiplication autotuner's launch shapes: every choice
 of every knob (XCD-aware workgroup order, instances per key load in the key
 inner product, values per thread of the fused tensor, lift and finish
-kernels) gives results bit-exact to the oracle, and the autotuner returns a
+kernels, half-row transforms for launches of any size) gives results
+bit-exact to the oracle, and the autotuner returns a
 valid choice with its timings."""
 import random
 
@@ -12,9 +13,10 @@ pytestmark = pytest.mark.gpu
 
 T = 65537
 KNOBS = [("xcd_order", 0), ("xcd_order", 1), ("ks_mac_instances", 2), ("ks_mac_instances", 4),
-         ("ks_mac_instances", 8), ("tensor_log2_values", 5), ("lift_log2_values", 5), ("finish_log2_values", 5)]
+         ("ks_mac_instances", 8), ("tensor_log2_values", 5), ("lift_log2_values", 5), ("finish_log2_values", 5),
+         ("halves", 0), ("halves", 1), ("halves", 2), ("halves", 4), ("halves", 7)]
 DEFAULT = {"xcd_order": 1, "ks_mac_instances": 4, "tensor_log2_values": 4, "lift_log2_values": 4,
-           "finish_log2_values": 4}
+           "finish_log2_values": 4, "halves": 1}
 
 
 @pytest.fixture(scope="module")
@@ -63,6 +65,15 @@ def test_every_choice_matches_oracle(fh, oracle, restore, n, sizes):
     want = om.multiply(lhs, rhs)
     want_rot = bfv.galois(gk, want[:w])
     dl, dr = fh.to_device(lhs), fh.to_device(rhs)
+    # no small-launch half rows: the halves knob alone decides
+    prev = fh.split_row_limit(0)
+    try:
+        _each_choice(fh, n, mult, g, dl, dr, batch, w, want, want_rot)
+    finally:
+        fh.split_row_limit(prev)
+
+
+def _each_choice(fh, n, mult, g, dl, dr, batch, w, want, want_rot):
     for knob, value in KNOBS:
         fh.tuning_set(n, **dict(DEFAULT, **{knob: value}))
         out = fh.empty(batch * w)
@@ -80,6 +91,7 @@ def test_autotune_returns_a_valid_choice(fh, oracle, restore):
     chosen, ms = fh.multiply_autotune(mult, 16)
     assert chosen["xcd_order"] in (0, 1) and chosen["ks_mac_instances"] in (2, 4, 8)
     assert all(chosen[k] in (4, 5) for k in ("tensor_log2_values", "lift_log2_values", "finish_log2_values"))
+    assert 0 <= chosen["halves"] <= 7
     assert all(v > 0 for v in ms.values()), ms
     assert fh.tuning_get(n) == chosen
     lhs = oracle.random_polys(moduli, n, 6, seed=5)
