@@ -59,8 +59,8 @@ def kernel_bytes(name, B, L, Lk, M):
         return B * 4 * (M - L) * row * 2
     if kind == "tensor_intt":    # read a0,a1,b0,b1 over M limbs, write 3 x M rows
         return B * (4 + 3) * M * row
-    if kind == "rns_extend":     # 2 polys: read L rows, write M - L rows
-        return B * 2 * M * row
+    if kind == "rns_extend":     # one launch for both operands, 4 polys: read L rows, write M - L rows
+        return B * 4 * M * row
     if kind == "rns_down":       # 3 polys: read M rows, write L rows
         return B * 3 * (M + L) * row
     if kind == "lift_ntt":       # read c2 (L rows), write the lifted, transformed digits (L x Lk rows)
@@ -71,6 +71,10 @@ def kernel_bytes(name, B, L, Lk, M):
         return B * 2 * row * 2
     if kind == "ks_finish":      # read 2 dropped-limb rows, 2L c0/c1 rows and 2L outputs; write 2L rows
         return B * (2 + 6 * L) * row
+    if kind == "ks_finish_paired":  # after ks_z_pair: read 2L z rows and 2L outputs; write 2L rows
+        return B * 6 * L * row
+    if kind == "ks_z_pair":      # read 2 dropped-limb rows and 2L c0/c1 rows; write 2L z rows
+        return B * (2 + 4 * L) * row
     return None
 
 
@@ -1036,11 +1040,19 @@ def main():
                          "bytes_per_call_pcie": (3 * per) * 8,
                          "bit_exact_vs_device_resident": bool(torch.equal(h_o, one_o.cpu()))}
 
-    # Roofline of the dominant kernel.
+    # Roofline of the dominant kernel. The key-switch finish reads less when
+    # ks_z_pair formed its first forward stage (the paired, half-row finish).
+    paired = any(k.startswith("ks_z_pair/") for k in kernels)
+
+    def step_bytes(kname):
+        if paired and kname.startswith("ks_finish/"):
+            kname = "ks_finish_paired"
+        return kernel_bytes(kname, B, L, Lk, M)
+
     dom = max(kernels.items(), key=lambda kv: kv[1][1])
     dname, (dcalls, dms) = dom
     avg_s = dms / dcalls / 1e3
-    abytes = kernel_bytes(dname, B, L, Lk, M)
+    abytes = step_bytes(dname)
     achieved = abytes / avg_s / 1e9 if abytes else None
     digest = kernel_sources_digest()
     provenance = profile_provenance(digest)
@@ -1056,7 +1068,7 @@ def main():
     # VALU columns come from the committed counter summary).
     kernel_roofline = {}
     for kname, (kc, kms) in sorted(kernels.items()):
-        kb = kernel_bytes(kname, B, L, Lk, M)
+        kb = step_bytes(kname)
         ks = kms / kc / 1e3
         vb, vr = load_valu(kname.split("/")[0], digest)
         kernel_roofline[kname] = {"avg_launch_ms": ks * 1e3, "algorithmic_bytes": kb,
