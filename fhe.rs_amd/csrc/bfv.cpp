@@ -123,6 +123,14 @@ static bool halves_any(const DeviceContext& c, int stage) {
 // scaling) and the function returns true.
 static bool intt_rows(const DeviceContext& c, const InttRowsArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s,
                       bool close = true) {
+    if (wide(c)) {
+        // copy (or gather) the rows, then transform them in place
+        if (a.dst_inst_stride != (size_t)parts * L * c.N || (parts > 1 && a.dst_part_stride != (size_t)L * c.N))
+            throw Error(-1, "the wide-row inverse NTT needs contiguous destination rows");
+        launch_rows_gather(a, L, parts, inst, c.logn, s);
+        launch_ntt_backward(c, RowMap{a.dst, L, L, 0, a.table_offset}, inst * parts * L, s);
+        return false;
+    }
     if (!split_rows(c, (size_t)L * parts * inst) && (close || !halves_any(c, HALVES_INPUT))) {
         launch_intt_rows(c.logn, a, L, parts, inst, s);
         return false;
@@ -140,6 +148,13 @@ static bool intt_rows(const DeviceContext& c, const InttRowsArgs& a, u32 L, u32 
 // (`close` as for intt_rows).
 static bool tensor_intt(const DeviceContext& mul, const TensorArgs& a, u32 M, u32 inst, hipStream_t s,
                         bool close = true) {
+    if (wide(mul)) {
+        if (a.out_inst_stride != 3 * (size_t)M * mul.N || a.out_part_stride != (size_t)M * mul.N)
+            throw Error(-1, "the wide-row tensor product needs contiguous output rows");
+        launch_tensor_ew(a, M, inst, mul.logn, s);
+        launch_ntt_backward(mul, RowMap{a.out, M, M, 0, 0}, inst * 3 * M, s);
+        return false;
+    }
     if (!split_rows(mul, (size_t)3 * M * inst) && (close || !halves_any(mul, HALVES_TENSOR))) {
         launch_tensor_intt(mul.logn, a, M, inst, s);
         return false;
@@ -331,7 +346,7 @@ size_t Ksk::lift_bytes(u32 inst) const { return (size_t)inst * digits * key_ctx-
 // [inst][2][L][N] for the paired finish.
 size_t Ksk::finish_bytes(u32 inst) const {
     const size_t n = ct_ctx->N, L = ct_ctx->L, Lk = key_ctx->L;
-    if (extra > 1) return (size_t)inst * 2 * (Lk + (fused(*ct_ctx) ? 0 : L)) * n * 8;
+    if (extra > 1 || wide(*ct_ctx)) return (size_t)inst * 2 * (Lk + (fused(*ct_ctx) ? 0 : L)) * n * 8;
     return (size_t)inst * 2 * (extra + (paired_finish(inst) ? L : 0)) * n * 8;
 }
 
@@ -350,6 +365,13 @@ bool Ksk::paired_finish(u32 inst) const {
 u64* Ksk::lift(const u64* src, size_t src_inst_stride, u32 inst, hipStream_t s, const HalfFwdRows* rows) const {
     u64* y = scratch(Scratch::Lift).get(s, lift_bytes(inst)).as<u64>();
     const KeySwitchArgs a = args(src, src_inst_stride);
+    if (wide(*key_ctx)) {
+        // lifted digits, then their forward transforms (reduced: < 2q)
+        if (rows && rows->rows) throw Error(-2, "paired finish rows at a wide degree");
+        launch_lift_ew(a, y, inst, key_ctx->logn, s);
+        launch_ntt_forward(*key_ctx, RowMap{y, key_ctx->L, key_ctx->L, 0, 0}, inst * digits * key_ctx->L, false, s);
+        return y;
+    }
     if (!split_rows(*key_ctx, (size_t)digits * key_ctx->L * inst)) {
         launch_lift_ntt(key_ctx->logn, a, y, inst, s);
         if (rows && rows->rows)
@@ -403,8 +425,9 @@ void Ksk::switch_and_finish(const u64* src, size_t src_inst_stride, u64* out, u3
     check_batch(inst);
     const size_t n = ct_ctx->N, Lk = key_ctx->L, L = ct_ctx->L;
     SdNttArgs a = sd_args(out, add_pb, add_inst, add_part, add_ntt, add_ntt_inst, add_ntt_part, add_ntt_parts, perm);
-    if (extra > 1) {
-        // Several dropped moduli (switch_down_to, math/src/rq/mod.rs:455-462):
+    if (extra > 1 || wide(*ct_ctx)) {
+        // Several dropped moduli (switch_down_to, math/src/rq/mod.rs:455-462),
+        // or rows too long for the fused finish (degrees above 32768):
         // key switch to power basis, then Poly::switch_down once per dropped
         // modulus, last first. The drops of limbs L' - 1 ... L + 1 run in
         // place on the key-switch rows; the last one (limb L) is fused with
@@ -727,6 +750,7 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out_final, u32
 FusedTuning autotune_multiply(Multiplicator& m, u32 inst, float (&ms)[17]) {
     check_batch(inst);
     if (!inst) throw Error(-1, "the autotuner needs at least one instance");
+    if (wide(*m.base)) throw Error(-2, "the autotuner covers degrees 8 .. 32768 (the fused row kernels)");
     const int logn = m.base->logn;
     const size_t n = m.base->N, L = m.base->L;
     const size_t words_in = (size_t)inst * 2 * L * n;

@@ -118,6 +118,11 @@ void launch_tensor_intt(int logn, const TensorArgs& a, u32 M, u32 inst, hipStrea
 void launch_intt_rows_half(const DeviceContext& c, const InttRowsArgs& a, u32 limbs, u32 parts, u32 inst,
                            hipStream_t s);
 void launch_tensor_intt_half(const DeviceContext& mul, const TensorArgs& a, u32 M, u32 inst, hipStream_t s);
+// Degrees above 32768 (bfv_wide.hip): the elementwise halves of intt_rows,
+// tensor_intt and lift_ntt, each followed by the plain transforms.
+void launch_rows_gather(const InttRowsArgs& a, u32 limbs, u32 parts, u32 inst, int logn, hipStream_t s);
+void launch_tensor_ew(const TensorArgs& a, u32 M, u32 inst, int logn, hipStream_t s);
+void launch_lift_ew(const KeySwitchArgs& a, u64* y, u32 inst, int logn, hipStream_t s);
 // Lifted, lazily transformed digits Y [inst][digits][key_limbs][N].
 void launch_lift_ntt(int logn, const KeySwitchArgs& a, u64* y, u32 inst, hipStream_t s);
 void launch_sd_add_ntt(int logn, const SdNttArgs& a, u32 L, u32 parts, u32 inst, hipStream_t s);

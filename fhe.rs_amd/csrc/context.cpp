@@ -40,10 +40,67 @@ static void set_last_stage(ModInfo& mi, u64 z) {
     mi.last_zr_shoup = shoup64(mi.last_zr, p);
 }
 
+// Half tables (see DeviceContext::half) of context c from its host tables.
+// Forward: local index k = 2^s + g of an N/2-point transform at stage s maps
+// to the global index 2^(s+1) + h 2^s + g. Inverse: local (N' - (N' >> v)) + g
+// maps to (N - (N >> v)) + h 2^(logn - 2 - v) + g. A half of 32768 or more
+// coefficients gets halves of its own, down to rows that fit a workgroup.
+static void attach_half(DeviceContext& c, const std::vector<ulonglong2>& fwd, const std::vector<ulonglong2>& inv) {
+    const u32 n = c.N, nh = n / 2;
+    const u32 lh = (u32)c.logn - 1;
+    auto hc = std::make_shared<DeviceContext>();
+    hc->L = 2 * c.L;
+    hc->N = nh;
+    hc->logn = (int)lh;
+    std::vector<ulonglong2> hf((size_t)hc->L * nh), hi((size_t)hc->L * nh);
+    std::vector<ulonglong2> sp;
+    for (u32 l = 0; l < c.L; l++) {
+        const u64 p = c.moduli[l];
+        for (u32 h = 0; h < 2; h++) {
+            ulonglong2* F = hf.data() + ((size_t)2 * l + h) * nh;
+            ulonglong2* I = hi.data() + ((size_t)2 * l + h) * nh;
+            F[0] = fwd[(size_t)l * n + 1];
+            for (u32 k = 1; k < nh; k++) {
+                const u32 sft = 31 - __builtin_clz(k);
+                F[k] = fwd[(size_t)l * n + k + ((1u + h) << sft)];
+            }
+            for (u32 v = 0; v < lh; v++) {
+                const u32 lo = nh - (nh >> v), go = n - (n >> v);
+                for (u32 g = 0; g < (1u << (lh - 1 - v)); g++)
+                    I[lo + g] = inv[(size_t)l * n + go + (h << (lh - 1 - v)) + g];
+            }
+            I[nh - 1] = inv[(size_t)l * n + n - 1];
+            ModInfo mi = c.h_mod[l];
+            mi.n_inv = 1;
+            mi.n_inv_shoup = shoup64(1, p);
+            mi.n_inv_r = (u64)(((u128)1 << 64) % p);
+            mi.n_inv_r_shoup = shoup64(mi.n_inv_r, p);
+            set_last_stage(mi, I[nh - 2].x);
+            hc->h_mod.push_back(mi);
+            hc->moduli.push_back(p);
+        }
+        const u64 ninv = c.h_mod[l].n_inv;
+        const u64 zl = mulmod64(inv[(size_t)l * n + n - 2].x, ninv, p);
+        sp.push_back(fwd[(size_t)l * n + 1]);
+        sp.push_back(make_ulonglong2(zl, shoup64(zl, p)));
+        sp.push_back(make_ulonglong2(ninv, shoup64(ninv, p)));
+    }
+    if (hc->logn >= 15) attach_half(*hc, hf, hi);
+    hc->fwd_tw.reset(hf.size() * sizeof(ulonglong2));
+    hc->inv_tw.reset(hi.size() * sizeof(ulonglong2));
+    hc->mod.reset(hc->h_mod.size() * sizeof(ModInfo));
+    HIPCHK(hipMemcpy(hc->fwd_tw.ptr, hf.data(), hc->fwd_tw.bytes, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(hc->inv_tw.ptr, hi.data(), hc->inv_tw.bytes, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(hc->mod.ptr, hc->h_mod.data(), hc->mod.bytes, hipMemcpyHostToDevice));
+    c.split.reset(sp.size() * sizeof(ulonglong2));
+    HIPCHK(hipMemcpy(c.split.ptr, sp.data(), c.split.bytes, hipMemcpyHostToDevice));
+    c.half = hc;
+}
+
 std::shared_ptr<DeviceContext> make_context(const std::vector<u64>& moduli, u32 n, const u64* psi) {
     if (moduli.empty()) throw Error(-1, "empty moduli list");
     if (n < 8 || (n & (n - 1))) throw Error(-1, "The degree is not a power of two larger or equal to 8");
-    if (n > 32768) throw Error(-2, "degree above 32768 is not supported");
+    if (n > kMaxDegree) throw Error(-2, "degree above 131072 is not supported");
     auto c = std::make_shared<DeviceContext>();
     c->moduli = moduli;
     c->L = (u32)moduli.size();
@@ -92,60 +149,7 @@ std::shared_ptr<DeviceContext> make_context(const std::vector<u64>& moduli, u32 
         set_last_stage(mi, inv[(size_t)l * n + n - 2].x);
         c->h_mod.push_back(mi);
     }
-    if (n >= 4096) {
-        // Half tables (see DeviceContext::half). Forward: local index k = 2^s + g
-        // of an N/2-point transform at stage s maps to the global index
-        // 2^(s+1) + h 2^s + g. Inverse: local (N' - (N' >> v)) + g maps to
-        // (N - (N >> v)) + h 2^(logn - 2 - v) + g.
-        const u32 nh = n / 2;
-        const u32 lh = (u32)c->logn - 1;
-        auto hc = std::make_shared<DeviceContext>();
-        hc->L = 2 * c->L;
-        hc->N = nh;
-        hc->logn = (int)lh;
-        std::vector<ulonglong2> hf((size_t)hc->L * nh), hi((size_t)hc->L * nh);
-        std::vector<ulonglong2> sp;
-        for (u32 l = 0; l < c->L; l++) {
-            const u64 p = moduli[l];
-            for (u32 h = 0; h < 2; h++) {
-                ulonglong2* F = hf.data() + ((size_t)2 * l + h) * nh;
-                ulonglong2* I = hi.data() + ((size_t)2 * l + h) * nh;
-                F[0] = fwd[(size_t)l * n + 1];
-                for (u32 k = 1; k < nh; k++) {
-                    const u32 sft = 31 - __builtin_clz(k);
-                    F[k] = fwd[(size_t)l * n + k + ((1u + h) << sft)];
-                }
-                for (u32 v = 0; v < lh; v++) {
-                    const u32 lo = nh - (nh >> v), go = n - (n >> v);
-                    for (u32 g = 0; g < (1u << (lh - 1 - v)); g++)
-                        I[lo + g] = inv[(size_t)l * n + go + (h << (lh - 1 - v)) + g];
-                }
-                I[nh - 1] = inv[(size_t)l * n + n - 1];
-                ModInfo mi = c->h_mod[l];
-                mi.n_inv = 1;
-                mi.n_inv_shoup = shoup64(1, p);
-                mi.n_inv_r = (u64)(((u128)1 << 64) % p);
-                mi.n_inv_r_shoup = shoup64(mi.n_inv_r, p);
-                set_last_stage(mi, I[nh - 2].x);
-                hc->h_mod.push_back(mi);
-                hc->moduli.push_back(p);
-            }
-            const u64 ninv = c->h_mod[l].n_inv;
-            const u64 zl = mulmod64(inv[(size_t)l * n + n - 2].x, ninv, p);
-            sp.push_back(fwd[(size_t)l * n + 1]);
-            sp.push_back(make_ulonglong2(zl, shoup64(zl, p)));
-            sp.push_back(make_ulonglong2(ninv, shoup64(ninv, p)));
-        }
-        hc->fwd_tw.reset(hf.size() * sizeof(ulonglong2));
-        hc->inv_tw.reset(hi.size() * sizeof(ulonglong2));
-        hc->mod.reset(hc->h_mod.size() * sizeof(ModInfo));
-        HIPCHK(hipMemcpy(hc->fwd_tw.ptr, hf.data(), hc->fwd_tw.bytes, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(hc->inv_tw.ptr, hi.data(), hc->inv_tw.bytes, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(hc->mod.ptr, hc->h_mod.data(), hc->mod.bytes, hipMemcpyHostToDevice));
-        c->split.reset(sp.size() * sizeof(ulonglong2));
-        HIPCHK(hipMemcpy(c->split.ptr, sp.data(), c->split.bytes, hipMemcpyHostToDevice));
-        c->half = hc;
-    }
+    if (n >= 4096) attach_half(*c, fwd, inv);
     c->fwd_tw.reset(fwd.size() * sizeof(ulonglong2));
     c->inv_tw.reset(inv.size() * sizeof(ulonglong2));
     c->mod.reset(c->h_mod.size() * sizeof(ModInfo));

@@ -38,6 +38,11 @@ struct DevBuf {
     T* as() const { return (T*)ptr; }
 };
 
+// Largest supported degree. Rows of up to 16384 coefficients are transformed
+// in one workgroup; longer rows are split into halves, one elementwise stage
+// pass per halving (DeviceContext::half, recursively).
+constexpr u32 kMaxDegree = 131072;
+
 // Resident per-context data (rq::Context, math/src/rq/context.rs:9-19).
 struct DeviceContext {
     std::vector<u64> moduli;
@@ -54,8 +59,9 @@ struct DeviceContext {
     // of a row; each half is then an independent N/2-point transform. `half`
     // is a context of 2L rows of degree N/2 whose tables are the full tables
     // re-indexed for half h of limb l at row 2l + h (its inverse leaves out the
-    // N^-1 factor). Degree 32768 always runs this way (a row does not fit one
-    // workgroup's LDS); smaller degrees do for small launches (split_rows).
+    // N^-1 factor). Degrees from 32768 always run this way (a row does not fit
+    // one workgroup's LDS), and a half of 32768 or more coefficients has halves
+    // of its own; degrees 4096 .. 16384 do for small launches (split_rows).
     std::shared_ptr<DeviceContext> half;
     DevBuf split;    // [L] ulonglong2 pairs: (omega_1, shoup), (zeta_last * N^-1, shoup), (N^-1, shoup)
 
@@ -75,11 +81,16 @@ struct RowMap {
     u32 table_offset;  // context limb index of limb first_limb
 };
 
-// Whether a launch of `rows` rows of context c transforms each row as two
-// halves: always at degree 32768; at 4096 .. 16384 when the launch has at
-// most split_row_limit() rows (fewer workgroups than the chip has CUs, so
-// that two workgroups per row shorten the launch).
+// Whether a launch of `rows` rows of context c runs the half-row kernels
+// (each row as two halves of at most 16384 coefficients, the stage pairing
+// the halves formed on load or left to the consumer): always at degree 32768;
+// at 4096 .. 16384 when the launch has at most split_row_limit() rows (fewer
+// workgroups than the chip has CUs, so that two workgroups per row shorten the
+// launch). Never above 32768, where rows take the wide path (wide()).
 bool split_rows(const DeviceContext& c, size_t rows);
+// Degrees above 32768: the BFV operations run as elementwise kernels around
+// the plain (recursively split) transforms.
+inline bool wide(const DeviceContext& c) { return c.logn > 15; }
 u32 split_row_limit();
 // Sets the limit (0: only degree 32768 splits); process-wide.
 void set_split_row_limit(u32 rows);

@@ -153,7 +153,7 @@ void set_split_row_limit(u32 rows) { g_split_rows.store(rows, std::memory_order_
 
 bool split_rows(const DeviceContext& c, size_t rows) {
     if (c.logn == 15) return true;
-    return c.logn >= 12 && c.half && rows <= split_row_limit();
+    return c.logn >= 12 && c.logn <= 14 && c.half && rows <= split_row_limit();
 }
 
 // N = 32768 digit lift (rq/mod.rs:518-540) with its whole lazy forward
@@ -270,34 +270,51 @@ static RowMap half_rows(RowMap m) {
     return RowMap{m.data, 2 * m.rows_per_poly, 2 * m.limbs_per_poly, 2 * m.first_limb, 2 * m.table_offset};
 }
 
-void launch_ntt_forward(const DeviceContext& c, RowMap m, u32 rows, bool lazy, hipStream_t s) {
-    if (rows == 0) return;
-    ProfScope ps(lazy ? "ntt_forward_lazy" : "ntt_forward", c.logn, s);
-    if (split_rows(c, rows)) {
+// The transforms of rows longer than one workgroup holds (degree 32768 and
+// up), or of small launches at 4096 .. 16384: the stage pairing the halves as
+// one elementwise pass, and the halves as rows of the half context, split
+// again while they are 32768 or more coefficients long.
+static void ntt_forward_rows(const DeviceContext& c, RowMap m, u32 rows, bool lazy, hipStream_t s) {
+    if (c.logn >= 15 || split_rows(c, rows)) {
         hipLaunchKernelGGL(ntt_split_fwd_kernel, dim3((c.N / 2) / 256, rows), dim3(256), 0, s, m,
                            c.split.as<ulonglong2>(), c.mod.as<ModInfo>(), (u32)c.logn);
         HIPCHK(hipGetLastError());
-        FHEHIP_LOGN_SWITCH(c.logn - 1, fwd_launch, *c.half, half_rows(m), 2 * rows, lazy, s);
+        ntt_forward_rows(*c.half, half_rows(m), 2 * rows, lazy, s);
     } else {
         FHEHIP_LOGN_SWITCH(c.logn, fwd_launch, c, m, rows, lazy, s);
     }
     HIPCHK(hipGetLastError());
 }
 
+static void ntt_backward_rows(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s) {
+    if (c.logn >= 15 || split_rows(c, rows)) {
+        ntt_backward_rows(*c.half, half_rows(m), 2 * rows, s);
+        hipLaunchKernelGGL(ntt_split_inv_kernel, dim3((c.N / 2) / 256, rows), dim3(256), 0, s, m,
+                           c.split.as<ulonglong2>(), c.mod.as<ModInfo>(), (u32)c.logn);
+    } else {
+        FHEHIP_LOGN_SWITCH(c.logn, inv_launch, c, m, rows, s);
+    }
+    HIPCHK(hipGetLastError());
+}
+
+void launch_ntt_forward(const DeviceContext& c, RowMap m, u32 rows, bool lazy, hipStream_t s) {
+    if (rows == 0) return;
+    ProfScope ps(lazy ? "ntt_forward_lazy" : "ntt_forward", c.logn, s);
+    ntt_forward_rows(c, m, rows, lazy, s);
+}
+
 void launch_ntt_forward_halves(const DeviceContext& c, RowMap m, u32 rows, bool lazy, hipStream_t s) {
     if (rows == 0) return;
     if (!c.half) throw Error(-2, "this context has no half-row tables");
     ProfScope ps(lazy ? "ntt_forward_lazy" : "ntt_forward", c.logn, s);
-    FHEHIP_LOGN_SWITCH(c.logn - 1, fwd_launch, *c.half, half_rows(m), 2 * rows, lazy, s);
-    HIPCHK(hipGetLastError());
+    ntt_forward_rows(*c.half, half_rows(m), 2 * rows, lazy, s);
 }
 
 void launch_ntt_backward_halves(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s) {
     if (rows == 0) return;
     if (!c.half) throw Error(-2, "this context has no half-row tables");
     ProfScope ps("ntt_backward", c.logn, s);
-    FHEHIP_LOGN_SWITCH(c.logn - 1, inv_launch, *c.half, half_rows(m), 2 * rows, s);
-    HIPCHK(hipGetLastError());
+    ntt_backward_rows(*c.half, half_rows(m), 2 * rows, s);
 }
 
 void launch_ntt_split_inv(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s) {
@@ -312,15 +329,7 @@ void launch_ntt_split_inv(const DeviceContext& c, RowMap m, u32 rows, hipStream_
 void launch_ntt_backward(const DeviceContext& c, RowMap m, u32 rows, hipStream_t s) {
     if (rows == 0) return;
     ProfScope ps("ntt_backward", c.logn, s);
-    if (split_rows(c, rows)) {
-        FHEHIP_LOGN_SWITCH(c.logn - 1, inv_launch, *c.half, half_rows(m), 2 * rows, s);
-        HIPCHK(hipGetLastError());
-        hipLaunchKernelGGL(ntt_split_inv_kernel, dim3((c.N / 2) / 256, rows), dim3(256), 0, s, m,
-                           c.split.as<ulonglong2>(), c.mod.as<ModInfo>(), (u32)c.logn);
-    } else {
-        FHEHIP_LOGN_SWITCH(c.logn, inv_launch, c, m, rows, s);
-    }
-    HIPCHK(hipGetLastError());
+    ntt_backward_rows(c, m, rows, s);
 }
 
 // Times forward and inverse transforms of `rows` rows of `c` (the half
@@ -328,8 +337,14 @@ void launch_ntt_backward(const DeviceContext& c, RowMap m, u32 rows, hipStream_t
 // faster shape per direction for this degree. Returns the chosen log2 values
 // per thread and the measured milliseconds per launch [fwd16, fwd32, inv16, inv32].
 void autotune_ntt(const DeviceContext& c, u32 rows, int& re_fwd, int& re_inv, float (&ms)[4]) {
-    const DeviceContext& t = c.logn == 15 ? *c.half : c;
-    const u32 trows = c.logn == 15 ? 2 * rows : rows;
+    // rows longer than 16384 coefficients: the shape of their pieces
+    const DeviceContext* tp = &c;
+    u32 trows = rows;
+    while (tp->logn > 14) {
+        tp = tp->half.get();
+        trows *= 2;
+    }
+    const DeviceContext& t = *tp;
     const int logn = t.logn;
     if (logn < 5) {  // one shape only
         re_fwd = re_inv = 4;
@@ -390,9 +405,9 @@ void autotune_ntt(const DeviceContext& c, u32 rows, int& re_fwd, int& re_inv, fl
 }
 
 void set_ntt_shape(int logn, int re_fwd, int re_inv) {
-    if (logn == 15) logn = 14;  // the half transforms
+    if (logn > 14 && logn <= 17) logn = 14;  // the pieces of longer rows
     if (logn < 3 || logn > 14 || (re_fwd != 4 && re_fwd != 5) || (re_inv != 4 && re_inv != 5))
-        throw Error(-1, "NTT shape: degree 8..32768, 4 or 5 log2 values per thread");
+        throw Error(-1, "NTT shape: degree 8..131072, 4 or 5 log2 values per thread");
     g_re_fwd[logn].store(re_fwd);
     g_re_inv[logn].store(re_inv);
 }

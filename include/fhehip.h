@@ -19,7 +19,10 @@
  * `stream` is a hipStream_t (NULL = default stream); calls are asynchronous
  * with respect to the host unless stated otherwise. Scratch memory is kept
  * per stream: calls on different streams may run concurrently, also on one
- * handle; calls on one stream run in stream order. Degrees 8 .. 32768.
+ * handle; calls on one stream run in stream order. Degrees 8 .. 131072:
+ * rows of up to 16384 coefficients are transformed in one workgroup, longer
+ * rows as halves (one elementwise stage pass per halving); above 32768 the
+ * BFV operations run as elementwise passes around those transforms.
  */
 #ifndef FHEHIP_H
 #define FHEHIP_H
@@ -196,12 +199,13 @@ typedef struct fhehip_tuning {
     uint32_t finish_log2_values;
     uint32_t halves;
 } fhehip_tuning;
-/* Times fhehip_multiply on `num` instances (pseudorandom residues, a private stream)
- * under every choice of each knob in turn, starting from the degree's
- * current choice, keeps the fastest for the multiplicator's degree and
- * writes it to *chosen; ms_out (may be NULL) receives 17 milliseconds per
- * multiply: xcd_order 0, 1; ks_mac_instances 2, 4, 8; tensor 4, 5; lift 4,
- * 5; finish 4, 5; then halves bit 1 off, on; bit 2 off, on; bit 4 off, on. Synchronous. Multiplications of the same degree running
+/* Times fhehip_multiply on `num` instances (pseudorandom residues, a
+ * private stream) under every choice of each knob in turn, starting from the
+ * degree's current choice, keeps the fastest for the multiplicator's degree
+ * (8 .. 32768) and writes it to *chosen; ms_out (may be NULL) receives 17
+ * milliseconds per multiply: xcd_order 0, 1; ks_mac_instances 2, 4, 8;
+ * tensor 4, 5; lift 4, 5; finish 4, 5; then halves bit 1 off, on; bit 2 off,
+ * on; bit 4 off, on. Synchronous. Multiplications of the same degree running
  * meanwhile on other threads may use the trial choices. */
 int fhehip_multiply_autotune(fhehip_multiplicator* m, uint32_t num, fhehip_tuning* chosen, float* ms_out);
 /* Sets or reads the choice for one degree directly. */
@@ -210,7 +214,7 @@ int fhehip_tuning_get(uint32_t degree, fhehip_tuning* t);
 
 /* ---- Parameters ---------------------------------------------------------------
  * BfvParameters (fhe/src/bfv/parameters.rs:431-641) built from the degree
- * (a power of two, 8 .. 32768), a plaintext modulus t (2 <= t < 2^62,
+ * (a power of two, 8 .. 131072), a plaintext modulus t (2 <= t < 2^62,
  * invertible modulo every ciphertext modulus) and the ciphertext moduli (see
  * fhehip_generate_moduli for BfvParametersBuilder::set_moduli_sizes): the
  * level chain (level l holds the first L - l moduli, max_level = L - 1), the

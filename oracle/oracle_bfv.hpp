@@ -349,12 +349,14 @@ struct BfvParams {
 // secret_key.rs:200-282 and plaintext.rs:310-330
 // ---------------------------------------------------------------------------
 static inline std::vector<u64> encode_simd(const BfvParams& par, const std::vector<u64>& slots) {
+    if (!par.t_op) throw std::invalid_argument("the plaintext modulus does not support SIMD encoding at this degree");
     std::vector<u64> v(par.degree, 0);
     for (size_t i = 0; i < slots.size(); i++) v[par.index_map[i]] = slots[i];
     par.t_op->backward(v.data());
     return v;
 }
 static inline std::vector<u64> decode_simd(const BfvParams& par, std::vector<u64> w) {
+    if (!par.t_op) throw std::invalid_argument("the plaintext modulus does not support SIMD encoding at this degree");
     par.t_op->forward(w.data());
     std::vector<u64> out(par.degree);
     for (size_t i = 0; i < par.degree; i++) out[i] = w[par.index_map[i]];
