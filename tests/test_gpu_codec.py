@@ -42,6 +42,7 @@ CASES = [
     (8192, [50, 55, 55], (1 << 20) + (1 << 19) + (1 << 17) + (1 << 16) + (1 << 14) + 1, 1, 2),
     (16384, [60] * 7, 65537, 1, 2),
     (32768, [60] * 4, 65537, 0, 1),
+    (65536, [60] * 3, 786433, 1, 1),  # 786433 = 3 * 2^18 + 1: SIMD slots at degree 65536
 ]
 
 

@@ -151,6 +151,24 @@ def test_degree_32768_job(fh, oracle):
         assert np.array_equal(fh.to_host(j[4]), e)
 
 
+def test_degree_65536_job(fh, oracle):
+    """A wide-degree job (elementwise passes around split transforms) in one
+    submit with fused-pipeline jobs."""
+    wide = ParamSet(fh, oracle, 65536, [60] * 3, 1, 0, [2], seed=12)
+    small = ParamSet(fh, oracle, 4096, [60] * 3, 1, 0, [2], seed=13)
+    jobs, exp = [], []
+    for ps_, seed in ((wide, 1), (small, 2)):
+        a = oracle.random_polys(ps_.moduli, ps_.n, 2, seed=seed)
+        b = oracle.random_polys(ps_.moduli, ps_.n, 2, seed=seed + 100)
+        o = fh.empty(ps_.ct_words)
+        jobs.append((ps_.mult, ps_.ek, fh.to_device(a), fh.to_device(b), o, [2]))
+        exp.append(_oracle(ps_, a, b, [2]))
+    fh.submit(jobs)
+    _sync()
+    for j, e in zip(jobs, exp):
+        assert np.array_equal(fh.to_host(j[4]), e)
+
+
 def test_invalid_job_rejected_before_any_work(fh, oracle):
     ps = ParamSet(fh, oracle, 1024, [60] * 3, 1, 0, [1], seed=11)
     a = fh.to_device(oracle.random_polys(ps.moduli, ps.n, 2, seed=1))

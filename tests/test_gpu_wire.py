@@ -30,6 +30,7 @@ CASES = [
     (8192, None, [50, 55, 55], 4),
     (16384, None, [60] * 7, 2),
     (32768, None, [62] * 3, 2),
+    (65536, None, [60] * 2, 2),
 ]
 
 
