@@ -408,8 +408,8 @@ __global__ __launch_bounds__(NttShape<LOGN>::T) void sd_add_ntt_kernel(SdNttArgs
 // (2, 4 or 8, chosen by the autotuner) instances (key_switching_key.rs:227-239): the key values of a digit are
 // loaded once and used for every instance of the group. The switch-down
 // factor q_l^-1 is folded into the key rows of the limbs below the dropped
-// one (fin0, fin1), Y < 2q and the key rows < q, so four products stay below
-// 2^127; their exact sum (column accumulator) gets one Barrett reduction.
+// one (fin0, fin1), Y < 2q and the key rows < q, so seven products stay below
+// 2^128; their exact sum (column accumulator) gets one reduction.
 // Limb i == L (the dropped one, when extra) goes to `special`.
 // grid = (N / 256, L + extra, ceil(instances / KS_MAC_INST))
 template <u32 KS_MAC_INST>
@@ -430,33 +430,37 @@ __global__ __launch_bounds__(256) void ks_mac_kernel(KsFinishArgs f, u32 logn, u
     u64 m0[KS_MAC_INST], m1[KS_MAC_INST];
 #pragma unroll
     for (u32 q = 0; q < KS_MAC_INST; q++) m0[q] = m1[q] = 0;
-    for (u32 d0 = 0; d0 < f.digits; d0 += 4) {
-        const u32 dn = f.digits - d0 < 4 ? f.digits - d0 : 4;
-        u64 kv0[4], kv1[4];
+    // Groups of up to 7 digits: each product is below 2q * q < 2^125, so a
+    // group's exact sum stays below 2^128 (two-column accumulator) and gets
+    // one exact reduction.
+    for (u32 d0 = 0; d0 < f.digits; d0 += 7) {
+        const u32 dn = f.digits - d0 < 7 ? f.digits - d0 : 7;
+        u64 kv0[7], kv1[7];
 #pragma unroll
-        for (u32 d = 0; d < 4; d++) {
+        for (u32 d = 0; d < 7; d++) {
             kv0[d] = d < dn ? k0[(d0 + d) * kstride] : 0;
             kv1[d] = d < dn ? k1[(d0 + d) * kstride] : 0;
         }
 #pragma unroll
         for (u32 q = 0; q < KS_MAC_INST; q++) {
             if (q < nb) {
-                Acc3 s0, s1;
-                acc3_init(s0, 0);
-                acc3_init(s1, 0);
+                Acc2 s0, s1;
+                const u64 y0 = yb[q * yinst + d0 * ystride];
+                acc2_first<false>(s0, y0, kv0[0]);
+                acc2_first<false>(s1, y0, kv1[0]);
 #pragma unroll
-                for (u32 d = 0; d < 4; d++) {
+                for (u32 d = 1; d < 7; d++) {
                     if (d < dn) {
                         const u64 yv = yb[q * yinst + (d0 + d) * ystride];
-                        acc3_mul<false>(s0, yv, kv0[d]);
-                        acc3_mul<false>(s1, yv, kv1[d]);
+                        acc2_mul<false>(s0, yv, kv0[d]);
+                        acc2_mul<false>(s1, yv, kv1[d]);
                     }
                 }
-                u64 l0, h0, t0, l1, h1, t1;
-                acc3_value(s0, l0, h0, t0);
-                acc3_value(s1, l1, h1, t1);
-                m0[q] = add_mod(m0[q], reduce_u128(l0, h0, mi.barrett_lo, mi.barrett_hi, p), p);
-                m1[q] = add_mod(m1[q], reduce_u128(l1, h1, mi.barrett_lo, mi.barrett_hi, p), p);
+                u64 l0, h0, l1, h1;
+                acc2_value(s0, l0, h0);
+                acc2_value(s1, l1, h1);
+                m0[q] = add_mod(m0[q], reduce128_shoup<true>(l0, h0, mi.r64, mi.r64_shoup, mi.w1s, p), p);
+                m1[q] = add_mod(m1[q], reduce128_shoup<true>(l1, h1, mi.r64, mi.r64_shoup, mi.w1s, p), p);
             }
         }
     }

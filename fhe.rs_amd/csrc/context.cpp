@@ -146,6 +146,9 @@ std::shared_ptr<DeviceContext> make_context(const std::vector<u64>& moduli, u32 
         mi.pinv_neg = (u64)0 - pinv;
         mi.n_inv_r = (u64)(((u128)mi.n_inv << 64) % p);
         mi.n_inv_r_shoup = shoup64(mi.n_inv_r, p);
+        mi.r64 = (u64)(((u128)1 << 64) % p);
+        mi.r64_shoup = shoup64(mi.r64, p);
+        mi.w1s = shoup64(1, p);
         set_last_stage(mi, inv[(size_t)l * n + n - 2].x);
         c->h_mod.push_back(mi);
     }

@@ -271,6 +271,9 @@ struct ModInfo {
     // N^-1 2^64): the final scaling folded into that stage.
     u64 last_zn, last_zn_shoup;
     u64 last_zr, last_zr_shoup;
+    // exact reduction of 128-bit sums (reduce128_shoup): [2^64]_p, its Shoup
+    // companion, floor(2^64 / p)
+    u64 r64, r64_shoup, w1s;
 };
 
 }  // namespace fhehip
