@@ -182,6 +182,25 @@ PolyScaler::PolyScaler(std::shared_ptr<DeviceContext> f, std::shared_ptr<DeviceC
     std::vector<u64> blob = scaler_blob(consts, from->h_mod, to->h_mod, common);
     d_consts.reset(blob.size() * sizeof(u64));
     HIPCHK(hipMemcpy(d_consts.ptr, blob.data(), d_consts.bytes, hipMemcpyHostToDevice));
+    if (common < to->L) {
+        const u64 pmax = *std::max_element(from->moduli.begin(), from->moduli.end());
+        const std::vector<u64> t = scaler_mfma_blob(blob, pmax, mfma);
+        if (!t.empty()) {
+            d_mfma.reset(t.size() * sizeof(u64));
+            HIPCHK(hipMemcpy(d_mfma.ptr, t.data(), d_mfma.bytes, hipMemcpyHostToDevice));
+        }
+    }
+}
+
+void PolyScaler::scale_rows(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst,
+                            size_t out_part, u32 parts, u32 inst, bool reduced, hipStream_t s, const char* name) const {
+    const size_t n = from->N;
+    if (reduced && d_mfma.ptr && fused_tuning(from->logn).scaler_mfma &&
+        launch_rns_scale_mfma(in, in_inst, in_part, out, out_inst, out_part, d_consts.as<u64>(), d_mfma.as<u64>(),
+                              mfma, (u32)n, parts, inst, s, name))
+        return;
+    launch_rns_scale(in, in_inst, in_part, out, out_inst, out_part, d_consts.as<u64>(), words(), from->L, (u32)n,
+                     parts, inst, s, name);
 }
 
 ScalePairArgs PolyScaler::pair(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst,
@@ -228,8 +247,9 @@ void PolyScaler::scale(const u64* in, u64* out, u32 polys, bool ntt_repr, hipStr
         intt_rows(*from, a, from->L, 1, polys, s);
         src = tmp;
     }
-    launch_rns_scale(src, isz, 0, out + (size_t)common * n, osz, 0, d_consts.as<u64>(), words(), from->L, (u32)n, 1,
-                     polys, s);
+    // the inverse transform's outputs are reduced; power-basis inputs are
+    // taken as given
+    scale_rows(src, isz, 0, out + (size_t)common * n, osz, 0, 1, polys, ntt_repr, s, "rns_scale");
     if (ntt_repr) {
         RowMap m{out, to->L - common, to->L, common, common};
         launch_ntt_forward(*to, m, polys * (to->L - common), false, s);
@@ -683,9 +703,8 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out_final, u32
         if (in_open || out_fwd)
             launch_scale_pair(sc.pair(in, 4 * L * n, L * n, o, 4 * xr * n, xr * n, in_open, out_fwd), (u32)L, parts,
                               inst, s, "rns_extend");
-        else
-            launch_rns_scale(in, 4 * L * n, L * n, o, 4 * xr * n, xr * n, sc.d_consts.as<u64>(), sc.words(), (u32)L,
-                             (u32)n, parts, inst, s, "rns_extend");
+        else  // inverse transform outputs: reduced
+            sc.scale_rows(in, 4 * L * n, L * n, o, 4 * xr * n, xr * n, parts, inst, true, s, "rns_extend");
     };
     if (ext_same) {
         extend(*ext_lhs, 0, 4);
@@ -728,9 +747,8 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out_final, u32
     if (t_open || d_fwd)
         launch_scale_pair(down->pair(wT, 3 * M * n, M * n, dst, 3 * L * n, L * n, t_open, d_fwd), (u32)M, 3, inst, s,
                           "rns_down");
-    else
-        launch_rns_scale(wT, 3 * M * n, M * n, dst, 3 * L * n, L * n, down->d_consts.as<u64>(), down->words(),
-                         (u32)M, (u32)n, 3, inst, s, "rns_down");
+    else  // inverse transform outputs: reduced
+        down->scale_rows(wT, 3 * M * n, M * n, dst, 3 * L * n, L * n, 3, inst, true, s, "rns_down");
     if (!rk) {
         RowMap m{out, (u32)L, (u32)L, 0, 0};
         if (d_fwd) launch_ntt_forward_halves(*base, m, inst * 3 * (u32)L, false, s);
@@ -747,7 +765,7 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out_final, u32
 }
 
 // ---------------------------------------------------------------------------
-FusedTuning autotune_multiply(Multiplicator& m, u32 inst, float (&ms)[17]) {
+FusedTuning autotune_multiply(Multiplicator& m, u32 inst, float (&ms)[19]) {
     check_batch(inst);
     if (!inst) throw Error(-1, "the autotuner needs at least one instance");
     if (wide(*m.base)) throw Error(-2, "the autotuner covers degrees 8 .. 32768 (the fused row kernels)");
@@ -804,6 +822,7 @@ FusedTuning autotune_multiply(Multiplicator& m, u32 inst, float (&ms)[17]) {
         knob(&FusedTuning::re_tensor, {4, 5});
         knob(&FusedTuning::re_lift, {4, 5});
         knob(&FusedTuning::re_finish, {4, 5});
+        knob(&FusedTuning::scaler_mfma, {0, 1});
         // each stage's half-row choice in turn
         for (int bit : {HALVES_FINISH, HALVES_INPUT, HALVES_TENSOR}) {
             float t[2];

@@ -3,6 +3,7 @@
 
 #include "bfv_launch.h"
 #include "rns_device.h"
+#include "scale_mfma.h"
 #include "tuning.h"
 
 #include <map>
@@ -88,6 +89,15 @@ struct PolyScaler {
     ScalerConsts consts;
     DevBuf d_consts;  // outputs start .. to->L
     u32 words() const { return (u32)(d_consts.bytes / sizeof(u64)); }
+    // Digit tables of the matrix-core form (empty when the shape has none).
+    DevBuf d_mfma;
+    ScaleMfmaLayout mfma;
+    // The scaling of `parts` power-basis polynomials per instance (rows of
+    // part k of instance b at in + b in_inst + k in_part, likewise out) on
+    // the matrix cores when the inputs are reduced residues, the shape has
+    // that form and the degree's tuning selects it, else on the VALU.
+    void scale_rows(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst, size_t out_part,
+                    u32 parts, u32 inst, bool reduced, hipStream_t s, const char* name) const;
     // A paired launch (launch_scale_pair) of these constants; in_open: the
     // inputs are half-row inverse transforms, out_fwd: the outputs get the
     // first forward stage of the destination context.
@@ -214,7 +224,7 @@ struct Parameters {
 // starting from the current choice, keeps the fastest for the degree and
 // returns it. ms: milliseconds per multiply for xcd 0/1, ks_mac instances
 // 2/4/8, tensor 4/5, lift 4/5, finish 4/5 log2 values per thread.
-FusedTuning autotune_multiply(Multiplicator& m, u32 inst, float (&ms)[17]);
+FusedTuning autotune_multiply(Multiplicator& m, u32 inst, float (&ms)[19]);
 
 // fhehip_multiply with all scratch in the library's fixed device region
 // (no device allocation); refuses batches whose scratch does not fit.

@@ -91,6 +91,13 @@ constexpr size_t kScaleLdsMaxBlocks = 1024;
 void launch_rns_scale(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst, size_t out_part,
                       const u64* d_sc, u32 sc_words, u32 nf, u32 n, u32 parts, u32 inst, hipStream_t s,
                       const char* name = "rns_scale");
+// The same on the matrix cores (scale_mfma.hip) with the digit tables d_mt
+// of layout `lay`, for inputs that are reduced residues; false (nothing
+// launched) when the shape or the degree has no matrix-core form.
+struct ScaleMfmaLayout;
+bool launch_rns_scale_mfma(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst,
+                           size_t out_part, const u64* d_sc, const u64* d_mt, const ScaleMfmaLayout& lay, u32 n,
+                           u32 parts, u32 inst, hipStream_t s, const char* name);
 // Scaling of rows transformed as two halves (scale_pair.hip): rows of part k
 // of instance b at in + b in_inst + k in_part (likewise out), degree n >= 512.
 struct ScalePairArgs {

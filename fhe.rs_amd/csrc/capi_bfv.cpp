@@ -276,12 +276,13 @@ static FusedTuning from_c(const fhehip_tuning& t) {
     f.re_lift = (int)t.lift_log2_values;
     f.re_finish = (int)t.finish_log2_values;
     f.halves = (int)t.halves;
+    f.scaler_mfma = (int)t.scaler_mfma;
     return f;
 }
 
 static fhehip_tuning to_c(const FusedTuning& f) {
-    return fhehip_tuning{(uint32_t)f.xcd,       (uint32_t)f.ks_mac_inst, (uint32_t)f.re_tensor,
-                         (uint32_t)f.re_lift,   (uint32_t)f.re_finish,   (uint32_t)f.halves};
+    return fhehip_tuning{(uint32_t)f.xcd,     (uint32_t)f.ks_mac_inst, (uint32_t)f.re_tensor, (uint32_t)f.re_lift,
+                         (uint32_t)f.re_finish, (uint32_t)f.halves,     (uint32_t)f.scaler_mfma};
 }
 
 static int degree_log2(uint32_t degree) {
@@ -292,7 +293,7 @@ static int degree_log2(uint32_t degree) {
 int fhehip_multiply_autotune(fhehip_multiplicator* m, uint32_t num, fhehip_tuning* chosen, float* ms_out) {
     FHEHIP_API_BEGIN
     if (!m) return fail(FHEHIP_ERR_INVALID_ARG, "null argument");
-    float ms[17] = {};
+    float ms[19] = {};
     const FusedTuning t = autotune_multiply(*m->m, num, ms);
     if (chosen) *chosen = to_c(t);
     if (ms_out) std::memcpy(ms_out, ms, sizeof ms);

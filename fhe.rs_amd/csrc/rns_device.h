@@ -343,6 +343,27 @@ __device__ __forceinline__ u128 u256_shr_lo128(const U256& a, unsigned s) {
     return ((u128)r1 << 64) | r0;
 }
 
+// v = ceil((S >> (shift - 1)) / 2) from the low 128 bits of the shifted
+// sum S = sum_j r_j theta_garner_j.
+__device__ __forceinline__ u128 sc_round_v(const U256& sum, unsigned shift) {
+    const u128 vv = u256_shr_lo128(sum, shift - 1);
+    return (vv >> 1) + (vv & 1);
+}
+
+// The magnitude of w = round(so / 2^127) with the reference's rounding of
+// each sign (so is a signed 256-bit value), and its sign.
+__device__ __forceinline__ u128 sc_round_w(const U256& so, bool& w_sign) {
+    w_sign = (so.w[3] >> 63) != 0;
+    if (w_sign) {
+        U256 nso;
+#pragma unroll
+        for (int i = 0; i < 4; i++) nso.w[i] = ~so.w[i];
+        return (u256_shr_lo128(nso, 126) + 1) >> 1;
+    }
+    const u128 ww = u256_shr_lo128(so, 126);
+    return (ww >> 1) + (ww & 1);
+}
+
 // A small header field of the constant block as a wave-uniform value: read
 // through the scalar cache (SW), or from a copy in LDS and moved to an SGPR.
 template <bool SW>
@@ -401,8 +422,7 @@ __device__ __forceinline__ ScColumn sc_column(const u64* __restrict__ sc, Rests 
         acc3_value(sb, b0, b1, b2);
         u256_combine(sum, a0, a1, a2, b0, b1, b2);
     }
-    u128 vv = u256_shr_lo128(sum, h.shift - 1);
-    const u128 v = (vv >> 1) + (vv & 1);
+    const u128 v = sc_round_v(sum, h.shift);
 
     bool w_sign = false;
     u128 w = 0;
@@ -458,17 +478,7 @@ __device__ __forceinline__ ScColumn sc_column(const u64* __restrict__ sc, Rests 
             if (sc_uniform<SW>(sc[SC_TG_SIGN])) u256_add(so, vt);
             else u256_sub(so, vt);
         }
-        w_sign = (so.w[3] >> 63) != 0;
-        if (w_sign) {
-            U256 nso;
-#pragma unroll
-            for (int i = 0; i < 4; i++) nso.w[i] = ~so.w[i];
-            w = u256_shr_lo128(nso, 126) + 1;
-            w >>= 1;
-        } else {
-            u128 ww = u256_shr_lo128(so, 126);
-            w = (ww >> 1) + (ww & 1);
-        }
+        w = sc_round_w(so, w_sign);
     }
     return ScColumn{(u64)v, (u64)(v >> 64), (u64)w, (u64)(w >> 64), w_sign};
 }

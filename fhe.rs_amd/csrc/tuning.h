@@ -22,6 +22,10 @@ struct FusedTuning {
     // scaler), HALVES_TENSOR the tensor product + inverse transform and the
     // down-scaling (likewise).
     int halves = HALVES_FINISH;
+    // The column sums of the extension and down-scaling on the matrix cores
+    // (scale_mfma.hip) where the scaler's shape allows it (1), or the
+    // per-column VALU kernels (0).
+    int scaler_mfma = 1;
 };
 
 // The current choice for degree 2^logn (the defaults until set).

@@ -371,11 +371,13 @@ class Multiplicator:
 class TuningC(C.Structure):
     """fhehip_tuning of include/fhehip.h."""
     _fields_ = [("xcd_order", C.c_uint32), ("ks_mac_instances", C.c_uint32), ("tensor_log2_values", C.c_uint32),
-                ("lift_log2_values", C.c_uint32), ("finish_log2_values", C.c_uint32), ("halves", C.c_uint32)]
+                ("lift_log2_values", C.c_uint32), ("finish_log2_values", C.c_uint32), ("halves", C.c_uint32),
+                ("scaler_mfma", C.c_uint32)]
 
 
 TUNING_TIMES = ["xcd0", "xcd1", "ks_mac2", "ks_mac4", "ks_mac8", "tensor4", "tensor5", "lift4", "lift5", "finish4",
-                "finish5", "finish_rows", "finish_halves", "input_rows", "input_halves", "tensor_rows", "tensor_halves"]
+                "finish5", "scaler_valu", "scaler_mfma", "finish_rows", "finish_halves", "input_rows", "input_halves",
+                "tensor_rows", "tensor_halves"]
 
 
 def tuning_get(degree):

@@ -189,8 +189,11 @@ int fhehip_multiply(fhehip_multiplicator* m, const uint64_t* d_lhs, const uint64
  * fhehip_split_row_limit rows do): 1 the key-switch finish, 2 the
  * multiplication's input inverse transforms, base extension and forward
  * transform, 4 the tensor product + inverse transform and the t/Q scaling
- * (2 and 4 pair the halves inside the scalers). Every choice gives the same,
- * exact results. */
+ * (2 and 4 pair the halves inside the scalers); whether the base extension
+ * and t/Q scaling of unpaired rows form their column sums on the matrix
+ * cores (scaler_mfma 1: 8-bit integer MFMA over base-256 digits, for up to 16
+ * source moduli and 8 outputs from N = 256; otherwise, or with 0, one thread
+ * per column on the VALU). Every choice gives the same, exact results. */
 typedef struct fhehip_tuning {
     uint32_t xcd_order;
     uint32_t ks_mac_instances;
@@ -198,14 +201,15 @@ typedef struct fhehip_tuning {
     uint32_t lift_log2_values;
     uint32_t finish_log2_values;
     uint32_t halves;
+    uint32_t scaler_mfma;
 } fhehip_tuning;
 /* Times fhehip_multiply on `num` instances (pseudorandom residues, a
  * private stream) under every choice of each knob in turn, starting from the
  * degree's current choice, keeps the fastest for the multiplicator's degree
- * (8 .. 32768) and writes it to *chosen; ms_out (may be NULL) receives 17
+ * (8 .. 32768) and writes it to *chosen; ms_out (may be NULL) receives 19
  * milliseconds per multiply: xcd_order 0, 1; ks_mac_instances 2, 4, 8;
- * tensor 4, 5; lift 4, 5; finish 4, 5; then halves bit 1 off, on; bit 2 off,
- * on; bit 4 off, on. Synchronous. Multiplications of the same degree running
+ * tensor 4, 5; lift 4, 5; finish 4, 5; scaler_mfma 0, 1; then halves bit 1
+ * off, on; bit 2 off, on; bit 4 off, on. Synchronous. Multiplications of the same degree running
  * meanwhile on other threads may use the trial choices. */
 int fhehip_multiply_autotune(fhehip_multiplicator* m, uint32_t num, fhehip_tuning* chosen, float* ms_out);
 /* Sets or reads the choice for one degree directly. */

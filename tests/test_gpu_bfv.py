@@ -48,6 +48,53 @@ def test_scaler_extension_and_down(fh, oracle, n):
         assert np.array_equal(fh.to_host(d_out2), oracle.PolyScaler(om, oq, 65537, Q).scale(b, ntt))
 
 
+# source moduli counts of the ciphertext part and the extension part: the
+# extension scales nq -> nq + ne by one, the down-scaling nq + ne -> nq by
+# t / Q; at most 8 source moduli take MFMA steps of 16 k, more take 32
+@pytest.mark.parametrize("nq,ne", [(1, 2), (3, 4), (6, 7), (8, 8), (5, 4), (2, 7)])
+def test_scaler_matrix_cores(fh, oracle, nq, ne):
+    """The scaler's matrix-core form (8-bit integer MFMA over base-256
+    digits), taken by reduced inputs (the inverse transform's outputs of
+    NTT-domain polynomials), against the oracle and against the per-column
+    VALU form. Power-basis columns 0, 1 and 2 hold all zeros, all p - 1 and
+    alternating zeros and p - 1."""
+    n = 4096
+    print(f"scaler matrix cores nq={nq} ne={ne}", flush=True)
+    q = oracle.generate_moduli([60] * nq, n)
+    ext = [m for m in oracle.generate_moduli([62] * (nq + ne), n) if m not in q][:ne]
+    full = q + ext
+    oq, of = oracle.Context(q, n), oracle.Context(full, n)
+    cq, cf = fh.Context(q, n), fh.Context(full, n)
+    Q = 1
+    for x in q:
+        Q *= x
+
+    def edge_ntt(moduli, octx, seed):
+        a = oracle.random_polys(moduli, n, 3, seed=seed).reshape(3, len(moduli), n)
+        for i, p in enumerate(moduli):
+            a[:, i, 0] = 0
+            a[:, i, 1] = p - 1
+            a[:, i, 2] = p - 1 if i % 2 else 0
+        return octx.forward(a.reshape(-1))
+
+    cases = [(cq, cf, oq, of, None, None, q), (cf, cq, of, oq, 65537, Q, full)]
+    prev = fh.tuning_get(n)
+    try:
+        for fc, tc, fo, to, num, den, src in cases:
+            a = edge_ntt(src, fo, seed=nq * 17 + ne + (num or 0))
+            want = oracle.PolyScaler(fo, to, num, den).scale(a, True) if num else \
+                oracle.PolyScaler(fo, to).scale(a, True)
+            sc = fh.Scaler(fc, tc, num, den) if num else fh.Scaler(fc, tc)
+            for mfma in (1, 0):
+                fh.tuning_set(n, scaler_mfma=mfma)
+                out = fh.empty(3 * tc.L * n)
+                sc.scale(fh.to_device(a), out, 3, True)
+                _sync()
+                assert np.array_equal(fh.to_host(out), want), (num, mfma)
+    finally:
+        fh.tuning_set(n, **prev)
+
+
 def test_scaler_general_factors(fh, oracle):
     """The reference's own scaler test grid (math/src/rns/scaler.rs:379-473) on
     polynomials, both directions of context size."""

@@ -1,7 +1,8 @@
 """GPU tests of the multiplication autotuner's launch shapes: every choice
 of every knob (XCD-aware workgroup order, instances per key load in the key
 inner product, values per thread of the fused tensor, lift and finish
-kernels, half-row transforms for launches of any size) gives results
+kernels, half-row transforms for launches of any size, the scalers' column
+sums on the matrix cores or the VALU) gives results
 bit-exact to the oracle, and the autotuner returns a
 valid choice with its timings."""
 import random
@@ -14,9 +15,10 @@ pytestmark = pytest.mark.gpu
 T = 65537
 KNOBS = [("xcd_order", 0), ("xcd_order", 1), ("ks_mac_instances", 2), ("ks_mac_instances", 4),
          ("ks_mac_instances", 8), ("tensor_log2_values", 5), ("lift_log2_values", 5), ("finish_log2_values", 5),
-         ("halves", 0), ("halves", 1), ("halves", 2), ("halves", 4), ("halves", 7)]
+         ("halves", 0), ("halves", 1), ("halves", 2), ("halves", 4), ("halves", 7), ("scaler_mfma", 0),
+         ("scaler_mfma", 1)]
 DEFAULT = {"xcd_order": 1, "ks_mac_instances": 4, "tensor_log2_values": 4, "lift_log2_values": 4,
-           "finish_log2_values": 4, "halves": 1}
+           "finish_log2_values": 4, "halves": 1, "scaler_mfma": 1}
 
 
 @pytest.fixture(scope="module")
@@ -91,7 +93,7 @@ def test_autotune_returns_a_valid_choice(fh, oracle, restore):
     chosen, ms = fh.multiply_autotune(mult, 16)
     assert chosen["xcd_order"] in (0, 1) and chosen["ks_mac_instances"] in (2, 4, 8)
     assert all(chosen[k] in (4, 5) for k in ("tensor_log2_values", "lift_log2_values", "finish_log2_values"))
-    assert 0 <= chosen["halves"] <= 7
+    assert 0 <= chosen["halves"] <= 7 and chosen["scaler_mfma"] in (0, 1)
     assert all(v > 0 for v in ms.values()), ms
     assert fh.tuning_get(n) == chosen
     lhs = oracle.random_polys(moduli, n, 6, seed=5)
