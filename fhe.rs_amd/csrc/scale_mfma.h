@@ -55,7 +55,8 @@ struct ScaleMfmaLayout {
 // whose inputs are reduced residues of source moduli up to pmax, for MFMA
 // steps of kw k values (16 or 32; any other value: 32 above 8 source
 // moduli, else 16); empty when the shape has no matrix-core form (more
-// than 16 source moduli, more than 8 outputs, or pmax of 2^62 or more).
+// than 16 source moduli, more than 8 outputs, pmax of 2^62 or more, or a
+// theta_garner shift outside 65 .. 128).
 std::vector<u64> scaler_mfma_blob(const std::vector<u64>& sc, u64 pmax, ScaleMfmaLayout& lay, int kw = 0);
 
 }  // namespace fhehip

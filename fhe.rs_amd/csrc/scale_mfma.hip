@@ -168,6 +168,23 @@ __device__ __forceinline__ U256 combine32(const i32x16& c0, const i32x16& c1) {
 // supplies, per K step kappa, the residues j = (KW / 8) kappa + (KW / 16)
 // (l / 32) .. + KW / 16 - 1 of columns l % 32 and 32 + l % 32 (zero from
 // nf on).
+// sc_round_v for a shift of 65 .. 128 (the table builder declines others):
+// the shifted bits come from words 1 .. 3 of the sum.
+__device__ __forceinline__ u128 round_v(const U256& sum, unsigned shift) {
+    const unsigned bs = shift - 65;  // 0 .. 63
+    const u64 r0 = bs ? (sum.w[1] >> bs) | (sum.w[2] << (64 - bs)) : sum.w[1];
+    const u64 r1 = bs ? (sum.w[2] >> bs) | (sum.w[3] << (64 - bs)) : sum.w[2];
+    return (((u128)r1 << 64 | r0) >> 1) + (r0 & 1);
+}
+
+// a w mod p in [0, 2p) for a below 2^32 (Shoup's product with ws =
+// floor(w 2^64 / p), its quotient estimate below 2^32).
+__device__ __forceinline__ u64 mul_shoup32(u32 a, u64 w, u64 ws, u64 p) {
+    const u64 t = (u64)a * (u32)ws;
+    const u32 q = (u32)(((u64)a * (u32)(ws >> 32) + (t >> 32)) >> 32);
+    return (u64)a * w - (u64)q * p;
+}
+
 template <int KW, int KS1, int NTO, bool HAS_W>
 __device__ __forceinline__ void rns_scale_mfma_cols(const u64* __restrict__ in, u64* __restrict__ out,
                                                     const u64* __restrict__ sc, const u64* tab, u32 n, int nf,
@@ -213,7 +230,7 @@ __device__ __forceinline__ void rns_scale_mfma_cols(const u64* __restrict__ in, 
     // S -> v
     pass1(0);
     own_column(c0, c1);
-    const u128 v = sc_round_v(combine32(c0, c1), hd.shift);
+    const u128 v = round_v(combine32(c0, c1), hd.shift);
     u64 v0, v1, w0 = 0, w1 = 0;
     digits128((u64)v, (u64)(v >> 64), v0, v1);
     // W (residue part, then v times theta_gamma) -> signed w
@@ -268,9 +285,11 @@ __device__ __forceinline__ void rns_scale_mfma_cols(const u64* __restrict__ in, 
             acc = mad_i64(c1[4 * s + 3], 16777216, acc);
             unsigned long long cy;
             const u64 lo = __builtin_addcll((u64)lo32 | ((u64)(u32)acc << 32), tab[AM + 2 * i], 0, &cy);
-            const u64 hi = (u64)(acc >> 32) + tab[AM + 2 * i + 1] + cy;
+            const u32 hi = (u32)((u64)(acc >> 32) + tab[AM + 2 * i + 1] + cy);  // X + M < 2^80
             const u64* o = outc + (size_t)i * sc_out_stride(nf);
-            dst[(size_t)i * n] = reduce128_shoup<true>(lo, hi, o[5], o[7], o[8], o[0]);
+            const u64 q = o[0];
+            const u64 x = lazy_reduce_shoup1<true>(lo, o[8], q) + mul_shoup32(hi, o[5], o[7], q);
+            dst[(size_t)i * n] = reduce1(reduce1(x, 2 * q), q);
         }
     }
 }
@@ -321,10 +340,10 @@ bool launch_rns_scale_mfma(const u64* in, size_t in_inst, size_t in_part, u64* o
         return false;
     if (!parts || !inst) return true;
     ProfScope ps(name, 31 - __builtin_clz(n), s);
-    // Each workgroup stages the tables once for up to 4 runs of 256 columns,
+    // Each workgroup stages the tables once for up to 8 runs of 256 columns,
     // as long as the launch keeps at least 2048 workgroups.
     u32 chunks = 1;
-    while (chunks < 4 && (n / 256) % (2 * chunks) == 0 && (size_t)(n / 256 / (2 * chunks)) * parts * inst >= 2048)
+    while (chunks < 8 && (n / 256) % (2 * chunks) == 0 && (size_t)(n / 256 / (2 * chunks)) * parts * inst >= 2048)
         chunks *= 2;
     const dim3 grid(n / 256 / chunks, parts, inst);
     const size_t lds = lay.words() * sizeof(u64);

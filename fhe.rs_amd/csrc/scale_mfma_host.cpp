@@ -62,7 +62,9 @@ void conv_rows(Tile& t, int k0, const int8_t* c, int nc, int na) {
 std::vector<u64> scaler_mfma_blob(const std::vector<u64>& sc, u64 pmax, ScaleMfmaLayout& lay, int kw) {
     lay = ScaleMfmaLayout{};
     const int nf = (int)sc[SC_NF], count = (int)sc[SC_COUNT];
-    if (nf < 1 || nf > 16 || count < 1 || count > 8 || pmax >= ((u64)1 << 62)) return {};
+    if (nf < 1 || nf > 16 || count < 1 || count > 8 || pmax >= ((u64)1 << 62) || sc[SC_SHIFT] < 65 ||
+        sc[SC_SHIFT] > 128)
+        return {};
     lay.nf = nf;
     lay.count = count;
     if (kw != 16 && kw != 32) kw = nf > 8 ? 32 : 16;
