@@ -192,6 +192,14 @@ PolyScaler::PolyScaler(std::shared_ptr<DeviceContext> f, std::shared_ptr<DeviceC
     }
 }
 
+void PolyScaler::scale_pair_rows(const ScalePairArgs& a, u32 parts, u32 inst, hipStream_t s,
+                                 const char* name) const {
+    if (d_mfma.ptr && fused_tuning(from->logn).scaler_mfma &&
+        launch_rns_scale_mfma_pair(a, d_mfma.as<u64>(), mfma, parts, inst, s, name))
+        return;
+    launch_scale_pair(a, from->L, parts, inst, s, name);
+}
+
 void PolyScaler::scale_rows(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst,
                             size_t out_part, u32 parts, u32 inst, bool reduced, hipStream_t s, const char* name) const {
     const size_t n = from->N;
@@ -701,8 +709,8 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out_final, u32
         const u64* in = wP + k0 * L * n;
         u64* o = wX + k0 * xr * n;
         if (in_open || out_fwd)
-            launch_scale_pair(sc.pair(in, 4 * L * n, L * n, o, 4 * xr * n, xr * n, in_open, out_fwd), (u32)L, parts,
-                              inst, s, "rns_extend");
+            sc.scale_pair_rows(sc.pair(in, 4 * L * n, L * n, o, 4 * xr * n, xr * n, in_open, out_fwd), parts, inst, s,
+                               "rns_extend");
         else  // inverse transform outputs: reduced
             sc.scale_rows(in, 4 * L * n, L * n, o, 4 * xr * n, xr * n, parts, inst, true, s, "rns_extend");
     };
@@ -745,8 +753,8 @@ void Multiplicator::multiply(const u64* lhs, const u64* rhs, u64* out_final, u32
     u64* dst = rk ? wD : out;
     const bool d_fwd = !rk && split_rows(*base, (size_t)inst * 3 * L);
     if (t_open || d_fwd)
-        launch_scale_pair(down->pair(wT, 3 * M * n, M * n, dst, 3 * L * n, L * n, t_open, d_fwd), (u32)M, 3, inst, s,
-                          "rns_down");
+        down->scale_pair_rows(down->pair(wT, 3 * M * n, M * n, dst, 3 * L * n, L * n, t_open, d_fwd), 3, inst, s,
+                              "rns_down");
     else  // inverse transform outputs: reduced
         down->scale_rows(wT, 3 * M * n, M * n, dst, 3 * L * n, L * n, 3, inst, true, s, "rns_down");
     if (!rk) {

@@ -98,6 +98,9 @@ struct PolyScaler {
     // that form and the degree's tuning selects it, else on the VALU.
     void scale_rows(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst, size_t out_part,
                     u32 parts, u32 inst, bool reduced, hipStream_t s, const char* name) const;
+    // A paired launch (pair()) likewise; its inputs are reduced residues, or
+    // open halves below 2p with the inverse stage.
+    void scale_pair_rows(const ScalePairArgs& a, u32 parts, u32 inst, hipStream_t s, const char* name) const;
     // A paired launch (launch_scale_pair) of these constants; in_open: the
     // inputs are half-row inverse transforms, out_fwd: the outputs get the
     // first forward stage of the destination context.

@@ -117,6 +117,11 @@ struct ScalePairArgs {
     const ulonglong2* out_split;
 };
 void launch_scale_pair(const ScalePairArgs& a, u32 nf, u32 parts, u32 inst, hipStream_t s, const char* name);
+// The same on the matrix cores (scale_mfma.hip), for inputs that are
+// reduced residues (or open halves below 2p, with in_split); false (nothing
+// launched) when the shape or the degree has no matrix-core form.
+bool launch_rns_scale_mfma_pair(const ScalePairArgs& a, const u64* d_mt, const ScaleMfmaLayout& lay, u32 parts,
+                                u32 inst, hipStream_t s, const char* name);
 void launch_intt_rows(int logn, const InttRowsArgs& a, u32 limbs, u32 parts, u32 inst, hipStream_t s);
 void launch_tensor_intt(int logn, const TensorArgs& a, u32 M, u32 inst, hipStream_t s);
 // Degree 32768 (`c` the context): the same with the two half-row inverse

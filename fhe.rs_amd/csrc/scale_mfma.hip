@@ -164,10 +164,6 @@ __device__ __forceinline__ U256 combine32(const i32x16& c0, const i32x16& c1) {
     return r;
 }
 
-// One wave's 64 columns from col0 (the tables already in LDS). Lane l
-// supplies, per K step kappa, the residues j = (KW / 8) kappa + (KW / 16)
-// (l / 32) .. + KW / 16 - 1 of columns l % 32 and 32 + l % 32 (zero from
-// nf on).
 // sc_round_v for a shift of 65 .. 128 (the table builder declines others):
 // the shifted bits come from words 1 .. 3 of the sum.
 __device__ __forceinline__ u128 round_v(const U256& sum, unsigned shift) {
@@ -185,10 +181,36 @@ __device__ __forceinline__ u64 mul_shoup32(u32 a, u64 w, u64 ws, u64 p) {
     return (u64)a * w - (u64)q * p;
 }
 
-template <int KW, int KS1, int NTO, bool HAS_W>
+// The stages pairing two half-row transforms (PAIR): column block 1 is
+// block 0 shifted by N/2, so lanes l and l + 32 hold the columns c and
+// c + N/2 of a pair. isp (optional): per source limb j, words 6 j .. 6 j + 5
+// = (w, w', z, z', n^-1, n^-1') of the last inverse stage, applied on load;
+// osp (optional): per output i, words 6 i, 6 i + 1 = (w, w') of the first
+// forward stage, applied on store.
+struct PairStages {
+    const u64* isp;
+    const ModInfo* in_mods;
+    const u64* osp;
+};
+
+// X = (x + y) n^-1 at c, Y = (x - y) zeta n^-1 at c + N/2 for x at c and y
+// at c + N/2 below 2p (native.rs:288-300, 130-131); both reduced.
+__device__ __forceinline__ void pair_inverse(u64& x, u64& y, const PairStages& ps, int j) {
+    const u64 p = ps.in_mods[j].p;
+    const u64* w = ps.isp + 6 * j;
+    const u64 a = x + y, b = x + 2 * p - y;
+    x = mul_shoup(a, w[4], w[5], p);
+    y = mul_shoup(b, w[2], w[3], p);
+}
+
+// One wave's 64 columns (the tables already in LDS): block 0 from col0,
+// block 1 from col0 + off1 (32, or N/2 for a pair). Lane l supplies, per K
+// step kappa, the residues j = (KW / 8) kappa + (KW / 16) (l / 32) .. +
+// KW / 16 - 1 of column l % 32 of each block (zero from nf on).
+template <int KW, int KS1, int NTO, bool HAS_W, bool PAIR>
 __device__ __forceinline__ void rns_scale_mfma_cols(const u64* __restrict__ in, u64* __restrict__ out,
                                                     const u64* __restrict__ sc, const u64* tab, u32 n, int nf,
-                                                    u32 col0) {
+                                                    u32 col0, u32 off1, const PairStages& ps) {
     constexpr int WL = KW / 16, KS3 = KW == 32 ? 1 : (HAS_W ? 2 : 1);
     constexpr int T1 = 1 + (HAS_W ? 1 : 0) + NTO;
     constexpr int A2 = T1 * KS1 * 64 * WL, A3 = A2 + (HAS_W ? 64 * WL : 0), AM = A3 + NTO * KS3 * 64 * WL;
@@ -201,7 +223,8 @@ __device__ __forceinline__ void rns_scale_mfma_cols(const u64* __restrict__ in, 
         u64 r[4] = {0, 0, 0, 0};
         if (j < nf) {
             r[0] = src[(size_t)j * n];
-            r[1] = src[(size_t)j * n + 32];
+            r[1] = src[(size_t)j * n + off1];
+            if (PAIR && ps.isp) pair_inverse(r[0], r[1], ps, j);
         }
         if constexpr (KW == 16) {
             b0[k] = (long)digits64(r[0]);
@@ -209,7 +232,8 @@ __device__ __forceinline__ void rns_scale_mfma_cols(const u64* __restrict__ in, 
         } else {
             if (j + 1 < nf) {
                 r[2] = src[(size_t)(j + 1) * n];
-                r[3] = src[(size_t)(j + 1) * n + 32];
+                r[3] = src[(size_t)(j + 1) * n + off1];
+                if (PAIR && ps.isp) pair_inverse(r[2], r[3], ps, j + 1);
             }
             b0[k] = pack16(digits64(r[0]), digits64(r[2]));
             b1[k] = pack16(digits64(r[1]), digits64(r[3]));
@@ -256,7 +280,8 @@ __device__ __forceinline__ void rns_scale_mfma_cols(const u64* __restrict__ in, 
     } else {
         value_operands(pack16(v0, v1), pack16(w0, w1), ob0[0], ob1[0]);
     }
-    u64* dst = out + col0 + lane;
+    const int side = lane >> 5;
+    u64* dst = out + col0 + (side ? off1 + lane - 32 : lane);
     const u64* outc = sc + sc_out_base(nf);
     const int count = hd.count;
 #pragma unroll
@@ -289,53 +314,87 @@ __device__ __forceinline__ void rns_scale_mfma_cols(const u64* __restrict__ in, 
             const u64* o = outc + (size_t)i * sc_out_stride(nf);
             const u64 q = o[0];
             const u64 x = lazy_reduce_shoup1<true>(lo, o[8], q) + mul_shoup32(hi, o[5], o[7], q);
-            dst[(size_t)i * n] = reduce1(reduce1(x, 2 * q), q);
+            u64 y = reduce1(reduce1(x, 2 * q), q);
+            if (PAIR && ps.osp) {
+                // Harvey forward butterfly of reduced inputs (native.rs:256-269),
+                // outputs < 4q: the lane at c + N/2 forms t = w y and sends it,
+                // the lane at c sends its y
+                const u64 t = lazy_mul_shoup_u(y, ps.osp[6 * i], ps.osp[6 * i + 1], q);
+                const u64 send = side ? t : y;
+                const auto lo2 = __builtin_amdgcn_permlane32_swap((unsigned)send, (unsigned)send, false, false);
+                const auto hi2 =
+                    __builtin_amdgcn_permlane32_swap((unsigned)(send >> 32), (unsigned)(send >> 32), false, false);
+                const u64 got = side ? ((u64)hi2[0] << 32 | lo2[0]) : ((u64)hi2[1] << 32 | lo2[1]);
+                y = side ? got + 2 * q - t : y + got;
+            }
+            dst[(size_t)i * n] = y;
         }
     }
 }
 
-template <int KW, int KS1, int NTO, bool HAS_W>
+template <int KW, int KS1, int NTO, bool HAS_W, bool PAIR>
 __global__ __launch_bounds__(256) void rns_scale_mfma_kernel(const u64* __restrict__ in, size_t in_inst,
                                                              size_t in_part, u64* __restrict__ out, size_t out_inst,
                                                              size_t out_part, const u64* __restrict__ sc,
                                                              const u64* __restrict__ mt, u32 mt_words, u32 n,
-                                                             u32 chunks) {
+                                                             u32 chunks, const u64* __restrict__ isp,
+                                                             const ModInfo* __restrict__ in_mods,
+                                                             const u64* __restrict__ osp) {
     extern __shared__ u64 tab[];
     for (u32 i = threadIdx.x; i < mt_words; i += 256) tab[i] = mt[i];
     __syncthreads();
     const int nf = (int)sc[SC_NF];
     const u64* src = in + blockIdx.z * in_inst + blockIdx.y * in_part;
     u64* dst = out + blockIdx.z * out_inst + blockIdx.y * out_part;
+    const PairStages ps{isp, in_mods, osp};
+    // a run: 256 columns, or 128 pairs (c, c + N/2)
+    constexpr u32 RUN = PAIR ? 128 : 256, WAVE = PAIR ? 32 : 64;
+    const u32 off1 = PAIR ? n / 2 : 32;
     for (u32 ch = 0; ch < chunks; ch++)
-        rns_scale_mfma_cols<KW, KS1, NTO, HAS_W>(src, dst, sc, tab, n, nf,
-                                             (blockIdx.x * chunks + ch) * 256 + (threadIdx.x & 192));
+        rns_scale_mfma_cols<KW, KS1, NTO, HAS_W, PAIR>(src, dst, sc, tab, n, nf,
+                                                       (blockIdx.x * chunks + ch) * RUN + (threadIdx.x >> 6) * WAVE,
+                                                       off1, ps);
 }
 
+// One scaling launch: rows of part k of instance b at in + b in_inst +
+// k in_part (likewise out); pair: the stages pairing half rows (PairStages).
+struct MfmaLaunch {
+    const u64 *in, *sc, *mt;
+    size_t in_inst, in_part;
+    u64* out;
+    size_t out_inst, out_part;
+    u32 words, n, chunks;
+    bool pair;
+    PairStages ps;
+};
+
 template <int KW, int KS1, int NTO, bool HAS_W>
-static void mfma_go(dim3 grid, size_t lds, hipStream_t s, const u64* in, size_t in_inst, size_t in_part, u64* out,
-                    size_t out_inst, size_t out_part, const u64* sc, const u64* mt, u32 words, u32 n, u32 chunks) {
-    hipLaunchKernelGGL((rns_scale_mfma_kernel<KW, KS1, NTO, HAS_W>), grid, dim3(256), lds, s, in, in_inst, in_part, out,
-                       out_inst, out_part, sc, mt, words, n, chunks);
+static void mfma_go(dim3 grid, size_t lds, hipStream_t s, const MfmaLaunch& m) {
+    if (m.pair)
+        hipLaunchKernelGGL((rns_scale_mfma_kernel<KW, KS1, NTO, HAS_W, true>), grid, dim3(256), lds, s, m.in, m.in_inst,
+                           m.in_part, m.out, m.out_inst, m.out_part, m.sc, m.mt, m.words, m.n, m.chunks, m.ps.isp,
+                           m.ps.in_mods, m.ps.osp);
+    else
+        hipLaunchKernelGGL((rns_scale_mfma_kernel<KW, KS1, NTO, HAS_W, false>), grid, dim3(256), lds, s, m.in,
+                           m.in_inst, m.in_part, m.out, m.out_inst, m.out_part, m.sc, m.mt, m.words, m.n, m.chunks,
+                           nullptr, nullptr, nullptr);
 }
 
 template <int KW, int KS1>
-static void mfma_ks(const ScaleMfmaLayout& lay, dim3 grid, size_t lds, hipStream_t s, const u64* in, size_t in_inst,
-                    size_t in_part, u64* out, size_t out_inst, size_t out_part, const u64* sc, const u64* mt, u32 n,
-                    u32 ch) {
-    const u32 w = (u32)lay.words();
+static void mfma_ks(const ScaleMfmaLayout& lay, dim3 grid, size_t lds, hipStream_t s, const MfmaLaunch& m) {
     if (lay.nto == 1) {
-        if (lay.has_w) mfma_go<KW, KS1, 1, true>(grid, lds, s, in, in_inst, in_part, out, out_inst, out_part, sc, mt, w, n, ch);
-        else mfma_go<KW, KS1, 1, false>(grid, lds, s, in, in_inst, in_part, out, out_inst, out_part, sc, mt, w, n, ch);
+        if (lay.has_w) mfma_go<KW, KS1, 1, true>(grid, lds, s, m);
+        else mfma_go<KW, KS1, 1, false>(grid, lds, s, m);
     } else {
-        if (lay.has_w) mfma_go<KW, KS1, 2, true>(grid, lds, s, in, in_inst, in_part, out, out_inst, out_part, sc, mt, w, n, ch);
-        else mfma_go<KW, KS1, 2, false>(grid, lds, s, in, in_inst, in_part, out, out_inst, out_part, sc, mt, w, n, ch);
+        if (lay.has_w) mfma_go<KW, KS1, 2, true>(grid, lds, s, m);
+        else mfma_go<KW, KS1, 2, false>(grid, lds, s, m);
     }
 }
 
-bool launch_rns_scale_mfma(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst,
-                           size_t out_part, const u64* d_sc, const u64* d_mt, const ScaleMfmaLayout& lay, u32 n,
-                           u32 parts, u32 inst, hipStream_t s, const char* name) {
-    if (!d_mt || (lay.kw != 16 && lay.kw != 32) || lay.ks1 < 1 || lay.ks1 > (lay.kw == 16 ? 8 : 4) || lay.nto < 1 ||
+static bool mfma_launch(MfmaLaunch m, const ScaleMfmaLayout& lay, u32 parts, u32 inst, hipStream_t s,
+                        const char* name) {
+    const u32 n = m.n;
+    if (!m.mt || (lay.kw != 16 && lay.kw != 32) || lay.ks1 < 1 || lay.ks1 > (lay.kw == 16 ? 8 : 4) || lay.nto < 1 ||
         lay.nto > 2 || n < 256 || n % 256)
         return false;
     if (!parts || !inst) return true;
@@ -345,10 +404,12 @@ bool launch_rns_scale_mfma(const u64* in, size_t in_inst, size_t in_part, u64* o
     u32 chunks = 1;
     while (chunks < 8 && (n / 256) % (2 * chunks) == 0 && (size_t)(n / 256 / (2 * chunks)) * parts * inst >= 2048)
         chunks *= 2;
+    m.chunks = chunks;
+    m.words = (u32)lay.words();
     const dim3 grid(n / 256 / chunks, parts, inst);
     const size_t lds = lay.words() * sizeof(u64);
 #define FHEHIP_MFMA_KS(W, K) \
-    case K: mfma_ks<W, K>(lay, grid, lds, s, in, in_inst, in_part, out, out_inst, out_part, d_sc, d_mt, n, chunks); break;
+    case K: mfma_ks<W, K>(lay, grid, lds, s, m); break;
     if (lay.kw == 16) {
         switch (lay.ks1) {
             FHEHIP_MFMA_KS(16, 1) FHEHIP_MFMA_KS(16, 2) FHEHIP_MFMA_KS(16, 3) FHEHIP_MFMA_KS(16, 4)
@@ -360,6 +421,21 @@ bool launch_rns_scale_mfma(const u64* in, size_t in_inst, size_t in_part, u64* o
 #undef FHEHIP_MFMA_KS
     HIPCHK(hipGetLastError());
     return true;
+}
+
+bool launch_rns_scale_mfma(const u64* in, size_t in_inst, size_t in_part, u64* out, size_t out_inst,
+                           size_t out_part, const u64* d_sc, const u64* d_mt, const ScaleMfmaLayout& lay, u32 n,
+                           u32 parts, u32 inst, hipStream_t s, const char* name) {
+    MfmaLaunch m{in, d_sc, d_mt, in_inst, in_part, out, out_inst, out_part, 0, n, 1, false, PairStages{}};
+    return mfma_launch(m, lay, parts, inst, s, name);
+}
+
+bool launch_rns_scale_mfma_pair(const ScalePairArgs& a, const u64* d_mt, const ScaleMfmaLayout& lay, u32 parts,
+                                u32 inst, hipStream_t s, const char* name) {
+    if (a.in_split && !a.in_mods) throw Error(-1, "paired scaling: the inverse stage needs the source moduli");
+    MfmaLaunch m{a.in, a.sc, d_mt, a.in_inst, a.in_part, a.out, a.out_inst, a.out_part, 0, a.n, 1, true,
+                 PairStages{(const u64*)a.in_split, a.in_mods, (const u64*)a.out_split}};
+    return mfma_launch(m, lay, parts, inst, s, name);
 }
 
 }  // namespace fhehip
