@@ -194,13 +194,13 @@ struct PairStages {
 };
 
 // X = (x + y) n^-1 at c, Y = (x - y) zeta n^-1 at c + N/2 for x at c and y
-// at c + N/2 below 2p (native.rs:288-300, 130-131); both reduced.
-__device__ __forceinline__ void pair_inverse(u64& x, u64& y, const PairStages& ps, int j) {
-    const u64 p = ps.in_mods[j].p;
-    const u64* w = ps.isp + 6 * j;
+// at c + N/2 below 2p (native.rs:288-300, 130-131); both reduced. w: the
+// LDS copy of the limb's (p, zeta n^-1, its Shoup, n^-1, its Shoup).
+__device__ __forceinline__ void pair_inverse(u64& x, u64& y, const u64* w) {
+    const u64 p = w[0];
     const u64 a = x + y, b = x + 2 * p - y;
-    x = mul_shoup(a, w[4], w[5], p);
-    y = mul_shoup(b, w[2], w[3], p);
+    x = mul_shoup(a, w[3], w[4], p);
+    y = mul_shoup(b, w[1], w[2], p);
 }
 
 // One wave's 64 columns (the tables already in LDS): block 0 from col0,
@@ -210,7 +210,7 @@ __device__ __forceinline__ void pair_inverse(u64& x, u64& y, const PairStages& p
 template <int KW, int KS1, int NTO, bool HAS_W, bool PAIR>
 __device__ __forceinline__ void rns_scale_mfma_cols(const u64* __restrict__ in, u64* __restrict__ out,
                                                     const u64* __restrict__ sc, const u64* tab, u32 n, int nf,
-                                                    u32 col0, u32 off1, const PairStages& ps) {
+                                                    u32 col0, u32 off1, const PairStages& ps, u32 pc) {
     constexpr int WL = KW / 16, KS3 = KW == 32 ? 1 : (HAS_W ? 2 : 1);
     constexpr int T1 = 1 + (HAS_W ? 1 : 0) + NTO;
     constexpr int A2 = T1 * KS1 * 64 * WL, A3 = A2 + (HAS_W ? 64 * WL : 0), AM = A3 + NTO * KS3 * 64 * WL;
@@ -224,7 +224,7 @@ __device__ __forceinline__ void rns_scale_mfma_cols(const u64* __restrict__ in, 
         if (j < nf) {
             r[0] = src[(size_t)j * n];
             r[1] = src[(size_t)j * n + off1];
-            if (PAIR && ps.isp) pair_inverse(r[0], r[1], ps, j);
+            if (PAIR && ps.isp) pair_inverse(r[0], r[1], tab + pc + 5 * j);
         }
         if constexpr (KW == 16) {
             b0[k] = (long)digits64(r[0]);
@@ -233,7 +233,7 @@ __device__ __forceinline__ void rns_scale_mfma_cols(const u64* __restrict__ in, 
             if (j + 1 < nf) {
                 r[2] = src[(size_t)(j + 1) * n];
                 r[3] = src[(size_t)(j + 1) * n + off1];
-                if (PAIR && ps.isp) pair_inverse(r[2], r[3], ps, j + 1);
+                if (PAIR && ps.isp) pair_inverse(r[2], r[3], tab + pc + 5 * (j + 1));
             }
             b0[k] = pack16(digits64(r[0]), digits64(r[2]));
             b1[k] = pack16(digits64(r[1]), digits64(r[3]));
@@ -341,9 +341,15 @@ __global__ __launch_bounds__(256) void rns_scale_mfma_kernel(const u64* __restri
                                                              const ModInfo* __restrict__ in_mods,
                                                              const u64* __restrict__ osp) {
     extern __shared__ u64 tab[];
-    for (u32 i = threadIdx.x; i < mt_words; i += 256) tab[i] = mt[i];
-    __syncthreads();
     const int nf = (int)sc[SC_NF];
+    for (u32 i = threadIdx.x; i < mt_words; i += 256) tab[i] = mt[i];
+    // the pair's inverse-stage constants per source limb after the tables
+    if (PAIR && isp)
+        for (u32 i = threadIdx.x; i < 5 * (u32)nf; i += 256) {
+            const u32 j = i / 5, k = i % 5;
+            tab[mt_words + i] = k == 0 ? in_mods[j].p : isp[6 * j + 1 + k];
+        }
+    __syncthreads();
     const u64* src = in + blockIdx.z * in_inst + blockIdx.y * in_part;
     u64* dst = out + blockIdx.z * out_inst + blockIdx.y * out_part;
     const PairStages ps{isp, in_mods, osp};
@@ -353,7 +359,7 @@ __global__ __launch_bounds__(256) void rns_scale_mfma_kernel(const u64* __restri
     for (u32 ch = 0; ch < chunks; ch++)
         rns_scale_mfma_cols<KW, KS1, NTO, HAS_W, PAIR>(src, dst, sc, tab, n, nf,
                                                        (blockIdx.x * chunks + ch) * RUN + (threadIdx.x >> 6) * WAVE,
-                                                       off1, ps);
+                                                       off1, ps, mt_words);
 }
 
 // One scaling launch: rows of part k of instance b at in + b in_inst +
@@ -407,7 +413,7 @@ static bool mfma_launch(MfmaLaunch m, const ScaleMfmaLayout& lay, u32 parts, u32
     m.chunks = chunks;
     m.words = (u32)lay.words();
     const dim3 grid(n / 256 / chunks, parts, inst);
-    const size_t lds = lay.words() * sizeof(u64);
+    const size_t lds = (lay.words() + (m.pair && m.ps.isp ? 5 * (size_t)lay.nf : 0)) * sizeof(u64);
 #define FHEHIP_MFMA_KS(W, K) \
     case K: mfma_ks<W, K>(lay, grid, lds, s, m); break;
     if (lay.kw == 16) {
