@@ -20,6 +20,7 @@ Prints one JSON line.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -100,7 +101,9 @@ def load_profile(name):
         with open(os.path.join(ROOT, "profiles", name)) as f:
             for line in f:
                 if line.startswith("Kernel sources:"):
-                    digest = line.split(":", 1)[1].strip().strip("`")
+                    # the first `…` token (the line may go on to say what it is)
+                    m = re.search(r"`([0-9a-f]+)`", line)
+                    digest = m.group(1) if m else line.split(":", 1)[1].strip().strip("`")
                 else:
                     rows.append(line.split())
     except OSError:

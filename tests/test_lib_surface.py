@@ -53,3 +53,20 @@ def test_generate_moduli_matches_oracle_and_config_lists(oracle):
         fhehip.generate_moduli([63], 1024)
     with pytest.raises(fhehip.FheHipError):
         fhehip.generate_moduli([60], 1000)
+
+
+def test_counter_summaries_name_their_sources():
+    """bench.py reports HBM traffic and VALU figures only from counter
+    summaries measured on the current kernel sources: each summary's
+    `Kernel sources:` line must parse to a digest of the form bench.py
+    computes, and the traffic summary must hold the dominant kernel's row."""
+    import sys
+    sys.path.insert(0, ROOT)
+    import bench
+    here = bench.kernel_sources_digest()
+    assert re.fullmatch(r"[0-9a-f]{16}", here)
+    for name in (bench.TRAFFIC_PROFILE, bench.VALU_PROFILE, "kernels_mul_relin_n16384_b512.md"):
+        rows, digest = bench.load_profile(name)
+        assert digest is not None and re.fullmatch(r"[0-9a-f]{16}", digest), (name, digest)
+    rows, digest = bench.load_profile(bench.TRAFFIC_PROFILE)
+    assert bench.load_traffic("tensor_intt", digest) > 0
