@@ -5,14 +5,17 @@
 // bit for bit.
 //
 // A wave takes 64 columns as two blocks of 32 (the N dimension of
-// v_mfma_i32_32x32x32_i8). Lane l supplies residues 4 kappa + 2 (l / 32) and
-// the next one of column l % 32 of each block as the B operand of K step
-// kappa (16 digit bytes); the A operand (32 table rows) is the same for both
-// blocks. After a tile's products, lanes
-// l and l + 32 swap halves (v_permlane32_swap) so that every lane holds the
-// 32 rows of its own column, and each lane finishes one column: the rounding
-// of v and w as rns_scale_column, then the outputs. A workgroup of 4 waves
-// stages the tables in LDS once and covers `chunks` runs of 256 columns;
+// v_mfma_i32_32x32x16_i8 or v_mfma_i32_32x32x32_i8, KW = 16 or 32 k per
+// MFMA). Per K step, lane l supplies the digits of KW / 16 residues of
+// column l % 32 of each block as the B operand; the A operand (32 table
+// rows) is the same for both blocks. After a tile's products, lanes l and
+// l + 32 swap halves (v_permlane32_swap) so that every lane holds the 32
+// rows of its own column, and each lane finishes one column: the rounding
+// of v and w as rns_scale_column, then the outputs. The blocks are columns
+// c .. c + 31 and c + 32 .. c + 63, or, for rows left as two open halves,
+// c .. c + 31 and c + N/2 .. c + N/2 + 31 (PAIR: the stages pairing the
+// halves run inside). A workgroup of 4 waves stages the tables in LDS once
+// and covers `chunks` runs of 256 columns (128 pairs);
 // grid = (N / (256 chunks), parts, instances).
 #include <type_traits>
 
