@@ -51,15 +51,15 @@ def test_scaler_extension_and_down(fh, oracle, n):
 # source moduli counts of the ciphertext part and the extension part: the
 # extension scales nq -> nq + ne by one, the down-scaling nq + ne -> nq by
 # t / Q; at most 8 source moduli take MFMA steps of 16 k, more take 32
-@pytest.mark.parametrize("nq,ne", [(1, 2), (3, 4), (6, 7), (8, 8), (5, 4), (2, 7)])
-def test_scaler_matrix_cores(fh, oracle, nq, ne):
+@pytest.mark.parametrize("nq,ne,n", [(1, 2, 4096), (3, 4, 4096), (6, 7, 4096), (8, 8, 4096), (5, 4, 4096),
+                                     (2, 7, 4096), (6, 7, 256), (3, 9, 512)])
+def test_scaler_matrix_cores(fh, oracle, nq, ne, n):
     """The scaler's matrix-core form (8-bit integer MFMA over base-256
     digits), taken by reduced inputs (the inverse transform's outputs of
     NTT-domain polynomials), against the oracle and against the per-column
     VALU form. Power-basis columns 0, 1 and 2 hold all zeros, all p - 1 and
-    alternating zeros and p - 1."""
-    n = 4096
-    print(f"scaler matrix cores nq={nq} ne={ne}", flush=True)
+    alternating zeros and p - 1. N = 256 is the smallest degree of that form."""
+    print(f"scaler matrix cores nq={nq} ne={ne} n={n}", flush=True)
     q = oracle.generate_moduli([60] * nq, n)
     ext = [m for m in oracle.generate_moduli([62] * (nq + ne), n) if m not in q][:ne]
     full = q + ext
